@@ -1,0 +1,216 @@
+"""Table generator: species thermo and rate-constant tables on a 1 K grid.
+
+FLINT never evaluates NASA polynomials or Arrhenius expressions at run time; every property
+is linearly interpolated from a 1 K table (Lib_Chemistry_data.f90:36-48,
+Lib_ThermoTransport.f90:516-542).  The tables are produced offline by a generator that is not
+part of FLINT; this module is this project's generator.  Its conventions were fixed by matching
+the tables FLINT does ship:
+
+* gas constant 8314.46261815324 J/(kmol K) and 4184 J/kcal (test/WD/INPUT/chemistry-Arrhenius.dat
+  is reproduced to all 13 printed digits);
+* per-mass cp, h, s and dcp/dT from NASA-7 or NASA-9 polynomials; below the lowest polynomial
+  temperature cp is held constant, h grows linearly and s logarithmically
+  (test/Pelucchi/INPUT/thermo.dat);
+* kf in kmol/m^3/s units without the third-body concentration, kb = kf/Kc for reversible
+  reactions and 0 for irreversible ones, Kc at the 1 atm standard state;
+* falloff tables hold k_inf, k_0, k_c = Kc (capped at 1e300) and the Troe F_cent
+  (test/Pelucchi/INPUT/chemistry-{Troe,Lindemann}.dat).
+
+Tables are returned in FLINT's own orientation, ``tab[T - Tmin, column]`` with one column per
+species or per reaction of the given kind.
+"""
+import json
+import os
+
+import numpy as np
+
+R_GEN = 8314.46261815324     # J/(kmol K), generator gas constant
+P_STD = 101325.0             # Pa, standard-state pressure of the NASA fits
+KC_CAP = 1.0e300
+
+_MECH_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "mechanisms")
+
+
+def load_descriptor(name):
+    with open(os.path.join(_MECH_DIR, name + ".json")) as f:
+        return json.load(f)
+
+
+# ----------------------------------------------------------------------------------------------
+# NASA polynomials (dimensionless cp/R, h/RT, s/R)
+# ----------------------------------------------------------------------------------------------
+def _nasa7(a, T):
+    cp = a[0] + T * (a[1] + T * (a[2] + T * (a[3] + T * a[4])))
+    h = a[0] + T * (a[1] / 2 + T * (a[2] / 3 + T * (a[3] / 4 + T * a[4] / 5))) + a[5] / T
+    s = a[0] * np.log(T) + T * (a[1] + T * (a[2] / 2 + T * (a[3] / 3 + T * a[4] / 4))) + a[6]
+    dcp = a[1] + T * (2 * a[2] + T * (3 * a[3] + T * 4 * a[4]))
+    return cp, h, s, dcp
+
+
+def _nasa9(a, T):
+    lnT = np.log(T)
+    cp = a[0] / T ** 2 + a[1] / T + a[2] + T * (a[3] + T * (a[4] + T * (a[5] + T * a[6])))
+    h = (-a[0] / T ** 2 + a[1] * lnT / T + a[2] + T * (a[3] / 2 + T * (a[4] / 3 + T * (a[5] / 4 + T * a[6] / 5)))
+         + a[7] / T)
+    s = (-a[0] / (2 * T ** 2) - a[1] / T + a[2] * lnT + T * (a[3] + T * (a[4] / 2 + T * (a[5] / 3 + T * a[6] / 4)))
+         + a[8])
+    dcp = -2 * a[0] / T ** 3 - a[1] / T ** 2 + a[3] + T * (2 * a[4] + T * (3 * a[5] + T * 4 * a[6]))
+    return cp, h, s, dcp
+
+
+def _species_nondim(th, T, extend=True):
+    """cp/R, h/RT, s/R, d(cp/R)/dT of one species on the array T.
+
+    With ``extend`` the fit is continued outside its range with constant cp (the thermo tables);
+    without it the end polynomials are evaluated as they are (the equilibrium constants).
+    """
+    T = np.asarray(T, dtype=np.float64)
+    ranges = th["ranges"]
+    f = _nasa7 if th["model"] == "NASA7" else _nasa9
+    tlo, thi = ranges[0], ranges[-1]
+    Tc = np.clip(T, tlo, thi) if extend else T
+    cp = np.empty_like(T)
+    h = np.empty_like(T)
+    s = np.empty_like(T)
+    dcp = np.empty_like(T)
+    nreg = len(th["data"])
+    for i in range(nreg):
+        lo, hi = ranges[i], ranges[i + 1]
+        m = np.ones(T.shape, dtype=bool)
+        if i > 0:
+            m &= Tc > lo
+        if i < nreg - 1:
+            m &= Tc <= hi
+        if not np.any(m):
+            continue
+        c, hh, ss, d = f(th["data"][i], Tc[m])
+        cp[m], h[m], s[m], dcp[m] = c, hh, ss, d
+    # constant cp outside the fitted range: h linear, s logarithmic
+    if extend:
+        dcp[(T <= tlo) | (T >= thi)] = 0.0
+    out = T != Tc
+    if np.any(out):
+        h[out] = (h[out] * Tc[out] + cp[out] * (T[out] - Tc[out])) / T[out]
+        s[out] = s[out] + cp[out] * np.log(T[out] / Tc[out])
+    return cp, h, s, dcp
+
+
+def thermo_tables(desc, Tmin, Tmax):
+    """Per-mass cp [J/kg/K], h [J/kg], s [J/kg/K], dcp/dT on T = Tmin..Tmax (rows)."""
+    T = np.arange(Tmin, Tmax + 1, dtype=np.float64)
+    ns = len(desc["species"])
+    cp = np.empty((T.size, ns))
+    h = np.empty_like(cp)
+    s = np.empty_like(cp)
+    dcp = np.empty_like(cp)
+    for k, th in enumerate(desc["thermo"]):
+        c, hh, ss, d = _species_nondim(th, T)
+        Rk = R_GEN / desc["W"][k]
+        cp[:, k] = c * Rk
+        h[:, k] = hh * Rk * T
+        s[:, k] = ss * Rk
+        dcp[:, k] = d * Rk
+    return dict(T=T, cp=cp, h=h, s=s, dcp=dcp)
+
+
+# ----------------------------------------------------------------------------------------------
+# Rate constants
+# ----------------------------------------------------------------------------------------------
+def _arrhenius(p, T):
+    with np.errstate(over="ignore", under="ignore"):
+        return p["A"] * T ** p["b"] * np.exp(-p["Ea"] / (R_GEN * T))
+
+
+def _kc(desc, r, T):
+    """Equilibrium constant in kmol/m^3 units at 1 atm standard state."""
+    ns = len(desc["species"])
+    g = np.zeros(T.size)
+    dn = 0.0
+    for k, v in r["nu_p"].items():
+        c, h, s, _ = _species_nondim(desc["thermo"][int(k)], T, extend=False)
+        g += v * (h - s)
+        dn += v
+    for k, v in r["nu_r"].items():
+        c, h, s, _ = _species_nondim(desc["thermo"][int(k)], T, extend=False)
+        g -= v * (h - s)
+        dn -= v
+    with np.errstate(over="ignore", under="ignore"):
+        kc = np.exp(-g) * (P_STD / (R_GEN * T)) ** dn
+    return np.minimum(kc, KC_CAP)
+
+
+def _fcent(tp, T):
+    with np.errstate(over="ignore", under="ignore"):
+        f = (1.0 - tp["A"]) * np.exp(-T / tp["T3"]) + tp["A"] * np.exp(-T / tp["T1"])
+        if tp["T2"] is not None:
+            f = f + np.exp(-tp["T2"] / T)
+    return f
+
+
+def kinetics_tables(desc, Tmin, Tmax):
+    """Rate tables on T = Tmin..Tmax.
+
+    Returns kf, kb [nT, n_arrh] for the Arrhenius and three-body reactions (in mechanism order),
+    troe [nT, n_troe, 4] = (k_inf, k_0, k_c, F_cent) and lind [nT, n_lind, 3] = (k_inf, k_0, k_c).
+    """
+    T = np.arange(Tmin, Tmax + 1, dtype=np.float64)
+    kf, kb, troe, lind = [], [], [], []
+    for r in desc["reactions"]:
+        if r["type"] in ("arrhenius", "three-body"):
+            f = _arrhenius(r["rate"], T)
+            kf.append(f)
+            if r["reversible"]:
+                kc = _kc(desc, r, T)
+                with np.errstate(divide="ignore", invalid="ignore", over="ignore", under="ignore"):
+                    kb.append(np.where(kc > 0, f / kc, 0.0))
+            else:
+                kb.append(np.zeros_like(T))
+        else:
+            kinf = _arrhenius(r["high"], T)
+            k0 = _arrhenius(r["low"], T)
+            kc = _kc(desc, r, T) if r["reversible"] else np.full_like(T, KC_CAP)
+            if r["type"] == "troe":
+                troe.append(np.stack([kinf, k0, kc, _fcent(r["troe"], T)], axis=1))
+            else:
+                lind.append(np.stack([kinf, k0, kc], axis=1))
+    out = dict(T=T, kf=np.stack(kf, axis=1), kb=np.stack(kb, axis=1))
+    out["troe"] = np.stack(troe, axis=1) if troe else np.zeros((T.size, 0, 4))
+    out["lind"] = np.stack(lind, axis=1) if lind else np.zeros((T.size, 0, 3))
+    return out
+
+
+# ----------------------------------------------------------------------------------------------
+# Tecplot-ASCII writer / reader for FLINT's native table files (docs/input_native.md)
+# ----------------------------------------------------------------------------------------------
+def write_tecplot(path, title, varnames, zone_names, T, blocks, fmt="%.12E"):
+    """Write one zone per column block: blocks is [nzone][nT, nvar]."""
+    with open(path, "w") as f:
+        f.write('TITLE = "%s"\n' % title)
+        f.write("VARIABLES = " + ", ".join('"%s"' % v for v in varnames) + "\n")
+        for name, blk in zip(zone_names, blocks):
+            f.write('ZONE T="%s"\nI=%d, F=POINT\n' % (name, T.size))
+            data = np.column_stack([T, blk])
+            np.savetxt(f, data, fmt=["%12.3f"] + [fmt] * blk.shape[1])
+
+
+def read_tecplot(path):
+    """Return (T, [zone arrays nT x nvar]) for a POINT-format Tecplot ASCII file."""
+    zones = []
+    cur = []
+    with open(path) as f:
+        for line in f:
+            s = line.strip()
+            if not s:
+                continue
+            if s.startswith(("TITLE", "VARIABLES", "I=")):
+                continue
+            if s.startswith("ZONE"):
+                if cur:
+                    zones.append(np.array(cur))
+                cur = []
+                continue
+            cur.append([float(v) for v in s.split()])
+    if cur:
+        zones.append(np.array(cur))
+    T = zones[0][:, 0]
+    return T, [z[:, 1:] for z in zones]

@@ -1,0 +1,346 @@
+/* CPU oracle of the FLINT per-cell thermochemistry path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Loaded by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, never by the
+ * shipped library.  See flint_oracle.h for layouts and oracle_body.inc for the restated routines.
+ *
+ * Pinning (what checks each function):
+ *   orc_wdot / orc_wd_hrr  against the reference Fortran itself, compiled by oracle/Makefile into
+ *                          oracle/_ref (tests/test_oracle_ref.py), and against the committed
+ *                          fixtures written from it (tests/golden/); mass conservation
+ *                          sum_k wdot_k = 0 and element conservation (tests/test_oracle.py)
+ *   orc_rhs                against oracle/_ref's restatement-free closed form of rhs_native
+ *                          (Lib_Chemistry_rhs.f90:15-52) and internal-energy conservation
+ *   orc_jac                against the complex-step derivative orc_jac_cstep of orc_rhs
+ *                          (no Jacobian exists in the reference: that is the mathematics' pin)
+ */
+#include <complex.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "flint_oracle.h"
+
+#define ORC_MAX_NS 64
+#define ORC_MAX_NR 512
+#define ORC_RUNIV 8314.51 /* Lib_ThermoTransport.f90:13 */
+
+int orc_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* ---- real instantiation ---- */
+#define REAL double
+#define SFX(x) x##_r
+#define RREAL(x) (x)
+#define RLOG10(x) log10(x)
+#define REXP10(x) pow(10.0, (x))
+#define RPOW(x, e) pow((x), (e))
+#define cabs_(x) fabs(x)
+#include "oracle_body.inc"
+#undef REAL
+#undef SFX
+#undef RREAL
+#undef RLOG10
+#undef REXP10
+#undef RPOW
+#undef cabs_
+
+/* ---- complex instantiation (complex-step differentiation) ---- */
+#define REAL double _Complex
+#define SFX(x) x##_c
+#define RREAL(x) creal(x)
+#define RLOG10(x) (clog(x) / M_LN10)
+#define REXP10(x) cexp((x) * M_LN10)
+#define RPOW(x, e) cpow((x), (e))
+#define cabs_(x) cabs(x)
+#include "oracle_body.inc"
+#undef REAL
+#undef SFX
+#undef RREAL
+#undef RLOG10
+#undef REXP10
+#undef RPOW
+#undef cabs_
+
+void orc_wdot(const orc_mech *m, long n, long ld, const double *rho, const double *T, double *wdot,
+              double *wdot_scale) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    double roi[ORC_MAX_NS], w[ORC_MAX_NS], sc[ORC_MAX_NS], pf[ORC_MAX_NR], pb[ORC_MAX_NR];
+    for (int k = 0; k < m->ns; ++k) roi[k] = rho[k * ld + i];
+    if (m->is_wd) {
+      wd_cell_r(m, roi, T[i], w);
+      for (int k = 0; k < m->ns; ++k) sc[k] = fabs(w[k]);
+    } else {
+      wdot_cell_r(m, roi, T[i], w, pf, pb, sc);
+    }
+    for (int k = 0; k < m->ns; ++k) {
+      wdot[k * ld + i] = w[k];
+      if (wdot_scale) wdot_scale[k * ld + i] = sc[k];
+    }
+  }
+}
+
+/* WD source term plus heat release -sum_k h_k(T) wdot_k (h from the thermo table,
+ * Lib_ThermoTransport.f90:77-96 H0i). */
+void orc_wd_hrr(const orc_mech *m, long n, long ld, const double *rho, const double *T, double *wdot,
+                double *hrr) {
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    double roi[ORC_MAX_NS], w[ORC_MAX_NS], pf[ORC_MAX_NR], pb[ORC_MAX_NR];
+    for (int k = 0; k < m->ns; ++k) roi[k] = rho[k * ld + i];
+    if (m->is_wd) wd_cell_r(m, roi, T[i], w);
+    else wdot_cell_r(m, roi, T[i], w, pf, pb, 0);
+    int T_i = (int)T[i];
+    double Tdiff = T[i] - T_i;
+    long row = T_i - m->Th_min;
+    double q = 0.0;
+    for (int k = 0; k < m->ns; ++k) {
+      wdot[k * ld + i] = w[k];
+      q = q + interp_r(m->h, row, m->ns, k, Tdiff) * w[k];
+    }
+    hrr[i] = -q;
+  }
+}
+
+void orc_rhs(const orc_mech *m, long n, long ld, const double *Z, double *F) {
+  int nz = m->ns + 1;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    double z[ORC_MAX_NS + 1], f[ORC_MAX_NS + 1];
+    for (int k = 0; k < nz; ++k) z[k] = Z[k * ld + i];
+    rhs_cell_r(m, z, f);
+    for (int k = 0; k < nz; ++k) F[k * ld + i] = f[k];
+  }
+}
+
+void orc_jac_cstep(const orc_mech *m, long n, long ld, const double *Z, double h, double *jac) {
+  int nz = m->ns + 1;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    double _Complex z[ORC_MAX_NS + 1], f[ORC_MAX_NS + 1];
+    for (int c = 0; c < nz; ++c) {
+      for (int k = 0; k < nz; ++k) z[k] = Z[k * ld + i];
+      z[c] += I * h;
+      rhs_cell_c(m, z, f);
+      for (int r = 0; r < nz; ++r) jac[((long)r * nz + c) * ld + i] = cimag(f[r]) / h;
+    }
+  }
+}
+
+/* ------------------------------------------------------------------------------------------
+ * Analytic Jacobian of the constant-volume RHS.  The reference has none (its integrator
+ * approximates one internally); this is the exact derivative of the table-interpolated model:
+ *   d(a+(b-a)Tdiff)/dT = b-a within the bin, d max(rho,0)/drho = 1 for rho >= 0 else 0.
+ * Species block  J_kj = W_k sum_r nu_kr dq_r/dc_j / W_j,  q_r = prodf_r - prodb_r
+ * T row          dTdot/drho_j = -(sum_k e_k J_kj + Tdot cv_j) / D
+ *                dTdot/dT     = -(sum_k (de_k/dT wdot_k + e_k J_kT) + Tdot sum_k rho_k dcv_k/dT) / D
+ * with e_k = h_k - R_k T, cv_k = cp_k - R_k, D = sum_k rho_k cv_k.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct {
+  double k1, k2;     /* effective forward / reverse coefficient (incl. M for three-body) */
+  double dk1T, dk2T; /* their T derivatives at fixed concentrations */
+  double dk1M, dk2M; /* derivatives with respect to M (falloff) or the bare factor (three-body) */
+} orc_rate;
+
+static void falloff_derivs(double kinf, double k0, double kc, double Fcent, double dkinf, double dk0, double dkc,
+                           double dFcent, double M, int troe, orc_rate *o) {
+  double Pr = k0 * M / kinf;
+  double g = Pr / (1.0 + Pr), dg = 1.0 / ((1.0 + Pr) * (1.0 + Pr));
+  double F = 1.0, dlnF_dPr = 0.0, dlnF_dFc = 0.0;
+  if (troe) {
+    double L = log10(Fcent), x = log10(Pr);
+    double c = -0.4 - 0.67 * L, n = 0.75 - 1.27 * L, u = x + c, den = n - 0.14 * u;
+    double f1 = u / den, q = 1.0 + f1 * f1;
+    F = pow(10.0, L / q);
+    double df1_du = n / (den * den);
+    double df1_dL = (-0.67 * n + 1.27 * u) / (den * den);
+    dlnF_dPr = -2.0 * L * f1 * df1_du / (q * q * Pr);
+    dlnF_dFc = M_LN10 * (1.0 / q - L * 2.0 * f1 * df1_dL / (q * q)) / (Fcent * M_LN10);
+  }
+  double k1 = kinf * g * F;
+  double dk1_dPr = kinf * F * (dg + g * dlnF_dPr);
+  double dPr_dT = M * (dk0 * kinf - k0 * dkinf) / (kinf * kinf);
+  double dk1T = dkinf * g * F + dk1_dPr * dPr_dT + k1 * dlnF_dFc * dFcent;
+  o->k1 = k1;
+  o->k2 = k1 / kc;
+  o->dk1T = dk1T;
+  o->dk2T = dk1T / kc - k1 * dkc / (kc * kc);
+  o->dk1M = dk1_dPr * k0 / kinf;
+  o->dk2M = o->dk1M / kc;
+}
+
+static double slope(const double *tab, long row, long stride, long col) {
+  return tab[(row + 1) * stride + col] - tab[row * stride + col];
+}
+
+/* dq/dc_j of prod_{t} c_{sp_t}^{pw_t} times k, accumulated into dq[] with sign */
+static void add_dprod(double k, const int *ptr, const int *sp, const int *pw, int r, const double *coi, double sign,
+                      double *dq, double *dqabs) {
+  for (int t = ptr[r]; t < ptr[r + 1]; ++t) {
+    double d = k * pw[t];
+    for (int p = 1; p < pw[t]; ++p) d *= coi[sp[t]];
+    for (int u = ptr[r]; u < ptr[r + 1]; ++u) {
+      if (u == t) continue;
+      for (int p = 0; p < pw[u]; ++p) d *= coi[sp[u]];
+    }
+    dq[sp[t]] += sign * d;
+    dqabs[sp[t]] += fabs(d);
+  }
+}
+
+static double prod_only(const int *ptr, const int *sp, const int *pw, int r, const double *coi) {
+  double v = 1.0;
+  for (int t = ptr[r]; t < ptr[r + 1]; ++t)
+    for (int p = 0; p < pw[t]; ++p) v *= coi[sp[t]];
+  return v;
+}
+
+static void jac_cell(const orc_mech *m, const double *nu, const double *Z, double *F, double *J, double *Jabs) {
+  int ns = m->ns, nz = ns + 1;
+  double roi[ORC_MAX_NS], coi[ORC_MAX_NS], dclip[ORC_MAX_NS];
+  double T = Z[ns];
+  rhs_cell_r(m, Z, F);
+  for (int a = 0; a < nz * nz; ++a) { J[a] = 0.0; Jabs[a] = 0.0; }
+  if (F[ns] == -1.0 && (T < m->Th_min || T > m->Th_max || isnan(T))) return;
+  for (int k = 0; k < ns; ++k) {
+    roi[k] = Z[k] < 0.0 ? 0.0 : Z[k];
+    dclip[k] = Z[k] < 0.0 ? 0.0 : 1.0;
+    coi[k] = roi[k] / m->W[k];
+  }
+  int T_i = (int)T;
+  double Tdiff = T - T_i;
+  long row = T_i - m->Tk_min;
+  double dwdc[ORC_MAX_NS][ORC_MAX_NS]; /* [k][j] = sum_r nu_kr dq_r/dc_j */
+  double dwdc_abs[ORC_MAX_NS][ORC_MAX_NS];
+  double dwdT[ORC_MAX_NS], dwdT_abs[ORC_MAX_NS];
+  memset(dwdc, 0, sizeof dwdc);
+  memset(dwdc_abs, 0, sizeof dwdc_abs);
+  memset(dwdT, 0, sizeof dwdT);
+  memset(dwdT_abs, 0, sizeof dwdT_abs);
+  for (int r = 0; r < m->nr; ++r) {
+    int kind = m->rx_kind[r], tb = m->rx_table[r];
+    double M = 1.0;
+    if (m->rx_mset[r] >= 0) M = msum_r(m, m->rx_mset[r], coi);
+    double Pf = prod_only(m->rx_fwd_ptr, m->rx_fwd_sp, m->rx_fwd_pow, r, coi);
+    double Pb = prod_only(m->rx_rev_ptr, m->rx_rev_sp, m->rx_rev_pow, r, coi);
+    double dq[ORC_MAX_NS], dqa[ORC_MAX_NS];
+    for (int j = 0; j < ns; ++j) { dq[j] = 0.0; dqa[j] = 0.0; }
+    double dqT, dqTa, dqM = 0.0, dqMa = 0.0;
+    if (kind <= 1) {
+      double kf = interp_r(m->kf, row, m->n_arrh, tb, Tdiff), kb = interp_r(m->kb, row, m->n_arrh, tb, Tdiff);
+      double dkf = slope(m->kf, row, m->n_arrh, tb), dkb = slope(m->kb, row, m->n_arrh, tb);
+      double Mf = (kind == 1) ? M : 1.0;
+      add_dprod(kf * Mf, m->rx_fwd_ptr, m->rx_fwd_sp, m->rx_fwd_pow, r, coi, 1.0, dq, dqa);
+      add_dprod(kb * Mf, m->rx_rev_ptr, m->rx_rev_sp, m->rx_rev_pow, r, coi, -1.0, dq, dqa);
+      dqT = (dkf * Pf - dkb * Pb) * Mf;
+      dqTa = (fabs(dkf * Pf) + fabs(dkb * Pb)) * fabs(Mf);
+      if (kind == 1) { dqM = kf * Pf - kb * Pb; dqMa = fabs(kf * Pf) + fabs(kb * Pb); }
+    } else {
+      orc_rate o;
+      if (kind == 2) {
+        long st = 4L * m->n_troe, c0 = 4L * tb;
+        falloff_derivs(interp_r(m->troe, row, st, c0, Tdiff), interp_r(m->troe, row, st, c0 + 1, Tdiff),
+                       interp_r(m->troe, row, st, c0 + 2, Tdiff), interp_r(m->troe, row, st, c0 + 3, Tdiff),
+                       slope(m->troe, row, st, c0), slope(m->troe, row, st, c0 + 1), slope(m->troe, row, st, c0 + 2),
+                       slope(m->troe, row, st, c0 + 3), M, 1, &o);
+      } else {
+        long st = 3L * m->n_lind, c0 = 3L * tb;
+        falloff_derivs(interp_r(m->lind, row, st, c0, Tdiff), interp_r(m->lind, row, st, c0 + 1, Tdiff),
+                       interp_r(m->lind, row, st, c0 + 2, Tdiff), 1.0, slope(m->lind, row, st, c0),
+                       slope(m->lind, row, st, c0 + 1), slope(m->lind, row, st, c0 + 2), 0.0, M, 0, &o);
+      }
+      add_dprod(o.k1, m->rx_fwd_ptr, m->rx_fwd_sp, m->rx_fwd_pow, r, coi, 1.0, dq, dqa);
+      add_dprod(o.k2, m->rx_rev_ptr, m->rx_rev_sp, m->rx_rev_pow, r, coi, -1.0, dq, dqa);
+      dqT = o.dk1T * Pf - o.dk2T * Pb;
+      dqTa = fabs(o.dk1T * Pf) + fabs(o.dk2T * Pb);
+      dqM = o.dk1M * Pf - o.dk2M * Pb;
+      dqMa = fabs(o.dk1M * Pf) + fabs(o.dk2M * Pb);
+    }
+    if (m->rx_mset[r] >= 0) { /* chain through M = sum_j eff_j c_j */
+      int set = m->rx_mset[r];
+      if (m->ms_is_sum[set]) {
+        for (int j = 0; j < ns; ++j) { dq[j] += dqM; dqa[j] += dqMa; }
+      } else {
+        for (int t = m->ms_ptr[set]; t < m->ms_ptr[set + 1]; ++t) {
+          double e = isnan(m->ms_eff[t]) ? 1.0 : m->ms_eff[t];
+          dq[m->ms_sp[t]] += e * dqM;
+          dqa[m->ms_sp[t]] += fabs(e * dqMa);
+        }
+      }
+    }
+    for (int k = 0; k < ns; ++k) {
+      double v = nu[(long)r * ns + k];
+      if (v == 0.0) continue;
+      for (int j = 0; j < ns; ++j) {
+        dwdc[k][j] += v * dq[j];
+        dwdc_abs[k][j] += fabs(v) * dqa[j];
+      }
+      dwdT[k] += v * dqT;
+      dwdT_abs[k] += fabs(v) * dqTa;
+    }
+  }
+  /* species rows */
+  for (int k = 0; k < ns; ++k) {
+    for (int j = 0; j < ns; ++j) {
+      double f = m->W[k] * dclip[j] / m->W[j];
+      J[k * nz + j] = f * dwdc[k][j];
+      Jabs[k * nz + j] = fabs(f) * dwdc_abs[k][j];
+    }
+    J[k * nz + ns] = m->W[k] * dwdT[k];
+    Jabs[k * nz + ns] = m->W[k] * dwdT_abs[k];
+  }
+  /* temperature row */
+  long rowh = T_i - m->Th_min;
+  double D = 0.0, Tdot = F[ns], s_dcv = 0.0, s_de = 0.0, s_de_abs = 0.0;
+  double e[ORC_MAX_NS], cv[ORC_MAX_NS];
+  for (int k = 0; k < ns; ++k) {
+    double Ri = ORC_RUNIV / m->W[k];
+    e[k] = interp_r(m->h, rowh, ns, k, Tdiff) - Ri * T;
+    cv[k] = interp_r(m->cp, rowh, ns, k, Tdiff) - Ri;
+    D += roi[k] * cv[k];
+    s_dcv += roi[k] * slope(m->cp, rowh, ns, k);
+    s_de += (slope(m->h, rowh, ns, k) - Ri) * F[k];
+    s_de_abs += fabs((slope(m->h, rowh, ns, k) - Ri) * F[k]);
+  }
+  for (int j = 0; j < nz; ++j) {
+    double acc = 0.0, acca = 0.0;
+    for (int k = 0; k < ns; ++k) { acc += e[k] * J[k * nz + j]; acca += fabs(e[k]) * Jabs[k * nz + j]; }
+    if (j < ns) {
+      J[ns * nz + j] = -(acc + Tdot * cv[j] * dclip[j]) / D;
+      Jabs[ns * nz + j] = (acca + fabs(Tdot * cv[j])) / fabs(D);
+    } else {
+      J[ns * nz + ns] = -(s_de + acc + Tdot * s_dcv) / D;
+      Jabs[ns * nz + ns] = (s_de_abs + acca + fabs(Tdot * s_dcv)) / fabs(D);
+    }
+  }
+}
+
+void orc_jac(const orc_mech *m, long n, long ld, const double *Z, double *F, double *jac, double *jabs) {
+  int ns = m->ns, nz = ns + 1;
+  if (m->is_wd) return; /* WD has fractional orders: no Jacobian program */
+  /* nu_kr from the net-production program */
+  double *nu = (double *)calloc((size_t)m->nr * ns, sizeof(double));
+  for (int k = 0; k < ns; ++k)
+    for (int t = m->wd_ptr[k]; t < m->wd_ptr[k + 1]; ++t) nu[(long)m->wd_rx[t] * ns + k] = m->wd_coef[t];
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    double z[ORC_MAX_NS + 1], f[ORC_MAX_NS + 1];
+    double J[(ORC_MAX_NS + 1) * (ORC_MAX_NS + 1)], Ja[(ORC_MAX_NS + 1) * (ORC_MAX_NS + 1)];
+    for (int k = 0; k < nz; ++k) z[k] = Z[k * ld + i];
+    jac_cell(m, nu, z, f, J, Ja);
+    for (int k = 0; k < nz; ++k) F[k * ld + i] = f[k];
+    for (int a = 0; a < nz * nz; ++a) {
+      jac[(long)a * ld + i] = J[a];
+      if (jabs) jabs[(long)a * ld + i] = Ja[a];
+    }
+  }
+  free(nu);
+}

@@ -1,0 +1,61 @@
+/* CPU oracle of the FLINT per-cell thermochemistry path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped library links or calls this code; it is
+ * loaded by tests/, by __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the
+ * checker and the CPU baseline.  Every function restates a routine of the FLINT reference
+ * (file:line cited at each definition) in plain C with the reference's operation order.
+ *
+ * Layouts:
+ *   tables   row-major, row = T - Tmin (FLINT stores tab(Tmin:Tmax, col) column-major; the
+ *            values and the interpolation a+(b-a)*Tdiff are the same)
+ *   states   structure of arrays with leading dimension ld (cell fastest):
+ *            rho[k*ld + i], T[i], wdot[k*ld + i], jac[(r*nz + c)*ld + i]
+ */
+#ifndef FLINT_ORACLE_H
+#define FLINT_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_mech {
+  int ns, nr, n_arrh, n_troe, n_lind, nms;
+  /* rate program (flint_amd/mechanism.py: flatten) */
+  const int *rx_kind, *rx_table, *rx_mset;
+  const int *rx_fwd_ptr, *rx_fwd_sp, *rx_fwd_pow;
+  const int *rx_rev_ptr, *rx_rev_sp, *rx_rev_pow;
+  const int *ms_ptr, *ms_sp, *ms_is_sum;
+  const double *ms_eff;                 /* NaN = bare term coi(k) */
+  const int *wd_ptr, *wd_rx;
+  const double *wd_coef;
+  const double *W;                       /* molecular weights (phase.txt) */
+  /* kinetics tables, rows Tk_min..Tk_max */
+  int Tk_min, Tk_max;
+  const double *kf, *kb;                 /* [nT][n_arrh] */
+  const double *troe;                    /* [nT][n_troe][4] k_inf,k_0,k_c,F_cent */
+  const double *lind;                    /* [nT][n_lind][3] k_inf,k_0,k_c */
+  /* thermo tables, rows Th_min..Th_max */
+  int Th_min, Th_max;
+  const double *cp, *h, *s;              /* [nT][ns] */
+  int is_wd;                             /* hand-written WD routine (WD.f90) */
+} orc_mech;
+
+/* source terms (chemistry_source) */
+void orc_wdot(const orc_mech *m, long n, long ld, const double *rho, const double *T, double *wdot,
+              double *wdot_scale);
+/* WD + heat release -sum h_k wdot_k */
+void orc_wd_hrr(const orc_mech *m, long n, long ld, const double *rho, const double *T, double *wdot,
+                double *hrr);
+/* constant-volume reactor RHS (rhs_native); Z = [rho_1..rho_ns, T] SoA with nz = ns+1 rows */
+void orc_rhs(const orc_mech *m, long n, long ld, const double *Z, double *F);
+/* analytic Jacobian dF/dZ of orc_rhs, jac[(r*nz+c)*ld+i]; jabs = sum of |partial terms| */
+void orc_jac(const orc_mech *m, long n, long ld, const double *Z, double *F, double *jac, double *jabs);
+/* complex-step Jacobian of orc_rhs (column c), step h: d F/d Z_c = Im F(Z + i h e_c)/h */
+void orc_jac_cstep(const orc_mech *m, long n, long ld, const double *Z, double h, double *jac);
+
+int orc_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

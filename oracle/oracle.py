@@ -1,0 +1,163 @@
+"""ctypes binding of the CPU checkers -- TEST INFRASTRUCTURE ONLY.
+
+Imported by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg; never by the
+flint_amd package.  Two libraries are bound:
+
+* libflint_oracle.so  the C restatement of the reference path (flint_oracle.c, oracle_bdf.c,
+                      oracle_cea.c), built by ``make -C oracle``;
+* _ref/libflint_ref.so the reference Fortran modules themselves, compiled by ``make -C oracle ref``
+                      where the reference checkout exists (never on the GPU host).
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from flint_amd.mechanism import flatten
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libflint_oracle.so")
+REF_LIB = os.path.join(HERE, "_ref", "libflint_ref.so")
+
+_P = C.c_void_p
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class _OrcMech(C.Structure):
+    _fields_ = [("ns", C.c_int), ("nr", C.c_int), ("n_arrh", C.c_int), ("n_troe", C.c_int),
+                ("n_lind", C.c_int), ("nms", C.c_int)] + \
+               [(f, _P) for f in ("rx_kind", "rx_table", "rx_mset", "rx_fwd_ptr", "rx_fwd_sp", "rx_fwd_pow",
+                                  "rx_rev_ptr", "rx_rev_sp", "rx_rev_pow", "ms_ptr", "ms_sp", "ms_is_sum",
+                                  "ms_eff", "wd_ptr", "wd_rx", "wd_coef", "W")] + \
+               [("Tk_min", C.c_int), ("Tk_max", C.c_int), ("kf", _P), ("kb", _P), ("troe", _P), ("lind", _P),
+                ("Th_min", C.c_int), ("Th_max", C.c_int), ("cp", _P), ("h", _P), ("s", _P), ("is_wd", C.c_int)]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = C.CDLL(LIB)
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_P)
+
+
+class Oracle:
+    """The C restatement bound to one mechanism's program and tables (a MechData)."""
+
+    def __init__(self, md):
+        self.md = md
+        self.ns = md.ns
+        self._keep = flatten(md.prog)
+        self._keep.update(kf=md.kf, kb=md.kb, troe=md.troe, lind=md.lind, cp=md.cp, h=md.h, s=md.s)
+        m = _OrcMech()
+        m.ns, m.nr = md.ns, len(md.prog.reactions)
+        m.n_arrh, m.n_troe, m.n_lind, m.nms = md.n_arrh, md.n_troe, md.n_lind, len(md.prog.m_sets)
+        for k, v in self._keep.items():
+            if hasattr(m, k) and isinstance(v, np.ndarray):
+                setattr(m, k, _ptr(v))
+        m.Tk_min, m.Tk_max, m.Th_min, m.Th_max = md.Tk_min, md.Tk_max, md.Th_min, md.Th_max
+        m.is_wd = int(md.is_wd)
+        self.m = m
+        self.L = lib()
+
+    def wdot(self, rho, T):
+        """rho [ns, n] SoA, T [n] -> (wdot [ns, n], scale [ns, n])."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        n = T.size
+        w = np.empty_like(rho)
+        sc = np.empty_like(rho)
+        self.L.orc_wdot(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(rho), _ptr(T), _ptr(w), _ptr(sc))
+        return w, sc
+
+    def wd_hrr(self, rho, T):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        n = T.size
+        w = np.empty_like(rho)
+        q = np.empty(n)
+        self.L.orc_wd_hrr(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(rho), _ptr(T), _ptr(w), _ptr(q))
+        return w, q
+
+    def rhs(self, Z):
+        Z = np.ascontiguousarray(Z, dtype=np.float64)
+        n = Z.shape[1]
+        F = np.empty_like(Z)
+        self.L.orc_rhs(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(Z), _ptr(F))
+        return F
+
+    def jac(self, Z):
+        """Z [nz, n] -> (F [nz, n], J [nz*nz, n] row-major per cell, |J| scale)."""
+        Z = np.ascontiguousarray(Z, dtype=np.float64)
+        nz, n = Z.shape
+        F = np.empty_like(Z)
+        J = np.empty((nz * nz, n))
+        Ja = np.empty((nz * nz, n))
+        self.L.orc_jac(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(Z), _ptr(F), _ptr(J), _ptr(Ja))
+        return F, J, Ja
+
+    def jac_cstep(self, Z, h=1e-30):
+        Z = np.ascontiguousarray(Z, dtype=np.float64)
+        nz, n = Z.shape
+        J = np.empty((nz * nz, n))
+        self.L.orc_jac_cstep(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(Z), C.c_double(h), _ptr(J))
+        return J
+
+    def num_threads(self):
+        return self.L.orc_num_threads()
+
+
+class Reference:
+    """The reference Fortran modules (oracle/_ref), loaded with one mechanism's tables."""
+
+    def __init__(self, md):
+        if not os.path.exists(REF_LIB):
+            raise FileNotFoundError(REF_LIB)
+        self.L = C.CDLL(REF_LIB)
+        self.md = md
+        ns = md.ns
+        nTk = md.kf.shape[0]
+        nTh = md.cp.shape[0]
+        names = "".join(s.ljust(20) for s in md.species).encode()
+        els = "".join(e.ljust(20) for e in md.elements).encode() or b" " * 20
+        comp = np.ascontiguousarray(md.composition.T)  # Fortran (ne, ns) column-major
+        troe = md.troe if md.n_troe else np.zeros((nTk, 1, 4))
+        lind = md.lind if md.n_lind else np.zeros((nTk, 1, 3))
+        self._keep = [md.W, md.kf, md.kb, troe, lind, md.cp, md.h, md.s, comp]
+        mech = md.desc["name"].encode()
+        self.L.ref_setup(C.c_int(ns), _ptr(md.W), C.c_int(nTk), C.c_int(md.Tk_min), C.c_int(md.n_arrh), _ptr(md.kf),
+                         _ptr(md.kb), C.c_int(md.n_troe), _ptr(np.ascontiguousarray(troe)), C.c_int(md.n_lind),
+                         _ptr(np.ascontiguousarray(lind)), C.c_int(nTh), C.c_int(md.Th_min), _ptr(md.cp),
+                         _ptr(md.h), _ptr(md.s), C.c_int(len(md.elements)), _ptr(comp), C.c_char_p(names),
+                         C.c_char_p(els), C.c_char_p(mech), C.c_int(len(mech)))
+
+    def wdot(self, rho, T):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        n = T.size
+        w = np.empty_like(rho)
+        self.L.ref_wdot(C.c_long(n), C.c_long(n), _ptr(rho), _ptr(T), _ptr(w))
+        return w
+
+    def cea(self, p, T, rho, hp=False):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        n = T.size
+        Tout = np.empty(n)
+        rout = np.empty_like(rho)
+        self.L.ref_cea(C.c_long(n), C.c_long(n), _ptr(np.ascontiguousarray(p, dtype=np.float64)),
+                       _ptr(np.ascontiguousarray(T, dtype=np.float64)), _ptr(rho), C.c_int(int(hp)), _ptr(Tout),
+                       _ptr(rout))
+        return Tout, rout
