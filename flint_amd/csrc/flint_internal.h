@@ -1,0 +1,38 @@
+// Internal definition of the opaque mechanism handle shared by the C-ABI translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "flint_registry.h"
+
+struct flint_extra;  // per-handle state of the reactor / equilibrium / MLP paths
+
+struct flint_mech {
+  std::string name, id;
+  int ns = 0, ne = 0;
+  int Tk_min = 0, Tk_max = 0, Th_min = 0, Th_max = 0;
+  int n_arrh = 0, n_troe = 0, n_lind = 0, wk = 0;
+  bool is_wd = false;
+  int device = 0;
+  const flint_mech_ops* ops = nullptr;
+  std::vector<double> W, comp;       // comp[ne*ns], species_composition(ne, ns) column-major
+  std::vector<double> h_tabk, h_tabh;  // host copies of the device tables
+  double* d_tabk = nullptr;
+  double* d_tabh = nullptr;
+  double* d_W = nullptr;
+  // host-pointer convenience calls
+  hipStream_t stream = nullptr;
+  double* d_stage = nullptr;
+  size_t stage_cap = 0;
+  std::mutex mu;
+  flint_extra* extra = nullptr;
+};
+
+int flint_fail(int code, const std::string& msg);
+std::string flint_mech_id(const std::string& name);
+int flint_mech_upload(flint_mech* m);
+int flint_stage(flint_mech* m, size_t doubles);
+void flint_mech_release_extra(flint_mech* m);

@@ -1,0 +1,18 @@
+// Registry of compiled mechanisms: each generated translation unit registers its launchers at
+// load time; the C-ABI looks them up by mechanism id (FLINT's Assign_Mechanism name, lower-cased
+// with non-alphanumerics as '_', Lib_Chemistry_wdot.f90:24-92).
+#pragma once
+#include <hip/hip_runtime.h>
+
+struct flint_mech_ops {
+  const char* id;
+  int ns;
+  int wk;  // kinetics table row width (doubles)
+  hipError_t (*wdot)(long n, long ld, const double* rho, const double* T, double* wdot, int* status,
+                     const double* tabk, int Tk_min, int Tk_max, hipStream_t s);
+  hipError_t (*rhs_jac)(long n, long ld, const double* Z, double* F, double* J, int* status, const double* tabk,
+                        int Tk_min, int Tk_max, const double* tabh, int Th_min, int Th_max, hipStream_t s);
+};
+
+int flint_register_mech(const flint_mech_ops* ops);
+const flint_mech_ops* flint_find_mech(const char* id);

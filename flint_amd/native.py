@@ -1,0 +1,151 @@
+"""ctypes binding of libflint_amd.so (the C ABI declared in include/flint_amd.h).
+
+Device arrays are torch tensors on a ROCm device (torch is used for HBM allocation and streams
+only); their data pointers go straight to the C ABI.  The library must have been built with
+``python -m flint_amd.build``: nothing here falls back to a CPU path.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libflint_amd.so")
+
+_lib = None
+
+FLINT_OK = 0
+ERRORS = {1: "invalid argument", 2: "HIP error", 3: "input file error", 4: "unknown mechanism", 5: "unsupported"}
+
+
+class FlintError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FlintError(f"{LIB_PATH} is missing: build it with `python -m flint_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, i, l, d = C.c_void_p, C.c_int, C.c_long, C.c_double
+        L.flint_last_error.restype = C.c_char_p
+        L.flint_mech_create.argtypes = [C.c_char_p, i, vp, i, i, i, vp, vp, i, vp, i, vp, i, i, vp, vp, i, vp,
+                                        C.POINTER(vp)]
+        L.flint_mech_load.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(vp)]
+        L.flint_mech_destroy.argtypes = [vp]
+        L.flint_mech_query.argtypes = [vp] + [C.POINTER(i)] * 5
+        L.flint_source.argtypes = [vp, l, l, vp, vp, vp, vp, vp, vp]
+        L.flint_rhs.argtypes = [vp, l, l, vp, vp, vp, vp, vp]
+        L.flint_source_host.argtypes = [vp, l, vp, vp, vp]
+        L.flint_rhs_host.argtypes = [vp, l, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != FLINT_OK:
+        msg = lib().flint_last_error().decode(errors="replace")
+        raise FlintError(f"flint error {rc} ({ERRORS.get(rc, '?')}): {msg}")
+
+
+def _p(a):
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data_as(C.c_void_p)
+    return C.c_void_p(a.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if isinstance(stream, int):
+        return C.c_void_p(stream)
+    return C.c_void_p(stream.cuda_stream)
+
+
+def _fortran(a):
+    """numpy [nT, ncol] (row = T) -> FLINT column-major tab(T, col) as a flat C buffer."""
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).T)
+
+
+class Mechanism:
+    """Opaque mechanism handle: tables on the device plus the compiled kernels for its name."""
+
+    def __init__(self, handle, ns, name):
+        self._h = handle
+        self.ns = ns
+        self.nz = ns + 1
+        self.name = name
+
+    @classmethod
+    def from_data(cls, md):
+        """From a flint_amd.mechdata.MechData (generated tables)."""
+        L = lib()
+        nTk = md.kf.shape[0]
+        kf, kb = _fortran(md.kf), _fortran(md.kb)
+        troe = np.ascontiguousarray(np.transpose(md.troe, (2, 1, 0))) if md.n_troe else None
+        lind = np.ascontiguousarray(np.transpose(md.lind, (2, 1, 0))) if md.n_lind else None
+        cp, h = _fortran(md.cp), _fortran(md.h)
+        comp = np.ascontiguousarray(md.composition.T) if md.composition.size else None
+        W = np.ascontiguousarray(md.W)
+        out = C.c_void_p()
+        check(L.flint_mech_create(md.desc["name"].encode(), md.ns, _p(W), md.Tk_min, md.Tk_min + nTk - 1, md.n_arrh,
+                                  _p(kf), _p(kb), md.n_troe, _p(troe), md.n_lind, _p(lind), md.Th_min, md.Th_max,
+                                  _p(cp), _p(h), len(md.elements), _p(comp), C.byref(out)))
+        return cls(out, md.ns, md.desc["name"])
+
+    @classmethod
+    def load(cls, folder, prefix=""):
+        """From FLINT's native input folder (read_idealgas_thermo + read_chemistry + Assign_Mechanism)."""
+        L = lib()
+        out = C.c_void_p()
+        check(L.flint_mech_load(str(folder).encode(), prefix.encode(), C.byref(out)))
+        ns = C.c_int()
+        check(L.flint_mech_query(out, C.byref(ns), None, None, None, None))
+        return cls(out, ns.value, os.path.basename(str(folder)))
+
+    def query(self):
+        v = [C.c_int() for _ in range(5)]
+        check(lib().flint_mech_query(self._h, *[C.byref(x) for x in v]))
+        return dict(zip(("ns", "Tk_min", "Tk_max", "Th_min", "Th_max"), (x.value for x in v)))
+
+    def close(self):
+        if self._h:
+            lib().flint_mech_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- batched device calls (torch tensors, SoA [rows, ld]) ----
+    def source(self, rho, T, wdot, hrr=None, status=None, n=None, stream=None):
+        ld = rho.shape[-1]
+        n = ld if n is None else n
+        check(lib().flint_source(self._h, n, ld, _p(rho), _p(T), _p(wdot), _p(hrr), _p(status), _stream(stream)))
+
+    def rhs(self, Z, F, J=None, status=None, n=None, stream=None):
+        ld = Z.shape[-1]
+        n = ld if n is None else n
+        check(lib().flint_rhs(self._h, n, ld, _p(Z), _p(F), _p(J), _p(status), _stream(stream)))
+
+    # ---- host-memory convenience (numpy, SoA [rows, n]) ----
+    def source_host(self, rho, T):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        w = np.empty_like(rho)
+        check(lib().flint_source_host(self._h, T.size, _p(rho), _p(T), _p(w)))
+        return w
+
+    def rhs_host(self, Z, jac=False):
+        Z = np.ascontiguousarray(Z, dtype=np.float64)
+        n = Z.shape[1]
+        F = np.empty_like(Z)
+        J = np.empty((self.nz * self.nz, n)) if jac else None
+        check(lib().flint_rhs_host(self._h, n, _p(Z), _p(F), _p(J)))
+        return (F, J) if jac else F

@@ -1,0 +1,90 @@
+/* flint_amd -- MI355X (gfx950) implementation of FLINT's per-cell thermochemistry hot path.
+ *
+ * C ABI.  Plain pointers and sizes; every function returns 0 on success or a FLINT_E* code, with
+ * a message available from flint_last_error().  "dev" pointers are device (HBM) pointers,
+ * "host" pointers are host memory.  Batched state arrays are structure-of-arrays with leading
+ * dimension ld >= n (cell fastest): species k of cell i is at a[k*ld + i].  A stream argument is a
+ * hipStream_t passed as void* (NULL = the default stream); calls are asynchronous on it unless
+ * stated otherwise.
+ *
+ * The FLINT reference keeps one mechanism in module globals; here a mechanism is an opaque handle,
+ * and every batched call on distinct handles or distinct output arrays may run concurrently.
+ */
+#ifndef FLINT_AMD_H
+#define FLINT_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct flint_mech flint_mech;
+
+enum {
+  FLINT_OK = 0,
+  FLINT_EARG = 1,         /* invalid argument */
+  FLINT_EHIP = 2,         /* HIP runtime error */
+  FLINT_EIO = 3,          /* missing or malformed input file */
+  FLINT_EMECH = 4,        /* no compiled kernel for this mechanism name */
+  FLINT_EUNSUPPORTED = 5  /* operation not available for this mechanism */
+};
+
+/* Per-cell status bits written by the batched calls (status may be NULL). */
+#define FLINT_ST_T_KINETICS 1 /* T outside the kinetics table: the first/last bin was used */
+#define FLINT_ST_T_THERMO 2   /* T outside [Tmin, Tmax]: rhs_native's F = -1 convention */
+
+const char *flint_last_error(void);
+
+/* Build a mechanism from tables already in memory, in FLINT's own layout.
+ * Replaces read_idealgas_thermo (Load_ThermoTransport.f90:7-89), read_chemistry
+ * (Load_Chemistry.f90:7-164) and Assign_Mechanism (Lib_Chemistry_wdot.f90:24-92) for callers that
+ * hold the module arrays already.
+ *   mech_name     FLINT mechanism name ("WD", "TSR-GP-24", "TSR-Rich-31")
+ *   W[ns]         molecular weights (wm_tab)
+ *   kf, kb        kf_tab/kb_tab(Tk_min:Tk_max, n_arrh), column-major (T fastest)
+ *   troe          [4][n_troe][nTk]: kinf_troe_tab, k0_troe_tab, kc_troe_tab, Fcent_tab (may be NULL if 0)
+ *   lind          [3][n_lind][nTk]: kinf_lind_tab, k0_lind_tab, kc_lind_tab (may be NULL if 0)
+ *   cp, h         cp_tab/h_tab(Th_min:Th_max, ns), column-major
+ *   ne, comp      element count and species_composition(ne, ns), column-major (may be 0 / NULL)
+ * Host pointers; the tables are copied to the device in the kernels' layout. */
+int flint_mech_create(const char *mech_name, int ns, const double *W, int Tk_min, int Tk_max, int n_arrh,
+                      const double *kf, const double *kb, int n_troe, const double *troe, int n_lind,
+                      const double *lind, int Th_min, int Th_max, const double *cp, const double *h, int ne,
+                      const double *comp, flint_mech **out);
+
+/* Read FLINT's native input folder (phase.txt, composition.txt, chemistry-info.txt, thermo.dat,
+ * chemistry-{Arrhenius,Troe,Lindemann}.dat; Tecplot ASCII only, .szplt is not supported) and
+ * select the compiled kernels for the mechanism named in chemistry-info.txt.
+ * Replaces read_idealgas_thermo(folder) + read_chemistry(folder, mech_name) + Assign_Mechanism;
+ * prefix plays the role of FLINT_phase_prefix (Lib_ThermoTransport.f90:8). */
+int flint_mech_load(const char *folder, const char *prefix, flint_mech **out);
+
+void flint_mech_destroy(flint_mech *m);
+
+/* ns, and the table ranges [Tk_min, Tk_max] (kinetics) and [Th_min, Th_max] (thermo). */
+int flint_mech_query(const flint_mech *m, int *ns, int *Tk_min, int *Tk_max, int *Th_min, int *Th_max);
+
+/* chemistry_source over a batch (Lib_Chemistry_wdot.f90:6-20 procedure pointer, e.g.
+ * TSR-GP-24.f90:4-415): rho_dev[ns][ld], T_dev[n] -> wdot_dev[ns][ld] in kg/m^3/s.
+ * hrr_dev (may be NULL) receives the heat release rate -sum_k h_k(T) wdot_k [W/m^3].
+ * The hand-written WD routine clips rho at 0 (WD.f90:18-21); the batch leaves rho_dev untouched. */
+int flint_source(const flint_mech *m, long n, long ld, const double *rho_dev, const double *T_dev, double *wdot_dev,
+                 double *hrr_dev, int *status_dev, void *stream);
+
+/* rhs_native over a batch (Lib_Chemistry_rhs.f90:15-52): Z_dev[ns+1][ld] = (rho_1..rho_ns, T)
+ * -> F_dev[ns+1][ld] = (wdot, dT/dt) of the constant-volume reactor; F = -1 where T is outside the
+ * thermo table.  With J_dev non-NULL also the analytic Jacobian dF/dZ,
+ * J_dev[(r*(ns+1)+c)*ld + i] = dF_r/dZ_c (no reference counterpart: FLINT's integrator builds
+ * its own; this is the exact derivative of the table-interpolated model). */
+int flint_rhs(const flint_mech *m, long n, long ld, const double *Z_dev, double *F_dev, double *J_dev, int *status_dev,
+              void *stream);
+
+/* Host-memory convenience for the per-cell drop-in (Fortran chemistry_source / rhs_native
+ * callers): copies in, runs the batched kernel on the library's own stream, copies out and
+ * synchronises.  Arrays as in flint_source / flint_rhs with ld = n. */
+int flint_source_host(flint_mech *m, long n, const double *rho, const double *T, double *wdot);
+int flint_rhs_host(flint_mech *m, long n, const double *Z, double *F, double *J);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

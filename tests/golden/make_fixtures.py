@@ -1,0 +1,73 @@
+"""Writes the committed golden fixtures from the FLINT reference (run where the reference exists).
+
+    make -C oracle ref && python tests/golden/make_fixtures.py
+
+* tables_ref.npz   samples of the tables FLINT ships (test/WD/INPUT/chemistry-Arrhenius.dat,
+                   test/Pelucchi/INPUT/{thermo,chemistry-Troe,chemistry-Lindemann}.dat): data
+                   only, used to pin this project's table generator.
+* wdot_<mech>.npz  chemistry_source outputs of the reference Fortran routines themselves
+                   (oracle/_ref: WD.f90, TSR-GP-24.f90, TSR-Rich-31.f90 compiled by flang) on seeded
+                   states; the inputs are regenerated from (seed, start, count) by flint_amd.states.
+"""
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+from flint_amd import states as S  # noqa: E402
+from flint_amd import tables as TB  # noqa: E402
+from flint_amd.mechdata import mech_data  # noqa: E402
+import oracle as O  # noqa: E402
+
+REF = "/root/reference/test"
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+WDOT_CASES = {
+    # mech: (generator, seed, start, count, kwargs)
+    "WD": ("wd", 11, 0, 2048, {}),
+    "TSR-GP-24": ("radical", 12, 0, 1024, {}),
+    "TSR-Rich-31": ("radical", 13, 0, 1024, {"ignition_half": True}),
+}
+
+
+def states_for(md, case):
+    gen, seed, start, count, kw = case
+    if gen == "wd":
+        return S.wd_states(md.species, md.W, seed, start, count)
+    return S.radical_states(md.species, md.W, seed, start, count, **kw)
+
+
+def tables():
+    rows = np.arange(0, 15000, 37)
+    T, z = TB.read_tecplot(os.path.join(REF, "WD/INPUT/chemistry-Arrhenius.dat"))
+    wd = np.stack(z, axis=1)[rows]                     # [nrow, 3, 2]
+    prow = np.arange(0, 3000, 7)
+    T2, z = TB.read_tecplot(os.path.join(REF, "Pelucchi/INPUT/thermo.dat"))
+    pth = np.stack(z, axis=1)[prow]                    # [nrow, 25, 4]
+    T3, z = TB.read_tecplot(os.path.join(REF, "Pelucchi/INPUT/chemistry-Troe.dat"))
+    ptr = np.stack(z, axis=1)[prow]
+    T4, z = TB.read_tecplot(os.path.join(REF, "Pelucchi/INPUT/chemistry-Lindemann.dat"))
+    pli = np.stack(z, axis=1)[prow]
+    np.savez_compressed(os.path.join(OUT, "tables_ref.npz"), wd_T=T[rows], wd_arrh=wd, pel_T=T2[prow], pel_thermo=pth,
+                        pel_troe=ptr, pel_lind=pli)
+
+
+def wdot():
+    for mech, case in WDOT_CASES.items():
+        md = mech_data(mech)
+        st = states_for(md, case)
+        rho = S.soa(st["rho"])
+        ref = O.Reference(md)
+        w = ref.wdot(rho, st["T"])
+        gen, seed, start, count, kw = case
+        np.savez_compressed(os.path.join(OUT, f"wdot_{mech}.npz"), wdot=w, seed=seed, start=start, count=count,
+                            gen=gen, ignition_half=bool(kw.get("ignition_half", False)))
+        print(mech, w.shape)
+
+
+if __name__ == "__main__":
+    tables()
+    wdot()
