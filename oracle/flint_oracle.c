@@ -203,18 +203,13 @@ static double prod_only(const int *ptr, const int *sp, const int *pw, int r, con
   return v;
 }
 
-static void jac_cell(const orc_mech *m, const double *nu, const double *Z, double *F, double *J, double *Jabs) {
-  int ns = m->ns, nz = ns + 1;
-  double roi[ORC_MAX_NS], coi[ORC_MAX_NS], dclip[ORC_MAX_NS];
-  double T = Z[ns];
-  rhs_cell_r(m, Z, F);
-  for (int a = 0; a < nz * nz; ++a) { J[a] = 0.0; Jabs[a] = 0.0; }
-  if (F[ns] == -1.0 && (T < m->Th_min || T > m->Th_max || isnan(T))) return;
-  for (int k = 0; k < ns; ++k) {
-    roi[k] = Z[k] < 0.0 ? 0.0 : Z[k];
-    dclip[k] = Z[k] < 0.0 ? 0.0 : 1.0;
-    coi[k] = roi[k] / m->W[k];
-  }
+/* Species block of the Jacobian at clipped densities roi (all >= 0):
+ * A[k*ns+j] = d wdot_k / d rho_j, b[k] = d wdot_k / dT at fixed rho (with |.| scales). */
+static void jac_core(const orc_mech *m, const double *nu, const double *roi, double T, double *A, double *Aa,
+                     double *b, double *ba) {
+  int ns = m->ns;
+  double coi[ORC_MAX_NS];
+  for (int k = 0; k < ns; ++k) coi[k] = roi[k] / m->W[k];
   int T_i = (int)T;
   double Tdiff = T - T_i;
   long row = T_i - m->Tk_min;
@@ -287,15 +282,40 @@ static void jac_cell(const orc_mech *m, const double *nu, const double *Z, doubl
       dwdT_abs[k] += fabs(v) * dqTa;
     }
   }
+  for (int k = 0; k < ns; ++k) {
+    for (int j = 0; j < ns; ++j) {
+      double f = m->W[k] / m->W[j];
+      A[k * ns + j] = f * dwdc[k][j];
+      Aa[k * ns + j] = fabs(f) * dwdc_abs[k][j];
+    }
+    b[k] = m->W[k] * dwdT[k];
+    ba[k] = m->W[k] * dwdT_abs[k];
+  }
+}
+
+static void jac_cell(const orc_mech *m, const double *nu, const double *Z, double *F, double *J, double *Jabs) {
+  int ns = m->ns, nz = ns + 1;
+  double roi[ORC_MAX_NS], dclip[ORC_MAX_NS];
+  double T = Z[ns];
+  rhs_cell_r(m, Z, F);
+  for (int a = 0; a < nz * nz; ++a) { J[a] = 0.0; Jabs[a] = 0.0; }
+  if (F[ns] == -1.0 && (T < m->Th_min || T > m->Th_max || isnan(T))) return;
+  for (int k = 0; k < ns; ++k) {
+    roi[k] = Z[k] < 0.0 ? 0.0 : Z[k];
+    dclip[k] = Z[k] < 0.0 ? 0.0 : 1.0;
+  }
+  double A[ORC_MAX_NS * ORC_MAX_NS], Aa[ORC_MAX_NS * ORC_MAX_NS], b[ORC_MAX_NS], ba[ORC_MAX_NS];
+  jac_core(m, nu, roi, T, A, Aa, b, ba);
+  int T_i = (int)T;
+  double Tdiff = T - T_i;
   /* species rows */
   for (int k = 0; k < ns; ++k) {
     for (int j = 0; j < ns; ++j) {
-      double f = m->W[k] * dclip[j] / m->W[j];
-      J[k * nz + j] = f * dwdc[k][j];
-      Jabs[k * nz + j] = fabs(f) * dwdc_abs[k][j];
+      J[k * nz + j] = dclip[j] * A[k * ns + j];
+      Jabs[k * nz + j] = dclip[j] * Aa[k * ns + j];
     }
-    J[k * nz + ns] = m->W[k] * dwdT[k];
-    Jabs[k * nz + ns] = m->W[k] * dwdT_abs[k];
+    J[k * nz + ns] = b[k];
+    Jabs[k * nz + ns] = ba[k];
   }
   /* temperature row */
   long rowh = T_i - m->Th_min;
@@ -343,4 +363,124 @@ void orc_jac(const orc_mech *m, long n, long ld, const double *Z, double *F, dou
     }
   }
   free(nu);
+}
+
+/* Constant-pressure Jacobian d(dY/dt, dT/dt)/d(Y, T) from the species block at rho_k = rho Y_k:
+ *   dF_k/dY_j = g_j [A_kj - (A Y)_k R_j/R + (wdot_k/rho) R_j/R]
+ *   dF_k/dT   = (b_k - (A rho)_k / T)/rho + wdot_k/(rho T)
+ *   dF_T/dY_j = -sum_k h_k dwdot_k/dY_j /(rho cp) - F_T g_j (cp_j/cp - R_j/R)
+ *   dF_T/dT   = -(sum_k h'_k wdot_k + h_k dwdot_k/dT)/(rho cp) - F_T (sum_k Y_k cp'_k/cp - 1/T)
+ * with dwdot_k/dY_j = rho g_j (A_kj - (A Y)_k R_j/R), dwdot_k/dT = b_k - (A rho)_k/T, R = R_mix. */
+static void jac_cp_cell(const orc_mech *m, const double *nu, const double *x, double p, double *F, double *J) {
+  int ns = m->ns, nz = ns + 1;
+  double T = x[ns];
+  rhs_cp_cell_r(m, x, p, F);
+  for (int a = 0; a < nz * nz; ++a) J[a] = 0.0;
+  if (T < m->Th_min || T > m->Th_max || isnan(T)) return;
+  double Yc[ORC_MAX_NS], g[ORC_MAX_NS], R[ORC_MAX_NS], roi[ORC_MAX_NS], Rmix = 0.0;
+  for (int k = 0; k < ns; ++k) {
+    Yc[k] = x[k] < 0.0 ? 0.0 : x[k];
+    g[k] = x[k] < 0.0 ? 0.0 : 1.0;
+    R[k] = ORC_RUNIV / m->W[k];
+    Rmix += Yc[k] * R[k];
+  }
+  double rho = p / (Rmix * T);
+  for (int k = 0; k < ns; ++k) roi[k] = rho * Yc[k];
+  double A[ORC_MAX_NS * ORC_MAX_NS], Aa[ORC_MAX_NS * ORC_MAX_NS], b[ORC_MAX_NS], ba[ORC_MAX_NS];
+  jac_core(m, nu, roi, T, A, Aa, b, ba);
+  int T_i = (int)T;
+  double Tdiff = T - T_i;
+  long row = T_i - m->Th_min;
+  double h[ORC_MAX_NS], cp[ORC_MAX_NS], dh[ORC_MAX_NS], dcp[ORC_MAX_NS], cpm = 0.0, sYdcp = 0.0;
+  for (int k = 0; k < ns; ++k) {
+    h[k] = interp_r(m->h, row, ns, k, Tdiff);
+    cp[k] = interp_r(m->cp, row, ns, k, Tdiff);
+    dh[k] = slope(m->h, row, ns, k);
+    dcp[k] = slope(m->cp, row, ns, k);
+    cpm += Yc[k] * cp[k];
+    sYdcp += Yc[k] * dcp[k];
+  }
+  double FT = F[ns];
+  double colh[ORC_MAX_NS + 1];
+  for (int j = 0; j < nz; ++j) colh[j] = 0.0;
+  double s_dhw = 0.0;
+  for (int k = 0; k < ns; ++k) {
+    double AY = 0.0;
+    for (int j = 0; j < ns; ++j) AY += A[k * ns + j] * Yc[j];
+    double w = F[k] * rho;
+    for (int j = 0; j < ns; ++j) {
+      double dwdY = rho * g[j] * (A[k * ns + j] - AY * R[j] / Rmix);
+      J[k * nz + j] = g[j] * (A[k * ns + j] - AY * R[j] / Rmix + (w / rho) * R[j] / Rmix);
+      colh[j] += h[k] * dwdY;
+    }
+    double dwdT = b[k] - rho * AY / T;
+    J[k * nz + ns] = dwdT / rho + w / (rho * T);
+    colh[ns] += h[k] * dwdT;
+    s_dhw += dh[k] * w;
+  }
+  for (int j = 0; j < ns; ++j) J[ns * nz + j] = -colh[j] / (rho * cpm) - FT * g[j] * (cp[j] / cpm - R[j] / Rmix);
+  J[ns * nz + ns] = -(s_dhw + colh[ns]) / (rho * cpm) - FT * (sYdcp / cpm - 1.0 / T);
+}
+
+double *orc_nu_alloc(const orc_mech *m) {
+  int ns = m->ns;
+  double *nu = (double *)calloc((size_t)m->nr * ns, sizeof(double));
+  for (int k = 0; k < ns; ++k)
+    for (int t = m->wd_ptr[k]; t < m->wd_ptr[k + 1]; ++t) nu[(long)m->wd_rx[t] * ns + k] = m->wd_coef[t];
+  return nu;
+}
+
+/* one cell, mode 0 = constant volume (x = rho_k, T), 1 = constant pressure (x = Y_k, T) */
+void orc_rhs_cell(const orc_mech *m, int mode, const double *x, double p, double *F) {
+  if (mode == 0) rhs_cell_r(m, x, F);
+  else rhs_cp_cell_r(m, x, p, F);
+}
+
+void orc_jac_cell(const orc_mech *m, const double *nu, int mode, const double *x, double p, double *F, double *J) {
+  if (mode == 0) {
+    double Ja[(ORC_MAX_NS + 1) * (ORC_MAX_NS + 1)];
+    jac_cell(m, nu, x, F, J, Ja);
+  } else {
+    jac_cp_cell(m, nu, x, p, F, J);
+  }
+}
+
+void orc_rhs_cp(const orc_mech *m, long n, long ld, const double *X, const double *p, double *F) {
+  int nz = m->ns + 1;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    double z[ORC_MAX_NS + 1], f[ORC_MAX_NS + 1];
+    for (int k = 0; k < nz; ++k) z[k] = X[k * ld + i];
+    rhs_cp_cell_r(m, z, p[i], f);
+    for (int k = 0; k < nz; ++k) F[k * ld + i] = f[k];
+  }
+}
+
+void orc_jac_cp(const orc_mech *m, long n, long ld, const double *X, const double *p, double *F, double *jac) {
+  int nz = m->ns + 1;
+  if (m->is_wd) return;
+  double *nu = orc_nu_alloc(m);
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    double z[ORC_MAX_NS + 1], f[ORC_MAX_NS + 1], J[(ORC_MAX_NS + 1) * (ORC_MAX_NS + 1)];
+    for (int k = 0; k < nz; ++k) z[k] = X[k * ld + i];
+    jac_cp_cell(m, nu, z, p[i], f, J);
+    for (int k = 0; k < nz; ++k) F[k * ld + i] = f[k];
+    for (int a = 0; a < nz * nz; ++a) jac[(long)a * ld + i] = J[a];
+  }
+  free(nu);
+}
+
+void orc_jac_cp_cstep(const orc_mech *m, long n, long ld, const double *X, const double *p, double h, double *jac) {
+  int nz = m->ns + 1;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    double _Complex z[ORC_MAX_NS + 1], f[ORC_MAX_NS + 1];
+    for (int c = 0; c < nz; ++c) {
+      for (int k = 0; k < nz; ++k) z[k] = X[k * ld + i];
+      z[c] += I * h;
+      rhs_cp_cell_c(m, z, p[i], f);
+      for (int r = 0; r < nz; ++r) jac[((long)r * nz + c) * ld + i] = cimag(f[r]) / h;
+    }
+  }
 }

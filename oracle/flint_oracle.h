@@ -53,6 +53,20 @@ void orc_jac(const orc_mech *m, long n, long ld, const double *Z, double *F, dou
 /* complex-step Jacobian of orc_rhs (column c), step h: d F/d Z_c = Im F(Z + i h e_c)/h */
 void orc_jac_cstep(const orc_mech *m, long n, long ld, const double *Z, double h, double *jac);
 
+/* constant-pressure reactor: X = [Y_1..Y_ns, T] SoA, p[n] in Pa */
+void orc_rhs_cp(const orc_mech *m, long n, long ld, const double *X, const double *p, double *F);
+void orc_jac_cp(const orc_mech *m, long n, long ld, const double *X, const double *p, double *F, double *jac);
+void orc_jac_cp_cstep(const orc_mech *m, long n, long ld, const double *X, const double *p, double h, double *jac);
+
+/* single-cell helpers used by the BDF oracle (mode 0 = constant volume, 1 = constant pressure) */
+double *orc_nu_alloc(const orc_mech *m);
+void orc_rhs_cell(const orc_mech *m, int mode, const double *x, double p, double *F);
+void orc_jac_cell(const orc_mech *m, const double *nu, int mode, const double *x, double p, double *F, double *J);
+
+/* BDF advance of n reactors over [0, t_end] (oracle_bdf.c); stats[i*8..] and status[i] per cell */
+void orc_bdf(const orc_mech *m, long n, long ld, int mode, double *X, const double *p, double rtol, double atol,
+             double t_end, int max_steps, int *stats, int *status);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

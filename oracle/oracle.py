@@ -116,6 +116,42 @@ class Oracle:
         self.L.orc_jac_cstep(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(Z), C.c_double(h), _ptr(J))
         return J
 
+    def rhs_cp(self, X, p):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        n = X.shape[1]
+        F = np.empty_like(X)
+        self.L.orc_rhs_cp(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(X), _ptr(p), _ptr(F))
+        return F
+
+    def jac_cp(self, X, p):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        nz, n = X.shape
+        F = np.empty_like(X)
+        J = np.empty((nz * nz, n))
+        self.L.orc_jac_cp(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(X), _ptr(p), _ptr(F), _ptr(J))
+        return F, J
+
+    def jac_cp_cstep(self, X, p, h=1e-30):
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        nz, n = X.shape
+        J = np.empty((nz * nz, n))
+        self.L.orc_jac_cp_cstep(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(X), _ptr(p), C.c_double(h), _ptr(J))
+        return J
+
+    def bdf(self, X, p=None, mode=0, rtol=1e-6, atol=1e-12, t_end=1e-4, max_steps=100000):
+        """Advance reactors (X [nz, n], mode 0 = (rho, T) constant volume, 1 = (Y, T) constant p)."""
+        X = np.array(X, dtype=np.float64, order="C", copy=True)
+        nz, n = X.shape
+        pp = np.ascontiguousarray(p if p is not None else np.zeros(n), dtype=np.float64)
+        stats = np.zeros((n, 8), dtype=np.int32)
+        status = np.zeros(n, dtype=np.int32)
+        self.L.orc_bdf(C.byref(self.m), C.c_long(n), C.c_long(n), C.c_int(mode), _ptr(X), _ptr(pp), C.c_double(rtol),
+                       C.c_double(atol), C.c_double(t_end), C.c_int(max_steps), _ptr(stats), _ptr(status))
+        return X, stats, status
+
     def num_threads(self):
         return self.L.orc_num_threads()
 
