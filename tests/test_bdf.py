@@ -91,15 +91,17 @@ def test_bdf_oracle_invariants(oracle_mod, mechs, mode):
 
 
 def test_bdf_oracle_accuracy_against_tight_radau(oracle_mod, mechs):
-    """Global error of the rtol 1e-6 BDF against Radau at rtol 1e-11 (stated, loosely bounded)."""
+    """Global error against Radau at rtol 1e-11.  At rtol 1e-6 igniting cells can miss the
+    ignition time by enough to be 100+ K off at t_end (the piecewise-linear tables make the RHS
+    only C0 in T); at rtol 1e-8 the method converges to the Radau solution."""
     from scipy.integrate import solve_ivp
     md = mechs["TSR-Rich-31"]
     o = oracle_mod.Oracle(md)
     X, p = _initial(md, 1, 4, 79)
-    Xo, stats, status = o.bdf(X, p, mode=1, rtol=1e-6, atol=1e-12, t_end=1e-4)
+    Xo, stats, status = o.bdf(X, p, mode=1, rtol=1e-8, atol=1e-14, t_end=1e-4)
     for i in range(X.shape[1]):
         f, jf = _system(o, 1, p[i])
         r = solve_ivp(f, (0.0, 1e-4), X[:, i], method="Radau", rtol=1e-11, atol=1e-18, jac=jf)
         ref = r.y[:, -1]
-        assert abs(Xo[-1, i] - ref[-1]) < 2.0          # K
-        assert np.max(np.abs(Xo[:-1, i] - ref[:-1])) < 1e-3  # mass fraction
+        assert abs(Xo[-1, i] - ref[-1]) < 0.5          # K
+        assert np.max(np.abs(Xo[:-1, i] - ref[:-1])) < 1e-4  # mass fraction
