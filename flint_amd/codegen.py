@@ -368,6 +368,53 @@ def emit(prog):
     a("  for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = F[k];")
     a("  if (status) status[i] = st;")
     a("}")
+    # host copy of the rate program for the reaction-parallel reactor kernel
+    kinds = {"arrhenius": 0, "three-body": 1, "troe": 2, "lindemann": 3}
+    rx_kind, rx_col, rx_mset, rx_fac = [], [], [], []
+    for r in prog.reactions:
+        rx_kind.append(kinds[r.kind] | (4 if r.reversible else 0))
+        if r.kind in ("arrhenius", "three-body"):
+            rx_col.append(2 * r.table)
+        elif r.kind == "troe":
+            rx_col.append(2 * na + 4 * r.table)
+        else:
+            rx_col.append(2 * na + 4 * nt + 3 * r.table)
+        rx_mset.append(r.m_set)
+        for side in (r.fwd, r.rev):
+            assert len(side) <= 4
+            codes = [k | (n << 8) for k, n in side] + [-1] * (4 - len(side))
+            rx_fac.extend(codes)
+    wd_ptr, wd_rx, wd_coef = [0], [], []
+    for terms in prog.wdot_terms:
+        for j, c in terms:
+            wd_rx.append(j)
+            wd_coef.append(c)
+        wd_ptr.append(len(wd_rx))
+    ms_ptr, ms_sp, ms_eff = [0], [], []
+    for m in prog.m_sets:
+        items = [(k, 1.0) for k in range(ns)] if m == "sum" else [(k, 1.0 if e is None else e) for k, e in m]
+        for k, e in items:
+            ms_sp.append(k)
+            ms_eff.append(e)
+        ms_ptr.append(len(ms_sp))
+
+    def arr(ty, name, vals):
+        vals = vals or [0]
+        body = ", ".join(_lit(v) if ty == "double" else str(int(v)) for v in vals)
+        a(f"static const {ty} {name}[] = {{{body}}};")
+    arr("int", "p_rx_kind", rx_kind)
+    arr("int", "p_rx_col", rx_col)
+    arr("int", "p_rx_mset", rx_mset)
+    arr("int", "p_rx_fac", rx_fac)
+    arr("int", "p_wd_ptr", wd_ptr)
+    arr("int", "p_wd_rx", wd_rx)
+    arr("double", "p_wd_coef", wd_coef)
+    arr("int", "p_ms_ptr", ms_ptr)
+    arr("int", "p_ms_sp", ms_sp)
+    arr("double", "p_ms_eff", ms_eff)
+    a(f"const flint_host_prog host_prog = {{{ns}, {len(prog.reactions)}, {len(prog.m_sets)}, {len(wd_rx)}, "
+      f"{len(ms_sp)}, p_rx_kind, p_rx_col, p_rx_mset, p_rx_fac, p_wd_ptr, p_wd_rx, p_wd_coef, p_ms_ptr, p_ms_sp, "
+      f"p_ms_eff}};")
     a(f"}}  // namespace flint_gen_{mid}")
     a("")
     a(f"FLINT_REGISTER_MECH({mid}, flint_gen_{mid})")

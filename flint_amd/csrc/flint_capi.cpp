@@ -144,10 +144,13 @@ int flint_mech_upload(flint_mech* m) {
   HIPCHK(hipGetDevice(&m->device));
   HIPCHK(hipMalloc(&m->d_tabk, m->h_tabk.size() * sizeof(double)));
   HIPCHK(hipMalloc(&m->d_tabh, m->h_tabh.size() * sizeof(double)));
-  HIPCHK(hipMalloc(&m->d_W, m->W.size() * sizeof(double)));
+  // padded to 64 entries so kernels may read W[j] for any j of a 32/64-wide unrolled loop
+  std::vector<double> Wp(std::max<size_t>(m->W.size(), 64), 1.0);
+  std::copy(m->W.begin(), m->W.end(), Wp.begin());
+  HIPCHK(hipMalloc(&m->d_W, Wp.size() * sizeof(double)));
   HIPCHK(hipMemcpy(m->d_tabk, m->h_tabk.data(), m->h_tabk.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(m->d_tabh, m->h_tabh.data(), m->h_tabh.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(m->d_W, m->W.data(), m->W.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m->d_W, Wp.data(), Wp.size() * sizeof(double), hipMemcpyHostToDevice));
   return FLINT_OK;
 }
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include "flint_registry.h"
+#include "flint_reactor.h"
 
 // 16-byte load of two adjacent table doubles (tables are padded so every pair is aligned).
 __device__ __forceinline__ double2 ld2(const double* __restrict__ p) {
@@ -53,6 +54,6 @@ struct NullSink {
                          st, tabk, Tk_min, Tk_max, tabh, Th_min, Th_max);                               \
     return hipGetLastError();                                                                           \
   }                                                                                                     \
-  const flint_mech_ops ID##_ops = {#ID, NSPACE::NS, NSPACE::WK, ID##_wdot, ID##_rhs_jac};               \
+  const flint_mech_ops ID##_ops = {#ID, NSPACE::NS, NSPACE::WK, ID##_wdot, ID##_rhs_jac, &NSPACE::host_prog}; \
   const int ID##_reg = flint_register_mech(&ID##_ops);                                                  \
   }

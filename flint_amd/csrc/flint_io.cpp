@@ -211,5 +211,3 @@ extern "C" int flint_mech_load(const char* folder, const char* prefix, flint_mec
                            nt ? troe.data() : nullptr, nl, nl ? lind.data() : nullptr, Th_min, Th_max, cp.data(),
                            h.data(), ne, ne ? comp.data() : nullptr, out);
 }
-
-void flint_mech_release_extra(flint_mech*) {}

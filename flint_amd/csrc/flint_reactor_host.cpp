@@ -1,0 +1,252 @@
+// Host side of the batched reactor: device copy of the rate program, launch-shape autotuner,
+// hipGraph capture/replay, and the C-ABI entry points of include/flint_amd.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/flint_amd.h"
+#include "flint_internal.h"
+#include "flint_reactor.h"
+
+hipError_t flint_reactor_set_max_lds(int bytes);
+
+struct flint_extra {
+  flint_reactor::Prog prog{};
+  std::vector<void*> bufs;  // device allocations backing prog
+  int* d_counter = nullptr;
+  int teams_per_block = 2;
+  int blocks = 0;
+  bool tuned = false;
+  // captured graph of the last flint_reactor_advance_graph call
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  std::vector<char> gkey;
+};
+
+#define HIPCHK(expr)                                                                                      \
+  do {                                                                                                    \
+    hipError_t e_ = (expr);                                                                               \
+    if (e_ != hipSuccess) return flint_fail(FLINT_EHIP, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+void flint_mech_release_extra(flint_mech* m) {
+  flint_extra* x = m->extra;
+  if (!x) return;
+  if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
+  if (x->graph) (void)hipGraphDestroy(x->graph);
+  for (void* p : x->bufs) (void)hipFree(p);
+  if (x->d_counter) (void)hipFree(x->d_counter);
+  delete x;
+  m->extra = nullptr;
+}
+
+template <class T>
+static int upload(flint_extra* x, const T* src, size_t count, const T** dst) {
+  void* p = nullptr;
+  HIPCHK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T)));
+  x->bufs.push_back(p);
+  if (count) HIPCHK(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
+  *dst = static_cast<const T*>(p);
+  return FLINT_OK;
+}
+
+static int ensure_extra(flint_mech* m) {
+  if (m->extra) return FLINT_OK;
+  if (m->is_wd || !m->ops || !m->ops->prog)
+    return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel needs a generated mechanism (not WD)");
+  if (m->ns + 1 > 32) return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports ns + 1 <= 32");
+  const flint_host_prog* hp = m->ops->prog;
+  auto* x = new flint_extra();
+  m->extra = x;
+  flint_reactor::Prog& P = x->prog;
+  P.ns = hp->ns;
+  P.nr = hp->nr;
+  P.nms = hp->nms;
+  std::vector<double> effd((size_t)std::max(hp->nms, 1) * 32, 0.0);
+  for (int s = 0; s < hp->nms; ++s)
+    for (int t = hp->ms_ptr[s]; t < hp->ms_ptr[s + 1]; ++t) effd[(size_t)s * 32 + hp->ms_sp[t]] = hp->ms_eff[t];
+  int rc = 0;
+  rc |= upload(x, hp->rx_kind, hp->nr, &P.rx_kind);
+  rc |= upload(x, hp->rx_col, hp->nr, &P.rx_col);
+  rc |= upload(x, hp->rx_mset, hp->nr, &P.rx_mset);
+  rc |= upload(x, hp->rx_fac, (size_t)hp->nr * 8, &P.rx_fac);
+  rc |= upload(x, hp->wd_ptr, (size_t)hp->ns + 1, &P.wd_ptr);
+  rc |= upload(x, hp->wd_rx, hp->nterm, &P.wd_rx);
+  rc |= upload(x, hp->wd_coef, hp->nterm, &P.wd_coef);
+  rc |= upload(x, hp->ms_ptr, (size_t)hp->nms + 1, &P.ms_ptr);
+  rc |= upload(x, hp->ms_sp, hp->nmterm, &P.ms_sp);
+  rc |= upload(x, hp->ms_eff, hp->nmterm, &P.ms_eff);
+  rc |= upload(x, effd.data(), effd.size(), &P.effd);
+  if (rc) return rc;
+  P.W = m->d_W;
+  P.tabk = m->d_tabk;
+  P.Tk_min = m->Tk_min;
+  P.Tk_max = m->Tk_max;
+  P.wk = m->wk;
+  P.tabh = m->d_tabh;
+  P.Th_min = m->Th_min;
+  P.Th_max = m->Th_max;
+  HIPCHK(hipMalloc(&x->d_counter, 64 * sizeof(int)));
+  // default launch: as many two-team blocks as fit in LDS on every CU
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, m->device));
+  const size_t tb = (flint_reactor_team_bytes(P.nr, P.nms) + 15) & ~size_t(15);
+  const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
+  const int per_cu = std::max<int>(1, (int)(lds_cu / (tb * x->teams_per_block)));
+  x->blocks = prop.multiProcessorCount * std::min(per_cu, 8);
+  HIPCHK(flint_reactor_set_max_lds((int)std::min<size_t>(lds_cu, 160 * 1024)));
+  return FLINT_OK;
+}
+
+static int check_args(const flint_mech* m, long n, long ld, int mode, const double* X, const double* p, double t_end,
+                      double rtol, double atol) {
+  if (!m) return flint_fail(FLINT_EARG, "null mechanism");
+  if (n < 0 || ld < n) return flint_fail(FLINT_EARG, "need 0 <= n <= ld");
+  if (mode != 0 && mode != 1) return flint_fail(FLINT_EARG, "mode must be 0 (constant volume) or 1 (constant pressure)");
+  if (n > 0 && !X) return flint_fail(FLINT_EARG, "null state");
+  if (mode == 1 && n > 0 && !p) return flint_fail(FLINT_EARG, "constant pressure needs p");
+  if (!(t_end > 0.0) || !(rtol > 0.0) || !(atol > 0.0)) return flint_fail(FLINT_EARG, "need t_end, rtol, atol > 0");
+  return FLINT_OK;
+}
+
+static int launch(flint_mech* m, long n, long ld, int mode, double* X, const double* p, double t_end, double rtol,
+                  double atol, int max_steps, int* stats, int* status, hipStream_t s, int tpb, int blocks) {
+  flint_reactor::Opts O{mode, rtol, atol, t_end, max_steps > 0 ? max_steps : 100000};
+  hipError_t e = flint_reactor_launch(m->extra->prog, O, n, ld, X, p, stats, status, m->extra->d_counter, tpb, blocks, s);
+  if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("reactor launch: ") + hipGetErrorString(e));
+  return FLINT_OK;
+}
+
+extern "C" int flint_reactor_advance(flint_mech* m, long n, long ld, int mode, double* X, const double* p, double t_end,
+                                     double rtol, double atol, int max_steps, int* stats, int* status, void* stream) {
+  if (int rc = check_args(m, n, ld, mode, X, p, t_end, rtol, atol)) return rc;
+  if (n == 0) return FLINT_OK;
+  if (int rc = ensure_extra(m)) return rc;
+  return launch(m, n, ld, mode, X, p, t_end, rtol, atol, max_steps, stats, status, (hipStream_t)stream,
+                m->extra->teams_per_block, m->extra->blocks);
+}
+
+extern "C" int flint_reactor_set_launch(flint_mech* m, int teams_per_block, int blocks) {
+  if (!m) return flint_fail(FLINT_EARG, "null mechanism");
+  if (int rc = ensure_extra(m)) return rc;
+  if (teams_per_block < 1 || teams_per_block > 8 || blocks < 1) return flint_fail(FLINT_EARG, "bad launch shape");
+  m->extra->teams_per_block = teams_per_block;
+  m->extra->blocks = blocks;
+  m->extra->tuned = true;
+  return FLINT_OK;
+}
+
+extern "C" int flint_reactor_get_launch(flint_mech* m, int* teams_per_block, int* blocks) {
+  if (!m) return flint_fail(FLINT_EARG, "null mechanism");
+  if (int rc = ensure_extra(m)) return rc;
+  if (teams_per_block) *teams_per_block = m->extra->teams_per_block;
+  if (blocks) *blocks = m->extra->blocks;
+  return FLINT_OK;
+}
+
+// Autotuner: times every candidate launch shape on a copy of the sample (the caller's data is not
+// modified) and keeps the fastest for this handle.
+extern "C" int flint_reactor_autotune(flint_mech* m, long n, long ld, int mode, const double* X, const double* p,
+                                      double t_end, double rtol, double atol, double* best_ms) {
+  if (int rc = check_args(m, n, ld, mode, X, p, t_end, rtol, atol)) return rc;
+  if (n == 0) return FLINT_OK;
+  if (int rc = ensure_extra(m)) return rc;
+  flint_extra* x = m->extra;
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, m->device));
+  const int cus = prop.multiProcessorCount;
+  const size_t tb = (flint_reactor_team_bytes(x->prog.nr, x->prog.nms) + 15) & ~size_t(15);
+  const size_t lds_cu = std::min<size_t>(prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536,
+                                         160 * 1024);
+  const int nz = m->ns + 1;
+  double* work = nullptr;
+  HIPCHK(hipMalloc(&work, (size_t)nz * n * sizeof(double)));
+  hipStream_t s;
+  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  float best = 1e30f;
+  int best_tpb = x->teams_per_block, best_blocks = x->blocks;
+  for (int tpb : {1, 2, 4, 8}) {
+    const int fit = (int)(lds_cu / (tb * tpb));
+    if (fit < 1) continue;
+    for (int per_cu : {1, 2, 3, 4, 6, 8}) {
+      if (per_cu > fit || per_cu * tpb > 16) continue;
+      const int blocks = cus * per_cu;
+      int rc = 0;
+      for (int rep = 0; rep < 2 && !rc; ++rep) {
+        HIPCHK(hipMemcpy2DAsync(work, n * sizeof(double), X, ld * sizeof(double), n * sizeof(double), nz,
+                                hipMemcpyDeviceToDevice, s));
+        HIPCHK(hipEventRecord(e0, s));
+        rc = launch(m, n, n, mode, work, p, t_end, rtol, atol, 0, nullptr, nullptr, s, tpb, blocks);
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipStreamSynchronize(s));
+      }
+      if (rc) continue;
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best) {
+        best = ms;
+        best_tpb = tpb;
+        best_blocks = blocks;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  (void)hipFree(work);
+  x->teams_per_block = best_tpb;
+  x->blocks = best_blocks;
+  x->tuned = true;
+  if (best_ms) *best_ms = best;
+  return FLINT_OK;
+}
+
+// Same as flint_reactor_advance, through a hipGraph captured on the first call with these
+// arguments and replayed on later identical calls (a CFD time loop on fixed buffers).
+extern "C" int flint_reactor_advance_graph(flint_mech* m, long n, long ld, int mode, double* X, const double* p,
+                                           double t_end, double rtol, double atol, int max_steps, int* stats, int* status,
+                                           void* stream) {
+  if (int rc = check_args(m, n, ld, mode, X, p, t_end, rtol, atol)) return rc;
+  if (n == 0) return FLINT_OK;
+  if (int rc = ensure_extra(m)) return rc;
+  flint_extra* x = m->extra;
+  struct Key {
+    long n, ld;
+    int mode;
+    double* X;
+    const double* p;
+    double t_end, rtol, atol;
+    int max_steps;
+    int* stats;
+    int* status;
+    int tpb, blocks;
+  } key{n, ld, mode, X, p, t_end, rtol, atol, max_steps, stats, status, x->teams_per_block, x->blocks};
+  std::vector<char> kb(reinterpret_cast<char*>(&key), reinterpret_cast<char*>(&key) + sizeof key);
+  hipStream_t s = (hipStream_t)stream;
+  if (!x->gexec || kb != x->gkey) {
+    if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
+    if (x->graph) (void)hipGraphDestroy(x->graph);
+    x->gexec = nullptr;
+    x->graph = nullptr;
+    hipStream_t cs;
+    HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+    int rc = launch(m, n, ld, mode, X, p, t_end, rtol, atol, max_steps, stats, status, cs, x->teams_per_block, x->blocks);
+    hipGraph_t g = nullptr;
+    hipError_t ee = hipStreamEndCapture(cs, &g);
+    (void)hipStreamDestroy(cs);
+    if (rc) return rc;
+    if (ee != hipSuccess) return flint_fail(FLINT_EHIP, std::string("capture: ") + hipGetErrorString(ee));
+    x->graph = g;
+    HIPCHK(hipGraphInstantiate(&x->gexec, g, nullptr, nullptr, 0));
+    x->gkey = kb;
+  }
+  HIPCHK(hipGraphLaunch(x->gexec, s));
+  return FLINT_OK;
+}

@@ -4,6 +4,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+struct flint_host_prog;
+
 struct flint_mech_ops {
   const char* id;
   int ns;
@@ -12,6 +14,7 @@ struct flint_mech_ops {
                      const double* tabk, int Tk_min, int Tk_max, hipStream_t s);
   hipError_t (*rhs_jac)(long n, long ld, const double* Z, double* F, double* J, int* status, const double* tabk,
                         int Tk_min, int Tk_max, const double* tabh, int Th_min, int Th_max, hipStream_t s);
+  const flint_host_prog* prog;  // rate program for the reaction-parallel reactor kernel
 };
 
 int flint_register_mech(const flint_mech_ops* ops);

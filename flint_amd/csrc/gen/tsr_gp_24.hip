@@ -8729,6 +8729,17 @@ __global__ __launch_bounds__(JAC_BLOCK) void rhs_jac_kernel(long n, long ld, con
   for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = F[k];
   if (status) status[i] = st;
 }
+static const int p_rx_kind[] = {5, 5, 4, 4, 4, 4, 4, 4, 4, 4, 5, 4, 4, 4, 4, 4, 4, 4, 4, 5, 5, 5, 4, 5, 5, 5, 5, 5, 4, 4, 4, 4, 4, 6, 6, 4, 6, 4, 6, 4, 4, 4, 4, 6, 6, 4, 6, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 6, 4, 4, 5, 5, 4, 4, 4, 4, 4, 4, 4, 4, 4, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1};
+static const int p_rx_col[] = {0, 2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38, 40, 42, 44, 46, 48, 50, 52, 54, 56, 58, 60, 62, 64, 204, 208, 66, 212, 68, 216, 70, 72, 74, 76, 220, 224, 78, 228, 80, 82, 84, 86, 88, 90, 92, 94, 96, 98, 100, 102, 104, 106, 108, 110, 112, 114, 116, 118, 120, 122, 124, 126, 128, 130, 132, 134, 136, 138, 140, 142, 144, 146, 148, 150, 152, 232, 154, 156, 158, 160, 162, 164, 166, 168, 170, 172, 174, 176, 178, 180, 182, 184, 186, 188, 190, 192, 194, 196, 198, 200, 202};
+static const int p_rx_mset[] = {0, 1, -1, -1, -1, -1, -1, -1, -1, -1, 2, -1, -1, -1, -1, -1, -1, -1, -1, 3, 4, 5, -1, 6, 7, 5, 8, 9, -1, -1, -1, -1, -1, 1, 1, -1, 1, -1, 1, -1, -1, -1, -1, 1, 1, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 1, -1, -1, 5, 10, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 11, 11, 11, 11};
+static const int p_rx_fac[] = {521, -1, -1, -1, 274, -1, -1, -1, 264, 265, -1, -1, 259, -1, -1, -1, 260, 265, -1, -1, 264, 259, -1, -1, 256, 265, -1, -1, 274, 259, -1, -1, 257, 265, -1, -1, 256, 259, -1, -1, 275, 265, -1, -1, 264, 261, -1, -1, 263, 265, -1, -1, 258, 260, -1, -1, 263, 265, -1, -1, 264, 261, -1, -1, 267, 265, -1, -1, 269, 264, -1, -1, 277, 265, -1, -1, 267, 259, -1, -1, 258, 265, -1, -1, 279, -1, -1, -1, 261, 265, -1, -1, 258, 259, -1, -1, 261, 265, -1, -1, 279, 264, -1, -1, 269, 265, -1, -1, 261, 259, -1, -1, 262, 265, -1, -1, 269, 259, -1, -1, 268, 265, -1, -1, 271, 259, -1, -1, 268, 265, -1, -1, 275, 258, -1, -1, 258, 274, -1, -1, 279, 265, -1, -1, 269, 274, -1, -1, 261, 256, -1, -1, 264, 274, -1, -1, 256, -1, -1, -1, 264, 274, -1, -1, 256, -1, -1, -1, 264, 274, -1, -1, 256, -1, -1, -1, 264, 274, -1, -1, 265, 259, -1, -1, 520, -1, -1, -1, 260, -1, -1, -1, 520, -1, -1, -1, 260, -1, -1, -1, 520, -1, -1, -1, 260, -1, -1, -1, 520, -1, -1, -1, 260, -1, -1, -1, 264, 259, -1, -1, 272, -1, -1, -1, 264, 256, -1, -1, 272, 265, -1, -1, 264, 256, -1, -1, 260, 274, -1, -1, 264, 256, -1, -1, 515, -1, -1, -1, 264, 257, -1, -1, 260, 256, -1, -1, 264, 257, -1, -1, 272, 259, -1, -1, 275, 264, -1, -1, 267, -1, -1, -1, 267, 264, -1, -1, 277, -1, -1, -1, 277, 264, -1, -1, 267, 260, -1, -1, 264, 261, -1, -1, 269, -1, -1, -1, 264, 261, -1, -1, 258, 260, -1, -1, 269, 264, -1, -1, 262, -1, -1, -1, 269, 264, -1, -1, 260, 261, -1, -1, 262, 264, -1, -1, 269, 260, -1, -1, 262, 264, -1, -1, 267, 259, -1, -1, 262, 264, -1, -1, 263, 272, -1, -1, 271, 264, -1, -1, 268, -1, -1, -1, 258, 260, -1, -1, 269, -1, -1, -1, 260, 259, -1, -1, 264, 272, -1, -1, 515, -1, -1, -1, 257, -1, -1, -1, 515, -1, -1, -1, 272, 265, -1, -1, 256, 259, -1, -1, 272, 274, -1, -1, 257, 259, -1, -1, 272, 256, -1, -1, 257, 259, -1, -1, 272, 256, -1, -1, 275, 259, -1, -1, 269, 264, -1, -1, 263, 259, -1, -1, 269, 264, -1, -1, 267, 259, -1, -1, 275, 272, -1, -1, 267, 259, -1, -1, 263, 272, -1, -1, 277, 259, -1, -1, 267, 272, -1, -1, 258, 259, -1, -1, 279, 264, -1, -1, 261, 259, -1, -1, 258, 272, -1, -1, 269, 259, -1, -1, 272, 261, -1, -1, 262, 259, -1, -1, 269, 272, -1, -1, 268, 259, -1, -1, 271, 272, -1, -1, 268, 259, -1, -1, 267, 258, -1, -1, 512, -1, -1, -1, 257, 274, -1, -1, 512, -1, -1, -1, 257, 274, -1, -1, 275, 256, -1, -1, 269, 259, -1, -1, 267, 256, -1, -1, 277, 274, -1, -1, 267, 256, -1, -1, 262, 259, -1, -1, 258, 256, -1, -1, 279, 259, -1, -1, 269, 256, -1, -1, 257, 261, -1, -1, 275, 274, -1, -1, 261, 259, -1, -1, 275, 260, -1, -1, 267, 264, -1, -1, 531, -1, -1, -1, 268, 260, -1, -1, 275, 277, -1, -1, 523, -1, -1, -1, 263, 274, -1, -1, 258, 264, 259, -1, 263, 274, -1, -1, 258, 272, -1, -1, 263, 260, -1, -1, 267, 264, -1, -1, 263, 272, -1, -1, 275, 272, -1, -1, 263, 277, -1, -1, 523, -1, -1, -1, 263, 258, -1, -1, 275, 258, -1, -1, 263, 279, -1, -1, 275, 279, -1, -1, 263, 279, -1, -1, 269, 258, -1, -1, 267, 274, -1, -1, 262, 265, -1, -1, 267, 274, -1, -1, 269, 259, -1, -1, 267, 257, -1, -1, 277, 256, -1, -1, 523, -1, -1, -1, 276, -1, -1, -1, 267, 261, -1, -1, 277, 258, -1, -1, 269, 267, -1, -1, 277, 261, -1, -1, 261, -1, -1, -1, 258, 264, -1, -1, 261, -1, -1, -1, 258, 264, -1, -1, 261, 274, -1, -1, 258, 256, -1, -1, 262, 274, -1, -1, 269, 256, -1, -1, 271, 274, -1, -1, 258, 261, -1, -1, 271, 260, -1, -1, 268, 264, -1, -1, 267, 259, -1, -1, 269, 260, -1, -1, 267, 274, -1, -1, 273, -1, -1, -1, 262, 256, -1, -1, 269, 257, -1, -1, 267, 262, -1, -1, 269, 277, -1, -1, 267, 273, -1, -1, 518, -1, -1, -1, 268, 274, -1, -1, 517, -1, -1, -1, 517, -1, -1, -1, 268, 274, -1, -1, 273, 256, -1, -1, 262, 274, 259, -1, 529, -1, -1, -1, 518, 274, -1, -1, 270, 265, -1, -1, 271, 269, -1, -1, 271, 269, -1, -1, 270, 265, -1, -1, 270, 259, -1, -1, 266, 272, -1, -1, 266, 272, -1, -1, 270, 259, -1, -1, 278, -1, -1, -1, 279, 264, -1, -1, 279, 264, -1, -1, 278, -1, -1, -1, 278, -1, -1, -1, 258, 259, -1, -1, 258, 259, -1, -1, 278, -1, -1, -1};
+static const int p_wd_ptr[] = {0, 25, 36, 58, 101, 120, 141, 155, 168, 211, 235, 237, 263, 272, 296, 300, 307, 327, 331, 357, 370, 371, 382, 386, 395};
+static const int p_wd_rx[] = {3, 4, 18, 19, 20, 21, 28, 29, 30, 31, 48, 49, 50, 62, 63, 64, 65, 66, 67, 68, 83, 89, 90, 95, 100, 4, 31, 32, 46, 49, 50, 62, 63, 68, 83, 95, 6, 10, 11, 16, 17, 37, 44, 56, 57, 61, 67, 73, 74, 78, 80, 85, 87, 88, 89, 91, 108, 109, 1, 2, 3, 4, 9, 11, 13, 14, 15, 22, 27, 30, 32, 41, 45, 46, 47, 48, 49, 50, 51, 52, 53, 54, 55, 56, 57, 58, 59, 60, 61, 64, 66, 67, 69, 73, 82, 93, 100, 104, 105, 108, 109, 2, 6, 23, 24, 25, 26, 29, 31, 35, 37, 39, 40, 44, 45, 70, 71, 75, 92, 93, 5, 7, 11, 12, 13, 18, 36, 37, 39, 57, 58, 68, 69, 85, 86, 87, 88, 89, 91, 98, 99, 14, 38, 40, 41, 42, 59, 66, 81, 90, 95, 96, 97, 100, 101, 6, 7, 42, 52, 54, 73, 74, 75, 76, 77, 78, 79, 80, 1, 2, 5, 7, 8, 12, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 45, 51, 52, 56, 70, 73, 75, 87, 88, 92, 106, 107, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 22, 28, 47, 81, 102, 103, 104, 105, 8, 9, 33, 34, 35, 41, 53, 54, 55, 61, 65, 66, 70, 72, 75, 77, 81, 82, 83, 84, 85, 86, 93, 94, 96, 97, 15, 16, 43, 60, 61, 71, 92, 98, 99, 8, 13, 14, 18, 36, 38, 39, 40, 44, 51, 52, 58, 59, 64, 68, 80, 82, 86, 90, 93, 95, 96, 102, 103, 102, 103, 104, 105, 15, 43, 60, 91, 92, 102, 103, 27, 28, 32, 42, 45, 47, 48, 49, 50, 53, 54, 55, 57, 58, 59, 60, 74, 76, 104, 105, 94, 97, 100, 101, 0, 3, 17, 18, 19, 20, 21, 22, 29, 48, 62, 63, 65, 69, 73, 74, 81, 82, 89, 90, 91, 94, 98, 99, 100, 101, 5, 16, 33, 51, 53, 64, 69, 70, 71, 72, 76, 78, 79, 84, 9, 34, 35, 55, 65, 72, 77, 83, 85, 86, 96, 106, 107, 108, 109, 10, 12, 17, 56, 67, 79, 80, 106, 107};
+static const double p_wd_coef[] = {-1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -2.0, -2.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, 1.0, 1.0, 0.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 2.0, 1.0, 1.0, -1.0, -2.0, -2.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, -1.0, 1.0, 2.0, -2.0, -1.0, 1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, -1.0, 2.0, 1.0, 2.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, -1.0, -1.0, -2.0, -2.0, -2.0, -2.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -2.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 2.0, 1.0, 2.0, -1.0, -1.0, -1.0, -2.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 0.0, 1.0, -1.0, 1.0, -1.0, -1.0, -2.0, 1.0, 1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, -1.0, -1.0, -2.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 0.0, -1.0, 1.0, -1.0};
+static const int p_ms_ptr[] = {0, 24, 48, 72, 94, 95, 96, 117, 118, 119, 143, 166, 190};
+static const int p_ms_sp[] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 17, 19, 20, 21, 22, 23, 18, 16, 0, 1, 2, 3, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 17, 18, 19, 20, 21, 22, 4, 23, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 17, 18, 19, 20, 21, 22, 23, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23};
+static const double p_ms_eff[] = {1.0, 1.0, 1.75, 1.0, 2.4000000953674316, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 15.399999618530273, 1.0, 1.0, 1.0, 3.0, 2.0, 1.0, 3.5999999046325684, 1.0, 1.0, 1.5, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 6.0, 1.0, 1.0, 1.0, 3.0, 2.0, 1.0, 2.0, 1.0, 1.0, 1.5, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 6.0, 1.0, 6.0, 1.0, 3.0, 2.0, 1.0, 3.5, 1.0, 1.0, 0.75, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.5, 1.0, 1.0, 1.5, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 3.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 0.7300000190734863, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 3.6500000953674316, 1.0, 1.0, 1.0, 3.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.5, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 3.0, 2.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+const flint_host_prog host_prog = {24, 110, 12, 395, 190, p_rx_kind, p_rx_col, p_rx_mset, p_rx_fac, p_wd_ptr, p_wd_rx, p_wd_coef, p_ms_ptr, p_ms_sp, p_ms_eff};
 }  // namespace flint_gen_tsr_gp_24
 
 FLINT_REGISTER_MECH(tsr_gp_24, flint_gen_tsr_gp_24)

@@ -13709,6 +13709,17 @@ __global__ __launch_bounds__(JAC_BLOCK) void rhs_jac_kernel(long n, long ld, con
   for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = F[k];
   if (status) status[i] = st;
 }
+static const int p_rx_kind[] = {5, 5, 4, 4, 4, 4, 4, 4, 4, 4, 4, 5, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 5, 5, 5, 4, 5, 5, 5, 5, 5, 4, 4, 4, 4, 4, 6, 4, 6, 4, 6, 4, 4, 6, 6, 4, 6, 4, 6, 4, 4, 4, 6, 4, 6, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 6, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 6, 4, 4, 4, 4, 4, 5, 5, 4, 4, 6, 4, 4, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+static const int p_rx_col[] = {0, 2, 4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24, 26, 28, 30, 32, 34, 36, 38, 40, 42, 44, 46, 48, 50, 52, 54, 56, 58, 60, 62, 64, 66, 68, 70, 72, 370, 74, 374, 76, 378, 78, 80, 382, 386, 82, 390, 84, 394, 86, 88, 90, 398, 92, 402, 94, 96, 98, 100, 102, 104, 106, 108, 110, 112, 114, 116, 118, 120, 122, 124, 126, 128, 130, 132, 134, 136, 138, 140, 142, 144, 146, 148, 150, 152, 154, 156, 158, 160, 162, 164, 166, 168, 406, 170, 172, 174, 176, 178, 180, 182, 184, 186, 188, 190, 192, 410, 194, 196, 198, 200, 202, 204, 206, 208, 210, 414, 212, 214, 216, 218, 220, 222, 224, 226, 228, 230, 232, 234, 236, 238, 240, 242, 244, 246, 248, 250, 252, 254, 256, 258, 260, 262, 264, 266, 268, 270, 272, 274, 276, 278, 280, 282, 284, 286, 288, 290, 292, 294, 296, 298, 300, 302, 304, 306, 308, 310, 312, 314, 316, 318, 320, 322, 324, 326, 328, 330, 332, 334, 336, 338, 340, 342, 344, 346, 348, 350, 352, 354, 356, 358, 360, 362, 364, 366, 368};
+static const int p_rx_mset[] = {0, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 2, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 3, 4, 5, -1, 6, 7, 5, 8, 9, -1, -1, -1, -1, -1, 1, -1, 1, -1, 1, -1, -1, 1, 1, -1, 1, -1, 1, -1, -1, -1, 1, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 1, -1, -1, -1, -1, -1, 5, 10, -1, -1, 1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, 11, 11, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+static const int p_rx_fac[] = {519, -1, -1, -1, 271, -1, -1, -1, 258, 263, -1, -1, 283, -1, -1, -1, 275, 263, -1, -1, 258, 283, -1, -1, 280, 263, -1, -1, 271, 283, -1, -1, 264, 263, -1, -1, 280, 283, -1, -1, 282, 263, -1, -1, 265, 258, -1, -1, 285, 263, -1, -1, 258, 273, -1, -1, 272, 263, -1, -1, 265, 275, -1, -1, 272, 263, -1, -1, 258, 273, -1, -1, 274, 263, -1, -1, 266, 258, -1, -1, 259, 263, -1, -1, 274, 283, -1, -1, 265, 263, -1, -1, 286, -1, -1, -1, 273, 263, -1, -1, 265, 283, -1, -1, 273, 263, -1, -1, 286, 258, -1, -1, 266, 263, -1, -1, 273, 283, -1, -1, 260, 263, -1, -1, 285, 265, -1, -1, 262, 263, -1, -1, 268, 258, -1, -1, 281, 263, -1, -1, 274, 273, -1, -1, 284, 263, -1, -1, 266, 274, -1, -1, 267, 263, -1, -1, 284, 283, -1, -1, 268, 263, -1, -1, 285, 286, -1, -1, 265, 271, -1, -1, 286, 263, -1, -1, 266, 271, -1, -1, 273, 280, -1, -1, 258, 271, -1, -1, 280, -1, -1, -1, 258, 271, -1, -1, 280, -1, -1, -1, 258, 271, -1, -1, 280, -1, -1, -1, 258, 271, -1, -1, 263, 283, -1, -1, 514, -1, -1, -1, 275, -1, -1, -1, 514, -1, -1, -1, 275, -1, -1, -1, 514, -1, -1, -1, 275, -1, -1, -1, 514, -1, -1, -1, 275, -1, -1, -1, 258, 283, -1, -1, 270, -1, -1, -1, 258, 280, -1, -1, 270, 263, -1, -1, 258, 280, -1, -1, 275, 271, -1, -1, 258, 280, -1, -1, 539, -1, -1, -1, 258, 264, -1, -1, 275, 280, -1, -1, 258, 264, -1, -1, 270, 283, -1, -1, 285, 258, -1, -1, 274, -1, -1, -1, 272, 258, -1, -1, 282, 275, -1, -1, 274, 258, -1, -1, 259, -1, -1, -1, 259, 258, -1, -1, 274, 275, -1, -1, 258, 273, -1, -1, 266, -1, -1, -1, 258, 273, -1, -1, 265, 275, -1, -1, 266, 258, -1, -1, 275, 273, -1, -1, 260, 258, -1, -1, 262, -1, -1, -1, 262, 258, -1, -1, 281, -1, -1, -1, 262, 258, -1, -1, 260, 275, -1, -1, 281, 258, -1, -1, 284, -1, -1, -1, 281, 258, -1, -1, 262, 275, -1, -1, 284, 258, -1, -1, 267, -1, -1, -1, 284, 258, -1, -1, 281, 275, -1, -1, 267, 258, -1, -1, 284, 275, -1, -1, 268, 258, -1, -1, 274, 265, -1, -1, 265, 275, -1, -1, 266, -1, -1, -1, 275, 283, -1, -1, 258, 270, -1, -1, 539, -1, -1, -1, 264, -1, -1, -1, 539, -1, -1, -1, 270, 263, -1, -1, 280, 283, -1, -1, 270, 271, -1, -1, 264, 283, -1, -1, 270, 280, -1, -1, 264, 283, -1, -1, 270, 280, -1, -1, 282, 283, -1, -1, 258, 273, -1, -1, 285, 283, -1, -1, 266, 258, -1, -1, 285, 283, -1, -1, 282, 270, -1, -1, 272, 283, -1, -1, 266, 258, -1, -1, 274, 283, -1, -1, 285, 270, -1, -1, 274, 283, -1, -1, 272, 270, -1, -1, 259, 283, -1, -1, 274, 270, -1, -1, 265, 283, -1, -1, 286, 258, -1, -1, 273, 283, -1, -1, 265, 270, -1, -1, 266, 283, -1, -1, 270, 273, -1, -1, 260, 283, -1, -1, 268, 258, -1, -1, 260, 283, -1, -1, 274, 265, -1, -1, 262, 283, -1, -1, 260, 270, -1, -1, 281, 283, -1, -1, 262, 270, -1, -1, 267, 283, -1, -1, 284, 270, -1, -1, 536, -1, -1, -1, 264, 271, -1, -1, 536, -1, -1, -1, 264, 271, -1, -1, 285, 280, -1, -1, 266, 283, -1, -1, 274, 280, -1, -1, 259, 271, -1, -1, 265, 280, -1, -1, 286, 283, -1, -1, 266, 280, -1, -1, 264, 273, -1, -1, 282, 271, -1, -1, 273, 263, -1, -1, 282, 275, -1, -1, 285, 258, -1, -1, 282, 270, -1, -1, 266, 258, -1, -1, 282, 285, -1, -1, 260, 258, -1, -1, 282, 274, -1, -1, 262, 258, -1, -1, 282, 259, -1, -1, 281, 258, -1, -1, 282, 286, -1, -1, 265, 273, -1, -1, 282, 266, -1, -1, 268, 258, -1, -1, 285, 271, -1, -1, 273, 283, -1, -1, 285, 275, -1, -1, 274, 258, -1, -1, 541, -1, -1, -1, 260, 275, -1, -1, 285, 274, -1, -1, 281, 258, -1, -1, 285, 259, -1, -1, 530, -1, -1, -1, 285, 265, -1, -1, 268, -1, -1, -1, 272, 271, -1, -1, 265, 258, 283, -1, 272, 271, -1, -1, 265, 270, -1, -1, 272, 275, -1, -1, 274, 258, -1, -1, 272, 270, -1, -1, 285, 270, -1, -1, 272, 274, -1, -1, 281, 258, -1, -1, 272, 259, -1, -1, 530, -1, -1, -1, 272, 265, -1, -1, 285, 265, -1, -1, 272, 286, -1, -1, 285, 286, -1, -1, 272, 286, -1, -1, 266, 265, -1, -1, 267, 272, -1, -1, 284, 274, -1, -1, 274, 271, -1, -1, 266, 283, -1, -1, 274, 264, -1, -1, 259, 280, -1, -1, 530, -1, -1, -1, 267, -1, -1, -1, 530, -1, -1, -1, 284, 258, -1, -1, 274, 273, -1, -1, 259, 265, -1, -1, 266, 274, -1, -1, 259, 273, -1, -1, 281, 274, -1, -1, 262, 259, -1, -1, 267, 274, -1, -1, 284, 259, -1, -1, 273, -1, -1, -1, 265, 258, -1, -1, 273, -1, -1, -1, 265, 258, -1, -1, 273, 271, -1, -1, 265, 280, -1, -1, 262, 271, -1, -1, 266, 273, -1, -1, 281, -1, -1, -1, 260, 275, -1, -1, 284, 271, -1, -1, 281, 280, -1, -1, 274, 283, -1, -1, 266, 275, -1, -1, 262, 284, -1, -1, 537, -1, -1, -1, 537, -1, -1, -1, 262, 284, -1, -1, 281, 263, -1, -1, 262, 283, -1, -1, 262, 283, -1, -1, 281, 263, -1, -1, 260, 271, -1, -1, 529, -1, -1, -1, 529, -1, -1, -1, 260, 271, -1, -1, 282, 271, -1, -1, 265, 283, -1, -1, 265, 283, -1, -1, 282, 271, -1, -1, 262, 271, -1, -1, 260, 280, -1, -1, 260, 280, -1, -1, 262, 271, -1, -1, 268, 263, -1, -1, 529, -1, -1, -1, 529, -1, -1, -1, 268, 263, -1, -1, 268, 283, -1, -1, 266, 273, -1, -1, 266, 273, -1, -1, 268, 283, -1, -1, 267, 271, -1, -1, 284, 280, -1, -1, 284, 280, -1, -1, 267, 271, -1, -1, 267, 280, -1, -1, 284, 264, -1, -1, 284, 264, -1, -1, 267, 280, -1, -1, 281, 267, -1, -1, 540, -1, -1, -1, 540, -1, -1, -1, 281, 267, -1, -1, 262, 274, -1, -1, 260, 259, -1, -1, 260, 259, -1, -1, 262, 274, -1, -1, 284, 274, -1, -1, 281, 259, -1, -1, 281, 259, -1, -1, 284, 274, -1, -1, 276, -1, -1, -1, 274, 265, -1, -1, 274, 265, -1, -1, 276, -1, -1, -1, 281, 271, -1, -1, 262, 280, -1, -1, 262, 280, -1, -1, 281, 271, -1, -1, 281, 280, -1, -1, 262, 264, -1, -1, 262, 264, -1, -1, 281, 280, -1, -1, 269, -1, -1, -1, 281, 274, -1, -1, 281, 274, -1, -1, 269, -1, -1, -1, 276, 263, -1, -1, 268, 283, -1, -1, 268, 283, -1, -1, 276, 263, -1, -1, 274, 276, -1, -1, 268, 259, -1, -1, 268, 259, -1, -1, 274, 276, -1, -1, 281, 263, -1, -1, 257, 258, -1, -1, 257, 258, -1, -1, 281, 263, -1, -1, 267, 282, -1, -1, 284, 285, -1, -1, 284, 285, -1, -1, 267, 282, -1, -1, 267, 285, -1, -1, 284, 274, -1, -1, 284, 274, -1, -1, 267, 285, -1, -1, 281, 283, -1, -1, 266, 274, -1, -1, 266, 274, -1, -1, 281, 283, -1, -1, 278, -1, -1, -1, 274, 273, -1, -1, 274, 273, -1, -1, 278, -1, -1, -1, 278, 283, -1, -1, 279, 270, -1, -1, 279, 270, -1, -1, 278, 283, -1, -1, 278, 258, -1, -1, 279, 275, -1, -1, 279, 275, -1, -1, 278, 258, -1, -1, 278, 280, -1, -1, 279, 264, -1, -1, 279, 264, -1, -1, 278, 280, -1, -1, 278, 274, -1, -1, 279, 259, -1, -1, 279, 259, -1, -1, 278, 274, -1, -1, 277, -1, -1, -1, 281, 266, -1, -1, 281, 266, -1, -1, 277, -1, -1, -1, 277, 283, -1, -1, 262, 266, 270, -1, 277, 263, -1, -1, 262, 266, 283, -1, 277, 258, -1, -1, 262, 266, 275, -1, 277, 280, -1, -1, 262, 266, 264, -1, 277, 274, -1, -1, 262, 266, 259, -1, 261, 271, -1, -1, 268, 273, -1, -1, 268, 273, -1, -1, 261, 271, -1, -1, 279, -1, -1, -1, 276, -1, -1, -1, 276, -1, -1, -1, 279, -1, -1, -1, 279, -1, -1, -1, 257, -1, -1, -1, 257, -1, -1, -1, 279, -1, -1, -1, 257, -1, -1, -1, 268, 258, -1, -1, 268, 258, -1, -1, 257, -1, -1, -1, 257, 271, -1, -1, 268, 280, -1, -1, 268, 280, -1, -1, 257, 271, -1, -1, 268, 283, -1, -1, 274, 286, -1, -1, 274, 286, -1, -1, 268, 283, -1, -1, 256, -1, -1, -1, 265, 269, -1, -1, 265, 269, -1, -1, 256, -1, -1, -1, 281, 280, -1, -1, 278, 283, -1, -1, 278, 283, -1, -1, 281, 280, -1, -1};
+static const int p_wd_ptr[] = {0, 2, 10, 75, 97, 112, 114, 141, 176, 193, 222, 252, 269, 291, 295, 318, 353, 368, 401, 452, 479, 487, 494, 506, 518, 557, 591, 607, 664, 689, 712, 723};
+static const int p_wd_rx[] = {193, 194, 156, 157, 185, 186, 187, 188, 189, 190, 1, 2, 5, 6, 8, 9, 13, 16, 23, 24, 25, 26, 27, 28, 29, 30, 31, 32, 33, 34, 35, 36, 37, 38, 39, 40, 41, 42, 43, 44, 45, 46, 47, 48, 49, 50, 51, 52, 54, 60, 61, 63, 67, 70, 82, 83, 84, 85, 86, 88, 90, 92, 95, 97, 99, 108, 113, 114, 156, 157, 168, 169, 178, 187, 188, 10, 39, 40, 66, 78, 86, 93, 100, 106, 109, 110, 111, 112, 140, 141, 142, 143, 154, 155, 172, 173, 180, 15, 44, 46, 70, 71, 72, 84, 91, 117, 124, 125, 128, 129, 140, 141, 181, 182, 16, 44, 45, 46, 48, 72, 73, 85, 111, 116, 120, 121, 122, 123, 128, 129, 140, 141, 146, 147, 148, 149, 176, 177, 178, 179, 180, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 26, 32, 56, 81, 122, 123, 130, 131, 152, 153, 156, 157, 177, 4, 35, 36, 55, 58, 59, 75, 76, 80, 106, 136, 137, 148, 149, 170, 171, 179, 5, 7, 11, 12, 15, 21, 42, 52, 53, 67, 68, 71, 79, 87, 94, 95, 96, 101, 103, 109, 113, 114, 115, 126, 127, 144, 145, 193, 194, 9, 14, 18, 22, 41, 43, 53, 61, 63, 69, 77, 80, 83, 88, 103, 105, 110, 116, 119, 132, 133, 162, 163, 174, 175, 176, 177, 178, 179, 180, 19, 49, 51, 74, 104, 107, 112, 134, 135, 136, 137, 138, 139, 158, 159, 160, 161, 16, 20, 52, 70, 88, 94, 130, 131, 132, 133, 152, 153, 154, 155, 181, 182, 187, 188, 189, 190, 191, 192, 150, 151, 193, 194, 31, 32, 36, 54, 56, 57, 58, 59, 62, 64, 65, 66, 68, 69, 72, 73, 74, 83, 96, 98, 166, 167, 176, 0, 3, 21, 22, 23, 24, 25, 26, 33, 57, 75, 76, 78, 81, 89, 95, 96, 105, 115, 116, 118, 124, 125, 126, 127, 128, 129, 134, 135, 146, 147, 181, 182, 189, 190, 7, 8, 38, 63, 65, 95, 96, 97, 98, 99, 100, 101, 102, 103, 104, 6, 8, 12, 13, 14, 17, 22, 41, 42, 43, 60, 68, 69, 80, 81, 87, 89, 109, 110, 113, 114, 115, 116, 124, 125, 130, 131, 132, 133, 164, 165, 181, 182, 9, 10, 17, 18, 37, 39, 40, 52, 64, 65, 66, 71, 78, 85, 90, 92, 93, 97, 99, 100, 104, 105, 106, 107, 108, 109, 110, 111, 112, 119, 140, 141, 142, 143, 144, 145, 150, 151, 154, 155, 160, 161, 162, 163, 164, 165, 172, 173, 180, 191, 192, 2, 7, 27, 28, 29, 30, 33, 35, 38, 40, 42, 43, 46, 48, 50, 51, 53, 54, 82, 90, 91, 97, 117, 119, 168, 169, 178, 144, 145, 152, 153, 154, 155, 183, 184, 174, 175, 176, 177, 178, 179, 180, 164, 165, 166, 167, 168, 169, 170, 171, 172, 173, 195, 196, 166, 167, 168, 169, 170, 171, 172, 173, 183, 184, 185, 186, 3, 4, 22, 23, 24, 25, 32, 33, 34, 35, 57, 58, 59, 75, 76, 77, 78, 79, 80, 106, 115, 118, 128, 129, 134, 135, 136, 137, 146, 147, 148, 149, 170, 171, 179, 189, 190, 195, 196, 17, 45, 47, 48, 50, 73, 86, 92, 99, 111, 117, 118, 120, 121, 122, 123, 138, 139, 142, 143, 146, 147, 148, 149, 150, 151, 156, 157, 162, 163, 174, 175, 195, 196, 5, 38, 60, 62, 81, 82, 83, 84, 85, 86, 87, 88, 126, 127, 158, 159, 1, 2, 3, 4, 10, 12, 14, 19, 26, 31, 34, 36, 54, 55, 56, 57, 58, 59, 60, 61, 62, 63, 64, 65, 66, 67, 68, 69, 70, 71, 72, 73, 74, 77, 79, 89, 95, 105, 119, 122, 123, 126, 127, 132, 133, 152, 153, 162, 163, 166, 167, 176, 177, 191, 192, 195, 196, 18, 19, 47, 49, 50, 51, 74, 104, 108, 112, 118, 120, 121, 134, 135, 136, 137, 138, 139, 142, 143, 158, 159, 160, 161, 6, 15, 20, 37, 61, 62, 64, 77, 82, 84, 89, 90, 91, 92, 93, 94, 98, 101, 102, 158, 159, 160, 161, 11, 13, 20, 21, 67, 79, 87, 102, 103, 191, 192};
+static const double p_wd_coef[] = {-1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, -1.0, -1.0, -2.0, -2.0, -2.0, -2.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -2.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, -1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 0.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, 0.0, 1.0, -1.0, 1.0, 1.0, 1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, -1.0, -1.0, 1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, -1.0, 1.0, 2.0, -2.0, 2.0, -2.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 2.0, 1.0, -1.0, 2.0, 1.0, -1.0, -1.0, -2.0, -2.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -2.0, -2.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 2.0, -2.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 2.0, 1.0, -1.0, -2.0, -2.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, 2.0, -2.0, -1.0, 1.0, 1.0, -1.0, 1.0, -1.0, -1.0, 1.0, 1.0, -1.0, -1.0, -1.0, 1.0, -1.0, 1.0, -1.0, -1.0, -1.0, -2.0, -1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, -1.0, -1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, -1.0, 0.0, -1.0, 1.0, -1.0};
+static const int p_ms_ptr[] = {0, 31, 62, 93, 122, 123, 124, 152, 153, 154, 185, 215, 246};
+static const int p_ms_sp[] = {0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 15, 14, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 15, 16, 17, 18, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 19, 30, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30, 0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 15, 16, 17, 18, 19, 20, 21, 22, 23, 24, 25, 26, 27, 28, 29, 30};
+static const double p_ms_eff[] = {1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.75, 1.0, 3.0, 1.0, 1.0, 15.399999618530273, 1.0, 1.0, 1.0, 1.0, 2.4000000953674316, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 3.5999999046325684, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.5, 1.0, 3.0, 1.0, 1.0, 6.0, 1.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.5, 1.0, 3.0, 1.0, 1.0, 6.0, 6.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 3.5, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 0.75, 1.0, 1.5, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.5, 1.0, 1.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 3.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 3.0, 1.0, 1.0, 3.6500000953674316, 1.0, 1.0, 1.0, 1.0, 0.7300000190734863, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.5, 1.0, 3.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0, 1.0};
+const flint_host_prog host_prog = {31, 197, 12, 723, 246, p_rx_kind, p_rx_col, p_rx_mset, p_rx_fac, p_wd_ptr, p_wd_rx, p_wd_coef, p_ms_ptr, p_ms_sp, p_ms_eff};
 }  // namespace flint_gen_tsr_rich_31
 
 FLINT_REGISTER_MECH(tsr_rich_31, flint_gen_tsr_rich_31)

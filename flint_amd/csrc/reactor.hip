@@ -1,0 +1,702 @@
+// Batched stiff 0D reactor integration on gfx950: one persistent launch, one 32-lane team per
+// cell, work-stealing refill from an atomic cell counter.
+//
+// Method: the variable-order (1..5) quasi-constant step BDF of Shampine & Reichelt ("The MATLAB
+// ODE Suite", SIAM J. Sci. Comput. 18, 1997; kappa = 0), modified Newton with the analytic
+// Jacobian refreshed only on Newton failure, RMS error norm of e/(atol + rtol|y|).  It is the same
+// algorithm, decision for decision, as the CPU checker in oracle/oracle_bdf.c.  FLINT itself
+// integrates through the external OSlo library (test-batchF.f90:81,105), absent from the
+// reference checkout.
+//
+// Layout of one team (lanes l = 0..31 of a half wavefront):
+//   lane k < ns owns species k, lane ns owns T: the state component, the 8-row BDF difference
+//   column D[.][k], row k of the Jacobian and row k of the LU factors of I - cJ live in VGPRs;
+//   partial pivoting picks rows by a team argmax and broadcasts the pivot row with ds_bpermute.
+//   The right-hand side is evaluated reaction-parallel (reaction r on lane r mod 32) from the
+//   kinetics table row of the cell's temperature (coalesced over the team), with the
+//   net production rate of species k summed by lane k in reaction order (the summation order of
+//   the reference routine), from per-reaction rates staged in LDS.
+//   Modes: 0 = constant volume, state (rho_k, T), rhs_native (Lib_Chemistry_rhs.f90:15-52);
+//          1 = constant pressure, state (Y_k, T) at fixed p.
+#include "flint_device.h"
+#include "flint_reactor.h"
+
+namespace flint_reactor {
+
+constexpr int TEAM = 32;
+constexpr int MAXO = 5;
+constexpr int NEWTON_MAXITER = 4;
+constexpr double MIN_FACTOR = 0.2, MAX_FACTOR = 10.0;
+
+enum { ST_STEPS, ST_REJECT, ST_NFAIL, ST_RHS, ST_JAC, ST_LU, ST_NEWTON, ST_ORDER };
+
+__device__ __forceinline__ double tsum(double v) {
+#pragma unroll
+  for (int m = 16; m >= 1; m >>= 1) v += __shfl_xor(v, m, TEAM);
+  return v;
+}
+__device__ __forceinline__ double tbc(double v, int src) { return __shfl(v, src, TEAM); }
+__device__ __forceinline__ int tbci(int v, int src) { return __shfl(v, src, TEAM); }
+// All LDS traffic of a team stays inside one wavefront: a wavefront-scope fence orders it.
+__device__ __forceinline__ void tsync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+struct Lds {
+  double* c;    // [32] concentrations
+  double* Mv;   // [nms] third-body sums
+  double* aux;  // [32] per-lane shared values (e_k, h_k, Y_k, g_k)
+  double* g;    // [32] clip derivative per column
+  double* q;    // [nr] rates of progress
+  double* kfe;  // [nr] effective forward coefficient (incl. M or falloff)
+  double* kbe;  // [nr]
+  double* Dm;   // [nr] dq/dM
+  double* dqT;  // [nr] dq/dT at fixed concentrations
+  double* row;  // [32][33] Jacobian staging (padded)
+  int* piv;     // [32] pivot lane of each elimination step
+};
+
+__host__ __device__ inline size_t team_lds_bytes(int nr, int nms) {
+  return (size_t)(32 * 3 + (nms > 0 ? nms : 1) + 5 * (size_t)nr + 32 * 33) * sizeof(double) + 32 * sizeof(int);
+}
+
+__device__ inline Lds carve(char* base, int nr, int nms) {
+  Lds L;
+  double* d = reinterpret_cast<double*>(base);
+  L.c = d; d += 32;
+  L.aux = d; d += 32;
+  L.g = d; d += 32;
+  L.Mv = d; d += (nms > 0 ? nms : 1);
+  L.q = d; d += nr;
+  L.kfe = d; d += nr;
+  L.kbe = d; d += nr;
+  L.Dm = d; d += nr;
+  L.dqT = d; d += nr;
+  L.row = d; d += 32 * 33;
+  L.piv = reinterpret_cast<int*>(d);
+  return L;
+}
+
+struct RhsOut {
+  double F;                 // this lane's RHS component
+  double xc, g, w, rho;     // clipped state, clip derivative, wdot_k, density
+  double cp, h, dcp, dh;    // thermo of species k at T
+  double Rmix, rho_cv, cpm; // team-uniform sums
+};
+
+// Rates of progress of reaction r (kinds: 0 Arrhenius, 1 three-body, 2 Troe, 3 Lindemann),
+// Lib_Chemistry_falloff.f90:12-80 for the falloff forms.
+template <bool JAC>
+__device__ __forceinline__ void reaction(const Prog& P, int r, const double* __restrict__ rk0, const double* __restrict__ rk1,
+                                         double Td, const Lds& L) {
+  const int kinfo = P.rx_kind[r];
+  const int kind = kinfo & 3;
+  const int col = P.rx_col[r];
+  const int ms = P.rx_mset[r];
+  const double M = ms >= 0 ? L.Mv[ms] : 1.0;
+  double Pf = 1.0, Pb = 1.0;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int code = P.rx_fac[r * 8 + s];
+    if (code >= 0) {
+      const double cv = L.c[code & 255];
+      double f = cv;
+      for (int e = 1; e < (code >> 8); ++e) f *= cv;
+      Pf *= f;
+    }
+  }
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int code = P.rx_fac[r * 8 + 4 + s];
+    if (code >= 0) {
+      const double cv = L.c[code & 255];
+      double f = cv;
+      for (int e = 1; e < (code >> 8); ++e) f *= cv;
+      Pb *= f;
+    }
+  }
+  double k1, k2, dk1T = 0.0, dk2T = 0.0, dqM = 0.0;
+  if (kind <= 1) {
+    const double2 a0 = ld2(rk0 + col), a1 = ld2(rk1 + col);
+    const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    const double Mf = kind == 1 ? M : 1.0;
+    k1 = kf * Mf;
+    k2 = kb * Mf;
+    if (JAC) {
+      dk1T = (a1.x - a0.x) * Mf;
+      dk2T = (a1.y - a0.y) * Mf;
+      if (kind == 1) dqM = kf * Pf - kb * Pb;
+    }
+  } else {
+    const bool troe = kind == 2;
+    double kinf, k0, kc, Fc = 1.0, skinf, sk0, skc, sFc = 0.0;
+    if (troe) {
+      const double2 x0 = ld2(rk0 + col), y0 = ld2(rk0 + col + 2), x1 = ld2(rk1 + col), y1 = ld2(rk1 + col + 2);
+      kinf = x0.x + (x1.x - x0.x) * Td; skinf = x1.x - x0.x;
+      k0 = x0.y + (x1.y - x0.y) * Td; sk0 = x1.y - x0.y;
+      kc = y0.x + (y1.x - y0.x) * Td; skc = y1.x - y0.x;
+      Fc = y0.y + (y1.y - y0.y) * Td; sFc = y1.y - y0.y;
+    } else {
+      kinf = rk0[col] + (rk1[col] - rk0[col]) * Td; skinf = rk1[col] - rk0[col];
+      k0 = rk0[col + 1] + (rk1[col + 1] - rk0[col + 1]) * Td; sk0 = rk1[col + 1] - rk0[col + 1];
+      kc = rk0[col + 2] + (rk1[col + 2] - rk0[col + 2]) * Td; skc = rk1[col + 2] - rk0[col + 2];
+    }
+    const double Pr = k0 * M / kinf;
+    const double gg = Pr / (1.0 + Pr);
+    double Fb = 1.0, lFc = 0.0, f1 = 0.0, cc = 0.0, nn = 0.0, lPr = 0.0;
+    if (troe) {
+      lFc = log10(Fc);
+      lPr = log10(Pr);
+      cc = -0.4 - 0.67 * lFc;
+      nn = 0.75 - 1.27 * lFc;
+      f1 = (lPr + cc) / (nn - 0.14 * (lPr + cc));
+      Fb = exp10(lFc / (1.0 + f1 * f1));
+    }
+    k1 = kinf * gg * Fb;
+    k2 = k1 / kc;
+    if (JAC) {
+      const double dg = 1.0 / ((1.0 + Pr) * (1.0 + Pr));
+      double dlnF_dPr = 0.0, dlnF_dFc = 0.0;
+      if (troe) {
+        const double u = lPr + cc, den = nn - 0.14 * u, qq = 1.0 + f1 * f1;
+        dlnF_dPr = -2.0 * lFc * f1 * (nn / (den * den)) / (qq * qq * Pr);
+        const double df1_dL = (-0.67 * nn + 1.27 * u) / (den * den);
+        dlnF_dFc = (1.0 / qq - lFc * 2.0 * f1 * df1_dL / (qq * qq)) / Fc;
+      }
+      const double dk1_dPr = kinf * Fb * (dg + gg * dlnF_dPr);
+      const double dPr_dT = M * (sk0 * kinf - k0 * skinf) / (kinf * kinf);
+      dk1T = skinf * gg * Fb + dk1_dPr * dPr_dT + k1 * dlnF_dFc * sFc;
+      dk2T = dk1T / kc - k1 * skc / (kc * kc);
+      const double dk1M = dk1_dPr * k0 / kinf;
+      dqM = dk1M * Pf - (dk1M / kc) * Pb;
+    }
+  }
+  L.q[r] = k1 * Pf - k2 * Pb;
+  if (JAC) {
+    L.kfe[r] = k1;
+    L.kbe[r] = k2;
+    L.Dm[r] = dqM;
+    L.dqT[r] = dk1T * Pf - dk2T * Pb;
+  }
+}
+
+// Team right-hand side.  Returns false (team-uniform) when T leaves the tables.
+template <bool JAC>
+__device__ bool team_rhs(const Prog& P, int mode, double p, double x, const Lds& L, RhsOut& o) {
+  const int lane = threadIdx.x & (TEAM - 1);
+  const int ns = P.ns;
+  const double T = tbc(x, ns);
+  if (!(T >= (double)P.Th_min && T < (double)P.Th_max && T >= (double)P.Tk_min && T < (double)P.Tk_max)) return false;
+  const bool sp = lane < ns;
+  const double Wk = sp ? P.W[lane] : 1.0;
+  const double Rk = sp ? 8314.51 / Wk : 0.0;
+  o.xc = sp ? fmax(x, 0.0) : 0.0;
+  o.g = (sp && x >= 0.0) ? 1.0 : 0.0;
+  double roi;
+  if (mode == 0) {
+    roi = o.xc;
+    o.rho = 1.0;
+    o.Rmix = 0.0;
+  } else {
+    o.Rmix = tsum(o.xc * Rk);
+    o.rho = p / (o.Rmix * T);
+    roi = o.rho * o.xc;
+  }
+  if (sp) L.c[lane] = roi / Wk;
+  tsync();
+  for (int s = lane; s < P.nms; s += TEAM) {
+    const int t0 = P.ms_ptr[s], t1 = P.ms_ptr[s + 1];
+    double m = L.c[P.ms_sp[t0]] * P.ms_eff[t0];
+    for (int t = t0 + 1; t < t1; ++t) m = m + L.c[P.ms_sp[t]] * P.ms_eff[t];
+    L.Mv[s] = m;
+  }
+  tsync();
+  const int T_i = (int)T;
+  const double Td = T - (double)T_i;
+  const double* __restrict__ rk0 = P.tabk + (long)(T_i - P.Tk_min) * P.wk;
+  const double* __restrict__ rk1 = rk0 + P.wk;
+  for (int r = lane; r < P.nr; r += TEAM) reaction<JAC>(P, r, rk0, rk1, Td, L);
+  tsync();
+  double w = 0.0;
+  if (sp) {
+    const int t0 = P.wd_ptr[lane], t1 = P.wd_ptr[lane + 1];
+    if (t1 > t0) {
+      double acc = P.wd_coef[t0] * L.q[P.wd_rx[t0]];
+      for (int t = t0 + 1; t < t1; ++t) acc = acc + P.wd_coef[t] * L.q[P.wd_rx[t]];
+      w = Wk * acc;
+    }
+  }
+  o.w = w;
+  const double* __restrict__ rh0 = P.tabh + (long)(T_i - P.Th_min) * (2 * ns);
+  const double* __restrict__ rh1 = rh0 + 2 * ns;
+  o.cp = o.h = o.dcp = o.dh = 0.0;
+  if (sp) {
+    const double2 t0 = ld2(rh0 + 2 * lane), t1 = ld2(rh1 + 2 * lane);
+    o.cp = t0.x + (t1.x - t0.x) * Td;
+    o.h = t0.y + (t1.y - t0.y) * Td;
+    o.dcp = t1.x - t0.x;
+    o.dh = t1.y - t0.y;
+  }
+  double FT;
+  if (mode == 0) {
+    const double eiroi = tsum((o.h - Rk * T) * w);
+    o.rho_cv = tsum(roi * (o.cp - Rk));
+    FT = -eiroi / o.rho_cv;
+    o.F = sp ? w : 0.0;
+  } else {
+    const double Q = tsum(o.h * w);
+    o.cpm = tsum(o.xc * o.cp);
+    FT = -Q / (o.rho * o.cpm);
+    o.F = sp ? w / o.rho : 0.0;
+  }
+  if (lane == ns) o.F = FT;
+  return true;
+}
+
+// Jacobian row of this lane (J[lane][0..31], zero beyond nz) after team_rhs<true>.
+__device__ void team_jac(const Prog& P, int mode, double x, const Lds& L, const RhsOut& o, double (&Jr)[TEAM]) {
+  const int lane = threadIdx.x & (TEAM - 1);
+  const int ns = P.ns, nz = ns + 1;
+  const bool sp = lane < ns;
+  const double T = tbc(x, ns);
+  const double FT = tbc(o.F, ns);
+  double* myrow = L.row + lane * 33;
+#pragma unroll
+  for (int j = 0; j < TEAM; ++j) {
+    Jr[j] = 0.0;
+    myrow[j] = 0.0;
+  }
+  L.g[lane] = o.g;
+  if (mode == 1) L.aux[lane] = o.xc;
+  tsync();
+  double bT = 0.0;
+  const double Wk = sp ? P.W[lane] : 1.0;
+  if (sp) {
+    const int t0 = P.wd_ptr[lane], t1 = P.wd_ptr[lane + 1];
+    for (int t = t0; t < t1; ++t) {
+      const int r = P.wd_rx[t];
+      const double coef = P.wd_coef[t];
+      for (int side = 0; side < 2; ++side) {
+        const double kk = side == 0 ? L.kfe[r] : -L.kbe[r];
+        for (int s = 0; s < 4; ++s) {
+          const int code = P.rx_fac[r * 8 + side * 4 + s];
+          if (code < 0) continue;
+          const int j = code & 255, pw = code >> 8;
+          double d = kk * pw;
+          for (int e = 1; e < pw; ++e) d *= L.c[j];
+          for (int u = 0; u < 4; ++u) {
+            const int cu = P.rx_fac[r * 8 + side * 4 + u];
+            if (u == s || cu < 0) continue;
+            for (int e = 0; e < (cu >> 8); ++e) d *= L.c[cu & 255];
+          }
+          myrow[j] += coef * d;
+        }
+      }
+      const int ms = P.rx_mset[r];
+      if (ms >= 0) {
+        const double Dv = coef * L.Dm[r];
+        const double* __restrict__ ef = P.effd + ms * TEAM;
+#pragma unroll
+        for (int j = 0; j < TEAM; ++j) Jr[j] += Dv * ef[j];
+      }
+      bT += coef * L.dqT[r];
+    }
+  }
+  tsync();
+  // species block A_kj = d wdot_k / d rho_j (unclipped), b_k = d wdot_k / dT at fixed rho
+  const double bk = Wk * bT;
+#pragma unroll
+  for (int j = 0; j < TEAM; ++j) {
+    const double Wj = j < ns ? P.W[j] : 1.0;
+    Jr[j] = (j < ns && sp) ? (Jr[j] + myrow[j]) * Wk / Wj : 0.0;
+  }
+  const double Rk = sp ? 8314.51 / Wk : 0.0;
+  if (mode == 0) {
+    // constant volume: J_kj = g_j A_kj, J_kT = b_k; T row from column sums of e_k J_kj
+#pragma unroll
+    for (int j = 0; j < TEAM; ++j) Jr[j] *= L.g[j];
+    if (sp) Jr[ns] = bk;
+    const double e = sp ? o.h - Rk * T : 0.0;
+    const double s_de = tsum(sp ? (o.dh - Rk) * o.w : 0.0);
+    const double s_dcv = tsum(sp ? o.xc * o.dcp : 0.0);
+#pragma unroll
+    for (int j = 0; j < TEAM; ++j) myrow[j] = Jr[j];
+    L.aux[lane] = e;
+    tsync();
+    if (lane < nz) {
+      double cs = 0.0;
+      for (int k = 0; k < ns; ++k) cs += L.aux[k] * L.row[k * 33 + lane];
+      double jt;
+      if (lane < ns) jt = -(cs + FT * (o.cp - Rk) * o.g) / o.rho_cv;
+      else jt = -(s_de + cs + FT * s_dcv) / o.rho_cv;
+      L.row[ns * 33 + lane] = jt;
+    }
+    tsync();
+  } else {
+    // constant pressure (Y, T): see oracle/flint_oracle.c jac_cp_cell for the formulas
+    double AY = 0.0;
+#pragma unroll
+    for (int j = 0; j < TEAM; ++j) AY += Jr[j] * (j < ns ? L.aux[j] : 0.0);
+    const double Rm = o.Rmix, rho = o.rho;
+    const double dwdT = bk - rho * AY / T;
+#pragma unroll
+    for (int j = 0; j < TEAM; ++j) {
+      const double Rj = j < ns ? 8314.51 / P.W[j] : 0.0;
+      const double gj = L.g[j];
+      const double a = Jr[j];
+      myrow[j] = sp ? rho * gj * (a - AY * Rj / Rm) : 0.0;
+      Jr[j] = sp ? gj * (a - AY * Rj / Rm + (o.w / rho) * Rj / Rm) : 0.0;
+    }
+    if (sp) {
+      Jr[ns] = dwdT / rho + o.w / (rho * T);
+      myrow[ns] = dwdT;
+    }
+    const double s_dhw = tsum(sp ? o.dh * o.w : 0.0);
+    const double sYdcp = tsum(sp ? o.xc * o.dcp : 0.0);
+    tsync();
+    L.aux[lane] = sp ? o.h : 0.0;
+    tsync();
+    if (lane < nz) {
+      double ch = 0.0;
+      for (int k = 0; k < ns; ++k) ch += L.aux[k] * L.row[k * 33 + lane];
+      double jt;
+      if (lane < ns) jt = -ch / (rho * o.cpm) - FT * o.g * (o.cp / o.cpm - Rk / Rm);
+      else jt = -(s_dhw + ch) / (rho * o.cpm) - FT * (sYdcp / o.cpm - 1.0 / T);
+      L.row[ns * 33 + lane] = jt;
+    }
+    tsync();
+  }
+  if (lane == ns) {
+#pragma unroll
+    for (int j = 0; j < TEAM; ++j) Jr[j] = j < nz ? L.row[ns * 33 + j] : 0.0;
+  }
+  tsync();
+}
+
+// LU of I - c J with partial pivoting, one row per lane; returns this lane's elimination step.
+__device__ int team_lu(const double (&Jr)[TEAM], double c, int nz, double (&a)[TEAM], const Lds& L) {
+  const int lane = threadIdx.x & (TEAM - 1);
+#pragma unroll
+  for (int j = 0; j < TEAM; ++j) a[j] = (lane == j ? 1.0 : 0.0) - (lane < nz ? c * Jr[j] : 0.0);
+  int mystep = -1;
+#pragma unroll
+  for (int k = 0; k < TEAM; ++k) {
+    double best = mystep < 0 ? fabs(a[k]) : -1.0;
+    int bl = lane;
+#pragma unroll
+    for (int m = 16; m >= 1; m >>= 1) {
+      const double ob = __shfl_xor(best, m, TEAM);
+      const int ol = __shfl_xor(bl, m, TEAM);
+      if (ob > best || (ob == best && ol < bl)) { best = ob; bl = ol; }
+    }
+    if (lane == bl) mystep = k;
+    if (lane == 0) L.piv[k] = bl;
+    const double pv = tbc(a[k], bl);
+    const bool elim = mystep < 0;
+    const double l = a[k] / pv;
+    if (elim) a[k] = l;
+#pragma unroll
+    for (int j = k + 1; j < TEAM; ++j) {
+      const double pr = tbc(a[j], bl);
+      if (elim) a[j] -= l * pr;
+    }
+  }
+  tsync();
+  return mystep;
+}
+
+// Solve (I - cJ) x = b with the factors above; b and x are this lane's component.
+__device__ double team_solve(const double (&a)[TEAM], int mystep, double b, const Lds& L) {
+  const int lane = threadIdx.x & (TEAM - 1);
+#pragma unroll
+  for (int k = 0; k < TEAM; ++k) {
+    const double zk = tbc(b, L.piv[k]);
+    if (mystep > k) b -= a[k] * zk;
+  }
+  double s = 0.0, x = 0.0;
+#pragma unroll
+  for (int k = TEAM - 1; k >= 0; --k) {
+    const int p = L.piv[k];
+    const double xk = tbc((b - s) / a[k], p);
+    if (mystep < k) s += a[k] * xk;
+    if (lane == k) x = xk;
+  }
+  return x;
+}
+
+__device__ __forceinline__ double team_rms(double v, double scale, int nz) {
+  const int lane = threadIdx.x & (TEAM - 1);
+  const double q = lane < nz ? v / scale : 0.0;
+  return sqrt(tsum(q * q) / nz);
+}
+
+// D[0..order] <- (R U)^T D[0..order] on this lane's column (R, U from compute_R of the method)
+__device__ void change_D(double (&D)[MAXO + 3], int order, double factor) {
+  double R[MAXO + 1][MAXO + 1], U[MAXO + 1][MAXO + 1];
+#pragma unroll
+  for (int j = 0; j <= MAXO; ++j) { R[0][j] = 1.0; U[0][j] = 1.0; }
+#pragma unroll
+  for (int i = 1; i <= MAXO; ++i)
+#pragma unroll
+    for (int j = 0; j <= MAXO; ++j) {
+      R[i][j] = R[i - 1][j] * (j == 0 ? 0.0 : (i - 1 - factor * j) / i);
+      U[i][j] = U[i - 1][j] * (j == 0 ? 0.0 : (i - 1 - 1.0 * j) / i);
+    }
+  double nd[MAXO + 1];
+#pragma unroll
+  for (int i = 0; i <= MAXO; ++i) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j <= MAXO; ++j) {
+      if (j > order) continue;
+      double ru = 0.0;  // RU[j][i] = sum_l R[j][l] U[l][i]
+#pragma unroll
+      for (int l = 0; l <= MAXO; ++l)
+        if (l <= order) ru += R[j][l] * U[l][i];
+      s += ru * D[j];
+    }
+    nd[i] = s;
+  }
+#pragma unroll
+  for (int i = 0; i <= MAXO; ++i)
+    if (i <= order) D[i] = nd[i];
+}
+
+// One cell; mirrors bdf_cell of oracle/oracle_bdf.c.  Returns 0 ok, 1 step underflow, 2 too many
+// steps, 3 invalid state.
+__device__ int integrate(const Prog& P, const Opts& O, double p, double& y, int (&st)[8], const Lds& L) {
+  const int lane = threadIdx.x & (TEAM - 1);
+  const int nz = P.ns + 1;
+  const double rtol = O.rtol, atol = O.atol, t_end = O.t_end;
+  double gamma[MAXO + 1];
+  gamma[0] = 0.0;
+#pragma unroll
+  for (int k = 1; k <= MAXO; ++k) gamma[k] = gamma[k - 1] + 1.0 / k;
+  const double newton_tol = fmax(10.0 * 2.220446049250313e-16 / rtol, fmin(0.03, sqrt(rtol)));
+  RhsOut o;
+  if (!team_rhs<false>(P, O.mode, p, y, L, o)) return 3;
+  st[ST_RHS]++;
+  const double f0 = o.F;
+  double sc = atol + fabs(y) * rtol;
+  const double d0 = team_rms(y, sc, nz), d1 = team_rms(f0, sc, nz);
+  double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+  h0 = fmin(h0, t_end);
+  if (!team_rhs<false>(P, O.mode, p, y + h0 * f0, L, o)) return 3;
+  st[ST_RHS]++;
+  const double d2 = team_rms(o.F - f0, sc, nz) / h0;
+  const double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? fmax(1e-6, h0 * 1e-3) : sqrt(0.01 / fmax(d1, d2));
+  double h_abs = fmin(100.0 * h0, fmin(h1, t_end));
+
+  double D[MAXO + 3];
+#pragma unroll
+  for (int j = 0; j < MAXO + 3; ++j) D[j] = 0.0;
+  D[0] = y;
+  D[1] = f0 * h_abs;
+  int order = 1, n_equal = 0;
+  bool have_lu = false, current_jac = false;
+  double Jr[TEAM], a[TEAM];
+  int mystep = lane;
+  // Jacobian at the initial state
+  if (!team_rhs<true>(P, O.mode, p, y, L, o)) return 3;
+  team_jac(P, O.mode, y, L, o, Jr);
+  st[ST_JAC]++;
+  double t = 0.0;
+
+  while (t < t_end) {
+    if (st[ST_STEPS] >= O.max_steps) return 2;
+    const double min_step = 10.0 * fabs(nextafter(t, INFINITY) - t);
+    if (h_abs < min_step) {
+      change_D(D, order, min_step / h_abs);
+      h_abs = min_step;
+      n_equal = 0;
+    }
+    double ynew = 0.0, d = 0.0, t_new = t, err_norm = 0.0, safety = 0.9;
+    bool accepted = false;
+    while (!accepted) {
+      if (h_abs < min_step) return 1;
+      t_new = t + h_abs;
+      if (t_new - t_end > 0) {
+        t_new = t_end;
+        change_D(D, order, fabs(t_new - t) / h_abs);
+        n_equal = 0;
+        have_lu = false;
+      }
+      const double h = t_new - t;
+      h_abs = fabs(h);
+      double ypred = 0.0, psi = 0.0;
+#pragma unroll
+      for (int j = 0; j <= MAXO; ++j)
+        if (j <= order) ypred += D[j];
+#pragma unroll
+      for (int j = 1; j <= MAXO; ++j)
+        if (j <= order) psi += D[j] * gamma[j];
+      double alpha_o = 0.0;
+#pragma unroll
+      for (int j = 1; j <= MAXO; ++j)
+        if (j == order) alpha_o = gamma[j];
+      psi /= alpha_o;
+      const double scale = atol + rtol * fabs(ypred);
+      const double cc = h / alpha_o;
+      bool converged = false;
+      int n_iter = 0;
+      while (!converged) {
+        if (!have_lu) {
+          mystep = team_lu(Jr, cc, nz, a, L);
+          st[ST_LU]++;
+          have_lu = true;
+        }
+        ynew = ypred;
+        d = 0.0;
+        double dy_norm_old = -1.0;
+        int k;
+        for (k = 0; k < NEWTON_MAXITER; ++k) {
+          const bool ok = team_rhs<false>(P, O.mode, p, ynew, L, o);
+          st[ST_RHS]++;
+          if (!ok || !isfinite(tsum(o.F))) break;
+          const double dy = team_solve(a, mystep, cc * o.F - psi - d, L);
+          st[ST_NEWTON]++;
+          const double dy_norm = team_rms(dy, scale, nz);
+          const double rate = dy_norm_old >= 0.0 ? dy_norm / dy_norm_old : -1.0;
+          if (rate >= 0.0 && (rate >= 1.0 || pow(rate, (double)(NEWTON_MAXITER - k)) / (1.0 - rate) * dy_norm > newton_tol))
+            break;
+          ynew += dy;
+          d += dy;
+          if (dy_norm == 0.0 || (rate >= 0.0 && rate / (1.0 - rate) * dy_norm < newton_tol)) {
+            converged = true;
+            break;
+          }
+          dy_norm_old = dy_norm;
+        }
+        n_iter = k + 1;
+        if (!converged) {
+          if (current_jac) break;
+          if (!team_rhs<true>(P, O.mode, p, ypred, L, o)) break;
+          team_jac(P, O.mode, ypred, L, o, Jr);
+          st[ST_JAC]++;
+          have_lu = false;
+          current_jac = true;
+        }
+      }
+      if (!converged) {
+        st[ST_NFAIL]++;
+        h_abs *= 0.5;
+        change_D(D, order, 0.5);
+        n_equal = 0;
+        have_lu = false;
+        continue;
+      }
+      safety = 0.9 * (2 * NEWTON_MAXITER + 1) / (double)(2 * NEWTON_MAXITER + n_iter);
+      const double scale2 = atol + rtol * fabs(ynew);
+      sc = scale2;
+      err_norm = team_rms(d / (order + 1), scale2, nz);
+      if (err_norm > 1.0) {
+        st[ST_REJECT]++;
+        const double factor = fmax(MIN_FACTOR, safety * pow(err_norm, -1.0 / (order + 1)));
+        h_abs *= factor;
+        change_D(D, order, factor);
+        n_equal = 0;
+      } else {
+        accepted = true;
+      }
+    }
+    st[ST_STEPS]++;
+    n_equal++;
+    t = t_new;
+    y = ynew;
+    current_jac = false;
+    // D^{j+1} y_n = D^j y_n - D^j y_{n-1}, d = D^{order+1} y_n
+    {
+      double Dop1 = 0.0;
+#pragma unroll
+      for (int j = 0; j < MAXO + 3; ++j)
+        if (j == order + 1) Dop1 = D[j];
+#pragma unroll
+      for (int j = 0; j < MAXO + 3; ++j) {
+        if (j == order + 2) D[j] = d - Dop1;
+        if (j == order + 1) D[j] = d;
+      }
+#pragma unroll
+      for (int j = MAXO; j >= 0; --j)
+        if (j <= order) D[j] += D[j + 1];
+    }
+    if (n_equal < order + 1) continue;
+    double Dm = 0.0, Dp = 0.0;
+#pragma unroll
+    for (int j = 0; j < MAXO + 3; ++j) {
+      if (j == order) Dm = D[j];
+      if (j == order + 2) Dp = D[j];
+    }
+    const double em = order > 1 ? team_rms(Dm / order, sc, nz) : INFINITY;
+    const double ep = order < MAXO ? team_rms(Dp / (order + 2), sc, nz) : INFINITY;
+    const double norms[3] = {em, err_norm, ep};
+    double best = -1.0;
+    int which = 1;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const double fct = norms[q] == 0.0 ? INFINITY : pow(norms[q], -1.0 / (order + q));
+      if (fct > best) { best = fct; which = q; }
+    }
+    order += which - 1;
+    const double factor = fmin(MAX_FACTOR, safety * best);
+    h_abs *= factor;
+    change_D(D, order, factor);
+    n_equal = 0;
+    have_lu = false;
+  }
+  st[ST_ORDER] = order;
+  return 0;
+}
+
+__global__ void zero_counter(int* c) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = 0;
+}
+
+__global__ __launch_bounds__(256) void bdf_kernel(Prog P, Opts O, long n, long ld, double* __restrict__ X,
+                                                  const double* __restrict__ pres, int* __restrict__ stats,
+                                                  int* __restrict__ status, int* __restrict__ counter,
+                                                  unsigned team_bytes) {
+  extern __shared__ __align__(16) char smem[];
+  const int lane = threadIdx.x & (TEAM - 1);
+  const int team = threadIdx.x / TEAM;
+  const Lds L = carve(smem + (size_t)team * team_bytes, P.nr, P.nms);
+  const int nz = P.ns + 1;
+  while (true) {
+    int cell = 0;
+    if (lane == 0) cell = atomicAdd(counter, 1);
+    cell = tbci(cell, 0);
+    if (cell >= n) break;
+    double y = lane < nz ? X[(long)lane * ld + cell] : 0.0;
+    const double p = O.mode == 1 ? pres[cell] : 0.0;
+    int st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const int rc = integrate(P, O, p, y, st, L);
+    if (lane < nz) X[(long)lane * ld + cell] = y;
+    if (stats && lane < 8) {
+      int v = st[0];
+#pragma unroll
+      for (int q = 1; q < 8; ++q)
+        if (lane == q) v = st[q];
+      stats[(long)lane * ld + cell] = v;
+    }
+    if (status && lane == 0) status[cell] = rc;
+  }
+}
+
+}  // namespace flint_reactor
+
+hipError_t flint_reactor_set_max_lds(int bytes) {
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::bdf_kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+size_t flint_reactor_team_bytes(int nr, int nms) { return flint_reactor::team_lds_bytes(nr, nms); }
+
+hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reactor::Opts& O, long n, long ld,
+                                double* X, const double* pres, int* stats, int* status, int* counter, int teams_per_block,
+                                int blocks, hipStream_t s) {
+  const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P.nr, P.nms) + 15) & ~size_t(15));
+  hipLaunchKernelGGL(flint_reactor::zero_counter, dim3(1), dim3(64), 0, s, counter);
+  hipLaunchKernelGGL(flint_reactor::bdf_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
+                     (size_t)tb * teams_per_block, s, P, O, n, ld, X, pres, stats, status, counter, tb);
+  return hipGetLastError();
+}

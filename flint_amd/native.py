@@ -37,6 +37,11 @@ def lib():
         L.flint_mech_query.argtypes = [vp] + [C.POINTER(i)] * 5
         L.flint_source.argtypes = [vp, l, l, vp, vp, vp, vp, vp, vp]
         L.flint_rhs.argtypes = [vp, l, l, vp, vp, vp, vp, vp]
+        L.flint_reactor_advance.argtypes = [vp, l, l, i, vp, vp, d, d, d, i, vp, vp, vp]
+        L.flint_reactor_advance_graph.argtypes = [vp, l, l, i, vp, vp, d, d, d, i, vp, vp, vp]
+        L.flint_reactor_autotune.argtypes = [vp, l, l, i, vp, vp, d, d, d, C.POINTER(d)]
+        L.flint_reactor_set_launch.argtypes = [vp, i, i]
+        L.flint_reactor_get_launch.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
         L.flint_source_host.argtypes = [vp, l, vp, vp, vp]
         L.flint_rhs_host.argtypes = [vp, l, vp, vp, vp]
         _lib = L
@@ -133,6 +138,32 @@ class Mechanism:
         ld = Z.shape[-1]
         n = ld if n is None else n
         check(lib().flint_rhs(self._h, n, ld, _p(Z), _p(F), _p(J), _p(status), _stream(stream)))
+
+    # ---- batched reactors ----
+    CV, CP = 0, 1
+
+    def advance(self, X, p=None, mode=0, t_end=1e-4, rtol=1e-6, atol=1e-12, max_steps=0, stats=None, status=None,
+                n=None, graph=False, stream=None):
+        """Advance X [ns+1, ld] (torch, device) in place from 0 to t_end."""
+        ld = X.shape[-1]
+        n = ld if n is None else n
+        fn = lib().flint_reactor_advance_graph if graph else lib().flint_reactor_advance
+        check(fn(self._h, n, ld, mode, _p(X), _p(p), t_end, rtol, atol, max_steps, _p(stats), _p(status),
+                 _stream(stream)))
+
+    def autotune(self, X, p=None, mode=0, t_end=1e-4, rtol=1e-6, atol=1e-12):
+        ld = X.shape[-1]
+        best = C.c_double()
+        check(lib().flint_reactor_autotune(self._h, ld, ld, mode, _p(X), _p(p), t_end, rtol, atol, C.byref(best)))
+        return best.value, self.get_launch()
+
+    def set_launch(self, teams_per_block, blocks):
+        check(lib().flint_reactor_set_launch(self._h, teams_per_block, blocks))
+
+    def get_launch(self):
+        a, b = C.c_int(), C.c_int()
+        check(lib().flint_reactor_get_launch(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     # ---- host-memory convenience (numpy, SoA [rows, n]) ----
     def source_host(self, rho, T):

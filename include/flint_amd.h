@@ -84,6 +84,32 @@ int flint_rhs(const flint_mech *m, long n, long ld, const double *Z_dev, double 
 int flint_source_host(flint_mech *m, long n, const double *rho, const double *T, double *wdot);
 int flint_rhs_host(flint_mech *m, long n, const double *Z, double *F, double *J);
 
+/* Batched 0D reactors (no reference counterpart in FLINT's own code: it advances one reactor at a
+ * time through the external OSlo integrator, test-batchF.f90:81,105, with rhs_native as the
+ * callback).  Advances n cells from t = 0 to t_end in one persistent launch, variable-order BDF
+ * (orders 1..5) with Newton and a per-cell dense LU, rtol/atol per component.
+ *   mode 0: constant volume, X_dev[(ns+1)][ld] = (rho_1..rho_ns, T) (rhs_native)
+ *   mode 1: constant pressure, X_dev = (Y_1..Y_ns, T), p_dev[n] in Pa
+ * X_dev is advanced in place.  stats_dev[8][ld] (may be NULL) receives per cell: accepted steps,
+ * rejected steps, Newton failures, RHS evaluations, Jacobians, LU factorisations, Newton
+ * iterations, final order.  status_dev[n] (may be NULL): 0 ok, 1 step underflow, 2 max_steps
+ * reached, 3 T outside the tables.  max_steps <= 0 means 100000.  Needs ns + 1 <= 32. */
+int flint_reactor_advance(flint_mech *m, long n, long ld, int mode, double *X_dev, const double *p_dev, double t_end,
+                          double rtol, double atol, int max_steps, int *stats_dev, int *status_dev, void *stream);
+
+/* As flint_reactor_advance, launched through a hipGraph that is captured on the first call and
+ * replayed while the arguments stay identical. */
+int flint_reactor_advance_graph(flint_mech *m, long n, long ld, int mode, double *X_dev, const double *p_dev,
+                                double t_end, double rtol, double atol, int max_steps, int *stats_dev, int *status_dev,
+                                void *stream);
+
+/* Launch-shape autotuner: times the candidate (teams per block, blocks) shapes on a copy of the
+ * given sample and keeps the fastest for this handle (synchronous; the sample is not modified). */
+int flint_reactor_autotune(flint_mech *m, long n, long ld, int mode, const double *X_dev, const double *p_dev,
+                           double t_end, double rtol, double atol, double *best_ms);
+int flint_reactor_set_launch(flint_mech *m, int teams_per_block, int blocks);
+int flint_reactor_get_launch(flint_mech *m, int *teams_per_block, int *blocks);
+
 #ifdef __cplusplus
 }
 #endif
