@@ -36,7 +36,11 @@ struct Opts {
 size_t flint_reactor_team_bytes(int nr, int nms);
 hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reactor::Opts& O, long n, long ld,
                                 double* X, const double* pres, int* stats, int* status, int* counter,
-                                int teams_per_block, int blocks, hipStream_t s);
+                                int teams_per_block, int blocks, double* jac_scratch, hipStream_t s);
+
+hipError_t flint_reactor_jac_launch(const flint_reactor::Prog& P, int mode, long n, long ld, const double* X,
+                                    const double* pres, double* F, double* J, int teams_per_block, int blocks,
+                                    double* jac_scratch, hipStream_t s);
 
 // Host copy of a generated mechanism's rate program (emitted by flint_amd/codegen.py).
 struct flint_host_prog {

@@ -17,6 +17,8 @@ struct flint_extra {
   flint_reactor::Prog prog{};
   std::vector<void*> bufs;  // device allocations backing prog
   int* d_counter = nullptr;
+  double* d_jac = nullptr;  // per-team Jacobian slots (32 x 33 doubles each)
+  size_t jac_teams = 0;
   int teams_per_block = 2;
   int blocks = 0;
   bool tuned = false;
@@ -39,6 +41,7 @@ void flint_mech_release_extra(flint_mech* m) {
   if (x->graph) (void)hipGraphDestroy(x->graph);
   for (void* p : x->bufs) (void)hipFree(p);
   if (x->d_counter) (void)hipFree(x->d_counter);
+  if (x->d_jac) (void)hipFree(x->d_jac);
   delete x;
   m->extra = nullptr;
 }
@@ -58,6 +61,8 @@ static int ensure_extra(flint_mech* m) {
   if (m->is_wd || !m->ops || !m->ops->prog)
     return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel needs a generated mechanism (not WD)");
   if (m->ns + 1 > 32) return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports ns + 1 <= 32");
+  if (4 * m->ops->prog->nr > 32 * 33)
+    return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports at most 264 reactions");
   const flint_host_prog* hp = m->ops->prog;
   auto* x = new flint_extra();
   m->extra = x;
@@ -112,10 +117,32 @@ static int check_args(const flint_mech* m, long n, long ld, int mode, const doub
   return FLINT_OK;
 }
 
+// Per-team Jacobian slots for a launch of tpb x blocks teams.  Grows only outside graph capture:
+// the capture path sizes the buffer before capturing.
+static int ensure_jac(flint_extra* x, int tpb, int blocks) {
+  const size_t teams = (size_t)tpb * blocks;
+  if (teams > x->jac_teams) {
+    // a captured graph holds the old slot address: drop it before the buffer moves
+    if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
+    if (x->graph) (void)hipGraphDestroy(x->graph);
+    x->gexec = nullptr;
+    x->graph = nullptr;
+    x->gkey.clear();
+    if (x->d_jac) HIPCHK(hipFree(x->d_jac));
+    x->d_jac = nullptr;
+    x->jac_teams = 0;
+    HIPCHK(hipMalloc(&x->d_jac, teams * 32 * 33 * sizeof(double)));
+    x->jac_teams = teams;
+  }
+  return FLINT_OK;
+}
+
 static int launch(flint_mech* m, long n, long ld, int mode, double* X, const double* p, double t_end, double rtol,
                   double atol, int max_steps, int* stats, int* status, hipStream_t s, int tpb, int blocks) {
-  flint_reactor::Opts O{mode, rtol, atol, t_end, max_steps > 0 ? max_steps : 100000};
-  hipError_t e = flint_reactor_launch(m->extra->prog, O, n, ld, X, p, stats, status, m->extra->d_counter, tpb, blocks, s);
+  flint_extra* x = m->extra;
+  if (int rc = ensure_jac(x, tpb, blocks)) return rc;
+  flint_reactor::Opts O{mode, rtol, atol, t_end, max_steps > 0 ? max_steps : 20000};
+  hipError_t e = flint_reactor_launch(x->prog, O, n, ld, X, p, stats, status, x->d_counter, tpb, blocks, x->d_jac, s);
   if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("reactor launch: ") + hipGetErrorString(e));
   return FLINT_OK;
 }
@@ -127,6 +154,21 @@ extern "C" int flint_reactor_advance(flint_mech* m, long n, long ld, int mode, d
   if (int rc = ensure_extra(m)) return rc;
   return launch(m, n, ld, mode, X, p, t_end, rtol, atol, max_steps, stats, status, (hipStream_t)stream,
                 m->extra->teams_per_block, m->extra->blocks);
+}
+
+extern "C" int flint_reactor_jacobian(flint_mech* m, long n, long ld, int mode, const double* X, const double* p,
+                                      double* F, double* J, void* stream) {
+  if (int rc = check_args(m, n, ld, mode, X, p, 1.0, 1.0, 1.0)) return rc;
+  if (n > 0 && (!F || !J)) return flint_fail(FLINT_EARG, "null output");
+  if (n == 0) return FLINT_OK;
+  if (int rc = ensure_extra(m)) return rc;
+  flint_extra* x = m->extra;
+  const int tpb = x->teams_per_block;
+  const int blocks = (int)std::min<long>(x->blocks, (n + tpb - 1) / tpb);
+  if (int rc = ensure_jac(x, tpb, blocks)) return rc;
+  hipError_t e = flint_reactor_jac_launch(x->prog, mode, n, ld, X, p, F, J, tpb, blocks, x->d_jac, (hipStream_t)stream);
+  if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("reactor jacobian launch: ") + hipGetErrorString(e));
+  return FLINT_OK;
 }
 
 extern "C" int flint_reactor_set_launch(flint_mech* m, int teams_per_block, int blocks) {
@@ -230,6 +272,7 @@ extern "C" int flint_reactor_advance_graph(flint_mech* m, long n, long ld, int m
   std::vector<char> kb(reinterpret_cast<char*>(&key), reinterpret_cast<char*>(&key) + sizeof key);
   hipStream_t s = (hipStream_t)stream;
   if (!x->gexec || kb != x->gkey) {
+    if (int rc = ensure_jac(x, x->teams_per_block, x->blocks)) return rc;
     if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
     if (x->graph) (void)hipGraphDestroy(x->graph);
     x->gexec = nullptr;

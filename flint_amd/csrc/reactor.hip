@@ -30,7 +30,13 @@ constexpr double MIN_FACTOR = 0.2, MAX_FACTOR = 10.0;
 
 enum { ST_STEPS, ST_REJECT, ST_NFAIL, ST_RHS, ST_JAC, ST_LU, ST_NEWTON, ST_ORDER };
 
+// Team sum, bit-identical on every lane: each butterfly level adds the same two rounded values
+// (a + b == b + a exactly).  The register barrier rounds v first; without it the compiler may
+// contract a caller's product into the first addition, fma(x_i, y_i, shfl(x_j y_j)), which differs
+// between partner lanes, and step sizes, orders and branch decisions would then diverge inside
+// one team.
 __device__ __forceinline__ double tsum(double v) {
+  asm volatile("" : "+v"(v));
 #pragma unroll
   for (int m = 16; m >= 1; m >>= 1) v += __shfl_xor(v, m, TEAM);
   return v;
@@ -46,19 +52,19 @@ __device__ __forceinline__ void tsync() {
 struct Lds {
   double* c;    // [32] concentrations
   double* Mv;   // [nms] third-body sums
-  double* aux;  // [32] per-lane shared values (e_k, h_k, Y_k, g_k)
+  double* aux;  // [32] per-lane shared values (Y_k, e_k or h_k)
   double* g;    // [32] clip derivative per column
   double* q;    // [nr] rates of progress
-  double* kfe;  // [nr] effective forward coefficient (incl. M or falloff)
-  double* kbe;  // [nr]
-  double* Dm;   // [nr] dq/dM
-  double* dqT;  // [nr] dq/dT at fixed concentrations
-  double* row;  // [32][33] Jacobian staging (padded)
+  double* lu;   // [32][33] LU factors of I - cJ (row = lane, padded)
+  double* kfe;  // [nr] effective forward coefficient (incl. M or falloff)   } alias lu while the
+  double* kbe;  // [nr]                                                     } Jacobian is built
+  double* Dm;   // [nr] dq/dM                                               } (the LU is rebuilt
+  double* dqT;  // [nr] dq/dT at fixed concentrations                       }  right after)
   int* piv;     // [32] pivot lane of each elimination step
 };
 
 __host__ __device__ inline size_t team_lds_bytes(int nr, int nms) {
-  return (size_t)(32 * 3 + (nms > 0 ? nms : 1) + 5 * (size_t)nr + 32 * 33) * sizeof(double) + 32 * sizeof(int);
+  return (size_t)(32 * 3 + (nms > 0 ? nms : 1) + (size_t)nr + 32 * 33) * sizeof(double) + 32 * sizeof(int);
 }
 
 __device__ inline Lds carve(char* base, int nr, int nms) {
@@ -69,16 +75,17 @@ __device__ inline Lds carve(char* base, int nr, int nms) {
   L.g = d; d += 32;
   L.Mv = d; d += (nms > 0 ? nms : 1);
   L.q = d; d += nr;
-  L.kfe = d; d += nr;
-  L.kbe = d; d += nr;
-  L.Dm = d; d += nr;
-  L.dqT = d; d += nr;
-  L.row = d; d += 32 * 33;
+  L.lu = d; d += 32 * 33;
+  L.kfe = L.lu;
+  L.kbe = L.lu + nr;
+  L.Dm = L.lu + 2 * nr;
+  L.dqT = L.lu + 3 * nr;
   L.piv = reinterpret_cast<int*>(d);
   return L;
 }
 
 struct RhsOut {
+  bool ok;                  // T inside the tables (team-uniform)
   double F;                 // this lane's RHS component
   double xc, g, w, rho;     // clipped state, clip derivative, wdot_k, density
   double cp, h, dcp, dh;    // thermo of species k at T
@@ -181,13 +188,16 @@ __device__ __forceinline__ void reaction(const Prog& P, int r, const double* __r
   }
 }
 
-// Team right-hand side.  Returns false (team-uniform) when T leaves the tables.
-template <bool JAC>
-__device__ bool team_rhs(const Prog& P, int mode, double p, double x, const Lds& L, RhsOut& o) {
+// Team right-hand side; o.ok is false (team-uniform) when T leaves the tables.  With jac the
+// per-reaction values the Jacobian needs are staged in LDS as well.  Kept out of line: it is called
+// from five places of the integrator and inlining every copy would multiply its register demand.
+__device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, double x, const Lds& L, bool jac) {
+  RhsOut o;
+  o.ok = false;
   const int lane = threadIdx.x & (TEAM - 1);
   const int ns = P.ns;
   const double T = tbc(x, ns);
-  if (!(T >= (double)P.Th_min && T < (double)P.Th_max && T >= (double)P.Tk_min && T < (double)P.Tk_max)) return false;
+  if (!(T >= (double)P.Th_min && T < (double)P.Th_max && T >= (double)P.Tk_min && T < (double)P.Tk_max)) return o;
   const bool sp = lane < ns;
   const double Wk = sp ? P.W[lane] : 1.0;
   const double Rk = sp ? 8314.51 / Wk : 0.0;
@@ -205,9 +215,11 @@ __device__ bool team_rhs(const Prog& P, int mode, double p, double x, const Lds&
   }
   if (sp) L.c[lane] = roi / Wk;
   tsync();
+#pragma unroll 1
   for (int s = lane; s < P.nms; s += TEAM) {
     const int t0 = P.ms_ptr[s], t1 = P.ms_ptr[s + 1];
     double m = L.c[P.ms_sp[t0]] * P.ms_eff[t0];
+#pragma unroll 1
     for (int t = t0 + 1; t < t1; ++t) m = m + L.c[P.ms_sp[t]] * P.ms_eff[t];
     L.Mv[s] = m;
   }
@@ -216,13 +228,20 @@ __device__ bool team_rhs(const Prog& P, int mode, double p, double x, const Lds&
   const double Td = T - (double)T_i;
   const double* __restrict__ rk0 = P.tabk + (long)(T_i - P.Tk_min) * P.wk;
   const double* __restrict__ rk1 = rk0 + P.wk;
-  for (int r = lane; r < P.nr; r += TEAM) reaction<JAC>(P, r, rk0, rk1, Td, L);
+  if (jac) {
+#pragma unroll 1
+    for (int r = lane; r < P.nr; r += TEAM) reaction<true>(P, r, rk0, rk1, Td, L);
+  } else {
+#pragma unroll 1
+    for (int r = lane; r < P.nr; r += TEAM) reaction<false>(P, r, rk0, rk1, Td, L);
+  }
   tsync();
   double w = 0.0;
   if (sp) {
     const int t0 = P.wd_ptr[lane], t1 = P.wd_ptr[lane + 1];
     if (t1 > t0) {
       double acc = P.wd_coef[t0] * L.q[P.wd_rx[t0]];
+#pragma unroll 1
       for (int t = t0 + 1; t < t1; ++t) acc = acc + P.wd_coef[t] * L.q[P.wd_rx[t]];
       w = Wk * acc;
     }
@@ -251,22 +270,21 @@ __device__ bool team_rhs(const Prog& P, int mode, double p, double x, const Lds&
     o.F = sp ? w / o.rho : 0.0;
   }
   if (lane == ns) o.F = FT;
-  return true;
+  o.ok = true;
+  return o;
 }
 
-// Jacobian row of this lane (J[lane][0..31], zero beyond nz) after team_rhs<true>.
-__device__ void team_jac(const Prog& P, int mode, double x, const Lds& L, const RhsOut& o, double (&Jr)[TEAM]) {
+// Jacobian of the team's cell, left in the team's global slot Jg (row k = lane k, stride 33),
+// after team_rhs<true>.
+__device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, const Lds& L, const RhsOut o,
+                                                   double* __restrict__ Jg) {
   const int lane = threadIdx.x & (TEAM - 1);
   const int ns = P.ns, nz = ns + 1;
   const bool sp = lane < ns;
   const double T = tbc(x, ns);
   const double FT = tbc(o.F, ns);
-  double* myrow = L.row + lane * 33;
-#pragma unroll
-  for (int j = 0; j < TEAM; ++j) {
-    Jr[j] = 0.0;
-    myrow[j] = 0.0;
-  }
+  double* myrow = Jg + lane * 33;
+  for (int j = 0; j < TEAM; ++j) myrow[j] = 0.0;
   L.g[lane] = o.g;
   if (mode == 1) L.aux[lane] = o.xc;
   tsync();
@@ -274,6 +292,7 @@ __device__ void team_jac(const Prog& P, int mode, double x, const Lds& L, const 
   const double Wk = sp ? P.W[lane] : 1.0;
   if (sp) {
     const int t0 = P.wd_ptr[lane], t1 = P.wd_ptr[lane + 1];
+#pragma unroll 1
     for (int t = t0; t < t1; ++t) {
       const int r = P.wd_rx[t];
       const double coef = P.wd_coef[t];
@@ -297,92 +316,76 @@ __device__ void team_jac(const Prog& P, int mode, double x, const Lds& L, const 
       if (ms >= 0) {
         const double Dv = coef * L.Dm[r];
         const double* __restrict__ ef = P.effd + ms * TEAM;
-#pragma unroll
-        for (int j = 0; j < TEAM; ++j) Jr[j] += Dv * ef[j];
+        for (int j = 0; j < ns; ++j) myrow[j] += Dv * ef[j];
       }
       bT += coef * L.dqT[r];
     }
+    // species block A_kj = d wdot_k / d rho_j (unclipped), b_k = d wdot_k / dT at fixed rho
+    for (int j = 0; j < ns; ++j) myrow[j] *= Wk / P.W[j];
   }
-  tsync();
-  // species block A_kj = d wdot_k / d rho_j (unclipped), b_k = d wdot_k / dT at fixed rho
   const double bk = Wk * bT;
-#pragma unroll
-  for (int j = 0; j < TEAM; ++j) {
-    const double Wj = j < ns ? P.W[j] : 1.0;
-    Jr[j] = (j < ns && sp) ? (Jr[j] + myrow[j]) * Wk / Wj : 0.0;
-  }
   const double Rk = sp ? 8314.51 / Wk : 0.0;
   if (mode == 0) {
     // constant volume: J_kj = g_j A_kj, J_kT = b_k; T row from column sums of e_k J_kj
-#pragma unroll
-    for (int j = 0; j < TEAM; ++j) Jr[j] *= L.g[j];
-    if (sp) Jr[ns] = bk;
-    const double e = sp ? o.h - Rk * T : 0.0;
+    if (sp) {
+      for (int j = 0; j < ns; ++j) myrow[j] *= L.g[j];
+      myrow[ns] = bk;
+    }
     const double s_de = tsum(sp ? (o.dh - Rk) * o.w : 0.0);
     const double s_dcv = tsum(sp ? o.xc * o.dcp : 0.0);
-#pragma unroll
-    for (int j = 0; j < TEAM; ++j) myrow[j] = Jr[j];
-    L.aux[lane] = e;
+    L.aux[lane] = sp ? o.h - Rk * T : 0.0;
     tsync();
     if (lane < nz) {
       double cs = 0.0;
-      for (int k = 0; k < ns; ++k) cs += L.aux[k] * L.row[k * 33 + lane];
-      double jt;
-      if (lane < ns) jt = -(cs + FT * (o.cp - Rk) * o.g) / o.rho_cv;
-      else jt = -(s_de + cs + FT * s_dcv) / o.rho_cv;
-      L.row[ns * 33 + lane] = jt;
+      for (int k = 0; k < ns; ++k) cs += L.aux[k] * Jg[k * 33 + lane];
+      const double jt = lane < ns ? -(cs + FT * (o.cp - Rk) * o.g) / o.rho_cv : -(s_de + cs + FT * s_dcv) / o.rho_cv;
+      Jg[ns * 33 + lane] = jt;
     }
     tsync();
   } else {
     // constant pressure (Y, T): see oracle/flint_oracle.c jac_cp_cell for the formulas
-    double AY = 0.0;
-#pragma unroll
-    for (int j = 0; j < TEAM; ++j) AY += Jr[j] * (j < ns ? L.aux[j] : 0.0);
     const double Rm = o.Rmix, rho = o.rho;
-    const double dwdT = bk - rho * AY / T;
-#pragma unroll
-    for (int j = 0; j < TEAM; ++j) {
-      const double Rj = j < ns ? 8314.51 / P.W[j] : 0.0;
-      const double gj = L.g[j];
-      const double a = Jr[j];
-      myrow[j] = sp ? rho * gj * (a - AY * Rj / Rm) : 0.0;
-      Jr[j] = sp ? gj * (a - AY * Rj / Rm + (o.w / rho) * Rj / Rm) : 0.0;
-    }
+    double AY = 0.0;
     if (sp) {
-      Jr[ns] = dwdT / rho + o.w / (rho * T);
-      myrow[ns] = dwdT;
+      for (int j = 0; j < ns; ++j) AY += myrow[j] * L.aux[j];
+      // stage dwdot_k/dY_j and dwdot_k/dT for the enthalpy-weighted column sums
+      for (int j = 0; j < ns; ++j) myrow[j] = rho * L.g[j] * (myrow[j] - AY * (8314.51 / P.W[j]) / Rm);
+      myrow[ns] = bk - rho * AY / T;
     }
     const double s_dhw = tsum(sp ? o.dh * o.w : 0.0);
     const double sYdcp = tsum(sp ? o.xc * o.dcp : 0.0);
     tsync();
     L.aux[lane] = sp ? o.h : 0.0;
     tsync();
+    double jt = 0.0;
     if (lane < nz) {
       double ch = 0.0;
-      for (int k = 0; k < ns; ++k) ch += L.aux[k] * L.row[k * 33 + lane];
-      double jt;
-      if (lane < ns) jt = -ch / (rho * o.cpm) - FT * o.g * (o.cp / o.cpm - Rk / Rm);
-      else jt = -(s_dhw + ch) / (rho * o.cpm) - FT * (sYdcp / o.cpm - 1.0 / T);
-      L.row[ns * 33 + lane] = jt;
+      for (int k = 0; k < ns; ++k) ch += L.aux[k] * Jg[k * 33 + lane];
+      jt = lane < ns ? -ch / (rho * o.cpm) - FT * o.g * (o.cp / o.cpm - Rk / Rm)
+                     : -(s_dhw + ch) / (rho * o.cpm) - FT * (sYdcp / o.cpm - 1.0 / T);
     }
     tsync();
+    if (sp) {
+      for (int j = 0; j < ns; ++j) myrow[j] = myrow[j] / rho + L.g[j] * (o.w / rho) * (8314.51 / P.W[j]) / Rm;
+      myrow[ns] = myrow[ns] / rho + o.w / (rho * T);
+    }
+    if (lane < nz) Jg[ns * 33 + lane] = jt;
+    tsync();
   }
-  if (lane == ns) {
-#pragma unroll
-    for (int j = 0; j < TEAM; ++j) Jr[j] = j < nz ? L.row[ns * 33 + j] : 0.0;
-  }
-  tsync();
 }
 
-// LU of I - c J with partial pivoting, one row per lane; returns this lane's elimination step.
-__device__ int team_lu(const double (&Jr)[TEAM], double c, int nz, double (&a)[TEAM], const Lds& L) {
+// LU of I - c J with partial pivoting in LDS, one row per lane (stride 33, conflict-free); the
+// pivot row is read by every lane as an LDS broadcast.  Returns this lane's elimination step.
+__device__ __forceinline__ int team_lu(double c, int nz, const Lds& L, const double* __restrict__ Jg) {
   const int lane = threadIdx.x & (TEAM - 1);
-#pragma unroll
-  for (int j = 0; j < TEAM; ++j) a[j] = (lane == j ? 1.0 : 0.0) - (lane < nz ? c * Jr[j] : 0.0);
+  double* arow = L.lu + lane * 33;
+  const double* __restrict__ jrow = Jg + lane * 33;
+  for (int j = 0; j < TEAM; ++j) arow[j] = (lane == j ? 1.0 : 0.0) - ((lane < nz && j < nz) ? c * jrow[j] : 0.0);
   int mystep = -1;
-#pragma unroll
+  tsync();
+#pragma unroll 1
   for (int k = 0; k < TEAM; ++k) {
-    double best = mystep < 0 ? fabs(a[k]) : -1.0;
+    double best = mystep < 0 ? fabs(arow[k]) : -1.0;
     int bl = lane;
 #pragma unroll
     for (int m = 16; m >= 1; m >>= 1) {
@@ -392,34 +395,32 @@ __device__ int team_lu(const double (&Jr)[TEAM], double c, int nz, double (&a)[T
     }
     if (lane == bl) mystep = k;
     if (lane == 0) L.piv[k] = bl;
-    const double pv = tbc(a[k], bl);
-    const bool elim = mystep < 0;
-    const double l = a[k] / pv;
-    if (elim) a[k] = l;
-#pragma unroll
-    for (int j = k + 1; j < TEAM; ++j) {
-      const double pr = tbc(a[j], bl);
-      if (elim) a[j] -= l * pr;
+    const double* prow = L.lu + bl * 33;
+    if (mystep < 0) {
+      const double l = arow[k] / prow[k];
+      arow[k] = l;
+#pragma unroll 1
+      for (int j = k + 1; j < TEAM; ++j) arow[j] -= l * prow[j];
     }
+    tsync();
   }
-  tsync();
   return mystep;
 }
 
 // Solve (I - cJ) x = b with the factors above; b and x are this lane's component.
-__device__ double team_solve(const double (&a)[TEAM], int mystep, double b, const Lds& L) {
+__device__ __forceinline__ double team_solve(int mystep, double b, const Lds& L) {
   const int lane = threadIdx.x & (TEAM - 1);
-#pragma unroll
+  const double* arow = L.lu + lane * 33;
+#pragma unroll 1
   for (int k = 0; k < TEAM; ++k) {
     const double zk = tbc(b, L.piv[k]);
-    if (mystep > k) b -= a[k] * zk;
+    if (mystep > k) b -= arow[k] * zk;
   }
   double s = 0.0, x = 0.0;
-#pragma unroll
+#pragma unroll 1
   for (int k = TEAM - 1; k >= 0; --k) {
-    const int p = L.piv[k];
-    const double xk = tbc((b - s) / a[k], p);
-    if (mystep < k) s += a[k] * xk;
+    const double xk = tbc((b - s) / arow[k], L.piv[k]);
+    if (mystep < k) s += arow[k] * xk;
     if (lane == k) x = xk;
   }
   return x;
@@ -431,41 +432,46 @@ __device__ __forceinline__ double team_rms(double v, double scale, int nz) {
   return sqrt(tsum(q * q) / nz);
 }
 
-// D[0..order] <- (R U)^T D[0..order] on this lane's column (R, U from compute_R of the method)
-__device__ void change_D(double (&D)[MAXO + 3], int order, double factor) {
-  double R[MAXO + 1][MAXO + 1], U[MAXO + 1][MAXO + 1];
+// D[0..order] <- (R U)^T D[0..order] on this lane's column, with R(order, factor) and U = R(order, 1)
+// of the method: R[j][l] = prod_{m=1..j} (m - 1 - factor l)/m, so R[0][l] = 1 and R[j][0] = [j == 0].
+// Because RU[j][0] = [j == 0] and RU[0][i] = [i == 0], D[0] (the solution itself) is left as it is
+// and never enters the differences: v_l = sum_{j>=1} R[j][l] D[j], then D_i = sum_{l>=1} U[l][i] v_l
+// for i >= 1.  Mixing D[0] into v_l and relying on sum_l U[l][i] = 0 to remove it again would leave
+// an absolute error of eps |y| in every difference, which later step-size increases amplify by
+// factor^j into a visible loss of mass after a long run of step halvings.
+__device__ __forceinline__ void change_D(double (&D)[MAXO + 3], int order, double factor) {
+  double v[MAXO + 1];
 #pragma unroll
-  for (int j = 0; j <= MAXO; ++j) { R[0][j] = 1.0; U[0][j] = 1.0; }
+  for (int l = 1; l <= MAXO; ++l) {
+    double prod = 1.0, acc = 0.0;
 #pragma unroll
-  for (int i = 1; i <= MAXO; ++i)
-#pragma unroll
-    for (int j = 0; j <= MAXO; ++j) {
-      R[i][j] = R[i - 1][j] * (j == 0 ? 0.0 : (i - 1 - factor * j) / i);
-      U[i][j] = U[i - 1][j] * (j == 0 ? 0.0 : (i - 1 - 1.0 * j) / i);
+    for (int j = 1; j <= MAXO; ++j) {
+      prod *= (j - 1 - factor * l) / j;
+      if (j <= order) acc += prod * D[j];
     }
-  double nd[MAXO + 1];
-#pragma unroll
-  for (int i = 0; i <= MAXO; ++i) {
-    double s = 0.0;
-#pragma unroll
-    for (int j = 0; j <= MAXO; ++j) {
-      if (j > order) continue;
-      double ru = 0.0;  // RU[j][i] = sum_l R[j][l] U[l][i]
-#pragma unroll
-      for (int l = 0; l <= MAXO; ++l)
-        if (l <= order) ru += R[j][l] * U[l][i];
-      s += ru * D[j];
-    }
-    nd[i] = s;
+    v[l] = (l <= order) ? acc : 0.0;
   }
 #pragma unroll
-  for (int i = 0; i <= MAXO; ++i)
-    if (i <= order) D[i] = nd[i];
+  for (int i = 1; i <= MAXO; ++i) {
+    if (i > order) continue;
+    double acc = 0.0;
+#pragma unroll
+    for (int l = 1; l <= MAXO; ++l) {
+      // U[l][i] = prod_{m=1..l} (m - 1 - i)/m: a signed binomial coefficient, exact in double
+      double u = 1.0;
+#pragma unroll
+      for (int m = 1; m <= MAXO; ++m)
+        if (m <= l) u *= (double)(m - 1 - i) / m;
+      if (l <= order) acc += u * v[l];
+    }
+    D[i] = acc;
+  }
 }
 
 // One cell; mirrors bdf_cell of oracle/oracle_bdf.c.  Returns 0 ok, 1 step underflow, 2 too many
 // steps, 3 invalid state.
-__device__ int integrate(const Prog& P, const Opts& O, double p, double& y, int (&st)[8], const Lds& L) {
+__device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p, double& y, int (&st)[8], const Lds& L,
+                                          double* __restrict__ Jg) {
   const int lane = threadIdx.x & (TEAM - 1);
   const int nz = P.ns + 1;
   const double rtol = O.rtol, atol = O.atol, t_end = O.t_end;
@@ -475,14 +481,16 @@ __device__ int integrate(const Prog& P, const Opts& O, double p, double& y, int 
   for (int k = 1; k <= MAXO; ++k) gamma[k] = gamma[k - 1] + 1.0 / k;
   const double newton_tol = fmax(10.0 * 2.220446049250313e-16 / rtol, fmin(0.03, sqrt(rtol)));
   RhsOut o;
-  if (!team_rhs<false>(P, O.mode, p, y, L, o)) return 3;
+  o = team_rhs(P, O.mode, p, y, L, false);
+  if (!o.ok) return 3;
   st[ST_RHS]++;
   const double f0 = o.F;
   double sc = atol + fabs(y) * rtol;
   const double d0 = team_rms(y, sc, nz), d1 = team_rms(f0, sc, nz);
   double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
   h0 = fmin(h0, t_end);
-  if (!team_rhs<false>(P, O.mode, p, y + h0 * f0, L, o)) return 3;
+  o = team_rhs(P, O.mode, p, y + h0 * f0, L, false);
+  if (!o.ok) return 3;
   st[ST_RHS]++;
   const double d2 = team_rms(o.F - f0, sc, nz) / h0;
   const double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? fmax(1e-6, h0 * 1e-3) : sqrt(0.01 / fmax(d1, d2));
@@ -495,11 +503,11 @@ __device__ int integrate(const Prog& P, const Opts& O, double p, double& y, int 
   D[1] = f0 * h_abs;
   int order = 1, n_equal = 0;
   bool have_lu = false, current_jac = false;
-  double Jr[TEAM], a[TEAM];
   int mystep = lane;
   // Jacobian at the initial state
-  if (!team_rhs<true>(P, O.mode, p, y, L, o)) return 3;
-  team_jac(P, O.mode, y, L, o, Jr);
+  o = team_rhs(P, O.mode, p, y, L, true);
+  if (!o.ok) return 3;
+  team_jac(P, O.mode, y, L, o, Jg);
   st[ST_JAC]++;
   double t = 0.0;
 
@@ -542,7 +550,7 @@ __device__ int integrate(const Prog& P, const Opts& O, double p, double& y, int 
       int n_iter = 0;
       while (!converged) {
         if (!have_lu) {
-          mystep = team_lu(Jr, cc, nz, a, L);
+          mystep = team_lu(cc, nz, L, Jg);
           st[ST_LU]++;
           have_lu = true;
         }
@@ -551,10 +559,11 @@ __device__ int integrate(const Prog& P, const Opts& O, double p, double& y, int 
         double dy_norm_old = -1.0;
         int k;
         for (k = 0; k < NEWTON_MAXITER; ++k) {
-          const bool ok = team_rhs<false>(P, O.mode, p, ynew, L, o);
+          o = team_rhs(P, O.mode, p, ynew, L, false);
+          const bool ok = o.ok;
           st[ST_RHS]++;
           if (!ok || !isfinite(tsum(o.F))) break;
-          const double dy = team_solve(a, mystep, cc * o.F - psi - d, L);
+          const double dy = team_solve(mystep, cc * o.F - psi - d, L);
           st[ST_NEWTON]++;
           const double dy_norm = team_rms(dy, scale, nz);
           const double rate = dy_norm_old >= 0.0 ? dy_norm / dy_norm_old : -1.0;
@@ -571,8 +580,9 @@ __device__ int integrate(const Prog& P, const Opts& O, double p, double& y, int 
         n_iter = k + 1;
         if (!converged) {
           if (current_jac) break;
-          if (!team_rhs<true>(P, O.mode, p, ypred, L, o)) break;
-          team_jac(P, O.mode, ypred, L, o, Jr);
+          o = team_rhs(P, O.mode, p, ypred, L, true);
+          if (!o.ok) break;
+          team_jac(P, O.mode, ypred, L, o, Jg);
           st[ST_JAC]++;
           have_lu = false;
           current_jac = true;
@@ -655,11 +665,12 @@ __global__ void zero_counter(int* c) {
 __global__ __launch_bounds__(256) void bdf_kernel(Prog P, Opts O, long n, long ld, double* __restrict__ X,
                                                   const double* __restrict__ pres, int* __restrict__ stats,
                                                   int* __restrict__ status, int* __restrict__ counter,
-                                                  unsigned team_bytes) {
+                                                  unsigned team_bytes, double* __restrict__ jac_scratch) {
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x & (TEAM - 1);
   const int team = threadIdx.x / TEAM;
   const Lds L = carve(smem + (size_t)team * team_bytes, P.nr, P.nms);
+  double* __restrict__ Jg = jac_scratch + ((size_t)blockIdx.x * (blockDim.x / TEAM) + team) * (TEAM * 33);
   const int nz = P.ns + 1;
   while (true) {
     int cell = 0;
@@ -669,7 +680,7 @@ __global__ __launch_bounds__(256) void bdf_kernel(Prog P, Opts O, long n, long l
     double y = lane < nz ? X[(long)lane * ld + cell] : 0.0;
     const double p = O.mode == 1 ? pres[cell] : 0.0;
     int st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    const int rc = integrate(P, O, p, y, st, L);
+    const int rc = integrate(P, O, p, y, st, L, Jg);
     if (lane < nz) X[(long)lane * ld + cell] = y;
     if (stats && lane < 8) {
       int v = st[0];
@@ -682,10 +693,47 @@ __global__ __launch_bounds__(256) void bdf_kernel(Prog P, Opts O, long n, long l
   }
 }
 
+// The reactor's right-hand side and Jacobian at given states (the same team_rhs/team_jac the
+// integrator uses), one team per cell over a grid-stride loop.  Outside the tables F = -1, J = 0.
+__global__ __launch_bounds__(256) void jac_kernel(Prog P, int mode, long n, long ld, const double* __restrict__ X,
+                                                  const double* __restrict__ pres, double* __restrict__ F,
+                                                  double* __restrict__ J, unsigned team_bytes,
+                                                  double* __restrict__ jac_scratch) {
+  extern __shared__ __align__(16) char smem[];
+  const int lane = threadIdx.x & (TEAM - 1);
+  const int team = threadIdx.x / TEAM;
+  const int tpb = blockDim.x / TEAM;
+  const Lds L = carve(smem + (size_t)team * team_bytes, P.nr, P.nms);
+  double* __restrict__ Jg = jac_scratch + ((size_t)blockIdx.x * tpb + team) * (TEAM * 33);
+  const int nz = P.ns + 1;
+  const long stride = (long)gridDim.x * tpb;
+  for (long cell = (long)blockIdx.x * tpb + team; cell < n; cell += stride) {
+    const double y = lane < nz ? X[(long)lane * ld + cell] : 0.0;
+    const double p = mode == 1 ? pres[cell] : 0.0;
+    const RhsOut o = team_rhs(P, mode, p, y, L, true);
+    if (!o.ok) {
+      if (lane < nz) {
+        F[(long)lane * ld + cell] = -1.0;
+        for (int c = 0; c < nz; ++c) J[((long)lane * nz + c) * ld + cell] = 0.0;
+      }
+      continue;
+    }
+    team_jac(P, mode, y, L, o, Jg);
+    if (lane < nz) {
+      F[(long)lane * ld + cell] = o.F;
+      for (int c = 0; c < nz; ++c) J[((long)lane * nz + c) * ld + cell] = Jg[lane * 33 + c];
+    }
+    tsync();
+  }
+}
+
 }  // namespace flint_reactor
 
 hipError_t flint_reactor_set_max_lds(int bytes) {
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::bdf_kernel),
+  const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::bdf_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e != hipSuccess) return e;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::jac_kernel),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
@@ -693,10 +741,19 @@ size_t flint_reactor_team_bytes(int nr, int nms) { return flint_reactor::team_ld
 
 hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reactor::Opts& O, long n, long ld,
                                 double* X, const double* pres, int* stats, int* status, int* counter, int teams_per_block,
-                                int blocks, hipStream_t s) {
+                                int blocks, double* jac_scratch, hipStream_t s) {
   const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P.nr, P.nms) + 15) & ~size_t(15));
   hipLaunchKernelGGL(flint_reactor::zero_counter, dim3(1), dim3(64), 0, s, counter);
   hipLaunchKernelGGL(flint_reactor::bdf_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
-                     (size_t)tb * teams_per_block, s, P, O, n, ld, X, pres, stats, status, counter, tb);
+                     (size_t)tb * teams_per_block, s, P, O, n, ld, X, pres, stats, status, counter, tb, jac_scratch);
+  return hipGetLastError();
+}
+
+hipError_t flint_reactor_jac_launch(const flint_reactor::Prog& P, int mode, long n, long ld, const double* X,
+                                    const double* pres, double* F, double* J, int teams_per_block, int blocks,
+                                    double* jac_scratch, hipStream_t s) {
+  const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P.nr, P.nms) + 15) & ~size_t(15));
+  hipLaunchKernelGGL(flint_reactor::jac_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
+                     (size_t)tb * teams_per_block, s, P, mode, n, ld, X, pres, F, J, tb, jac_scratch);
   return hipGetLastError();
 }

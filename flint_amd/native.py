@@ -39,6 +39,7 @@ def lib():
         L.flint_rhs.argtypes = [vp, l, l, vp, vp, vp, vp, vp]
         L.flint_reactor_advance.argtypes = [vp, l, l, i, vp, vp, d, d, d, i, vp, vp, vp]
         L.flint_reactor_advance_graph.argtypes = [vp, l, l, i, vp, vp, d, d, d, i, vp, vp, vp]
+        L.flint_reactor_jacobian.argtypes = [vp, l, l, i, vp, vp, vp, vp, vp]
         L.flint_reactor_autotune.argtypes = [vp, l, l, i, vp, vp, d, d, d, C.POINTER(d)]
         L.flint_reactor_set_launch.argtypes = [vp, i, i]
         L.flint_reactor_get_launch.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
@@ -150,6 +151,12 @@ class Mechanism:
         fn = lib().flint_reactor_advance_graph if graph else lib().flint_reactor_advance
         check(fn(self._h, n, ld, mode, _p(X), _p(p), t_end, rtol, atol, max_steps, _p(stats), _p(status),
                  _stream(stream)))
+
+    def reactor_jacobian(self, X, F, J, p=None, mode=0, n=None, stream=None):
+        """F [ns+1, ld] and J [(ns+1)^2, ld] of the reactor model at X [ns+1, ld] (torch, device)."""
+        ld = X.shape[-1]
+        n = ld if n is None else n
+        check(lib().flint_reactor_jacobian(self._h, n, ld, mode, _p(X), _p(p), _p(F), _p(J), _stream(stream)))
 
     def autotune(self, X, p=None, mode=0, t_end=1e-4, rtol=1e-6, atol=1e-12):
         ld = X.shape[-1]

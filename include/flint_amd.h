@@ -93,9 +93,17 @@ int flint_rhs_host(flint_mech *m, long n, const double *Z, double *F, double *J)
  * X_dev is advanced in place.  stats_dev[8][ld] (may be NULL) receives per cell: accepted steps,
  * rejected steps, Newton failures, RHS evaluations, Jacobians, LU factorisations, Newton
  * iterations, final order.  status_dev[n] (may be NULL): 0 ok, 1 step underflow, 2 max_steps
- * reached, 3 T outside the tables.  max_steps <= 0 means 100000.  Needs ns + 1 <= 32. */
+ * reached, 3 T outside the tables.  max_steps <= 0 means 20000.  Needs ns + 1 <= 32. */
 int flint_reactor_advance(flint_mech *m, long n, long ld, int mode, double *X_dev, const double *p_dev, double t_end,
                           double rtol, double atol, int max_steps, int *stats_dev, int *status_dev, void *stream);
+
+/* Right-hand side and analytic Jacobian of the reactor model that flint_reactor_advance integrates
+ * (mode 0: rhs_native and d/d(rho_k, T); mode 1: d(Y_k, T)/dt at fixed p_dev and d/d(Y_k, T)), at the
+ * states X_dev[(ns+1)][ld]: F_dev[(ns+1)][ld], J_dev[((ns+1)*r + c)][ld] = dF_r/dX_c.  F = -1 and
+ * J = 0 for cells outside the tables.  No reference counterpart (FLINT hands rhs_native to its
+ * external integrator, test-batchF.f90:81,105, which differentiates it itself). */
+int flint_reactor_jacobian(flint_mech *m, long n, long ld, int mode, const double *X_dev, const double *p_dev,
+                           double *F_dev, double *J_dev, void *stream);
 
 /* As flint_reactor_advance, launched through a hipGraph that is captured on the first call and
  * replayed while the arguments stay identical. */

@@ -141,7 +141,7 @@ class Oracle:
         self.L.orc_jac_cp_cstep(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(X), _ptr(p), C.c_double(h), _ptr(J))
         return J
 
-    def bdf(self, X, p=None, mode=0, rtol=1e-6, atol=1e-12, t_end=1e-4, max_steps=100000):
+    def bdf(self, X, p=None, mode=0, rtol=1e-6, atol=1e-12, t_end=1e-4, max_steps=20000):
         """Advance reactors (X [nz, n], mode 0 = (rho, T) constant volume, 1 = (Y, T) constant p)."""
         X = np.array(X, dtype=np.float64, order="C", copy=True)
         nz, n = X.shape
