@@ -662,7 +662,10 @@ __global__ void zero_counter(int* c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = 0;
 }
 
-__global__ __launch_bounds__(256) void bdf_kernel(Prog P, Opts O, long n, long ld, double* __restrict__ X,
+// Two waves per SIMD: the per-team LDS (LU factors) allows about seven resident waves per CU, and
+// at two waves the register budget costs only a small spill (measured faster than one wave at 400
+// registers, and than three waves).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void bdf_kernel(Prog P, Opts O, long n, long ld, double* __restrict__ X,
                                                   const double* __restrict__ pres, int* __restrict__ stats,
                                                   int* __restrict__ status, int* __restrict__ counter,
                                                   unsigned team_bytes, double* __restrict__ jac_scratch) {
