@@ -9,9 +9,10 @@
 // reference checkout.
 //
 // Layout of one team (lanes l = 0..31 of a half wavefront):
-//   lane k < ns owns species k, lane ns owns T: the state component, the 8-row BDF difference
-//   column D[.][k], row k of the Jacobian and row k of the LU factors of I - cJ live in VGPRs;
-//   partial pivoting picks rows by a team argmax and broadcasts the pivot row with ds_bpermute.
+//   lane k < ns owns species k, lane ns owns T: the state component and the 8-row BDF difference
+//   column D[.][k] live in VGPRs; row k of the Jacobian sits in a per-team global slot and row k
+//   of the LU factors of I - cJ in LDS (stride 33, conflict-free); partial pivoting picks rows by
+//   a team argmax and every lane reads the pivot row as an LDS broadcast.
 //   The right-hand side is evaluated reaction-parallel (reaction r on lane r mod 32) from the
 //   kinetics table row of the cell's temperature (coalesced over the team), with the
 //   net production rate of species k summed by lane k in reaction order (the summation order of

@@ -101,6 +101,14 @@ def radical_states(species, W, seed, start, count, T_lo=900.0, T_hi=2800.0, igni
     return dict(rho=_to_rho(Y, W, T, p), T=T, p=p, Y=Y)
 
 
+def cea_states(species, W, seed, start, count):
+    """Seeded states of the batched HP-equilibrium workload: the composition and pressure of
+    :func:`radical_states` with the unburnt temperature T0~U[300,900] K in place of T.  Returns Y
+    (the solver's input amounts, summing to 1), T0 and p in bar (the solver's pressure unit)."""
+    st = radical_states(species, W, seed, start, count, T_lo=300.0, T_hi=900.0)
+    return dict(Y=st["Y"], T0=st["T"], p_bar=st["p"] / 1e5)
+
+
 def soa(a):
     """(n, k) -> contiguous (k, n) structure-of-arrays."""
     return np.ascontiguousarray(np.asarray(a).T)

@@ -67,6 +67,22 @@ void orc_jac_cell(const orc_mech *m, const double *nu, int mode, const double *x
 void orc_bdf(const orc_mech *m, long n, long ld, int mode, double *X, const double *p, double rtol, double atol,
              double t_end, int max_steps, int *stats, int *status);
 
+/* CEA-style equilibrium (oracle_cea.c).  A[ne][ns] = species_composition (element rows),
+ * elw[ne] = atomic weights of the elements. */
+typedef struct orc_cea_data {
+  int ne;
+  const double *A;
+  const double *elw;
+} orc_cea_data;
+/* one state: hp = 1 HP (fixed h, p in bar), 0 UV (fixed e, v); rho_in[ns] = amounts (kg or kg/m^3);
+ * guess = NULL or [T, ln n_1..ln n_ns]; returns status (0 ok, 1 no convergence, 2 unsupported
+ * branch, 3 T outside the table) */
+int orc_cea_cell(const orc_mech *m, const orc_cea_data *c, int hp, double p_in, double T_in, const double *rho_in,
+                 const double *guess, double *T_out, double *Y_out, int *iters);
+/* batch, SoA: rho[k*ld+i], guess[k*ld+i] (k = 0 is T), Y_out[k*ld+i] */
+void orc_cea(const orc_mech *m, const orc_cea_data *c, long n, long ld, int hp, const double *p, const double *T,
+             const double *rho, const double *guess, double *T_out, double *Y_out, int *iters, int *status);
+
 int orc_num_threads(void);
 
 #ifdef __cplusplus

@@ -34,6 +34,10 @@ class _OrcMech(C.Structure):
                 ("Th_min", C.c_int), ("Th_max", C.c_int), ("cp", _P), ("h", _P), ("s", _P), ("is_wd", C.c_int)]
 
 
+class _OrcCea(C.Structure):
+    _fields_ = [("ne", C.c_int), ("A", _P), ("elw", _P)]
+
+
 def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
@@ -140,6 +144,30 @@ class Oracle:
         J = np.empty((nz * nz, n))
         self.L.orc_jac_cp_cstep(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(X), _ptr(p), C.c_double(h), _ptr(J))
         return J
+
+    def cea(self, p, T, rho, hp=True, guess=None):
+        """Equilibrium of n states (orc_cea): p [n] (bar for HP), T [n] (the input mixture's
+        temperature, also the first iterate), rho [ns, n] amounts; guess [ns+1, n] = (T, ln n_k) or
+        None.  Returns (T_eq [n], Y_eq [ns, n], iterations [n], status [n])."""
+        from flint_amd.elements import element_weights
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        ns, n = rho.shape
+        if not hasattr(self, "_cea"):
+            A = np.ascontiguousarray(self.md.composition, dtype=np.float64)
+            elw = element_weights(self.md.elements)
+            c = _OrcCea()
+            c.ne, c.A, c.elw = A.shape[0], _ptr(A), _ptr(elw)
+            self._cea = (c, A, elw)
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+        To = np.empty(n)
+        Yo = np.empty((ns, n))
+        it = np.empty(n, dtype=np.int32)
+        st = np.empty(n, dtype=np.int32)
+        self.L.orc_cea(C.byref(self.m), C.byref(self._cea[0]), C.c_long(n), C.c_long(n), C.c_int(int(hp)), _ptr(p),
+                       _ptr(T), _ptr(rho), None if g is None else _ptr(g), _ptr(To), _ptr(Yo), _ptr(it), _ptr(st))
+        return To, Yo, it, st
 
     def bdf(self, X, p=None, mode=0, rtol=1e-6, atol=1e-12, t_end=1e-4, max_steps=20000):
         """Advance reactors (X [nz, n], mode 0 = (rho, T) constant volume, 1 = (Y, T) constant p)."""

@@ -22,3 +22,14 @@ def test_oracle_equals_reference_build(oracle_mod, mechs, mech, seed):
     w, sc = oracle_mod.Oracle(md).wdot(rho, st["T"])
     wr = oracle_mod.Reference(md).wdot(rho, st["T"])
     assert normwise(w, wr, sc) <= 1e-14
+
+
+@pytest.mark.parametrize("hp", [True, False])
+def test_cea_oracle_equals_reference_build(oracle_mod, mechs, hp):
+    md = mechs["TSR-GP-24"]
+    st = S.cea_states(md.species, md.W, 104, 0, 2000)
+    Y = S.soa(st["Y"])
+    T, Yo, it, status = oracle_mod.Oracle(md).cea(st["p_bar"], st["T0"], Y, hp=hp)
+    assert np.all(status == 0)
+    Tr, Yr = oracle_mod.Reference(md).cea(st["p_bar"], st["T0"], Y, hp=hp)
+    assert np.array_equal(T, Tr) and np.array_equal(Yo, Yr)
