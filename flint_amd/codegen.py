@@ -415,9 +415,29 @@ def emit(prog):
     a(f"const flint_host_prog host_prog = {{{ns}, {len(prog.reactions)}, {len(prog.m_sets)}, {len(wd_rx)}, "
       f"{len(ms_sp)}, p_rx_kind, p_rx_col, p_rx_mset, p_rx_fac, p_wd_ptr, p_wd_rx, p_wd_coef, p_ms_ptr, p_ms_sp, "
       f"p_ms_eff}};")
+    # element matrix of the equilibrium solver (species_composition, Lib_CEA_setup.f90:49-52) as a
+    # compile-time table, so the per-species element sums fold at compile time
+    comp = prog.desc.get("composition") or []
+    if comp:
+        ne = len(comp)
+        rows = ", ".join("{" + ", ".join(repr(float(v)) for v in row) + "}" for row in comp)
+        a("// element matrix for the equilibrium kernel: a(e, j) = atoms of element e in species j")
+        a("struct Cea {")
+        a(f"  static constexpr int NS = {ns}, NE = {ne};")
+        a("  __host__ __device__ static constexpr double a(int e, int j) {")
+        a(f"    constexpr double t[NE][NS] = {{{rows}}};")
+        a("    return t[e][j];")
+        a("  }")
+        a("};")
+        flat = ", ".join(repr(float(v)) for row in comp for v in row)
+        a(f"static const double cea_A[] = {{{flat}}};")
     a(f"}}  // namespace flint_gen_{mid}")
     a("")
-    a(f"FLINT_REGISTER_MECH({mid}, flint_gen_{mid})")
+    if comp:
+        a('#include "../flint_cea_kernel.h"')
+        a(f"FLINT_REGISTER_MECH_CEA({mid}, flint_gen_{mid})")
+    else:
+        a(f"FLINT_REGISTER_MECH({mid}, flint_gen_{mid})")
     return "\n".join(L) + "\n"
 
 

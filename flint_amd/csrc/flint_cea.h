@@ -1,0 +1,16 @@
+// Device data of the batched equilibrium kernel (flint_cea_kernel.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace flint_cea {
+
+struct CeaProg {
+  const double* tabh;  // [nT+1][ns][2] (cp, h) per mass, the mechanism's thermo table
+  const double* tabs;  // [nT+1][ns] s per mass
+  const double* W;     // [ns] molecular weights (wm_tab)
+  const double* rm;    // [ns] species weights from the elements' atomic weights
+  int Th_min, Th_max;
+};
+
+}  // namespace flint_cea
+

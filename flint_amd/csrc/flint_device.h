@@ -54,6 +54,42 @@ struct NullSink {
                          st, tabk, Tk_min, Tk_max, tabh, Th_min, Th_max);                               \
     return hipGetLastError();                                                                           \
   }                                                                                                     \
-  const flint_mech_ops ID##_ops = {#ID, NSPACE::NS, NSPACE::WK, ID##_wdot, ID##_rhs_jac, &NSPACE::host_prog}; \
+  const flint_mech_ops ID##_ops = {#ID,         NSPACE::NS,          NSPACE::WK, ID##_wdot, ID##_rhs_jac, \
+                                   &NSPACE::host_prog, nullptr, 0,          nullptr};                        \
+  const int ID##_reg = flint_register_mech(&ID##_ops);                                                  \
+  }
+
+// As FLINT_REGISTER_MECH, for a mechanism that also has an equilibrium kernel (NSPACE::Cea).
+#define FLINT_REGISTER_MECH_CEA(ID, NSPACE)                                                             \
+  namespace {                                                                                           \
+  hipError_t ID##_wdot(long n, long ld, const double* rho, const double* T, double* w, int* st,         \
+                       const double* tabk, int Tk_min, int Tk_max, hipStream_t s) {                     \
+    if (n <= 0) return hipSuccess;                                                                      \
+    const unsigned blocks = (unsigned)((n + 255) / 256);                                                \
+    hipLaunchKernelGGL(NSPACE::wdot_kernel, dim3(blocks), dim3(256), 0, s, n, ld, rho, T, w, st, tabk,  \
+                       Tk_min, Tk_max);                                                                 \
+    return hipGetLastError();                                                                           \
+  }                                                                                                     \
+  hipError_t ID##_rhs_jac(long n, long ld, const double* Z, double* F, double* J, int* st,              \
+                          const double* tabk, int Tk_min, int Tk_max, const double* tabh, int Th_min,   \
+                          int Th_max, hipStream_t s) {                                                  \
+    if (n <= 0) return hipSuccess;                                                                      \
+    const int B = NSPACE::JAC_BLOCK;                                                                    \
+    const unsigned blocks = (unsigned)((n + B - 1) / B);                                                \
+    if (J)                                                                                              \
+      hipLaunchKernelGGL(NSPACE::rhs_jac_kernel<true>, dim3(blocks), dim3(B), 0, s, n, ld, Z, F, J,     \
+                         st, tabk, Tk_min, Tk_max, tabh, Th_min, Th_max);                               \
+    else                                                                                                \
+      hipLaunchKernelGGL(NSPACE::rhs_jac_kernel<false>, dim3(blocks), dim3(B), 0, s, n, ld, Z, F, J,    \
+                         st, tabk, Tk_min, Tk_max, tabh, Th_min, Th_max);                               \
+    return hipGetLastError();                                                                           \
+  }                                                                                                     \
+  hipError_t ID##_cea(const flint_cea::CeaProg& P, bool hp, long n, long ld, const double* p,           \
+                      const double* T, const double* rho, const double* guess, double* To, double* Yo,  \
+                      int* it, int* st, hipStream_t s) {                                                \
+    return flint_cea::launch<NSPACE::Cea>(P, hp, n, ld, p, T, rho, guess, To, Yo, it, st, s);           \
+  }                                                                                                     \
+  const flint_mech_ops ID##_ops = {#ID,           NSPACE::NS,       NSPACE::WK, ID##_wdot, ID##_rhs_jac, \
+                                   &NSPACE::host_prog, ID##_cea, NSPACE::Cea::NE, NSPACE::cea_A};        \
   const int ID##_reg = flint_register_mech(&ID##_ops);                                                  \
   }

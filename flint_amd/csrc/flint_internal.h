@@ -8,7 +8,8 @@
 
 #include "flint_registry.h"
 
-struct flint_extra;  // per-handle state of the reactor / equilibrium / MLP paths
+struct flint_extra;     // per-handle state of the reactor path
+struct flint_cea_data;  // per-handle state of the equilibrium path
 
 struct flint_mech {
   std::string name, id;
@@ -29,6 +30,7 @@ struct flint_mech {
   size_t stage_cap = 0;
   std::mutex mu;
   flint_extra* extra = nullptr;
+  flint_cea_data* cea = nullptr;
 };
 
 int flint_fail(int code, const std::string& msg);
@@ -36,3 +38,5 @@ std::string flint_mech_id(const std::string& name);
 int flint_mech_upload(flint_mech* m);
 int flint_stage(flint_mech* m, size_t doubles);
 void flint_mech_release_extra(flint_mech* m);
+void flint_mech_release_cea(flint_mech* m);
+double flint_atomic_weight(const char* symbol);

@@ -8,6 +8,7 @@
 // chemistry-Lindemann.dat (T, k_inf, k_0, k_c) one zone per falloff reaction in file order.
 // Only Tecplot ASCII (.dat) is read; FLINT's binary .szplt fallback needs TecIO and is not
 // supported.
+#include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -120,12 +121,16 @@ extern "C" int flint_mech_load(const char* folder, const char* prefix, flint_mec
   // composition.txt (optional, as in read_idealgas_composition)
   int ne = 0;
   std::vector<double> comp;
+  std::vector<std::string> elements;
   {
     std::ifstream cf(base + "composition.txt");
     if (cf) {
       cf >> ne;
       std::string tok;
-      for (int e = 0; e < ne; ++e) cf >> tok;
+      for (int e = 0; e < ne; ++e) {
+        cf >> tok;
+        elements.push_back(tok);
+      }
       cf >> tok;  // "composition"
       comp.assign((size_t)ne * ns, 0.0);
       for (int k = 0; k < ns; ++k)
@@ -141,16 +146,18 @@ extern "C" int flint_mech_load(const char* folder, const char* prefix, flint_mec
   const int start = Th_min;
   if (Th_min == 1) Th_min = 0;  // Load_ThermoTransport.f90:70,80-85
   const int nTh = Th_max - Th_min + 1;
-  std::vector<double> cp((size_t)ns * nTh), h((size_t)ns * nTh);
+  std::vector<double> cp((size_t)ns * nTh), h((size_t)ns * nTh), sv((size_t)ns * nTh);
   for (int k = 0; k < ns; ++k) {
     const auto& v = th.vars[k];
     for (int i = 0; i < th.npts; ++i) {
       cp[(size_t)k * nTh + (start - Th_min) + i] = v[(size_t)0 * th.npts + i];
       h[(size_t)k * nTh + (start - Th_min) + i] = v[(size_t)1 * th.npts + i];
+      sv[(size_t)k * nTh + (start - Th_min) + i] = v[(size_t)2 * th.npts + i];
     }
     if (start != Th_min) {
       cp[(size_t)k * nTh] = v[0];
       h[(size_t)k * nTh] = 0.0;
+      sv[(size_t)k * nTh] = v[(size_t)2 * th.npts];
     }
   }
   // chemistry-info.txt
@@ -207,7 +214,44 @@ extern "C" int flint_mech_load(const char* folder, const char* prefix, flint_mec
         for (int i = 0; i < nTk; ++i) lind[((size_t)v * nl + q) * nTk + i] = lz.vars[q][(size_t)v * nTk + i];
   }
   // species_composition(ne, ns) column-major == comp[k*ne + e]
-  return flint_mech_create(mech_name.c_str(), ns, W.data(), Tk_min, Tk_max, na, kf.data(), kb.data(), nt,
-                           nt ? troe.data() : nullptr, nl, nl ? lind.data() : nullptr, Th_min, Th_max, cp.data(),
-                           h.data(), ne, ne ? comp.data() : nullptr, out);
+  int rc = flint_mech_create(mech_name.c_str(), ns, W.data(), Tk_min, Tk_max, na, kf.data(), kb.data(), nt,
+                             nt ? troe.data() : nullptr, nl, nl ? lind.data() : nullptr, Th_min, Th_max, cp.data(),
+                             h.data(), ne, ne ? comp.data() : nullptr, out);
+  if (rc || ne == 0) return rc;
+  // equilibrium data (CEA_initialize_global), where the mechanism has an equilibrium kernel and
+  // every element has a standard atomic weight
+  std::vector<double> elw;
+  for (const auto& e : elements) {
+    const double w = flint_atomic_weight(e.c_str());
+    if (!(w > 0.0)) return FLINT_OK;
+    elw.push_back(w);
+  }
+  if ((*out)->ops && (*out)->ops->cea) {
+    if (int rc2 = flint_cea_setup(*out, sv.data(), elw.data())) {
+      flint_mech_destroy(*out);
+      *out = nullptr;
+      return rc2;
+    }
+  }
+  return FLINT_OK;
+}
+
+// Standard atomic weights (IUPAC 1999, Coplen, J. Phys. Chem. Ref. Data 30 (2001) 701-712): the
+// table the reference's equilibrium solver takes its element weights from by symbol
+// (Lib_CEA_data.f90:37-59, Lib_CEA_setup.f90:157-170).  0 for an unknown symbol.
+double flint_atomic_weight(const char* sym) {
+  static const struct {
+    const char* s;
+    double w;
+  } tab[] = {{"H", 1.00794},   {"D", 2.014102},    {"HE", 4.002602}, {"LI", 6.941},     {"BE", 9.012182},
+             {"B", 10.811},    {"C", 12.0107},     {"N", 14.0067},   {"O", 15.9994},    {"F", 18.9984032},
+             {"NE", 20.1797},  {"NA", 22.989770},  {"MG", 24.305},   {"AL", 26.981538}, {"SI", 28.0855},
+             {"P", 30.973761}, {"S", 32.065},      {"CL", 35.453},   {"AR", 39.948},    {"K", 39.0983},
+             {"CA", 40.078},   {"FE", 55.845},     {"KR", 83.80},    {"XE", 131.293}};
+  std::string u;
+  for (const char* c = sym; *c; ++c)
+    if (!std::isspace((unsigned char)*c)) u.push_back((char)std::toupper((unsigned char)*c));
+  for (const auto& t : tab)
+    if (u == t.s) return t.w;
+  return 0.0;
 }

@@ -5,6 +5,12 @@
 #include <hip/hip_runtime.h>
 
 struct flint_host_prog;
+namespace flint_cea {
+struct CeaProg;
+}
+typedef hipError_t (*flint_cea_fn)(const flint_cea::CeaProg& P, bool hp, long n, long ld, const double* p,
+                                   const double* T, const double* rho, const double* guess, double* Tout,
+                                   double* Yout, int* iters, int* status, hipStream_t s);
 
 struct flint_mech_ops {
   const char* id;
@@ -15,6 +21,9 @@ struct flint_mech_ops {
   hipError_t (*rhs_jac)(long n, long ld, const double* Z, double* F, double* J, int* status, const double* tabk,
                         int Tk_min, int Tk_max, const double* tabh, int Th_min, int Th_max, hipStream_t s);
   const flint_host_prog* prog;  // rate program for the reaction-parallel reactor kernel
+  flint_cea_fn cea;              // equilibrium kernel (null if the mechanism has no composition)
+  int cea_ne;                    // element count of the compiled element matrix
+  const double* cea_A;           // [ne][ns] compiled element matrix
 };
 
 int flint_register_mech(const flint_mech_ops* ops);

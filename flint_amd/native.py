@@ -43,6 +43,8 @@ def lib():
         L.flint_reactor_autotune.argtypes = [vp, l, l, i, vp, vp, d, d, d, C.POINTER(d)]
         L.flint_reactor_set_launch.argtypes = [vp, i, i]
         L.flint_reactor_get_launch.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
+        L.flint_cea_setup.argtypes = [vp, vp, vp]
+        L.flint_cea_solve.argtypes = [vp, l, l, i, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.flint_source_host.argtypes = [vp, l, vp, vp, vp]
         L.flint_rhs_host.argtypes = [vp, l, vp, vp, vp]
         _lib = L
@@ -101,7 +103,11 @@ class Mechanism:
         check(L.flint_mech_create(md.desc["name"].encode(), md.ns, _p(W), md.Tk_min, md.Tk_min + nTk - 1, md.n_arrh,
                                   _p(kf), _p(kb), md.n_troe, _p(troe), md.n_lind, _p(lind), md.Th_min, md.Th_max,
                                   _p(cp), _p(h), len(md.elements), _p(comp), C.byref(out)))
-        return cls(out, md.ns, md.desc["name"])
+        mech = cls(out, md.ns, md.desc["name"])
+        if md.composition.size:
+            from .elements import element_weights
+            check(L.flint_cea_setup(out, _p(_fortran(md.s)), _p(element_weights(md.elements))))
+        return mech
 
     @classmethod
     def load(cls, folder, prefix=""):
@@ -171,6 +177,15 @@ class Mechanism:
         a, b = C.c_int(), C.c_int()
         check(lib().flint_reactor_get_launch(self._h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    # ---- batched equilibrium ----
+    def equilibrium(self, T, rho, T_out, Y_out, p=None, hp=True, guess=None, iters=None, status=None, n=None,
+                    stream=None):
+        """CEA_solve over a batch (torch, device): T [ld], rho [ns, ld] -> T_out [ld], Y_out [ns, ld]."""
+        ld = T.shape[-1]
+        n = ld if n is None else n
+        check(lib().flint_cea_solve(self._h, n, ld, int(hp), _p(p), _p(T), _p(rho), _p(guess), _p(T_out), _p(Y_out),
+                                    _p(iters), _p(status), _stream(stream)))
 
     # ---- host-memory convenience (numpy, SoA [rows, n]) ----
     def source_host(self, rho, T):

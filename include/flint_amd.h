@@ -118,6 +118,30 @@ int flint_reactor_autotune(flint_mech *m, long n, long ld, int mode, const doubl
 int flint_reactor_set_launch(flint_mech *m, int teams_per_block, int blocks);
 int flint_reactor_get_launch(flint_mech *m, int *teams_per_block, int *blocks);
 
+/* Equilibrium data of a mechanism: CEA_initialize_global (Lib_CEA_setup.f90:6-99) for the
+ * mechanism's species set.  s = s_tab(Th_min:Th_max, ns) per-mass entropy at 1 bar, column-major
+ * like cp/h (read_idealgas_thermo, Load_ThermoTransport.f90:30-85); el_weight[ne] = atomic weights
+ * of the elements in species_composition's row order (the reference takes them from its
+ * Atmwt table by symbol, Lib_CEA_setup.f90:157-170).  flint_mech_load calls this itself. */
+int flint_cea_setup(flint_mech *m, const double *s, const double *el_weight);
+
+/* CEA_solve over a batch (Lib_CEA_solver.f90:6-856, with CEA_initialize_local per state):
+ *   hp = 1  HP: fixed pressure p_dev[n] (bar) and the enthalpy of the input mixture at T_dev[n]
+ *           (the reference with Vol = .false.);
+ *   hp = 0  UV: fixed internal energy at T_dev[n] and volume 1e5/sum(rho) (Vol = .true., the
+ *           reference's default; p_dev is ignored and may be NULL).
+ * rho_dev[ns][ld] = input amounts (mass fractions, or densities: the reference normalises the
+ * element amounts by their sum but not the energy, so HP inputs should sum to 1).  Outputs
+ * T_out_dev[n], Y_out_dev[ns][ld] = n_k W_k (kg/kg), iters_dev[n] Newton iterations and
+ * status_dev[n]: 0 converged, 1 no convergence in 50 iterations, 2 singular Newton matrix after
+ * the reference's retries, 3 temperature outside the thermo table (T_out and Y_out are NaN
+ * unless 0).  guess_dev[ns+1][ld] (may be NULL) = (T, ln n_1..ln n_ns) starts the Newton
+ * iteration there instead of at the reference's default (T_in, n_j = 0.1/ns).  iters_dev and
+ * status_dev may be NULL.  Needs a generated mechanism (its element matrix is compiled in). */
+int flint_cea_solve(flint_mech *m, long n, long ld, int hp, const double *p_dev, const double *T_dev,
+                    const double *rho_dev, const double *guess_dev, double *T_out_dev, double *Y_out_dev,
+                    int *iters_dev, int *status_dev, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
