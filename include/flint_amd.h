@@ -142,6 +142,26 @@ int flint_cea_solve(flint_mech *m, long n, long ld, int hp, const double *p_dev,
                     const double *rho_dev, const double *guess_dev, double *T_out_dev, double *Y_out_dev,
                     int *iters_dev, int *status_dev, void *stream);
 
+/* Multilayer perceptron on MFMA (no reference counterpart: the learned reactor step and the
+ * equilibrium warm start of this library).  Layers d_0 -> d_1 -> ... -> d_L with GELU on every
+ * hidden layer and a linear output; BF16 weights and activations, FP32 accumulation, FP32 master
+ * weights.  Parameters are a flat FP32 vector, per layer W_l [d_l][d_{l+1}] row-major then b_l.
+ * Batches are row-major FP32 device arrays X [n][d_0], targets/outputs [n][d_L]. */
+typedef struct flint_mlp flint_mlp;
+int flint_mlp_create(int nlayers, const int *dims, unsigned long long seed, flint_mlp **out);
+void flint_mlp_destroy(flint_mlp *m);
+int flint_mlp_num_params(const flint_mlp *m, long *count);
+int flint_mlp_get_params(flint_mlp *m, float *host_params);        /* synchronous */
+int flint_mlp_set_params(flint_mlp *m, const float *host_params);  /* synchronous; resets Adam */
+int flint_mlp_forward(flint_mlp *m, long n, const float *X_dev, float *Y_dev, void *stream);
+/* gradient of the mean squared error mean((Y - T)^2) over the batch (grad_dev [nparams] and the
+ * loss_dev scalar may be NULL) */
+int flint_mlp_gradients(flint_mlp *m, long n, const float *X_dev, const float *T_dev, float *grad_dev,
+                        float *loss_dev, void *stream);
+/* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch's mean squared error */
+int flint_mlp_train_step(flint_mlp *m, long n, const float *X_dev, const float *T_dev, float lr, float *loss_dev,
+                         void *stream);
+
 #ifdef __cplusplus
 }
 #endif
