@@ -158,6 +158,7 @@ extern "C" void flint_mech_destroy(flint_mech* m) {
   if (!m) return;
   flint_mech_release_extra(m);
   flint_mech_release_cea(m);
+  flint_mech_release_sur(m);
   if (m->d_tabk) (void)hipFree(m->d_tabk);
   if (m->d_tabh) (void)hipFree(m->d_tabh);
   if (m->d_W) (void)hipFree(m->d_W);

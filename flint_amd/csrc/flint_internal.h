@@ -10,6 +10,7 @@
 
 struct flint_extra;     // per-handle state of the reactor path
 struct flint_cea_data;  // per-handle state of the equilibrium path
+struct flint_sur_data;  // per-handle buffers of the learned reactor step
 
 struct flint_mech {
   std::string name, id;
@@ -31,6 +32,7 @@ struct flint_mech {
   std::mutex mu;
   flint_extra* extra = nullptr;
   flint_cea_data* cea = nullptr;
+  flint_sur_data* sur = nullptr;
 };
 
 int flint_fail(int code, const std::string& msg);
@@ -39,4 +41,5 @@ int flint_mech_upload(flint_mech* m);
 int flint_stage(flint_mech* m, size_t doubles);
 void flint_mech_release_extra(flint_mech* m);
 void flint_mech_release_cea(flint_mech* m);
+void flint_mech_release_sur(flint_mech* m);
 double flint_atomic_weight(const char* symbol);

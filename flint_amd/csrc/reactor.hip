@@ -697,8 +697,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   }
 }
 
-// The reactor's right-hand side and Jacobian at given states (the same team_rhs/team_jac the
+// The reactor's right-hand side and (JAC) Jacobian at given states (the same team_rhs/team_jac the
 // integrator uses), one team per cell over a grid-stride loop.  Outside the tables F = -1, J = 0.
+template <bool JAC>
 __global__ __launch_bounds__(256) void jac_kernel(Prog P, int mode, long n, long ld, const double* __restrict__ X,
                                                   const double* __restrict__ pres, double* __restrict__ F,
                                                   double* __restrict__ J, unsigned team_bytes,
@@ -714,12 +715,18 @@ __global__ __launch_bounds__(256) void jac_kernel(Prog P, int mode, long n, long
   for (long cell = (long)blockIdx.x * tpb + team; cell < n; cell += stride) {
     const double y = lane < nz ? X[(long)lane * ld + cell] : 0.0;
     const double p = mode == 1 ? pres[cell] : 0.0;
-    const RhsOut o = team_rhs(P, mode, p, y, L, true);
+    const RhsOut o = team_rhs(P, mode, p, y, L, JAC);
     if (!o.ok) {
       if (lane < nz) {
         F[(long)lane * ld + cell] = -1.0;
-        for (int c = 0; c < nz; ++c) J[((long)lane * nz + c) * ld + cell] = 0.0;
+        if (JAC)
+          for (int c = 0; c < nz; ++c) J[((long)lane * nz + c) * ld + cell] = 0.0;
       }
+      continue;
+    }
+    if (!JAC) {
+      if (lane < nz) F[(long)lane * ld + cell] = o.F;
+      tsync();
       continue;
     }
     team_jac(P, mode, y, L, o, Jg);
@@ -737,7 +744,10 @@ hipError_t flint_reactor_set_max_lds(int bytes) {
   const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::bdf_kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::jac_kernel),
+  const hipError_t e2 = hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::jac_kernel<true>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e2 != hipSuccess) return e2;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::jac_kernel<false>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
@@ -757,7 +767,11 @@ hipError_t flint_reactor_jac_launch(const flint_reactor::Prog& P, int mode, long
                                     const double* pres, double* F, double* J, int teams_per_block, int blocks,
                                     double* jac_scratch, hipStream_t s) {
   const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P.nr, P.nms) + 15) & ~size_t(15));
-  hipLaunchKernelGGL(flint_reactor::jac_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
-                     (size_t)tb * teams_per_block, s, P, mode, n, ld, X, pres, F, J, tb, jac_scratch);
+  if (J)
+    hipLaunchKernelGGL(flint_reactor::jac_kernel<true>, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
+                       (size_t)tb * teams_per_block, s, P, mode, n, ld, X, pres, F, J, tb, jac_scratch);
+  else
+    hipLaunchKernelGGL(flint_reactor::jac_kernel<false>, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
+                       (size_t)tb * teams_per_block, s, P, mode, n, ld, X, pres, F, J, tb, jac_scratch);
   return hipGetLastError();
 }

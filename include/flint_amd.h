@@ -99,8 +99,8 @@ int flint_reactor_advance(flint_mech *m, long n, long ld, int mode, double *X_de
 
 /* Right-hand side and analytic Jacobian of the reactor model that flint_reactor_advance integrates
  * (mode 0: rhs_native and d/d(rho_k, T); mode 1: d(Y_k, T)/dt at fixed p_dev and d/d(Y_k, T)), at the
- * states X_dev[(ns+1)][ld]: F_dev[(ns+1)][ld], J_dev[((ns+1)*r + c)][ld] = dF_r/dX_c.  F = -1 and
- * J = 0 for cells outside the tables.  No reference counterpart (FLINT hands rhs_native to its
+ * states X_dev[(ns+1)][ld]: F_dev[(ns+1)][ld], J_dev[((ns+1)*r + c)][ld] = dF_r/dX_c (J_dev may be
+ * NULL: right-hand side only).  F = -1 and J = 0 for cells outside the tables.  No reference counterpart (FLINT hands rhs_native to its
  * external integrator, test-batchF.f90:81,105, which differentiates it itself). */
 int flint_reactor_jacobian(flint_mech *m, long n, long ld, int mode, const double *X_dev, const double *p_dev,
                            double *F_dev, double *J_dev, void *stream);
@@ -161,6 +161,27 @@ int flint_mlp_gradients(flint_mlp *m, long n, const float *X_dev, const float *T
 /* one Adam step (beta1 0.9, beta2 0.999, eps 1e-8) on the batch's mean squared error */
 int flint_mlp_train_step(flint_mlp *m, long n, const float *X_dev, const float *T_dev, float lr, float *loss_dev,
                          void *stream);
+
+/* Learned constant-pressure reactor step with the batched BDF as fallback (no reference
+ * counterpart).  X_dev = (Y_1..Y_ns, T) [ns+1][ld] and p_dev [n] (Pa) as in flint_reactor_advance
+ * mode 1; net maps the standardised features (log max(Y_k, ymin), T, log p) [ns+2] to the
+ * standardised increments over t_end in units of the local rate, o_k = dY_k / (t_end |dY_k/dt| +
+ * yfloor) and o_T = dT / (t_end |dT/dt| + tfloor) with dX/dt at the old state (the right-hand side
+ * of flint_reactor_jacobian mode 1) [ns+1].  scaling (host, 4 ns + 6 doubles) =
+ * in_mean[ns+2], in_scale[ns+2], out_mean[ns+1], out_scale[ns+1]; el_weight (host, [ne]) =
+ * atomic weights of the elements.  A cell falls back to flint_reactor_advance(mode 1, t_end,
+ * rtol, atol) when its raw prediction changes an element mass fraction or the total mass by more
+ * than max_residual or predicts |dT| > max_dT; an accepted prediction is projected onto the old
+ * element amounts and total mass (exactly, to round-off) and T solved from the old mixture
+ * enthalpy, and the cell still falls back when that T differs from the predicted T + dT by more
+ * than max_T_mismatch.  fallback_dev [ld] (may be NULL) receives 1 for the cells that took the BDF,
+ * *n_fallback their number, stats_dev/status_dev as flint_reactor_advance (zero for accepted
+ * cells).  Synchronises the stream once (to size the fallback launch).  Needs ns = 24 or 31. */
+int flint_reactor_advance_mlp(flint_mech *m, flint_mlp *net, long n, long ld, double *X_dev, const double *p_dev,
+                              double t_end, double rtol, double atol, const double *scaling, const double *el_weight,
+                              double ymin, double yfloor, double tfloor, double max_dT, double max_residual,
+                              double max_T_mismatch, int *fallback_dev, long *n_fallback, int *stats_dev,
+                              int *status_dev, void *stream);
 
 #ifdef __cplusplus
 }
