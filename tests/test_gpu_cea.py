@@ -110,3 +110,30 @@ def test_cea_gpu_rejects_bad_arguments(gp24, cuda):
         mech.equilibrium(T, Y, T.clone(), Y.clone(), p=None, hp=True)  # HP needs p
     with pytest.raises(FlintError):
         mech.equilibrium(T, Y, T.clone(), Y.clone(), p=T, hp=2)
+
+
+def test_cea_learned_warm_start(gp24, cuda):
+    """The trained warm-start network (flint_amd/data/cea_warmstart_gp24.npz) reaches the same
+    equilibria as the default guess, within the solver's own convergence tolerances, in fewer
+    Newton iterations."""
+    import torch
+    from flint_amd.warmstart import WarmStart, DEFAULT_MODEL
+    md, mech = gp24
+    ws = WarmStart.load(DEFAULT_MODEL)
+    n = 4096
+    st = S.cea_states(md.species, md.W, 23, 0, n)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    Y0, T0, p = dev(S.soa(st["Y"])), dev(st["T0"]), dev(st["p_bar"])
+    T1, Y1, T2, Y2 = torch.empty_like(T0), torch.empty_like(Y0), torch.empty_like(T0), torch.empty_like(Y0)
+    it1, s1 = torch.empty(n, dtype=torch.int32, device=cuda), torch.empty(n, dtype=torch.int32, device=cuda)
+    it2, s2 = torch.empty_like(it1), torch.empty_like(s1)
+    print("[gpu] cea learned warm start", flush=True)
+    mech.equilibrium(T0, Y0, T1, Y1, p=p, hp=True, iters=it1, status=s1)
+    ws.solve(mech, md, Y0, T0, p, T2, Y2, iters=it2, status=s2)
+    torch.cuda.synchronize()
+    assert int((s1 != 0).sum()) == 0 and int((s2 != 0).sum()) == 0
+    dT = (T2 - T1).abs().cpu().numpy()
+    dY = ((Y2 - Y1).abs() / Y1.abs().max(0).values).max(0).values.cpu().numpy()
+    assert dT.max() < 1e-2 and np.quantile(dT, 0.99) < 1e-4
+    assert dY.max() < 1e-5
+    assert it2.float().mean().item() < 0.5 * it1.float().mean().item()
