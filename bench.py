@@ -316,9 +316,14 @@ def main():
         out["config"].update({k: rx[k] for k in ("fails", "steps_mean", "steps_max", "rhs_mean", "jac_mean",
                                                  "lu_mean", "newton_mean", "launch", "wall")})
         out["rhs_evals_per_s"] = rx["rhs_per_s"]
+        traffic = None
+        tp = os.path.join(ROOT, "profiles", "bdf_kernel_rich31_262144.traffic.json")
+        if os.path.exists(tp) and args.cells == 1 << 18:
+            with open(tp) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
         out["roofline"] = {"bound": "mfma", "kernel": "flint_reactor::bdf_kernel",
                            "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
-                           "frac": achieved / PEAK_FP64, "traffic": None,
+                           "frac": achieved / PEAK_FP64, "traffic": traffic,
                            "note": "FP64 vector work (no MFMA): flops from the kernel's per-cell counters x "
                                    "8e3/RHS, 2e4/Jacobian, 2/3*32^3/LU, 2*32^2/solve; peak = MI355X FP64"}
         if "cpu" in rx:
