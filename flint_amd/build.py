@@ -36,6 +36,35 @@ def _compile(src):
     return obj
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
+
+
+def build_fortran(verbose=False):
+    """The iso_c_binding drop-in module (fortran/flint_amd.f90) and its check program, linked
+    against libflint_amd.so; skipped when no flang is installed."""
+    if not os.path.exists(FLANG):
+        if verbose:
+            print("flang not found: Fortran module not built")
+        return None
+    out = os.path.join(FORTRAN, "_build")
+    os.makedirs(out, exist_ok=True)
+    mod = os.path.join(FORTRAN, "flint_amd.f90")
+    obj = os.path.join(out, "flint_amd.o")
+    exe = os.path.join(out, "drop_in_check")
+    steps = [[FLANG, "-O2", "-fPIC", "-c", mod, "-module-dir", out, "-o", obj],
+             [FLANG, "-O2", os.path.join(FORTRAN, "drop_in_check.f90"), obj, "-module-dir", out, "-I", out,
+              "-L" + HERE, "-lflint_amd", "-Wl,-rpath,$ORIGIN/../..", "-o", exe + ".tmp"]]
+    for cmd in steps:
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=out)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(exe + ".tmp", exe)
+    if verbose:
+        print("built", exe)
+    return exe
+
+
 def build(verbose=False, jobs=8):
     from .codegen import generate
     generate()
@@ -51,6 +80,7 @@ def build(verbose=False, jobs=8):
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_fortran(verbose)
     return LIB
 
 

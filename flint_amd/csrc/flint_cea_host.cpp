@@ -92,3 +92,32 @@ extern "C" int flint_cea_solve(flint_mech* m, long n, long ld, int hp, const dou
   if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("equilibrium launch: ") + hipGetErrorString(e));
   return FLINT_OK;
 }
+
+extern "C" int flint_cea_solve_host(flint_mech* m, long n, int hp, const double* p, const double* T,
+                                    const double* rho, double* T_out, double* Y_out, int* status) {
+  if (!m) return flint_fail(FLINT_EARG, "null mechanism");
+  if (n < 0) return flint_fail(FLINT_EARG, "need n >= 0");
+  if (n == 0) return FLINT_OK;
+  if (!T || !rho || !T_out || !Y_out || (hp == 1 && !p)) return flint_fail(FLINT_EARG, "null array");
+  std::lock_guard<std::mutex> g(m->mu);
+  const size_t ns = m->ns, nn = (size_t)n;
+  // p, T, rho, T_out, Y_out, status (ints in a double-sized slot)
+  if (int rc = flint_stage(m, (2 * ns + 4) * nn)) return rc;
+  double* d_p = m->d_stage;
+  double* d_T = d_p + nn;
+  double* d_rho = d_T + nn;
+  double* d_To = d_rho + ns * nn;
+  double* d_Yo = d_To + nn;
+  int* d_st = reinterpret_cast<int*>(d_Yo + ns * nn);
+  if (hp == 1) HIPCHK(hipMemcpyAsync(d_p, p, nn * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(d_T, T, nn * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(d_rho, rho, ns * nn * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  if (int rc = flint_cea_solve(m, n, n, hp, hp == 1 ? d_p : nullptr, d_T, d_rho, nullptr, d_To, d_Yo, nullptr, d_st,
+                               m->stream))
+    return rc;
+  HIPCHK(hipMemcpyAsync(T_out, d_To, nn * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipMemcpyAsync(Y_out, d_Yo, ns * nn * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  if (status) HIPCHK(hipMemcpyAsync(status, d_st, nn * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return FLINT_OK;
+}

@@ -293,3 +293,25 @@ extern "C" int flint_reactor_advance_graph(flint_mech* m, long n, long ld, int m
   HIPCHK(hipGraphLaunch(x->gexec, s));
   return FLINT_OK;
 }
+
+extern "C" int flint_reactor_advance_host(flint_mech* m, long n, int mode, double* X, const double* p, double t_end,
+                                          double rtol, double atol, int max_steps, int* status) {
+  if (int rc = check_args(m, n, n, mode, X, p, t_end, rtol, atol)) return rc;
+  if (n == 0) return FLINT_OK;
+  std::lock_guard<std::mutex> g(m->mu);
+  const size_t nz = m->ns + 1, nn = (size_t)n;
+  // X, p, status (ints in a double-sized slot)
+  if (int rc = flint_stage(m, (nz + 2) * nn)) return rc;
+  double* d_X = m->d_stage;
+  double* d_p = d_X + nz * nn;
+  int* d_st = reinterpret_cast<int*>(d_p + nn);
+  HIPCHK(hipMemcpyAsync(d_X, X, nz * nn * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  if (mode == 1) HIPCHK(hipMemcpyAsync(d_p, p, nn * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  if (int rc = flint_reactor_advance(m, n, n, mode, d_X, mode == 1 ? d_p : nullptr, t_end, rtol, atol, max_steps,
+                                     nullptr, d_st, m->stream))
+    return rc;
+  HIPCHK(hipMemcpyAsync(X, d_X, nz * nn * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  if (status) HIPCHK(hipMemcpyAsync(status, d_st, nn * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return FLINT_OK;
+}

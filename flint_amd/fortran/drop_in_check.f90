@@ -1,0 +1,115 @@
+! Calls every procedure of module flint_amd the way a FLINT caller does and writes the results
+! at full precision, so that they can be compared with the same calls made through the C ABI.
+!
+!   drop_in_check <native input folder> <states file> <results file>
+!
+! states file:  n, then n lines "T rho_1 .. rho_ns"          (chemistry_source, rhs_native)
+!               m, then m lines "T p_bar Y_1 .. Y_ns"        (CEA_solve, UV and HP)
+!               k, then k lines "p_Pa Y_1 .. Y_ns T"         (constant-pressure reactors)
+!               t_end rtol atol                              (reactor integration)
+! results file: blocks "# name rows cols" followed by the rows.
+program drop_in_check
+  use flint_amd
+  implicit none
+  character(len=512) :: folder, fin, fout
+  character(len=64) :: mech_name
+  integer :: err, n, m, k, i, u, v, nz
+  real(8), allocatable :: T(:), rho(:, :), w1(:, :), w2(:, :), F1(:, :), F2(:, :), J(:, :, :)
+  real(8), allocatable :: Tc(:), pc(:), Yc(:, :), To(:), Yo(:, :), Tb(:), Yb(:, :)
+  real(8), allocatable :: pr(:), X(:, :), row(:)
+  integer, allocatable :: st(:)
+  real(8) :: t_end, rtol, atol
+
+  call get_command_argument(1, folder)
+  call get_command_argument(2, fin)
+  call get_command_argument(3, fout)
+
+  err = read_idealgas_thermo(trim(folder))
+  if (err /= 0) stop 2
+  err = read_chemistry(folder=trim(folder), mech_name=mech_name)
+  if (err /= 0) stop 3
+  call Assign_Mechanism(mech_name)
+  nz = ns + 1
+
+  open(newunit=u, file=trim(fin), status='old', action='read')
+  read(u, *) n
+  allocate(T(n), rho(n, ns), row(nz + 1))
+  do i = 1, n
+    read(u, *) row(1:nz)
+    T(i) = row(1)
+    rho(i, :) = row(2:nz)
+  end do
+  read(u, *) m
+  allocate(Tc(m), pc(m), Yc(m, ns))
+  do i = 1, m
+    read(u, *) row(1:ns + 2)
+    Tc(i) = row(1)
+    pc(i) = row(2)
+    Yc(i, :) = row(3:ns + 2)
+  end do
+  read(u, *) k
+  allocate(pr(k), X(k, nz))
+  do i = 1, k
+    read(u, *) row(1:nz + 1)
+    pr(i) = row(1)
+    X(i, :) = row(2:nz + 1)
+  end do
+  read(u, *) t_end, rtol, atol
+  close(u)
+
+  open(newunit=v, file=trim(fout), status='replace', action='write')
+
+  ! per-cell calls, as a CFD code makes them
+  allocate(w1(n, ns), F1(n, nz))
+  do i = 1, n
+    row(1:ns) = rho(i, :)
+    call chemistry_source(row(1:ns), T(i), w1(i, :))
+    row(1:ns) = rho(i, :)
+    row(nz) = T(i)
+    call rhs_native(nz, 0d0, row(1:nz), F1(i, :))
+  end do
+  call put(v, 'chemistry_source', w1)
+  call put(v, 'rhs_native', F1)
+
+  ! the same cells in one call each
+  allocate(w2(n, ns), F2(n, nz), J(n, nz, nz))
+  call chemistry_source_batch(rho, T, w2)
+  call put(v, 'chemistry_source_batch', w2)
+  deallocate(F2)
+  allocate(F2(n, nz))
+  call rhs_native_batch(reshape([rho, T], [n, nz]), F2, J)
+  call put(v, 'rhs_native_batch', F2)
+  call put(v, 'rhs_native_batch_J', reshape(J, [n, nz * nz]))
+
+  ! equilibrium, one state at a time (UV, FLINT's default) and batched (HP)
+  call CEA_initialize_global()
+  allocate(To(m), Yo(m, ns), Tb(m), Yb(m, ns), st(max(m, k)))
+  do i = 1, m
+    call CEA_solve(pc(i), Tc(i), Yc(i, :), To(i), Yo(i, :))
+  end do
+  call put(v, 'CEA_solve_uv', reshape([To, Yo], [m, nz]))
+  Vol = .false.
+  call CEA_solve_batch(pc, Tc, Yc, Tb, Yb, st(1:m))
+  call put(v, 'CEA_solve_batch_hp', reshape([Tb, Yb, dble(st(1:m))], [m, nz + 1]))
+
+  ! constant-pressure reactors
+  call reactor_advance_batch(X, t_end, rtol, atol, constant_pressure=.true., p=pr, status=st(1:k))
+  call put(v, 'reactor_advance_batch', reshape([X, dble(st(1:k))], [k, nz + 1]))
+  close(v)
+
+  call free_chemistry_data()
+
+contains
+
+  subroutine put(unit, name, a)
+    integer, intent(in) :: unit
+    character(len=*), intent(in) :: name
+    real(8), intent(in) :: a(:, :)
+    integer :: r
+    write(unit, '(A,1X,A,1X,I0,1X,I0)') '#', name, size(a, 1), size(a, 2)
+    do r = 1, size(a, 1)
+      write(unit, '(*(ES25.17E3,1X))') a(r, :)
+    end do
+  end subroutine
+
+end program drop_in_check

@@ -47,6 +47,8 @@ def lib():
         L.flint_cea_solve.argtypes = [vp, l, l, i, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.flint_source_host.argtypes = [vp, l, vp, vp, vp]
         L.flint_rhs_host.argtypes = [vp, l, vp, vp, vp]
+        L.flint_reactor_advance_host.argtypes = [vp, l, i, vp, vp, d, d, d, i, vp]
+        L.flint_cea_solve_host.argtypes = [vp, l, i, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -204,3 +206,21 @@ class Mechanism:
         J = np.empty((self.nz * self.nz, n)) if jac else None
         check(lib().flint_rhs_host(self._h, n, _p(Z), _p(F), _p(J)))
         return (F, J) if jac else F
+
+    def advance_host(self, X, p=None, mode=0, t_end=1e-4, rtol=1e-6, atol=1e-12, max_steps=0):
+        """X [ns+1, n] (numpy) advanced from 0 to t_end; returns (X_new, status)."""
+        X = np.array(X, dtype=np.float64, order="C")
+        n = X.shape[1]
+        p = None if p is None else np.ascontiguousarray(p, dtype=np.float64)
+        st = np.empty(n, dtype=np.int32)
+        check(lib().flint_reactor_advance_host(self._h, n, mode, _p(X), _p(p), t_end, rtol, atol, max_steps, _p(st)))
+        return X, st
+
+    def equilibrium_host(self, T, rho, p=None, hp=True):
+        """CEA_solve over host arrays T [n], rho [ns, n]; returns (T_out, Y_out, status)."""
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        p = None if p is None else np.ascontiguousarray(p, dtype=np.float64)
+        To, Yo, st = np.empty_like(T), np.empty_like(rho), np.empty(T.size, dtype=np.int32)
+        check(lib().flint_cea_solve_host(self._h, T.size, int(hp), _p(p), _p(T), _p(rho), _p(To), _p(Yo), _p(st)))
+        return To, Yo, st

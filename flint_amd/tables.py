@@ -214,3 +214,63 @@ def read_tecplot(path):
         zones.append(np.array(cur))
     T = zones[0][:, 0]
     return T, [z[:, 1:] for z in zones]
+
+
+# ----------------------------------------------------------------------------------------------
+# Native input folder (docs/input_native.md): what read_idealgas_thermo and read_chemistry read
+# ----------------------------------------------------------------------------------------------
+_KIND = {"arrhenius": "Arrhenius", "three-body": "three-body-Arrhenius", "troe": "falloff-Troe",
+         "lindemann": "falloff-Lindemann"}
+
+
+def write_native(name, folder, prefix="", fmt="%.16E"):
+    """Write the native input folder of mechanism ``name`` (phase.txt, composition.txt,
+    chemistry-info.txt, thermo.dat and chemistry-{Arrhenius,Troe,Lindemann}.dat) in the layout
+    FLINT's loaders read (Load_ThermoTransport.f90:7-126, Load_Chemistry.f90:7-164).
+
+    The reference ships such folders with 13 significant digits; the default ``fmt`` prints 17
+    so that a folder written here loads into exactly the tables generated in memory.  The
+    chemistry-info.txt "Reaction definition" block of the data-driven ``general`` kernel is not
+    written (the compiled mechanisms do not read it)."""
+    desc = load_descriptor(name)
+    Tmin, Tmax = desc["T_range"]
+    os.makedirs(folder, exist_ok=True)
+    path = lambda f: os.path.join(folder, prefix + f)
+    sp, W = desc["species"], desc["W"]
+    with open(path("phase.txt"), "w") as f:
+        f.write("ideal-gas phase\n")
+        for s, w in zip(sp, W):
+            f.write("%s %f\n" % (s, w))
+    el, comp = desc["elements"], np.asarray(desc["composition"])
+    with open(path("composition.txt"), "w") as f:
+        f.write("%d\n" % len(el))
+        f.write("".join("%s\n" % e for e in el))
+        f.write("\ncomposition\n")
+        for k in range(len(sp)):
+            f.write(" ".join("%d" % round(v) for v in comp[:, k]) + "\n")
+    rx = desc["reactions"]
+    with open(path("chemistry-info.txt"), "w") as f:
+        f.write("%s\nN.ro species = %4d\nN.ro reactions = %4d\n\n" % (desc["name"], len(sp), len(rx)))
+        f.write("General loop info. They are not used if the mechanism is exlplicitly defined\n\n")
+        f.write("Reaction type\n")
+        for j, r in enumerate(rx):
+            f.write("%d %s\n" % (j + 1, _KIND[r["type"]]))
+    th = thermo_tables(desc, Tmin, Tmax)
+    T = th["T"]
+    write_tecplot(path("thermo.dat"), "Mass Thermodynamic Properties",
+                  ["Temperature", "Cp", "Enthalpy", "Entropy", "dCp"], sp, T,
+                  [np.column_stack([th[v][:, k] for v in ("cp", "h", "s", "dcp")]) for k in range(len(sp))], fmt)
+    kin = kinetics_tables(desc, Tmin, Tmax)
+    na = kin["kf"].shape[1]
+    write_tecplot(path("chemistry-Arrhenius.dat"), "Arrhenius data", ["Temperature", "kf", "kb"],
+                  ["Reaction %d" % (a + 1) for a in range(na)], T,
+                  [np.column_stack([kin["kf"][:, a], kin["kb"][:, a]]) for a in range(na)], fmt)
+    for key, title, var in (("troe", "Fall-off Troe data", ["k_inf", "k_0", "k_c", "F_cent"]),
+                            ("lind", "Fall-off Lindemann data", ["k_inf", "k_0", "k_c"])):
+        tab = kin[key]
+        if tab.shape[1]:
+            fname = "chemistry-Troe.dat" if key == "troe" else "chemistry-Lindemann.dat"
+            write_tecplot(path(fname), title, ["Temperature"] + var,
+                          ["Reaction %d" % (q + 1) for q in range(tab.shape[1])], T,
+                          [tab[:, q, :] for q in range(tab.shape[1])], fmt)
+    return folder

@@ -111,6 +111,13 @@ int flint_reactor_advance_graph(flint_mech *m, long n, long ld, int mode, double
                                 double t_end, double rtol, double atol, int max_steps, int *stats_dev, int *status_dev,
                                 void *stream);
 
+/* Host-memory convenience for the reactor batch (the time loop of test-batchF.f90:99-108, which
+ * advances one reactor through OSlo's run_odesolver with rhs_native): copies X[(ns+1)][n] and p[n] in,
+ * runs flint_reactor_advance with ld = n on the library's own stream, copies X and status[n] (may
+ * be NULL) back and synchronises.  p is read for mode 1 only. */
+int flint_reactor_advance_host(flint_mech *m, long n, int mode, double *X, const double *p, double t_end,
+                               double rtol, double atol, int max_steps, int *status);
+
 /* Launch-shape autotuner: times the candidate (teams per block, blocks) shapes on a copy of the
  * given sample and keeps the fastest for this handle (synchronous; the sample is not modified). */
 int flint_reactor_autotune(flint_mech *m, long n, long ld, int mode, const double *X_dev, const double *p_dev,
@@ -141,6 +148,12 @@ int flint_cea_setup(flint_mech *m, const double *s, const double *el_weight);
 int flint_cea_solve(flint_mech *m, long n, long ld, int hp, const double *p_dev, const double *T_dev,
                     const double *rho_dev, const double *guess_dev, double *T_out_dev, double *Y_out_dev,
                     int *iters_dev, int *status_dev, void *stream);
+
+/* Host-memory convenience for per-state CEA_solve(p_in, T_in, rhoi_in, T_out, rhoi_out) callers
+ * (Lib_CEA_solver.f90:6-15): as flint_cea_solve with ld = n and the default initial guess, on the
+ * library's own stream, synchronous.  p[n] is read for hp = 1 only; status[n] may be NULL. */
+int flint_cea_solve_host(flint_mech *m, long n, int hp, const double *p, const double *T, const double *rho,
+                         double *T_out, double *Y_out, int *status);
 
 /* Multilayer perceptron on MFMA (no reference counterpart: the learned reactor step and the
  * equilibrium warm start of this library).  Layers d_0 -> d_1 -> ... -> d_L with GELU on every

@@ -50,3 +50,15 @@ def test_create_rejects_bad_sizes():
     rc = L.flint_mech_create(b"no-such-mechanism", 5, p(W), 1, 2, 3, p(t), p(t), 0, None, 0, None, 1, 2, p(t), p(t), 0,
                              None, C.byref(out))
     assert rc == 4
+
+
+def test_fortran_drop_in_reports_missing_folder(tmp_path):
+    """The Fortran module's read_idealgas_thermo returns FLINT's nonzero ios with the library's
+    message for a folder without phase.txt (no GPU call is made)."""
+    import subprocess
+    exe = os.path.join(ROOT, "flint_amd", "fortran", "_build", "drop_in_check")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran module not built (no flang)")
+    r = subprocess.run([exe, str(tmp_path / "nothing"), "x", "y"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2
+    assert "phase.txt" in r.stdout

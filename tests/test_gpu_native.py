@@ -1,0 +1,124 @@
+"""The native input folder and the Fortran drop-in, end to end on the GPU.
+
+* flint_mech_load on a folder written by flint_amd.tables.write_native (FLINT's phase.txt,
+  composition.txt, chemistry-info.txt and Tecplot tables, Load_ThermoTransport.f90:7-126,
+  Load_Chemistry.f90:7-164) gives a mechanism whose source terms, Jacobians, equilibria and
+  reactor steps are bit-identical to the one built from the same tables in memory;
+* the Fortran module (flint_amd/fortran/flint_amd.f90) called the way a FLINT caller calls
+  read_idealgas_thermo / read_chemistry / Assign_Mechanism / chemistry_source / rhs_native /
+  CEA_solve returns, per cell, exactly what the C ABI returns for the same cells."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from flint_amd import states as S
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "flint_amd", "fortran", "_build", "drop_in_check")
+NAME = "TSR-Rich-31"
+
+
+@pytest.fixture(scope="module")
+def native(tmp_path_factory, mechs, cuda):
+    from flint_amd.native import Mechanism
+    from flint_amd.tables import write_native
+    folder = str(tmp_path_factory.mktemp("native") / NAME)
+    print(f"[gpu] write_native {NAME}", flush=True)
+    write_native(NAME, folder)
+    md = mechs[NAME]
+    return folder, md, Mechanism.load(folder), Mechanism.from_data(md)
+
+
+def _eq(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+
+
+def test_loaded_folder_matches_in_memory_tables(native, cuda):
+    import torch
+    folder, md, ml, mm = native
+    assert ml.ns == md.ns and ml.query() == mm.query()
+    n = 2048
+    st = S.radical_states(md.species, md.W, 91, 0, n)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    rho, T = dev(S.soa(st["rho"])), dev(st["T"])
+    out = []
+    for m in (ml, mm):
+        w = torch.empty_like(rho)
+        m.source(rho, T, w)
+        Z = torch.cat([rho, T[None]], 0)
+        F, J = torch.empty_like(Z), torch.empty((md.ns + 1) ** 2, n, dtype=torch.float64, device=cuda)
+        m.rhs(Z, F, J)
+        X = dev(np.vstack([S.soa(st["Y"]), st["T"][None]]))
+        m.advance(X, dev(st["p"]), mode=1, t_end=1e-5)
+        out.append([x.cpu().numpy() for x in (w, F, J, X)])
+    torch.cuda.synchronize()
+    print("[gpu] native folder vs in-memory tables", flush=True)
+    for a, b in zip(*out):
+        assert _eq(a, b)
+    cs = S.radical_states(md.species, md.W, 92, 0, 512, T_lo=600.0, T_hi=900.0)
+    r1 = ml.equilibrium_host(cs["T"], S.soa(cs["Y"]), p=cs["p"] / 1e5, hp=True)
+    r2 = mm.equilibrium_host(cs["T"], S.soa(cs["Y"]), p=cs["p"] / 1e5, hp=True)
+    assert all(_eq(a, b) for a, b in zip(r1, r2))
+    assert np.mean(r1[2] == 0) > 0.9
+
+
+def _blocks(path):
+    out, name, rows = {}, None, []
+    with open(path) as f:
+        for line in f:
+            if line.startswith("#"):
+                if name:
+                    out[name] = np.array(rows)
+                _, name, _, _ = line.split()
+                rows = []
+            else:
+                rows.append([float(v) for v in line.split()])
+    out[name] = np.array(rows)
+    return out
+
+
+def test_fortran_drop_in_matches_c_abi(native, tmp_path):
+    folder, md, ml, mm = native
+    if not os.path.exists(EXE):
+        pytest.fail(f"{EXE} is missing: build with python -m flint_amd.build (needs flang)")
+    ns, nz = md.ns, md.ns + 1
+    n, m, k = 64, 32, 48
+    st = S.radical_states(md.species, md.W, 93, 0, n)
+    cs = S.radical_states(md.species, md.W, 94, 0, m, T_lo=600.0, T_hi=900.0)
+    rs = S.radical_states(md.species, md.W, 95, 0, k, ignition_half=True)
+    t_end, rtol, atol = 2e-5, 1e-6, 1e-12
+    fin, fout = tmp_path / "states.txt", tmp_path / "results.txt"
+    with open(fin, "w") as f:
+        f.write(f"{n}\n")
+        for i in range(n):
+            f.write(" ".join("%.17e" % v for v in [st["T"][i], *st["rho"][i]]) + "\n")
+        f.write(f"{m}\n")
+        for i in range(m):
+            f.write(" ".join("%.17e" % v for v in [cs["T"][i], cs["p"][i] / 1e5, *cs["Y"][i]]) + "\n")
+        f.write(f"{k}\n")
+        for i in range(k):
+            f.write(" ".join("%.17e" % v for v in [rs["p"][i], *rs["Y"][i], rs["T"][i]]) + "\n")
+        f.write("%.17e %.17e %.17e\n" % (t_end, rtol, atol))
+    print("[gpu] fortran drop_in_check", flush=True)
+    r = subprocess.run([EXE, folder, str(fin), str(fout)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = _blocks(fout)
+    # the same cells through the C ABI (host-memory entry points, in-memory tables)
+    rho, T = S.soa(st["rho"]), st["T"]
+    w = mm.source_host(rho, T)
+    F, J = mm.rhs_host(np.vstack([rho, T[None]]), jac=True)
+    F1 = np.stack([mm.rhs_host(np.concatenate([rho[:, i], T[i:i + 1]])[:, None])[:, 0] for i in range(n)], 1)
+    assert _eq(got["chemistry_source"], w.T) and _eq(got["chemistry_source_batch"], w.T)
+    assert _eq(got["rhs_native"], F1.T) and _eq(got["rhs_native_batch"], F.T)
+    assert _eq(got["rhs_native_batch_J"], J.T)
+    Tu, Yu, su = mm.equilibrium_host(cs["T"], S.soa(cs["Y"]), hp=False)
+    assert _eq(got["CEA_solve_uv"], np.vstack([Tu[None], Yu]).T)
+    Th, Yh, sh = mm.equilibrium_host(cs["T"], S.soa(cs["Y"]), p=cs["p"] / 1e5, hp=True)
+    assert _eq(got["CEA_solve_batch_hp"], np.vstack([Th[None], Yh, sh[None]]).T)
+    X, sx = mm.advance_host(np.vstack([S.soa(rs["Y"]), rs["T"][None]]), rs["p"], mode=1, t_end=t_end, rtol=rtol,
+                            atol=atol)
+    assert _eq(got["reactor_advance_batch"], np.vstack([X, sx[None]]).T)
+    assert np.mean(sx == 0) > 0.9 and np.mean(sh == 0) > 0.9

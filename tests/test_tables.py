@@ -52,3 +52,38 @@ def test_tecplot_roundtrip(tmp_path):
     T, z = TB.read_tecplot(str(p))
     assert np.array_equal(T, k["T"])
     assert np.array_equal(z[2][:, 0], k["kf"][:, 2])
+
+
+def test_write_native_loads_back_exactly(tmp_path):
+    """A native folder written for Pelucchi (Arrhenius, three-body, Troe and Lindemann reactions)
+    parses back into exactly the generated tables, with FLINT's file names, headers and the
+    (A17,I4) reaction-count line that read_chemistry reads (Load_Chemistry.f90:38)."""
+    from flint_amd.mechdata import MechData
+    folder = tmp_path / "pel"
+    TB.write_native("Pelucchi", str(folder))
+    md = MechData("Pelucchi")
+    names = sorted(p.name for p in folder.iterdir())
+    assert names == ["chemistry-Arrhenius.dat", "chemistry-Lindemann.dat", "chemistry-Troe.dat",
+                     "chemistry-info.txt", "composition.txt", "phase.txt", "thermo.dat"]
+    info = (folder / "chemistry-info.txt").read_text().splitlines()
+    assert info[0] == "Pelucchi" and int(info[2][17:21]) == 103
+    kinds = [l.split()[1] for l in info[7:7 + 103]]
+    assert kinds.count("falloff-Troe") == md.n_troe and kinds.count("falloff-Lindemann") == md.n_lind
+    T, zones = TB.read_tecplot(str(folder / "chemistry-Arrhenius.dat"))
+    assert T[0] == md.Tk_min and len(zones) == md.n_arrh
+    assert np.array_equal(np.stack([z[:, 0] for z in zones], 1), md.kf)
+    assert np.array_equal(np.stack([z[:, 1] for z in zones], 1), md.kb)
+    _, zones = TB.read_tecplot(str(folder / "chemistry-Troe.dat"))
+    assert np.array_equal(np.stack(zones, 1), md.troe)
+    _, zones = TB.read_tecplot(str(folder / "chemistry-Lindemann.dat"))
+    assert np.array_equal(np.stack(zones, 1), md.lind)
+    T, zones = TB.read_tecplot(str(folder / "thermo.dat"))
+    assert T[0] == 1 and md.Th_min == 0  # the loader adds the T = 0 row (Load_ThermoTransport.f90:80-85)
+    for j, tab in enumerate((md.cp, md.h, md.s)):
+        assert np.array_equal(np.stack([z[:, j] for z in zones], 1), tab[1:])
+    phase = (folder / "phase.txt").read_text().splitlines()
+    assert phase[0] == "ideal-gas phase" and [l.split()[0] for l in phase[1:]] == md.species
+    comp = (folder / "composition.txt").read_text().split()
+    ne = int(comp[0])
+    assert comp[1:1 + ne] == list(md.elements) and comp[1 + ne] == "composition"
+    assert np.array_equal(np.array(comp[2 + ne:], dtype=float).reshape(md.ns, ne).T, md.composition)
