@@ -202,8 +202,11 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   const bool sp = lane < ns;
   const double Wk = sp ? P.W[lane] : 1.0;
   const double Rk = sp ? 8314.51 / Wk : 0.0;
-  o.xc = sp ? fmax(x, 0.0) : 0.0;
-  o.g = (sp && x >= 0.0) ? 1.0 : 0.0;
+  // constant volume clips rho at 0 as rhs_native does (Lib_Chemistry_rhs.f90:37-39); the
+  // constant-pressure model keeps Y as it is, so its right-hand side stays smooth through Y = 0
+  const bool clip = mode == 0 && x < 0.0;
+  o.xc = sp ? (clip ? 0.0 : x) : 0.0;
+  o.g = (sp && !clip) ? 1.0 : 0.0;
   double roi;
   if (mode == 0) {
     roi = o.xc;

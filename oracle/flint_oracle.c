@@ -370,7 +370,8 @@ void orc_jac(const orc_mech *m, long n, long ld, const double *Z, double *F, dou
  *   dF_k/dT   = (b_k - (A rho)_k / T)/rho + wdot_k/(rho T)
  *   dF_T/dY_j = -sum_k h_k dwdot_k/dY_j /(rho cp) - F_T g_j (cp_j/cp - R_j/R)
  *   dF_T/dT   = -(sum_k h'_k wdot_k + h_k dwdot_k/dT)/(rho cp) - F_T (sum_k Y_k cp'_k/cp - 1/T)
- * with dwdot_k/dY_j = rho g_j (A_kj - (A Y)_k R_j/R), dwdot_k/dT = b_k - (A rho)_k/T, R = R_mix. */
+ * with dwdot_k/dY_j = rho g_j (A_kj - (A Y)_k R_j/R), dwdot_k/dT = b_k - (A rho)_k/T, R = R_mix,
+ * and g_j = 1 (the constant-pressure model does not clip Y at 0). */
 static void jac_cp_cell(const orc_mech *m, const double *nu, const double *x, double p, double *F, double *J) {
   int ns = m->ns, nz = ns + 1;
   double T = x[ns];
@@ -379,8 +380,8 @@ static void jac_cp_cell(const orc_mech *m, const double *nu, const double *x, do
   if (T < m->Th_min || T > m->Th_max || isnan(T)) return;
   double Yc[ORC_MAX_NS], g[ORC_MAX_NS], R[ORC_MAX_NS], roi[ORC_MAX_NS], Rmix = 0.0;
   for (int k = 0; k < ns; ++k) {
-    Yc[k] = x[k] < 0.0 ? 0.0 : x[k];
-    g[k] = x[k] < 0.0 ? 0.0 : 1.0;
+    Yc[k] = x[k];
+    g[k] = 1.0;
     R[k] = ORC_RUNIV / m->W[k];
     Rmix += Yc[k] * R[k];
   }

@@ -42,10 +42,14 @@ def test_reactor_matches_oracle(gmech, mechs, oracle_mod, cuda, mech, mode):
     print(f"[gpu] reactor {mech} mode={mode} n={n}", flush=True)
     G, gst, gstatus, untouched = _run(gmech[mech], X, st["p"], mode, cuda)
     assert np.all(untouched == -7)
-    # a few cells whose solution crosses the rhs_native clip rho = max(rho, 0) with a
-    # 1e18 1/s decomposition rate (IC3H7CO in Rich-31) defeat modified Newton at atol 1e-12 in
-    # both implementations and are reported with status 1/2; they must stay rare
-    assert np.mean(gstatus != 0) <= 0.04 and set(np.unique(gstatus)) <= {0, 1, 2}
+    # constant volume: a few cells whose solution crosses the rhs_native clip rho = max(rho, 0)
+    # with a 1e18 1/s decomposition rate (IC3H7CO in Rich-31) defeat modified Newton at atol 1e-12
+    # in both implementations and are reported with status 1/2; they must stay rare.  The
+    # constant-pressure model does not clip and integrates every cell.
+    if mode == 0:
+        assert np.mean(gstatus != 0) <= 0.04 and set(np.unique(gstatus)) <= {0, 1, 2}
+    else:
+        assert np.all(gstatus == 0)
     o = oracle_mod.Oracle(md)
     A, ast, astatus = o.bdf(X, st["p"], mode=mode)
     Xp = X * (1 + 1e-14 * np.sin(np.arange(X.size)).reshape(X.shape))
@@ -76,7 +80,8 @@ def test_reactor_matches_oracle(gmech, mechs, oracle_mod, cuda, mech, mode):
 @pytest.mark.parametrize("mode", [0, 1])
 def test_reactor_jacobian_matches_oracle(gmech, mechs, oracle_mod, cuda, mech, mode):
     """The integrator's own F and J (team kernels) against the oracle's rhs/jac (mode 0) and
-    rhs_cp/jac_cp (mode 1), including clipped negative components and out-of-table cells."""
+    rhs_cp/jac_cp (mode 1), including negative components (clipped in mode 0 only) and out-of-table
+    cells."""
     import torch
     md = mechs[mech]
     ns, nz, n = md.ns, md.ns + 1, 300
@@ -111,9 +116,12 @@ def test_reactor_jacobian_matches_oracle(gmech, mechs, oracle_mod, cuda, mech, m
     Jr, Jor = J[:, ins].reshape(nz, nz, -1), Jo[:, ins].reshape(nz, nz, -1)
     rowscale = np.maximum(np.abs(Jor).max(1, keepdims=True), 1e-300)  # inert species: zero rows
     assert np.max(np.abs(Jr - Jor) / rowscale) < 1e-10
-    # species rows: mass conservation makes every column sum vanish
+    # species rows: mass conservation makes every column sum vanish, to the round-off of the
+    # entries' own cancellation (A_kj - (A Y)_k R_j/R in mode 1: the oracle's sums reach 2e-11
+    # relative on these states)
     colsum = np.abs(Jr[:ns].sum(0)) / np.abs(Jr[:ns]).sum(0).clip(1e-300)
-    assert np.max(colsum) < 1e-12
+    Jos = Jor[:ns].sum(0)
+    assert np.max(colsum) < max(1e-12, 10.0 * np.max(np.abs(Jos) / np.abs(Jor[:ns]).sum(0).clip(1e-300)))
 
 
 @pytest.mark.parametrize("mode", [0, 1])
