@@ -18,6 +18,13 @@ struct Prog {
   const int* ms_sp;
   const double* ms_eff;   // efficiency (1 for a bare term)
   const double* effd;     // [nms*32] dense efficiencies, 0 where a species is absent
+  // The same program re-encoded for the right-hand side of the integrator (see reactor.hip):
+  int R;                  // reaction rounds: slot 32*round + lane
+  const int4* rrec;       // [R*32] {kind | (mset + 1) << 3 (kind 7: empty), column, reaction, factor slots}
+  const float* effT;      // [ns][nms] efficiency of species j in third-body set s (0 if absent)
+  int C, CW, MAXP;        // production-sum chunk rounds, chunk width, most chunks of one species
+  const int2* cterm;      // [C][CW][32] {reaction (nr: empty), coefficient bits} of chunk (round, lane)
+  const int* kpart;       // [MAXP][32] partial-sum slots (32*round + lane) of species k in order, -1 none
   const double* W;        // [ns]
   const double* tabk;
   int Tk_min, Tk_max, wk;
@@ -33,7 +40,7 @@ struct Opts {
 
 }  // namespace flint_reactor
 
-size_t flint_reactor_team_bytes(int nr, int nms);
+size_t flint_reactor_team_bytes(const flint_reactor::Prog& P);
 hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reactor::Opts& O, long n, long ld,
                                 double* X, const double* pres, int* stats, int* status, int* counter,
                                 int teams_per_block, int blocks, double* jac_scratch, hipStream_t s);

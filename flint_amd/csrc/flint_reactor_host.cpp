@@ -56,6 +56,84 @@ static int upload(flint_extra* x, const T* src, size_t count, const T** dst) {
   return FLINT_OK;
 }
 
+// Re-encodes the rate program for the integrator's right-hand side:
+//  * reactions in rounds of 32 (slot 32*round + lane), grouped by kind so that a round runs one
+//    code path; each slot holds its kind, third-body set, table column, reaction index and up to
+//    three reactant and three product factor slots (5 bits each, 31 = a constant 1);
+//  * third-body efficiencies as a dense [ns][nms] table;
+//  * the net production sum of species k split into chunks of at most CW terms in reaction order,
+//    chunks spread over (round, lane) slots, and per species the list of its chunk slots.
+static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::vector<int>& rrec,
+                      std::vector<float>& effT, std::vector<int>& cterm, std::vector<int>& kpart) {
+  const int nr = hp->nr, ns = hp->ns, nms = hp->nms;
+  if (ns > 31) return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports ns <= 31");
+  std::vector<int> order;
+  for (int kind = 0; kind < 4; ++kind)
+    for (int r = 0; r < nr; ++r)
+      if ((hp->rx_kind[r] & 3) == kind) order.push_back(r);
+  P.R = (nr + 31) / 32;
+  rrec.assign((size_t)P.R * 32 * 4, 0);
+  for (int s = 0; s < P.R * 32; ++s) {
+    int* rec = &rrec[(size_t)s * 4];
+    if (s >= nr) {
+      rec[0] = 7;
+      rec[2] = nr;
+      rec[3] = 0x3fffffff;  // every factor slot 31
+      continue;
+    }
+    const int r = order[s];
+    rec[0] = (hp->rx_kind[r] & 3) | ((hp->rx_mset[r] + 1) << 3);
+    rec[1] = hp->rx_col[r];
+    rec[2] = r;
+    unsigned packed = 0;
+    for (int side = 0; side < 2; ++side) {
+      int slots[3] = {31, 31, 31}, nf = 0;
+      for (int f = 0; f < 4; ++f) {
+        const int code = hp->rx_fac[r * 8 + side * 4 + f];
+        if (code < 0) continue;
+        for (int e = 0; e < (code >> 8); ++e) {
+          if (nf == 3) return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports at most 3 factors per side");
+          slots[nf++] = code & 255;
+        }
+      }
+      for (int f = 0; f < 3; ++f) packed |= (unsigned)slots[f] << (5 * (3 * side + f));
+    }
+    rec[3] = (int)packed;
+  }
+  const int nmsp = std::max(nms, 1);
+  effT.assign((size_t)ns * nmsp, 0.0f);
+  for (int s = 0; s < nms; ++s)
+    for (int t = hp->ms_ptr[s]; t < hp->ms_ptr[s + 1]; ++t) effT[(size_t)hp->ms_sp[t] * nmsp + s] = (float)hp->ms_eff[t];
+  // production-sum chunks
+  P.CW = 16;
+  std::vector<std::pair<int, int>> chunks;  // (species, first term)
+  for (int k = 0; k < ns; ++k)
+    for (int t = hp->wd_ptr[k]; t < hp->wd_ptr[k + 1]; t += P.CW) chunks.push_back({k, t});
+  const int nch = (int)chunks.size();
+  P.C = std::max(1, (nch + 31) / 32);
+  cterm.assign((size_t)P.C * P.CW * 32 * 2, 0);
+  for (int i = 0; i < P.C * P.CW * 32; ++i) cterm[(size_t)i * 2] = nr;  // empty term: q[nr] = 0
+  std::vector<std::vector<int>> slots_of(ns);
+  for (int c = 0; c < nch; ++c) {
+    const int k = chunks[c].first, t0 = chunks[c].second, t1 = std::min(t0 + P.CW, hp->wd_ptr[k + 1]);
+    const int round = c / 32, lane = c % 32;
+    for (int t = t0; t < t1; ++t) {
+      const size_t at = (((size_t)round * P.CW + (t - t0)) * 32 + lane) * 2;
+      const float coef = (float)hp->wd_coef[t];
+      if ((double)coef != hp->wd_coef[t]) return flint_fail(FLINT_EUNSUPPORTED, "non-float32 stoichiometric coefficient");
+      cterm[at] = hp->wd_rx[t];
+      std::memcpy(&cterm[at + 1], &coef, sizeof coef);
+    }
+    slots_of[k].push_back(round * 32 + lane);
+  }
+  P.MAXP = 1;
+  for (int k = 0; k < ns; ++k) P.MAXP = std::max<int>(P.MAXP, (int)slots_of[k].size());
+  kpart.assign((size_t)P.MAXP * 32, -1);
+  for (int k = 0; k < ns; ++k)
+    for (size_t i = 0; i < slots_of[k].size(); ++i) kpart[i * 32 + k] = slots_of[k][i];
+  return FLINT_OK;
+}
+
 static int ensure_extra(flint_mech* m) {
   if (m->extra) return FLINT_OK;
   if (m->is_wd || !m->ops || !m->ops->prog)
@@ -73,7 +151,15 @@ static int ensure_extra(flint_mech* m) {
   std::vector<double> effd((size_t)std::max(hp->nms, 1) * 32, 0.0);
   for (int s = 0; s < hp->nms; ++s)
     for (int t = hp->ms_ptr[s]; t < hp->ms_ptr[s + 1]; ++t) effd[(size_t)s * 32 + hp->ms_sp[t]] = hp->ms_eff[t];
+  std::vector<int> rrec, kpart;
+  std::vector<float> effT;
+  std::vector<int> cterm;
+  if (int rc = encode_rhs(hp, P, rrec, effT, cterm, kpart)) return rc;
   int rc = 0;
+  rc |= upload(x, reinterpret_cast<const int4*>(rrec.data()), rrec.size() / 4, &P.rrec);
+  rc |= upload(x, effT.data(), effT.size(), &P.effT);
+  rc |= upload(x, reinterpret_cast<const int2*>(cterm.data()), cterm.size() / 2, &P.cterm);
+  rc |= upload(x, kpart.data(), kpart.size(), &P.kpart);
   rc |= upload(x, hp->rx_kind, hp->nr, &P.rx_kind);
   rc |= upload(x, hp->rx_col, hp->nr, &P.rx_col);
   rc |= upload(x, hp->rx_mset, hp->nr, &P.rx_mset);
@@ -98,7 +184,7 @@ static int ensure_extra(flint_mech* m) {
   // default launch: as many two-team blocks as fit in LDS on every CU
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, m->device));
-  const size_t tb = (flint_reactor_team_bytes(P.nr, P.nms) + 15) & ~size_t(15);
+  const size_t tb = (flint_reactor_team_bytes(P) + 15) & ~size_t(15);
   const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
   const int per_cu = std::max<int>(1, (int)(lds_cu / (tb * x->teams_per_block)));
   x->blocks = prop.multiProcessorCount * std::min(per_cu, 8);
@@ -200,7 +286,7 @@ extern "C" int flint_reactor_autotune(flint_mech* m, long n, long ld, int mode, 
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, m->device));
   const int cus = prop.multiProcessorCount;
-  const size_t tb = (flint_reactor_team_bytes(x->prog.nr, x->prog.nms) + 15) & ~size_t(15);
+  const size_t tb = (flint_reactor_team_bytes(x->prog) + 15) & ~size_t(15);
   const size_t lds_cu = std::min<size_t>(prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536,
                                          160 * 1024);
   const int nz = m->ns + 1;

@@ -11,12 +11,16 @@
 // Layout of one team (lanes l = 0..31 of a half wavefront):
 //   lane k < ns owns species k, lane ns owns T: the state component and the 8-row BDF difference
 //   column D[.][k] live in VGPRs; row k of the Jacobian sits in a per-team global slot and row k
-//   of the LU factors of I - cJ in LDS (stride 33, conflict-free); partial pivoting picks rows by
-//   a team argmax and every lane reads the pivot row as an LDS broadcast.
-//   The right-hand side is evaluated reaction-parallel (reaction r on lane r mod 32) from the
-//   kinetics table row of the cell's temperature (coalesced over the team), with the
-//   net production rate of species k summed by lane k in reaction order (the summation order of
-//   the reference routine), from per-reaction rates staged in LDS.
+//   of the inverse of I - cJ in LDS (stride 33, conflict-free).  The inverse comes from an
+//   in-place Gauss-Jordan elimination with partial pivoting (team argmax, pivot row read as an LDS
+//   broadcast), so that each Newton correction is one matrix-vector product instead of two
+//   32-step triangular sweeps.
+//   The right-hand side is evaluated reaction-parallel: reactions sit in rounds of 32, grouped by
+//   kind so that a round runs one code path, each lane reading its reaction's kinetics-table
+//   entries for the cell's temperature; net production rates are summed in chunks of at most 16
+//   terms (in reaction order) spread over the lanes, then per species over its chunks.  Every
+//   program load of a phase is independent of the others, so a phase costs one memory latency
+//   instead of one per term.
 //   Modes: 0 = constant volume, state (rho_k, T), rhs_native (Lib_Chemistry_rhs.f90:15-52);
 //          1 = constant pressure, state (Y_k, T) at fixed p.
 #include "flint_device.h"
@@ -27,6 +31,7 @@ namespace flint_reactor {
 constexpr int TEAM = 32;
 constexpr int MAXO = 5;
 constexpr int NEWTON_MAXITER = 4;
+constexpr int CW = 16;  // production-sum chunk width (the host encoder uses the same value)
 constexpr double MIN_FACTOR = 0.2, MAX_FACTOR = 10.0;
 
 enum { ST_STEPS, ST_REJECT, ST_NFAIL, ST_RHS, ST_JAC, ST_LU, ST_NEWTON, ST_ORDER };
@@ -51,37 +56,41 @@ __device__ __forceinline__ void tsync() {
 }
 
 struct Lds {
-  double* c;    // [32] concentrations
+  double* c;    // [32] concentrations; slot 31 (never a species) holds 1 for empty factors
   double* Mv;   // [nms] third-body sums
   double* aux;  // [32] per-lane shared values (Y_k, e_k or h_k)
   double* g;    // [32] clip derivative per column
-  double* q;    // [nr] rates of progress
-  double* lu;   // [32][33] LU factors of I - cJ (row = lane, padded)
+  double* q;    // [nr + 1] rates of progress; q[nr] = 0 for empty sum terms
+  double* part; // [C*32] partial production sums of the chunks
+  double* vb;   // [32] right-hand side of a Newton correction, in pivot order
+  double* vx;   // [32] its solution
+  double* lu;   // [32][33] inverse of I - cJ (row = lane, padded)
   double* kfe;  // [nr] effective forward coefficient (incl. M or falloff)   } alias lu while the
   double* kbe;  // [nr]                                                     } Jacobian is built
-  double* Dm;   // [nr] dq/dM                                               } (the LU is rebuilt
-  double* dqT;  // [nr] dq/dT at fixed concentrations                       }  right after)
-  int* piv;     // [32] pivot lane of each elimination step
+  double* Dm;   // [nr] dq/dM                                               } (the inverse is
+  double* dqT;  // [nr] dq/dT at fixed concentrations                       }  rebuilt right after)
 };
 
-__host__ __device__ inline size_t team_lds_bytes(int nr, int nms) {
-  return (size_t)(32 * 3 + (nms > 0 ? nms : 1) + (size_t)nr + 32 * 33) * sizeof(double) + 32 * sizeof(int);
+__host__ __device__ inline size_t team_lds_bytes(const Prog& P) {
+  return (size_t)(32 * 5 + (P.nms > 0 ? P.nms : 1) + (size_t)P.nr + 1 + (size_t)P.C * 32 + 32 * 33) * sizeof(double);
 }
 
-__device__ inline Lds carve(char* base, int nr, int nms) {
+__device__ inline Lds carve(char* base, const Prog& P) {
   Lds L;
   double* d = reinterpret_cast<double*>(base);
   L.c = d; d += 32;
   L.aux = d; d += 32;
   L.g = d; d += 32;
-  L.Mv = d; d += (nms > 0 ? nms : 1);
-  L.q = d; d += nr;
-  L.lu = d; d += 32 * 33;
+  L.vb = d; d += 32;
+  L.vx = d; d += 32;
+  L.Mv = d; d += (P.nms > 0 ? P.nms : 1);
+  L.q = d; d += P.nr + 1;
+  L.part = d; d += P.C * 32;
+  L.lu = d;
   L.kfe = L.lu;
-  L.kbe = L.lu + nr;
-  L.Dm = L.lu + 2 * nr;
-  L.dqT = L.lu + 3 * nr;
-  L.piv = reinterpret_cast<int*>(d);
+  L.kbe = L.lu + P.nr;
+  L.Dm = L.lu + 2 * P.nr;
+  L.dqT = L.lu + 3 * P.nr;
   return L;
 }
 
@@ -93,37 +102,19 @@ struct RhsOut {
   double Rmix, rho_cv, cpm; // team-uniform sums
 };
 
-// Rates of progress of reaction r (kinds: 0 Arrhenius, 1 three-body, 2 Troe, 3 Lindemann),
-// Lib_Chemistry_falloff.f90:12-80 for the falloff forms.
+// Rates of progress of the reaction in record rec (kinds: 0 Arrhenius, 1 three-body, 2 Troe,
+// 3 Lindemann), Lib_Chemistry_falloff.f90:12-80 for the falloff forms.
 template <bool JAC>
-__device__ __forceinline__ void reaction(const Prog& P, int r, const double* __restrict__ rk0, const double* __restrict__ rk1,
+__device__ __forceinline__ void reaction(const int4 rec, const double* __restrict__ rk0, const double* __restrict__ rk1,
                                          double Td, const Lds& L) {
-  const int kinfo = P.rx_kind[r];
-  const int kind = kinfo & 3;
-  const int col = P.rx_col[r];
-  const int ms = P.rx_mset[r];
+  const int kind = rec.x & 7;
+  const int ms = (rec.x >> 3) - 1;
+  const int col = rec.y;
+  const int r = rec.z;
+  const unsigned fs = (unsigned)rec.w;
   const double M = ms >= 0 ? L.Mv[ms] : 1.0;
-  double Pf = 1.0, Pb = 1.0;
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int code = P.rx_fac[r * 8 + s];
-    if (code >= 0) {
-      const double cv = L.c[code & 255];
-      double f = cv;
-      for (int e = 1; e < (code >> 8); ++e) f *= cv;
-      Pf *= f;
-    }
-  }
-#pragma unroll
-  for (int s = 0; s < 4; ++s) {
-    const int code = P.rx_fac[r * 8 + 4 + s];
-    if (code >= 0) {
-      const double cv = L.c[code & 255];
-      double f = cv;
-      for (int e = 1; e < (code >> 8); ++e) f *= cv;
-      Pb *= f;
-    }
-  }
+  const double Pf = L.c[fs & 31] * L.c[(fs >> 5) & 31] * L.c[(fs >> 10) & 31];
+  const double Pb = L.c[(fs >> 15) & 31] * L.c[(fs >> 20) & 31] * L.c[(fs >> 25) & 31];
   double k1, k2, dk1T = 0.0, dk2T = 0.0, dqM = 0.0;
   if (kind <= 1) {
     const double2 a0 = ld2(rk0 + col), a1 = ld2(rk1 + col);
@@ -217,49 +208,73 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
     o.rho = p / (o.Rmix * T);
     roi = o.rho * o.xc;
   }
-  if (sp) L.c[lane] = roi / Wk;
-  tsync();
-#pragma unroll 1
-  for (int s = lane; s < P.nms; s += TEAM) {
-    const int t0 = P.ms_ptr[s], t1 = P.ms_ptr[s + 1];
-    double m = L.c[P.ms_sp[t0]] * P.ms_eff[t0];
-#pragma unroll 1
-    for (int t = t0 + 1; t < t1; ++t) m = m + L.c[P.ms_sp[t]] * P.ms_eff[t];
-    L.Mv[s] = m;
-  }
-  tsync();
+  L.c[lane] = sp ? roi / Wk : (lane == TEAM - 1 ? 1.0 : 0.0);
+  if (lane == 0) L.q[P.nr] = 0.0;
   const int T_i = (int)T;
   const double Td = T - (double)T_i;
   const double* __restrict__ rk0 = P.tabk + (long)(T_i - P.Tk_min) * P.wk;
   const double* __restrict__ rk1 = rk0 + P.wk;
-  if (jac) {
+  // thermo rows and the first reaction record do not depend on the sums below: issue them now
+  const double* __restrict__ rh0 = P.tabh + (long)(T_i - P.Th_min) * (2 * ns);
+  const double* __restrict__ rh1 = rh0 + 2 * ns;
+  double2 th0 = make_double2(0.0, 0.0), th1 = th0;
+  if (sp) {
+    th0 = ld2(rh0 + 2 * lane);
+    th1 = ld2(rh1 + 2 * lane);
+  }
+  int4 rec = P.rrec[lane];
+  tsync();
+  // third-body sums M_s = sum_j eff_sj c_j, lane s
+  if (lane < P.nms) {
+    const int nms = P.nms;
+    double m = 0.0;
+#pragma unroll 8
+    for (int j = 0; j < ns; ++j) m = m + (double)P.effT[j * nms + lane] * L.c[j];
+    L.Mv[lane] = m;
+  }
+  tsync();
+  // rates of progress, one reaction per lane and round; the next round's record is loaded ahead
 #pragma unroll 1
-    for (int r = lane; r < P.nr; r += TEAM) reaction<true>(P, r, rk0, rk1, Td, L);
-  } else {
+  for (int i = 0; i < P.R; ++i) {
+    const int4 cur = rec;
+    if (i + 1 < P.R) rec = P.rrec[(i + 1) * TEAM + lane];
+    if ((cur.x & 7) != 7) {
+      if (jac) reaction<true>(cur, rk0, rk1, Td, L);
+      else reaction<false>(cur, rk0, rk1, Td, L);
+    }
+  }
+  tsync();
+  // net production: chunk sums in reaction order, then the chunks of species k in order
 #pragma unroll 1
-    for (int r = lane; r < P.nr; r += TEAM) reaction<false>(P, r, rk0, rk1, Td, L);
+  for (int c = 0; c < P.C; ++c) {
+    const int2* __restrict__ ct = P.cterm + (long)c * CW * TEAM + lane;
+    int2 tm[CW];
+#pragma unroll
+    for (int t = 0; t < CW; ++t) tm[t] = ct[t * TEAM];
+    double acc = 0.0;
+#pragma unroll
+    for (int t = 0; t < CW; ++t) acc = acc + (double)__int_as_float(tm[t].y) * L.q[tm[t].x];
+    L.part[c * TEAM + lane] = acc;
   }
   tsync();
   double w = 0.0;
   if (sp) {
-    const int t0 = P.wd_ptr[lane], t1 = P.wd_ptr[lane + 1];
-    if (t1 > t0) {
-      double acc = P.wd_coef[t0] * L.q[P.wd_rx[t0]];
+    double acc = 0.0;
 #pragma unroll 1
-      for (int t = t0 + 1; t < t1; ++t) acc = acc + P.wd_coef[t] * L.q[P.wd_rx[t]];
-      w = Wk * acc;
+    for (int i = 0; i < P.MAXP; ++i) {
+      const int slot = P.kpart[i * TEAM + lane];
+      if (slot < 0) break;
+      acc = acc + L.part[slot];
     }
+    w = Wk * acc;
   }
   o.w = w;
-  const double* __restrict__ rh0 = P.tabh + (long)(T_i - P.Th_min) * (2 * ns);
-  const double* __restrict__ rh1 = rh0 + 2 * ns;
   o.cp = o.h = o.dcp = o.dh = 0.0;
   if (sp) {
-    const double2 t0 = ld2(rh0 + 2 * lane), t1 = ld2(rh1 + 2 * lane);
-    o.cp = t0.x + (t1.x - t0.x) * Td;
-    o.h = t0.y + (t1.y - t0.y) * Td;
-    o.dcp = t1.x - t0.x;
-    o.dh = t1.y - t0.y;
+    o.cp = th0.x + (th1.x - th0.x) * Td;
+    o.h = th0.y + (th1.y - th0.y) * Td;
+    o.dcp = th1.x - th0.x;
+    o.dh = th1.y - th0.y;
   }
   double FT;
   if (mode == 0) {
@@ -378,9 +393,17 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
   }
 }
 
-// LU of I - c J with partial pivoting in LDS, one row per lane (stride 33, conflict-free); the
-// pivot row is read by every lane as an LDS broadcast.  Returns this lane's elimination step.
-__device__ __forceinline__ int team_lu(double c, int nz, const Lds& L, const double* __restrict__ Jg) {
+// Inverse of A = I - c J by in-place Gauss-Jordan elimination, one row per lane in LDS (stride 33,
+// conflict-free), partial pivoting without row exchanges.  Step k takes the unused row p with the
+// largest |a_pk| (team argmax; ties to the lower lane) and, with the pivot row read by every lane
+// as an LDS broadcast,
+//   row p:        a_pj <- a_pj / a_pk (j != k),   a_pk <- 1 / a_pk
+//   rows i != p:  a_ij <- a_ij - a_ik a_pj (j != k, a_pj scaled),   a_ik <- -a_ik / a_pk
+// (the column-k storage takes over the identity column p, as in the classic in-place scheme).
+// Afterwards lane i holds row step(i) of A^-1 with its columns in pivot order:
+// A^-1[step(i)][p_m] = a_im, where p_m is the row chosen at step m.  Rows and columns beyond nz
+// are the identity and take no part.  Returns step(this lane).
+__device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const double* __restrict__ Jg) {
   const int lane = threadIdx.x & (TEAM - 1);
   double* arow = L.lu + lane * 33;
   const double* __restrict__ jrow = Jg + lane * 33;
@@ -388,7 +411,7 @@ __device__ __forceinline__ int team_lu(double c, int nz, const Lds& L, const dou
   int mystep = -1;
   tsync();
 #pragma unroll 1
-  for (int k = 0; k < TEAM; ++k) {
+  for (int k = 0; k < nz; ++k) {
     double best = mystep < 0 ? fabs(arow[k]) : -1.0;
     int bl = lane;
 #pragma unroll
@@ -397,36 +420,41 @@ __device__ __forceinline__ int team_lu(double c, int nz, const Lds& L, const dou
       const int ol = __shfl_xor(bl, m, TEAM);
       if (ob > best || (ob == best && ol < bl)) { best = ob; bl = ol; }
     }
-    if (lane == bl) mystep = k;
-    if (lane == 0) L.piv[k] = bl;
+    const bool me = lane == bl;
+    if (me) mystep = k;
     const double* prow = L.lu + bl * 33;
-    if (mystep < 0) {
-      const double l = arow[k] / prow[k];
-      arow[k] = l;
-#pragma unroll 1
-      for (int j = k + 1; j < TEAM; ++j) arow[j] -= l * prow[j];
+    const double inv = 1.0 / prow[k];
+    const double f = arow[k];
+#pragma unroll 4
+    for (int j = 0; j < nz; ++j) {
+      if (j == k) continue;
+      const double pj = prow[j] * inv;
+      arow[j] = me ? pj : arow[j] - f * pj;
     }
+    arow[k] = me ? inv : -f * inv;
     tsync();
   }
-  return mystep;
+  return mystep < 0 ? lane : mystep;
 }
 
-// Solve (I - cJ) x = b with the factors above; b and x are this lane's component.
-__device__ __forceinline__ double team_solve(int mystep, double b, const Lds& L) {
+// x = A^-1 b with the rows above; b and x are this lane's component.  b is placed in pivot order
+// (vb[step(l)] = b_l), each lane forms its dot product, and the result goes to its step's slot.
+__device__ __forceinline__ double team_solve(int mystep, double b, int nz, const Lds& L) {
   const int lane = threadIdx.x & (TEAM - 1);
   const double* arow = L.lu + lane * 33;
-#pragma unroll 1
-  for (int k = 0; k < TEAM; ++k) {
-    const double zk = tbc(b, L.piv[k]);
-    if (mystep > k) b -= arow[k] * zk;
+  L.vb[mystep] = b;
+  tsync();
+  double y = 0.0;
+  if (lane < nz) {
+#pragma unroll 8
+    for (int m = 0; m < nz; ++m) y = y + arow[m] * L.vb[m];
+  } else {
+    y = b;
   }
-  double s = 0.0, x = 0.0;
-#pragma unroll 1
-  for (int k = TEAM - 1; k >= 0; --k) {
-    const double xk = tbc((b - s) / arow[k], L.piv[k]);
-    if (mystep < k) s += arow[k] * xk;
-    if (lane == k) x = xk;
-  }
+  L.vx[mystep] = y;
+  tsync();
+  const double x = L.vx[lane];
+  tsync();
   return x;
 }
 
@@ -472,8 +500,14 @@ __device__ __forceinline__ void change_D(double (&D)[MAXO + 3], int order, doubl
   }
 }
 
-// One cell; mirrors bdf_cell of oracle/oracle_bdf.c.  Returns 0 ok, 1 step underflow, 2 too many
-// steps, 3 invalid state.
+// One cell; mirrors bdf_cell of oracle/oracle_bdf.c decision for decision.  Returns 0 ok, 1 step
+// underflow, 2 too many steps, 3 invalid state.
+//
+// The integrator is written as a state machine around a single right-hand-side evaluation (and a
+// single Jacobian, inverse and solve site): every place where the method needs f(x) sets x and the
+// phase that consumes the result.  One copy of the evaluation keeps the kernel's code and register
+// footprint small, and two teams of a wavefront that both need an evaluation, whatever their
+// phase, run it together.
 __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p, double& y, int (&st)[8], const Lds& L,
                                           double* __restrict__ Jg) {
   const int lane = threadIdx.x & (TEAM - 1);
@@ -484,182 +518,238 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
 #pragma unroll
   for (int k = 1; k <= MAXO; ++k) gamma[k] = gamma[k - 1] + 1.0 / k;
   const double newton_tol = fmax(10.0 * 2.220446049250313e-16 / rtol, fmin(0.03, sqrt(rtol)));
-  RhsOut o;
-  o = team_rhs(P, O.mode, p, y, L, false);
-  if (!o.ok) return 3;
-  st[ST_RHS]++;
-  const double f0 = o.F;
-  double sc = atol + fabs(y) * rtol;
-  const double d0 = team_rms(y, sc, nz), d1 = team_rms(f0, sc, nz);
-  double h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
-  h0 = fmin(h0, t_end);
-  o = team_rhs(P, O.mode, p, y + h0 * f0, L, false);
-  if (!o.ok) return 3;
-  st[ST_RHS]++;
-  const double d2 = team_rms(o.F - f0, sc, nz) / h0;
-  const double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? fmax(1e-6, h0 * 1e-3) : sqrt(0.01 / fmax(d1, d2));
-  double h_abs = fmin(100.0 * h0, fmin(h1, t_end));
 
+  enum { PH_F0, PH_F1, PH_J0, PH_NEWTON, PH_JREF };
+  enum { GO_EVAL, GO_STEP, GO_ATTEMPT, GO_NEWTON, GO_HALVE, GO_ACCEPT };
+  // start-up values
+  double f0 = 0.0, sc = 0.0, h0 = 0.0, d1 = 0.0;
+  // method state
   double D[MAXO + 3];
 #pragma unroll
   for (int j = 0; j < MAXO + 3; ++j) D[j] = 0.0;
-  D[0] = y;
-  D[1] = f0 * h_abs;
   int order = 1, n_equal = 0;
   bool have_lu = false, current_jac = false;
   int mystep = lane;
-  // Jacobian at the initial state
-  o = team_rhs(P, O.mode, p, y, L, true);
-  if (!o.ok) return 3;
-  team_jac(P, O.mode, y, L, o, Jg);
-  st[ST_JAC]++;
-  double t = 0.0;
+  double t = 0.0, h_abs = 0.0, min_step = 0.0;
+  // current step attempt and Newton iteration
+  double t_new = 0.0, ypred = 0.0, psi = 0.0, scale = 0.0, cc = 0.0, ynew = 0.0, d = 0.0;
+  double dy_norm_old = -1.0, err_norm = 0.0, safety = 0.9;
+  int k_it = 0;
 
-  while (t < t_end) {
-    if (st[ST_STEPS] >= O.max_steps) return 2;
-    const double min_step = 10.0 * fabs(nextafter(t, INFINITY) - t);
-    if (h_abs < min_step) {
-      change_D(D, order, min_step / h_abs);
-      h_abs = min_step;
-      n_equal = 0;
-    }
-    double ynew = 0.0, d = 0.0, t_new = t, err_norm = 0.0, safety = 0.9;
-    bool accepted = false;
-    while (!accepted) {
-      if (h_abs < min_step) return 1;
-      t_new = t + h_abs;
-      if (t_new - t_end > 0) {
-        t_new = t_end;
-        change_D(D, order, fabs(t_new - t) / h_abs);
-        n_equal = 0;
-        have_lu = false;
+  int phase = PH_F0;
+  double xe = y;
+  bool wj = false;
+  while (true) {
+    const RhsOut o = team_rhs(P, O.mode, p, xe, L, wj);
+    if (wj) {
+      if (o.ok) {
+        team_jac(P, O.mode, xe, L, o, Jg);
+        st[ST_JAC]++;
       }
-      const double h = t_new - t;
-      h_abs = fabs(h);
-      double ypred = 0.0, psi = 0.0;
-#pragma unroll
-      for (int j = 0; j <= MAXO; ++j)
-        if (j <= order) ypred += D[j];
-#pragma unroll
-      for (int j = 1; j <= MAXO; ++j)
-        if (j <= order) psi += D[j] * gamma[j];
-      double alpha_o = 0.0;
-#pragma unroll
-      for (int j = 1; j <= MAXO; ++j)
-        if (j == order) alpha_o = gamma[j];
-      psi /= alpha_o;
-      const double scale = atol + rtol * fabs(ypred);
-      const double cc = h / alpha_o;
-      bool converged = false;
-      int n_iter = 0;
-      while (!converged) {
-        if (!have_lu) {
-          mystep = team_lu(cc, nz, L, Jg);
-          st[ST_LU]++;
-          have_lu = true;
-        }
-        ynew = ypred;
-        d = 0.0;
-        double dy_norm_old = -1.0;
-        int k;
-        for (k = 0; k < NEWTON_MAXITER; ++k) {
-          o = team_rhs(P, O.mode, p, ynew, L, false);
-          const bool ok = o.ok;
-          st[ST_RHS]++;
-          if (!ok || !isfinite(tsum(o.F))) break;
-          const double dy = team_solve(mystep, cc * o.F - psi - d, L);
-          st[ST_NEWTON]++;
-          const double dy_norm = team_rms(dy, scale, nz);
-          const double rate = dy_norm_old >= 0.0 ? dy_norm / dy_norm_old : -1.0;
-          if (rate >= 0.0 && (rate >= 1.0 || pow(rate, (double)(NEWTON_MAXITER - k)) / (1.0 - rate) * dy_norm > newton_tol))
-            break;
+    } else if (o.ok || phase == PH_NEWTON) {
+      st[ST_RHS]++;
+    }
+    int go = GO_EVAL;
+    if (phase == PH_F0) {
+      if (!o.ok) return 3;
+      f0 = o.F;
+      sc = atol + fabs(y) * rtol;
+      const double d0 = team_rms(y, sc, nz);
+      d1 = team_rms(f0, sc, nz);
+      h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
+      h0 = fmin(h0, t_end);
+      xe = y + h0 * f0;
+      phase = PH_F1;
+    } else if (phase == PH_F1) {
+      if (!o.ok) return 3;
+      const double d2 = team_rms(o.F - f0, sc, nz) / h0;
+      const double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? fmax(1e-6, h0 * 1e-3) : sqrt(0.01 / fmax(d1, d2));
+      h_abs = fmin(100.0 * h0, fmin(h1, t_end));
+      D[0] = y;
+      D[1] = f0 * h_abs;
+      // Jacobian at the initial state
+      xe = y;
+      wj = true;
+      phase = PH_J0;
+    } else if (phase == PH_J0) {
+      if (!o.ok) return 3;
+      t = 0.0;
+      go = GO_STEP;
+    } else if (phase == PH_JREF) {
+      if (!o.ok) {
+        go = GO_HALVE;
+      } else {
+        have_lu = false;
+        current_jac = true;
+        go = GO_NEWTON;
+      }
+    } else {  // PH_NEWTON: f at the current iterate ynew
+      bool conv = false, fail = false;
+      int n_iter = k_it + 1;
+      if (!o.ok || !isfinite(tsum(o.F))) {
+        fail = true;
+      } else {
+        const double dy = team_solve(mystep, cc * o.F - psi - d, nz, L);
+        st[ST_NEWTON]++;
+        const double dy_norm = team_rms(dy, scale, nz);
+        const double rate = dy_norm_old >= 0.0 ? dy_norm / dy_norm_old : -1.0;
+        if (rate >= 0.0 && (rate >= 1.0 || pow(rate, (double)(NEWTON_MAXITER - k_it)) / (1.0 - rate) * dy_norm > newton_tol)) {
+          fail = true;
+        } else {
           ynew += dy;
           d += dy;
           if (dy_norm == 0.0 || (rate >= 0.0 && rate / (1.0 - rate) * dy_norm < newton_tol)) {
-            converged = true;
-            break;
+            conv = true;
+          } else {
+            dy_norm_old = dy_norm;
+            ++k_it;
+            if (k_it < NEWTON_MAXITER) {
+              xe = ynew;  // next Newton iteration
+            } else {
+              fail = true;
+              n_iter = k_it + 1;
+            }
           }
-          dy_norm_old = dy_norm;
-        }
-        n_iter = k + 1;
-        if (!converged) {
-          if (current_jac) break;
-          o = team_rhs(P, O.mode, p, ypred, L, true);
-          if (!o.ok) break;
-          team_jac(P, O.mode, ypred, L, o, Jg);
-          st[ST_JAC]++;
-          have_lu = false;
-          current_jac = true;
         }
       }
-      if (!converged) {
+      if (fail) {
+        if (!current_jac) {
+          xe = ypred;  // refresh the Jacobian at the predictor and iterate again
+          wj = true;
+          phase = PH_JREF;
+        } else {
+          go = GO_HALVE;
+        }
+      } else if (conv) {
+        safety = 0.9 * (2 * NEWTON_MAXITER + 1) / (double)(2 * NEWTON_MAXITER + n_iter);
+        const double scale2 = atol + rtol * fabs(ynew);
+        sc = scale2;
+        err_norm = team_rms(d / (order + 1), scale2, nz);
+        if (err_norm > 1.0) {
+          st[ST_REJECT]++;
+          const double factor = fmax(MIN_FACTOR, safety * pow(err_norm, -1.0 / (order + 1)));
+          h_abs *= factor;
+          change_D(D, order, factor);
+          n_equal = 0;
+          go = GO_ATTEMPT;
+        } else {
+          go = GO_ACCEPT;
+        }
+      }
+    }
+    // control flow between evaluations
+    while (go != GO_EVAL) {
+      if (go == GO_HALVE) {  // Newton failed with a current Jacobian
         st[ST_NFAIL]++;
         h_abs *= 0.5;
         change_D(D, order, 0.5);
         n_equal = 0;
         have_lu = false;
-        continue;
-      }
-      safety = 0.9 * (2 * NEWTON_MAXITER + 1) / (double)(2 * NEWTON_MAXITER + n_iter);
-      const double scale2 = atol + rtol * fabs(ynew);
-      sc = scale2;
-      err_norm = team_rms(d / (order + 1), scale2, nz);
-      if (err_norm > 1.0) {
-        st[ST_REJECT]++;
-        const double factor = fmax(MIN_FACTOR, safety * pow(err_norm, -1.0 / (order + 1)));
+        go = GO_ATTEMPT;
+      } else if (go == GO_ACCEPT) {
+        st[ST_STEPS]++;
+        n_equal++;
+        t = t_new;
+        y = ynew;
+        current_jac = false;
+        // D^{j+1} y_n = D^j y_n - D^j y_{n-1}, d = D^{order+1} y_n
+        {
+          double Dop1 = 0.0;
+#pragma unroll
+          for (int j = 0; j < MAXO + 3; ++j)
+            if (j == order + 1) Dop1 = D[j];
+#pragma unroll
+          for (int j = 0; j < MAXO + 3; ++j) {
+            if (j == order + 2) D[j] = d - Dop1;
+            if (j == order + 1) D[j] = d;
+          }
+#pragma unroll
+          for (int j = MAXO; j >= 0; --j)
+            if (j <= order) D[j] += D[j + 1];
+        }
+        go = GO_STEP;
+        if (n_equal < order + 1) continue;
+        double Dm = 0.0, Dp = 0.0;
+#pragma unroll
+        for (int j = 0; j < MAXO + 3; ++j) {
+          if (j == order) Dm = D[j];
+          if (j == order + 2) Dp = D[j];
+        }
+        const double em = order > 1 ? team_rms(Dm / order, sc, nz) : INFINITY;
+        const double ep = order < MAXO ? team_rms(Dp / (order + 2), sc, nz) : INFINITY;
+        const double norms[3] = {em, err_norm, ep};
+        double best = -1.0;
+        int which = 1;
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+          const double fct = norms[q] == 0.0 ? INFINITY : pow(norms[q], -1.0 / (order + q));
+          if (fct > best) { best = fct; which = q; }
+        }
+        order += which - 1;
+        const double factor = fmin(MAX_FACTOR, safety * best);
         h_abs *= factor;
         change_D(D, order, factor);
         n_equal = 0;
-      } else {
-        accepted = true;
+        have_lu = false;
+      } else if (go == GO_STEP) {
+        if (!(t < t_end)) {
+          st[ST_ORDER] = order;
+          return 0;
+        }
+        if (st[ST_STEPS] >= O.max_steps) return 2;
+        min_step = 10.0 * fabs(nextafter(t, INFINITY) - t);
+        if (h_abs < min_step) {
+          change_D(D, order, min_step / h_abs);
+          h_abs = min_step;
+          n_equal = 0;
+        }
+        safety = 0.9;
+        err_norm = 0.0;
+        go = GO_ATTEMPT;
+      } else if (go == GO_ATTEMPT) {
+        if (h_abs < min_step) return 1;
+        t_new = t + h_abs;
+        if (t_new - t_end > 0) {
+          t_new = t_end;
+          change_D(D, order, fabs(t_new - t) / h_abs);
+          n_equal = 0;
+          have_lu = false;
+        }
+        const double h = t_new - t;
+        h_abs = fabs(h);
+        ypred = 0.0;
+        psi = 0.0;
+#pragma unroll
+        for (int j = 0; j <= MAXO; ++j)
+          if (j <= order) ypred += D[j];
+#pragma unroll
+        for (int j = 1; j <= MAXO; ++j)
+          if (j <= order) psi += D[j] * gamma[j];
+        double alpha_o = 0.0;
+#pragma unroll
+        for (int j = 1; j <= MAXO; ++j)
+          if (j == order) alpha_o = gamma[j];
+        psi /= alpha_o;
+        scale = atol + rtol * fabs(ypred);
+        cc = h / alpha_o;
+        go = GO_NEWTON;
+      } else {  // GO_NEWTON: (re)start the Newton iteration from the predictor
+        if (!have_lu) {
+          mystep = team_inv(cc, nz, L, Jg);
+          st[ST_LU]++;
+          have_lu = true;
+        }
+        ynew = ypred;
+        d = 0.0;
+        dy_norm_old = -1.0;
+        k_it = 0;
+        xe = ynew;
+        wj = false;
+        phase = PH_NEWTON;
+        go = GO_EVAL;
       }
     }
-    st[ST_STEPS]++;
-    n_equal++;
-    t = t_new;
-    y = ynew;
-    current_jac = false;
-    // D^{j+1} y_n = D^j y_n - D^j y_{n-1}, d = D^{order+1} y_n
-    {
-      double Dop1 = 0.0;
-#pragma unroll
-      for (int j = 0; j < MAXO + 3; ++j)
-        if (j == order + 1) Dop1 = D[j];
-#pragma unroll
-      for (int j = 0; j < MAXO + 3; ++j) {
-        if (j == order + 2) D[j] = d - Dop1;
-        if (j == order + 1) D[j] = d;
-      }
-#pragma unroll
-      for (int j = MAXO; j >= 0; --j)
-        if (j <= order) D[j] += D[j + 1];
-    }
-    if (n_equal < order + 1) continue;
-    double Dm = 0.0, Dp = 0.0;
-#pragma unroll
-    for (int j = 0; j < MAXO + 3; ++j) {
-      if (j == order) Dm = D[j];
-      if (j == order + 2) Dp = D[j];
-    }
-    const double em = order > 1 ? team_rms(Dm / order, sc, nz) : INFINITY;
-    const double ep = order < MAXO ? team_rms(Dp / (order + 2), sc, nz) : INFINITY;
-    const double norms[3] = {em, err_norm, ep};
-    double best = -1.0;
-    int which = 1;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const double fct = norms[q] == 0.0 ? INFINITY : pow(norms[q], -1.0 / (order + q));
-      if (fct > best) { best = fct; which = q; }
-    }
-    order += which - 1;
-    const double factor = fmin(MAX_FACTOR, safety * best);
-    h_abs *= factor;
-    change_D(D, order, factor);
-    n_equal = 0;
-    have_lu = false;
   }
-  st[ST_ORDER] = order;
-  return 0;
 }
 
 __global__ void zero_counter(int* c) {
@@ -676,7 +766,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x & (TEAM - 1);
   const int team = threadIdx.x / TEAM;
-  const Lds L = carve(smem + (size_t)team * team_bytes, P.nr, P.nms);
+  const Lds L = carve(smem + (size_t)team * team_bytes, P);
   double* __restrict__ Jg = jac_scratch + ((size_t)blockIdx.x * (blockDim.x / TEAM) + team) * (TEAM * 33);
   const int nz = P.ns + 1;
   while (true) {
@@ -711,7 +801,7 @@ __global__ __launch_bounds__(256) void jac_kernel(Prog P, int mode, long n, long
   const int lane = threadIdx.x & (TEAM - 1);
   const int team = threadIdx.x / TEAM;
   const int tpb = blockDim.x / TEAM;
-  const Lds L = carve(smem + (size_t)team * team_bytes, P.nr, P.nms);
+  const Lds L = carve(smem + (size_t)team * team_bytes, P);
   double* __restrict__ Jg = jac_scratch + ((size_t)blockIdx.x * tpb + team) * (TEAM * 33);
   const int nz = P.ns + 1;
   const long stride = (long)gridDim.x * tpb;
@@ -754,12 +844,12 @@ hipError_t flint_reactor_set_max_lds(int bytes) {
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
-size_t flint_reactor_team_bytes(int nr, int nms) { return flint_reactor::team_lds_bytes(nr, nms); }
+size_t flint_reactor_team_bytes(const flint_reactor::Prog& P) { return flint_reactor::team_lds_bytes(P); }
 
 hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reactor::Opts& O, long n, long ld,
                                 double* X, const double* pres, int* stats, int* status, int* counter, int teams_per_block,
                                 int blocks, double* jac_scratch, hipStream_t s) {
-  const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P.nr, P.nms) + 15) & ~size_t(15));
+  const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P) + 15) & ~size_t(15));
   hipLaunchKernelGGL(flint_reactor::zero_counter, dim3(1), dim3(64), 0, s, counter);
   hipLaunchKernelGGL(flint_reactor::bdf_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
                      (size_t)tb * teams_per_block, s, P, O, n, ld, X, pres, stats, status, counter, tb, jac_scratch);
@@ -769,7 +859,7 @@ hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reacto
 hipError_t flint_reactor_jac_launch(const flint_reactor::Prog& P, int mode, long n, long ld, const double* X,
                                     const double* pres, double* F, double* J, int teams_per_block, int blocks,
                                     double* jac_scratch, hipStream_t s) {
-  const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P.nr, P.nms) + 15) & ~size_t(15));
+  const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P) + 15) & ~size_t(15));
   if (J)
     hipLaunchKernelGGL(flint_reactor::jac_kernel<true>, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
                        (size_t)tb * teams_per_block, s, P, mode, n, ld, X, pres, F, J, tb, jac_scratch);
