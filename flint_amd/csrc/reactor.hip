@@ -393,26 +393,30 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
   }
 }
 
-// Inverse of A = I - c J by in-place Gauss-Jordan elimination, one row per lane in LDS (stride 33,
-// conflict-free), partial pivoting without row exchanges.  Step k takes the unused row p with the
-// largest |a_pk| (team argmax; ties to the lower lane) and, with the pivot row read by every lane
-// as an LDS broadcast,
+// Inverse of A = I - c J by Gauss-Jordan elimination with partial pivoting and no row exchanges.
+// Lane i keeps row i in registers for the whole elimination; step k takes the unused row p with
+// the largest |a_pk| (team argmax; ties to the lower lane), lane p publishes its row in LDS and
 //   row p:        a_pj <- a_pj / a_pk (j != k),   a_pk <- 1 / a_pk
 //   rows i != p:  a_ij <- a_ij - a_ik a_pj (j != k, a_pj scaled),   a_ik <- -a_ik / a_pk
 // (the column-k storage takes over the identity column p, as in the classic in-place scheme).
-// Afterwards lane i holds row step(i) of A^-1 with its columns in pivot order:
-// A^-1[step(i)][p_m] = a_im, where p_m is the row chosen at step m.  Rows and columns beyond nz
-// are the identity and take no part.  Returns step(this lane).
+// Afterwards lane i holds row step(i) of A^-1 with its columns in pivot order,
+// A^-1[step(i)][p_m] = a_im (p_m: the row chosen at step m); the rows go to LDS (stride 33) for
+// the solves.  Rows and columns beyond nz are the identity and take no part.  Returns step(lane).
 __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const double* __restrict__ Jg) {
   const int lane = threadIdx.x & (TEAM - 1);
-  double* arow = L.lu + lane * 33;
   const double* __restrict__ jrow = Jg + lane * 33;
-  for (int j = 0; j < TEAM; ++j) arow[j] = (lane == j ? 1.0 : 0.0) - ((lane < nz && j < nz) ? c * jrow[j] : 0.0);
+  double r[TEAM];
+#pragma unroll
+  for (int j = 0; j < TEAM; ++j) r[j] = (lane == j ? 1.0 : 0.0) - ((lane < nz && j < nz) ? c * jrow[j] : 0.0);
+  double* __restrict__ pb = L.vx;  // the pivot row of the current step
   int mystep = -1;
-  tsync();
 #pragma unroll 1
   for (int k = 0; k < nz; ++k) {
-    double best = mystep < 0 ? fabs(arow[k]) : -1.0;
+    double rk = r[0];
+#pragma unroll
+    for (int j = 1; j < TEAM; ++j)
+      if (j == k) rk = r[j];
+    double best = mystep < 0 ? fabs(rk) : -1.0;
     int bl = lane;
 #pragma unroll
     for (int m = 16; m >= 1; m >>= 1) {
@@ -421,19 +425,26 @@ __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const do
       if (ob > best || (ob == best && ol < bl)) { best = ob; bl = ol; }
     }
     const bool me = lane == bl;
-    if (me) mystep = k;
-    const double* prow = L.lu + bl * 33;
-    const double inv = 1.0 / prow[k];
-    const double f = arow[k];
-#pragma unroll 4
-    for (int j = 0; j < nz; ++j) {
-      if (j == k) continue;
-      const double pj = prow[j] * inv;
-      arow[j] = me ? pj : arow[j] - f * pj;
+    if (me) {
+      mystep = k;
+#pragma unroll
+      for (int j = 0; j < TEAM; ++j) pb[j] = r[j];
     }
-    arow[k] = me ? inv : -f * inv;
+    tsync();
+    const double inv = 1.0 / pb[k];
+    const double f = rk;
+#pragma unroll
+    for (int j = 0; j < TEAM; ++j) {
+      const double pj = pb[j] * inv;
+      const double nv = me ? pj : r[j] - f * pj;
+      r[j] = j == k ? (me ? inv : -f * inv) : nv;
+    }
     tsync();
   }
+  double* arow = L.lu + lane * 33;
+#pragma unroll
+  for (int j = 0; j < TEAM; ++j) arow[j] = r[j];
+  tsync();
   return mystep < 0 ? lane : mystep;
 }
 
