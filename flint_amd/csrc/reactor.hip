@@ -36,17 +36,61 @@ constexpr double MIN_FACTOR = 0.2, MAX_FACTOR = 10.0;
 
 enum { ST_STEPS, ST_REJECT, ST_NFAIL, ST_RHS, ST_JAC, ST_LU, ST_NEWTON, ST_ORDER };
 
-// Team sum, bit-identical on every lane: each butterfly level adds the same two rounded values
-// (a + b == b + a exactly).  The register barrier rounds v first; without it the compiler may
-// contract a caller's product into the first addition, fma(x_i, y_i, shfl(x_j y_j)), which differs
-// between partner lanes, and step sizes, orders and branch decisions would then diverge inside
-// one team.
+// Cross-lane exchanges inside a 32-lane team without the LDS crossbar where possible: DPP quad
+// permutations (lane ^ 1, lane ^ 2) and row mirrors (lane i <-> 7 - i within 8, i <-> 15 - i within
+// 16), and a bit-mode swizzle for lane ^ 16 (all verified on gfx950).  Each level pairs disjoint
+// groups, so five levels combine all 32 lanes.
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+template <int CTRL>
+__device__ __forceinline__ int dppi(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+__device__ __forceinline__ double swz16(double v) {
+  const int lo = __builtin_amdgcn_ds_swizzle(__double2loint(v), 0x401F);
+  const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), 0x401F);
+  return __hiloint2double(hi, lo);
+}
+constexpr int DPP_X1 = 0xB1, DPP_X2 = 0x4E, DPP_HMIR = 0x141, DPP_MIR = 0x140;
+
+// Team sum, bit-identical on every lane: each level adds the same two rounded values (a + b ==
+// b + a exactly).  The register barrier rounds v first; without it the compiler may contract a
+// caller's product into the first addition, fma(x_i, y_i, partner's x_j y_j), which differs between
+// partner lanes, and step sizes, orders and branch decisions would then diverge inside one team.
+// Every call site runs with all 32 lanes of the team active.
 __device__ __forceinline__ double tsum(double v) {
   asm volatile("" : "+v"(v));
-#pragma unroll
-  for (int m = 16; m >= 1; m >>= 1) v += __shfl_xor(v, m, TEAM);
+  v += dpp<DPP_X1>(v);
+  v += dpp<DPP_X2>(v);
+  v += dpp<DPP_HMIR>(v);
+  v += dpp<DPP_MIR>(v);
+  v += swz16(v);
   return v;
 }
+
+// Team argmax of (best, lane), ties to the lower lane: the same pairing levels as tsum.
+__device__ __forceinline__ void targmax(double& best, int& bl) {
+#define FLINT_ARGMAX_LEVEL(OPD, OPI)                          \
+  {                                                           \
+    const double ob = OPD;                                    \
+    const int ol = OPI;                                       \
+    if (ob > best || (ob == best && ol < bl)) {               \
+      best = ob;                                              \
+      bl = ol;                                                \
+    }                                                         \
+  }
+  FLINT_ARGMAX_LEVEL(dpp<DPP_X1>(best), dppi<DPP_X1>(bl))
+  FLINT_ARGMAX_LEVEL(dpp<DPP_X2>(best), dppi<DPP_X2>(bl))
+  FLINT_ARGMAX_LEVEL(dpp<DPP_HMIR>(best), dppi<DPP_HMIR>(bl))
+  FLINT_ARGMAX_LEVEL(dpp<DPP_MIR>(best), dppi<DPP_MIR>(bl))
+  FLINT_ARGMAX_LEVEL(swz16(best), __builtin_amdgcn_ds_swizzle(bl, 0x401F))
+#undef FLINT_ARGMAX_LEVEL
+}
+
 __device__ __forceinline__ double tbc(double v, int src) { return __shfl(v, src, TEAM); }
 __device__ __forceinline__ int tbci(int v, int src) { return __shfl(v, src, TEAM); }
 // All LDS traffic of a team stays inside one wavefront: a wavefront-scope fence orders it.
@@ -418,12 +462,7 @@ __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const do
       if (j == k) rk = r[j];
     double best = mystep < 0 ? fabs(rk) : -1.0;
     int bl = lane;
-#pragma unroll
-    for (int m = 16; m >= 1; m >>= 1) {
-      const double ob = __shfl_xor(best, m, TEAM);
-      const int ol = __shfl_xor(bl, m, TEAM);
-      if (ob > best || (ob == best && ol < bl)) { best = ob; bl = ol; }
-    }
+    targmax(best, bl);
     const bool me = lane == bl;
     if (me) {
       mystep = k;
