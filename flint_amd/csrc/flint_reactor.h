@@ -41,6 +41,7 @@ struct Opts {
 }  // namespace flint_reactor
 
 size_t flint_reactor_team_bytes(const flint_reactor::Prog& P);
+size_t flint_reactor_shared_bytes(const flint_reactor::Prog& P);  // block-shared program in LDS
 hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reactor::Opts& O, long n, long ld,
                                 double* X, const double* pres, int* stats, int* status, int* counter,
                                 int teams_per_block, int blocks, double* jac_scratch, hipStream_t s);

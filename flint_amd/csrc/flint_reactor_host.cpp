@@ -19,7 +19,7 @@ struct flint_extra {
   int* d_counter = nullptr;
   double* d_jac = nullptr;  // per-team Jacobian slots (32 x 33 doubles each)
   size_t jac_teams = 0;
-  int teams_per_block = 2;
+  int teams_per_block = 4;
   int blocks = 0;
   bool tuned = false;
   // captured graph of the last flint_reactor_advance_graph call
@@ -181,12 +181,13 @@ static int ensure_extra(flint_mech* m) {
   P.Th_min = m->Th_min;
   P.Th_max = m->Th_max;
   HIPCHK(hipMalloc(&x->d_counter, 64 * sizeof(int)));
-  // default launch: as many two-team blocks as fit in LDS on every CU
+  // default launch: as many four-team blocks as fit in LDS on every CU (the program is shared per block)
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, m->device));
   const size_t tb = (flint_reactor_team_bytes(P) + 15) & ~size_t(15);
   const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
-  const int per_cu = std::max<int>(1, (int)(lds_cu / (tb * x->teams_per_block)));
+  const size_t sh = flint_reactor_shared_bytes(P);
+  const int per_cu = std::max<int>(1, (int)(std::min<size_t>(lds_cu, 160 * 1024) / (sh + tb * x->teams_per_block)));
   x->blocks = prop.multiProcessorCount * std::min(per_cu, 8);
   HIPCHK(flint_reactor_set_max_lds((int)std::min<size_t>(lds_cu, 160 * 1024)));
   return FLINT_OK;
@@ -300,7 +301,7 @@ extern "C" int flint_reactor_autotune(flint_mech* m, long n, long ld, int mode, 
   float best = 1e30f;
   int best_tpb = x->teams_per_block, best_blocks = x->blocks;
   for (int tpb : {1, 2, 4, 8}) {
-    const int fit = (int)(lds_cu / (tb * tpb));
+    const int fit = (int)(lds_cu / (flint_reactor_shared_bytes(x->prog) + tb * tpb));
     if (fit < 1) continue;
     for (int per_cu : {1, 2, 3, 4, 6, 8}) {
       if (per_cu > fit || per_cu * tpb > 16) continue;

@@ -100,6 +100,11 @@ __device__ __forceinline__ void tsync() {
 }
 
 struct Lds {
+  // block-shared copy of the program (read-only after the kernel's first barrier)
+  const int4* rrec;   // [R*32] reaction records
+  const float* effT;  // [ns][nms] third-body efficiencies
+  const int* kpart;   // [MAXP][32] chunk slots of each species
+  // per team
   double* c;    // [32] concentrations; slot 31 (never a species) holds 1 for empty factors
   double* Mv;   // [nms] third-body sums
   double* aux;  // [32] per-lane shared values (Y_k, e_k or h_k)
@@ -119,8 +124,33 @@ __host__ __device__ inline size_t team_lds_bytes(const Prog& P) {
   return (size_t)(32 * 5 + (P.nms > 0 ? P.nms : 1) + (size_t)P.nr + 1 + (size_t)P.C * 32 + 32 * 33) * sizeof(double);
 }
 
-__device__ inline Lds carve(char* base, const Prog& P) {
+// Block-shared program bytes (16-byte multiples): records, efficiencies, chunk slots.
+__host__ __device__ inline size_t shared_prog_bytes(const Prog& P) {
+  const size_t rec = (size_t)P.R * 32 * 16;
+  const size_t eff = (((size_t)P.ns * (P.nms > 0 ? P.nms : 1) * 4) + 15) & ~size_t(15);
+  const size_t kp = (((size_t)P.MAXP * 32 * 4) + 15) & ~size_t(15);
+  return rec + eff + kp;
+}
+
+// Copies the program into the block's LDS (all threads of the block; a block barrier follows).
+__device__ inline void load_shared_prog(const Prog& P, char* base) {
+  const size_t eff_off = (size_t)P.R * 32 * 16;
+  const size_t kp_off = eff_off + ((((size_t)P.ns * (P.nms > 0 ? P.nms : 1) * 4) + 15) & ~size_t(15));
+  int4* rec = reinterpret_cast<int4*>(base);
+  for (int i = threadIdx.x; i < P.R * 32; i += blockDim.x) rec[i] = P.rrec[i];
+  float* eff = reinterpret_cast<float*>(base + eff_off);
+  for (int i = threadIdx.x; i < P.ns * P.nms; i += blockDim.x) eff[i] = P.effT[i];
+  int* kp = reinterpret_cast<int*>(base + kp_off);
+  for (int i = threadIdx.x; i < P.MAXP * 32; i += blockDim.x) kp[i] = P.kpart[i];
+}
+
+__device__ inline Lds carve(char* shared, char* base, const Prog& P) {
   Lds L;
+  const size_t eff_off = (size_t)P.R * 32 * 16;
+  const size_t kp_off = eff_off + ((((size_t)P.ns * (P.nms > 0 ? P.nms : 1) * 4) + 15) & ~size_t(15));
+  L.rrec = reinterpret_cast<const int4*>(shared);
+  L.effT = reinterpret_cast<const float*>(shared + eff_off);
+  L.kpart = reinterpret_cast<const int*>(shared + kp_off);
   double* d = reinterpret_cast<double*>(base);
   L.c = d; d += 32;
   L.aux = d; d += 32;
@@ -266,14 +296,14 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
     th0 = ld2(rh0 + 2 * lane);
     th1 = ld2(rh1 + 2 * lane);
   }
-  int4 rec = P.rrec[lane];
+  int4 rec = L.rrec[lane];
   tsync();
   // third-body sums M_s = sum_j eff_sj c_j, lane s
   if (lane < P.nms) {
     const int nms = P.nms;
     double m = 0.0;
 #pragma unroll 8
-    for (int j = 0; j < ns; ++j) m = m + (double)P.effT[j * nms + lane] * L.c[j];
+    for (int j = 0; j < ns; ++j) m = m + (double)L.effT[j * nms + lane] * L.c[j];
     L.Mv[lane] = m;
   }
   tsync();
@@ -281,7 +311,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
 #pragma unroll 1
   for (int i = 0; i < P.R; ++i) {
     const int4 cur = rec;
-    if (i + 1 < P.R) rec = P.rrec[(i + 1) * TEAM + lane];
+    if (i + 1 < P.R) rec = L.rrec[(i + 1) * TEAM + lane];
     if ((cur.x & 7) != 7) {
       if (jac) reaction<true>(cur, rk0, rk1, Td, L);
       else reaction<false>(cur, rk0, rk1, Td, L);
@@ -306,7 +336,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
     double acc = 0.0;
 #pragma unroll 1
     for (int i = 0; i < P.MAXP; ++i) {
-      const int slot = P.kpart[i * TEAM + lane];
+      const int slot = L.kpart[i * TEAM + lane];
       if (slot < 0) break;
       acc = acc + L.part[slot];
     }
@@ -456,7 +486,7 @@ __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const do
   int mystep = -1;
 #pragma unroll 1
   for (int k = 0; k < nz; ++k) {
-    double rk = r[0];
+    double rk = r[0];  // r[k] by selects: a dynamic register index would move r[] to scratch
 #pragma unroll
     for (int j = 1; j < TEAM; ++j)
       if (j == k) rk = r[j];
@@ -471,13 +501,14 @@ __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const do
     }
     tsync();
     const double inv = 1.0 / pb[k];
-    const double f = rk;
+    // pivot lane: a_pj * inv; other lanes: a_ij - a_ik (a_pj * inv); one fused form for both
+    const double keep = me ? 0.0 : 1.0, fm = me ? -1.0 : rk;
 #pragma unroll
-    for (int j = 0; j < TEAM; ++j) {
-      const double pj = pb[j] * inv;
-      const double nv = me ? pj : r[j] - f * pj;
-      r[j] = j == k ? (me ? inv : -f * inv) : nv;
-    }
+    for (int j = 0; j < TEAM; ++j) r[j] = keep * r[j] - fm * (pb[j] * inv);
+    const double nk = me ? inv : -rk * inv;
+#pragma unroll
+    for (int j = 0; j < TEAM; ++j)
+      if (j == k) r[j] = nk;
     tsync();
   }
   double* arow = L.lu + lane * 33;
@@ -816,7 +847,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x & (TEAM - 1);
   const int team = threadIdx.x / TEAM;
-  const Lds L = carve(smem + (size_t)team * team_bytes, P);
+  const size_t sh = shared_prog_bytes(P);
+  load_shared_prog(P, smem);
+  __syncthreads();
+  const Lds L = carve(smem, smem + sh + (size_t)team * team_bytes, P);
   double* __restrict__ Jg = jac_scratch + ((size_t)blockIdx.x * (blockDim.x / TEAM) + team) * (TEAM * 33);
   const int nz = P.ns + 1;
   while (true) {
@@ -851,7 +885,10 @@ __global__ __launch_bounds__(256) void jac_kernel(Prog P, int mode, long n, long
   const int lane = threadIdx.x & (TEAM - 1);
   const int team = threadIdx.x / TEAM;
   const int tpb = blockDim.x / TEAM;
-  const Lds L = carve(smem + (size_t)team * team_bytes, P);
+  const size_t sh = shared_prog_bytes(P);
+  load_shared_prog(P, smem);
+  __syncthreads();
+  const Lds L = carve(smem, smem + sh + (size_t)team * team_bytes, P);
   double* __restrict__ Jg = jac_scratch + ((size_t)blockIdx.x * tpb + team) * (TEAM * 33);
   const int nz = P.ns + 1;
   const long stride = (long)gridDim.x * tpb;
@@ -895,6 +932,7 @@ hipError_t flint_reactor_set_max_lds(int bytes) {
 }
 
 size_t flint_reactor_team_bytes(const flint_reactor::Prog& P) { return flint_reactor::team_lds_bytes(P); }
+size_t flint_reactor_shared_bytes(const flint_reactor::Prog& P) { return flint_reactor::shared_prog_bytes(P); }
 
 hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reactor::Opts& O, long n, long ld,
                                 double* X, const double* pres, int* stats, int* status, int* counter, int teams_per_block,
@@ -902,7 +940,7 @@ hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reacto
   const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P) + 15) & ~size_t(15));
   hipLaunchKernelGGL(flint_reactor::zero_counter, dim3(1), dim3(64), 0, s, counter);
   hipLaunchKernelGGL(flint_reactor::bdf_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
-                     (size_t)tb * teams_per_block, s, P, O, n, ld, X, pres, stats, status, counter, tb, jac_scratch);
+                     flint_reactor::shared_prog_bytes(P) + (size_t)tb * teams_per_block, s, P, O, n, ld, X, pres, stats, status, counter, tb, jac_scratch);
   return hipGetLastError();
 }
 
@@ -912,9 +950,9 @@ hipError_t flint_reactor_jac_launch(const flint_reactor::Prog& P, int mode, long
   const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P) + 15) & ~size_t(15));
   if (J)
     hipLaunchKernelGGL(flint_reactor::jac_kernel<true>, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
-                       (size_t)tb * teams_per_block, s, P, mode, n, ld, X, pres, F, J, tb, jac_scratch);
+                       flint_reactor::shared_prog_bytes(P) + (size_t)tb * teams_per_block, s, P, mode, n, ld, X, pres, F, J, tb, jac_scratch);
   else
     hipLaunchKernelGGL(flint_reactor::jac_kernel<false>, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
-                       (size_t)tb * teams_per_block, s, P, mode, n, ld, X, pres, F, J, tb, jac_scratch);
+                       flint_reactor::shared_prog_bytes(P) + (size_t)tb * teams_per_block, s, P, mode, n, ld, X, pres, F, J, tb, jac_scratch);
   return hipGetLastError();
 }
