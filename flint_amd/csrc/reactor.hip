@@ -72,16 +72,16 @@ __device__ __forceinline__ double tsum(double v) {
   return v;
 }
 
-// Team argmax of (best, lane), ties to the lower lane: the same pairing levels as tsum.
+// Team argmax of (best, lane), ties to the lower lane: the same pairing levels as tsum, with
+// branch-free selects.
 __device__ __forceinline__ void targmax(double& best, int& bl) {
-#define FLINT_ARGMAX_LEVEL(OPD, OPI)                          \
-  {                                                           \
-    const double ob = OPD;                                    \
-    const int ol = OPI;                                       \
-    if (ob > best || (ob == best && ol < bl)) {               \
-      best = ob;                                              \
-      bl = ol;                                                \
-    }                                                         \
+#define FLINT_ARGMAX_LEVEL(OPD, OPI)                              \
+  {                                                               \
+    const double ob = OPD;                                        \
+    const int ol = OPI;                                           \
+    const bool take = (ob > best) | ((ob == best) & (ol < bl));   \
+    best = take ? ob : best;                                      \
+    bl = take ? ol : bl;                                          \
   }
   FLINT_ARGMAX_LEVEL(dpp<DPP_X1>(best), dppi<DPP_X1>(bl))
   FLINT_ARGMAX_LEVEL(dpp<DPP_X2>(best), dppi<DPP_X2>(bl))
@@ -473,6 +473,8 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
 //   row p:        a_pj <- a_pj / a_pk (j != k),   a_pk <- 1 / a_pk
 //   rows i != p:  a_ij <- a_ij - a_ik a_pj (j != k, a_pj scaled),   a_ik <- -a_ik / a_pk
 // (the column-k storage takes over the identity column p, as in the classic in-place scheme).
+// The register row is rotated by one column after every step, so that the current column is
+// always r[0]: every register index stays static (a dynamic one would move the row to scratch).
 // Afterwards lane i holds row step(i) of A^-1 with its columns in pivot order,
 // A^-1[step(i)][p_m] = a_im (p_m: the row chosen at step m); the rows go to LDS (stride 33) for
 // the solves.  Rows and columns beyond nz are the identity and take no part.  Returns step(lane).
@@ -482,14 +484,11 @@ __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const do
   double r[TEAM];
 #pragma unroll
   for (int j = 0; j < TEAM; ++j) r[j] = (lane == j ? 1.0 : 0.0) - ((lane < nz && j < nz) ? c * jrow[j] : 0.0);
-  double* __restrict__ pb = L.vx;  // the pivot row of the current step
+  double* __restrict__ pb = L.vx;  // the pivot row of the current step, in rotated order
   int mystep = -1;
 #pragma unroll 1
   for (int k = 0; k < nz; ++k) {
-    double rk = r[0];  // r[k] by selects: a dynamic register index would move r[] to scratch
-#pragma unroll
-    for (int j = 1; j < TEAM; ++j)
-      if (j == k) rk = r[j];
+    const double rk = r[0];
     double best = mystep < 0 ? fabs(rk) : -1.0;
     int bl = lane;
     targmax(best, bl);
@@ -500,20 +499,19 @@ __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const do
       for (int j = 0; j < TEAM; ++j) pb[j] = r[j];
     }
     tsync();
-    const double inv = 1.0 / pb[k];
+    const double inv = 1.0 / pb[0];
     // pivot lane: a_pj * inv; other lanes: a_ij - a_ik (a_pj * inv); one fused form for both
     const double keep = me ? 0.0 : 1.0, fm = me ? -1.0 : rk;
+    const double r0 = me ? inv : -rk * inv;
 #pragma unroll
-    for (int j = 0; j < TEAM; ++j) r[j] = keep * r[j] - fm * (pb[j] * inv);
-    const double nk = me ? inv : -rk * inv;
-#pragma unroll
-    for (int j = 0; j < TEAM; ++j)
-      if (j == k) r[j] = nk;
+    for (int j = 0; j < TEAM - 1; ++j) r[j] = keep * r[j + 1] - fm * (pb[j + 1] * inv);
+    r[TEAM - 1] = r0;
     tsync();
   }
+  // undo the rotation by nz columns: register j holds column (j + nz) mod 32
   double* arow = L.lu + lane * 33;
 #pragma unroll
-  for (int j = 0; j < TEAM; ++j) arow[j] = r[j];
+  for (int j = 0; j < TEAM; ++j) arow[(j + nz) & (TEAM - 1)] = r[j];
   tsync();
   return mystep < 0 ? lane : mystep;
 }
