@@ -25,6 +25,7 @@ struct Prog {
   int C, CW, MAXP;        // production-sum chunk rounds, chunk width, most chunks of one species
   const int2* cterm;      // [C][CW][32] {reaction (nr: empty), coefficient bits} of chunk (round, lane)
   const int* kpart;       // [MAXP][32] partial-sum slots (32*round + lane) of species k in order, -1 none
+  const int4* jterm;      // [nterm] terms of wd_ptr order: {reaction, coefficient bits, mset, factor slots}
   const double* W;        // [ns]
   const double* tabk;
   int Tk_min, Tk_max, wk;

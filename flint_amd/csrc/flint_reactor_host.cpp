@@ -64,7 +64,8 @@ static int upload(flint_extra* x, const T* src, size_t count, const T** dst) {
 //  * the net production sum of species k split into chunks of at most CW terms in reaction order,
 //    chunks spread over (round, lane) slots, and per species the list of its chunk slots.
 static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::vector<int>& rrec,
-                      std::vector<float>& effT, std::vector<int>& cterm, std::vector<int>& kpart) {
+                      std::vector<float>& effT, std::vector<int>& cterm, std::vector<int>& kpart,
+                      std::vector<int>& jterm) {
   const int nr = hp->nr, ns = hp->ns, nms = hp->nms;
   if (ns > 31) return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports ns <= 31");
   std::vector<int> order;
@@ -73,6 +74,7 @@ static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::ve
       if ((hp->rx_kind[r] & 3) == kind) order.push_back(r);
   P.R = (nr + 31) / 32;
   rrec.assign((size_t)P.R * 32 * 4, 0);
+  std::vector<int> rpack(nr, 0);
   for (int s = 0; s < P.R * 32; ++s) {
     int* rec = &rrec[(size_t)s * 4];
     if (s >= nr) {
@@ -99,11 +101,16 @@ static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::ve
       for (int f = 0; f < 3; ++f) packed |= (unsigned)slots[f] << (5 * (3 * side + f));
     }
     rec[3] = (int)packed;
+    rpack[r] = (int)packed;
   }
   const int nmsp = std::max(nms, 1);
   effT.assign((size_t)ns * nmsp, 0.0f);
   for (int s = 0; s < nms; ++s)
-    for (int t = hp->ms_ptr[s]; t < hp->ms_ptr[s + 1]; ++t) effT[(size_t)hp->ms_sp[t] * nmsp + s] = (float)hp->ms_eff[t];
+    for (int t = hp->ms_ptr[s]; t < hp->ms_ptr[s + 1]; ++t) {
+      effT[(size_t)hp->ms_sp[t] * nmsp + s] = (float)hp->ms_eff[t];
+      if ((double)(float)hp->ms_eff[t] != hp->ms_eff[t])
+        return flint_fail(FLINT_EUNSUPPORTED, "non-float32 third-body efficiency");
+    }
   // production-sum chunks
   P.CW = 16;
   std::vector<std::pair<int, int>> chunks;  // (species, first term)
@@ -125,6 +132,15 @@ static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::ve
       std::memcpy(&cterm[at + 1], &coef, sizeof coef);
     }
     slots_of[k].push_back(round * 32 + lane);
+  }
+  jterm.assign((size_t)std::max(hp->nterm, 1) * 4, 0);
+  for (int t = 0; t < hp->nterm; ++t) {
+    const int r = hp->wd_rx[t];
+    const float coef = (float)hp->wd_coef[t];
+    jterm[(size_t)t * 4 + 0] = r;
+    std::memcpy(&jterm[(size_t)t * 4 + 1], &coef, sizeof coef);
+    jterm[(size_t)t * 4 + 2] = hp->rx_mset[r];
+    jterm[(size_t)t * 4 + 3] = rpack[r];
   }
   P.MAXP = 1;
   for (int k = 0; k < ns; ++k) P.MAXP = std::max<int>(P.MAXP, (int)slots_of[k].size());
@@ -153,13 +169,14 @@ static int ensure_extra(flint_mech* m) {
     for (int t = hp->ms_ptr[s]; t < hp->ms_ptr[s + 1]; ++t) effd[(size_t)s * 32 + hp->ms_sp[t]] = hp->ms_eff[t];
   std::vector<int> rrec, kpart;
   std::vector<float> effT;
-  std::vector<int> cterm;
-  if (int rc = encode_rhs(hp, P, rrec, effT, cterm, kpart)) return rc;
+  std::vector<int> cterm, jterm;
+  if (int rc = encode_rhs(hp, P, rrec, effT, cterm, kpart, jterm)) return rc;
   int rc = 0;
   rc |= upload(x, reinterpret_cast<const int4*>(rrec.data()), rrec.size() / 4, &P.rrec);
   rc |= upload(x, effT.data(), effT.size(), &P.effT);
   rc |= upload(x, reinterpret_cast<const int2*>(cterm.data()), cterm.size() / 2, &P.cterm);
   rc |= upload(x, kpart.data(), kpart.size(), &P.kpart);
+  rc |= upload(x, reinterpret_cast<const int4*>(jterm.data()), jterm.size() / 4, &P.jterm);
   rc |= upload(x, hp->rx_kind, hp->nr, &P.rx_kind);
   rc |= upload(x, hp->rx_col, hp->nr, &P.rx_col);
   rc |= upload(x, hp->rx_mset, hp->nr, &P.rx_mset);

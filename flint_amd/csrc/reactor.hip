@@ -113,11 +113,7 @@ struct Lds {
   double* part; // [C*32] partial production sums of the chunks
   double* vb;   // [32] right-hand side of a Newton correction, in pivot order
   double* vx;   // [32] its solution
-  double* lu;   // [32][33] inverse of I - cJ (row = lane, padded)
-  double* kfe;  // [nr] effective forward coefficient (incl. M or falloff)   } alias lu while the
-  double* kbe;  // [nr]                                                     } Jacobian is built
-  double* Dm;   // [nr] dq/dM                                               } (the inverse is
-  double* dqT;  // [nr] dq/dT at fixed concentrations                       }  rebuilt right after)
+  double* lu;   // [32][33] inverse of I - cJ (row = lane, padded); Jacobian rows while one is built
 };
 
 __host__ __device__ inline size_t team_lds_bytes(const Prog& P) {
@@ -161,10 +157,6 @@ __device__ inline Lds carve(char* shared, char* base, const Prog& P) {
   L.q = d; d += P.nr + 1;
   L.part = d; d += P.C * 32;
   L.lu = d;
-  L.kfe = L.lu;
-  L.kbe = L.lu + P.nr;
-  L.Dm = L.lu + 2 * P.nr;
-  L.dqT = L.lu + 3 * P.nr;
   return L;
 }
 
@@ -180,7 +172,7 @@ struct RhsOut {
 // 3 Lindemann), Lib_Chemistry_falloff.f90:12-80 for the falloff forms.
 template <bool JAC>
 __device__ __forceinline__ void reaction(const int4 rec, const double* __restrict__ rk0, const double* __restrict__ rk1,
-                                         double Td, const Lds& L) {
+                                         double Td, const Lds& L, double* __restrict__ jx, int nr) {
   const int kind = rec.x & 7;
   const int ms = (rec.x >> 3) - 1;
   const int col = rec.y;
@@ -247,17 +239,18 @@ __device__ __forceinline__ void reaction(const int4 rec, const double* __restric
   }
   L.q[r] = k1 * Pf - k2 * Pb;
   if (JAC) {
-    L.kfe[r] = k1;
-    L.kbe[r] = k2;
-    L.Dm[r] = dqM;
-    L.dqT[r] = dk1T * Pf - dk2T * Pb;
+    jx[r] = k1;
+    jx[nr + r] = k2;
+    jx[2 * nr + r] = dqM;
+    jx[3 * nr + r] = dk1T * Pf - dk2T * Pb;
   }
 }
 
 // Team right-hand side; o.ok is false (team-uniform) when T leaves the tables.  With jac the
 // per-reaction values the Jacobian needs are staged in LDS as well.  Kept out of line: it is called
 // from five places of the integrator and inlining every copy would multiply its register demand.
-__device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, double x, const Lds& L, bool jac) {
+__device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, double x, const Lds& L, bool jac,
+                                          double* __restrict__ jx) {
   RhsOut o;
   o.ok = false;
   const int lane = threadIdx.x & (TEAM - 1);
@@ -313,8 +306,8 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
     const int4 cur = rec;
     if (i + 1 < P.R) rec = L.rrec[(i + 1) * TEAM + lane];
     if ((cur.x & 7) != 7) {
-      if (jac) reaction<true>(cur, rk0, rk1, Td, L);
-      else reaction<false>(cur, rk0, rk1, Td, L);
+      if (jac) reaction<true>(cur, rk0, rk1, Td, L, jx, P.nr);
+      else reaction<false>(cur, rk0, rk1, Td, L, jx, P.nr);
     }
   }
   tsync();
@@ -367,83 +360,89 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   return o;
 }
 
-// Jacobian of the team's cell, left in the team's global slot Jg (row k = lane k, stride 33),
-// after team_rhs<true>.
+// Jacobian of the team's cell, after team_rhs with jac (which left the per-reaction effective
+// coefficients k_f, k_b, dq/dM and dq/dT in the team's global slot Jg: [4][nr]).  Row k is
+// accumulated by lane k in LDS (the inverse's area: a new Jacobian always gets a new inverse) from
+// the species' production terms in reaction order, then stored to Jg (row k = lane k, stride 33).
 __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, const Lds& L, const RhsOut o,
-                                                   double* __restrict__ Jg) {
+                                         double* __restrict__ Jg) {
   const int lane = threadIdx.x & (TEAM - 1);
-  const int ns = P.ns, nz = ns + 1;
+  const int ns = P.ns, nz = ns + 1, nr = P.nr;
   const bool sp = lane < ns;
   const double T = tbc(x, ns);
   const double FT = tbc(o.F, ns);
-  double* myrow = Jg + lane * 33;
-  for (int j = 0; j < TEAM; ++j) myrow[j] = 0.0;
+  double* row = L.lu + lane * 33;
+#pragma unroll
+  for (int j = 0; j < TEAM; ++j) row[j] = 0.0;
   L.g[lane] = o.g;
   if (mode == 1) L.aux[lane] = o.xc;
+  // the reaction coefficients in Jg were stored by other lanes of this wavefront
+  __threadfence_block();
   tsync();
   double bT = 0.0;
   const double Wk = sp ? P.W[lane] : 1.0;
   if (sp) {
     const int t0 = P.wd_ptr[lane], t1 = P.wd_ptr[lane + 1];
-#pragma unroll 1
+#pragma unroll 2
     for (int t = t0; t < t1; ++t) {
-      const int r = P.wd_rx[t];
-      const double coef = P.wd_coef[t];
-      for (int side = 0; side < 2; ++side) {
-        const double kk = side == 0 ? L.kfe[r] : -L.kbe[r];
-        for (int s = 0; s < 4; ++s) {
-          const int code = P.rx_fac[r * 8 + side * 4 + s];
-          if (code < 0) continue;
-          const int j = code & 255, pw = code >> 8;
-          double d = kk * pw;
-          for (int e = 1; e < pw; ++e) d *= L.c[j];
-          for (int u = 0; u < 4; ++u) {
-            const int cu = P.rx_fac[r * 8 + side * 4 + u];
-            if (u == s || cu < 0) continue;
-            for (int e = 0; e < (cu >> 8); ++e) d *= L.c[cu & 255];
-          }
-          myrow[j] += coef * d;
-        }
-      }
-      const int ms = P.rx_mset[r];
+      const int4 tr = P.jterm[t];
+      const int r = tr.x;
+      const double coef = (double)__int_as_float(tr.y);
+      const int ms = tr.z;
+      const unsigned fs = (unsigned)tr.w;
+      const double kf = Jg[r], kb = Jg[nr + r];
+      // d(k c_a c_b c_c)/dc at each factor slot; a repeated species gets one term per slot
+      // (the power rule), slot 31 is the constant 1 and has no column
+      const int a0 = fs & 31, a1 = (fs >> 5) & 31, a2 = (fs >> 10) & 31;
+      const int b0 = (fs >> 15) & 31, b1 = (fs >> 20) & 31, b2 = (fs >> 25) & 31;
+      const double ca0 = L.c[a0], ca1 = L.c[a1], ca2 = L.c[a2];
+      const double cb0 = L.c[b0], cb1 = L.c[b1], cb2 = L.c[b2];
+      const double f = coef * kf, g = coef * kb;
+      if (a0 != 31) row[a0] += f * (ca1 * ca2);
+      if (a1 != 31) row[a1] += f * (ca0 * ca2);
+      if (a2 != 31) row[a2] += f * (ca0 * ca1);
+      if (b0 != 31) row[b0] -= g * (cb1 * cb2);
+      if (b1 != 31) row[b1] -= g * (cb0 * cb2);
+      if (b2 != 31) row[b2] -= g * (cb0 * cb1);
       if (ms >= 0) {
-        const double Dv = coef * L.Dm[r];
-        const double* __restrict__ ef = P.effd + ms * TEAM;
-        for (int j = 0; j < ns; ++j) myrow[j] += Dv * ef[j];
+        const double Dv = coef * Jg[2 * nr + r];
+        for (int j = 0; j < ns; ++j) row[j] += Dv * (double)L.effT[j * P.nms + ms];
       }
-      bT += coef * L.dqT[r];
+      bT += coef * Jg[3 * nr + r];
     }
     // species block A_kj = d wdot_k / d rho_j (unclipped), b_k = d wdot_k / dT at fixed rho
-    for (int j = 0; j < ns; ++j) myrow[j] *= Wk / P.W[j];
+    for (int j = 0; j < ns; ++j) row[j] *= Wk / P.W[j];
   }
   const double bk = Wk * bT;
   const double Rk = sp ? 8314.51 / Wk : 0.0;
   if (mode == 0) {
     // constant volume: J_kj = g_j A_kj, J_kT = b_k; T row from column sums of e_k J_kj
     if (sp) {
-      for (int j = 0; j < ns; ++j) myrow[j] *= L.g[j];
-      myrow[ns] = bk;
+      for (int j = 0; j < ns; ++j) row[j] *= L.g[j];
+      row[ns] = bk;
     }
     const double s_de = tsum(sp ? (o.dh - Rk) * o.w : 0.0);
     const double s_dcv = tsum(sp ? o.xc * o.dcp : 0.0);
     L.aux[lane] = sp ? o.h - Rk * T : 0.0;
     tsync();
+    double jt = 0.0;
     if (lane < nz) {
       double cs = 0.0;
-      for (int k = 0; k < ns; ++k) cs += L.aux[k] * Jg[k * 33 + lane];
-      const double jt = lane < ns ? -(cs + FT * (o.cp - Rk) * o.g) / o.rho_cv : -(s_de + cs + FT * s_dcv) / o.rho_cv;
-      Jg[ns * 33 + lane] = jt;
+      for (int k = 0; k < ns; ++k) cs += L.aux[k] * L.lu[k * 33 + lane];
+      jt = lane < ns ? -(cs + FT * (o.cp - Rk) * o.g) / o.rho_cv : -(s_de + cs + FT * s_dcv) / o.rho_cv;
     }
+    tsync();
+    if (lane < nz) L.lu[ns * 33 + lane] = jt;
     tsync();
   } else {
     // constant pressure (Y, T): see oracle/flint_oracle.c jac_cp_cell for the formulas
     const double Rm = o.Rmix, rho = o.rho;
     double AY = 0.0;
     if (sp) {
-      for (int j = 0; j < ns; ++j) AY += myrow[j] * L.aux[j];
+      for (int j = 0; j < ns; ++j) AY += row[j] * L.aux[j];
       // stage dwdot_k/dY_j and dwdot_k/dT for the enthalpy-weighted column sums
-      for (int j = 0; j < ns; ++j) myrow[j] = rho * L.g[j] * (myrow[j] - AY * (8314.51 / P.W[j]) / Rm);
-      myrow[ns] = bk - rho * AY / T;
+      for (int j = 0; j < ns; ++j) row[j] = rho * L.g[j] * (row[j] - AY * (8314.51 / P.W[j]) / Rm);
+      row[ns] = bk - rho * AY / T;
     }
     const double s_dhw = tsum(sp ? o.dh * o.w : 0.0);
     const double sYdcp = tsum(sp ? o.xc * o.dcp : 0.0);
@@ -453,18 +452,25 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
     double jt = 0.0;
     if (lane < nz) {
       double ch = 0.0;
-      for (int k = 0; k < ns; ++k) ch += L.aux[k] * Jg[k * 33 + lane];
+      for (int k = 0; k < ns; ++k) ch += L.aux[k] * L.lu[k * 33 + lane];
       jt = lane < ns ? -ch / (rho * o.cpm) - FT * o.g * (o.cp / o.cpm - Rk / Rm)
                      : -(s_dhw + ch) / (rho * o.cpm) - FT * (sYdcp / o.cpm - 1.0 / T);
     }
     tsync();
     if (sp) {
-      for (int j = 0; j < ns; ++j) myrow[j] = myrow[j] / rho + L.g[j] * (o.w / rho) * (8314.51 / P.W[j]) / Rm;
-      myrow[ns] = myrow[ns] / rho + o.w / (rho * T);
+      for (int j = 0; j < ns; ++j) row[j] = row[j] / rho + L.g[j] * (o.w / rho) * (8314.51 / P.W[j]) / Rm;
+      row[ns] = row[ns] / rho + o.w / (rho * T);
     }
-    if (lane < nz) Jg[ns * 33 + lane] = jt;
+    if (lane < nz) L.lu[ns * 33 + lane] = jt;
     tsync();
   }
+  // the rows go to the global slot, which the inverse reads row by row (same lane); every lane's
+  // loads of the reaction coefficients there have completed first
+  __threadfence_block();
+  double* __restrict__ jrow = Jg + lane * 33;
+#pragma unroll
+  for (int j = 0; j < TEAM; ++j) jrow[j] = row[j];
+  tsync();
 }
 
 // Inverse of A = I - c J by Gauss-Jordan elimination with partial pivoting and no row exchanges.
@@ -619,7 +625,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
   double xe = y;
   bool wj = false;
   while (true) {
-    const RhsOut o = team_rhs(P, O.mode, p, xe, L, wj);
+    const RhsOut o = team_rhs(P, O.mode, p, xe, L, wj, Jg);
     if (wj) {
       if (o.ok) {
         team_jac(P, O.mode, xe, L, o, Jg);
@@ -893,7 +899,7 @@ __global__ __launch_bounds__(256) void jac_kernel(Prog P, int mode, long n, long
   for (long cell = (long)blockIdx.x * tpb + team; cell < n; cell += stride) {
     const double y = lane < nz ? X[(long)lane * ld + cell] : 0.0;
     const double p = mode == 1 ? pres[cell] : 0.0;
-    const RhsOut o = team_rhs(P, mode, p, y, L, JAC);
+    const RhsOut o = team_rhs(P, mode, p, y, L, JAC, Jg);
     if (!o.ok) {
       if (lane < nz) {
         F[(long)lane * ld + cell] = -1.0;
