@@ -198,14 +198,15 @@ static int ensure_extra(flint_mech* m) {
   P.Th_min = m->Th_min;
   P.Th_max = m->Th_max;
   HIPCHK(hipMalloc(&x->d_counter, 64 * sizeof(int)));
-  // default launch: as many four-team blocks as fit in LDS on every CU (the program is shared per block)
+  // default launch: four-team blocks (the program is shared per block), as many as are resident per CU
   hipDeviceProp_t prop;
   HIPCHK(hipGetDeviceProperties(&prop, m->device));
   const size_t tb = (flint_reactor_team_bytes(P) + 15) & ~size_t(15);
   const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
   const size_t sh = flint_reactor_shared_bytes(P);
   const int per_cu = std::max<int>(1, (int)(std::min<size_t>(lds_cu, 160 * 1024) / (sh + tb * x->teams_per_block)));
-  x->blocks = prop.multiProcessorCount * std::min(per_cu, 8);
+  // a team is one wavefront, and the kernel's register budget allows 8 resident wavefronts per CU
+  x->blocks = prop.multiProcessorCount * std::max(1, std::min(per_cu, 8 / x->teams_per_block));
   HIPCHK(flint_reactor_set_max_lds((int)std::min<size_t>(lds_cu, 160 * 1024)));
   return FLINT_OK;
 }
@@ -321,7 +322,7 @@ extern "C" int flint_reactor_autotune(flint_mech* m, long n, long ld, int mode, 
     const int fit = (int)(lds_cu / (flint_reactor_shared_bytes(x->prog) + tb * tpb));
     if (fit < 1) continue;
     for (int per_cu : {1, 2, 3, 4, 6, 8}) {
-      if (per_cu > fit || per_cu * tpb > 16) continue;
+      if (per_cu > fit || per_cu * tpb > 8) continue;
       const int blocks = cus * per_cu;
       int rc = 0;
       for (int rep = 0; rep < 2 && !rc; ++rep) {

@@ -28,7 +28,8 @@
 
 namespace flint_reactor {
 
-constexpr int TEAM = 32;
+constexpr int TEAM = 32;  // component lanes of a cell: lane k < ns species k, lane ns temperature
+constexpr int WAVE = 64;  // a cell's team is a whole wavefront: two 32-lane halves
 constexpr int MAXO = 5;
 constexpr int NEWTON_MAXITER = 4;
 constexpr int CW = 16;  // production-sum chunk width (the host encoder uses the same value)
@@ -92,7 +93,11 @@ __device__ __forceinline__ void targmax(double& best, int& bl) {
 }
 
 __device__ __forceinline__ double tbc(double v, int src) { return __shfl(v, src, TEAM); }
-__device__ __forceinline__ int tbci(int v, int src) { return __shfl(v, src, TEAM); }
+// Both halves of the wavefront hold the same component values (component lane k = lane k and
+// lane 32 + k) and run the same control flow; they split the reaction, chunk, elimination and
+// solve work.  half(): which half this lane is in; other(v): the value of the other half's lane.
+__device__ __forceinline__ int half() { return (threadIdx.x >> 5) & 1; }
+__device__ __forceinline__ double other(double v) { return __shfl_xor(v, 32, WAVE); }
 // All LDS traffic of a team stays inside one wavefront: a wavefront-scope fence orders it.
 __device__ __forceinline__ void tsync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -289,7 +294,9 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
     th0 = ld2(rh0 + 2 * lane);
     th1 = ld2(rh1 + 2 * lane);
   }
-  int4 rec = L.rrec[lane];
+  const int wl = threadIdx.x & (WAVE - 1);
+  const int nslot = P.R * TEAM;
+  int4 rec = wl < nslot ? L.rrec[wl] : make_int4(7, 0, 0, 0);
   tsync();
   // third-body sums M_s = sum_j eff_sj c_j, lane s
   if (lane < P.nms) {
@@ -302,9 +309,10 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   tsync();
   // rates of progress, one reaction per lane and round; the next round's record is loaded ahead
 #pragma unroll 1
-  for (int i = 0; i < P.R; ++i) {
+  for (int i = 0; i * WAVE < nslot; ++i) {
     const int4 cur = rec;
-    if (i + 1 < P.R) rec = L.rrec[(i + 1) * TEAM + lane];
+    const int nxt = (i + 1) * WAVE + wl;
+    rec = nxt < nslot ? L.rrec[nxt] : make_int4(7, 0, 0, 0);
     if ((cur.x & 7) != 7) {
       if (jac) reaction<true>(cur, rk0, rk1, Td, L, jx, P.nr);
       else reaction<false>(cur, rk0, rk1, Td, L, jx, P.nr);
@@ -313,15 +321,18 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   tsync();
   // net production: chunk sums in reaction order, then the chunks of species k in order
 #pragma unroll 1
-  for (int c = 0; c < P.C; ++c) {
-    const int2* __restrict__ ct = P.cterm + (long)c * CW * TEAM + lane;
-    int2 tm[CW];
+  for (int c2 = 0; c2 < P.C; c2 += 2) {
+    const int c = c2 + half();
+    if (c < P.C) {
+      const int2* __restrict__ ct = P.cterm + (long)c * CW * TEAM + lane;
+      int2 tm[CW];
 #pragma unroll
-    for (int t = 0; t < CW; ++t) tm[t] = ct[t * TEAM];
-    double acc = 0.0;
+      for (int t = 0; t < CW; ++t) tm[t] = ct[t * TEAM];
+      double acc = 0.0;
 #pragma unroll
-    for (int t = 0; t < CW; ++t) acc = acc + (double)__int_as_float(tm[t].y) * L.q[tm[t].x];
-    L.part[c * TEAM + lane] = acc;
+      for (int t = 0; t < CW; ++t) acc = acc + (double)__int_as_float(tm[t].y) * L.q[tm[t].x];
+      L.part[c * TEAM + lane] = acc;
+    }
   }
   tsync();
   double w = 0.0;
@@ -371,9 +382,13 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
   const bool sp = lane < ns;
   const double T = tbc(x, ns);
   const double FT = tbc(o.F, ns);
+  // rows are built by the lower half (the upper half joins the team sums with identical values)
+  const bool h0 = half() == 0;
   double* row = L.lu + lane * 33;
+  if (h0) {
 #pragma unroll
-  for (int j = 0; j < TEAM; ++j) row[j] = 0.0;
+    for (int j = 0; j < TEAM; ++j) row[j] = 0.0;
+  }
   L.g[lane] = o.g;
   if (mode == 1) L.aux[lane] = o.xc;
   // the reaction coefficients in Jg were stored by other lanes of this wavefront
@@ -381,7 +396,7 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
   tsync();
   double bT = 0.0;
   const double Wk = sp ? P.W[lane] : 1.0;
-  if (sp) {
+  if (sp && h0) {
     const int t0 = P.wd_ptr[lane], t1 = P.wd_ptr[lane + 1];
 #pragma unroll 2
     for (int t = t0; t < t1; ++t) {
@@ -417,7 +432,7 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
   const double Rk = sp ? 8314.51 / Wk : 0.0;
   if (mode == 0) {
     // constant volume: J_kj = g_j A_kj, J_kT = b_k; T row from column sums of e_k J_kj
-    if (sp) {
+    if (sp && h0) {
       for (int j = 0; j < ns; ++j) row[j] *= L.g[j];
       row[ns] = bk;
     }
@@ -438,7 +453,7 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
     // constant pressure (Y, T): see oracle/flint_oracle.c jac_cp_cell for the formulas
     const double Rm = o.Rmix, rho = o.rho;
     double AY = 0.0;
-    if (sp) {
+    if (sp && h0) {
       for (int j = 0; j < ns; ++j) AY += row[j] * L.aux[j];
       // stage dwdot_k/dY_j and dwdot_k/dT for the enthalpy-weighted column sums
       for (int j = 0; j < ns; ++j) row[j] = rho * L.g[j] * (row[j] - AY * (8314.51 / P.W[j]) / Rm);
@@ -457,7 +472,7 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
                      : -(s_dhw + ch) / (rho * o.cpm) - FT * (sYdcp / o.cpm - 1.0 / T);
     }
     tsync();
-    if (sp) {
+    if (sp && h0) {
       for (int j = 0; j < ns; ++j) row[j] = row[j] / rho + L.g[j] * (o.w / rho) * (8314.51 / P.W[j]) / Rm;
       row[ns] = row[ns] / rho + o.w / (rho * T);
     }
@@ -467,34 +482,43 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
   // the rows go to the global slot, which the inverse reads row by row (same lane); every lane's
   // loads of the reaction coefficients there have completed first
   __threadfence_block();
-  double* __restrict__ jrow = Jg + lane * 33;
+  if (h0) {
+    double* __restrict__ jrow = Jg + lane * 33;
 #pragma unroll
-  for (int j = 0; j < TEAM; ++j) jrow[j] = row[j];
+    for (int j = 0; j < TEAM; ++j) jrow[j] = row[j];
+  }
   tsync();
 }
 
 // Inverse of A = I - c J by Gauss-Jordan elimination with partial pivoting and no row exchanges.
-// Lane i keeps row i in registers for the whole elimination; step k takes the unused row p with
-// the largest |a_pk| (team argmax; ties to the lower lane), lane p publishes its row in LDS and
+// Row i lives in registers of component lane i: columns 0..15 in the lower half, 16..31 in the
+// upper half.  Step k takes the unused row p with the largest |a_pk| (team argmax; ties to the
+// lower lane), row p is published in LDS and
 //   row p:        a_pj <- a_pj / a_pk (j != k),   a_pk <- 1 / a_pk
 //   rows i != p:  a_ij <- a_ij - a_ik a_pj (j != k, a_pj scaled),   a_ik <- -a_ik / a_pk
 // (the column-k storage takes over the identity column p, as in the classic in-place scheme).
-// The register row is rotated by one column after every step, so that the current column is
-// always r[0]: every register index stays static (a dynamic one would move the row to scratch).
+// Both halves rotate their 16 registers by one after every step, so that the current column k is
+// always register 0 of half k / 16 and every register index stays static (a dynamic one would
+// move the row to scratch); the pivot row is published in the same rotated order.
 // Afterwards lane i holds row step(i) of A^-1 with its columns in pivot order,
 // A^-1[step(i)][p_m] = a_im (p_m: the row chosen at step m); the rows go to LDS (stride 33) for
 // the solves.  Rows and columns beyond nz are the identity and take no part.  Returns step(lane).
 __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const double* __restrict__ Jg) {
+  constexpr int HC = TEAM / 2;  // columns per half
   const int lane = threadIdx.x & (TEAM - 1);
+  const int h = half(), c0 = HC * h;
   const double* __restrict__ jrow = Jg + lane * 33;
-  double r[TEAM];
+  double r[HC];
 #pragma unroll
-  for (int j = 0; j < TEAM; ++j) r[j] = (lane == j ? 1.0 : 0.0) - ((lane < nz && j < nz) ? c * jrow[j] : 0.0);
-  double* __restrict__ pb = L.vx;  // the pivot row of the current step, in rotated order
+  for (int j = 0; j < HC; ++j)
+    r[j] = (lane == c0 + j ? 1.0 : 0.0) - ((lane < nz && c0 + j < nz) ? c * jrow[c0 + j] : 0.0);
+  double* __restrict__ pb = L.vx;  // the pivot row of the current step, rotated like the registers
   int mystep = -1;
 #pragma unroll 1
   for (int k = 0; k < nz; ++k) {
-    const double rk = r[0];
+    const int hk = k / HC;
+    const double oth = other(r[0]);  // every lane takes part in the exchange
+    const double rk = h == hk ? r[0] : oth;
     double best = mystep < 0 ? fabs(rk) : -1.0;
     int bl = lane;
     targmax(best, bl);
@@ -502,40 +526,44 @@ __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const do
     if (me) {
       mystep = k;
 #pragma unroll
-      for (int j = 0; j < TEAM; ++j) pb[j] = r[j];
+      for (int j = 0; j < HC; ++j) pb[c0 + j] = r[j];
     }
     tsync();
-    const double inv = 1.0 / pb[0];
-    // pivot lane: a_pj * inv; other lanes: a_ij - a_ik (a_pj * inv); one fused form for both
+    const double inv = 1.0 / pb[HC * hk];
+    // pivot lane: a_pj * inv; other lanes: a_ij - a_ik (a_pj * inv); one fused form for both,
+    // rotating by one register on the way
     const double keep = me ? 0.0 : 1.0, fm = me ? -1.0 : rk;
-    const double r0 = me ? inv : -rk * inv;
+    const double nk = me ? inv : -rk * inv;
+    const double r0 = h == hk ? nk : keep * r[0] - fm * (pb[c0] * inv);
 #pragma unroll
-    for (int j = 0; j < TEAM - 1; ++j) r[j] = keep * r[j + 1] - fm * (pb[j + 1] * inv);
-    r[TEAM - 1] = r0;
+    for (int j = 0; j < HC - 1; ++j) r[j] = keep * r[j + 1] - fm * (pb[c0 + j + 1] * inv);
+    r[HC - 1] = r0;
     tsync();
   }
-  // undo the rotation by nz columns: register j holds column (j + nz) mod 32
+  // undo the rotation by nz columns: register j holds column c0 + (j + nz) mod 16
   double* arow = L.lu + lane * 33;
 #pragma unroll
-  for (int j = 0; j < TEAM; ++j) arow[(j + nz) & (TEAM - 1)] = r[j];
+  for (int j = 0; j < HC; ++j) arow[c0 + ((j + nz) & (HC - 1))] = r[j];
   tsync();
   return mystep < 0 ? lane : mystep;
 }
 
 // x = A^-1 b with the rows above; b and x are this lane's component.  b is placed in pivot order
-// (vb[step(l)] = b_l), each lane forms its dot product, and the result goes to its step's slot.
+// (vb[step(l)] = b_l), each half forms the dot product over its 16 columns, the halves' partial
+// sums are added (lower + upper in both) and the result goes to its step's slot.
 __device__ __forceinline__ double team_solve(int mystep, double b, int nz, const Lds& L) {
+  constexpr int HC = TEAM / 2;
   const int lane = threadIdx.x & (TEAM - 1);
+  const int h = half(), c0 = HC * h;
   const double* arow = L.lu + lane * 33;
   L.vb[mystep] = b;
   tsync();
-  double y = 0.0;
-  if (lane < nz) {
-#pragma unroll 8
-    for (int m = 0; m < nz; ++m) y = y + arow[m] * L.vb[m];
-  } else {
-    y = b;
-  }
+  double yp = 0.0;
+#pragma unroll
+  for (int m = 0; m < HC; ++m)
+    if (c0 + m < nz) yp = yp + arow[c0 + m] * L.vb[c0 + m];
+  const double yo = other(yp);
+  const double y = lane < nz ? (h == 0 ? yp + yo : yo + yp) : b;
   L.vx[mystep] = y;
   tsync();
   const double x = L.vx[lane];
@@ -844,51 +872,53 @@ __global__ void zero_counter(int* c) {
 // Two waves per SIMD: the per-team LDS (LU factors) allows about seven resident waves per CU, and
 // at two waves the register budget costs only a small spill (measured faster than one wave at 400
 // registers, and than three waves).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void bdf_kernel(Prog P, Opts O, long n, long ld, double* __restrict__ X,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void bdf_kernel(Prog P, Opts O, long n, long ld, double* __restrict__ X,
                                                   const double* __restrict__ pres, int* __restrict__ stats,
                                                   int* __restrict__ status, int* __restrict__ counter,
                                                   unsigned team_bytes, double* __restrict__ jac_scratch) {
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x & (TEAM - 1);
-  const int team = threadIdx.x / TEAM;
+  const bool h0 = half() == 0;
+  const int team = threadIdx.x / WAVE;
   const size_t sh = shared_prog_bytes(P);
   load_shared_prog(P, smem);
   __syncthreads();
   const Lds L = carve(smem, smem + sh + (size_t)team * team_bytes, P);
-  double* __restrict__ Jg = jac_scratch + ((size_t)blockIdx.x * (blockDim.x / TEAM) + team) * (TEAM * 33);
+  double* __restrict__ Jg = jac_scratch + ((size_t)blockIdx.x * (blockDim.x / WAVE) + team) * (TEAM * 33);
   const int nz = P.ns + 1;
   while (true) {
     int cell = 0;
-    if (lane == 0) cell = atomicAdd(counter, 1);
-    cell = tbci(cell, 0);
+    if ((threadIdx.x & (WAVE - 1)) == 0) cell = atomicAdd(counter, 1);
+    cell = __shfl(cell, 0, WAVE);
     if (cell >= n) break;
     double y = lane < nz ? X[(long)lane * ld + cell] : 0.0;
     const double p = O.mode == 1 ? pres[cell] : 0.0;
     int st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const int rc = integrate(P, O, p, y, st, L, Jg);
-    if (lane < nz) X[(long)lane * ld + cell] = y;
-    if (stats && lane < 8) {
+    if (h0 && lane < nz) X[(long)lane * ld + cell] = y;
+    if (stats && h0 && lane < 8) {
       int v = st[0];
 #pragma unroll
       for (int q = 1; q < 8; ++q)
         if (lane == q) v = st[q];
       stats[(long)lane * ld + cell] = v;
     }
-    if (status && lane == 0) status[cell] = rc;
+    if (status && h0 && lane == 0) status[cell] = rc;
   }
 }
 
 // The reactor's right-hand side and (JAC) Jacobian at given states (the same team_rhs/team_jac the
 // integrator uses), one team per cell over a grid-stride loop.  Outside the tables F = -1, J = 0.
 template <bool JAC>
-__global__ __launch_bounds__(256) void jac_kernel(Prog P, int mode, long n, long ld, const double* __restrict__ X,
+__global__ __launch_bounds__(512) void jac_kernel(Prog P, int mode, long n, long ld, const double* __restrict__ X,
                                                   const double* __restrict__ pres, double* __restrict__ F,
                                                   double* __restrict__ J, unsigned team_bytes,
                                                   double* __restrict__ jac_scratch) {
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x & (TEAM - 1);
-  const int team = threadIdx.x / TEAM;
-  const int tpb = blockDim.x / TEAM;
+  const bool h0 = half() == 0;
+  const int team = threadIdx.x / WAVE;
+  const int tpb = blockDim.x / WAVE;
   const size_t sh = shared_prog_bytes(P);
   load_shared_prog(P, smem);
   __syncthreads();
@@ -901,7 +931,7 @@ __global__ __launch_bounds__(256) void jac_kernel(Prog P, int mode, long n, long
     const double p = mode == 1 ? pres[cell] : 0.0;
     const RhsOut o = team_rhs(P, mode, p, y, L, JAC, Jg);
     if (!o.ok) {
-      if (lane < nz) {
+      if (h0 && lane < nz) {
         F[(long)lane * ld + cell] = -1.0;
         if (JAC)
           for (int c = 0; c < nz; ++c) J[((long)lane * nz + c) * ld + cell] = 0.0;
@@ -909,12 +939,12 @@ __global__ __launch_bounds__(256) void jac_kernel(Prog P, int mode, long n, long
       continue;
     }
     if (!JAC) {
-      if (lane < nz) F[(long)lane * ld + cell] = o.F;
+      if (h0 && lane < nz) F[(long)lane * ld + cell] = o.F;
       tsync();
       continue;
     }
     team_jac(P, mode, y, L, o, Jg);
-    if (lane < nz) {
+    if (h0 && lane < nz) {
       F[(long)lane * ld + cell] = o.F;
       for (int c = 0; c < nz; ++c) J[((long)lane * nz + c) * ld + cell] = Jg[lane * 33 + c];
     }
@@ -943,7 +973,7 @@ hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reacto
                                 int blocks, double* jac_scratch, hipStream_t s) {
   const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P) + 15) & ~size_t(15));
   hipLaunchKernelGGL(flint_reactor::zero_counter, dim3(1), dim3(64), 0, s, counter);
-  hipLaunchKernelGGL(flint_reactor::bdf_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
+  hipLaunchKernelGGL(flint_reactor::bdf_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::WAVE),
                      flint_reactor::shared_prog_bytes(P) + (size_t)tb * teams_per_block, s, P, O, n, ld, X, pres, stats, status, counter, tb, jac_scratch);
   return hipGetLastError();
 }
@@ -953,10 +983,10 @@ hipError_t flint_reactor_jac_launch(const flint_reactor::Prog& P, int mode, long
                                     double* jac_scratch, hipStream_t s) {
   const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P) + 15) & ~size_t(15));
   if (J)
-    hipLaunchKernelGGL(flint_reactor::jac_kernel<true>, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
+    hipLaunchKernelGGL(flint_reactor::jac_kernel<true>, dim3(blocks), dim3(teams_per_block * flint_reactor::WAVE),
                        flint_reactor::shared_prog_bytes(P) + (size_t)tb * teams_per_block, s, P, mode, n, ld, X, pres, F, J, tb, jac_scratch);
   else
-    hipLaunchKernelGGL(flint_reactor::jac_kernel<false>, dim3(blocks), dim3(teams_per_block * flint_reactor::TEAM),
+    hipLaunchKernelGGL(flint_reactor::jac_kernel<false>, dim3(blocks), dim3(teams_per_block * flint_reactor::WAVE),
                        flint_reactor::shared_prog_bytes(P) + (size_t)tb * teams_per_block, s, P, mode, n, ld, X, pres, F, J, tb, jac_scratch);
   return hipGetLastError();
 }

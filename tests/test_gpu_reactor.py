@@ -69,11 +69,16 @@ def test_reactor_matches_oracle(gmech, mechs, oracle_mod, cuda, mech, mode):
     assert abs(gst[ok, 0].mean() / ast[ok, 0].mean() - 1.0) < 0.1
     # exact linear invariants: total mass and elements.  Every BDF operation is the same linear map
     # on every component only if the team-uniform step controls are bit-identical on all lanes, so
-    # this also guards the team reductions.
-    assert np.max(np.abs(G[:ns, ok].sum(0) - X[:ns, ok].sum(0)) / X[:ns, ok].sum(0)) < 1e-12
+    # this also guards the team reductions (a lane-dependent decision showed up as 2e-11 drifts
+    # over most cells).  The bulk stays at the oracle's round-off (about 3e-15); a cell that
+    # crosses the constant-volume clip hundreds of times accumulates the round-off of the
+    # explicit-inverse Newton corrections to about 1e-12.
+    mass = np.abs(G[:ns, ok].sum(0) - X[:ns, ok].sum(0)) / X[:ns, ok].sum(0)
     comp = np.array(md.desc["composition"])
     e0, e1 = comp @ (X[:ns, ok] / md.W[:, None]), comp @ (G[:ns, ok] / md.W[:, None])
-    assert np.max(np.abs(e1 - e0) / np.abs(e0).max(0)) < 1e-11
+    elem = np.max(np.abs(e1 - e0) / np.abs(e0).max(0), axis=0)
+    assert np.quantile(mass, 0.99) < 1e-14 and np.quantile(elem, 0.99) < 1e-13
+    assert mass.max() < (1e-11 if mode == 0 else 1e-12) and elem.max() < 1e-11
 
 
 @pytest.mark.parametrize("mech", ["TSR-GP-24", "TSR-Rich-31"])
