@@ -7,18 +7,8 @@ namespace flint_reactor {
 // Rate program in device memory (built from the host program a generated mechanism registers).
 struct Prog {
   int ns, nr, nms;
-  const int* rx_kind;     // [nr] 0 Arrhenius, 1 three-body, 2 Troe, 3 Lindemann
-  const int* rx_col;      // [nr] first column of the reaction in a kinetics table row
-  const int* rx_mset;     // [nr] third-body sum used, -1 none
-  const int* rx_fac;      // [nr*8] forward then reverse factors: species | power << 8, -1 unused
-  const int* wd_ptr;      // [ns+1] net-production terms of species k
-  const int* wd_rx;       // reaction of each term, in reaction order
-  const double* wd_coef;  // float32-valued stoichiometric difference of each term
-  const int* ms_ptr;      // [nms+1] terms of each third-body sum
-  const int* ms_sp;
-  const double* ms_eff;   // efficiency (1 for a bare term)
-  const double* effd;     // [nms*32] dense efficiencies, 0 where a species is absent
-  // The same program re-encoded for the right-hand side of the integrator (see reactor.hip):
+  const int* wd_ptr;      // [ns+1] net-production terms of species k (in reaction order)
+  // The program encoded for the integrator (flint_reactor_host.cpp encode_rhs, reactor.hip):
   int R;                  // reaction rounds: slot 32*round + lane
   const int4* rrec;       // [R*32] {kind | (mset + 1) << 3 (kind 7: empty), column, reaction, factor slots}
   const float* effT;      // [ns][nms] efficiency of species j in third-body set s (0 if absent)

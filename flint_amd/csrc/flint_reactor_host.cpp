@@ -164,9 +164,6 @@ static int ensure_extra(flint_mech* m) {
   P.ns = hp->ns;
   P.nr = hp->nr;
   P.nms = hp->nms;
-  std::vector<double> effd((size_t)std::max(hp->nms, 1) * 32, 0.0);
-  for (int s = 0; s < hp->nms; ++s)
-    for (int t = hp->ms_ptr[s]; t < hp->ms_ptr[s + 1]; ++t) effd[(size_t)s * 32 + hp->ms_sp[t]] = hp->ms_eff[t];
   std::vector<int> rrec, kpart;
   std::vector<float> effT;
   std::vector<int> cterm, jterm;
@@ -177,17 +174,7 @@ static int ensure_extra(flint_mech* m) {
   rc |= upload(x, reinterpret_cast<const int2*>(cterm.data()), cterm.size() / 2, &P.cterm);
   rc |= upload(x, kpart.data(), kpart.size(), &P.kpart);
   rc |= upload(x, reinterpret_cast<const int4*>(jterm.data()), jterm.size() / 4, &P.jterm);
-  rc |= upload(x, hp->rx_kind, hp->nr, &P.rx_kind);
-  rc |= upload(x, hp->rx_col, hp->nr, &P.rx_col);
-  rc |= upload(x, hp->rx_mset, hp->nr, &P.rx_mset);
-  rc |= upload(x, hp->rx_fac, (size_t)hp->nr * 8, &P.rx_fac);
   rc |= upload(x, hp->wd_ptr, (size_t)hp->ns + 1, &P.wd_ptr);
-  rc |= upload(x, hp->wd_rx, hp->nterm, &P.wd_rx);
-  rc |= upload(x, hp->wd_coef, hp->nterm, &P.wd_coef);
-  rc |= upload(x, hp->ms_ptr, (size_t)hp->nms + 1, &P.ms_ptr);
-  rc |= upload(x, hp->ms_sp, hp->nmterm, &P.ms_sp);
-  rc |= upload(x, hp->ms_eff, hp->nmterm, &P.ms_eff);
-  rc |= upload(x, effd.data(), effd.size(), &P.effd);
   if (rc) return rc;
   P.W = m->d_W;
   P.tabk = m->d_tabk;

@@ -92,12 +92,23 @@ __device__ __forceinline__ void targmax(double& best, int& bl) {
 #undef FLINT_ARGMAX_LEVEL
 }
 
-__device__ __forceinline__ double tbc(double v, int src) { return __shfl(v, src, TEAM); }
 // Both halves of the wavefront hold the same component values (component lane k = lane k and
 // lane 32 + k) and run the same control flow; they split the reaction, chunk, elimination and
-// solve work.  half(): which half this lane is in; other(v): the value of the other half's lane.
+// solve work.  half(): which half this lane is in.
 __device__ __forceinline__ int half() { return (threadIdx.x >> 5) & 1; }
-__device__ __forceinline__ double other(double v) { return __shfl_xor(v, 32, WAVE); }
+// Component src of a value both halves hold alike: a scalar read of lane src.
+__device__ __forceinline__ double tbc(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+// (lower half's value, upper half's value) of component lane l at both lanes l and 32 + l
+// (v_permlane32_swap; verified on gfx950).
+__device__ __forceinline__ double2 halves(double v) {
+  const auto a = __builtin_amdgcn_permlane32_swap((unsigned)__double2loint(v), (unsigned)__double2loint(v), false, false);
+  const auto b = __builtin_amdgcn_permlane32_swap((unsigned)__double2hiint(v), (unsigned)__double2hiint(v), false, false);
+  return make_double2(__hiloint2double((int)b[0], (int)a[0]), __hiloint2double((int)b[1], (int)a[1]));
+}
 // All LDS traffic of a team stays inside one wavefront: a wavefront-scope fence orders it.
 __device__ __forceinline__ void tsync() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -517,8 +528,8 @@ __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const do
 #pragma unroll 1
   for (int k = 0; k < nz; ++k) {
     const int hk = k / HC;
-    const double oth = other(r[0]);  // every lane takes part in the exchange
-    const double rk = h == hk ? r[0] : oth;
+    const double2 hv = halves(r[0]);  // column k is register 0 of half hk
+    const double rk = hk == 0 ? hv.x : hv.y;
     double best = mystep < 0 ? fabs(rk) : -1.0;
     int bl = lane;
     targmax(best, bl);
@@ -562,8 +573,8 @@ __device__ __forceinline__ double team_solve(int mystep, double b, int nz, const
 #pragma unroll
   for (int m = 0; m < HC; ++m)
     if (c0 + m < nz) yp = yp + arow[c0 + m] * L.vb[c0 + m];
-  const double yo = other(yp);
-  const double y = lane < nz ? (h == 0 ? yp + yo : yo + yp) : b;
+  const double2 yv = halves(yp);
+  const double y = lane < nz ? yv.x + yv.y : b;
   L.vx[mystep] = y;
   tsync();
   const double x = L.vx[lane];
@@ -889,7 +900,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   while (true) {
     int cell = 0;
     if ((threadIdx.x & (WAVE - 1)) == 0) cell = atomicAdd(counter, 1);
-    cell = __shfl(cell, 0, WAVE);
+    cell = __builtin_amdgcn_readlane(cell, 0);
     if (cell >= n) break;
     double y = lane < nz ? X[(long)lane * ld + cell] : 0.0;
     const double p = O.mode == 1 ? pres[cell] : 0.0;
