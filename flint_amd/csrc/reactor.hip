@@ -10,7 +10,7 @@
 //
 // Layout of one team (lanes l = 0..31 of a half wavefront):
 //   lane k < ns owns species k, lane ns owns T: the state component and the 8-row BDF difference
-//   column D[.][k] live in VGPRs; row k of the Jacobian sits in a per-team global slot and row k
+//   column D[.][k] live in VGPRs and LDS; row k of the Jacobian sits in a per-team global slot and row k
 //   of the inverse of I - cJ in LDS (stride 33, conflict-free).  The inverse comes from an
 //   in-place Gauss-Jordan elimination with partial pivoting (team argmax, pivot row read as an LDS
 //   broadcast), so that each Newton correction is one matrix-vector product instead of two
@@ -129,11 +129,13 @@ struct Lds {
   double* part; // [C*32] partial production sums of the chunks
   double* vb;   // [32] right-hand side of a Newton correction, in pivot order
   double* vx;   // [32] its solution
+  double* D;    // [8][32] BDF difference array (row j, component k); kept in LDS to spare registers
   double* lu;   // [32][33] inverse of I - cJ (row = lane, padded); Jacobian rows while one is built
 };
 
 __host__ __device__ inline size_t team_lds_bytes(const Prog& P) {
-  return (size_t)(32 * 5 + (P.nms > 0 ? P.nms : 1) + (size_t)P.nr + 1 + (size_t)P.C * 32 + 32 * 33) * sizeof(double);
+  return (size_t)(32 * 5 + 8 * 32 + (P.nms > 0 ? P.nms : 1) + (size_t)P.nr + 1 + (size_t)P.C * 32 + 32 * 33) *
+         sizeof(double);
 }
 
 // Block-shared program bytes (16-byte multiples): records, efficiencies, chunk slots.
@@ -169,6 +171,7 @@ __device__ inline Lds carve(char* shared, char* base, const Prog& P) {
   L.g = d; d += 32;
   L.vb = d; d += 32;
   L.vx = d; d += 32;
+  L.D = d; d += 8 * 32;
   L.Mv = d; d += (P.nms > 0 ? P.nms : 1);
   L.q = d; d += P.nr + 1;
   L.part = d; d += P.C * 32;
@@ -588,6 +591,12 @@ __device__ __forceinline__ double team_rms(double v, double scale, int nz) {
   return sqrt(tsum(q * q) / nz);
 }
 
+// This lane's column of the BDF difference array in LDS (both halves hold the same values).
+struct Dcol {
+  double* p;
+  __device__ __forceinline__ double& operator[](int j) const { return p[j * TEAM]; }
+};
+
 // D[0..order] <- (R U)^T D[0..order] on this lane's column, with R(order, factor) and U = R(order, 1)
 // of the method: R[j][l] = prod_{m=1..j} (m - 1 - factor l)/m, so R[0][l] = 1 and R[j][0] = [j == 0].
 // Because RU[j][0] = [j == 0] and RU[0][i] = [i == 0], D[0] (the solution itself) is left as it is
@@ -595,7 +604,7 @@ __device__ __forceinline__ double team_rms(double v, double scale, int nz) {
 // for i >= 1.  Mixing D[0] into v_l and relying on sum_l U[l][i] = 0 to remove it again would leave
 // an absolute error of eps |y| in every difference, which later step-size increases amplify by
 // factor^j into a visible loss of mass after a long run of step halvings.
-__device__ __forceinline__ void change_D(double (&D)[MAXO + 3], int order, double factor) {
+__device__ __forceinline__ void change_D(const Dcol D, int order, double factor) {
   double v[MAXO + 1];
 #pragma unroll
   for (int l = 1; l <= MAXO; ++l) {
@@ -648,7 +657,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
   // start-up values
   double f0 = 0.0, sc = 0.0, h0 = 0.0, d1 = 0.0;
   // method state
-  double D[MAXO + 3];
+  const Dcol D{L.D + lane};
 #pragma unroll
   for (int j = 0; j < MAXO + 3; ++j) D[j] = 0.0;
   int order = 1, n_equal = 0;
@@ -778,27 +787,15 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         current_jac = false;
         // D^{j+1} y_n = D^j y_n - D^j y_{n-1}, d = D^{order+1} y_n
         {
-          double Dop1 = 0.0;
-#pragma unroll
-          for (int j = 0; j < MAXO + 3; ++j)
-            if (j == order + 1) Dop1 = D[j];
-#pragma unroll
-          for (int j = 0; j < MAXO + 3; ++j) {
-            if (j == order + 2) D[j] = d - Dop1;
-            if (j == order + 1) D[j] = d;
-          }
-#pragma unroll
-          for (int j = MAXO; j >= 0; --j)
-            if (j <= order) D[j] += D[j + 1];
+          const double Dop1 = D[order + 1];
+          D[order + 2] = d - Dop1;
+          D[order + 1] = d;
+#pragma unroll 1
+          for (int j = order; j >= 0; --j) D[j] += D[j + 1];
         }
         go = GO_STEP;
         if (n_equal < order + 1) continue;
-        double Dm = 0.0, Dp = 0.0;
-#pragma unroll
-        for (int j = 0; j < MAXO + 3; ++j) {
-          if (j == order) Dm = D[j];
-          if (j == order + 2) Dp = D[j];
-        }
+        const double Dm = D[order], Dp = D[order + 2];
         const double em = order > 1 ? team_rms(Dm / order, sc, nz) : INFINITY;
         const double ep = order < MAXO ? team_rms(Dp / (order + 2), sc, nz) : INFINITY;
         const double norms[3] = {em, err_norm, ep};

@@ -100,8 +100,30 @@ __global__ __launch_bounds__(128) void apply_kernel(SurProg P, long n, long ld, 
   r[ne < MAXE ? ne : MAXE] = s0 - s1;
   bool fall = !(fabs(dT) <= P.max_dT) || !(res <= P.max_res);
   double T = T0 + dT;
-  if (!fall) {
-    // weighted least-norm correction onto C Y = C Y0 (ne element rows and the mass row)
+  // weighted least-norm correction onto C Y = C Y0 (ne element rows and the mass row), applied
+  // twice: the second pass removes what the first left of the residual (the normal matrix is
+  // ill-conditioned when an element is nearly absent)
+#pragma unroll 1
+  for (int pass = 0; pass < 2 && !fall; ++pass) {
+    if (pass > 0) {
+#pragma unroll
+      for (int e = 0; e < MAXE; ++e) {
+        double z0 = 0.0, z1 = 0.0;
+        if (e < ne) {
+#pragma unroll
+          for (int k = 0; k < NS; ++k) {
+            const double a = P.comp[e * NS + k] / P.W[k];
+            z0 += a * Y0[k];
+            z1 += a * Y[k];
+          }
+        }
+        r[e] = z0 - z1;
+      }
+      s1 = 0.0;
+#pragma unroll
+      for (int k = 0; k < NS; ++k) s1 += Y[k];
+      r[ne < MAXE ? ne : MAXE] = s0 - s1;
+    }
     const int nc = ne + 1;
     double M[MAXE + 1][MAXE + 2];
 #pragma unroll
@@ -172,6 +194,10 @@ __global__ __launch_bounds__(128) void apply_kernel(SurProg P, long n, long ld, 
           if (i < nc) ct += (i < ne ? P.comp[i * NS + k] / P.W[k] : 1.0) * lam[i];
         Y[k] += fmax(Y[k], 0.0) * ct;
       }
+    }
+  }
+  if (!fall) {
+    {
       // temperature from the conserved mixture enthalpy
       double h0, dh0;
       if (!hmix<NS>(P, Y0, T0, h0, dh0)) fall = true;
