@@ -242,7 +242,7 @@ def cea_leg(args, dev, stream):
     it1, s1 = torch.empty(n, dtype=torch.int32, device=dev), torch.empty(n, dtype=torch.int32, device=dev)
     it2, s2 = torch.empty_like(it1), torch.empty_like(s1)
     run1 = lambda: mech.equilibrium(T0, Y0, T1, Y1, p=p, hp=True, iters=it1, status=s1, stream=stream)
-    run2 = lambda: ws.solve(mech, md, Y0, T0, p, T2, Y2, iters=it2, status=s2)
+    run2 = lambda: ws.solve(mech, md, Y0, T0, p, T2, Y2, iters=it2, status=s2, stream=stream)
     run1(), run2()
     ms1, ms2 = timed(stream, run1, 5), timed(stream, run2, 5)
     ok = (s1 == 0) & (s2 == 0)

@@ -10,6 +10,8 @@ struct CeaProg {
   const double* W;     // [ns] molecular weights (wm_tab)
   const double* rm;    // [ns] species weights from the elements' atomic weights
   int Th_min, Th_max;
+  int redo;  // nonzero: solve only the states whose status is nonzero, from the default guess, adding
+             // to their iteration counts (the second pass of a warm-started batch)
 };
 
 }  // namespace flint_cea

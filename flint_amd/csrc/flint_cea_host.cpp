@@ -9,10 +9,18 @@
 #include "../../include/flint_amd.h"
 #include "flint_cea.h"
 #include "flint_internal.h"
+#include "flint_mlp.h"
+#include "flint_warm.h"
 
 struct flint_cea_data {
   flint_cea::CeaProg P{};
   std::vector<void*> bufs;
+  // learned warm start workspace (flint_cea_solve_warm), grown on demand
+  long wcap = 0;
+  float *F = nullptr, *O = nullptr;
+  double *G = nullptr, *scal = nullptr;
+  int* st = nullptr;
+  std::vector<void*> wbufs;
 };
 
 #define HIPCHK(expr)                                                                                      \
@@ -24,6 +32,7 @@ struct flint_cea_data {
 void flint_mech_release_cea(flint_mech* m) {
   if (!m->cea) return;
   for (void* p : m->cea->bufs) (void)hipFree(p);
+  for (void* p : m->cea->wbufs) (void)hipFree(p);
   delete m->cea;
   m->cea = nullptr;
 }
@@ -119,5 +128,55 @@ extern "C" int flint_cea_solve_host(flint_mech* m, long n, int hp, const double*
   HIPCHK(hipMemcpyAsync(Y_out, d_Yo, ns * nn * sizeof(double), hipMemcpyDeviceToHost, m->stream));
   if (status) HIPCHK(hipMemcpyAsync(status, d_st, nn * sizeof(int), hipMemcpyDeviceToHost, m->stream));
   HIPCHK(hipStreamSynchronize(m->stream));
+  return FLINT_OK;
+}
+
+template <class T>
+static int wgrab(flint_cea_data* c, T** p, size_t count) {
+  void* q = nullptr;
+  HIPCHK(hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)));
+  c->wbufs.push_back(q);
+  *p = static_cast<T*>(q);
+  return FLINT_OK;
+}
+
+extern "C" int flint_cea_solve_warm(flint_mech* m, flint_mlp* net, long n, long ld, const double* p, const double* T,
+                                    const double* rho, const double* scaling, double* T_out, double* Y_out,
+                                    int* iters, int* status, void* stream) {
+  if (!m || !net || !scaling) return flint_fail(FLINT_EARG, "null argument");
+  if (!m->cea) return flint_fail(FLINT_EARG, "flint_cea_setup has not been called for this mechanism");
+  if (n < 0 || ld < n) return flint_fail(FLINT_EARG, "need 0 <= n <= ld");
+  if (n == 0) return FLINT_OK;
+  if (!p || !T || !rho || !T_out || !Y_out) return flint_fail(FLINT_EARG, "null array");
+  const int ns = m->ns;
+  if (net->dims.front() != ns + 2 || net->dims.back() != ns + 1)
+    return flint_fail(FLINT_EARG, "network widths must be ns+2 inputs and ns+1 outputs");
+  flint_cea_data* c = m->cea;
+  if (c->wcap < ld) {
+    for (void* q : c->wbufs) (void)hipFree(q);
+    c->wbufs.clear();
+    c->wcap = 0;
+    int rc = 0;
+    rc |= wgrab(c, &c->F, (size_t)ld * (ns + 2));
+    rc |= wgrab(c, &c->O, (size_t)ld * (ns + 1));
+    rc |= wgrab(c, &c->G, (size_t)ld * (ns + 1));
+    rc |= wgrab(c, &c->scal, (size_t)4 * ns + 6);
+    rc |= wgrab(c, &c->st, (size_t)ld);
+    if (rc) return rc;
+    c->wcap = ld;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  HIPCHK(hipMemcpyAsync(c->scal, scaling, (size_t)(4 * ns + 6) * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(flint_ws_features(n, ld, ns, rho, T, p, c->P.tabh, m->Th_min, m->Th_max, c->scal, c->F, s));
+  if (int rc = flint_mlp_forward(net, n, c->F, c->O, s)) return rc;
+  HIPCHK(flint_ws_guess(n, ld, ns, c->O, c->scal, c->G, s));
+  int* st = status ? status : c->st;
+  hipError_t e = m->ops->cea(c->P, true, n, ld, p, T, rho, c->G, T_out, Y_out, iters, st, s);
+  if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("equilibrium launch: ") + hipGetErrorString(e));
+  // states that did not converge from the predicted start: again from the default start
+  flint_cea::CeaProg R = c->P;
+  R.redo = 1;
+  e = m->ops->cea(R, true, n, ld, p, T, rho, nullptr, T_out, Y_out, iters, st, s);
+  if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("equilibrium launch: ") + hipGetErrorString(e));
   return FLINT_OK;
 }

@@ -131,6 +131,7 @@ __global__ __launch_bounds__(64) void cea_kernel(CeaProg P, long n, long ld, con
   __syncthreads();
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
+  if (P.redo && status[c] == 0) return;
 
   double En[NS], Enln[NS], H0[NS], G0[NS];
   double B0[NE];
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(64) void cea_kernel(CeaProg P, long n, long ld, con
 #pragma unroll
     for (int j = 0; j < NS; ++j) Yout[(long)j * ld + c] = __builtin_nan("");
   }
-  if (iters) iters[c] = numb;
+  if (iters) iters[c] = P.redo ? iters[c] + numb : numb;
   if (status) status[c] = rc;
 }
 

@@ -11,6 +11,9 @@ struct flint_mlp {
   long np = 0, nw = 0;                     // parameters, weights
   float *w32 = nullptr, *g32 = nullptr, *m32 = nullptr, *v32 = nullptr;  // master, gradient, Adam moments
   __bf16* w16 = nullptr;                   // BF16 weights
+  __bf16* w16t = nullptr;                  // BF16 weights transposed per layer ([dout][din]) for the forward path
+  std::vector<long> w16t_off;
+  bool wt_stale = true;                    // w16t behind w16 (set on every weight update)
   int* w16_index = nullptr;                // FP32 index -> BF16 index (-1 for biases)
   float* loss = nullptr;                   // device scalar
   int adam_t = 0;

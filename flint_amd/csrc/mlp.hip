@@ -80,6 +80,60 @@ __global__ __launch_bounds__(64) void gemm_kernel(int M, int N, int K, const __b
   }
 }
 
+// Forward product for K % 8 == 0: A [M][K] row-major and the transposed weights WT [N][K], so that
+// every fragment is one 16-byte load; a wavefront computes a 32 x (32 NT) strip, reusing its A
+// fragment for NT tiles.  Same epilogues as gemm_kernel.
+template <int EPI, int NT>
+__global__ __launch_bounds__(64) void gemm_fwd_kernel(int M, int N, int K, const __bf16* __restrict__ A,
+                                                      const __bf16* __restrict__ WT, const float* __restrict__ bias,
+                                                      float* __restrict__ O32, __bf16* __restrict__ O16, long ldo) {
+  const int l = threadIdx.x;
+  const int m0 = blockIdx.x * 32, n0 = blockIdx.y * 32 * NT;
+  const int am = m0 + (l & 31), kk = 8 * (l >> 5);
+  f32x16 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = f32x16{};
+  const bf16x8 zero = {};
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    const int k = k0 + kk;
+    const bf16x8 a = (am < M && k < K) ? *reinterpret_cast<const bf16x8*>(A + (long)am * K + k) : zero;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int bn = n0 + 32 * t + (l & 31);
+      const bf16x8 b = (bn < N && k < K) ? *reinterpret_cast<const bf16x8*>(WT + (long)bn * K + k) : zero;
+      acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int m = m0 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+      const int n = n0 + 32 * t + (l & 31);
+      if (m >= M || n >= N) continue;
+      const long o = (long)m * ldo + n;
+      float v = acc[t][r];
+      if (EPI == EPI_LINEAR) {
+        if (bias) v += bias[n];
+        if (O32) O32[o] = v;
+        if (O16) O16[o] = (__bf16)v;
+      } else {
+        v += bias[n];
+        if (O32) O32[o] = v;
+        O16[o] = (__bf16)gelu(v);
+      }
+    }
+  }
+}
+
+// WT_l [dout][din] from W_l [din][dout] (bf16), all layers
+__global__ void transpose_w16(int din, int dout, const __bf16* __restrict__ w, __bf16* __restrict__ wt) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long)din * dout) return;
+  const int r = (int)(i / dout), c = (int)(i % dout);
+  wt[(long)c * din + r] = w[i];
+}
+
 __global__ void to_bf16(long n, const float* __restrict__ x, __bf16* __restrict__ y) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = (__bf16)x[i];
@@ -156,6 +210,20 @@ hipError_t gemm(int M, int N, int K, const __bf16* A, long sam, long sak, const 
 
 inline unsigned nb(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+template <int EPI>
+hipError_t gemm_fwd(int M, int N, int K, const __bf16* A, const __bf16* WT, const float* bias, float* O32,
+                    __bf16* O16, long ldo, hipStream_t s) {
+  if (N <= 32) {
+    hipLaunchKernelGGL((gemm_fwd_kernel<EPI, 1>), dim3(nb(M, 32), 1), dim3(64), 0, s, M, N, K, A, WT, bias, O32, O16, ldo);
+  } else if (N <= 64) {
+    hipLaunchKernelGGL((gemm_fwd_kernel<EPI, 2>), dim3(nb(M, 32), 1), dim3(64), 0, s, M, N, K, A, WT, bias, O32, O16, ldo);
+  } else {
+    hipLaunchKernelGGL((gemm_fwd_kernel<EPI, 4>), dim3(nb(M, 32), nb(N, 128)), dim3(64), 0, s, M, N, K, A, WT, bias, O32,
+                       O16, ldo);
+  }
+  return hipGetLastError();
+}
+
 }  // namespace flint_mlp_k
 
 using namespace flint_mlp_k;
@@ -172,7 +240,7 @@ static int dalloc(void** p, size_t bytes) {
 }
 
 flint_mlp::~flint_mlp() {
-  for (void* p : {(void*)w32, (void*)g32, (void*)m32, (void*)v32, (void*)w16, (void*)w16_index, (void*)loss})
+  for (void* p : {(void*)w32, (void*)g32, (void*)m32, (void*)v32, (void*)w16, (void*)w16t, (void*)w16_index, (void*)loss})
     if (p) (void)hipFree(p);
   for (void* p : ws) (void)hipFree(p);
 }
@@ -214,17 +282,27 @@ int flint_mlp::reserve(long n, bool train) {
 
 // forward from a16[0]; the last layer writes FP32 to out (z32[L-1] when out is null)
 int flint_mlp::forward(long n, float* out, hipStream_t s, bool keep) {
+  if (wt_stale) {
+    for (int l = 0; l < L; ++l)
+      hipLaunchKernelGGL(transpose_w16, dim3(nb((long)dims[l] * dims[l + 1], 256)), dim3(256), 0, s, dims[l], dims[l + 1],
+                         w16 + w16_off[l], w16t + w16t_off[l]);
+    wt_stale = false;
+  }
   for (int l = 0; l < L; ++l) {
     const int din = dims[l], dout = dims[l + 1];
     const __bf16* W = w16 + w16_off[l];
+    const __bf16* WT = w16t + w16t_off[l];
     const float* b = w32 + b_off[l];
+    const bool fast = din % 8 == 0;
     hipError_t e;
     if (l + 1 < L)
-      e = gemm<EPI_GELU>((int)n, dout, din, a16[l], din, 1, W, dout, 1, b, nullptr, keep ? z32[l] : nullptr,
-                         a16[l + 1], dout, s);
+      e = fast ? gemm_fwd<EPI_GELU>((int)n, dout, din, a16[l], WT, b, keep ? z32[l] : nullptr, a16[l + 1], dout, s)
+               : gemm<EPI_GELU>((int)n, dout, din, a16[l], din, 1, W, dout, 1, b, nullptr, keep ? z32[l] : nullptr,
+                                a16[l + 1], dout, s);
     else
-      e = gemm<EPI_LINEAR>((int)n, dout, din, a16[l], din, 1, W, dout, 1, b, nullptr, out ? out : z32[l], nullptr,
-                           dout, s);
+      e = fast ? gemm_fwd<EPI_LINEAR>((int)n, dout, din, a16[l], WT, b, out ? out : z32[l], nullptr, dout, s)
+               : gemm<EPI_LINEAR>((int)n, dout, din, a16[l], din, 1, W, dout, 1, b, nullptr, out ? out : z32[l],
+                                  nullptr, dout, s);
     if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("mlp forward: ") + hipGetErrorString(e));
   }
   return FLINT_OK;
@@ -267,7 +345,10 @@ extern "C" int flint_mlp_create(int nlayers, const int* dims, unsigned long long
   m->L = nlayers;
   m->dims.assign(dims, dims + nlayers + 1);
   long np = 0, nw = 0;
+  long nwt = 0;
   for (int l = 0; l < nlayers; ++l) {
+    m->w16t_off.push_back(nwt);  // 16-byte aligned layer starts for the transposed copy
+    nwt += ((long)dims[l] * dims[l + 1] + 7) / 8 * 8;
     m->w_off.push_back(np);
     m->w16_off.push_back(nw);
     np += (long)dims[l] * dims[l + 1];
@@ -298,6 +379,7 @@ extern "C" int flint_mlp_create(int nlayers, const int* dims, unsigned long long
   rc |= dalloc((void**)&m->m32, np * sizeof(float));
   rc |= dalloc((void**)&m->v32, np * sizeof(float));
   rc |= dalloc((void**)&m->w16, nw * sizeof(__bf16));
+  rc |= dalloc((void**)&m->w16t, nwt * sizeof(__bf16));
   rc |= dalloc((void**)&m->w16_index, np * sizeof(int));
   rc |= dalloc((void**)&m->loss, sizeof(float));
   if (rc) {
@@ -344,6 +426,7 @@ extern "C" int flint_mlp_set_params(flint_mlp* m, const float* host) {
   HIPCHK(hipMemcpy(m->w32, host, m->np * sizeof(float), hipMemcpyHostToDevice));
   hipLaunchKernelGGL(sync_bf16, dim3(nb(m->np, 256)), dim3(256), 0, 0, m->np, m->w32, m->w16, m->w16_index);
   HIPCHK(hipDeviceSynchronize());
+  m->wt_stale = true;
   m->adam_t = 0;
   std::vector<float> zeros(m->np, 0.0f);
   HIPCHK(hipMemcpy(m->m32, zeros.data(), m->np * sizeof(float), hipMemcpyHostToDevice));
@@ -385,5 +468,6 @@ extern "C" int flint_mlp_train_step(flint_mlp* m, long n, const float* X, const 
   hipLaunchKernelGGL(adam_kernel, dim3(nb(m->np, 256)), dim3(256), 0, s, m->np, m->w32, m->g32, m->m32, m->v32, m->w16,
                      m->w16_index, lr, b1, b2, 1e-8f, c1, c2);
   HIPCHK(hipGetLastError());
+  m->wt_stale = true;
   return FLINT_OK;
 }

@@ -49,6 +49,7 @@ def lib():
         L.flint_rhs_host.argtypes = [vp, l, vp, vp, vp]
         L.flint_reactor_advance_host.argtypes = [vp, l, i, vp, vp, d, d, d, i, vp]
         L.flint_cea_solve_host.argtypes = [vp, l, i, vp, vp, vp, vp, vp, vp]
+        L.flint_cea_solve_warm.argtypes = [vp, vp, l, l, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -190,6 +191,17 @@ class Mechanism:
         n = ld if n is None else n
         check(lib().flint_cea_solve(self._h, n, ld, int(hp), _p(p), _p(T), _p(rho), _p(guess), _p(T_out), _p(Y_out),
                                     _p(iters), _p(status), _stream(stream)))
+
+    def equilibrium_warm(self, net, scaling, T, rho, T_out, Y_out, p, iters=None, status=None, n=None, stream=None):
+        """HP equilibria warm-started by the network net (flint_amd.mlp.MLP, ns+2 -> ns+1) with the
+        standardisation scaling = (in_mean, in_scale, out_mean, out_scale) (flint_cea_solve_warm)."""
+        ld = T.shape[-1]
+        n = ld if n is None else n
+        sc = np.ascontiguousarray(np.concatenate([np.asarray(a, dtype=np.float64) for a in scaling]))
+        if sc.size != 4 * self.ns + 6:
+            raise FlintError("scaling must hold 4 ns + 6 values")
+        check(lib().flint_cea_solve_warm(self._h, net._h, n, ld, _p(p), _p(T), _p(rho), _p(sc), _p(T_out), _p(Y_out),
+                                         _p(iters), _p(status), _stream(stream)))
 
     # ---- host-memory convenience (numpy, SoA [rows, n]) ----
     def source_host(self, rho, T):

@@ -4,7 +4,8 @@ An MFMA MLP (flint_amd.mlp, 26 -> 128 -> 128 -> 25 for TSR-GP-24) maps the unbur
 fractions Y0_k, mixture enthalpy h0 = sum_k Y0_k h_k(T0), log p) to the equilibrium (T, ln n_k);
 flint_cea_solve starts its Newton iteration there.  States that do not converge from the predicted
 guess are solved again from the reference's default guess, so the warm-started batch returns
-exactly the states the default path returns a solution for.
+exactly the states the default path returns a solution for.  `solve` runs all of it in one native
+call (flint_cea_solve_warm); `guess` is the same network evaluated through torch, for inspection.
 
     python -m flint_amd.warmstart --states 262144 --out flint_amd/data/cea_warmstart_gp24.npz
 """
@@ -75,27 +76,12 @@ class WarmStart:
         O = O * torch.from_numpy(self.ys.scale).to(dev) + torch.from_numpy(self.ys.mean).to(dev)
         return O.T.contiguous()
 
-    def solve(self, mech, md, Y0, T0, p_bar, T_out, Y_out, iters=None, status=None):
-        """HP equilibrium of the states warm-started by the network; states that fail from the
-        guess are solved again from the default guess.  Returns the number re-solved."""
-        import torch
-        n = T0.shape[0]
-        it = torch.empty(n, dtype=torch.int32, device=T0.device) if iters is None else iters
-        st = torch.empty(n, dtype=torch.int32, device=T0.device) if status is None else status
-        g = self.guess(md, Y0, T0, p_bar)
-        mech.equilibrium(T0, Y0, T_out, Y_out, p=p_bar, hp=True, guess=g, iters=it, status=st)
-        bad = torch.nonzero(st != 0).flatten()
-        nb = int(bad.numel())
-        if nb:
-            Tb, Yb = torch.empty(nb, dtype=T0.dtype, device=T0.device), torch.empty((md.ns, nb), dtype=T0.dtype,
-                                                                                   device=T0.device)
-            itb, stb = torch.empty(nb, dtype=torch.int32, device=T0.device), torch.empty(nb, dtype=torch.int32,
-                                                                                        device=T0.device)
-            mech.equilibrium(T0[bad].contiguous(), Y0[:, bad].contiguous(), Tb, Yb, p=p_bar[bad].contiguous(),
-                             hp=True, iters=itb, status=stb)
-            T_out[bad], Y_out[:, bad], st[bad] = Tb, Yb, stb
-            it[bad] += itb
-        return nb
+    def solve(self, mech, md, Y0, T0, p_bar, T_out, Y_out, iters=None, status=None, stream=None):
+        """HP equilibrium of the states warm-started by the network, in one native call
+        (flint_cea_solve_warm: features, MFMA network, warm solve, and a second pass from the
+        default guess for the states that did not converge from theirs)."""
+        mech.equilibrium_warm(self.net, (self.xs.mean, self.xs.scale, self.ys.mean, self.ys.scale), T0, Y0, T_out,
+                              Y_out, p_bar, iters=iters, status=status, stream=stream)
 
 
 def train(mech, md, n=1 << 18, seed=41, epochs=60, batch=4096, lr=2e-3, log=print):

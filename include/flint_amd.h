@@ -155,12 +155,25 @@ int flint_cea_solve(flint_mech *m, long n, long ld, int hp, const double *p_dev,
 int flint_cea_solve_host(flint_mech *m, long n, int hp, const double *p, const double *T, const double *rho,
                          double *T_out, double *Y_out, int *status);
 
+/* Learned warm start of HP equilibria (no reference counterpart): the network net (ns+2 inputs,
+ * ns+1 outputs, see flint_mlp_create below) maps the standardised (Y0_k, h0, ln p) of each state
+ * (h0 = sum_k Y0_k h_k(T0)) to the standardised Newton start (T, ln n_1..ln n_ns); scaling (host,
+ * 4 ns + 6 doubles) = in_mean[ns+2], in_scale[ns+2], out_mean[ns+1], out_scale[ns+1].  The batch is
+ * solved as flint_cea_solve with hp = 1 from those starts, and the states that do not converge
+ * from theirs are solved again from the default start in a second pass over the batch (iters_dev
+ * then counts both), so the results are those of the default path wherever it converges.
+ * Asynchronous on stream, no host synchronisation.  Arguments as in flint_cea_solve (rho_dev =
+ * input mass fractions, p_dev in bar); status_dev may be NULL. */
+typedef struct flint_mlp flint_mlp;
+int flint_cea_solve_warm(flint_mech *m, flint_mlp *net, long n, long ld, const double *p_dev, const double *T_dev,
+                         const double *rho_dev, const double *scaling, double *T_out_dev, double *Y_out_dev,
+                         int *iters_dev, int *status_dev, void *stream);
+
 /* Multilayer perceptron on MFMA (no reference counterpart: the learned reactor step and the
  * equilibrium warm start of this library).  Layers d_0 -> d_1 -> ... -> d_L with GELU on every
  * hidden layer and a linear output; BF16 weights and activations, FP32 accumulation, FP32 master
  * weights.  Parameters are a flat FP32 vector, per layer W_l [d_l][d_{l+1}] row-major then b_l.
  * Batches are row-major FP32 device arrays X [n][d_0], targets/outputs [n][d_L]. */
-typedef struct flint_mlp flint_mlp;
 int flint_mlp_create(int nlayers, const int *dims, unsigned long long seed, flint_mlp **out);
 void flint_mlp_destroy(flint_mlp *m);
 int flint_mlp_num_params(const flint_mlp *m, long *count);
