@@ -58,6 +58,11 @@ def _dprod_expr(fac, j):
     return "*".join(terms) if terms else "1.0"
 
 
+# Launch shape of the source-term/Jacobian kernel: threads per block, the waves per SIMD the
+# compiler is held to (0: its choice) and how many per-cell scratch values live in LDS
+JAC_OPTS = dict(block=128, waves=0, lds_slots=1 << 30)
+
+
 def emit(prog):
     mid = ident(prog.name)
     ns = prog.ns
@@ -83,11 +88,17 @@ def emit(prog):
         if r.kind in ("troe", "lindemann"):
             slot[("K1", r.index)] = len(slot)
             slot[("K2", r.index)] = len(slot)
-    for k in range(ns):
-        slot[("e", k)] = len(slot)
+    # slots past the LDS budget stay in registers
+    keep = list(slot)[:JAC_OPTS["lds_slots"]]
+    reg = [k for k in slot if k not in keep]
+    slot = {k: i for i, k in enumerate(keep)}
     nslot = len(slot)
 
     def S(kind, i):
+        if kind == "e":
+            return f"e{i}"
+        if (kind, i) not in slot:
+            return f"v{kind}{i}"
         return f"scr[{slot[(kind, i)]} * SST]"
 
     L = []
@@ -103,17 +114,21 @@ def emit(prog):
     a("// RHS applies rho = max(rho, 0) as rhs_native does (Lib_Chemistry_rhs.f90:37-39) and adds dT/dt.")
     a("// With JAC, scr[s*SST] (s < NSLOT) is per-cell scratch for values reused by the columns.")
     a("// Returns a status: bit 0 T outside the kinetics table, bit 1 T outside the thermo table.")
-    a("template <bool JAC, bool RHS, int SST, class Sink>")
+    a("// With JAC, fout(F) receives F as soon as it is final, so that F need not stay live across the")
+    a("// Jacobian columns.")
+    a("template <bool JAC, bool RHS, int SST, class Sink, class FOut>")
     a("__device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min, int Tk_max,")
     a("                                    const double* __restrict__ tabh, int Th_min, int Th_max,")
     a("                                    const double (&r)[NS], double T, double (&F)[NZ],")
-    a("                                    double* __restrict__ scr, Sink&& sink) {")
+    a("                                    double* __restrict__ scr, const double* __restrict__ zr, long zld,")
+    a("                                    Sink&& sink, FOut&& fout) {")
     a("  int status = 0;")
     a("  if (!(T >= (double)Tk_min && T <= (double)Tk_max)) status |= 1;")
     a("  if (RHS && !(T >= (double)Th_min && T <= (double)Th_max)) {")
     a("    status |= 2;")
     a("#pragma unroll")
     a("    for (int k = 0; k < NZ; ++k) F[k] = -1.0;")
+    a("    if (JAC) fout(F);")
     a("    if (JAC) {")
     a("      for (int x = 0; x < NZ; ++x)")
     a("        for (int y = 0; y < NZ; ++y) sink(x, y, 0.0);")
@@ -147,6 +162,8 @@ def emit(prog):
             a(f"  double w{k} = 0.0;")
         else:
             a(f"  double w{k};")
+    if reg:
+        a(f"  [[maybe_unused]] double {', '.join(f'v{k}{i}' for k, i in reg)};")
     a("  // JAC: dT-derivative accumulators")
     a("  [[maybe_unused]] double dT[NS] = {};")
 
@@ -244,15 +261,20 @@ def emit(prog):
     a("  const double* __restrict__ rh1 = rh0 + WH;")
     a("  double eiroi = 0.0, rho_cv = 0.0;")
     a("  [[maybe_unused]] double s_de = 0.0, s_dcv = 0.0;")
+    a(f"  [[maybe_unused]] double {', '.join(f'e{k}' for k in range(ns))};")
+    a("  // JAC: rho_k is read again from zr[k*zld] here rather than kept live through the rates")
+    a("  [[maybe_unused]] const double* __restrict__ zre = JAC ? opaque(zr) : nullptr;")
     for k in range(ns):
         Rk = RUNIV / W[k]
         a(f"  {{ const double2 t0 = ld2(rh0 + {2 * k}), t1 = ld2(rh1 + {2 * k});")
         a("    const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;")
         a(f"    const double ek = hk - {_lit(Rk)} * T;")
         a(f"    eiroi = eiroi + ek * F[{k}];")
-        a(f"    rho_cv = rho_cv + r{k} * (cpk - {_lit(Rk)});")
-        a(f"    if (JAC) {{ {S('e', k)} = ek; s_de += ((t1.y - t0.y) - {_lit(Rk)}) * F[{k}]; s_dcv += r{k} * (t1.x - t0.x); }} }}")
+        a(f"    const double rr = JAC ? fmax(zre[{k} * zld], 0.0) : r{k};")
+        a(f"    rho_cv = rho_cv + rr * (cpk - {_lit(Rk)});")
+        a(f"    if (JAC) {{ {S('e', k)} = ek; s_de += ((t1.y - t0.y) - {_lit(Rk)}) * F[{k}]; s_dcv += rr * (t1.x - t0.x); }} }}")
     a("  F[NS] = -eiroi / rho_cv;")
+    a("  if (JAC) fout(F);")
     a("  if (!JAC) return status;")
     a("  const double Tdot = F[NS], invD = 1.0 / rho_cv;")
     a("  const double* const rk0_ = rk0;")
@@ -340,17 +362,20 @@ def emit(prog):
     a("  double r[NS], F[NZ];")
     a("#pragma unroll")
     a("  for (int k = 0; k < NS; ++k) r[k] = rho[k * ld + i];")
-    a("  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, T[i], F, nullptr, NullSink{});")
+    a("  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, T[i], F, nullptr, nullptr, 0,")
+    a("      NullSink{}, NullOut{});")
     a("#pragma unroll")
     a("  for (int k = 0; k < NS; ++k) wdot[k * ld + i] = F[k];")
     a("  if (status) status[i] = st;")
     a("}")
     a("")
-    a("constexpr int JAC_BLOCK = 128;")
+    a(f"constexpr int JAC_BLOCK = {JAC_OPTS['block']};")
     a("// rhs_native and its Jacobian over a batch: Z[k*ld+i] (rho_1..rho_NS, T) -> F[k*ld+i],")
     a("// J[(row*NZ+col)*ld+i] (J may be null: RHS only).")
     a("template <bool JAC>")
-    a("__global__ __launch_bounds__(JAC_BLOCK) void rhs_jac_kernel(long n, long ld, const double* __restrict__ Z,")
+    wpe = JAC_OPTS["waves"]
+    att = f" __attribute__((amdgpu_waves_per_eu({wpe}, {wpe})))" if wpe else ""
+    a(f"__global__ __launch_bounds__(JAC_BLOCK){att} void rhs_jac_kernel(long n, long ld, const double* __restrict__ Z,")
     a("    double* __restrict__ Fo, double* __restrict__ J, int* __restrict__ status,")
     a("    const double* __restrict__ tabk, int Tk_min, int Tk_max, const double* __restrict__ tabh,")
     a("    int Th_min, int Th_max) {")
@@ -363,9 +388,16 @@ def emit(prog):
     a("  double* __restrict__ Ji = J + i;")
     a("  __shared__ double smem[JAC ? NSLOT * JAC_BLOCK : 1];")
     a("  const int st = eval<JAC, true, JAC_BLOCK>(tabk, Tk_min, Tk_max, tabh, Th_min, Th_max, r, T, F, smem + threadIdx.x,")
-    a("      [&](int row, int col, double v) { stream_store(Ji + (long)(row * NZ + col) * ld, v); });")
+    a("      Z + i, ld,")
+    a("      [&](int row, int col, double v) { stream_store(Ji + (long)(row * NZ + col) * ld, v); },")
+    a("      [&](const double (&Fv)[NZ]) {")
     a("#pragma unroll")
-    a("  for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = F[k];")
+    a("        for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = Fv[k];")
+    a("      });")
+    a("  if (!JAC) {")
+    a("#pragma unroll")
+    a("    for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = F[k];")
+    a("  }")
     a("  if (status) status[i] = st;")
     a("}")
     # host copy of the rate program for the reaction-parallel reactor kernel

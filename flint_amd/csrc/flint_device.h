@@ -27,6 +27,11 @@ struct NullSink {
   __device__ __forceinline__ void operator()(int, int, double) const {}
 };
 
+struct NullOut {
+  template <class A>
+  __device__ __forceinline__ void operator()(const A&) const {}
+};
+
 #define FLINT_CHECK_LAUNCH() hipGetLastError()
 
 // Registers the launchers of one generated mechanism with the C-ABI dispatcher.

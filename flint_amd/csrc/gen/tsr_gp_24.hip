@@ -2,24 +2,28 @@
 // 24 species, 110 reactions (102 Arrhenius/three-body, 8 Troe, 0 Lindemann).
 #include "../flint_device.h"
 namespace flint_gen_tsr_gp_24 {
-constexpr int NS = 24, NZ = 25, WK = 236, WH = 48, NSLOT = 65;
+constexpr int NS = 24, NZ = 25, WK = 236, WH = 48, NSLOT = 41;
 
 // Rates, net production and (JAC) the analytic Jacobian of the constant-volume RHS for one cell.
 // r[k] = rho_k, T; outputs F[0..NS-1] = wdot, F[NS] = dT/dt (rhs_native); sink(row, col, v).
 // RHS applies rho = max(rho, 0) as rhs_native does (Lib_Chemistry_rhs.f90:37-39) and adds dT/dt.
 // With JAC, scr[s*SST] (s < NSLOT) is per-cell scratch for values reused by the columns.
 // Returns a status: bit 0 T outside the kinetics table, bit 1 T outside the thermo table.
-template <bool JAC, bool RHS, int SST, class Sink>
+// With JAC, fout(F) receives F as soon as it is final, so that F need not stay live across the
+// Jacobian columns.
+template <bool JAC, bool RHS, int SST, class Sink, class FOut>
 __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min, int Tk_max,
                                     const double* __restrict__ tabh, int Th_min, int Th_max,
                                     const double (&r)[NS], double T, double (&F)[NZ],
-                                    double* __restrict__ scr, Sink&& sink) {
+                                    double* __restrict__ scr, const double* __restrict__ zr, long zld,
+                                    Sink&& sink, FOut&& fout) {
   int status = 0;
   if (!(T >= (double)Tk_min && T <= (double)Tk_max)) status |= 1;
   if (RHS && !(T >= (double)Th_min && T <= (double)Th_max)) {
     status |= 2;
 #pragma unroll
     for (int k = 0; k < NZ; ++k) F[k] = -1.0;
+    if (JAC) fout(F);
     if (JAC) {
       for (int x = 0; x < NZ; ++x)
         for (int y = 0; y < NZ; ++y) sink(x, y, 0.0);
@@ -2365,180 +2369,208 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   const double* __restrict__ rh1 = rh0 + WH;
   double eiroi = 0.0, rho_cv = 0.0;
   [[maybe_unused]] double s_de = 0.0, s_dcv = 0.0;
+  [[maybe_unused]] double e0, e1, e2, e3, e4, e5, e6, e7, e8, e9, e10, e11, e12, e13, e14, e15, e16, e17, e18, e19, e20, e21, e22, e23;
+  // JAC: rho_k is read again from zr[k*zld] here rather than kept live through the rates
+  [[maybe_unused]] const double* __restrict__ zre = JAC ? opaque(zr) : nullptr;
   { const double2 t0 = ld2(rh0 + 0), t1 = ld2(rh1 + 0);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 251.90904683996848 * T;
     eiroi = eiroi + ek * F[0];
-    rho_cv = rho_cv + r0 * (cpk - 251.90904683996848);
-    if (JAC) { scr[41 * SST] = ek; s_de += ((t1.y - t0.y) - 251.90904683996848) * F[0]; s_dcv += r0 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[0 * zld], 0.0) : r0;
+    rho_cv = rho_cv + rr * (cpk - 251.90904683996848);
+    if (JAC) { e0 = ek; s_de += ((t1.y - t0.y) - 251.90904683996848) * F[0]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 2), t1 = ld2(rh1 + 2);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 244.44375845240194 * T;
     eiroi = eiroi + ek * F[1];
-    rho_cv = rho_cv + r1 * (cpk - 244.44375845240194);
-    if (JAC) { scr[42 * SST] = ek; s_de += ((t1.y - t0.y) - 244.44375845240194) * F[1]; s_dcv += r1 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[1 * zld], 0.0) : r1;
+    rho_cv = rho_cv + rr * (cpk - 244.44375845240194);
+    if (JAC) { e1 = ek; s_de += ((t1.y - t0.y) - 244.44375845240194) * F[1]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 4), t1 = ld2(rh1 + 4);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 296.84077115315955 * T;
     eiroi = eiroi + ek * F[2];
-    rho_cv = rho_cv + r2 * (cpk - 296.84077115315955);
-    if (JAC) { scr[43 * SST] = ek; s_de += ((t1.y - t0.y) - 296.84077115315955) * F[2]; s_dcv += r2 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[2 * zld], 0.0) : r2;
+    rho_cv = rho_cv + rr * (cpk - 296.84077115315955);
+    if (JAC) { e2 = ek; s_de += ((t1.y - t0.y) - 296.84077115315955) * F[2]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 6), t1 = ld2(rh1 + 6);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 488.8875169048039 * T;
     eiroi = eiroi + ek * F[3];
-    rho_cv = rho_cv + r3 * (cpk - 488.8875169048039);
-    if (JAC) { scr[44 * SST] = ek; s_de += ((t1.y - t0.y) - 488.8875169048039) * F[3]; s_dcv += r3 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[3 * zld], 0.0) : r3;
+    rho_cv = rho_cv + rr * (cpk - 488.8875169048039);
+    if (JAC) { e3 = ek; s_de += ((t1.y - t0.y) - 488.8875169048039) * F[3]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 8), t1 = ld2(rh1 + 8);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 4124.2609126984125 * T;
     eiroi = eiroi + ek * F[4];
-    rho_cv = rho_cv + r4 * (cpk - 4124.2609126984125);
-    if (JAC) { scr[45 * SST] = ek; s_de += ((t1.y - t0.y) - 4124.2609126984125) * F[4]; s_dcv += r4 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[4 * zld], 0.0) : r4;
+    rho_cv = rho_cv + rr * (cpk - 4124.2609126984125);
+    if (JAC) { e4 = ek; s_de += ((t1.y - t0.y) - 4124.2609126984125) * F[4]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 10), t1 = ld2(rh1 + 10);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 286.52939554759115 * T;
     eiroi = eiroi + ek * F[5];
-    rho_cv = rho_cv + r5 * (cpk - 286.52939554759115);
-    if (JAC) { scr[46 * SST] = ek; s_de += ((t1.y - t0.y) - 286.52939554759115) * F[5]; s_dcv += r5 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[5 * zld], 0.0) : r5;
+    rho_cv = rho_cv + rr * (cpk - 286.52939554759115);
+    if (JAC) { e5 = ek; s_de += ((t1.y - t0.y) - 286.52939554759115) * F[5]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 12), t1 = ld2(rh1 + 12);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 267.9161564735452 * T;
     eiroi = eiroi + ek * F[6];
-    rho_cv = rho_cv + r6 * (cpk - 267.9161564735452);
-    if (JAC) { scr[47 * SST] = ek; s_de += ((t1.y - t0.y) - 267.9161564735452) * F[6]; s_dcv += r6 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[6 * zld], 0.0) : r6;
+    rho_cv = rho_cv + rr * (cpk - 267.9161564735452);
+    if (JAC) { e6 = ek; s_de += ((t1.y - t0.y) - 267.9161564735452) * F[6]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 14), t1 = ld2(rh1 + 14);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 592.7504099237186 * T;
     eiroi = eiroi + ek * F[7];
-    rho_cv = rho_cv + r7 * (cpk - 592.7504099237186);
-    if (JAC) { scr[48 * SST] = ek; s_de += ((t1.y - t0.y) - 592.7504099237186) * F[7]; s_dcv += r7 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[7 * zld], 0.0) : r7;
+    rho_cv = rho_cv + rr * (cpk - 592.7504099237186);
+    if (JAC) { e7 = ek; s_de += ((t1.y - t0.y) - 592.7504099237186) * F[7]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 16), t1 = ld2(rh1 + 16);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 8248.521825396825 * T;
     eiroi = eiroi + ek * F[8];
-    rho_cv = rho_cv + r8 * (cpk - 8248.521825396825);
-    if (JAC) { scr[49 * SST] = ek; s_de += ((t1.y - t0.y) - 8248.521825396825) * F[8]; s_dcv += r8 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[8 * zld], 0.0) : r8;
+    rho_cv = rho_cv + rr * (cpk - 8248.521825396825);
+    if (JAC) { e8 = ek; s_de += ((t1.y - t0.y) - 8248.521825396825) * F[8]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 18), t1 = ld2(rh1 + 18);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 519.689355584724 * T;
     eiroi = eiroi + ek * F[9];
-    rho_cv = rho_cv + r9 * (cpk - 519.689355584724);
-    if (JAC) { scr[50 * SST] = ek; s_de += ((t1.y - t0.y) - 519.689355584724) * F[9]; s_dcv += r9 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[9 * zld], 0.0) : r9;
+    rho_cv = rho_cv + rr * (cpk - 519.689355584724);
+    if (JAC) { e9 = ek; s_de += ((t1.y - t0.y) - 519.689355584724) * F[9]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 20), t1 = ld2(rh1 + 20);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 218.52111750637337 * T;
     eiroi = eiroi + ek * F[10];
-    rho_cv = rho_cv + r10 * (cpk - 218.52111750637337);
-    if (JAC) { scr[51 * SST] = ek; s_de += ((t1.y - t0.y) - 218.52111750637337) * F[10]; s_dcv += r10 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[10 * zld], 0.0) : r10;
+    rho_cv = rho_cv + rr * (cpk - 218.52111750637337);
+    if (JAC) { e10 = ek; s_de += ((t1.y - t0.y) - 218.52111750637337) * F[10]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 22), t1 = ld2(rh1 + 22);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 553.0103092783505 * T;
     eiroi = eiroi + ek * F[11];
-    rho_cv = rho_cv + r11 * (cpk - 553.0103092783505);
-    if (JAC) { scr[52 * SST] = ek; s_de += ((t1.y - t0.y) - 553.0103092783505) * F[11]; s_dcv += r11 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[11 * zld], 0.0) : r11;
+    rho_cv = rho_cv + rr * (cpk - 553.0103092783505);
+    if (JAC) { e11 = ek; s_de += ((t1.y - t0.y) - 553.0103092783505) * F[11]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 24), t1 = ld2(rh1 + 24);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 319.3221445579538 * T;
     eiroi = eiroi + ek * F[12];
-    rho_cv = rho_cv + r12 * (cpk - 319.3221445579538);
-    if (JAC) { scr[53 * SST] = ek; s_de += ((t1.y - t0.y) - 319.3221445579538) * F[12]; s_dcv += r12 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[12 * zld], 0.0) : r12;
+    rho_cv = rho_cv + rr * (cpk - 319.3221445579538);
+    if (JAC) { e12 = ek; s_de += ((t1.y - t0.y) - 319.3221445579538) * F[12]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 26), t1 = ld2(rh1 + 26);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 276.9103443682142 * T;
     eiroi = eiroi + ek * F[13];
-    rho_cv = rho_cv + r13 * (cpk - 276.9103443682142);
-    if (JAC) { scr[54 * SST] = ek; s_de += ((t1.y - t0.y) - 276.9103443682142) * F[13]; s_dcv += r13 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[13 * zld], 0.0) : r13;
+    rho_cv = rho_cv + rr * (cpk - 276.9103443682142);
+    if (JAC) { e13 = ek; s_de += ((t1.y - t0.y) - 276.9103443682142) * F[13]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 28), t1 = ld2(rh1 + 28);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 212.8814297053025 * T;
     eiroi = eiroi + ek * F[14];
-    rho_cv = rho_cv + r14 * (cpk - 212.8814297053025);
-    if (JAC) { scr[55 * SST] = ek; s_de += ((t1.y - t0.y) - 212.8814297053025) * F[14]; s_dcv += r14 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[14 * zld], 0.0) : r14;
+    rho_cv = rho_cv + rr * (cpk - 212.8814297053025);
+    if (JAC) { e14 = ek; s_de += ((t1.y - t0.y) - 212.8814297053025) * F[14]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 30), t1 = ld2(rh1 + 30);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 332.1817818617659 * T;
     eiroi = eiroi + ek * F[15];
-    rho_cv = rho_cv + r15 * (cpk - 332.1817818617659);
-    if (JAC) { scr[56 * SST] = ek; s_de += ((t1.y - t0.y) - 332.1817818617659) * F[15]; s_dcv += r15 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[15 * zld], 0.0) : r15;
+    rho_cv = rho_cv + rr * (cpk - 332.1817818617659);
+    if (JAC) { e15 = ek; s_de += ((t1.y - t0.y) - 332.1817818617659) * F[15]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 32), t1 = ld2(rh1 + 32);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 461.53261171246186 * T;
     eiroi = eiroi + ek * F[16];
-    rho_cv = rho_cv + r16 * (cpk - 461.53261171246186);
-    if (JAC) { scr[57 * SST] = ek; s_de += ((t1.y - t0.y) - 461.53261171246186) * F[16]; s_dcv += r16 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[16 * zld], 0.0) : r16;
+    rho_cv = rho_cv + rr * (cpk - 461.53261171246186);
+    if (JAC) { e16 = ek; s_de += ((t1.y - t0.y) - 461.53261171246186) * F[16]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 34), t1 = ld2(rh1 + 34);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 176.78034571471096 * T;
     eiroi = eiroi + ek * F[17];
-    rho_cv = rho_cv + r17 * (cpk - 176.78034571471096);
-    if (JAC) { scr[58 * SST] = ek; s_de += ((t1.y - t0.y) - 176.78034571471096) * F[17]; s_dcv += r17 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[17 * zld], 0.0) : r17;
+    rho_cv = rho_cv + rr * (cpk - 176.78034571471096);
+    if (JAC) { e17 = ek; s_de += ((t1.y - t0.y) - 176.78034571471096) * F[17]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 36), t1 = ld2(rh1 + 36);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 259.844677792362 * T;
     eiroi = eiroi + ek * F[18];
-    rho_cv = rho_cv + r18 * (cpk - 259.844677792362);
-    if (JAC) { scr[59 * SST] = ek; s_de += ((t1.y - t0.y) - 259.844677792362) * F[18]; s_dcv += r18 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[18 * zld], 0.0) : r18;
+    rho_cv = rho_cv + rr * (cpk - 259.844677792362);
+    if (JAC) { e18 = ek; s_de += ((t1.y - t0.y) - 259.844677792362) * F[18]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 38), t1 = ld2(rh1 + 38);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 592.7504099237186 * T;
     eiroi = eiroi + ek * F[19];
-    rho_cv = rho_cv + r19 * (cpk - 592.7504099237186);
-    if (JAC) { scr[60 * SST] = ek; s_de += ((t1.y - t0.y) - 592.7504099237186) * F[19]; s_dcv += r19 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[19 * zld], 0.0) : r19;
+    rho_cv = rho_cv + rr * (cpk - 592.7504099237186);
+    if (JAC) { e19 = ek; s_de += ((t1.y - t0.y) - 592.7504099237186) * F[19]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 40), t1 = ld2(rh1 + 40);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 276.50515463917526 * T;
     eiroi = eiroi + ek * F[20];
-    rho_cv = rho_cv + r20 * (cpk - 276.50515463917526);
-    if (JAC) { scr[61 * SST] = ek; s_de += ((t1.y - t0.y) - 276.50515463917526) * F[20]; s_dcv += r20 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[20 * zld], 0.0) : r20;
+    rho_cv = rho_cv + rr * (cpk - 276.50515463917526);
+    if (JAC) { e20 = ek; s_de += ((t1.y - t0.y) - 276.50515463917526) * F[20]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 42), t1 = ld2(rh1 + 42);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 518.264040391448 * T;
     eiroi = eiroi + ek * F[21];
-    rho_cv = rho_cv + r21 * (cpk - 518.264040391448);
-    if (JAC) { scr[62 * SST] = ek; s_de += ((t1.y - t0.y) - 518.264040391448) * F[21]; s_dcv += r21 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[21 * zld], 0.0) : r21;
+    rho_cv = rho_cv + rr * (cpk - 518.264040391448);
+    if (JAC) { e21 = ek; s_de += ((t1.y - t0.y) - 518.264040391448) * F[21]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 44), t1 = ld2(rh1 + 44);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 184.69711442344 * T;
     eiroi = eiroi + ek * F[22];
-    rho_cv = rho_cv + r22 * (cpk - 184.69711442344);
-    if (JAC) { scr[63 * SST] = ek; s_de += ((t1.y - t0.y) - 184.69711442344) * F[22]; s_dcv += r22 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[22 * zld], 0.0) : r22;
+    rho_cv = rho_cv + rr * (cpk - 184.69711442344);
+    if (JAC) { e22 = ek; s_de += ((t1.y - t0.y) - 184.69711442344) * F[22]; s_dcv += rr * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 46), t1 = ld2(rh1 + 46);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 188.92749210388786 * T;
     eiroi = eiroi + ek * F[23];
-    rho_cv = rho_cv + r23 * (cpk - 188.92749210388786);
-    if (JAC) { scr[64 * SST] = ek; s_de += ((t1.y - t0.y) - 188.92749210388786) * F[23]; s_dcv += r23 * (t1.x - t0.x); } }
+    const double rr = JAC ? fmax(zre[23 * zld], 0.0) : r23;
+    rho_cv = rho_cv + rr * (cpk - 188.92749210388786);
+    if (JAC) { e23 = ek; s_de += ((t1.y - t0.y) - 188.92749210388786) * F[23]; s_dcv += rr * (t1.x - t0.x); } }
   F[NS] = -eiroi / rho_cv;
+  if (JAC) fout(F);
   if (!JAC) return status;
   const double Tdot = F[NS], invD = 1.0 / rho_cv;
   const double* const rk0_ = rk0;
   {  // column NS: d/dT
     double acc = s_de;
-    { const double v = 33.006 * dT[0]; sink(0, NS, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * dT[1]; sink(1, NS, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * dT[2]; sink(2, NS, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * dT[3]; sink(3, NS, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * dT[4]; sink(4, NS, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * dT[5]; sink(5, NS, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * dT[6]; sink(6, NS, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * dT[7]; sink(7, NS, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * dT[8]; sink(8, NS, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * dT[9]; sink(9, NS, v); acc += scr[50 * SST] * v; }
-    { const double v = 38.049 * dT[10]; sink(10, NS, v); acc += scr[51 * SST] * v; }
-    { const double v = 15.035 * dT[11]; sink(11, NS, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * dT[12]; sink(12, NS, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * dT[13]; sink(13, NS, v); acc += scr[54 * SST] * v; }
-    { const double v = 39.057 * dT[14]; sink(14, NS, v); acc += scr[55 * SST] * v; }
-    { const double v = 25.03 * dT[15]; sink(15, NS, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * dT[16]; sink(16, NS, v); acc += scr[57 * SST] * v; }
-    { const double v = 47.033 * dT[17]; sink(17, NS, v); acc += scr[58 * SST] * v; }
-    { const double v = 31.998 * dT[18]; sink(18, NS, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * dT[19]; sink(19, NS, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * dT[20]; sink(20, NS, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * dT[21]; sink(21, NS, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * dT[22]; sink(22, NS, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * dT[23]; sink(23, NS, v); acc += scr[64 * SST] * v; }
+    { const double v = 33.006 * dT[0]; sink(0, NS, v); acc += e0 * v; }
+    { const double v = 34.014 * dT[1]; sink(1, NS, v); acc += e1 * v; }
+    { const double v = 28.01 * dT[2]; sink(2, NS, v); acc += e2 * v; }
+    { const double v = 17.007 * dT[3]; sink(3, NS, v); acc += e3 * v; }
+    { const double v = 2.016 * dT[4]; sink(4, NS, v); acc += e4 * v; }
+    { const double v = 29.018 * dT[5]; sink(5, NS, v); acc += e5 * v; }
+    { const double v = 31.034 * dT[6]; sink(6, NS, v); acc += e6 * v; }
+    { const double v = 14.027 * dT[7]; sink(7, NS, v); acc += e7 * v; }
+    { const double v = 1.008 * dT[8]; sink(8, NS, v); acc += e8 * v; }
+    { const double v = 15.999 * dT[9]; sink(9, NS, v); acc += e9 * v; }
+    { const double v = 38.049 * dT[10]; sink(10, NS, v); acc += e10 * v; }
+    { const double v = 15.035 * dT[11]; sink(11, NS, v); acc += e11 * v; }
+    { const double v = 26.038 * dT[12]; sink(12, NS, v); acc += e12 * v; }
+    { const double v = 30.026 * dT[13]; sink(13, NS, v); acc += e13 * v; }
+    { const double v = 39.057 * dT[14]; sink(14, NS, v); acc += e14 * v; }
+    { const double v = 25.03 * dT[15]; sink(15, NS, v); acc += e15 * v; }
+    { const double v = 18.015 * dT[16]; sink(16, NS, v); acc += e16 * v; }
+    { const double v = 47.033 * dT[17]; sink(17, NS, v); acc += e17 * v; }
+    { const double v = 31.998 * dT[18]; sink(18, NS, v); acc += e18 * v; }
+    { const double v = 14.027 * dT[19]; sink(19, NS, v); acc += e19 * v; }
+    { const double v = 30.07 * dT[20]; sink(20, NS, v); acc += e20 * v; }
+    { const double v = 16.043 * dT[21]; sink(21, NS, v); acc += e21 * v; }
+    { const double v = 45.017 * dT[22]; sink(22, NS, v); acc += e22 * v; }
+    { const double v = 44.009 * dT[23]; sink(23, NS, v); acc += e23 * v; }
     sink(NS, NS, -(acc + Tdot * s_dcv) * invD);
   }
   {  // column 0: d/drho_0 (HO2)
@@ -2841,30 +2873,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 0, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 0, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 0, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 0, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 0, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 0, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 0, v); acc += scr[47 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 0, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 0, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 0, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 0, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 0, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 0, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 0, v); acc += e6 * v; }
     sink(7, 0, 0.0);
-    { const double v = 1.008 * gj * jc8; sink(8, 0, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 0, v); acc += scr[50 * SST] * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 0, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 0, v); acc += e9 * v; }
     sink(10, 0, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 0, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 0, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 0, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 0, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 0, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 0, v); acc += e13 * v; }
     sink(14, 0, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 0, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 0, v); acc += scr[57 * SST] * v; }
-    { const double v = 47.033 * gj * jc17; sink(17, 0, v); acc += scr[58 * SST] * v; }
-    { const double v = 31.998 * gj * jc18; sink(18, 0, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 0, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 0, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 0, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 0, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 0, v); acc += scr[64 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 0, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 0, v); acc += e16 * v; }
+    { const double v = 47.033 * gj * jc17; sink(17, 0, v); acc += e17 * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 0, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 0, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 0, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 0, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 0, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 0, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 0).x + (ld2(rh1 + 0).x - ld2(rh0 + 0).x) * Td) - 251.90904683996848;
     sink(NS, 0, -(acc + Tdot * cvj * (((negmask >> 0) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -3071,30 +3103,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 1, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 1, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 1, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 1, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 1, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 1, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 1, v); acc += scr[47 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 1, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 1, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 1, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 1, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 1, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 1, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 1, v); acc += e6 * v; }
     sink(7, 1, 0.0);
-    { const double v = 1.008 * gj * jc8; sink(8, 1, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 1, v); acc += scr[50 * SST] * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 1, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 1, v); acc += e9 * v; }
     sink(10, 1, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 1, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 1, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 1, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 1, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 1, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 1, v); acc += e13 * v; }
     sink(14, 1, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 1, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 1, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 1, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 1, v); acc += e16 * v; }
     sink(17, 1, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 1, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 1, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 1, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 1, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 1, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 1, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 1, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 1, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 1, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 1, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 1, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 1, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 2).x + (ld2(rh1 + 2).x - ld2(rh0 + 2).x) * Td) - 244.44375845240194;
     sink(NS, 1, -(acc + Tdot * cvj * (((negmask >> 1) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -3352,30 +3384,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 2, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 2, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 2, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 2, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 2, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 2, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 2, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 2, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 2, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 2, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 2, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 2, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 2, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 2, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 2, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 2, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 2, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 2, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 2, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 2, v); acc += e9 * v; }
     sink(10, 2, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 2, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 2, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 2, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 2, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 2, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 2, v); acc += e13 * v; }
     sink(14, 2, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 2, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 2, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 2, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 2, v); acc += e16 * v; }
     sink(17, 2, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 2, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 2, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 2, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 2, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 2, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 2, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 2, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 2, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 2, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 2, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 2, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 2, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 4).x + (ld2(rh1 + 4).x - ld2(rh0 + 4).x) * Td) - 296.84077115315955;
     sink(NS, 2, -(acc + Tdot * cvj * (((negmask >> 2) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -3768,30 +3800,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 3, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 3, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 3, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 3, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 3, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 3, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 3, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 3, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 3, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 3, v); acc += scr[50 * SST] * v; }
-    { const double v = 38.049 * gj * jc10; sink(10, 3, v); acc += scr[51 * SST] * v; }
-    { const double v = 15.035 * gj * jc11; sink(11, 3, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 3, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 3, v); acc += scr[54 * SST] * v; }
-    { const double v = 39.057 * gj * jc14; sink(14, 3, v); acc += scr[55 * SST] * v; }
-    { const double v = 25.03 * gj * jc15; sink(15, 3, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 3, v); acc += scr[57 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 3, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 3, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 3, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 3, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 3, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 3, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 3, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 3, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 3, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 3, v); acc += e9 * v; }
+    { const double v = 38.049 * gj * jc10; sink(10, 3, v); acc += e10 * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 3, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 3, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 3, v); acc += e13 * v; }
+    { const double v = 39.057 * gj * jc14; sink(14, 3, v); acc += e14 * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 3, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 3, v); acc += e16 * v; }
     sink(17, 3, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 3, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 3, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 3, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 3, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 3, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 3, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 3, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 3, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 3, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 3, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 3, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 3, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 6).x + (ld2(rh1 + 6).x - ld2(rh0 + 6).x) * Td) - 488.8875169048039;
     sink(NS, 3, -(acc + Tdot * cvj * (((negmask >> 3) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -4043,30 +4075,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 4, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 4, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 4, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 4, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 4, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 4, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 4, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 4, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 4, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 4, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 4, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 4, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 4, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 4, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 4, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 4, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 4, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 4, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 4, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 4, v); acc += e9 * v; }
     sink(10, 4, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 4, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 4, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 4, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 4, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 4, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 4, v); acc += e13 * v; }
     sink(14, 4, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 4, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 4, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 4, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 4, v); acc += e16 * v; }
     sink(17, 4, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 4, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 4, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 4, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 4, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 4, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 4, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 4, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 4, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 4, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 4, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 4, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 4, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 8).x + (ld2(rh1 + 8).x - ld2(rh0 + 8).x) * Td) - 4124.2609126984125;
     sink(NS, 4, -(acc + Tdot * cvj * (((negmask >> 4) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -4330,30 +4362,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 5, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 5, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 5, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 5, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 5, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 5, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 5, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 5, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 5, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 5, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 5, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 5, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 5, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 5, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 5, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 5, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 5, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 5, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 5, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 5, v); acc += e9 * v; }
     sink(10, 5, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 5, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 5, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 5, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 5, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 5, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 5, v); acc += e13 * v; }
     sink(14, 5, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 5, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 5, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 5, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 5, v); acc += e16 * v; }
     sink(17, 5, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 5, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 5, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 5, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 5, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 5, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 5, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 5, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 5, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 5, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 5, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 5, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 5, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 10).x + (ld2(rh1 + 10).x - ld2(rh0 + 10).x) * Td) - 286.52939554759115;
     sink(NS, 5, -(acc + Tdot * cvj * (((negmask >> 5) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -4570,30 +4602,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 6, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 6, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 6, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 6, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 6, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 6, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 6, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 6, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 6, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 6, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 6, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 6, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 6, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 6, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 6, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 6, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 6, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 6, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 6, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 6, v); acc += e9 * v; }
     sink(10, 6, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 6, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 6, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 6, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 6, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 6, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 6, v); acc += e13 * v; }
     sink(14, 6, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 6, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 6, v); acc += scr[57 * SST] * v; }
-    { const double v = 47.033 * gj * jc17; sink(17, 6, v); acc += scr[58 * SST] * v; }
-    { const double v = 31.998 * gj * jc18; sink(18, 6, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 6, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 6, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 6, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 6, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 6, v); acc += scr[64 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 6, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 6, v); acc += e16 * v; }
+    { const double v = 47.033 * gj * jc17; sink(17, 6, v); acc += e17 * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 6, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 6, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 6, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 6, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 6, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 6, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 12).x + (ld2(rh1 + 12).x - ld2(rh0 + 12).x) * Td) - 267.9161564735452;
     sink(NS, 6, -(acc + Tdot * cvj * (((negmask >> 6) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -4821,30 +4853,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 7, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 7, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 7, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 7, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 7, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 7, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 7, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 7, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 7, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 7, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 7, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 7, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 7, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 7, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 7, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 7, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 7, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 7, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 7, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 7, v); acc += e9 * v; }
     sink(10, 7, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 7, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 7, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 7, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 7, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 7, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 7, v); acc += e13 * v; }
     sink(14, 7, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 7, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 7, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 7, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 7, v); acc += e16 * v; }
     sink(17, 7, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 7, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 7, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 7, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 7, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 7, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 7, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 7, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 7, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 7, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 7, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 7, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 7, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 14).x + (ld2(rh1 + 14).x - ld2(rh0 + 14).x) * Td) - 592.7504099237186;
     sink(NS, 7, -(acc + Tdot * cvj * (((negmask >> 7) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -5192,30 +5224,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 8, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 8, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 8, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 8, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 8, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 8, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 8, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 8, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 8, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 8, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 8, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 8, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 8, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 8, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 8, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 8, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 8, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 8, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 8, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 8, v); acc += e9 * v; }
     sink(10, 8, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 8, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 8, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 8, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 8, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 8, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 8, v); acc += e13 * v; }
     sink(14, 8, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 8, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 8, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 8, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 8, v); acc += e16 * v; }
     sink(17, 8, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 8, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 8, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 8, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 8, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 8, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 8, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 8, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 8, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 8, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 8, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 8, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 8, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 16).x + (ld2(rh1 + 16).x - ld2(rh0 + 16).x) * Td) - 8248.521825396825;
     sink(NS, 8, -(acc + Tdot * cvj * (((negmask >> 8) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -5495,30 +5527,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 9, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 9, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 9, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 9, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 9, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 9, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 9, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 9, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 9, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 9, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 9, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 9, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 9, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 9, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 9, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 9, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 9, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 9, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 9, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 9, v); acc += e9 * v; }
     sink(10, 9, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 9, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 9, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 9, v); acc += scr[54 * SST] * v; }
-    { const double v = 39.057 * gj * jc14; sink(14, 9, v); acc += scr[55 * SST] * v; }
-    { const double v = 25.03 * gj * jc15; sink(15, 9, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 9, v); acc += scr[57 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 9, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 9, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 9, v); acc += e13 * v; }
+    { const double v = 39.057 * gj * jc14; sink(14, 9, v); acc += e14 * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 9, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 9, v); acc += e16 * v; }
     sink(17, 9, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 9, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 9, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 9, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 9, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 9, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 9, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 9, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 9, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 9, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 9, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 9, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 9, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 18).x + (ld2(rh1 + 18).x - ld2(rh0 + 18).x) * Td) - 519.689355584724;
     sink(NS, 9, -(acc + Tdot * cvj * (((negmask >> 9) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -5666,30 +5698,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 10, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 10, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 10, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 10, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 10, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 10, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 10, v); acc += scr[47 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 10, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 10, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 10, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 10, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 10, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 10, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 10, v); acc += e6 * v; }
     sink(7, 10, 0.0);
-    { const double v = 1.008 * gj * jc8; sink(8, 10, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 10, v); acc += scr[50 * SST] * v; }
-    { const double v = 38.049 * gj * jc10; sink(10, 10, v); acc += scr[51 * SST] * v; }
-    { const double v = 15.035 * gj * jc11; sink(11, 10, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 10, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 10, v); acc += scr[54 * SST] * v; }
-    { const double v = 39.057 * gj * jc14; sink(14, 10, v); acc += scr[55 * SST] * v; }
-    { const double v = 25.03 * gj * jc15; sink(15, 10, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 10, v); acc += scr[57 * SST] * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 10, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 10, v); acc += e9 * v; }
+    { const double v = 38.049 * gj * jc10; sink(10, 10, v); acc += e10 * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 10, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 10, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 10, v); acc += e13 * v; }
+    { const double v = 39.057 * gj * jc14; sink(14, 10, v); acc += e14 * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 10, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 10, v); acc += e16 * v; }
     sink(17, 10, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 10, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 10, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 10, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 10, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 10, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 10, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 10, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 10, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 10, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 10, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 10, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 10, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 20).x + (ld2(rh1 + 20).x - ld2(rh0 + 20).x) * Td) - 218.52111750637337;
     sink(NS, 10, -(acc + Tdot * cvj * (((negmask >> 10) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -5987,30 +6019,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 11, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 11, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 11, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 11, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 11, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 11, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 11, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 11, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 11, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 11, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 11, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 11, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 11, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 11, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 11, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 11, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 11, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 11, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 11, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 11, v); acc += e9 * v; }
     sink(10, 11, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 11, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 11, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 11, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 11, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 11, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 11, v); acc += e13 * v; }
     sink(14, 11, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 11, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 11, v); acc += scr[57 * SST] * v; }
-    { const double v = 47.033 * gj * jc17; sink(17, 11, v); acc += scr[58 * SST] * v; }
-    { const double v = 31.998 * gj * jc18; sink(18, 11, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 11, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 11, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 11, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 11, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 11, v); acc += scr[64 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 11, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 11, v); acc += e16 * v; }
+    { const double v = 47.033 * gj * jc17; sink(17, 11, v); acc += e17 * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 11, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 11, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 11, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 11, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 11, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 11, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 22).x + (ld2(rh1 + 22).x - ld2(rh0 + 22).x) * Td) - 553.0103092783505;
     sink(NS, 11, -(acc + Tdot * cvj * (((negmask >> 11) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -6196,30 +6228,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 12, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 12, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 12, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 12, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 12, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 12, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 12, v); acc += scr[47 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 12, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 12, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 12, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 12, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 12, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 12, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 12, v); acc += e6 * v; }
     sink(7, 12, 0.0);
-    { const double v = 1.008 * gj * jc8; sink(8, 12, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 12, v); acc += scr[50 * SST] * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 12, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 12, v); acc += e9 * v; }
     sink(10, 12, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 12, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 12, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 12, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 12, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 12, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 12, v); acc += e13 * v; }
     sink(14, 12, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 12, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 12, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 12, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 12, v); acc += e16 * v; }
     sink(17, 12, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 12, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 12, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 12, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 12, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 12, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 12, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 12, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 12, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 12, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 12, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 12, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 12, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 24).x + (ld2(rh1 + 24).x - ld2(rh0 + 24).x) * Td) - 319.3221445579538;
     sink(NS, 12, -(acc + Tdot * cvj * (((negmask >> 12) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -6500,30 +6532,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 13, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 13, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 13, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 13, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 13, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 13, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 13, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 13, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 13, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 13, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 13, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 13, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 13, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 13, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 13, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 13, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 13, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 13, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 13, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 13, v); acc += e9 * v; }
     sink(10, 13, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 13, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 13, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 13, v); acc += scr[54 * SST] * v; }
-    { const double v = 39.057 * gj * jc14; sink(14, 13, v); acc += scr[55 * SST] * v; }
-    { const double v = 25.03 * gj * jc15; sink(15, 13, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 13, v); acc += scr[57 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 13, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 13, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 13, v); acc += e13 * v; }
+    { const double v = 39.057 * gj * jc14; sink(14, 13, v); acc += e14 * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 13, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 13, v); acc += e16 * v; }
     sink(17, 13, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 13, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 13, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 13, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 13, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 13, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 13, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 13, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 13, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 13, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 13, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 13, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 13, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 26).x + (ld2(rh1 + 26).x - ld2(rh0 + 26).x) * Td) - 276.9103443682142;
     sink(NS, 13, -(acc + Tdot * cvj * (((negmask >> 13) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -6678,30 +6710,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 14, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 14, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 14, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 14, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 14, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 14, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 14, v); acc += scr[47 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 14, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 14, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 14, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 14, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 14, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 14, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 14, v); acc += e6 * v; }
     sink(7, 14, 0.0);
-    { const double v = 1.008 * gj * jc8; sink(8, 14, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 14, v); acc += scr[50 * SST] * v; }
-    { const double v = 38.049 * gj * jc10; sink(10, 14, v); acc += scr[51 * SST] * v; }
-    { const double v = 15.035 * gj * jc11; sink(11, 14, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 14, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 14, v); acc += scr[54 * SST] * v; }
-    { const double v = 39.057 * gj * jc14; sink(14, 14, v); acc += scr[55 * SST] * v; }
-    { const double v = 25.03 * gj * jc15; sink(15, 14, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 14, v); acc += scr[57 * SST] * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 14, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 14, v); acc += e9 * v; }
+    { const double v = 38.049 * gj * jc10; sink(10, 14, v); acc += e10 * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 14, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 14, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 14, v); acc += e13 * v; }
+    { const double v = 39.057 * gj * jc14; sink(14, 14, v); acc += e14 * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 14, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 14, v); acc += e16 * v; }
     sink(17, 14, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 14, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 14, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 14, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 14, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 14, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 14, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 14, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 14, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 14, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 14, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 14, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 14, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 28).x + (ld2(rh1 + 28).x - ld2(rh0 + 28).x) * Td) - 212.8814297053025;
     sink(NS, 14, -(acc + Tdot * cvj * (((negmask >> 14) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -6876,30 +6908,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 15, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 15, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 15, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 15, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 15, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 15, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 15, v); acc += scr[47 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 15, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 15, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 15, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 15, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 15, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 15, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 15, v); acc += e6 * v; }
     sink(7, 15, 0.0);
-    { const double v = 1.008 * gj * jc8; sink(8, 15, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 15, v); acc += scr[50 * SST] * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 15, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 15, v); acc += e9 * v; }
     sink(10, 15, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 15, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 15, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 15, v); acc += scr[54 * SST] * v; }
-    { const double v = 39.057 * gj * jc14; sink(14, 15, v); acc += scr[55 * SST] * v; }
-    { const double v = 25.03 * gj * jc15; sink(15, 15, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 15, v); acc += scr[57 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 15, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 15, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 15, v); acc += e13 * v; }
+    { const double v = 39.057 * gj * jc14; sink(14, 15, v); acc += e14 * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 15, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 15, v); acc += e16 * v; }
     sink(17, 15, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 15, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 15, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 15, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 15, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 15, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 15, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 15, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 15, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 15, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 15, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 15, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 15, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 30).x + (ld2(rh1 + 30).x - ld2(rh0 + 30).x) * Td) - 332.1817818617659;
     sink(NS, 15, -(acc + Tdot * cvj * (((negmask >> 15) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -7165,30 +7197,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 16, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 16, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 16, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 16, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 16, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 16, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 16, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 16, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 16, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 16, v); acc += scr[50 * SST] * v; }
-    { const double v = 38.049 * gj * jc10; sink(10, 16, v); acc += scr[51 * SST] * v; }
-    { const double v = 15.035 * gj * jc11; sink(11, 16, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 16, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 16, v); acc += scr[54 * SST] * v; }
-    { const double v = 39.057 * gj * jc14; sink(14, 16, v); acc += scr[55 * SST] * v; }
-    { const double v = 25.03 * gj * jc15; sink(15, 16, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 16, v); acc += scr[57 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 16, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 16, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 16, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 16, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 16, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 16, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 16, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 16, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 16, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 16, v); acc += e9 * v; }
+    { const double v = 38.049 * gj * jc10; sink(10, 16, v); acc += e10 * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 16, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 16, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 16, v); acc += e13 * v; }
+    { const double v = 39.057 * gj * jc14; sink(14, 16, v); acc += e14 * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 16, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 16, v); acc += e16 * v; }
     sink(17, 16, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 16, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 16, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 16, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 16, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 16, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 16, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 16, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 16, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 16, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 16, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 16, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 16, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 32).x + (ld2(rh1 + 32).x - ld2(rh0 + 32).x) * Td) - 461.53261171246186;
     sink(NS, 16, -(acc + Tdot * cvj * (((negmask >> 16) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -7354,30 +7386,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 17, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 17, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 17, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 17, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 17, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 17, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 17, v); acc += scr[47 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 17, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 17, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 17, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 17, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 17, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 17, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 17, v); acc += e6 * v; }
     sink(7, 17, 0.0);
-    { const double v = 1.008 * gj * jc8; sink(8, 17, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 17, v); acc += scr[50 * SST] * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 17, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 17, v); acc += e9 * v; }
     sink(10, 17, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 17, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 17, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 17, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 17, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 17, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 17, v); acc += e13 * v; }
     sink(14, 17, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 17, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 17, v); acc += scr[57 * SST] * v; }
-    { const double v = 47.033 * gj * jc17; sink(17, 17, v); acc += scr[58 * SST] * v; }
-    { const double v = 31.998 * gj * jc18; sink(18, 17, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 17, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 17, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 17, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 17, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 17, v); acc += scr[64 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 17, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 17, v); acc += e16 * v; }
+    { const double v = 47.033 * gj * jc17; sink(17, 17, v); acc += e17 * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 17, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 17, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 17, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 17, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 17, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 17, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 34).x + (ld2(rh1 + 34).x - ld2(rh0 + 34).x) * Td) - 176.78034571471096;
     sink(NS, 17, -(acc + Tdot * cvj * (((negmask >> 17) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -7660,30 +7692,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 18, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 18, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 18, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 18, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 18, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 18, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 18, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 18, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 18, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 18, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 18, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 18, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 18, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 18, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 18, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 18, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 18, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 18, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 18, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 18, v); acc += e9 * v; }
     sink(10, 18, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 18, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 18, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 18, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 18, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 18, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 18, v); acc += e13 * v; }
     sink(14, 18, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 18, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 18, v); acc += scr[57 * SST] * v; }
-    { const double v = 47.033 * gj * jc17; sink(17, 18, v); acc += scr[58 * SST] * v; }
-    { const double v = 31.998 * gj * jc18; sink(18, 18, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 18, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 18, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 18, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 18, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 18, v); acc += scr[64 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 18, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 18, v); acc += e16 * v; }
+    { const double v = 47.033 * gj * jc17; sink(17, 18, v); acc += e17 * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 18, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 18, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 18, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 18, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 18, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 18, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 36).x + (ld2(rh1 + 36).x - ld2(rh0 + 36).x) * Td) - 259.844677792362;
     sink(NS, 18, -(acc + Tdot * cvj * (((negmask >> 18) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -7902,30 +7934,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 19, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 19, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 19, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 19, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 19, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 19, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 19, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 19, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 19, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 19, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 19, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 19, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 19, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 19, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 19, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 19, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 19, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 19, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 19, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 19, v); acc += e9 * v; }
     sink(10, 19, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 19, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 19, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 19, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 19, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 19, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 19, v); acc += e13 * v; }
     sink(14, 19, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 19, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 19, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 19, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 19, v); acc += e16 * v; }
     sink(17, 19, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 19, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 19, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 19, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 19, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 19, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 19, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 19, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 19, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 19, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 19, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 19, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 19, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 38).x + (ld2(rh1 + 38).x - ld2(rh0 + 38).x) * Td) - 592.7504099237186;
     sink(NS, 19, -(acc + Tdot * cvj * (((negmask >> 19) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -8064,30 +8096,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 20, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 20, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 20, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 20, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 20, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 20, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 20, v); acc += scr[47 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 20, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 20, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 20, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 20, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 20, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 20, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 20, v); acc += e6 * v; }
     sink(7, 20, 0.0);
-    { const double v = 1.008 * gj * jc8; sink(8, 20, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 20, v); acc += scr[50 * SST] * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 20, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 20, v); acc += e9 * v; }
     sink(10, 20, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 20, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 20, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 20, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 20, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 20, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 20, v); acc += e13 * v; }
     sink(14, 20, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 20, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 20, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 20, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 20, v); acc += e16 * v; }
     sink(17, 20, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 20, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 20, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 20, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 20, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 20, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 20, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 20, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 20, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 20, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 20, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 20, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 20, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 40).x + (ld2(rh1 + 40).x - ld2(rh0 + 40).x) * Td) - 276.50515463917526;
     sink(NS, 20, -(acc + Tdot * cvj * (((negmask >> 20) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -8295,30 +8327,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 21, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 21, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 21, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 21, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 21, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 21, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 21, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 21, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 21, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 21, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 21, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 21, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 21, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 21, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 21, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 21, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 21, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 21, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 21, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 21, v); acc += e9 * v; }
     sink(10, 21, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 21, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 21, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 21, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 21, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 21, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 21, v); acc += e13 * v; }
     sink(14, 21, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 21, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 21, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 21, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 21, v); acc += e16 * v; }
     sink(17, 21, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 21, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 21, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 21, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 21, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 21, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 21, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 21, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 21, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 21, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 21, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 21, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 21, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 42).x + (ld2(rh1 + 42).x - ld2(rh0 + 42).x) * Td) - 518.264040391448;
     sink(NS, 21, -(acc + Tdot * cvj * (((negmask >> 21) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -8457,30 +8489,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 22, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 22, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 22, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 22, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 22, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 22, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 22, v); acc += scr[47 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 22, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 22, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 22, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 22, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 22, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 22, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 22, v); acc += e6 * v; }
     sink(7, 22, 0.0);
-    { const double v = 1.008 * gj * jc8; sink(8, 22, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 22, v); acc += scr[50 * SST] * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 22, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 22, v); acc += e9 * v; }
     sink(10, 22, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 22, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 22, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 22, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 22, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 22, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 22, v); acc += e13 * v; }
     sink(14, 22, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 22, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 22, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 22, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 22, v); acc += e16 * v; }
     sink(17, 22, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 22, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 22, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 22, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 22, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 22, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 22, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 22, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 22, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 22, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 22, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 22, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 22, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 44).x + (ld2(rh1 + 44).x - ld2(rh0 + 44).x) * Td) - 184.69711442344;
     sink(NS, 22, -(acc + Tdot * cvj * (((negmask >> 22) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -8661,30 +8693,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += 1.0 * dq;
     }
     double acc = 0.0;
-    { const double v = 33.006 * gj * jc0; sink(0, 23, v); acc += scr[41 * SST] * v; }
-    { const double v = 34.014 * gj * jc1; sink(1, 23, v); acc += scr[42 * SST] * v; }
-    { const double v = 28.01 * gj * jc2; sink(2, 23, v); acc += scr[43 * SST] * v; }
-    { const double v = 17.007 * gj * jc3; sink(3, 23, v); acc += scr[44 * SST] * v; }
-    { const double v = 2.016 * gj * jc4; sink(4, 23, v); acc += scr[45 * SST] * v; }
-    { const double v = 29.018 * gj * jc5; sink(5, 23, v); acc += scr[46 * SST] * v; }
-    { const double v = 31.034 * gj * jc6; sink(6, 23, v); acc += scr[47 * SST] * v; }
-    { const double v = 14.027 * gj * jc7; sink(7, 23, v); acc += scr[48 * SST] * v; }
-    { const double v = 1.008 * gj * jc8; sink(8, 23, v); acc += scr[49 * SST] * v; }
-    { const double v = 15.999 * gj * jc9; sink(9, 23, v); acc += scr[50 * SST] * v; }
+    { const double v = 33.006 * gj * jc0; sink(0, 23, v); acc += e0 * v; }
+    { const double v = 34.014 * gj * jc1; sink(1, 23, v); acc += e1 * v; }
+    { const double v = 28.01 * gj * jc2; sink(2, 23, v); acc += e2 * v; }
+    { const double v = 17.007 * gj * jc3; sink(3, 23, v); acc += e3 * v; }
+    { const double v = 2.016 * gj * jc4; sink(4, 23, v); acc += e4 * v; }
+    { const double v = 29.018 * gj * jc5; sink(5, 23, v); acc += e5 * v; }
+    { const double v = 31.034 * gj * jc6; sink(6, 23, v); acc += e6 * v; }
+    { const double v = 14.027 * gj * jc7; sink(7, 23, v); acc += e7 * v; }
+    { const double v = 1.008 * gj * jc8; sink(8, 23, v); acc += e8 * v; }
+    { const double v = 15.999 * gj * jc9; sink(9, 23, v); acc += e9 * v; }
     sink(10, 23, 0.0);
-    { const double v = 15.035 * gj * jc11; sink(11, 23, v); acc += scr[52 * SST] * v; }
-    { const double v = 26.038 * gj * jc12; sink(12, 23, v); acc += scr[53 * SST] * v; }
-    { const double v = 30.026 * gj * jc13; sink(13, 23, v); acc += scr[54 * SST] * v; }
+    { const double v = 15.035 * gj * jc11; sink(11, 23, v); acc += e11 * v; }
+    { const double v = 26.038 * gj * jc12; sink(12, 23, v); acc += e12 * v; }
+    { const double v = 30.026 * gj * jc13; sink(13, 23, v); acc += e13 * v; }
     sink(14, 23, 0.0);
-    { const double v = 25.03 * gj * jc15; sink(15, 23, v); acc += scr[56 * SST] * v; }
-    { const double v = 18.015 * gj * jc16; sink(16, 23, v); acc += scr[57 * SST] * v; }
+    { const double v = 25.03 * gj * jc15; sink(15, 23, v); acc += e15 * v; }
+    { const double v = 18.015 * gj * jc16; sink(16, 23, v); acc += e16 * v; }
     sink(17, 23, 0.0);
-    { const double v = 31.998 * gj * jc18; sink(18, 23, v); acc += scr[59 * SST] * v; }
-    { const double v = 14.027 * gj * jc19; sink(19, 23, v); acc += scr[60 * SST] * v; }
-    { const double v = 30.07 * gj * jc20; sink(20, 23, v); acc += scr[61 * SST] * v; }
-    { const double v = 16.043 * gj * jc21; sink(21, 23, v); acc += scr[62 * SST] * v; }
-    { const double v = 45.017 * gj * jc22; sink(22, 23, v); acc += scr[63 * SST] * v; }
-    { const double v = 44.009 * gj * jc23; sink(23, 23, v); acc += scr[64 * SST] * v; }
+    { const double v = 31.998 * gj * jc18; sink(18, 23, v); acc += e18 * v; }
+    { const double v = 14.027 * gj * jc19; sink(19, 23, v); acc += e19 * v; }
+    { const double v = 30.07 * gj * jc20; sink(20, 23, v); acc += e20 * v; }
+    { const double v = 16.043 * gj * jc21; sink(21, 23, v); acc += e21 * v; }
+    { const double v = 45.017 * gj * jc22; sink(22, 23, v); acc += e22 * v; }
+    { const double v = 44.009 * gj * jc23; sink(23, 23, v); acc += e23 * v; }
     const double cvj = (ld2(rh0 + 46).x + (ld2(rh1 + 46).x - ld2(rh0 + 46).x) * Td) - 188.92749210388786;
     sink(NS, 23, -(acc + Tdot * cvj * (((negmask >> 23) & 1u) ? 0.0 : 1.0)) * invD);
   }
@@ -8701,7 +8733,8 @@ __global__ __launch_bounds__(256) void wdot_kernel(long n, long ld, const double
   double r[NS], F[NZ];
 #pragma unroll
   for (int k = 0; k < NS; ++k) r[k] = rho[k * ld + i];
-  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, T[i], F, nullptr, NullSink{});
+  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, T[i], F, nullptr, nullptr, 0,
+      NullSink{}, NullOut{});
 #pragma unroll
   for (int k = 0; k < NS; ++k) wdot[k * ld + i] = F[k];
   if (status) status[i] = st;
@@ -8724,9 +8757,16 @@ __global__ __launch_bounds__(JAC_BLOCK) void rhs_jac_kernel(long n, long ld, con
   double* __restrict__ Ji = J + i;
   __shared__ double smem[JAC ? NSLOT * JAC_BLOCK : 1];
   const int st = eval<JAC, true, JAC_BLOCK>(tabk, Tk_min, Tk_max, tabh, Th_min, Th_max, r, T, F, smem + threadIdx.x,
-      [&](int row, int col, double v) { stream_store(Ji + (long)(row * NZ + col) * ld, v); });
+      Z + i, ld,
+      [&](int row, int col, double v) { stream_store(Ji + (long)(row * NZ + col) * ld, v); },
+      [&](const double (&Fv)[NZ]) {
 #pragma unroll
-  for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = F[k];
+        for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = Fv[k];
+      });
+  if (!JAC) {
+#pragma unroll
+    for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = F[k];
+  }
   if (status) status[i] = st;
 }
 static const int p_rx_kind[] = {5, 5, 4, 4, 4, 4, 4, 4, 4, 4, 5, 4, 4, 4, 4, 4, 4, 4, 4, 5, 5, 5, 4, 5, 5, 5, 5, 5, 4, 4, 4, 4, 4, 6, 6, 4, 6, 4, 6, 4, 4, 4, 4, 6, 6, 4, 6, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 4, 6, 4, 4, 5, 5, 4, 4, 4, 4, 4, 4, 4, 4, 4, 0, 0, 0, 0, 0, 0, 0, 0, 1, 1, 1, 1};

@@ -87,7 +87,7 @@ class Surrogate:
     """A trained network with its feature scalings and acceptance thresholds."""
 
     def __init__(self, net, xs, ys, el_weight, dt, ymin=YMIN, yfloor=YFLOOR, tfloor=TFLOOR, max_dT=50.0,
-                 max_residual=1e-6, max_T_mismatch=1.0, meta=None):
+                 max_residual=1e-7, max_T_mismatch=1.0, meta=None):
         self.net, self.xs, self.ys = net, xs, ys
         self.el_weight = np.ascontiguousarray(el_weight, dtype=np.float64)
         self.dt, self.ymin, self.yfloor, self.tfloor = dt, ymin, yfloor, tfloor

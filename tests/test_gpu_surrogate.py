@@ -5,8 +5,7 @@ flint_amd/data/surrogate_rich31.npz.
   on the same cells;
 * accepted cells keep element amounts, total mass and mixture enthalpy of the old state to
   round-off (the projection and the enthalpy solve);
-* accepted cells stay close to the plain FP64 BDF step (the bound here is loose: the measured
-  errors are reported by bench.py and stated in the design notes)."""
+* accepted cells stay within max |dY| <= 1e-4 and max |dT| <= 1 K of the plain FP64 BDF step."""
 import numpy as np
 import pytest
 
@@ -68,9 +67,11 @@ def test_surrogate_step_conservation_and_fallback(setup, cuda):
     assert np.max(np.abs(Xm[:ns, a].sum(0) - X0[:ns, a].sum(0))) < 1e-12
     h0, h1 = _mix_h(md, X0[:ns, a], X0[ns, a]), _mix_h(md, Xm[:ns, a], Xm[ns, a])
     assert np.max(np.abs(h1 - h0) / np.maximum(np.abs(h0), 1e5)) < 1e-9
-    # accuracy against the plain FP64 step
-    assert np.max(np.abs(Xm[ns, a] - Xp[ns, a])) < 2.0
-    assert np.quantile(np.abs(Xm[:ns, a] - Xp[:ns, a]).max(0), 0.99) < 1e-4
+    # accuracy against the plain FP64 step: the learned step's bars, max |dY| <= 1e-4 and
+    # max |dT| <= 1 K over the accepted cells (the element-residual gate of 1e-7 keeps them;
+    # at 2^18 cells it measured 2.6e-5 and 0.16 K)
+    assert np.max(np.abs(Xm[ns, a] - Xp[ns, a])) <= 1.0
+    assert np.max(np.abs(Xm[:ns, a] - Xp[:ns, a])) <= 1e-4
 
 
 def test_surrogate_with_zero_tolerance_is_the_plain_step(setup, cuda):
