@@ -20,6 +20,10 @@ struct flint_cea_data {
   float *F = nullptr, *O = nullptr;
   double *G = nullptr, *scal = nullptr;
   int* st = nullptr;
+  int* redo_idx = nullptr;  // [ld + 1]: compacted list of the states to solve again, then its length
+  int warm_iters = 50;      // iteration cap of the warm-started pass (flint_cea_warm_options)
+  int cus = 0;              // compute units of the device (grid of the fused warm-start kernel)
+  bool warm_fused = true;   // features, network and start in one kernel where the network fits
   std::vector<void*> wbufs;
 };
 
@@ -84,6 +88,7 @@ extern "C" int flint_cea_setup(flint_mech* m, const double* s, const double* el_
   P.W = m->d_W;
   P.Th_min = m->Th_min;
   P.Th_max = m->Th_max;
+  P.max_iter = 50;
   return FLINT_OK;
 }
 
@@ -162,21 +167,53 @@ extern "C" int flint_cea_solve_warm(flint_mech* m, flint_mlp* net, long n, long 
     rc |= wgrab(c, &c->G, (size_t)ld * (ns + 1));
     rc |= wgrab(c, &c->scal, (size_t)4 * ns + 6);
     rc |= wgrab(c, &c->st, (size_t)ld);
+    rc |= wgrab(c, &c->redo_idx, (size_t)ld + 1);
     if (rc) return rc;
     c->wcap = ld;
   }
   hipStream_t s = (hipStream_t)stream;
   HIPCHK(hipMemcpyAsync(c->scal, scaling, (size_t)(4 * ns + 6) * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHK(flint_ws_features(n, ld, ns, rho, T, p, c->P.tabh, m->Th_min, m->Th_max, c->scal, c->F, s));
-  if (int rc = flint_mlp_forward(net, n, c->F, c->O, s)) return rc;
-  HIPCHK(flint_ws_guess(n, ld, ns, c->O, c->scal, c->G, s));
+  if (c->warm_fused && net->L == 3 && flint_ws_fused_fits(net->dims.data())) {
+    // features, the network and the Newton start in one kernel
+    if (!c->cus) {
+      int dev = 0;
+      HIPCHK(hipGetDevice(&dev));
+      HIPCHK(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    if (int rc = net->refresh_wt(s)) return rc;
+    const __bf16* WT[3] = {net->w16t + net->w16t_off[0], net->w16t + net->w16t_off[1], net->w16t + net->w16t_off[2]};
+    const float* b[3] = {net->w32 + net->b_off[0], net->w32 + net->b_off[1], net->w32 + net->b_off[2]};
+    HIPCHK(flint_ws_fused(n, ld, ns, rho, T, p, c->P.tabh, m->Th_min, m->Th_max, c->scal, net->dims.data(), WT, b, c->G,
+                          2 * c->cus, s));
+  } else {
+    HIPCHK(flint_ws_features(n, ld, ns, rho, T, p, c->P.tabh, m->Th_min, m->Th_max, c->scal, c->F, s));
+    if (int rc = flint_mlp_forward(net, n, c->F, c->O, s)) return rc;
+    HIPCHK(flint_ws_guess(n, ld, ns, c->O, c->scal, c->G, s));
+  }
   int* st = status ? status : c->st;
-  hipError_t e = m->ops->cea(c->P, true, n, ld, p, T, rho, c->G, T_out, Y_out, iters, st, s);
+  // the warm-started pass, with the iteration cap set by flint_cea_warm_options
+  flint_cea::CeaProg Wp = c->P;
+  Wp.max_iter = c->warm_iters;
+  hipError_t e = m->ops->cea(Wp, true, n, ld, p, T, rho, c->G, T_out, Y_out, iters, st, s);
   if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("equilibrium launch: ") + hipGetErrorString(e));
-  // states that did not converge from the predicted start: again from the default start
+  // the states that did not converge, compacted, again from the default start with the full cap
+  int* count = c->redo_idx + ld;
+  HIPCHK(flint_ws_collect(n, st, c->redo_idx, count, s));
   flint_cea::CeaProg R = c->P;
   R.redo = 1;
+  R.idx = c->redo_idx;
+  R.count = count;
   e = m->ops->cea(R, true, n, ld, p, T, rho, nullptr, T_out, Y_out, iters, st, s);
   if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("equilibrium launch: ") + hipGetErrorString(e));
+  return FLINT_OK;
+}
+
+extern "C" int flint_cea_warm_options(flint_mech* m, int max_iter, int fused) {
+  if (!m) return flint_fail(FLINT_EARG, "null mechanism");
+  if (!m->cea) return flint_fail(FLINT_EARG, "flint_cea_setup has not been called for this mechanism");
+  if (max_iter < 1 || max_iter > 50) return flint_fail(FLINT_EARG, "the warm-pass iteration cap must be in 1..50");
+  if (fused != 0 && fused != 1) return flint_fail(FLINT_EARG, "fused must be 0 or 1");
+  m->cea->warm_iters = max_iter;
+  m->cea->warm_fused = fused == 1;
   return FLINT_OK;
 }

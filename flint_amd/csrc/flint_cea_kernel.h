@@ -12,9 +12,10 @@
 // Arithmetic follows the reference's operation order with contraction off (scoped to each
 // function); only exp/log differ from the host's in the last bit.
 //
-// Status per state: 0 converged, 1 more than 50 iterations, 2 singular Newton matrix after the
-// reference's retries (its component switch, Lib_CEA_solver.f90:725-848, is not taken),
-// 3 temperature outside the thermo table.
+// Status per state: 0 converged, 1 more than P.max_iter iterations (the reference's maxitn = 50,
+// Lib_CEA_solver.f90:68,318), 2 singular Newton matrix after the reference's retries (its
+// component switch, Lib_CEA_solver.f90:725-848, is not taken), 3 temperature outside the thermo
+// table.
 #pragma once
 #include "flint_device.h"
 #include "flint_cea.h"
@@ -129,8 +130,13 @@ __global__ __launch_bounds__(64) void cea_kernel(CeaProg P, long n, long ld, con
     sRm[t] = P.rm[t];
   }
   __syncthreads();
-  const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (P.idx) {
+    if (c >= (long)*P.count) return;
+    c = P.idx[c];
+  } else if (c >= n) {
+    return;
+  }
   if (P.redo && status[c] == 0) return;
 
   double En[NS], Enln[NS], H0[NS], G0[NS];
@@ -335,7 +341,7 @@ __global__ __launch_bounds__(64) void cea_kernel(CeaProg P, long n, long ld, con
           Enn = exp(Ennl);
         }
         Tm = log(Pp / Enn);
-        if (numb > 50) {
+        if (numb > P.max_iter) {
           rc = 1;
           break;
         }

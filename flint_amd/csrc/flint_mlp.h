@@ -29,6 +29,7 @@ struct flint_mlp {
   ~flint_mlp();
   int reserve(long n, bool train);
   int load_input(long n, const float* X, hipStream_t s);
+  int refresh_wt(hipStream_t s);
   int forward(long n, float* out, hipStream_t s, bool keep = true);
   int backward(long n, const float* t, hipStream_t s);
 };

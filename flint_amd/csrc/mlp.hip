@@ -280,14 +280,21 @@ int flint_mlp::reserve(long n, bool train) {
   return FLINT_OK;
 }
 
-// forward from a16[0]; the last layer writes FP32 to out (z32[L-1] when out is null)
-int flint_mlp::forward(long n, float* out, hipStream_t s, bool keep) {
+// the transposed BF16 weights w16t, rebuilt after a weight update
+int flint_mlp::refresh_wt(hipStream_t s) {
   if (wt_stale) {
     for (int l = 0; l < L; ++l)
       hipLaunchKernelGGL(transpose_w16, dim3(nb((long)dims[l] * dims[l + 1], 256)), dim3(256), 0, s, dims[l], dims[l + 1],
                          w16 + w16_off[l], w16t + w16t_off[l]);
+    HIPCHK(hipGetLastError());
     wt_stale = false;
   }
+  return FLINT_OK;
+}
+
+// forward from a16[0]; the last layer writes FP32 to out (z32[L-1] when out is null)
+int flint_mlp::forward(long n, float* out, hipStream_t s, bool keep) {
+  if (int rc = refresh_wt(s)) return rc;
   for (int l = 0; l < L; ++l) {
     const int din = dims[l], dout = dims[l + 1];
     const __bf16* W = w16 + w16_off[l];

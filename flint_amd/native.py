@@ -50,6 +50,7 @@ def lib():
         L.flint_reactor_advance_host.argtypes = [vp, l, i, vp, vp, d, d, d, i, vp]
         L.flint_cea_solve_host.argtypes = [vp, l, i, vp, vp, vp, vp, vp, vp]
         L.flint_cea_solve_warm.argtypes = [vp, vp, l, l, vp, vp, vp, vp, vp, vp, vp, vp, vp]
+        L.flint_cea_warm_options.argtypes = [vp, i, i]
         _lib = L
     return _lib
 
@@ -202,6 +203,11 @@ class Mechanism:
             raise FlintError("scaling must hold 4 ns + 6 values")
         check(lib().flint_cea_solve_warm(self._h, net._h, n, ld, _p(p), _p(T), _p(rho), _p(sc), _p(T_out), _p(Y_out),
                                          _p(iters), _p(status), _stream(stream)))
+
+    def warm_options(self, max_iter=50, fused=True):
+        """Iteration cap of the warm-started pass of equilibrium_warm, and whether its features,
+        network and Newton start run as one kernel (flint_cea_warm_options)."""
+        check(lib().flint_cea_warm_options(self._h, int(max_iter), int(bool(fused))))
 
     # ---- host-memory convenience (numpy, SoA [rows, n]) ----
     def source_host(self, rho, T):

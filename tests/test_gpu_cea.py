@@ -137,3 +137,79 @@ def test_cea_learned_warm_start(gp24, cuda):
     assert dT.max() < 1e-2 and np.quantile(dT, 0.99) < 1e-4
     assert dY.max() < 1e-5
     assert it2.float().mean().item() < 0.5 * it1.float().mean().item()
+
+
+def test_cea_warm_start_iteration_cap(gp24, cuda):
+    """flint_cea_warm_options: with the warm-started pass capped at 4 iterations, the states that
+    converge within the cap are those of the uncapped warm start (same iterations, same result),
+    and every other state is the default-start solution, solved in the compacted second pass
+    (its iteration count adds the 5 capped iterations)."""
+    import torch
+    from flint_amd.warmstart import WarmStart, DEFAULT_MODEL
+    md, mech = gp24
+    ws = WarmStart.load(DEFAULT_MODEL)
+    n = 4096
+    st = S.cea_states(md.species, md.W, 24, 0, n)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    Y0, T0, p = dev(S.soa(st["Y"])), dev(st["T0"]), dev(st["p_bar"])
+    out = {}
+    print("[gpu] cea warm start iteration cap", flush=True)
+    try:
+        for cap in (50, 4, None):
+            T, Y = torch.empty_like(T0), torch.empty_like(Y0)
+            it, s = torch.empty(n, dtype=torch.int32, device=cuda), torch.empty(n, dtype=torch.int32, device=cuda)
+            if cap is None:
+                mech.equilibrium(T0, Y0, T, Y, p=p, hp=True, iters=it, status=s)
+            else:
+                mech.warm_options(cap)
+                ws.solve(mech, md, Y0, T0, p, T, Y, iters=it, status=s)
+            torch.cuda.synchronize()
+            out[cap] = [x.cpu().numpy() for x in (T, Y, it, s)]
+    finally:
+        mech.warm_options()
+    (T50, Y50, i50, s50), (T4, Y4, i4, s4), (Td, Yd, idf, sd) = out[50], out[4], out[None]
+    assert np.all(s50 == 0) and np.all(s4 == 0) and np.all(sd == 0)
+    fast = i50 <= 4
+    assert 0.2 * n < fast.sum() < n
+    assert np.array_equal(T4[fast], T50[fast]) and np.array_equal(Y4[:, fast], Y50[:, fast])
+    assert np.array_equal(i4[fast], i50[fast])
+    slow = ~fast
+    assert np.array_equal(T4[slow], Td[slow]) and np.array_equal(Y4[:, slow], Yd[:, slow])
+    assert np.array_equal(i4[slow], idf[slow] + 5)
+    with pytest.raises(Exception):
+        mech.warm_options(0)
+
+
+def test_cea_fused_warm_start_matches_layered(gp24, cuda):
+    """The one-kernel warm start (features, MFMA network with LDS-resident weights, Newton start)
+    against the same network run layer by layer (flint_mlp_forward between the feature and start
+    kernels): both feed the same solver, so the equilibria agree to the solver's tolerances and
+    the iteration counts are those of (nearly) the same starts."""
+    import torch
+    from flint_amd.warmstart import WarmStart, DEFAULT_MODEL
+    md, mech = gp24
+    ws = WarmStart.load(DEFAULT_MODEL)
+    n = 4096 + 13
+    st = S.cea_states(md.species, md.W, 25, 0, n)
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    Y0, T0, p = dev(S.soa(st["Y"])), dev(st["T0"]), dev(st["p_bar"])
+    out = {}
+    print("[gpu] cea fused vs layered warm start", flush=True)
+    try:
+        for fused in (True, False):
+            mech.warm_options(fused=fused)
+            T, Y = torch.empty_like(T0), torch.empty_like(Y0)
+            it, s = torch.empty(n, dtype=torch.int32, device=cuda), torch.empty(n, dtype=torch.int32, device=cuda)
+            ws.solve(mech, md, Y0, T0, p, T, Y, iters=it, status=s)
+            torch.cuda.synchronize()
+            out[fused] = [x.cpu().numpy() for x in (T, Y, it, s)]
+    finally:
+        mech.warm_options()
+    (Tf, Yf, itf, sf), (Tl, Yl, itl, sl) = out[True], out[False]
+    assert np.all(sf == 0) and np.all(sl == 0)
+    dT = np.abs(Tf - Tl)
+    assert dT.max() < 1e-2 and np.quantile(dT, 0.99) < 1e-4
+    assert np.max(np.abs(Yf - Yl) / np.abs(Yl).max(0)) < 1e-5
+    assert np.mean(itf == itl) > 0.95 and abs(itf.mean() - itl.mean()) < 0.05 * itl.mean()
+    print(f"[gpu] identical iteration counts {np.mean(itf == itl):.4f}, identical T {np.mean(Tf == Tl):.4f}",
+          flush=True)
