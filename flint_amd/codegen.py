@@ -120,8 +120,7 @@ def emit(prog):
     a("__device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min, int Tk_max,")
     a("                                    const double* __restrict__ tabh, int Th_min, int Th_max,")
     a("                                    const double (&r)[NS], double T, double (&F)[NZ],")
-    a("                                    double* __restrict__ scr, const double* __restrict__ zr, long zld,")
-    a("                                    Sink&& sink, FOut&& fout) {")
+    a("                                    double* __restrict__ scr, Sink&& sink, FOut&& fout) {")
     a("  int status = 0;")
     a("  if (!(T >= (double)Tk_min && T <= (double)Tk_max)) status |= 1;")
     a("  if (RHS && !(T >= (double)Th_min && T <= (double)Th_max)) {")
@@ -262,17 +261,14 @@ def emit(prog):
     a("  double eiroi = 0.0, rho_cv = 0.0;")
     a("  [[maybe_unused]] double s_de = 0.0, s_dcv = 0.0;")
     a(f"  [[maybe_unused]] double {', '.join(f'e{k}' for k in range(ns))};")
-    a("  // JAC: rho_k is read again from zr[k*zld] here rather than kept live through the rates")
-    a("  [[maybe_unused]] const double* __restrict__ zre = JAC ? opaque(zr) : nullptr;")
     for k in range(ns):
         Rk = RUNIV / W[k]
         a(f"  {{ const double2 t0 = ld2(rh0 + {2 * k}), t1 = ld2(rh1 + {2 * k});")
         a("    const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;")
         a(f"    const double ek = hk - {_lit(Rk)} * T;")
         a(f"    eiroi = eiroi + ek * F[{k}];")
-        a(f"    const double rr = JAC ? fmax(zre[{k} * zld], 0.0) : r{k};")
-        a(f"    rho_cv = rho_cv + rr * (cpk - {_lit(Rk)});")
-        a(f"    if (JAC) {{ {S('e', k)} = ek; s_de += ((t1.y - t0.y) - {_lit(Rk)}) * F[{k}]; s_dcv += rr * (t1.x - t0.x); }} }}")
+        a(f"    rho_cv = rho_cv + r{k} * (cpk - {_lit(Rk)});")
+        a(f"    if (JAC) {{ {S('e', k)} = ek; s_de += ((t1.y - t0.y) - {_lit(Rk)}) * F[{k}]; s_dcv += r{k} * (t1.x - t0.x); }} }}")
     a("  F[NS] = -eiroi / rho_cv;")
     a("  if (JAC) fout(F);")
     a("  if (!JAC) return status;")
@@ -362,7 +358,7 @@ def emit(prog):
     a("  double r[NS], F[NZ];")
     a("#pragma unroll")
     a("  for (int k = 0; k < NS; ++k) r[k] = rho[k * ld + i];")
-    a("  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, T[i], F, nullptr, nullptr, 0,")
+    a("  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, T[i], F, nullptr,")
     a("      NullSink{}, NullOut{});")
     a("#pragma unroll")
     a("  for (int k = 0; k < NS; ++k) wdot[k * ld + i] = F[k];")
@@ -388,7 +384,6 @@ def emit(prog):
     a("  double* __restrict__ Ji = J + i;")
     a("  __shared__ double smem[JAC ? NSLOT * JAC_BLOCK : 1];")
     a("  const int st = eval<JAC, true, JAC_BLOCK>(tabk, Tk_min, Tk_max, tabh, Th_min, Th_max, r, T, F, smem + threadIdx.x,")
-    a("      Z + i, ld,")
     a("      [&](int row, int col, double v) { stream_store(Ji + (long)(row * NZ + col) * ld, v); },")
     a("      [&](const double (&Fv)[NZ]) {")
     a("#pragma unroll")

@@ -21,9 +21,11 @@ struct flint_cea_data {
   double *G = nullptr, *scal = nullptr;
   int* st = nullptr;
   int* redo_idx = nullptr;  // [ld + 1]: compacted list of the states to solve again, then its length
-  int warm_iters = 50;      // iteration cap of the warm-started pass (flint_cea_warm_options)
+  int warm_iters = 5;       // iteration cap of the warm-started pass (flint_cea_warm_options)
   int cus = 0;              // compute units of the device (grid of the fused warm-start kernel)
   bool warm_fused = true;   // features, network and start in one kernel where the network fits
+  void* wpk = nullptr;      // the network's weights packed for that kernel ...
+  unsigned long long pk_uid = 0, pk_version = 0;  // ... from this network at this weight version
   std::vector<void*> wbufs;
 };
 
@@ -180,11 +182,21 @@ extern "C" int flint_cea_solve_warm(flint_mech* m, flint_mlp* net, long n, long 
       HIPCHK(hipGetDevice(&dev));
       HIPCHK(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
-    if (int rc = net->refresh_wt(s)) return rc;
-    const __bf16* WT[3] = {net->w16t + net->w16t_off[0], net->w16t + net->w16t_off[1], net->w16t + net->w16t_off[2]};
+    if (!c->wpk) {
+      HIPCHK(hipMalloc(&c->wpk, flint_ws_packed_bytes()));
+      c->bufs.push_back(c->wpk);
+    }
+    if (c->pk_uid != net->uid || c->pk_version != net->version) {
+      if (int rc = net->refresh_wt(s)) return rc;
+      const __bf16* WT[3] = {net->w16t + net->w16t_off[0], net->w16t + net->w16t_off[1],
+                             net->w16t + net->w16t_off[2]};
+      HIPCHK(flint_ws_pack(net->dims.data(), WT, c->wpk, s));
+      c->pk_uid = net->uid;
+      c->pk_version = net->version;
+    }
     const float* b[3] = {net->w32 + net->b_off[0], net->w32 + net->b_off[1], net->w32 + net->b_off[2]};
-    HIPCHK(flint_ws_fused(n, ld, ns, rho, T, p, c->P.tabh, m->Th_min, m->Th_max, c->scal, net->dims.data(), WT, b, c->G,
-                          2 * c->cus, s));
+    HIPCHK(flint_ws_fused(n, ld, ns, rho, T, p, c->P.tabh, m->Th_min, m->Th_max, c->scal, net->dims.data(), c->wpk, b,
+                          c->G, c->cus, s));
   } else {
     HIPCHK(flint_ws_features(n, ld, ns, rho, T, p, c->P.tabh, m->Th_min, m->Th_max, c->scal, c->F, s));
     if (int rc = flint_mlp_forward(net, n, c->F, c->O, s)) return rc;

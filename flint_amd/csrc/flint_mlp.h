@@ -14,6 +14,8 @@ struct flint_mlp {
   __bf16* w16t = nullptr;                  // BF16 weights transposed per layer ([dout][din]) for the forward path
   std::vector<long> w16t_off;
   bool wt_stale = true;                    // w16t behind w16 (set on every weight update)
+  unsigned long long uid = 0;              // unique over the process (flint_mlp_create)
+  unsigned long long version = 0;          // counts weight updates (copies derived elsewhere check both)
   int* w16_index = nullptr;                // FP32 index -> BF16 index (-1 for biases)
   float* loss = nullptr;                   // device scalar
   int adam_t = 0;

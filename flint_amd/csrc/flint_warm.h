@@ -21,9 +21,14 @@ inline bool flint_ws_fused_fits(const int* dims) {
   return dims[0] >= 1 && dims[0] <= 32 && dims[1] == 128 && dims[2] == 128 && dims[3] >= 1 && dims[3] <= 32;
 }
 
+// The network's weights rearranged for flint_ws_fused (flint_ws_packed_bytes of device memory):
+// dims[0..3] the layer widths, WT[l] the BF16 weights of layer l transposed ([dout][din]).
+size_t flint_ws_packed_bytes();
+hipError_t flint_ws_pack(const int* dims, const __bf16* const* WT, void* packed, hipStream_t s);
+
 // flint_ws_features, the network's forward pass and flint_ws_guess in one kernel (G as in
-// flint_ws_guess): dims[0..3] the layer widths, WT[l] the BF16 weights of layer l transposed
-// ([dout][din]), b[l] its FP32 bias; at most `blocks` blocks of 8 wavefronts.
+// flint_ws_guess): packed from flint_ws_pack, b[l] the FP32 bias of layer l; at most `blocks`
+// blocks of 8 wavefronts.
 hipError_t flint_ws_fused(long n, long ld, int ns, const double* Y0, const double* T0, const double* p,
                           const double* tabh, int Th_min, int Th_max, const double* scal, const int* dims,
-                          const __bf16* const* WT, const float* const* b, double* G, int blocks, hipStream_t s);
+                          const void* packed, const float* const* b, double* G, int blocks, hipStream_t s);

@@ -15,8 +15,7 @@ template <bool JAC, bool RHS, int SST, class Sink, class FOut>
 __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min, int Tk_max,
                                     const double* __restrict__ tabh, int Th_min, int Th_max,
                                     const double (&r)[NS], double T, double (&F)[NZ],
-                                    double* __restrict__ scr, const double* __restrict__ zr, long zld,
-                                    Sink&& sink, FOut&& fout) {
+                                    double* __restrict__ scr, Sink&& sink, FOut&& fout) {
   int status = 0;
   if (!(T >= (double)Tk_min && T <= (double)Tk_max)) status |= 1;
   if (RHS && !(T >= (double)Th_min && T <= (double)Th_max)) {
@@ -4104,225 +4103,192 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   double eiroi = 0.0, rho_cv = 0.0;
   [[maybe_unused]] double s_de = 0.0, s_dcv = 0.0;
   [[maybe_unused]] double e0, e1, e2, e3, e4, e5, e6, e7, e8, e9, e10, e11, e12, e13, e14, e15, e16, e17, e18, e19, e20, e21, e22, e23, e24, e25, e26, e27, e28, e29, e30;
-  // JAC: rho_k is read again from zr[k*zld] here rather than kept live through the rates
-  [[maybe_unused]] const double* __restrict__ zre = JAC ? opaque(zr) : nullptr;
   { const double2 t0 = ld2(rh0 + 0), t1 = ld2(rh1 + 0);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 116.94271368092377 * T;
     eiroi = eiroi + ek * F[0];
-    const double rr = JAC ? fmax(zre[0 * zld], 0.0) : r0;
-    rho_cv = rho_cv + rr * (cpk - 116.94271368092377);
-    if (JAC) { e0 = ek; s_de += ((t1.y - t0.y) - 116.94271368092377) * F[0]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r0 * (cpk - 116.94271368092377);
+    if (JAC) { e0 = ek; s_de += ((t1.y - t0.y) - 116.94271368092377) * F[0]; s_dcv += r0 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 2), t1 = ld2(rh1 + 2);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 193.15855500058078 * T;
     eiroi = eiroi + ek * F[1];
-    const double rr = JAC ? fmax(zre[1 * zld], 0.0) : r1;
-    rho_cv = rho_cv + rr * (cpk - 193.15855500058078);
-    if (JAC) { e1 = ek; s_de += ((t1.y - t0.y) - 193.15855500058078) * F[1]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r1 * (cpk - 193.15855500058078);
+    if (JAC) { e1 = ek; s_de += ((t1.y - t0.y) - 193.15855500058078) * F[1]; s_dcv += r1 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 4), t1 = ld2(rh1 + 4);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 8248.521825396825 * T;
     eiroi = eiroi + ek * F[2];
-    const double rr = JAC ? fmax(zre[2 * zld], 0.0) : r2;
-    rho_cv = rho_cv + rr * (cpk - 8248.521825396825);
-    if (JAC) { e2 = ek; s_de += ((t1.y - t0.y) - 8248.521825396825) * F[2]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r2 * (cpk - 8248.521825396825);
+    if (JAC) { e2 = ek; s_de += ((t1.y - t0.y) - 8248.521825396825) * F[2]; s_dcv += r2 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 6), t1 = ld2(rh1 + 6);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 518.264040391448 * T;
     eiroi = eiroi + ek * F[3];
-    const double rr = JAC ? fmax(zre[3 * zld], 0.0) : r3;
-    rho_cv = rho_cv + rr * (cpk - 518.264040391448);
-    if (JAC) { e3 = ek; s_de += ((t1.y - t0.y) - 518.264040391448) * F[3]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r3 * (cpk - 518.264040391448);
+    if (JAC) { e3 = ek; s_de += ((t1.y - t0.y) - 518.264040391448) * F[3]; s_dcv += r3 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 8), t1 = ld2(rh1 + 8);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 319.3221445579538 * T;
     eiroi = eiroi + ek * F[4];
-    const double rr = JAC ? fmax(zre[4 * zld], 0.0) : r4;
-    rho_cv = rho_cv + rr * (cpk - 319.3221445579538);
-    if (JAC) { e4 = ek; s_de += ((t1.y - t0.y) - 319.3221445579538) * F[4]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r4 * (cpk - 319.3221445579538);
+    if (JAC) { e4 = ek; s_de += ((t1.y - t0.y) - 319.3221445579538) * F[4]; s_dcv += r4 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 10), t1 = ld2(rh1 + 10);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 212.8814297053025 * T;
     eiroi = eiroi + ek * F[5];
-    const double rr = JAC ? fmax(zre[5 * zld], 0.0) : r5;
-    rho_cv = rho_cv + rr * (cpk - 212.8814297053025);
-    if (JAC) { e5 = ek; s_de += ((t1.y - t0.y) - 212.8814297053025) * F[5]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r5 * (cpk - 212.8814297053025);
+    if (JAC) { e5 = ek; s_de += ((t1.y - t0.y) - 212.8814297053025) * F[5]; s_dcv += r5 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 12), t1 = ld2(rh1 + 12);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 307.4210604155883 * T;
     eiroi = eiroi + ek * F[6];
-    const double rr = JAC ? fmax(zre[6 * zld], 0.0) : r6;
-    rho_cv = rho_cv + rr * (cpk - 307.4210604155883);
-    if (JAC) { e6 = ek; s_de += ((t1.y - t0.y) - 307.4210604155883) * F[6]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r6 * (cpk - 307.4210604155883);
+    if (JAC) { e6 = ek; s_de += ((t1.y - t0.y) - 307.4210604155883) * F[6]; s_dcv += r6 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 14), t1 = ld2(rh1 + 14);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 519.689355584724 * T;
     eiroi = eiroi + ek * F[7];
-    const double rr = JAC ? fmax(zre[7 * zld], 0.0) : r7;
-    rho_cv = rho_cv + rr * (cpk - 519.689355584724);
-    if (JAC) { e7 = ek; s_de += ((t1.y - t0.y) - 519.689355584724) * F[7]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r7 * (cpk - 519.689355584724);
+    if (JAC) { e7 = ek; s_de += ((t1.y - t0.y) - 519.689355584724) * F[7]; s_dcv += r7 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 16), t1 = ld2(rh1 + 16);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 244.44375845240194 * T;
     eiroi = eiroi + ek * F[8];
-    const double rr = JAC ? fmax(zre[8 * zld], 0.0) : r8;
-    rho_cv = rho_cv + rr * (cpk - 244.44375845240194);
-    if (JAC) { e8 = ek; s_de += ((t1.y - t0.y) - 244.44375845240194) * F[8]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r8 * (cpk - 244.44375845240194);
+    if (JAC) { e8 = ek; s_de += ((t1.y - t0.y) - 244.44375845240194) * F[8]; s_dcv += r8 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 18), t1 = ld2(rh1 + 18);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 296.84077115315955 * T;
     eiroi = eiroi + ek * F[9];
-    const double rr = JAC ? fmax(zre[9 * zld], 0.0) : r9;
-    rho_cv = rho_cv + rr * (cpk - 296.84077115315955);
-    if (JAC) { e9 = ek; s_de += ((t1.y - t0.y) - 296.84077115315955) * F[9]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r9 * (cpk - 296.84077115315955);
+    if (JAC) { e9 = ek; s_de += ((t1.y - t0.y) - 296.84077115315955) * F[9]; s_dcv += r9 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 20), t1 = ld2(rh1 + 20);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 276.9103443682142 * T;
     eiroi = eiroi + ek * F[10];
-    const double rr = JAC ? fmax(zre[10 * zld], 0.0) : r10;
-    rho_cv = rho_cv + rr * (cpk - 276.9103443682142);
-    if (JAC) { e10 = ek; s_de += ((t1.y - t0.y) - 276.9103443682142) * F[10]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r10 * (cpk - 276.9103443682142);
+    if (JAC) { e10 = ek; s_de += ((t1.y - t0.y) - 276.9103443682142) * F[10]; s_dcv += r10 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 22), t1 = ld2(rh1 + 22);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 276.50515463917526 * T;
     eiroi = eiroi + ek * F[11];
-    const double rr = JAC ? fmax(zre[11 * zld], 0.0) : r11;
-    rho_cv = rho_cv + rr * (cpk - 276.50515463917526);
-    if (JAC) { e11 = ek; s_de += ((t1.y - t0.y) - 276.50515463917526) * F[11]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r11 * (cpk - 276.50515463917526);
+    if (JAC) { e11 = ek; s_de += ((t1.y - t0.y) - 276.50515463917526) * F[11]; s_dcv += r11 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 24), t1 = ld2(rh1 + 24);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 197.79027999143614 * T;
     eiroi = eiroi + ek * F[12];
-    const double rr = JAC ? fmax(zre[12 * zld], 0.0) : r12;
-    rho_cv = rho_cv + rr * (cpk - 197.79027999143614);
-    if (JAC) { e12 = ek; s_de += ((t1.y - t0.y) - 197.79027999143614) * F[12]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r12 * (cpk - 197.79027999143614);
+    if (JAC) { e12 = ek; s_de += ((t1.y - t0.y) - 197.79027999143614) * F[12]; s_dcv += r12 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 26), t1 = ld2(rh1 + 26);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 192.96131263199425 * T;
     eiroi = eiroi + ek * F[13];
-    const double rr = JAC ? fmax(zre[13 * zld], 0.0) : r13;
-    rho_cv = rho_cv + rr * (cpk - 192.96131263199425);
-    if (JAC) { e13 = ek; s_de += ((t1.y - t0.y) - 192.96131263199425) * F[13]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r13 * (cpk - 192.96131263199425);
+    if (JAC) { e13 = ek; s_de += ((t1.y - t0.y) - 192.96131263199425) * F[13]; s_dcv += r13 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 28), t1 = ld2(rh1 + 28);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 461.53261171246186 * T;
     eiroi = eiroi + ek * F[14];
-    const double rr = JAC ? fmax(zre[14 * zld], 0.0) : r14;
-    rho_cv = rho_cv + rr * (cpk - 461.53261171246186);
-    if (JAC) { e14 = ek; s_de += ((t1.y - t0.y) - 461.53261171246186) * F[14]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r14 * (cpk - 461.53261171246186);
+    if (JAC) { e14 = ek; s_de += ((t1.y - t0.y) - 461.53261171246186) * F[14]; s_dcv += r14 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 30), t1 = ld2(rh1 + 30);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 259.844677792362 * T;
     eiroi = eiroi + ek * F[15];
-    const double rr = JAC ? fmax(zre[15 * zld], 0.0) : r15;
-    rho_cv = rho_cv + rr * (cpk - 259.844677792362);
-    if (JAC) { e15 = ek; s_de += ((t1.y - t0.y) - 259.844677792362) * F[15]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r15 * (cpk - 259.844677792362);
+    if (JAC) { e15 = ek; s_de += ((t1.y - t0.y) - 259.844677792362) * F[15]; s_dcv += r15 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 32), t1 = ld2(rh1 + 32);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 592.7504099237186 * T;
     eiroi = eiroi + ek * F[16];
-    const double rr = JAC ? fmax(zre[16 * zld], 0.0) : r16;
-    rho_cv = rho_cv + rr * (cpk - 592.7504099237186);
-    if (JAC) { e16 = ek; s_de += ((t1.y - t0.y) - 592.7504099237186) * F[16]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r16 * (cpk - 592.7504099237186);
+    if (JAC) { e16 = ek; s_de += ((t1.y - t0.y) - 592.7504099237186) * F[16]; s_dcv += r16 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 34), t1 = ld2(rh1 + 34);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 286.52939554759115 * T;
     eiroi = eiroi + ek * F[17];
-    const double rr = JAC ? fmax(zre[17 * zld], 0.0) : r17;
-    rho_cv = rho_cv + rr * (cpk - 286.52939554759115);
-    if (JAC) { e17 = ek; s_de += ((t1.y - t0.y) - 286.52939554759115) * F[17]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r17 * (cpk - 286.52939554759115);
+    if (JAC) { e17 = ek; s_de += ((t1.y - t0.y) - 286.52939554759115) * F[17]; s_dcv += r17 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 36), t1 = ld2(rh1 + 36);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 553.0103092783505 * T;
     eiroi = eiroi + ek * F[18];
-    const double rr = JAC ? fmax(zre[18 * zld], 0.0) : r18;
-    rho_cv = rho_cv + rr * (cpk - 553.0103092783505);
-    if (JAC) { e18 = ek; s_de += ((t1.y - t0.y) - 553.0103092783505) * F[18]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r18 * (cpk - 553.0103092783505);
+    if (JAC) { e18 = ek; s_de += ((t1.y - t0.y) - 553.0103092783505) * F[18]; s_dcv += r18 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 38), t1 = ld2(rh1 + 38);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 4124.2609126984125 * T;
     eiroi = eiroi + ek * F[19];
-    const double rr = JAC ? fmax(zre[19 * zld], 0.0) : r19;
-    rho_cv = rho_cv + rr * (cpk - 4124.2609126984125);
-    if (JAC) { e19 = ek; s_de += ((t1.y - t0.y) - 4124.2609126984125) * F[19]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r19 * (cpk - 4124.2609126984125);
+    if (JAC) { e19 = ek; s_de += ((t1.y - t0.y) - 4124.2609126984125) * F[19]; s_dcv += r19 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 40), t1 = ld2(rh1 + 40);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 193.15855500058078 * T;
     eiroi = eiroi + ek * F[20];
-    const double rr = JAC ? fmax(zre[20 * zld], 0.0) : r20;
-    rho_cv = rho_cv + rr * (cpk - 193.15855500058078);
-    if (JAC) { e20 = ek; s_de += ((t1.y - t0.y) - 193.15855500058078) * F[20]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r20 * (cpk - 193.15855500058078);
+    if (JAC) { e20 = ek; s_de += ((t1.y - t0.y) - 193.15855500058078) * F[20]; s_dcv += r20 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 42), t1 = ld2(rh1 + 42);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 143.15616391184574 * T;
     eiroi = eiroi + ek * F[21];
-    const double rr = JAC ? fmax(zre[21 * zld], 0.0) : r21;
-    rho_cv = rho_cv + rr * (cpk - 143.15616391184574);
-    if (JAC) { e21 = ek; s_de += ((t1.y - t0.y) - 143.15616391184574) * F[21]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r21 * (cpk - 143.15616391184574);
+    if (JAC) { e21 = ek; s_de += ((t1.y - t0.y) - 143.15616391184574) * F[21]; s_dcv += r21 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 44), t1 = ld2(rh1 + 44);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 188.7387919097451 * T;
     eiroi = eiroi + ek * F[22];
-    const double rr = JAC ? fmax(zre[22 * zld], 0.0) : r22;
-    rho_cv = rho_cv + rr * (cpk - 188.7387919097451);
-    if (JAC) { e22 = ek; s_de += ((t1.y - t0.y) - 188.7387919097451) * F[22]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r22 * (cpk - 188.7387919097451);
+    if (JAC) { e22 = ek; s_de += ((t1.y - t0.y) - 188.7387919097451) * F[22]; s_dcv += r22 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 46), t1 = ld2(rh1 + 46);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 193.15855500058078 * T;
     eiroi = eiroi + ek * F[23];
-    const double rr = JAC ? fmax(zre[23 * zld], 0.0) : r23;
-    rho_cv = rho_cv + rr * (cpk - 193.15855500058078);
-    if (JAC) { e23 = ek; s_de += ((t1.y - t0.y) - 193.15855500058078) * F[23]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r23 * (cpk - 193.15855500058078);
+    if (JAC) { e23 = ek; s_de += ((t1.y - t0.y) - 193.15855500058078) * F[23]; s_dcv += r23 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 48), t1 = ld2(rh1 + 48);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 251.90904683996848 * T;
     eiroi = eiroi + ek * F[24];
-    const double rr = JAC ? fmax(zre[24 * zld], 0.0) : r24;
-    rho_cv = rho_cv + rr * (cpk - 251.90904683996848);
-    if (JAC) { e24 = ek; s_de += ((t1.y - t0.y) - 251.90904683996848) * F[24]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r24 * (cpk - 251.90904683996848);
+    if (JAC) { e24 = ek; s_de += ((t1.y - t0.y) - 251.90904683996848) * F[24]; s_dcv += r24 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 50), t1 = ld2(rh1 + 50);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 296.3752049618593 * T;
     eiroi = eiroi + ek * F[25];
-    const double rr = JAC ? fmax(zre[25 * zld], 0.0) : r25;
-    rho_cv = rho_cv + rr * (cpk - 296.3752049618593);
-    if (JAC) { e25 = ek; s_de += ((t1.y - t0.y) - 296.3752049618593) * F[25]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r25 * (cpk - 296.3752049618593);
+    if (JAC) { e25 = ek; s_de += ((t1.y - t0.y) - 296.3752049618593) * F[25]; s_dcv += r25 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 52), t1 = ld2(rh1 + 52);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 638.6442891159076 * T;
     eiroi = eiroi + ek * F[26];
-    const double rr = JAC ? fmax(zre[26 * zld], 0.0) : r26;
-    rho_cv = rho_cv + rr * (cpk - 638.6442891159076);
-    if (JAC) { e26 = ek; s_de += ((t1.y - t0.y) - 638.6442891159076) * F[26]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r26 * (cpk - 638.6442891159076);
+    if (JAC) { e26 = ek; s_de += ((t1.y - t0.y) - 638.6442891159076) * F[26]; s_dcv += r26 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 54), t1 = ld2(rh1 + 54);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 488.8875169048039 * T;
     eiroi = eiroi + ek * F[27];
-    const double rr = JAC ? fmax(zre[27 * zld], 0.0) : r27;
-    rho_cv = rho_cv + rr * (cpk - 488.8875169048039);
-    if (JAC) { e27 = ek; s_de += ((t1.y - t0.y) - 488.8875169048039) * F[27]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r27 * (cpk - 488.8875169048039);
+    if (JAC) { e27 = ek; s_de += ((t1.y - t0.y) - 488.8875169048039) * F[27]; s_dcv += r27 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 56), t1 = ld2(rh1 + 56);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 286.0955887413117 * T;
     eiroi = eiroi + ek * F[28];
-    const double rr = JAC ? fmax(zre[28 * zld], 0.0) : r28;
-    rho_cv = rho_cv + rr * (cpk - 286.0955887413117);
-    if (JAC) { e28 = ek; s_de += ((t1.y - t0.y) - 286.0955887413117) * F[28]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r28 * (cpk - 286.0955887413117);
+    if (JAC) { e28 = ek; s_de += ((t1.y - t0.y) - 286.0955887413117) * F[28]; s_dcv += r28 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 58), t1 = ld2(rh1 + 58);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 592.7504099237186 * T;
     eiroi = eiroi + ek * F[29];
-    const double rr = JAC ? fmax(zre[29 * zld], 0.0) : r29;
-    rho_cv = rho_cv + rr * (cpk - 592.7504099237186);
-    if (JAC) { e29 = ek; s_de += ((t1.y - t0.y) - 592.7504099237186) * F[29]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r29 * (cpk - 592.7504099237186);
+    if (JAC) { e29 = ek; s_de += ((t1.y - t0.y) - 592.7504099237186) * F[29]; s_dcv += r29 * (t1.x - t0.x); } }
   { const double2 t0 = ld2(rh0 + 60), t1 = ld2(rh1 + 60);
     const double cpk = t0.x + (t1.x - t0.x) * Td, hk = t0.y + (t1.y - t0.y) * Td;
     const double ek = hk - 188.92749210388786 * T;
     eiroi = eiroi + ek * F[30];
-    const double rr = JAC ? fmax(zre[30 * zld], 0.0) : r30;
-    rho_cv = rho_cv + rr * (cpk - 188.92749210388786);
-    if (JAC) { e30 = ek; s_de += ((t1.y - t0.y) - 188.92749210388786) * F[30]; s_dcv += rr * (t1.x - t0.x); } }
+    rho_cv = rho_cv + r30 * (cpk - 188.92749210388786);
+    if (JAC) { e30 = ek; s_de += ((t1.y - t0.y) - 188.92749210388786) * F[30]; s_dcv += r30 * (t1.x - t0.x); } }
   F[NS] = -eiroi / rho_cv;
   if (JAC) fout(F);
   if (!JAC) return status;
@@ -13720,7 +13686,7 @@ __global__ __launch_bounds__(256) void wdot_kernel(long n, long ld, const double
   double r[NS], F[NZ];
 #pragma unroll
   for (int k = 0; k < NS; ++k) r[k] = rho[k * ld + i];
-  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, T[i], F, nullptr, nullptr, 0,
+  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, T[i], F, nullptr,
       NullSink{}, NullOut{});
 #pragma unroll
   for (int k = 0; k < NS; ++k) wdot[k * ld + i] = F[k];
@@ -13744,7 +13710,6 @@ __global__ __launch_bounds__(JAC_BLOCK) void rhs_jac_kernel(long n, long ld, con
   double* __restrict__ Ji = J + i;
   __shared__ double smem[JAC ? NSLOT * JAC_BLOCK : 1];
   const int st = eval<JAC, true, JAC_BLOCK>(tabk, Tk_min, Tk_max, tabh, Th_min, Th_max, r, T, F, smem + threadIdx.x,
-      Z + i, ld,
       [&](int row, int col, double v) { stream_store(Ji + (long)(row * NZ + col) * ld, v); },
       [&](const double (&Fv)[NZ]) {
 #pragma unroll

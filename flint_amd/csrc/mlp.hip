@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <string>
@@ -348,7 +349,9 @@ extern "C" int flint_mlp_create(int nlayers, const int* dims, unsigned long long
   *out = nullptr;
   for (int l = 0; l <= nlayers; ++l)
     if (dims[l] < 1 || dims[l] > 4096) return flint_fail(FLINT_EARG, "layer widths must be in 1..4096");
+  static std::atomic<unsigned long long> next_uid{1};
   auto* m = new flint_mlp();
+  m->uid = next_uid++;
   m->L = nlayers;
   m->dims.assign(dims, dims + nlayers + 1);
   long np = 0, nw = 0;
@@ -434,6 +437,7 @@ extern "C" int flint_mlp_set_params(flint_mlp* m, const float* host) {
   hipLaunchKernelGGL(sync_bf16, dim3(nb(m->np, 256)), dim3(256), 0, 0, m->np, m->w32, m->w16, m->w16_index);
   HIPCHK(hipDeviceSynchronize());
   m->wt_stale = true;
+  m->version += 1;
   m->adam_t = 0;
   std::vector<float> zeros(m->np, 0.0f);
   HIPCHK(hipMemcpy(m->m32, zeros.data(), m->np * sizeof(float), hipMemcpyHostToDevice));
@@ -476,5 +480,6 @@ extern "C" int flint_mlp_train_step(flint_mlp* m, long n, const float* X, const 
                      m->w16_index, lr, b1, b2, 1e-8f, c1, c2);
   HIPCHK(hipGetLastError());
   m->wt_stale = true;
+  m->version += 1;
   return FLINT_OK;
 }

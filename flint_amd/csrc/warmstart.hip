@@ -65,45 +65,54 @@ __device__ __forceinline__ int chained_k(int ks, int h, int e) {
   return 32 * (ks >> 1) + 16 * (ks & 1) + 8 * (e >> 2) + 4 * h + (e & 3);
 }
 
+// Fragments of the three layers' weights in the order the fused kernel reads them: per layer,
+// (output tile, K-step, lane) -> 8 BF16 of row 32 tile + lane % 32 of WT at the input indices of
+// that lane's half (natural for the first layer, chained_k for the others), zero outside.
 template <int H1T, int H2T>
-__global__ __launch_bounds__(512) void fused_kernel(long n, long ld, int ns, const double* __restrict__ Y0,
-                                                    const double* __restrict__ T0, const double* __restrict__ p,
-                                                    const double* __restrict__ tabh, int Th_min, int Th_max,
-                                                    const double* __restrict__ scal, int d0, int dL,
-                                                    const __bf16* __restrict__ WT1, const __bf16* __restrict__ WT2,
-                                                    const __bf16* __restrict__ WT3, const float* __restrict__ b1,
-                                                    const float* __restrict__ b2, const float* __restrict__ b3,
-                                                    double* __restrict__ G) {
+__global__ __launch_bounds__(256) void pack_kernel(int d0, int dL, const __bf16* __restrict__ WT1,
+                                                   const __bf16* __restrict__ WT2, const __bf16* __restrict__ WT3,
+                                                   bf16x8* __restrict__ out) {
   constexpr int H1 = 32 * H1T, H2 = 32 * H2T, KS1 = 2, KS2 = 2 * H1T, KS3 = 2 * H2T;
-  __shared__ bf16x8 w1[H1T * KS1 * 64], w2[H2T * KS2 * 64], w3[KS3 * 64];
-  __shared__ float sb1[H1], sb2[H2], sb3[32];
-  for (int q = threadIdx.x; q < H1T * KS1 * 64; q += blockDim.x) {
-    const int ln = q & 63, ks = (q >> 6) % KS1, mt = q / (64 * KS1);
-    const int i = 32 * mt + (ln & 31), h = ln >> 5;
-    bf16x8 v;
+  constexpr int N1 = H1T * KS1 * 64, N2 = H2T * KS2 * 64, N3 = KS3 * 64;
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= N1 + N2 + N3) return;
+  const int ln = q & 63, h = ln >> 5;
+  bf16x8 v;
+  if (q < N1) {
+    const int ks = (q >> 6) % KS1, mt = q / (64 * KS1), i = 32 * mt + (ln & 31);
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
       const int k = 16 * ks + 8 * h + e;
       v[e] = k < d0 ? WT1[(long)i * d0 + k] : (__bf16)0.0f;
     }
-    w1[q] = v;
-  }
-  for (int q = threadIdx.x; q < H2T * KS2 * 64; q += blockDim.x) {
-    const int ln = q & 63, ks = (q >> 6) % KS2, mt = q / (64 * KS2);
-    const int i = 32 * mt + (ln & 31), h = ln >> 5;
-    bf16x8 v;
+  } else if (q < N1 + N2) {
+    const int r = q - N1, ks = (r >> 6) % KS2, mt = r / (64 * KS2), i = 32 * mt + (ln & 31);
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] = WT2[(long)i * H1 + chained_k(ks, h, e)];
-    w2[q] = v;
-  }
-  for (int q = threadIdx.x; q < KS3 * 64; q += blockDim.x) {
-    const int ln = q & 63, ks = q >> 6;
-    const int i = ln & 31, h = ln >> 5;
-    bf16x8 v;
+  } else {
+    const int ks = (q - N1 - N2) >> 6, i = ln & 31;
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] = i < dL ? WT3[(long)i * H2 + chained_k(ks, h, e)] : (__bf16)0.0f;
-    w3[q] = v;
   }
+  out[q] = v;
+}
+
+template <int H1T, int H2T>
+__global__ __launch_bounds__(512) void fused_kernel(long n, long ld, int ns, const double* __restrict__ Y0,
+                                                    const double* __restrict__ T0, const double* __restrict__ p,
+                                                    const double* __restrict__ tabh, int Th_min, int Th_max,
+                                                    const double* __restrict__ scal, int d0, int dL,
+                                                    const bf16x8* __restrict__ wpk, const float* __restrict__ b1,
+                                                    const float* __restrict__ b2, const float* __restrict__ b3,
+                                                    double* __restrict__ G) {
+  constexpr int H1 = 32 * H1T, H2 = 32 * H2T, KS1 = 2, KS2 = 2 * H1T, KS3 = 2 * H2T;
+  constexpr int N1 = H1T * KS1 * 64, N2 = H2T * KS2 * 64, N3 = KS3 * 64;
+  __shared__ bf16x8 wl[N1 + N2 + N3];
+  __shared__ float sb1[H1], sb2[H2], sb3[32];
+  const bf16x8* w1 = wl;
+  const bf16x8* w2 = wl + N1;
+  const bf16x8* w3 = wl + N1 + N2;
+  for (int q = threadIdx.x; q < N1 + N2 + N3; q += blockDim.x) wl[q] = wpk[q];
   for (int q = threadIdx.x; q < H1; q += blockDim.x) sb1[q] = b1[q];
   for (int q = threadIdx.x; q < H2; q += blockDim.x) sb2[q] = b2[q];
   for (int q = threadIdx.x; q < 32; q += blockDim.x) sb3[q] = q < dL ? b3[q] : 0.0f;
@@ -236,15 +245,26 @@ hipError_t flint_ws_collect(long n, const int* status, int* idx, int* count, hip
   return hipGetLastError();
 }
 
+size_t flint_ws_packed_bytes() { return (size_t)(4 * 2 + 4 * 8 + 8) * 64 * 16; }
+
+hipError_t flint_ws_pack(const int* dims, const __bf16* const* WT, void* packed, hipStream_t s) {
+  if (!flint_ws_fused_fits(dims)) return hipErrorInvalidValue;
+  const int total = (int)(flint_ws_packed_bytes() / 16);
+  hipLaunchKernelGGL((flint_ws_k::pack_kernel<4, 4>), dim3((total + 255) / 256), dim3(256), 0, s, dims[0], dims[3], WT[0],
+                     WT[1], WT[2], static_cast<flint_ws_k::bf16x8*>(packed));
+  return hipGetLastError();
+}
+
 hipError_t flint_ws_fused(long n, long ld, int ns, const double* Y0, const double* T0, const double* p,
                           const double* tabh, int Th_min, int Th_max, const double* scal, const int* dims,
-                          const __bf16* const* WT, const float* const* b, double* G, int blocks, hipStream_t s) {
+                          const void* packed, const float* const* b, double* G, int blocks, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (!flint_ws_fused_fits(dims)) return hipErrorInvalidValue;
   const long tiles = (n + 31) / 32;
   const long need = (tiles + 7) / 8;
   const unsigned nb = (unsigned)std::max<long>(1, std::min<long>(blocks, need));
   hipLaunchKernelGGL((flint_ws_k::fused_kernel<4, 4>), dim3(nb), dim3(512), 0, s, n, ld, ns, Y0, T0, p, tabh, Th_min,
-                     Th_max, scal, dims[0], dims[3], WT[0], WT[1], WT[2], b[0], b[1], b[2], G);
+                     Th_max, scal, dims[0], dims[3], static_cast<const flint_ws_k::bf16x8*>(packed), b[0], b[1], b[2],
+                     G);
   return hipGetLastError();
 }

@@ -204,7 +204,7 @@ class Mechanism:
         check(lib().flint_cea_solve_warm(self._h, net._h, n, ld, _p(p), _p(T), _p(rho), _p(sc), _p(T_out), _p(Y_out),
                                          _p(iters), _p(status), _stream(stream)))
 
-    def warm_options(self, max_iter=50, fused=True):
+    def warm_options(self, max_iter=5, fused=True):
         """Iteration cap of the warm-started pass of equilibrium_warm, and whether its features,
         network and Newton start run as one kernel (flint_cea_warm_options)."""
         check(lib().flint_cea_warm_options(self._h, int(max_iter), int(bool(fused))))

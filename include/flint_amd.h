@@ -159,11 +159,11 @@ int flint_cea_solve_host(flint_mech *m, long n, int hp, const double *p, const d
  * ns+1 outputs, see flint_mlp_create below) maps the standardised (Y0_k, h0, ln p) of each state
  * (h0 = sum_k Y0_k h_k(T0)) to the standardised Newton start (T, ln n_1..ln n_ns); scaling (host,
  * 4 ns + 6 doubles) = in_mean[ns+2], in_scale[ns+2], out_mean[ns+1], out_scale[ns+1].  The batch is
- * solved as flint_cea_solve with hp = 1 from those starts (with the reference's 50-iteration limit
- * unless flint_cea_warm_options sets a lower one), and the states that do not converge from theirs
- * are solved again from the default start, with the 50-iteration limit, in a second pass over just
- * those states (iters_dev then counts both), so the results are those of the default path wherever
- * it converges.  Asynchronous on stream, no host synchronisation.  Arguments as in
+ * solved as flint_cea_solve with hp = 1 from those starts, held to 5 iterations (see
+ * flint_cea_warm_options), and the states that do not converge within them are solved again from
+ * the default start, with the reference's 50-iteration limit, in a second pass over just those
+ * states (iters_dev then counts both), so the results are those of the default path wherever it
+ * converges.  Asynchronous on stream, no host synchronisation.  Arguments as in
  * flint_cea_solve (rho_dev = input mass fractions, p_dev in bar); status_dev may be NULL. */
 typedef struct flint_mlp flint_mlp;
 int flint_cea_solve_warm(flint_mech *m, flint_mlp *net, long n, long ld, const double *p_dev, const double *T_dev,
@@ -171,8 +171,9 @@ int flint_cea_solve_warm(flint_mech *m, flint_mlp *net, long n, long ld, const d
                          int *iters_dev, int *status_dev, void *stream);
 
 /* Options of flint_cea_solve_warm (no reference counterpart), held until the next
- * flint_cea_setup: max_iter (1..50, default 50) caps the warm-started pass, so that the states
- * that converge slowly from their predicted start move to the second pass; fused (default 1)
+ * flint_cea_setup: max_iter (1..50, default 5) caps the warm-started pass, so that the states
+ * that converge slowly from their predicted start move to the second pass (a wavefront runs as
+ * many iterations as its slowest state); fused (default 1)
  * computes the features, a 3-layer network (<= 32 inputs, hidden widths 128 and 128, <= 32
  * outputs) and the Newton start in one kernel, 0 runs them as separate launches (the layered
  * forward pass of flint_mlp_forward). */
