@@ -11,6 +11,8 @@ Beside it in the same JSON line:
   omega_dot    TSR-GP-24 source terms + analytic Jacobian, 2^20 cells (HBM roofline);
   omega_dot_wd WD source terms + heat release, 65,536 cells;
   equilibrium  TSR-GP-24 HP equilibrium of 2^18 states, default guess and learned warm start;
+  cons2prim    TSR-GP-24 conservative -> primitive variables (Newton on T), 2^20 cells with three
+               velocity components;
   cpu_baseline the C oracle (same algorithms, OpenMP over cells) on bounded samples of the same
                workloads on this host's cores.
 
@@ -39,7 +41,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--cells", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["reactor", "omega", "cea", "mlp"], default=None)
+    ap.add_argument("--only", choices=["reactor", "omega", "cea", "mlp", "thermo"], default=None)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run a plain device-to-device copy of this many bytes (counter calibration)")
@@ -266,6 +268,50 @@ def cea_leg(args, dev, stream):
     return res
 
 
+def thermo_leg(args, dev, stream):
+    import numpy as np
+    import torch
+    from flint_amd import states as S
+    from flint_amd.mechdata import mech_data
+    from flint_amd.native import Mechanism
+    md = mech_data("TSR-GP-24")
+    mech = Mechanism.from_data(md)
+    n, nd = 1 << 20, 3
+    fs = S.flow_states(md.species, md.W, 2026, 0, n, nd=nd)
+    q = torch.from_numpy(fs["prim"]).to(dev)
+    T0 = torch.from_numpy(fs["T0"]).to(dev)
+    cons, prim = torch.empty_like(q), torch.empty_like(q)
+    T = torch.empty(n, dtype=torch.float64, device=dev)
+    it = torch.empty(n, dtype=torch.int32, device=dev)
+    mech.prim2cons(q, cons, stream=stream)
+    run = lambda: mech.cons2prim(cons, T0, prim, T_out=T, stream=stream)
+    for _ in range(3):
+        run()
+    ms = timed(stream, run, 20)
+    mech.cons2prim(cons, T0, prim, T_out=T, iters=it, stream=stream)
+    torch.cuda.synchronize()
+    npr = md.ns + nd + 1
+    bpc = (2 * npr + 2) * 8  # cons and T0 in, prim and T out
+    res = dict(n=n, nd=nd, ms=ms, cells_per_s=n / (ms * 1e-3), bytes_per_cell=bpc,
+               achieved_GBps=bpc * n / (ms * 1e-3) / 1e9, iters_mean=float(it.float().mean()),
+               max_rel_dp=float((prim[-1] / q[-1] - 1.0).abs().max()))
+    if not args.no_cpu:
+        O = oracle()
+        o = O.Oracle(md)
+        m = 1 << 16
+        cs = cons[:, :m].cpu().numpy()
+        t0 = fs["T0"][:m].copy()
+        reps, t = 0, time.perf_counter()
+        while time.perf_counter() - t < 0.3 * args.cpu_seconds:
+            o.cons2prim(cs, t0)
+            reps += 1
+        dt = time.perf_counter() - t
+        res["cpu"] = dict(value=reps * m / dt, unit="cells/s", cores=o.num_threads(), kind="port",
+                          sample=f"{reps} passes over the first {m} cells, C oracle (OpenMP over cells), "
+                                 f"wall {dt:.1f} s")
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -292,6 +338,8 @@ def main():
         extra["omega_dot"], extra["omega_dot_wd"] = omega_leg(args, dev, stream)
     if only in (None, "cea"):
         extra["equilibrium"] = cea_leg(args, dev, stream)
+    if only in (None, "thermo"):
+        extra["cons2prim"] = thermo_leg(args, dev, stream)
     out = {
         "metric": "reactor cells advanced/s at rtol=1e-6 (plain batched FP64 BDF); MLP-assisted step and "
                   "omega_dot evals/s reported beside it",

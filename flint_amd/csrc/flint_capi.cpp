@@ -15,6 +15,7 @@
 
 #include "../../include/flint_amd.h"
 #include "flint_internal.h"
+#include "flint_thermo.h"
 #include "flint_registry.h"
 
 // ------------------------------------------------------------------------------------------------
@@ -216,6 +217,35 @@ extern "C" int flint_rhs(const flint_mech* m, long n, long ld, const double* Z, 
   return FLINT_OK;
 }
 
+static int check_thermo(const flint_mech* m, long n, long ld, int nd) {
+  if (int rc = check_batch(m, n, ld)) return rc;
+  if (nd < 0 || nd > 3) return flint_fail(FLINT_EARG, "need 0 <= nd <= 3 velocity components");
+  if (m->ns > FLINT_THERMO_MAXNS) return flint_fail(FLINT_EUNSUPPORTED, "more species than the kernel holds");
+  return FLINT_OK;
+}
+
+extern "C" int flint_cons2prim(const flint_mech* m, long n, long ld, int nd, const double* cons, const double* T,
+                               double* prim, double* T_out, int* iters, int* status, void* stream) {
+  if (int rc = check_thermo(m, n, ld, nd)) return rc;
+  if (n == 0) return FLINT_OK;
+  if (!cons || !T || !prim) return flint_fail(FLINT_EARG, "null array");
+  hipError_t e = flint_thermo_cons2prim(m->d_tabh, m->Th_min, m->Th_max, m->ns, m->d_W, n, ld, nd, cons, T, prim, T_out,
+                                        iters, status, (hipStream_t)stream);
+  if (e != hipSuccess) return flint_fail(FLINT_EHIP, hipGetErrorString(e));
+  return FLINT_OK;
+}
+
+extern "C" int flint_prim2cons(const flint_mech* m, long n, long ld, int nd, const double* prim, double* cons,
+                               int* status, void* stream) {
+  if (int rc = check_thermo(m, n, ld, nd)) return rc;
+  if (n == 0) return FLINT_OK;
+  if (!prim || !cons) return flint_fail(FLINT_EARG, "null array");
+  hipError_t e = flint_thermo_prim2cons(m->d_tabh, m->Th_min, m->Th_max, m->ns, m->d_W, n, ld, nd, prim, cons, status,
+                                        (hipStream_t)stream);
+  if (e != hipSuccess) return flint_fail(FLINT_EHIP, hipGetErrorString(e));
+  return FLINT_OK;
+}
+
 // ------------------------------------------------------------------------------------------------
 // host-memory convenience
 // ------------------------------------------------------------------------------------------------
@@ -243,6 +273,45 @@ extern "C" int flint_source_host(flint_mech* m, long n, const double* rho, const
   HIPCHK(hipMemcpyAsync(d_T, T, n * sizeof(double), hipMemcpyHostToDevice, m->stream));
   if (int rc = flint_source(m, n, n, d_rho, d_T, d_w, nullptr, nullptr, m->stream)) return rc;
   HIPCHK(hipMemcpyAsync(wdot, d_w, ns * n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return FLINT_OK;
+}
+
+extern "C" int flint_cons2prim_host(flint_mech* m, long n, int nd, const double* cons, const double* T, double* prim,
+                                    double* T_out, int* status) {
+  if (int rc = check_thermo(m, n, n, nd)) return rc;
+  if (n == 0) return FLINT_OK;
+  if (!cons || !T || !prim) return flint_fail(FLINT_EARG, "null array");
+  std::lock_guard<std::mutex> g(m->mu);
+  const size_t np = m->ns + nd + 1;
+  if (int rc = flint_stage(m, (2 * np + 3) * (size_t)n)) return rc;
+  double* d_c = m->d_stage;
+  double* d_p = d_c + np * n;
+  double* d_T = d_p + np * n;
+  double* d_To = d_T + n;
+  int* d_st = reinterpret_cast<int*>(d_To + n);
+  HIPCHK(hipMemcpyAsync(d_c, cons, np * n * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(d_T, T, n * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  if (int rc = flint_cons2prim(m, n, n, nd, d_c, d_T, d_p, d_To, nullptr, d_st, m->stream)) return rc;
+  HIPCHK(hipMemcpyAsync(prim, d_p, np * n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  if (T_out) HIPCHK(hipMemcpyAsync(T_out, d_To, n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  if (status) HIPCHK(hipMemcpyAsync(status, d_st, n * sizeof(int), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return FLINT_OK;
+}
+
+extern "C" int flint_prim2cons_host(flint_mech* m, long n, int nd, const double* prim, double* cons) {
+  if (int rc = check_thermo(m, n, n, nd)) return rc;
+  if (n == 0) return FLINT_OK;
+  if (!prim || !cons) return flint_fail(FLINT_EARG, "null array");
+  std::lock_guard<std::mutex> g(m->mu);
+  const size_t np = m->ns + nd + 1;
+  if (int rc = flint_stage(m, 2 * np * (size_t)n)) return rc;
+  double* d_q = m->d_stage;
+  double* d_c = d_q + np * n;
+  HIPCHK(hipMemcpyAsync(d_q, prim, np * n * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  if (int rc = flint_prim2cons(m, n, n, nd, d_q, d_c, nullptr, m->stream)) return rc;
+  HIPCHK(hipMemcpyAsync(cons, d_c, np * n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
   HIPCHK(hipStreamSynchronize(m->stream));
   return FLINT_OK;
 }

@@ -7,6 +7,7 @@
 !               m, then m lines "T p_bar Y_1 .. Y_ns"        (CEA_solve, UV and HP)
 !               k, then k lines "p_Pa Y_1 .. Y_ns T"         (constant-pressure reactors)
 !               t_end rtol atol                              (reactor integration)
+!               q nd, then q lines "T0 rho_1 .. rho_ns u_1 .. u_nd p"  (prim2cons, cons2prim)
 ! results file: blocks "# name rows cols" followed by the rows.
 program drop_in_check
   use flint_amd
@@ -19,6 +20,9 @@ program drop_in_check
   real(8), allocatable :: pr(:), X(:, :), row(:)
   integer, allocatable :: st(:)
   real(8) :: t_end, rtol, atol
+  integer :: q, nd, npr
+  real(8), allocatable :: T0(:), P0(:, :), C1(:, :), C2(:, :), Q1(:, :), Q2(:, :), Tq(:)
+  integer, allocatable :: sq(:)
 
   call get_command_argument(1, folder)
   call get_command_argument(2, fin)
@@ -55,6 +59,15 @@ program drop_in_check
     X(i, :) = row(2:nz + 1)
   end do
   read(u, *) t_end, rtol, atol
+  read(u, *) q, nd
+  npr = ns + nd + 1
+  deallocate(row)
+  allocate(row(max(nz + 1, npr + 1)), T0(q), P0(q, npr))
+  do i = 1, q
+    read(u, *) row(1:npr + 1)
+    T0(i) = row(1)
+    P0(i, :) = row(2:npr + 1)
+  end do
   close(u)
 
   open(newunit=v, file=trim(fout), status='replace', action='write')
@@ -95,6 +108,19 @@ program drop_in_check
   ! constant-pressure reactors
   call reactor_advance_batch(X, t_end, rtol, atol, constant_pressure=.true., p=pr, status=st(1:k))
   call put(v, 'reactor_advance_batch', reshape([X, dble(st(1:k))], [k, nz + 1]))
+
+  ! conservative <-> primitive variables, per cell and batched
+  allocate(C1(q, npr), Q1(q, npr), C2(q, npr), Q2(q, npr), Tq(q), sq(q))
+  do i = 1, q
+    C1(i, :) = prim2cons(P0(i, :))
+    Q1(i, :) = cons2prim(C1(i, :), T0(i))
+  end do
+  call put(v, 'prim2cons', C1)
+  call put(v, 'cons2prim', Q1)
+  call prim2cons_batch(P0, C2)
+  call cons2prim_batch(C2, T0, Q2, Tq, sq)
+  call put(v, 'prim2cons_batch', C2)
+  call put(v, 'cons2prim_batch', reshape([Q2, Tq, dble(sq)], [q, npr + 2]))
   close(v)
 
   call free_chemistry_data()

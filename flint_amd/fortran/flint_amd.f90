@@ -87,6 +87,26 @@ module flint_amd_c
       real(c_double), intent(out) :: T_out(*), Y_out(*)
       integer(c_int) :: rc
     end function
+    function flint_cons2prim_host(m, n, nd, cons, T, prim, T_out, status) bind(C, name="flint_cons2prim_host") &
+        result(rc)
+      import :: c_ptr, c_long, c_double, c_int
+      type(c_ptr), value :: m
+      integer(c_long), value :: n
+      integer(c_int), value :: nd
+      real(c_double), intent(in) :: cons(*), T(*)
+      real(c_double), intent(out) :: prim(*)
+      type(c_ptr), value :: T_out, status
+      integer(c_int) :: rc
+    end function
+    function flint_prim2cons_host(m, n, nd, prim, cons) bind(C, name="flint_prim2cons_host") result(rc)
+      import :: c_ptr, c_long, c_double, c_int
+      type(c_ptr), value :: m
+      integer(c_long), value :: n
+      integer(c_int), value :: nd
+      real(c_double), intent(in) :: prim(*)
+      real(c_double), intent(out) :: cons(*)
+      integer(c_int) :: rc
+    end function
   end interface
 end module flint_amd_c
 
@@ -110,7 +130,8 @@ module flint_amd
 
   public :: read_idealgas_thermo, read_chemistry, Assign_Mechanism, chemistry_source, rhs_native, &
             CEA_initialize_global, CEA_solve, free_chemistry_data, chemistry_source_batch, &
-            rhs_native_batch, reactor_advance_batch, CEA_solve_batch, flint_error_message
+            rhs_native_batch, reactor_advance_batch, CEA_solve_batch, flint_error_message, cons2prim, &
+            prim2cons, cons2prim_batch, prim2cons_batch
 
 contains
 
@@ -282,6 +303,74 @@ contains
     mech = c_null_ptr
     ns = 0
     loaded_mech_name = ''
+  end subroutine
+
+  !> Primitive variables (rho_1..rho_ns, u_1..u_nd, p) of one cell from its conservative variables
+  !> (rho_1..rho_ns, rho u_1..rho u_nd, rho E), Newton on T from T (Lib_ThermoTransport.f90:130);
+  !> 0 <= nd <= 3.  p is NaN when T leaves the thermo table.  Not pure, unlike the reference's.
+  function cons2prim(cons, T) result(prim)
+    real(8), intent(in) :: cons(:)
+    real(8), intent(in) :: T
+    real(8) :: prim(size(cons))
+    real(c_double) :: c(size(cons)), q(size(cons)), tt(1)
+    call need_mech('cons2prim')
+    c = cons
+    tt(1) = T
+    call fail(flint_cons2prim_host(mech, 1_c_long, int(size(cons) - ns - 1, c_int), c, tt, q, c_null_ptr, &
+                                   c_null_ptr), 'cons2prim')
+    prim = q
+  end function
+
+  !> Conservative variables of one cell from its primitive ones (Lib_ThermoTransport.f90:168).
+  function prim2cons(prim) result(cons)
+    real(8), intent(in) :: prim(:)
+    real(8) :: cons(size(prim))
+    real(c_double) :: q(size(prim)), c(size(prim))
+    call need_mech('prim2cons')
+    q = prim
+    call fail(flint_prim2cons_host(mech, 1_c_long, int(size(prim) - ns - 1, c_int), q, c), 'prim2cons')
+    cons = c
+  end function
+
+  !> cons2prim over a block of cells: cons(n, np), T(n) -> prim(n, np), with the converged T in
+  !> T_out(n) and status(n) (0 ok, 2 T outside the thermo table) if present.
+  subroutine cons2prim_batch(cons, T, prim, T_out, status)
+    real(8), intent(in) :: cons(:, :), T(:)
+    real(8), intent(out) :: prim(:, :)
+    real(8), intent(out), optional :: T_out(:)
+    integer, intent(out), optional :: status(:)
+    real(c_double), allocatable :: c(:, :), q(:, :), tt(:)
+    real(c_double), allocatable, target :: to(:)
+    integer(c_int), allocatable, target :: st(:)
+    integer(c_long) :: n
+    integer :: npr
+    call need_mech('cons2prim_batch')
+    n = size(T, kind=c_long)
+    npr = size(cons, 2)
+    c = cons(1:n, 1:npr)
+    tt = T
+    allocate(q(n, npr), to(n), st(n))
+    call fail(flint_cons2prim_host(mech, n, int(npr - ns - 1, c_int), c, tt, q, c_loc(to), c_loc(st)), &
+              'cons2prim_batch')
+    prim(1:n, 1:npr) = q
+    if (present(T_out)) T_out(1:n) = to
+    if (present(status)) status(1:n) = st
+  end subroutine
+
+  !> prim2cons over a block of cells: prim(n, np) -> cons(n, np).
+  subroutine prim2cons_batch(prim, cons)
+    real(8), intent(in) :: prim(:, :)
+    real(8), intent(out) :: cons(:, :)
+    real(c_double), allocatable :: q(:, :), c(:, :)
+    integer(c_long) :: n
+    integer :: npr
+    call need_mech('prim2cons_batch')
+    n = size(prim, 1, kind=c_long)
+    npr = size(prim, 2)
+    q = prim
+    allocate(c(n, npr))
+    call fail(flint_prim2cons_host(mech, n, int(npr - ns - 1, c_int), q, c), 'prim2cons_batch')
+    cons(1:n, 1:npr) = c
   end subroutine
 
   !> chemistry_source over a block of cells: rho(n, ns), T(n) -> wdot(n, ns).

@@ -47,6 +47,10 @@ def lib():
         L.flint_cea_solve.argtypes = [vp, l, l, i, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.flint_source_host.argtypes = [vp, l, vp, vp, vp]
         L.flint_rhs_host.argtypes = [vp, l, vp, vp, vp]
+        L.flint_cons2prim.argtypes = [vp, l, l, i] + [vp] * 7
+        L.flint_prim2cons.argtypes = [vp, l, l, i, vp, vp, vp, vp]
+        L.flint_cons2prim_host.argtypes = [vp, l, i] + [vp] * 5
+        L.flint_prim2cons_host.argtypes = [vp, l, i, vp, vp]
         L.flint_reactor_advance_host.argtypes = [vp, l, i, vp, vp, d, d, d, i, vp]
         L.flint_cea_solve_host.argtypes = [vp, l, i, vp, vp, vp, vp, vp, vp]
         L.flint_cea_solve_warm.argtypes = [vp, vp, l, l, vp, vp, vp, vp, vp, vp, vp, vp, vp]
@@ -152,6 +156,23 @@ class Mechanism:
         n = ld if n is None else n
         check(lib().flint_rhs(self._h, n, ld, _p(Z), _p(F), _p(J), _p(status), _stream(stream)))
 
+    # ---- conservative <-> primitive variables ----
+    def cons2prim(self, cons, T, prim, T_out=None, iters=None, status=None, n=None, stream=None):
+        """cons [ns+nd+1, ld] (rho_k, rho u, rho E) -> prim (rho_k, u, p), Newton on T from T
+        (flint_cons2prim)."""
+        ld = cons.shape[-1]
+        n = ld if n is None else n
+        nd = cons.shape[0] - self.ns - 1
+        check(lib().flint_cons2prim(self._h, n, ld, nd, _p(cons), _p(T), _p(prim), _p(T_out), _p(iters), _p(status),
+                                    _stream(stream)))
+
+    def prim2cons(self, prim, cons, status=None, n=None, stream=None):
+        """prim [ns+nd+1, ld] (rho_k, u, p) -> cons (rho_k, rho u, rho E) (flint_prim2cons)."""
+        ld = prim.shape[-1]
+        n = ld if n is None else n
+        nd = prim.shape[0] - self.ns - 1
+        check(lib().flint_prim2cons(self._h, n, ld, nd, _p(prim), _p(cons), _p(status), _stream(stream)))
+
     # ---- batched reactors ----
     CV, CP = 0, 1
 
@@ -224,6 +245,23 @@ class Mechanism:
         J = np.empty((self.nz * self.nz, n)) if jac else None
         check(lib().flint_rhs_host(self._h, n, _p(Z), _p(F), _p(J)))
         return (F, J) if jac else F
+
+    def cons2prim_host(self, cons, T):
+        """cons [ns+nd+1, n] (numpy) -> (prim, T_out, status)."""
+        cons = np.ascontiguousarray(cons, dtype=np.float64)
+        npr, n = cons.shape
+        prim, T_out = np.empty_like(cons), np.empty(n)
+        st = np.empty(n, dtype=np.int32)
+        check(lib().flint_cons2prim_host(self._h, n, npr - self.ns - 1, _p(cons),
+                                         _p(np.ascontiguousarray(T, dtype=np.float64)), _p(prim), _p(T_out), _p(st)))
+        return prim, T_out, st
+
+    def prim2cons_host(self, prim):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        npr, n = prim.shape
+        cons = np.empty_like(prim)
+        check(lib().flint_prim2cons_host(self._h, n, npr - self.ns - 1, _p(prim), _p(cons)))
+        return cons
 
     def advance_host(self, X, p=None, mode=0, t_end=1e-4, rtol=1e-6, atol=1e-12, max_steps=0):
         """X [ns+1, n] (numpy) advanced from 0 to t_end; returns (X_new, status)."""

@@ -109,6 +109,19 @@ def cea_states(species, W, seed, start, count):
     return dict(Y=st["Y"], T0=st["T"], p_bar=st["p"] / 1e5)
 
 
+def flow_states(species, W, seed, start, count, nd=3):
+    """Seeded CFD cells of the cons2prim / prim2cons workload: the composition and pressure of
+    :func:`radical_states` with T~U[600,2800] K, nd velocity components ~U[-300,300] m/s, and a
+    Newton start T0 within 2 % of the cell's T (as a CFD step starts from the previous step's
+    temperature).  Returns prim [ns+nd+1, count] = (rho_k, u, p), T and T0."""
+    st = radical_states(species, W, seed, start, count, T_lo=600.0, T_hi=2800.0)
+    u = uniforms(seed + 7919, np.arange(start, start + count), nd + 1)
+    vel = -300.0 + 600.0 * u[:, :nd]
+    T0 = st["T"] * (1.0 + 0.04 * (u[:, nd] - 0.5))
+    prim = np.ascontiguousarray(np.vstack([soa(st["rho"]), soa(vel), st["p"][None, :]]))
+    return dict(prim=prim, T=st["T"], T0=T0)
+
+
 def soa(a):
     """(n, k) -> contiguous (k, n) structure-of-arrays."""
     return np.ascontiguousarray(np.asarray(a).T)

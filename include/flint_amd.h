@@ -84,6 +84,27 @@ int flint_rhs(const flint_mech *m, long n, long ld, const double *Z_dev, double 
 int flint_source_host(flint_mech *m, long n, const double *rho, const double *T, double *wdot);
 int flint_rhs_host(flint_mech *m, long n, const double *Z, double *F, double *J);
 
+/* cons2prim over a batch (Lib_ThermoTransport.f90:130-165): cons_dev[np][ld] = (rho_1..rho_ns,
+ * rho u_1..rho u_nd, rho E) with np = ns + nd + 1 and 0 <= nd <= 3, T_dev[n] the Newton start ->
+ * prim_dev[np][ld] = (rho_1..rho_ns, u_1..u_nd, p).  Newton on T from the internal energy with
+ * the reference's tolerance (|dE/E| < 1e-6) and limit (20 iterations), then p = rho R T.  T_out_dev
+ * (the converged T), iters_dev and status_dev may be NULL; status 2 (and NaN outputs) where T
+ * leaves the thermo table, which the reference does not check. */
+int flint_cons2prim(const flint_mech *m, long n, long ld, int nd, const double *cons_dev, const double *T_dev,
+                    double *prim_dev, double *T_out_dev, int *iters_dev, int *status_dev, void *stream);
+
+/* prim2cons over a batch (Lib_ThermoTransport.f90:168-185), the inverse map: prim_dev -> cons_dev
+ * with rho E = rho (E(p, rho_k) + |u|^2 / 2); status 2 (NaN energy) where p / (R rho) is outside
+ * the thermo table. */
+int flint_prim2cons(const flint_mech *m, long n, long ld, int nd, const double *prim_dev, double *cons_dev,
+                    int *status_dev, void *stream);
+
+/* Host-memory forms of flint_cons2prim / flint_prim2cons (ld = n, synchronous, as flint_rhs_host);
+ * T_out and status may be NULL. */
+int flint_cons2prim_host(flint_mech *m, long n, int nd, const double *cons, const double *T, double *prim,
+                         double *T_out, int *status);
+int flint_prim2cons_host(flint_mech *m, long n, int nd, const double *prim, double *cons);
+
 /* Batched 0D reactors (no reference counterpart in FLINT's own code: it advances one reactor at a
  * time through the external OSlo integrator, test-batchF.f90:81,105, with rhs_native as the
  * callback).  Advances n cells from t = 0 to t_end in one persistent launch, variable-order BDF

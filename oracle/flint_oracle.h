@@ -63,6 +63,12 @@ double *orc_nu_alloc(const orc_mech *m);
 void orc_rhs_cell(const orc_mech *m, int mode, const double *x, double p, double *F);
 void orc_jac_cell(const orc_mech *m, const double *nu, int mode, const double *x, double p, double *F, double *J);
 
+/* conservative <-> primitive variables (oracle_thermo.c): cons = (rho_1..rho_ns, rho u_1..rho u_nd,
+ * rho E), prim = (rho_1..rho_ns, u_1..u_nd, p), np = ns + nd + 1 rows, ns <= 64, nd <= 8 */
+void orc_cons2prim(const orc_mech *m, long n, long ld, int np, const double *cons, const double *T, double *prim,
+                   double *Tout, int *iters, int *status);
+void orc_prim2cons(const orc_mech *m, long n, long ld, int np, const double *prim, double *cons, int *status);
+
 /* BDF advance of n reactors over [0, t_end] (oracle_bdf.c); stats[i*8..] and status[i] per cell */
 void orc_bdf(const orc_mech *m, long n, long ld, int mode, double *X, const double *p, double rtol, double atol,
              double t_end, int max_steps, int *stats, int *status);

@@ -96,6 +96,26 @@ class Oracle:
         self.L.orc_wd_hrr(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(rho), _ptr(T), _ptr(w), _ptr(q))
         return w, q
 
+    def cons2prim(self, cons, T):
+        """cons [ns+nd+1, n] -> (prim [ns+nd+1, n], T_out [n], iters [n], status [n]) (cons2prim)."""
+        cons = np.ascontiguousarray(cons, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        npr, n = cons.shape
+        prim, Tout = np.empty_like(cons), np.empty(n)
+        it, st = np.empty(n, dtype=np.int32), np.empty(n, dtype=np.int32)
+        self.L.orc_cons2prim(C.byref(self.m), C.c_long(n), C.c_long(n), C.c_int(npr), _ptr(cons), _ptr(T), _ptr(prim),
+                             _ptr(Tout), _ptr(it), _ptr(st))
+        return prim, Tout, it, st
+
+    def prim2cons(self, prim):
+        """prim [ns+nd+1, n] -> (cons, status) (prim2cons)."""
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        npr, n = prim.shape
+        cons = np.empty_like(prim)
+        st = np.empty(n, dtype=np.int32)
+        self.L.orc_prim2cons(C.byref(self.m), C.c_long(n), C.c_long(n), C.c_int(npr), _ptr(prim), _ptr(cons), _ptr(st))
+        return cons, st
+
     def rhs(self, Z):
         Z = np.ascontiguousarray(Z, dtype=np.float64)
         n = Z.shape[1]
@@ -215,6 +235,21 @@ class Reference:
         w = np.empty_like(rho)
         self.L.ref_wdot(C.c_long(n), C.c_long(n), _ptr(rho), _ptr(T), _ptr(w))
         return w
+
+    def cons2prim(self, cons, T):
+        cons = np.ascontiguousarray(cons, dtype=np.float64)
+        npr, n = cons.shape
+        prim = np.empty_like(cons)
+        self.L.ref_cons2prim(C.c_long(n), C.c_long(n), C.c_int(npr), _ptr(cons),
+                             _ptr(np.ascontiguousarray(T, dtype=np.float64)), _ptr(prim))
+        return prim
+
+    def prim2cons(self, prim):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        npr, n = prim.shape
+        cons = np.empty_like(prim)
+        self.L.ref_prim2cons(C.c_long(n), C.c_long(n), C.c_int(npr), _ptr(prim), _ptr(cons))
+        return cons
 
     def cea(self, p, T, rho, hp=False):
         rho = np.ascontiguousarray(rho, dtype=np.float64)

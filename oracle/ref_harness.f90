@@ -140,4 +140,35 @@ contains
     enddo
   end subroutine ref_cea
 
+  ! cons2prim (Lib_ThermoTransport.f90:130-165) over n cells: cons(ld, np) = (rho_1..rho_ns,
+  ! rho u_1..rho u_nd, rho E) -> prim(ld, np) = (rho_1..rho_ns, u_1..u_nd, p), T the Newton start
+  subroutine ref_cons2prim(n, ld, np, cons, T, prim) bind(C, name='ref_cons2prim')
+    integer(c_long), value :: n, ld
+    integer(c_int), value :: np
+    real(c_double), intent(in) :: cons(ld, *), T(*)
+    real(c_double), intent(out) :: prim(ld, *)
+    real(8) :: c(np), q(np)
+    integer(c_long) :: i
+    do i = 1, n
+      c = cons(i, 1:np)
+      q = cons2prim(c, T(i))
+      prim(i, 1:np) = q
+    enddo
+  end subroutine ref_cons2prim
+
+  ! prim2cons (Lib_ThermoTransport.f90:168-185) over n cells, the inverse map
+  subroutine ref_prim2cons(n, ld, np, prim, cons) bind(C, name='ref_prim2cons')
+    integer(c_long), value :: n, ld
+    integer(c_int), value :: np
+    real(c_double), intent(in) :: prim(ld, *)
+    real(c_double), intent(out) :: cons(ld, *)
+    real(8) :: c(np), q(np)
+    integer(c_long) :: i
+    do i = 1, n
+      q = prim(i, 1:np)
+      c = prim2cons(q)
+      cons(i, 1:np) = c
+    enddo
+  end subroutine ref_prim2cons
+
 end module ref_harness

@@ -8,6 +8,9 @@
 * cea_<mech>.npz   CEA_solve outputs (HP and UV, oracle/_ref: Lib_CEA_setup.f90 and
                    Lib_CEA_solver.f90 compiled by flang) on seeded states from
                    flint_amd.states.cea_states (seed, start, count stored with them).
+* cons2prim_<mech>.npz  prim2cons and cons2prim outputs of the reference (oracle/_ref:
+                   Lib_ThermoTransport.f90 compiled by flang) on seeded cells from
+                   flint_amd.states.flow_states (seed, start, count, nd stored with them).
 * wdot_<mech>.npz  chemistry_source outputs of the reference Fortran routines themselves
                    (oracle/_ref: WD.f90, TSR-GP-24.f90, TSR-Rich-31.f90 compiled by flang) on seeded
                    states; the inputs are regenerated from (seed, start, count) by flint_amd.states.
@@ -88,7 +91,22 @@ def cea():
         print(mech, "cea", count)
 
 
+C2P_CASES = {"TSR-GP-24": (15, 0, 512, 3), "TSR-Rich-31": (16, 0, 512, 2)}
+
+
+def cons2prim():
+    for mech, (seed, start, count, nd) in C2P_CASES.items():
+        md = mech_data(mech)
+        fs = S.flow_states(md.species, md.W, seed, start, count, nd=nd)
+        ref = O.Reference(md)
+        cons = ref.prim2cons(fs["prim"])
+        prim = ref.cons2prim(cons, fs["T0"])
+        np.savez_compressed(os.path.join(OUT, f"cons2prim_{mech}.npz"), seed=seed, start=start, count=count, nd=nd,
+                            cons=cons, prim=prim)
+        print(mech, "cons2prim", count, nd)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["tables", "wdot", "cea"]
+    which = sys.argv[1:] or ["tables", "wdot", "cea", "cons2prim"]
     for w in which:
         globals()[w]()

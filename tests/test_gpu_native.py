@@ -6,7 +6,8 @@
   reactor steps are bit-identical to the one built from the same tables in memory;
 * the Fortran module (flint_amd/fortran/flint_amd.f90) called the way a FLINT caller calls
   read_idealgas_thermo / read_chemistry / Assign_Mechanism / chemistry_source / rhs_native /
-  CEA_solve returns, per cell, exactly what the C ABI returns for the same cells."""
+  CEA_solve / cons2prim / prim2cons returns, per cell, exactly what the C ABI returns for the
+  same cells."""
 import os
 import subprocess
 
@@ -90,6 +91,8 @@ def test_fortran_drop_in_matches_c_abi(native, tmp_path):
     cs = S.radical_states(md.species, md.W, 94, 0, m, T_lo=600.0, T_hi=900.0)
     rs = S.radical_states(md.species, md.W, 95, 0, k, ignition_half=True)
     t_end, rtol, atol = 2e-5, 1e-6, 1e-12
+    q, nd = 40, 3
+    fs = S.flow_states(md.species, md.W, 96, 0, q, nd=nd)
     fin, fout = tmp_path / "states.txt", tmp_path / "results.txt"
     with open(fin, "w") as f:
         f.write(f"{n}\n")
@@ -102,6 +105,9 @@ def test_fortran_drop_in_matches_c_abi(native, tmp_path):
         for i in range(k):
             f.write(" ".join("%.17e" % v for v in [rs["p"][i], *rs["Y"][i], rs["T"][i]]) + "\n")
         f.write("%.17e %.17e %.17e\n" % (t_end, rtol, atol))
+        f.write(f"{q} {nd}\n")
+        for i in range(q):
+            f.write(" ".join("%.17e" % v for v in [fs["T0"][i], *fs["prim"][:, i]]) + "\n")
     print("[gpu] fortran drop_in_check", flush=True)
     r = subprocess.run([EXE, folder, str(fin), str(fout)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
@@ -122,3 +128,9 @@ def test_fortran_drop_in_matches_c_abi(native, tmp_path):
                             atol=atol)
     assert _eq(got["reactor_advance_batch"], np.vstack([X, sx[None]]).T)
     assert np.mean(sx == 0) > 0.9 and np.mean(sh == 0) > 0.9
+    cons = mm.prim2cons_host(fs["prim"])
+    prim, Tq, sq = mm.cons2prim_host(cons, fs["T0"])
+    assert _eq(got["prim2cons"], cons.T) and _eq(got["prim2cons_batch"], cons.T)
+    assert _eq(got["cons2prim"], prim.T)
+    assert _eq(got["cons2prim_batch"], np.vstack([prim, Tq[None], sq[None]]).T)
+    assert np.all(sq == 0)
