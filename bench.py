@@ -295,6 +295,10 @@ def thermo_leg(args, dev, stream):
     res = dict(n=n, nd=nd, ms=ms, cells_per_s=n / (ms * 1e-3), bytes_per_cell=bpc,
                achieved_GBps=bpc * n / (ms * 1e-3) / 1e9, iters_mean=float(it.float().mean()),
                max_rel_dp=float((prim[-1] / q[-1] - 1.0).abs().max()))
+    tp = os.path.join(ROOT, "profiles", "cons2prim_tsr_gp_24_1M.traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            res["traffic"] = json.load(f).get("hbm_bytes_per_launch")
     if not args.no_cpu:
         O = oracle()
         o = O.Oracle(md)

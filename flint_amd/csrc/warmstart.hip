@@ -213,11 +213,19 @@ __global__ void zero_kernel(int* count) {
   if (threadIdx.x == 0 && blockIdx.x == 0) count[0] = 0;
 }
 
+// one atomic per wavefront: the first flagged lane reserves the wavefront's slots
 __global__ __launch_bounds__(256) void collect_kernel(long n, const int* __restrict__ status, int* __restrict__ idx,
                                                       int* __restrict__ count) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= n) return;
-  if (status[c] != 0) idx[atomicAdd(count, 1)] = (int)c;
+  const int lane = threadIdx.x & 63;
+  const bool f = c < n && status[c] != 0;
+  const unsigned long long mask = __ballot(f);
+  if (mask == 0) return;
+  const int leader = __ffsll((long long)mask) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(count, __popcll(mask));
+  base = __shfl(base, leader, 64);
+  if (f) idx[base + __popcll(mask & ((1ull << lane) - 1))] = (int)c;
 }
 
 }  // namespace flint_ws_k
