@@ -13,6 +13,8 @@ Beside it in the same JSON line:
   equilibrium  TSR-GP-24 HP equilibrium of 2^18 states, default guess and learned warm start;
   cons2prim    TSR-GP-24 conservative -> primitive variables (Newton on T), 2^20 cells with three
                velocity components;
+  source_general  TSR-GP-24 source terms by the data-driven general routine, 2^20 cells, beside
+               the generated explicit routine on the same cells;
   cpu_baseline the C oracle (same algorithms, OpenMP over cells) on bounded samples of the same
                workloads on this host's cores.
 
@@ -41,7 +43,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--cells", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["reactor", "omega", "cea", "mlp", "thermo"], default=None)
+    ap.add_argument("--only", choices=["reactor", "omega", "cea", "mlp", "thermo", "general"], default=None)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run a plain device-to-device copy of this many bytes (counter calibration)")
@@ -316,6 +318,45 @@ def thermo_leg(args, dev, stream):
     return res
 
 
+def general_leg(args, dev, stream):
+    import numpy as np
+    import torch
+    from flint_amd import states as S
+    from flint_amd.mechdata import mech_data
+    from flint_amd.native import Mechanism
+    md = mech_data("TSR-GP-24")
+    mech = Mechanism.from_data(md)
+    mech.general_setup(md.general)
+    n = 1 << 20
+    st = S.radical_states(md.species, md.W, 2027, 0, n)
+    rho = torch.from_numpy(S.soa(st["rho"])).to(dev)
+    T = torch.from_numpy(st["T"]).to(dev)
+    w = torch.empty_like(rho)
+    run_g = lambda: mech.source_general(rho, T, w, stream=stream)
+    run_e = lambda: mech.source(rho, T, w, stream=stream)
+    for _ in range(3):
+        run_g(), run_e()
+    ms_g, ms_e = timed(stream, run_g, 10), timed(stream, run_e, 10)
+    bpc = (2 * md.ns + 1) * 8
+    res = dict(n=n, ms=ms_g, cells_per_s=n / (ms_g * 1e-3), bytes_per_cell=bpc,
+               achieved_GBps=bpc * n / (ms_g * 1e-3) / 1e9, explicit_ms=ms_e,
+               explicit_cells_per_s=n / (ms_e * 1e-3))
+    if not args.no_cpu:
+        O = oracle()
+        o = O.Oracle(md)
+        m = 1 << 14
+        rs, Ts = S.soa(st["rho"][:m]), st["T"][:m].copy()
+        reps, t = 0, time.perf_counter()
+        while time.perf_counter() - t < 0.3 * args.cpu_seconds:
+            o.general(rs, Ts)
+            reps += 1
+        dt = time.perf_counter() - t
+        res["cpu"] = dict(value=reps * m / dt, unit="cells/s", cores=o.num_threads(), kind="port",
+                          sample=f"{reps} passes over the first {m} cells, C oracle (OpenMP over cells), "
+                                 f"wall {dt:.1f} s")
+    return res
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -344,6 +385,8 @@ def main():
         extra["equilibrium"] = cea_leg(args, dev, stream)
     if only in (None, "thermo"):
         extra["cons2prim"] = thermo_leg(args, dev, stream)
+    if only in (None, "general"):
+        extra["source_general"] = general_leg(args, dev, stream)
     out = {
         "metric": "reactor cells advanced/s at rtol=1e-6 (plain batched FP64 BDF); MLP-assisted step and "
                   "omega_dot evals/s reported beside it",

@@ -9,6 +9,7 @@
 #include "flint_registry.h"
 
 struct flint_extra;     // per-handle state of the reactor path
+struct flint_general_data;  // program of the data-driven general source term
 struct flint_cea_data;  // per-handle state of the equilibrium path
 struct flint_sur_data;  // per-handle buffers of the learned reactor step
 
@@ -31,6 +32,7 @@ struct flint_mech {
   size_t stage_cap = 0;
   std::mutex mu;
   flint_extra* extra = nullptr;
+  flint_general_data* gen = nullptr;
   flint_cea_data* cea = nullptr;
   flint_sur_data* sur = nullptr;
 };
@@ -42,4 +44,5 @@ int flint_stage(flint_mech* m, size_t doubles);
 void flint_mech_release_extra(flint_mech* m);
 void flint_mech_release_cea(flint_mech* m);
 void flint_mech_release_sur(flint_mech* m);
+void flint_mech_release_general(flint_mech* m);
 double flint_atomic_weight(const char* symbol);

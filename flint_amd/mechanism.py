@@ -156,6 +156,37 @@ def compile_mechanism(name_or_desc):
 KIND_CODE = {"arrhenius": 0, "three-body": 1, "troe": 2, "lindemann": 3}
 
 
+def general_tables(desc):
+    """Tables of FLINT's data-driven ``general`` routine (Lib_Chemistry_wdot.f90:96-221), as the
+    Reaction definition block of chemistry-info.txt holds them (Load_Chemistry.f90:55-89).
+
+    Returns (na, nt, nl, ni1, ni2, eps): for every reaction of each kind (Arrhenius and three-body,
+    then Troe, then Lindemann, each in file order) ns+1 values of the reactant and product
+    coefficients and the third-body efficiencies, the last entry being the third body M
+    (ni1 = ni2 = 1 for three-body and fall-off reactions; its efficiency 0).  Efficiencies are the
+    exact doubles of the mechanism file (not the float32-rounded literals of the generated
+    routines).  Arrays are [na+nt+nl][ns+1], the memory order of FLINT's (ns+1, nrc) tables."""
+    sp = desc["species"]
+    ns = len(sp)
+    rx = desc["reactions"]
+    rows = [r for r in rx if r["type"] in ("arrhenius", "three-body")] + \
+           [r for r in rx if r["type"] == "troe"] + [r for r in rx if r["type"] == "lindemann"]
+    ni1 = np.zeros((len(rows), ns + 1))
+    ni2 = np.zeros_like(ni1)
+    eps = np.zeros_like(ni1)
+    for j, r in enumerate(rows):
+        for k, v in r["nu_r"].items():
+            ni1[j, int(k)] = v
+        for k, v in r["nu_p"].items():
+            ni2[j, int(k)] = v
+        if r["type"] != "arrhenius":
+            ni1[j, ns] = ni2[j, ns] = 1.0
+            eps[j, :ns] = r.get("eff_ytf", r["eff"])
+    na = sum(r["type"] in ("arrhenius", "three-body") for r in rows)
+    nt = sum(r["type"] == "troe" for r in rows)
+    return na, nt, len(rows) - na - nt, ni1, ni2, eps
+
+
 def flatten(prog):
     """Encode the program as flat arrays (all int32 unless noted).
 

@@ -10,7 +10,7 @@ import functools
 import numpy as np
 
 from . import tables as _tables
-from .mechanism import compile_mechanism
+from .mechanism import compile_mechanism, general_tables
 
 
 class MechData:
@@ -42,6 +42,11 @@ class MechData:
         self.cp, self.h, self.s = (np.ascontiguousarray(a) for a in (cp, h, s))
         self.elements = self.desc["elements"]
         self.composition = np.array(self.desc["composition"], dtype=np.float64)  # [ne, ns]
+
+    @functools.cached_property
+    def general(self):
+        """(na, nt, nl, ni1, ni2, eps) of FLINT's data-driven general routine (general_tables)."""
+        return general_tables(self.desc)
 
     @property
     def n_arrh(self):

@@ -47,6 +47,8 @@ def lib():
         L.flint_cea_solve.argtypes = [vp, l, l, i, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.flint_source_host.argtypes = [vp, l, vp, vp, vp]
         L.flint_rhs_host.argtypes = [vp, l, vp, vp, vp]
+        L.flint_general_setup.argtypes = [vp, i, i, i, vp, vp, vp]
+        L.flint_source_general.argtypes = [vp, l, l, vp, vp, vp, vp, vp]
         L.flint_cons2prim.argtypes = [vp, l, l, i] + [vp] * 7
         L.flint_prim2cons.argtypes = [vp, l, l, i, vp, vp, vp, vp]
         L.flint_cons2prim_host.argtypes = [vp, l, i] + [vp] * 5
@@ -155,6 +157,20 @@ class Mechanism:
         ld = Z.shape[-1]
         n = ld if n is None else n
         check(lib().flint_rhs(self._h, n, ld, _p(Z), _p(F), _p(J), _p(status), _stream(stream)))
+
+    # ---- the data-driven general routine ----
+    def general_setup(self, tables):
+        """tables = (na, nt, nl, ni1, ni2, eps) as flint_amd.mechanism.general_tables returns them
+        (flint_general_setup)."""
+        na, nt, nl, ni1, ni2, eps = tables
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (ni1, ni2, eps)]
+        check(lib().flint_general_setup(self._h, int(na), int(nt), int(nl), *(_p(x) for x in a)))
+
+    def source_general(self, rho, T, wdot, status=None, n=None, stream=None):
+        """rho [ns, ld], T [ld] -> wdot [ns, ld] with the general routine (flint_source_general)."""
+        ld = rho.shape[-1]
+        n = ld if n is None else n
+        check(lib().flint_source_general(self._h, n, ld, _p(rho), _p(T), _p(wdot), _p(status), _stream(stream)))
 
     # ---- conservative <-> primitive variables ----
     def cons2prim(self, cons, T, prim, T_out=None, iters=None, status=None, n=None, stream=None):

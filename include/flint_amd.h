@@ -70,6 +70,18 @@ int flint_mech_query(const flint_mech *m, int *ns, int *Tk_min, int *Tk_max, int
 int flint_source(const flint_mech *m, long n, long ld, const double *rho_dev, const double *T_dev, double *wdot_dev,
                  double *hrr_dev, int *status_dev, void *stream);
 
+/* The data-driven general routine (Lib_Chemistry_wdot.f90:96-221, the chemistry_source FLINT's
+ * Assign_Mechanism falls back to for a mechanism without an explicit routine): its tables, as
+ * chemistry-info.txt's Reaction definition block holds them (Load_Chemistry.f90:55-89), for every
+ * reaction of each kind (Arrhenius and three-body, then Troe, then Lindemann, each in file order,
+ * na/nt/nl matching the mechanism's rate tables): ni1, ni2, eps [na+nt+nl][ns+1] (host), the
+ * last entry of a row being the third body M.  flint_source_general then evaluates it over a batch
+ * as flint_source does (status bit 0: T outside the kinetics table, the row clamped). */
+int flint_general_setup(flint_mech *m, int na, int nt, int nl, const double *ni1, const double *ni2,
+                        const double *eps);
+int flint_source_general(const flint_mech *m, long n, long ld, const double *rho_dev, const double *T_dev,
+                         double *wdot_dev, int *status_dev, void *stream);
+
 /* rhs_native over a batch (Lib_Chemistry_rhs.f90:15-52): Z_dev[ns+1][ld] = (rho_1..rho_ns, T)
  * -> F_dev[ns+1][ld] = (wdot, dT/dt) of the constant-volume reactor; F = -1 where T is outside the
  * thermo table.  With J_dev non-NULL also the analytic Jacobian dF/dZ,

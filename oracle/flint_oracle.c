@@ -52,6 +52,96 @@ int orc_num_threads(void) {
 #undef RPOW
 #undef cabs_
 
+/* x**n for an integer n, by binary powering (the runtime routine a Fortran compiler calls for
+ * real**integer with a run-time exponent) */
+static double powi_r(double a, int b) {
+  const int recip = b < 0;
+  if (recip) b = -b;
+  double r = 1.0;
+  for (;;) {
+    if (b & 1) r *= a;
+    b /= 2;
+    if (b == 0) break;
+    a *= a;
+  }
+  return recip ? 1.0 / r : r;
+}
+
+/* The data-driven general routine, Lib_Chemistry_wdot.f90:96-221: per kind (Arrhenius and
+ * three-body, Troe, Lindemann) a loop over its reactions with the dense (ns+1)-row tables ni1,
+ * ni2, eps ([reaction][ns+1], last row the third body): coM over all species when ni1(M) > 0,
+ * products over the species with a nonzero coefficient (integer powers for Arrhenius, real
+ * powers for fall-off), and omegadot(k) += W_k (ni2 - ni1) net for every species.  scale[k] =
+ * sum over reactions of |W_k (ni2 - ni1)| (|rate_fwd| + |rate_rev|) (may be NULL). */
+void orc_general(const orc_mech *m, int na, int nt, int nl, const double *ni1, const double *ni2,
+                 const double *eps, long n, long ld, const double *rho, const double *T, double *wdot,
+                 double *scale) {
+  const int ns = m->ns, nz = ns + 1;
+#pragma omp parallel for schedule(static)
+  for (long i = 0; i < n; ++i) {
+    double coi[ORC_MAX_NS], om[ORC_MAX_NS], sc[ORC_MAX_NS];
+    for (int k = 0; k < ns; ++k) coi[k] = rho[(long)k * ld + i] / m->W[k];
+    const double temp = T[i];
+    const int T_i = (int)temp;
+    const double Tdiff = temp - T_i;
+    const long row = T_i - m->Tk_min;
+    for (int k = 0; k < ns; ++k) om[k] = sc[k] = 0.0;
+    for (int j = 0; j < na + nt + nl; ++j) {
+      const double *a1 = ni1 + (long)j * nz, *a2 = ni2 + (long)j * nz, *e = eps + (long)j * nz;
+      double coM = 1.0;
+      if (a1[ns] > 0.0) {
+        coM = 0.0;
+        for (int k = 0; k < ns; ++k) coM = coM + coi[k] * e[k];
+      }
+      double pf = 1.0, pb = 1.0;
+      for (int k = 0; k < ns; ++k) {
+        if (j < na) {
+          if (a1[k] != 0) pf = pf * powi_r(coi[k], (int)lrint(a1[k]));
+          if (a2[k] != 0) pb = pb * powi_r(coi[k], (int)lrint(a2[k]));
+        } else {
+          if (a1[k] != 0) pf = pf * pow(coi[k], a1[k]);
+          if (a2[k] != 0) pb = pb * pow(coi[k], a2[k]);
+        }
+      }
+      double rf, rb;
+      if (j < na) {
+        rf = interp_r(m->kf, row, m->n_arrh, j, Tdiff) * pf * coM;
+        rb = interp_r(m->kb, row, m->n_arrh, j, Tdiff) * pb * coM;
+      } else {
+        double kinf, k0, kc, k1;
+        if (j < na + nt) {
+          const int tb = j - na;
+          kinf = interp_r(m->troe, row, 4L * m->n_troe, 4L * tb + 0, Tdiff);
+          k0 = interp_r(m->troe, row, 4L * m->n_troe, 4L * tb + 1, Tdiff);
+          kc = interp_r(m->troe, row, 4L * m->n_troe, 4L * tb + 2, Tdiff);
+          const double Fcent = interp_r(m->troe, row, 4L * m->n_troe, 4L * tb + 3, Tdiff);
+          const double Pr = k0 * coM / kinf;
+          k1 = kinf * (Pr / (1.0 + Pr)) * troeF_r(Pr, Fcent);
+        } else {
+          const int tb = j - na - nt;
+          kinf = interp_r(m->lind, row, 3L * m->n_lind, 3L * tb + 0, Tdiff);
+          k0 = interp_r(m->lind, row, 3L * m->n_lind, 3L * tb + 1, Tdiff);
+          kc = interp_r(m->lind, row, 3L * m->n_lind, 3L * tb + 2, Tdiff);
+          const double Pr = k0 * coM / kinf;
+          k1 = kinf * (Pr / (1.0 + Pr));
+        }
+        rf = k1 * pf;
+        rb = k1 / kc * pb;
+      }
+      const double net = rf - rb;
+      for (int k = 0; k < ns; ++k) {
+        const double d = a2[k] - a1[k];
+        om[k] = om[k] + m->W[k] * d * net;
+        sc[k] += fabs(m->W[k] * d) * (fabs(rf) + fabs(rb));
+      }
+    }
+    for (int k = 0; k < ns; ++k) {
+      wdot[(long)k * ld + i] = om[k];
+      if (scale) scale[(long)k * ld + i] = sc[k];
+    }
+  }
+}
+
 /* ---- complex instantiation (complex-step differentiation) ---- */
 #define REAL double _Complex
 #define SFX(x) x##_c

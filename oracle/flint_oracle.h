@@ -63,6 +63,12 @@ double *orc_nu_alloc(const orc_mech *m);
 void orc_rhs_cell(const orc_mech *m, int mode, const double *x, double p, double *F);
 void orc_jac_cell(const orc_mech *m, const double *nu, int mode, const double *x, double p, double *F, double *J);
 
+/* FLINT's data-driven general routine (Lib_Chemistry_wdot.f90:96-221) with its dense tables
+ * ni1, ni2, eps [na+nt+nl][ns+1] (Arrhenius and three-body, Troe, Lindemann); scale may be NULL */
+void orc_general(const orc_mech *m, int na, int nt, int nl, const double *ni1, const double *ni2,
+                 const double *eps, long n, long ld, const double *rho, const double *T, double *wdot,
+                 double *scale);
+
 /* conservative <-> primitive variables (oracle_thermo.c): cons = (rho_1..rho_ns, rho u_1..rho u_nd,
  * rho E), prim = (rho_1..rho_ns, u_1..u_nd, p), np = ns + nd + 1 rows, ns <= 64, nd <= 8 */
 void orc_cons2prim(const orc_mech *m, long n, long ld, int np, const double *cons, const double *T, double *prim,

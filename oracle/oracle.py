@@ -96,6 +96,18 @@ class Oracle:
         self.L.orc_wd_hrr(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(rho), _ptr(T), _ptr(w), _ptr(q))
         return w, q
 
+    def general(self, rho, T):
+        """The data-driven general routine with the mechanism's tables: rho [ns, n], T [n] ->
+        (wdot [ns, n], scale [ns, n])."""
+        na, nt, nl, ni1, ni2, eps = self.md.general
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        n = T.size
+        w, sc = np.empty_like(rho), np.empty_like(rho)
+        self.L.orc_general(C.byref(self.m), C.c_int(na), C.c_int(nt), C.c_int(nl), _ptr(ni1), _ptr(ni2), _ptr(eps),
+                           C.c_long(n), C.c_long(n), _ptr(rho), _ptr(T), _ptr(w), _ptr(sc))
+        return w, sc
+
     def cons2prim(self, cons, T):
         """cons [ns+nd+1, n] -> (prim [ns+nd+1, n], T_out [n], iters [n], status [n]) (cons2prim)."""
         cons = np.ascontiguousarray(cons, dtype=np.float64)
@@ -234,6 +246,17 @@ class Reference:
         n = T.size
         w = np.empty_like(rho)
         self.L.ref_wdot(C.c_long(n), C.c_long(n), _ptr(rho), _ptr(T), _ptr(w))
+        return w
+
+    def general(self, rho, T):
+        na, nt, nl, ni1, ni2, eps = self.md.general
+        self._gkeep = (ni1, ni2, eps)
+        self.L.ref_general_setup(C.c_int(na), C.c_int(nt), C.c_int(nl), _ptr(ni1), _ptr(ni2), _ptr(eps))
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        n = T.size
+        w = np.empty_like(rho)
+        self.L.ref_general(C.c_long(n), C.c_long(n), _ptr(rho), _ptr(T), _ptr(w))
         return w
 
     def cons2prim(self, cons, T):

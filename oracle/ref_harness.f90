@@ -140,6 +140,47 @@ contains
     enddo
   end subroutine ref_cea
 
+  ! the tables of the data-driven general routine (Load_Chemistry.f90:55-89): per reaction of each
+  ! kind (Arrhenius and three-body, then Troe, then Lindemann, each in file order) ns+1 values of
+  ! ni1, ni2 and epsch, the last row being the third body M
+  subroutine ref_general_setup(na, nt, nl, ni1, ni2, eps) bind(C, name='ref_general_setup')
+    integer(c_int), value :: na, nt, nl
+    real(c_double), intent(in) :: ni1(ns + 1, *), ni2(ns + 1, *), eps(ns + 1, *)
+    if (allocated(ni1_arrh_tab)) deallocate(ni1_arrh_tab, ni2_arrh_tab, epsch_arrh_tab)
+    if (allocated(ni1_troe_tab)) deallocate(ni1_troe_tab, ni2_troe_tab, epsch_troe_tab)
+    if (allocated(ni1_lind_tab)) deallocate(ni1_lind_tab, ni2_lind_tab, epsch_lind_tab)
+    allocate(ni1_arrh_tab(ns + 1, na), ni2_arrh_tab(ns + 1, na), epsch_arrh_tab(ns + 1, na))
+    ni1_arrh_tab = ni1(:, 1:na)
+    ni2_arrh_tab = ni2(:, 1:na)
+    epsch_arrh_tab = eps(:, 1:na)
+    if (nt > 0) then
+      allocate(ni1_troe_tab(ns + 1, nt), ni2_troe_tab(ns + 1, nt), epsch_troe_tab(ns + 1, nt))
+      ni1_troe_tab = ni1(:, na + 1:na + nt)
+      ni2_troe_tab = ni2(:, na + 1:na + nt)
+      epsch_troe_tab = eps(:, na + 1:na + nt)
+    endif
+    if (nl > 0) then
+      allocate(ni1_lind_tab(ns + 1, nl), ni2_lind_tab(ns + 1, nl), epsch_lind_tab(ns + 1, nl))
+      ni1_lind_tab = ni1(:, na + nt + 1:na + nt + nl)
+      ni2_lind_tab = ni2(:, na + nt + 1:na + nt + nl)
+      epsch_lind_tab = eps(:, na + nt + 1:na + nt + nl)
+    endif
+  end subroutine ref_general_setup
+
+  ! the general routine (Lib_Chemistry_wdot.f90:96-221) over n cells
+  subroutine ref_general(n, ld, rho, T, wdot) bind(C, name='ref_general')
+    integer(c_long), value :: n, ld
+    real(c_double), intent(in) :: rho(ld, *), T(*)
+    real(c_double), intent(out) :: wdot(ld, *)
+    real(8) :: roi(ns), om(ns)
+    integer(c_long) :: i
+    do i = 1, n
+      roi = rho(i, 1:ns)
+      call general(roi, T(i), om)
+      wdot(i, 1:ns) = om
+    enddo
+  end subroutine ref_general
+
   ! cons2prim (Lib_ThermoTransport.f90:130-165) over n cells: cons(ld, np) = (rho_1..rho_ns,
   ! rho u_1..rho u_nd, rho E) -> prim(ld, np) = (rho_1..rho_ns, u_1..u_nd, p), T the Newton start
   subroutine ref_cons2prim(n, ld, np, cons, T, prim) bind(C, name='ref_cons2prim')

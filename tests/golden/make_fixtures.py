@@ -8,6 +8,9 @@
 * cea_<mech>.npz   CEA_solve outputs (HP and UV, oracle/_ref: Lib_CEA_setup.f90 and
                    Lib_CEA_solver.f90 compiled by flang) on seeded states from
                    flint_amd.states.cea_states (seed, start, count stored with them).
+* general_<mech>.npz  outputs of the reference's data-driven general routine (oracle/_ref:
+                   Lib_Chemistry_wdot.f90 compiled by flang, tables from the mechanism descriptor,
+                   identical to the shipped chemistry-info.txt) on seeded radical_states.
 * cons2prim_<mech>.npz  prim2cons and cons2prim outputs of the reference (oracle/_ref:
                    Lib_ThermoTransport.f90 compiled by flang) on seeded cells from
                    flint_amd.states.flow_states (seed, start, count, nd stored with them).
@@ -91,6 +94,18 @@ def cea():
         print(mech, "cea", count)
 
 
+GENERAL_CASES = {"TSR-GP-24": (17, 0, 1024), "TSR-Rich-31": (18, 0, 1024)}
+
+
+def general():
+    for mech, (seed, start, count) in GENERAL_CASES.items():
+        md = mech_data(mech)
+        st = S.radical_states(md.species, md.W, seed, start, count, ignition_half=True)
+        w = O.Reference(md).general(S.soa(st["rho"]), st["T"])
+        np.savez_compressed(os.path.join(OUT, f"general_{mech}.npz"), seed=seed, start=start, count=count, wdot=w)
+        print(mech, "general", count)
+
+
 C2P_CASES = {"TSR-GP-24": (15, 0, 512, 3), "TSR-Rich-31": (16, 0, 512, 2)}
 
 
@@ -107,6 +122,6 @@ def cons2prim():
 
 
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["tables", "wdot", "cea", "cons2prim"]
+    which = sys.argv[1:] or ["tables", "wdot", "cea", "cons2prim", "general"]
     for w in which:
         globals()[w]()
