@@ -174,11 +174,41 @@ extern "C" int flint_mech_load(const char* folder, const char* prefix, flint_mec
   int nrc = std::atoi(line.substr(line.find('=') + 1).c_str());
   for (int j = 0; j < 4; ++j) std::getline(ci, line);
   int na = 0, nt = 0, nl = 0;
+  std::vector<int> kind(nrc);
   for (int j = 0; j < nrc; ++j) {
     std::getline(ci, line);
-    if (line.find("Troe") != std::string::npos) ++nt;
-    else if (line.find("Lindemann") != std::string::npos) ++nl;
-    else ++na;
+    if (line.find("Troe") != std::string::npos) kind[j] = 1, ++nt;
+    else if (line.find("Lindemann") != std::string::npos) kind[j] = 2, ++nl;
+    else kind[j] = 0, ++na;
+  }
+  // Reaction definition block of the data-driven general routine (Load_Chemistry.f90:70-89): for
+  // every reaction in file order, ns+1 lines "j name ni1 ni2 epsch", stored per kind; optional
+  const int nz = ns + 1;
+  std::vector<double> g1((size_t)nrc * nz), g2((size_t)nrc * nz), ge((size_t)nrc * nz);
+  bool general = false;
+  if (std::getline(ci, line) && std::getline(ci, line) && line.find("Reaction definition") != std::string::npos) {
+    std::vector<int> slot(nrc);
+    int next = 0;
+    for (int q = 0; q < 3; ++q)
+      for (int j = 0; j < nrc; ++j)
+        if (kind[j] == q) slot[j] = next++;
+    general = true;
+    for (int j = 0; j < nrc && general; ++j)
+      for (int k = 0; k < nz && general; ++k) {
+        if (!std::getline(ci, line)) {
+          general = false;
+          break;
+        }
+        std::istringstream is(line);
+        int idx;
+        std::string nm;
+        double a, b, e;
+        if (!(is >> idx >> nm >> a >> b >> e) || idx != j + 1) general = false;
+        const size_t o = (size_t)slot[j] * nz + k;
+        g1[o] = a;
+        g2[o] = b;
+        ge[o] = e;
+      }
   }
   Zones ar;
   if (int rc = read_tecplot(base + "chemistry-Arrhenius.dat", 2, ar))
@@ -217,7 +247,15 @@ extern "C" int flint_mech_load(const char* folder, const char* prefix, flint_mec
   int rc = flint_mech_create(mech_name.c_str(), ns, W.data(), Tk_min, Tk_max, na, kf.data(), kb.data(), nt,
                              nt ? troe.data() : nullptr, nl, nl ? lind.data() : nullptr, Th_min, Th_max, cp.data(),
                              h.data(), ne, ne ? comp.data() : nullptr, out);
-  if (rc || ne == 0) return rc;
+  if (rc) return rc;
+  if (general && !(*out)->is_wd) {
+    if (int rc2 = flint_general_setup(*out, na, nt, nl, g1.data(), g2.data(), ge.data())) {
+      flint_mech_destroy(*out);
+      *out = nullptr;
+      return rc2;
+    }
+  }
+  if (ne == 0) return FLINT_OK;
   // equilibrium data (CEA_initialize_global), where the mechanism has an equilibrium kernel and
   // every element has a standard atomic weight
   std::vector<double> elw;

@@ -230,8 +230,8 @@ def write_native(name, folder, prefix="", fmt="%.16E"):
 
     The reference ships such folders with 13 significant digits; the default ``fmt`` prints 17
     so that a folder written here loads into exactly the tables generated in memory.  The
-    chemistry-info.txt "Reaction definition" block of the data-driven ``general`` kernel is not
-    written (the compiled mechanisms do not read it)."""
+    chemistry-info.txt carries the Reaction definition block of the data-driven ``general``
+    routine (Load_Chemistry.f90:55-89), from :func:`flint_amd.mechanism.general_tables`."""
     desc = load_descriptor(name)
     Tmin, Tmax = desc["T_range"]
     os.makedirs(folder, exist_ok=True)
@@ -250,11 +250,24 @@ def write_native(name, folder, prefix="", fmt="%.16E"):
             f.write(" ".join("%d" % round(v) for v in comp[:, k]) + "\n")
     rx = desc["reactions"]
     with open(path("chemistry-info.txt"), "w") as f:
-        f.write("%s\nN.ro species = %4d\nN.ro reactions = %4d\n\n" % (desc["name"], len(sp), len(rx)))
+        f.write("%s\nN.ro species = %d\nN.ro reactions = %d\n\n" % (desc["name"], len(sp), len(rx)))
         f.write("General loop info. They are not used if the mechanism is exlplicitly defined\n\n")
         f.write("Reaction type\n")
         for j, r in enumerate(rx):
             f.write("%d %s\n" % (j + 1, _KIND[r["type"]]))
+        f.write("\nReaction definition\n")
+        from .mechanism import general_tables  # mechanism imports this module
+        na, nt, nl, ni1, ni2, eps = general_tables(desc)
+        kinds = [0 if r["type"] in ("arrhenius", "three-body") else (1 if r["type"] == "troe" else 2) for r in rx]
+        row_of = {}
+        for q in (0, 1, 2):
+            for j in [j for j in range(len(rx)) if kinds[j] == q]:
+                row_of[j] = len(row_of)
+        names = list(sp) + ["M"]
+        for j in range(len(rx)):
+            a = row_of[j]
+            for k, nm in enumerate(names):
+                f.write("%d %s %r %r %r\n" % (j + 1, nm, float(ni1[a, k]), float(ni2[a, k]), float(eps[a, k])))
     th = thermo_tables(desc, Tmin, Tmax)
     T = th["T"]
     write_tecplot(path("thermo.dat"), "Mass Thermodynamic Properties",

@@ -59,6 +59,14 @@ def test_loaded_folder_matches_in_memory_tables(native, cuda):
     print("[gpu] native folder vs in-memory tables", flush=True)
     for a, b in zip(*out):
         assert _eq(a, b)
+    # the general routine's tables come with the folder (chemistry-info.txt's Reaction definition)
+    mm.general_setup(md.general)
+    rho_d, T_d = dev(S.soa(st["rho"])), dev(st["T"])
+    wg = [torch.empty_like(rho_d) for _ in range(2)]
+    ml.source_general(rho_d, T_d, wg[0])
+    mm.source_general(rho_d, T_d, wg[1])
+    torch.cuda.synchronize()
+    assert torch.equal(wg[0], wg[1])
     cs = S.radical_states(md.species, md.W, 92, 0, 512, T_lo=600.0, T_hi=900.0)
     r1 = ml.equilibrium_host(cs["T"], S.soa(cs["Y"]), p=cs["p"] / 1e5, hp=True)
     r2 = mm.equilibrium_host(cs["T"], S.soa(cs["Y"]), p=cs["p"] / 1e5, hp=True)

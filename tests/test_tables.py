@@ -1,5 +1,8 @@
 """Table generator pinned against the tables FLINT ships (tests/golden/tables_ref.npz)."""
+import os
+
 import numpy as np
+import pytest
 
 from conftest import golden
 from flint_amd import tables as TB
@@ -87,3 +90,17 @@ def test_write_native_loads_back_exactly(tmp_path):
     ne = int(comp[0])
     assert comp[1:1 + ne] == list(md.elements) and comp[1 + ne] == "composition"
     assert np.array_equal(np.array(comp[2 + ne:], dtype=float).reshape(md.ns, ne).T, md.composition)
+
+
+@pytest.mark.skipif(not os.path.isdir("/root/reference/test"), reason="reference checkout absent")
+@pytest.mark.parametrize("mech", ["WD", "TSR-GP-24", "TSR-Rich-31"])
+def test_written_chemistry_info_equals_shipped_file(tmp_path, mech):
+    """chemistry-info.txt written from the descriptor (with the general routine's Reaction
+    definition block) is byte-identical to the file the reference ships."""
+    from flint_amd.tables import write_native
+    folder = write_native(mech, str(tmp_path / mech))
+    with open(os.path.join(folder, "chemistry-info.txt")) as f:
+        mine = f.read()
+    with open(f"/root/reference/test/{mech}/INPUT/chemistry-info.txt") as f:
+        ref = f.read()
+    assert mine == ref
