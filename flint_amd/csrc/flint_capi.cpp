@@ -76,11 +76,11 @@ extern "C" int flint_mech_create(const char* mech_name, int ns, const double* W,
   m->name = mech_name;
   m->id = flint_mech_id(mech_name);
   m->is_wd = (m->id == "wd");
+  // A mechanism without compiled kernels is kept (FLINT's Assign_Mechanism falls back to the
+  // data-driven general routine for it): its source terms then come from flint_source_general
+  // once flint_general_setup has run, and the entry points that need compiled kernels (RHS,
+  // Jacobian, reactors, equilibrium) report FLINT_EUNSUPPORTED.
   m->ops = m->is_wd ? nullptr : flint_find_mech(m->id.c_str());
-  if (!m->is_wd && !m->ops) {
-    delete m;
-    return flint_fail(FLINT_EMECH, std::string("no compiled kernels for mechanism ") + mech_name);
-  }
   if (m->ops && m->ops->ns != ns) {
     delete m;
     return flint_fail(FLINT_EARG, "species count does not match the compiled mechanism");
@@ -198,6 +198,13 @@ extern "C" int flint_source(const flint_mech* m, long n, long ld, const double* 
   if (m->is_wd) {
     e = flint_wd_launch(n, ld, const_cast<double*>(rho), T, wdot, hrr, status, m->d_W, m->d_tabk, m->Tk_min, m->Tk_max,
                         m->wk, m->d_tabh, m->Th_min, m->Th_max, 0, s);
+  } else if (!m->ops) {
+    // chemistry_source => general (Lib_Chemistry_wdot.f90:86-89)
+    if (hrr) return flint_fail(FLINT_EUNSUPPORTED, "heat release output is provided for WD only");
+    if (!m->gen)
+      return flint_fail(FLINT_EMECH, "no compiled kernels for mechanism " + m->name +
+                                         " and no general tables (flint_general_setup)");
+    return flint_source_general(m, n, ld, rho, T, wdot, status, stream);
   } else {
     if (hrr) return flint_fail(FLINT_EUNSUPPORTED, "heat release output is provided for WD only");
     e = m->ops->wdot(n, ld, rho, T, wdot, status, m->d_tabk, m->Tk_min, m->Tk_max, s);
@@ -212,6 +219,7 @@ extern "C" int flint_rhs(const flint_mech* m, long n, long ld, const double* Z, 
   if (n == 0) return FLINT_OK;
   if (!Z || !F) return flint_fail(FLINT_EARG, "null array");
   if (m->is_wd) return flint_fail(FLINT_EUNSUPPORTED, "rhs/Jacobian kernels are generated for YTF mechanisms only");
+  if (!m->ops) return flint_fail(FLINT_EUNSUPPORTED, "no compiled rhs/Jacobian kernel for mechanism " + m->name);
   hipError_t e = m->ops->rhs_jac(n, ld, Z, F, J, status, m->d_tabk, m->Tk_min, m->Tk_max, m->d_tabh, m->Th_min,
                                  m->Th_max, (hipStream_t)stream);
   if (e != hipSuccess) return flint_fail(FLINT_EHIP, hipGetErrorString(e));

@@ -181,7 +181,7 @@ def general_tables(desc):
             ni2[j, int(k)] = v
         if r["type"] != "arrhenius":
             ni1[j, ns] = ni2[j, ns] = 1.0
-            eps[j, :ns] = r.get("eff_ytf", r["eff"])
+            eps[j, :ns] = r.get("eff_ytf") or r["eff"]
     na = sum(r["type"] in ("arrhenius", "three-body") for r in rows)
     nt = sum(r["type"] == "troe" for r in rows)
     return na, nt, len(rows) - na - nt, ni1, ni2, eps

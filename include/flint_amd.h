@@ -38,7 +38,8 @@ const char *flint_last_error(void);
  * Replaces read_idealgas_thermo (Load_ThermoTransport.f90:7-89), read_chemistry
  * (Load_Chemistry.f90:7-164) and Assign_Mechanism (Lib_Chemistry_wdot.f90:24-92) for callers that
  * hold the module arrays already.
- *   mech_name     FLINT mechanism name ("WD", "TSR-GP-24", "TSR-Rich-31")
+ *   mech_name     FLINT mechanism name: "WD", "TSR-GP-24" and "TSR-Rich-31" have compiled kernels;
+ *                 any other name gives a mechanism served by the general routine (flint_general_setup)
  *   W[ns]         molecular weights (wm_tab)
  *   kf, kb        kf_tab/kb_tab(Tk_min:Tk_max, n_arrh), column-major (T fastest)
  *   troe          [4][n_troe][nTk]: kinf_troe_tab, k0_troe_tab, kc_troe_tab, Fcent_tab (may be NULL if 0)
@@ -76,7 +77,10 @@ int flint_source(const flint_mech *m, long n, long ld, const double *rho_dev, co
  * reaction of each kind (Arrhenius and three-body, then Troe, then Lindemann, each in file order,
  * na/nt/nl matching the mechanism's rate tables): ni1, ni2, eps [na+nt+nl][ns+1] (host), the
  * last entry of a row being the third body M.  flint_source_general then evaluates it over a batch
- * as flint_source does (status bit 0: T outside the kinetics table, the row clamped). */
+ * as flint_source does (status bit 0: T outside the kinetics table, the row clamped).  For a
+ * mechanism created without compiled kernels flint_source itself runs general, as FLINT's
+ * Assign_Mechanism falls back to it (Lib_Chemistry_wdot.f90:86-89); flint_mech_load sets the
+ * tables up from chemistry-info.txt's Reaction definition block when the folder has one. */
 int flint_general_setup(flint_mech *m, int na, int nt, int nl, const double *ni1, const double *ni2,
                         const double *eps);
 int flint_source_general(const flint_mech *m, long n, long ld, const double *rho_dev, const double *T_dev,

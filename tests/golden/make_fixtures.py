@@ -104,6 +104,14 @@ def general():
         w = O.Reference(md).general(S.soa(st["rho"]), st["T"])
         np.savez_compressed(os.path.join(OUT, f"general_{mech}.npz"), seed=seed, start=start, count=count, wdot=w)
         print(mech, "general", count)
+    # Pelucchi (Arrhenius, Troe and Lindemann; no compiled kernels): inputs stored with the outputs
+    md = mech_data("Pelucchi")
+    n = 512
+    T = 300.0 + 2600.0 * (np.arange(n) % 97) / 97.0
+    rho = 10.0 ** np.random.default_rng(19).uniform(-9, -1, (md.ns, n))
+    w = O.Reference(md).general(rho, T)
+    np.savez_compressed(os.path.join(OUT, "general_Pelucchi.npz"), rho=rho, T=T, wdot=w)
+    print("Pelucchi general", n)
 
 
 C2P_CASES = {"TSR-GP-24": (15, 0, 512, 3), "TSR-Rich-31": (16, 0, 512, 2)}

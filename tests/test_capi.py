@@ -47,9 +47,9 @@ def test_create_rejects_bad_sizes():
     rc = L.flint_mech_create(b"WD", 0, p(W), 1, 2, 3, p(t), p(t), 0, None, 0, None, 1, 2, p(t), p(t), 0, None,
                              C.byref(out))
     assert rc == 1
-    rc = L.flint_mech_create(b"no-such-mechanism", 5, p(W), 1, 2, 3, p(t), p(t), 0, None, 0, None, 1, 2, p(t), p(t), 0,
+    rc = L.flint_mech_create(b"no-such-mechanism", 5, p(W), 2, 1, 3, p(t), p(t), 0, None, 0, None, 1, 2, p(t), p(t), 0,
                              None, C.byref(out))
-    assert rc == 4
+    assert rc == 1  # Tk_max <= Tk_min
 
 
 def test_fortran_drop_in_reports_missing_folder(tmp_path):

@@ -33,9 +33,10 @@ def _reaction_definition(path, ns):
 
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="reference checkout absent")
-@pytest.mark.parametrize("mech", ["TSR-GP-24", "TSR-Rich-31"])
+@pytest.mark.parametrize("mech", ["TSR-GP-24", "TSR-Rich-31", "Pelucchi"])
 def test_general_tables_match_shipped_chemistry_info(mechs, mech):
-    md = mechs[mech]
+    from flint_amd.mechdata import mech_data
+    md = mech_data(mech)
     na, nt, nl, ni1, ni2, eps = md.general
     types, a = _reaction_definition(os.path.join(REF, mech, "INPUT", "chemistry-info.txt"), md.ns)
     kind = np.array([2 if "Lindemann" in t else (1 if "Troe" in t else 0) for t in types])
@@ -58,11 +59,25 @@ def test_general_oracle_matches_reference_outputs(oracle_mod, mechs, mech):
     assert 1e-12 < d < 1e-6
 
 
+def test_general_oracle_matches_reference_outputs_pelucchi(oracle_mod):
+    from flint_amd.mechdata import mech_data
+    g = golden("general_Pelucchi.npz")
+    w, _ = oracle_mod.Oracle(mech_data("Pelucchi")).general(g["rho"], g["T"])
+    assert np.array_equal(w, g["wdot"])
+
+
 @pytest.mark.skipif(not os.path.exists(REF_LIB), reason="reference build (make -C oracle ref) absent")
 def test_general_oracle_equals_reference_build(oracle_mod, mechs):
+    from flint_amd.mechdata import mech_data
     for mech in ("TSR-GP-24", "TSR-Rich-31"):
         md = mechs[mech]
         st = S.radical_states(md.species, md.W, 400, 0, 3000, T_lo=600.0, T_hi=4000.0, ignition_half=True)
         rho = S.soa(st["rho"])
         w, _ = oracle_mod.Oracle(md).general(rho, st["T"])
         assert np.array_equal(w, oracle_mod.Reference(md).general(rho, st["T"]))
+    # Pelucchi: Arrhenius, Troe and Lindemann reactions, no compiled kernels here
+    md = mech_data("Pelucchi")
+    T = 300.0 + 2600.0 * (np.arange(2048) % 997) / 997.0
+    rho = 10.0 ** np.random.default_rng(5).uniform(-9, -1, (md.ns, 2048))
+    w, _ = oracle_mod.Oracle(md).general(rho, T)
+    assert np.all(np.isfinite(w)) and np.array_equal(w, oracle_mod.Reference(md).general(rho, T))

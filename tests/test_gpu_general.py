@@ -4,6 +4,9 @@
   <= 1e-12 normwise (FLINT's metric: the scale is the sum of |W_k (ni2 - ni1)| (|rate_fwd| +
   |rate_rev|) over reactions); only log10 and 10** of the Troe factor differ from the host's in
   the last bit;
+* a mechanism without compiled kernels (Pelucchi: Arrhenius, Troe and Lindemann reactions) gets
+  its source terms from general, as FLINT's Assign_Mechanism falls back to it, and reports the
+  entry points that need compiled kernels as unsupported;
 * out-of-table temperatures set status bit 0, padded leading dimensions leave the padding
   untouched, bad tables are rejected."""
 import numpy as np
@@ -70,3 +73,35 @@ def test_general_edges(mechs, oracle_mod, cuda):
     ok = np.arange(n) != 7
     wo, sc = oracle_mod.Oracle(md).general(rho[:, :n][:, ok], T[ok])
     assert np.max(np.abs(wn[:, :n][:, ok] - wo) / np.maximum(sc, 1e-300)) <= 1e-12
+
+
+def test_mechanism_without_compiled_kernels_uses_general(oracle_mod, cuda):
+    import torch
+    from flint_amd.mechdata import mech_data
+    from flint_amd.native import Mechanism, FlintError
+    md = mech_data("Pelucchi")
+    assert md.n_lind > 0 and md.n_troe > 0
+    m = Mechanism.from_data(md)
+    n = 2048
+    T = 300.0 + 2600.0 * (np.arange(n) % 997) / 997.0
+    rng = np.random.default_rng(5)
+    rho = 10.0 ** rng.uniform(-9, -1, (md.ns, n))
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    w = torch.empty((md.ns, n), dtype=torch.float64, device=cuda)
+    with pytest.raises(FlintError):
+        m.source(dev(rho), dev(T), w)  # no compiled routine and no general tables yet
+    with pytest.raises(FlintError):
+        m.rhs(dev(np.vstack([rho, T[None]])), torch.empty((md.ns + 1, n), dtype=torch.float64, device=cuda))
+    m.general_setup(md.general)
+    print(f"[gpu] Pelucchi through general n={n}", flush=True)
+    m.source(dev(rho), dev(T), w)
+    torch.cuda.synchronize()
+    wo, sc = oracle_mod.Oracle(md).general(rho, T)
+    assert np.max(np.abs(w.cpu().numpy() - wo) / np.maximum(sc, 1e-300)) <= 1e-12
+    # against the reference's own outputs (tests/golden/general_Pelucchi.npz)
+    g = golden("general_Pelucchi.npz")
+    wg = torch.empty((md.ns, g["T"].size), dtype=torch.float64, device=cuda)
+    m.source(dev(g["rho"]), dev(g["T"]), wg)
+    torch.cuda.synchronize()
+    _, sg = oracle_mod.Oracle(md).general(g["rho"], g["T"])
+    assert np.max(np.abs(wg.cpu().numpy() - g["wdot"]) / np.maximum(sg, 1e-300)) <= 1e-12

@@ -93,7 +93,7 @@ def test_write_native_loads_back_exactly(tmp_path):
 
 
 @pytest.mark.skipif(not os.path.isdir("/root/reference/test"), reason="reference checkout absent")
-@pytest.mark.parametrize("mech", ["WD", "TSR-GP-24", "TSR-Rich-31"])
+@pytest.mark.parametrize("mech", ["WD", "TSR-GP-24", "TSR-Rich-31", "Pelucchi"])
 def test_written_chemistry_info_equals_shipped_file(tmp_path, mech):
     """chemistry-info.txt written from the descriptor (with the general routine's Reaction
     definition block) is byte-identical to the file the reference ships."""
