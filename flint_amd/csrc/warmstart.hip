@@ -213,19 +213,30 @@ __global__ void zero_kernel(int* count) {
   if (threadIdx.x == 0 && blockIdx.x == 0) count[0] = 0;
 }
 
-// one atomic per wavefront: the first flagged lane reserves the wavefront's slots
-__global__ __launch_bounds__(256) void collect_kernel(long n, const int* __restrict__ status, int* __restrict__ idx,
-                                                      int* __restrict__ count) {
+// one atomic per block of 1024 states: every state of a batch may be flagged, and atomics on
+// one address serialise (one per wavefront still took 48 us for 2^18 states)
+constexpr int COLLECT_BLOCK = 1024;
+__global__ __launch_bounds__(COLLECT_BLOCK) void collect_kernel(long n, const int* __restrict__ status,
+                                                                int* __restrict__ idx, int* __restrict__ count) {
+  __shared__ int wbase[COLLECT_BLOCK / 64];
+  __shared__ int bbase;
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bool f = c < n && status[c] != 0;
   const unsigned long long mask = __ballot(f);
-  if (mask == 0) return;
-  const int leader = __ffsll((long long)mask) - 1;
-  int base = 0;
-  if (lane == leader) base = atomicAdd(count, __popcll(mask));
-  base = __shfl(base, leader, 64);
-  if (f) idx[base + __popcll(mask & ((1ull << lane) - 1))] = (int)c;
+  if (lane == 0) wbase[w] = __popcll(mask);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+      const int t = wbase[i];
+      wbase[i] = tot;
+      tot += t;
+    }
+    bbase = tot ? atomicAdd(count, tot) : 0;
+  }
+  __syncthreads();
+  if (f) idx[bbase + wbase[w] + __popcll(mask & ((1ull << lane) - 1))] = (int)c;
 }
 
 }  // namespace flint_ws_k
@@ -248,8 +259,9 @@ hipError_t flint_ws_guess(long n, long ld, int ns, const float* O, const double*
 hipError_t flint_ws_collect(long n, const int* status, int* idx, int* count, hipStream_t s) {
   hipLaunchKernelGGL(flint_ws_k::zero_kernel, dim3(1), dim3(64), 0, s, count);
   if (n > 0)
-    hipLaunchKernelGGL(flint_ws_k::collect_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, n, status, idx,
-                       count);
+    hipLaunchKernelGGL(flint_ws_k::collect_kernel, dim3((unsigned)((n + flint_ws_k::COLLECT_BLOCK - 1) /
+                                                               flint_ws_k::COLLECT_BLOCK)),
+                       dim3(flint_ws_k::COLLECT_BLOCK), 0, s, n, status, idx, count);
   return hipGetLastError();
 }
 
