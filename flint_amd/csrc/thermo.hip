@@ -98,8 +98,9 @@ __global__ __launch_bounds__(256) void cons2prim_kernel(Tab t, const double* __r
   const double* __restrict__ c = cons + i;
   double rho, Rgas;
   rho_R(sRi, ns, c, ld, rho, Rgas);
-  for (int s = 0; s < ns; ++s) prim[(long)s * ld + i] = c[(long)s * ld];
-  for (int d = 0; d < nd; ++d) prim[(long)(ns + d) * ld + i] = c[(long)(ns + d) * ld] / rho;
+  // outputs bypass L2 (streaming stores), which keeps the table rows the Newton loop gathers there
+  for (int s = 0; s < ns; ++s) stream_store(prim + (long)s * ld + i, c[(long)s * ld]);
+  for (int d = 0; d < nd; ++d) stream_store(prim + (long)(ns + d) * ld + i, c[(long)(ns + d) * ld] / rho);
   const double vel = norm2(c + (long)ns * ld, ld, nd, rho, true);
   const double energy = c[(long)(np - 1) * ld] / rho - 0.5 * vel * vel;
   // Newton on T (Lib_ThermoTransport.f90:153-161).  f_cp reads cp at T and E reads h at
@@ -141,8 +142,8 @@ __global__ __launch_bounds__(256) void cons2prim_kernel(Tab t, const double* __r
     T = T + (energy - e_it) / cv;
     it = it + 1;
   }
-  prim[(long)(np - 1) * ld + i] = st ? __builtin_nan("") : rho * Rgas * T;
-  if (Tout) Tout[i] = st ? __builtin_nan("") : T;
+  stream_store(prim + (long)(np - 1) * ld + i, st ? __builtin_nan("") : rho * Rgas * T);
+  if (Tout) stream_store(Tout + i, st ? __builtin_nan("") : T);
   if (iters) iters[i] = it - 1;
   if (status) status[i] = st;
 }
