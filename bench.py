@@ -212,6 +212,21 @@ def omega_leg(args, dev, stream):
     ms = timed(stream, lambda: mech.rhs(Zd, F, J, stream=stream), 20)
     bpc = nz * 8 + (nz + nz * nz) * 8
     res = dict(n=n, ms=ms, cells_per_s=n / (ms * 1e-3), bytes_per_cell=bpc, achieved_GBps=bpc * n / (ms * 1e-3) / 1e9)
+    # the same kernel for TSR-Rich-31 (32 x 32 Jacobian), 2^18 cells
+    mr = mech_data("TSR-Rich-31")
+    mech_r = Mechanism.from_data(mr)
+    nr_, nzr = 1 << 18, mr.ns + 1
+    sr = S.radical_states(mr.species, mr.W, 2025, 0, nr_)
+    Zr = torch.from_numpy(np.vstack([S.soa(sr["rho"]), sr["T"][None, :]])).to(dev)
+    Fr = torch.empty((nzr, nr_), dtype=torch.float64, device=dev)
+    Jr = torch.empty((nzr * nzr, nr_), dtype=torch.float64, device=dev)
+    for _ in range(3):
+        mech_r.rhs(Zr, Fr, Jr, stream=stream)
+    ms_r = timed(stream, lambda: mech_r.rhs(Zr, Fr, Jr, stream=stream), 10)
+    bpr = nzr * 8 + (nzr + nzr * nzr) * 8
+    res["rich31"] = dict(n=nr_, ms=ms_r, cells_per_s=nr_ / (ms_r * 1e-3), bytes_per_cell=bpr,
+                         achieved_GBps=bpr * nr_ / (ms_r * 1e-3) / 1e9)
+    del Zr, Fr, Jr
     p = os.path.join(ROOT, "profiles", "rhs_jac_tsr_gp_24_1M.traffic.json")
     if os.path.exists(p):
         with open(p) as f:
