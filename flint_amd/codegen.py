@@ -60,7 +60,7 @@ def _dprod_expr(fac, j):
 
 # Launch shape of the source-term/Jacobian kernel: threads per block, the waves per SIMD the
 # compiler is held to (0: its choice) and how many per-cell scratch values live in LDS
-JAC_OPTS = dict(block=128, waves=0, lds_slots=1 << 30)
+JAC_OPTS = dict(block=128, waves=0, lds_slots=1 << 30, cache_slots=80)
 
 
 def emit(prog):
@@ -92,6 +92,20 @@ def emit(prog):
     keep = list(slot)[:JAC_OPTS["lds_slots"]]
     reg = [k for k in slot if k not in keep]
     slot = {k: i for i, k in enumerate(keep)}
+    # up to cache_slots in all: the interpolated k_f, k_b of the Arrhenius/three-body reactions
+    # that enter the most Jacobian columns, kept from the rates pass so that those columns skip
+    # the reaction's two table-row gathers
+    def columns(r):
+        return len({k for k, _ in r.fwd} | ({k for k, _ in r.rev} if r.reversible else set()))
+    cands = sorted((r for r in prog.reactions if r.kind in ("arrhenius", "three-body") and parts[r.index]),
+                   key=lambda r: -columns(r))
+    cached = set()
+    for r in cands:
+        if len(slot) + 2 > JAC_OPTS["cache_slots"]:
+            break
+        slot[("C1", r.index)] = len(slot)
+        slot[("C2", r.index)] = len(slot)
+        cached.add(r.index)
     nslot = len(slot)
 
     def S(kind, i):
@@ -176,6 +190,8 @@ def emit(prog):
         if r.kind in ("arrhenius", "three-body"):
             a(f"    const double2 a0 = ld2(rk0 + {2 * r.table}), a1 = ld2(rk1 + {2 * r.table});")
             a("    const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;")
+            if r.index in cached:
+                a(f"    if (JAC) {{ {S('C1', r.index)} = kf; {S('C2', r.index)} = kb; }}")
             mf = f"*M{r.m_set}" if r.kind == "three-body" else ""
             a(f"    const double pf = kf{''.join('*' + p for p in pfac)}{mf};")
             if r.reversible:
@@ -304,7 +320,14 @@ def emit(prog):
                 continue
             terms = []
             if fd is not None or bd is not None:
-                if r.kind in ("arrhenius", "three-body"):
+                if r.kind in ("arrhenius", "three-body") and r.index in cached:
+                    body.append("    {")
+                    mf = f" * M{r.m_set}" if r.kind == "three-body" else ""
+                    if fd is not None:
+                        terms.append(f"{S('C1', r.index)}{mf} * {fd}")
+                    if bd is not None:
+                        terms.append(f"-{S('C2', r.index)}{mf} * {bd}")
+                elif r.kind in ("arrhenius", "three-body"):
                     body.append(f"    {{ const double2 a0 = ld2(rk0 + {2 * r.table}), a1 = ld2(rk1 + {2 * r.table});")
                     mf = f" * M{r.m_set}" if r.kind == "three-body" else ""
                     if fd is not None:

@@ -2,7 +2,7 @@
 // 24 species, 110 reactions (102 Arrhenius/three-body, 8 Troe, 0 Lindemann).
 #include "../flint_device.h"
 namespace flint_gen_tsr_gp_24 {
-constexpr int NS = 24, NZ = 25, WK = 236, WH = 48, NSLOT = 41;
+constexpr int NS = 24, NZ = 25, WK = 236, WH = 48, NSLOT = 79;
 
 // Rates, net production and (JAC) the analytic Jacobian of the constant-volume RHS for one cell.
 // r[k] = rho_k, T; outputs F[0..NS-1] = wdot, F[NS] = dT/dt (rhs_native); sink(row, col, v).
@@ -182,6 +182,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 3: H2 + O <=> H + OH
     const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[43 * SST] = kf; scr[44 * SST] = kb; }
     const double pf = kf*(c4)*(c9);
     const double pb = kb*(c8)*(c3);
     const double q = pf - pb;
@@ -201,6 +202,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 4: HO2 + O <=> O2 + OH
     const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[45 * SST] = kf; scr[46 * SST] = kb; }
     const double pf = kf*(c0)*(c9);
     const double pb = kb*(c18)*(c3);
     const double q = pf - pb;
@@ -220,6 +222,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 5: H2O2 + O <=> HO2 + OH
     const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[47 * SST] = kf; scr[48 * SST] = kb; }
     const double pf = kf*(c1)*(c9);
     const double pb = kb*(c0)*(c3);
     const double q = pf - pb;
@@ -239,6 +242,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 6: CH2 + O <=> H + HCO
     const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[49 * SST] = kf; scr[50 * SST] = kb; }
     const double pf = kf*(c19)*(c9);
     const double pb = kb*(c8)*(c5);
     const double q = pf - pb;
@@ -258,6 +262,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 7: CH2(S) + O <=> CO + H2
     const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[51 * SST] = kf; scr[52 * SST] = kb; }
     const double pf = kf*(c7)*(c9);
     const double pb = kb*(c2)*(c4);
     const double q = pf - pb;
@@ -277,6 +282,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 8: CH2(S) + O <=> H + HCO
     const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[53 * SST] = kf; scr[54 * SST] = kb; }
     const double pf = kf*(c7)*(c9);
     const double pb = kb*(c8)*(c5);
     const double q = pf - pb;
@@ -296,6 +302,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 9: CH3 + O <=> CH2O + H
     const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[55 * SST] = kf; scr[56 * SST] = kb; }
     const double pf = kf*(c11)*(c9);
     const double pb = kb*(c13)*(c8);
     const double q = pf - pb;
@@ -315,6 +322,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 10: CH4 + O <=> CH3 + OH
     const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[57 * SST] = kf; scr[58 * SST] = kb; }
     const double pf = kf*(c21)*(c9);
     const double pb = kb*(c11)*(c3);
     const double q = pf - pb;
@@ -352,6 +360,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 12: HCO + O <=> CO + OH
     const double2 a0 = ld2(rk0 + 22), a1 = ld2(rk1 + 22);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[59 * SST] = kf; scr[60 * SST] = kb; }
     const double pf = kf*(c5)*(c9);
     const double pb = kb*(c2)*(c3);
     const double q = pf - pb;
@@ -371,6 +380,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 13: HCO + O <=> CO2 + H
     const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[61 * SST] = kf; scr[62 * SST] = kb; }
     const double pf = kf*(c5)*(c9);
     const double pb = kb*(c23)*(c8);
     const double q = pf - pb;
@@ -390,6 +400,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 14: CH2O + O <=> HCO + OH
     const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[63 * SST] = kf; scr[64 * SST] = kb; }
     const double pf = kf*(c13)*(c9);
     const double pb = kb*(c5)*(c3);
     const double q = pf - pb;
@@ -409,6 +420,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 15: CH3O + O <=> CH2O + OH
     const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[65 * SST] = kf; scr[66 * SST] = kb; }
     const double pf = kf*(c6)*(c9);
     const double pb = kb*(c13)*(c3);
     const double q = pf - pb;
@@ -428,6 +440,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 16: C2H2 + O <=> C2H + OH
     const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[67 * SST] = kf; scr[68 * SST] = kb; }
     const double pf = kf*(c12)*(c9);
     const double pb = kb*(c15)*(c3);
     const double q = pf - pb;
@@ -447,6 +460,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 17: C2H2 + O <=> CH2 + CO
     const double2 a0 = ld2(rk0 + 32), a1 = ld2(rk1 + 32);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[69 * SST] = kf; scr[70 * SST] = kb; }
     const double pf = kf*(c12)*(c9);
     const double pb = kb*(c19)*(c2);
     const double q = pf - pb;
@@ -466,6 +480,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 18: CO + O2 <=> CO2 + O
     const double2 a0 = ld2(rk0 + 34), a1 = ld2(rk1 + 34);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[71 * SST] = kf; scr[72 * SST] = kb; }
     const double pf = kf*(c2)*(c18);
     const double pb = kb*(c23)*(c9);
     const double q = pf - pb;
@@ -485,6 +500,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 19: CH2O + O2 <=> HCO + HO2
     const double2 a0 = ld2(rk0 + 36), a1 = ld2(rk1 + 36);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[73 * SST] = kf; scr[74 * SST] = kb; }
     const double pf = kf*(c13)*(c18);
     const double pb = kb*(c5)*(c0);
     const double q = pf - pb;
@@ -558,6 +574,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 23: H + O2 <=> O + OH
     const double2 a0 = ld2(rk0 + 44), a1 = ld2(rk1 + 44);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[75 * SST] = kf; scr[76 * SST] = kb; }
     const double pf = kf*(c8)*(c18);
     const double pb = kb*(c9)*(c3);
     const double q = pf - pb;
@@ -659,6 +676,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 29: H + HO2 <=> H2O + O
     const double2 a0 = ld2(rk0 + 56), a1 = ld2(rk1 + 56);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[77 * SST] = kf; scr[78 * SST] = kb; }
     const double pf = kf*(c8)*(c0);
     const double pb = kb*(c16)*(c9);
     const double q = pf - pb;
@@ -1640,6 +1658,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 74: CH2(S) + O2 <=> CO + H + OH
     const double2 a0 = ld2(rk0 + 132), a1 = ld2(rk1 + 132);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[41 * SST] = kf; scr[42 * SST] = kb; }
     const double pf = kf*(c7)*(c18);
     const double pb = kb*(c2)*(c8)*(c3);
     const double q = pf - pb;
@@ -2582,15 +2601,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[45 * SST] * c9;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
+    {
+      const double dq = -scr[48 * SST] * c3;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += 1.0 * dq;
@@ -2602,8 +2621,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 36), a1 = ld2(rk1 + 36);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c5;
+    {
+      const double dq = -scr[74 * SST] * c5;
       jc0 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
@@ -2638,8 +2657,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 56), a1 = ld2(rk1 + 56);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
+    {
+      const double dq = scr[77 * SST] * c8;
       jc0 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -2908,8 +2927,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[47 * SST] * c9;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += 1.0 * dq;
@@ -3139,8 +3158,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
+    {
+      const double dq = -scr[52 * SST] * c4;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -3152,22 +3171,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 22), a1 = ld2(rk1 + 22);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
+    {
+      const double dq = -scr[60 * SST] * c3;
       jc2 += 1.0 * dq;
       jc3 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 32), a1 = ld2(rk1 + 32);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
+    {
+      const double dq = -scr[70 * SST] * c19;
       jc2 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 34), a1 = ld2(rk1 + 34);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
+    {
+      const double dq = scr[71 * SST] * c18;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc18 += -1.0 * dq;
@@ -3266,8 +3285,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc3 += 1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 132), a1 = ld2(rk1 + 132);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8*c3;
+    {
+      const double dq = -scr[42 * SST] * c8*c3;
       jc2 += 1.0 * dq;
       jc3 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -3422,29 +3441,29 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
+    {
+      const double dq = -scr[44 * SST] * c8;
       jc3 += 1.0 * dq;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
+    {
+      const double dq = -scr[46 * SST] * c18;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c0;
+    {
+      const double dq = -scr[48 * SST] * c0;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c11;
+    {
+      const double dq = -scr[58 * SST] * c11;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc11 += 1.0 * dq;
@@ -3456,29 +3475,29 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 22), a1 = ld2(rk1 + 22);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
+    {
+      const double dq = -scr[60 * SST] * c2;
       jc2 += 1.0 * dq;
       jc3 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c5;
+    {
+      const double dq = -scr[64 * SST] * c5;
       jc3 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
+    {
+      const double dq = -scr[66 * SST] * c13;
       jc3 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc9 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c15;
+    {
+      const double dq = -scr[68 * SST] * c15;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc12 += -1.0 * dq;
@@ -3490,8 +3509,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 44), a1 = ld2(rk1 + 44);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
+    {
+      const double dq = -scr[76 * SST] * c9;
       jc3 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -3708,8 +3727,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 132), a1 = ld2(rk1 + 132);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2*c8;
+    {
+      const double dq = -scr[42 * SST] * c2*c8;
       jc2 += 1.0 * dq;
       jc3 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -3836,15 +3855,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[43 * SST] * c9;
       jc3 += 1.0 * dq;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
+    {
+      const double dq = -scr[52 * SST] * c2;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -4111,15 +4130,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
+    {
+      const double dq = -scr[50 * SST] * c8;
       jc5 += 1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
+    {
+      const double dq = -scr[54 * SST] * c8;
       jc5 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc8 += 1.0 * dq;
@@ -4131,29 +4150,29 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 22), a1 = ld2(rk1 + 22);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[59 * SST] * c9;
       jc2 += 1.0 * dq;
       jc3 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[61 * SST] * c9;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
+    {
+      const double dq = -scr[64 * SST] * c3;
       jc3 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 36), a1 = ld2(rk1 + 36);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c0;
+    {
+      const double dq = -scr[74 * SST] * c0;
       jc0 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
@@ -4405,8 +4424,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[65 * SST] * c9;
       jc3 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc9 += -1.0 * dq;
@@ -4638,15 +4657,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[51 * SST] * c9;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[53 * SST] * c9;
       jc5 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc8 += 1.0 * dq;
@@ -4737,8 +4756,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 132), a1 = ld2(rk1 + 132);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
+    {
+      const double dq = scr[41 * SST] * c18;
       jc2 += 1.0 * dq;
       jc3 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -4889,29 +4908,29 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
+    {
+      const double dq = -scr[44 * SST] * c3;
       jc3 += 1.0 * dq;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c5;
+    {
+      const double dq = -scr[50 * SST] * c5;
       jc5 += 1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c5;
+    {
+      const double dq = -scr[54 * SST] * c5;
       jc5 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
+    {
+      const double dq = -scr[56 * SST] * c13;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc11 += -1.0 * dq;
@@ -4923,8 +4942,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c23;
+    {
+      const double dq = -scr[62 * SST] * c23;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
@@ -4948,8 +4967,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 44), a1 = ld2(rk1 + 44);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
+    {
+      const double dq = scr[75 * SST] * c18;
       jc3 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -4981,8 +5000,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 56), a1 = ld2(rk1 + 56);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
+    {
+      const double dq = scr[77 * SST] * c0;
       jc0 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -5133,8 +5152,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 132), a1 = ld2(rk1 + 132);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2*c3;
+    {
+      const double dq = -scr[42 * SST] * c2*c3;
       jc2 += 1.0 * dq;
       jc3 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -5261,57 +5280,57 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c4;
+    {
+      const double dq = scr[43 * SST] * c4;
       jc3 += 1.0 * dq;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
+    {
+      const double dq = scr[45 * SST] * c0;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c1;
+    {
+      const double dq = scr[47 * SST] * c1;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
+    {
+      const double dq = scr[49 * SST] * c19;
       jc5 += 1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[51 * SST] * c7;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[53 * SST] * c7;
       jc5 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
+    {
+      const double dq = scr[55 * SST] * c11;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c21;
+    {
+      const double dq = scr[57 * SST] * c21;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc11 += 1.0 * dq;
@@ -5323,50 +5342,50 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 22), a1 = ld2(rk1 + 22);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c5;
+    {
+      const double dq = scr[59 * SST] * c5;
       jc2 += 1.0 * dq;
       jc3 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c5;
+    {
+      const double dq = scr[61 * SST] * c5;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c13;
+    {
+      const double dq = scr[63 * SST] * c13;
       jc3 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
+    {
+      const double dq = scr[65 * SST] * c6;
       jc3 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc9 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
+    {
+      const double dq = scr[67 * SST] * c12;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc15 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 32), a1 = ld2(rk1 + 32);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
+    {
+      const double dq = scr[69 * SST] * c12;
       jc2 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 34), a1 = ld2(rk1 + 34);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c23;
+    {
+      const double dq = -scr[72 * SST] * c23;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc18 += -1.0 * dq;
@@ -5378,8 +5397,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 44), a1 = ld2(rk1 + 44);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
+    {
+      const double dq = -scr[76 * SST] * c3;
       jc3 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -5396,8 +5415,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 56), a1 = ld2(rk1 + 56);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
+    {
+      const double dq = -scr[78 * SST] * c16;
       jc0 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -5735,15 +5754,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[55 * SST] * c9;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
+    {
+      const double dq = -scr[58 * SST] * c3;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc11 += 1.0 * dq;
@@ -6060,15 +6079,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[67 * SST] * c9;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc15 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 32), a1 = ld2(rk1 + 32);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[69 * SST] * c9;
       jc2 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc12 += -1.0 * dq;
@@ -6265,8 +6284,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
+    {
+      const double dq = -scr[56 * SST] * c8;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc11 += -1.0 * dq;
@@ -6278,22 +6297,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[63 * SST] * c9;
       jc3 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
+    {
+      const double dq = -scr[66 * SST] * c3;
       jc3 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc9 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 36), a1 = ld2(rk1 + 36);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
+    {
+      const double dq = scr[73 * SST] * c18;
       jc0 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
@@ -6752,8 +6771,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
+    {
+      const double dq = -scr[68 * SST] * c3;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc12 += -1.0 * dq;
@@ -6969,8 +6988,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 56), a1 = ld2(rk1 + 56);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
+    {
+      const double dq = -scr[78 * SST] * c9;
       jc0 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -7423,8 +7442,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
+    {
+      const double dq = -scr[46 * SST] * c3;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
@@ -7436,15 +7455,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 34), a1 = ld2(rk1 + 34);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
+    {
+      const double dq = scr[71 * SST] * c2;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 36), a1 = ld2(rk1 + 36);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c13;
+    {
+      const double dq = scr[73 * SST] * c13;
       jc0 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
@@ -7468,8 +7487,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 44), a1 = ld2(rk1 + 44);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
+    {
+      const double dq = scr[75 * SST] * c8;
       jc3 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -7567,8 +7586,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 132), a1 = ld2(rk1 + 132);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[41 * SST] * c7;
       jc2 += 1.0 * dq;
       jc3 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -7728,8 +7747,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[49 * SST] * c9;
       jc5 += 1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
@@ -7741,8 +7760,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 32), a1 = ld2(rk1 + 32);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
+    {
+      const double dq = -scr[70 * SST] * c2;
       jc2 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc12 += -1.0 * dq;
@@ -8132,8 +8151,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
+    {
+      const double dq = scr[57 * SST] * c9;
       jc3 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc11 += 1.0 * dq;
@@ -8531,15 +8550,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
+    {
+      const double dq = -scr[62 * SST] * c8;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 34), a1 = ld2(rk1 + 34);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
+    {
+      const double dq = -scr[72 * SST] * c9;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc18 += -1.0 * dq;

@@ -2,7 +2,7 @@
 // 31 species, 197 reactions (185 Arrhenius/three-body, 12 Troe, 0 Lindemann).
 #include "../flint_device.h"
 namespace flint_gen_tsr_rich_31 {
-constexpr int NS = 31, NZ = 32, WK = 418, WH = 62, NSLOT = 51;
+constexpr int NS = 31, NZ = 32, WK = 418, WH = 62, NSLOT = 79;
 
 // Rates, net production and (JAC) the analytic Jacobian of the constant-volume RHS for one cell.
 // r[k] = rho_k, T; outputs F[0..NS-1] = wdot, F[NS] = dT/dt (rhs_native); sink(row, col, v).
@@ -210,6 +210,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 3: H2 + O <=> H + OH
     const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[53 * SST] = kf; scr[54 * SST] = kb; }
     const double pf = kf*(c19)*(c7);
     const double pb = kb*(c2)*(c27);
     const double q = pf - pb;
@@ -229,6 +230,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 4: HO2 + O <=> O2 + OH
     const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[55 * SST] = kf; scr[56 * SST] = kb; }
     const double pf = kf*(c24)*(c7);
     const double pb = kb*(c15)*(c27);
     const double q = pf - pb;
@@ -248,6 +250,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 5: H2O2 + O <=> HO2 + OH
     const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[57 * SST] = kf; scr[58 * SST] = kb; }
     const double pf = kf*(c8)*(c7);
     const double pb = kb*(c24)*(c27);
     const double q = pf - pb;
@@ -267,6 +270,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 6: CH + O <=> CO + H
     const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[59 * SST] = kf; scr[60 * SST] = kb; }
     const double pf = kf*(c26)*(c7);
     const double pb = kb*(c9)*(c2);
     const double q = pf - pb;
@@ -286,6 +290,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 7: CH2 + O <=> H + HCO
     const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[61 * SST] = kf; scr[62 * SST] = kb; }
     const double pf = kf*(c29)*(c7);
     const double pb = kb*(c2)*(c17);
     const double q = pf - pb;
@@ -305,6 +310,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 8: CH2(S) + O <=> CO + H2
     const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[63 * SST] = kf; scr[64 * SST] = kb; }
     const double pf = kf*(c16)*(c7);
     const double pb = kb*(c9)*(c19);
     const double q = pf - pb;
@@ -324,6 +330,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 9: CH2(S) + O <=> H + HCO
     const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[65 * SST] = kf; scr[66 * SST] = kb; }
     const double pf = kf*(c16)*(c7);
     const double pb = kb*(c2)*(c17);
     const double q = pf - pb;
@@ -343,6 +350,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 10: CH3 + O <=> CH2O + H
     const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[67 * SST] = kf; scr[68 * SST] = kb; }
     const double pf = kf*(c18)*(c7);
     const double pb = kb*(c10)*(c2);
     const double q = pf - pb;
@@ -362,6 +370,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 11: CH4 + O <=> CH3 + OH
     const double2 a0 = ld2(rk0 + 20), a1 = ld2(rk1 + 20);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[69 * SST] = kf; scr[70 * SST] = kb; }
     const double pf = kf*(c3)*(c7);
     const double pb = kb*(c18)*(c27);
     const double q = pf - pb;
@@ -399,6 +408,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 13: HCO + O <=> CO + OH
     const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[71 * SST] = kf; scr[72 * SST] = kb; }
     const double pf = kf*(c17)*(c7);
     const double pb = kb*(c9)*(c27);
     const double q = pf - pb;
@@ -418,6 +428,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 14: HCO + O <=> CO2 + H
     const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[73 * SST] = kf; scr[74 * SST] = kb; }
     const double pf = kf*(c17)*(c7);
     const double pb = kb*(c30)*(c2);
     const double q = pf - pb;
@@ -437,6 +448,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 15: CH2O + O <=> HCO + OH
     const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[75 * SST] = kf; scr[76 * SST] = kb; }
     const double pf = kf*(c10)*(c7);
     const double pb = kb*(c17)*(c27);
     const double q = pf - pb;
@@ -456,6 +468,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 16: C2H2 + O <=> CH2 + CO
     const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[77 * SST] = kf; scr[78 * SST] = kb; }
     const double pf = kf*(c4)*(c7);
     const double pb = kb*(c29)*(c9);
     const double q = pf - pb;
@@ -2146,6 +2159,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   {  // reaction 96: CH2(S) + O2 <=> CO + H + OH
     const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
+    if (JAC) { scr[51 * SST] = kf; scr[52 * SST] = kb; }
     const double pf = kf*(c16)*(c15);
     const double pb = kb*(c9)*(c2)*(c27);
     const double q = pf - pb;
@@ -4773,36 +4787,36 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
+    {
+      const double dq = -scr[54 * SST] * c27;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc19 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
+    {
+      const double dq = -scr[60 * SST] * c9;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
+    {
+      const double dq = -scr[62 * SST] * c17;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
+    {
+      const double dq = -scr[66 * SST] * c17;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc16 += -1.0 * dq;
       jc17 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c10;
+    {
+      const double dq = -scr[68 * SST] * c10;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc10 += 1.0 * dq;
@@ -4814,8 +4828,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c30;
+    {
+      const double dq = -scr[74 * SST] * c30;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc17 += -1.0 * dq;
@@ -5133,8 +5147,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9*c27;
+    {
+      const double dq = -scr[52 * SST] * c9*c27;
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc15 += -1.0 * dq;
@@ -5300,8 +5314,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 20), a1 = ld2(rk1 + 20);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[69 * SST] * c7;
       jc3 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc18 += 1.0 * dq;
@@ -5600,8 +5614,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[77 * SST] * c7;
       jc4 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -6321,64 +6335,64 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
+    {
+      const double dq = scr[53 * SST] * c19;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc19 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
+    {
+      const double dq = scr[55 * SST] * c24;
       jc7 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
+    {
+      const double dq = scr[57 * SST] * c8;
       jc7 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc24 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
+    {
+      const double dq = scr[59 * SST] * c26;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c29;
+    {
+      const double dq = scr[61 * SST] * c29;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
+    {
+      const double dq = scr[63 * SST] * c16;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
+    {
+      const double dq = scr[65 * SST] * c16;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc16 += -1.0 * dq;
       jc17 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
+    {
+      const double dq = scr[67 * SST] * c18;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc10 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 20), a1 = ld2(rk1 + 20);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
+    {
+      const double dq = scr[69 * SST] * c3;
       jc3 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc18 += 1.0 * dq;
@@ -6390,29 +6404,29 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c17;
+    {
+      const double dq = scr[71 * SST] * c17;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c17;
+    {
+      const double dq = scr[73 * SST] * c17;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc17 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c10;
+    {
+      const double dq = scr[75 * SST] * c10;
       jc7 += -1.0 * dq;
       jc10 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c4;
+    {
+      const double dq = scr[77 * SST] * c4;
       jc4 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -6701,8 +6715,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[57 * SST] * c7;
       jc7 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc24 += 1.0 * dq;
@@ -6970,15 +6984,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
+    {
+      const double dq = -scr[60 * SST] * c2;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
+    {
+      const double dq = -scr[64 * SST] * c19;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc16 += -1.0 * dq;
@@ -6990,15 +7004,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
+    {
+      const double dq = -scr[72 * SST] * c27;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c29;
+    {
+      const double dq = -scr[78 * SST] * c29;
       jc4 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -7136,8 +7150,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2*c27;
+    {
+      const double dq = -scr[52 * SST] * c2*c27;
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc15 += -1.0 * dq;
@@ -7301,8 +7315,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
+    {
+      const double dq = -scr[68 * SST] * c2;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc10 += 1.0 * dq;
@@ -7314,8 +7328,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[75 * SST] * c7;
       jc7 += -1.0 * dq;
       jc10 += -1.0 * dq;
       jc17 += 1.0 * dq;
@@ -8678,8 +8692,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
+    {
+      const double dq = -scr[56 * SST] * c27;
       jc7 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
@@ -8847,8 +8861,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
+    {
+      const double dq = scr[51 * SST] * c16;
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc15 += -1.0 * dq;
@@ -9041,15 +9055,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[63 * SST] * c7;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[65 * SST] * c7;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc16 += -1.0 * dq;
@@ -9158,8 +9172,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
+    {
+      const double dq = scr[51 * SST] * c15;
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc15 += -1.0 * dq;
@@ -9330,15 +9344,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
+    {
+      const double dq = -scr[62 * SST] * c2;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
+    {
+      const double dq = -scr[66 * SST] * c2;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc16 += -1.0 * dq;
@@ -9350,22 +9364,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[71 * SST] * c7;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[73 * SST] * c7;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc17 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
+    {
+      const double dq = -scr[76 * SST] * c27;
       jc7 += -1.0 * dq;
       jc10 += -1.0 * dq;
       jc17 += 1.0 * dq;
@@ -9697,15 +9711,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[67 * SST] * c7;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc10 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 20), a1 = ld2(rk1 + 20);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
+    {
+      const double dq = -scr[70 * SST] * c27;
       jc3 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc18 += 1.0 * dq;
@@ -10139,15 +10153,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[53 * SST] * c7;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc19 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
+    {
+      const double dq = -scr[64 * SST] * c9;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc16 += -1.0 * dq;
@@ -11356,15 +11370,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[55 * SST] * c7;
       jc7 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
+    {
+      const double dq = -scr[58 * SST] * c27;
       jc7 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc24 += 1.0 * dq;
@@ -12072,8 +12086,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[59 * SST] * c7;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -12358,29 +12372,29 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
+    {
+      const double dq = -scr[54 * SST] * c2;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc19 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c15;
+    {
+      const double dq = -scr[56 * SST] * c15;
       jc7 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c24;
+    {
+      const double dq = -scr[58 * SST] * c24;
       jc7 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc24 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 20), a1 = ld2(rk1 + 20);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
+    {
+      const double dq = -scr[70 * SST] * c18;
       jc3 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc18 += 1.0 * dq;
@@ -12392,15 +12406,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
+    {
+      const double dq = -scr[72 * SST] * c9;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
+    {
+      const double dq = -scr[76 * SST] * c17;
       jc7 += -1.0 * dq;
       jc10 += -1.0 * dq;
       jc17 += 1.0 * dq;
@@ -12669,8 +12683,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9*c2;
+    {
+      const double dq = -scr[52 * SST] * c9*c2;
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc15 += -1.0 * dq;
@@ -13152,8 +13166,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
-      const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
+    {
+      const double dq = scr[61 * SST] * c7;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc17 += 1.0 * dq;
@@ -13165,8 +13179,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
+    {
+      const double dq = -scr[78 * SST] * c9;
       jc4 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -13471,8 +13485,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
-      const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
+    {
+      const double dq = -scr[74 * SST] * c2;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc17 += -1.0 * dq;
