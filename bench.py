@@ -43,7 +43,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--cells", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", choices=["reactor", "omega", "cea", "mlp", "thermo", "general"], default=None)
+    ap.add_argument("--only", choices=["reactor", "omega", "cea", "mlp", "thermo", "general", "transport"],
+                    default=None)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run a plain device-to-device copy of this many bytes (counter calibration)")
@@ -359,6 +360,51 @@ def thermo_leg(args, dev, stream):
     return res
 
 
+def transport_leg(args, dev, stream):
+    """Wilke mixture viscosity and conductivity of 2^20 GP-24 flow cells (flint_transport)."""
+    import numpy as np
+    import torch
+    from flint_amd import states as S
+    from flint_amd.mechdata import mech_data
+    from flint_amd.native import Mechanism
+    from flint_amd.transport import species_tables
+    md = mech_data("TSR-GP-24")
+    ns = md.ns
+    mech = Mechanism.from_data(md)
+    tb = species_tables(md)
+    mech.transport_setup(tb)
+    n = 1 << 20
+    fs = S.flow_states(md.species, md.W, 2028, 0, n)
+    r_h = np.ascontiguousarray(fs["prim"][:ns])
+    rho_h = r_h.sum(0)
+    r = torch.from_numpy(r_h).to(dev)
+    rho = torch.from_numpy(rho_h).to(dev)
+    T = torch.from_numpy(fs["T"]).to(dev)
+    mu, lam = torch.empty(n, dtype=torch.float64, device=dev), torch.empty(n, dtype=torch.float64, device=dev)
+    run = lambda: mech.transport(r, rho, T, mu, lam, stream=stream)
+    for _ in range(3):
+        run()
+    ms = timed(stream, run, 20)
+    torch.cuda.synchronize()
+    bpc = (ns + 4) * 8  # rho_k, rho, T in; mu, lambda out
+    fpc = 6 * ns * ns + 20 * ns  # 3 FMAs per species pair, ~20 flops per species (X, r, interpolation)
+    res = dict(n=n, ms=ms, cells_per_s=n / (ms * 1e-3), bytes_per_cell=bpc, flops_per_cell=fpc,
+               achieved_GBps=bpc * n / (ms * 1e-3) / 1e9, achieved_TFLOPs=fpc * n / (ms * 1e-3) / 1e12)
+    if not args.no_cpu:
+        O = oracle()
+        o = O.Oracle(md)
+        m = 1 << 14
+        reps, t = 0, time.perf_counter()
+        while time.perf_counter() - t < 0.3 * args.cpu_seconds:
+            o.wilke(tb, r_h[:, :m], rho_h[:m], fs["T"][:m])
+            reps += 1
+        dt = time.perf_counter() - t
+        res["cpu"] = dict(value=reps * m / dt, unit="cells/s", cores=o.num_threads(), kind="port",
+                          sample=f"{reps} passes over the first {m} cells, C oracle (the reference's ns^2 "
+                                 f"form, OpenMP over cells), wall {dt:.1f} s")
+    return res
+
+
 def general_leg(args, dev, stream):
     import numpy as np
     import torch
@@ -426,6 +472,8 @@ def main():
         extra["equilibrium"] = cea_leg(args, dev, stream)
     if only in (None, "thermo"):
         extra["cons2prim"] = thermo_leg(args, dev, stream)
+    if only in (None, "transport"):
+        extra["transport"] = transport_leg(args, dev, stream)
     if only in (None, "general"):
         extra["source_general"] = general_leg(args, dev, stream)
     out = {

@@ -160,6 +160,7 @@ extern "C" void flint_mech_destroy(flint_mech* m) {
   flint_mech_release_extra(m);
   flint_mech_release_cea(m);
   flint_mech_release_sur(m);
+  flint_mech_release_transport(m);
   flint_mech_release_general(m);
   if (m->d_tabk) (void)hipFree(m->d_tabk);
   if (m->d_tabh) (void)hipFree(m->d_tabh);

@@ -12,6 +12,7 @@ struct flint_extra;     // per-handle state of the reactor path
 struct flint_general_data;  // program of the data-driven general source term
 struct flint_cea_data;  // per-handle state of the equilibrium path
 struct flint_sur_data;  // per-handle buffers of the learned reactor step
+struct flint_transport_data;  // species transport tables and the Wilke weight-factor matrices
 
 struct flint_mech {
   std::string name, id;
@@ -35,6 +36,7 @@ struct flint_mech {
   flint_general_data* gen = nullptr;
   flint_cea_data* cea = nullptr;
   flint_sur_data* sur = nullptr;
+  flint_transport_data* tr = nullptr;
 };
 
 int flint_fail(int code, const std::string& msg);
@@ -45,4 +47,5 @@ void flint_mech_release_extra(flint_mech* m);
 void flint_mech_release_cea(flint_mech* m);
 void flint_mech_release_sur(flint_mech* m);
 void flint_mech_release_general(flint_mech* m);
+void flint_mech_release_transport(flint_mech* m);
 double flint_atomic_weight(const char* symbol);

@@ -255,6 +255,38 @@ extern "C" int flint_mech_load(const char* folder, const char* prefix, flint_mec
       return rc2;
     }
   }
+  // transport.dat (optional): one zone per species, (T, mu, k), as read_idealgas_transport
+  // (Load_ThermoTransport.f90:129-174), row Tmin = 0 copied from row 1 when the file starts at 1
+  if (FILE* tf = std::fopen((base + "transport.dat").c_str(), "rb")) {
+    std::fclose(tf);
+    Zones tz;
+    int rc2 = read_tecplot(base + "transport.dat", 2, tz);
+    if (!rc2 && (int)tz.vars.size() != ns) rc2 = flint_fail(FLINT_EIO, "transport.dat: zone count differs from phase.txt");
+    if (!rc2) {
+      int Tt_min = (int)std::lround(tz.T0[0]);
+      const int Tt_max = Tt_min + tz.npts - 1, t0 = Tt_min;
+      if (Tt_min == 1) Tt_min = 0;
+      const size_t nT = (size_t)(Tt_max - Tt_min + 1);
+      std::vector<double> mu((size_t)ns * nT), kt((size_t)ns * nT);
+      for (int k = 0; k < ns; ++k) {
+        const auto& v = tz.vars[k];
+        for (int i = 0; i < tz.npts; ++i) {
+          mu[(size_t)k * nT + (t0 - Tt_min) + i] = v[i];
+          kt[(size_t)k * nT + (t0 - Tt_min) + i] = v[(size_t)tz.npts + i];
+        }
+        if (t0 != Tt_min) {
+          mu[(size_t)k * nT] = v[0];
+          kt[(size_t)k * nT] = v[(size_t)tz.npts];
+        }
+      }
+      rc2 = flint_transport_setup(*out, Tt_min, Tt_max, mu.data(), kt.data());
+    }
+    if (rc2) {
+      flint_mech_destroy(*out);
+      *out = nullptr;
+      return rc2;
+    }
+  }
   if (ne == 0) return FLINT_OK;
   // equilibrium data (CEA_initialize_global), where the mechanism has an equilibrium kernel and
   // every element has a standard atomic weight

@@ -52,6 +52,9 @@ def lib():
         L.flint_cons2prim.argtypes = [vp, l, l, i] + [vp] * 7
         L.flint_prim2cons.argtypes = [vp, l, l, i, vp, vp, vp, vp]
         L.flint_cons2prim_host.argtypes = [vp, l, i] + [vp] * 5
+        L.flint_transport_setup.argtypes = [vp, i, i, vp, vp]
+        L.flint_transport.argtypes = [vp, l, l] + [vp] * 7
+        L.flint_transport_host.argtypes = [vp, l] + [vp] * 6
         L.flint_prim2cons_host.argtypes = [vp, l, i, vp, vp]
         L.flint_reactor_advance_host.argtypes = [vp, l, i, vp, vp, d, d, d, i, vp]
         L.flint_cea_solve_host.argtypes = [vp, l, i, vp, vp, vp, vp, vp, vp]
@@ -188,6 +191,33 @@ class Mechanism:
         n = ld if n is None else n
         nd = prim.shape[0] - self.ns - 1
         check(lib().flint_prim2cons(self._h, n, ld, nd, _p(prim), _p(cons), _p(status), _stream(stream)))
+
+    # ---- mixture transport (Wilke) ----
+    def transport_setup(self, tables):
+        """Species tables (Tt_min, Tt_max, mu_tab [ns, nT], k_tab [ns, nT]), temperature fastest
+        (flint_transport_setup)."""
+        Tt_min, Tt_max, mu_tab, k_tab = tables
+        mu_tab = np.ascontiguousarray(mu_tab, dtype=np.float64)
+        k_tab = np.ascontiguousarray(k_tab, dtype=np.float64)
+        if mu_tab.shape != (self.ns, Tt_max - Tt_min + 1) or k_tab.shape != mu_tab.shape:
+            raise FlintError("transport tables must be [ns, Tt_max - Tt_min + 1]")
+        check(lib().flint_transport_setup(self._h, int(Tt_min), int(Tt_max), _p(mu_tab), _p(k_tab)))
+
+    def transport(self, rhoi, rho, T, mu, lam, status=None, n=None, stream=None):
+        """rhoi [ns, ld], rho [ld], T [ld] (device) -> mu, lam [ld] by Wilke's rule (flint_transport)."""
+        ld = rhoi.shape[-1]
+        n = ld if n is None else n
+        check(lib().flint_transport(self._h, n, ld, _p(rhoi), _p(rho), _p(T), _p(mu), _p(lam), _p(status),
+                                    _stream(stream)))
+
+    def transport_host(self, rhoi, rho, T):
+        """rhoi [ns, n], rho [n], T [n] (numpy) -> (mu, lam, status)."""
+        rhoi = np.ascontiguousarray(rhoi, dtype=np.float64)
+        n = rhoi.shape[1]
+        mu, lam, st = np.empty(n), np.empty(n), np.empty(n, dtype=np.int32)
+        check(lib().flint_transport_host(self._h, n, _p(rhoi), _p(np.ascontiguousarray(rho, dtype=np.float64)),
+                                         _p(np.ascontiguousarray(T, dtype=np.float64)), _p(mu), _p(lam), _p(st)))
+        return mu, lam, st
 
     # ---- batched reactors ----
     CV, CP = 0, 1

@@ -121,6 +121,26 @@ int flint_cons2prim_host(flint_mech *m, long n, int nd, const double *cons, cons
                          double *T_out, int *status);
 int flint_prim2cons_host(flint_mech *m, long n, int nd, const double *prim, double *cons);
 
+/* Species transport tables (replaces read_idealgas_transport's fill of mi_tab / k_tab,
+ * Load_ThermoTransport.f90:129-174; flint_mech_load calls it itself when the folder holds
+ * transport.dat).  mu_tab, k_tab: host arrays laid out as the reference's mi_tab(Tt_min:Tt_max, ns)
+ * and k_tab, temperature fastest, viscosity in Pa s and conductivity in W/(m K).  Replaces any
+ * tables set before. */
+int flint_transport_setup(flint_mech *m, int Tt_min, int Tt_max, const double *mu_tab, const double *k_tab);
+
+/* Mixture viscosity and thermal conductivity by Wilke's rule over a batch (replaces
+ * co_k_mi_lam_Wilke, Lib_ThermoTransport.f90:410-466, with co_fiij :350-362; f_laminarViscosity
+ * :365-405 is the same mu with T = p / (Rtot rho)): rhoi_dev[ns][ld], rho_dev[n] (the density the
+ * mole fractions divide by, as the reference's rho argument), T_dev[n] -> mu_dev[n], lam_dev[n].
+ * status_dev (may be NULL): 0, or 2 with NaN outputs where T is outside [Tt_min, Tt_max), which
+ * the reference does not check.  FLINT_EUNSUPPORTED before flint_transport_setup. */
+int flint_transport(const flint_mech *m, long n, long ld, const double *rhoi_dev, const double *rho_dev,
+                    const double *T_dev, double *mu_dev, double *lam_dev, int *status_dev, void *stream);
+
+/* Host-memory form of flint_transport (ld = n, synchronous, as flint_rhs_host). */
+int flint_transport_host(flint_mech *m, long n, const double *rhoi, const double *rho, const double *T, double *mu,
+                         double *lam, int *status);
+
 /* Batched 0D reactors (no reference counterpart in FLINT's own code: it advances one reactor at a
  * time through the external OSlo integrator, test-batchF.f90:81,105, with rhs_native as the
  * callback).  Advances n cells from t = 0 to t_end in one persistent launch, variable-order BDF

@@ -75,6 +75,13 @@ void orc_cons2prim(const orc_mech *m, long n, long ld, int np, const double *con
                    double *Tout, int *iters, int *status);
 void orc_prim2cons(const orc_mech *m, long n, long ld, int np, const double *prim, double *cons, int *status);
 
+/* Wilke mixture viscosity and conductivity (oracle_transport.c, co_k_mi_lam_Wilke) and the
+ * viscosity alone with T = p / (Rtot rho) (f_laminarViscosity); tables [ns][nT], T fastest */
+void orc_wilke(int ns, const double *W, int Tt_min, int Tt_max, const double *mu_tab, const double *k_tab, long n,
+               long ld, const double *rhoi, const double *rho, const double *T, double *mu, double *lam, int *status);
+void orc_laminar_viscosity(int ns, const double *W, int Tt_min, int Tt_max, const double *mu_tab, long n, long ld,
+                           const double *rhoi, const double *p, const double *rho, const double *Rtot, double *mu);
+
 /* BDF advance of n reactors over [0, t_end] (oracle_bdf.c); stats[i*8..] and status[i] per cell */
 void orc_bdf(const orc_mech *m, long n, long ld, int mode, double *X, const double *p, double rtol, double atol,
              double t_end, int max_steps, int *stats, int *status);

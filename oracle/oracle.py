@@ -119,6 +119,31 @@ class Oracle:
                              _ptr(Tout), _ptr(it), _ptr(st))
         return prim, Tout, it, st
 
+    def wilke(self, tables, rhoi, rho, T):
+        """co_k_mi_lam_Wilke with species tables (Tt_min, Tt_max, mu_tab [ns, nT], k_tab [ns, nT]):
+        rhoi [ns, n], rho [n], T [n] -> (mu [n], lam [n], status [n])."""
+        Tt_min, Tt_max, mu_tab, k_tab = tables
+        rhoi = np.ascontiguousarray(rhoi, dtype=np.float64)
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        n = T.size
+        mu, lam, st = np.empty(n), np.empty(n), np.empty(n, dtype=np.int32)
+        self.L.orc_wilke(C.c_int(self.md.ns), _ptr(np.ascontiguousarray(self.md.W, dtype=np.float64)),
+                         C.c_int(Tt_min), C.c_int(Tt_max), _ptr(mu_tab), _ptr(k_tab), C.c_long(n), C.c_long(n),
+                         _ptr(rhoi), _ptr(rho), _ptr(T), _ptr(mu), _ptr(lam), _ptr(st))
+        return mu, lam, st
+
+    def laminar_viscosity(self, tables, rhoi, p, rho, Rtot):
+        """f_laminarViscosity: the Wilke viscosity at T = p / (Rtot rho)."""
+        Tt_min, Tt_max, mu_tab, _ = tables
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rhoi, p, rho, Rtot)]
+        n = arrs[1].size
+        mu = np.empty(n)
+        self.L.orc_laminar_viscosity(C.c_int(self.md.ns), _ptr(np.ascontiguousarray(self.md.W, dtype=np.float64)),
+                                     C.c_int(Tt_min), C.c_int(Tt_max), _ptr(mu_tab), C.c_long(n), C.c_long(n),
+                                     *[_ptr(a) for a in arrs], _ptr(mu))
+        return mu
+
     def prim2cons(self, prim):
         """prim [ns+nd+1, n] -> (cons, status) (prim2cons)."""
         prim = np.ascontiguousarray(prim, dtype=np.float64)
@@ -266,6 +291,19 @@ class Reference:
         self.L.ref_cons2prim(C.c_long(n), C.c_long(n), C.c_int(npr), _ptr(cons),
                              _ptr(np.ascontiguousarray(T, dtype=np.float64)), _ptr(prim))
         return prim
+
+    def wilke(self, tables, rhoi, rho, T, p, Rtot):
+        """co_k_mi_lam_Wilke and f_laminarViscosity of the reference modules per cell, with the
+        species tables on the thermo rows -> (mu, lam, mu_laminar)."""
+        Tt_min, Tt_max, mu_tab, k_tab = tables
+        assert Tt_min == self.md.Th_min and mu_tab.shape[1] == self.md.cp.shape[0]
+        self._tkeep = (mu_tab, k_tab)
+        self.L.ref_transport_setup(C.c_int(mu_tab.shape[1]), _ptr(mu_tab), _ptr(k_tab))
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rhoi, rho, T, p, Rtot)]
+        n = arrs[2].size
+        mu, lam, mul = np.empty(n), np.empty(n), np.empty(n)
+        self.L.ref_wilke(C.c_long(n), C.c_long(n), *[_ptr(a) for a in arrs], _ptr(mu), _ptr(lam), _ptr(mul))
+        return mu, lam, mul
 
     def prim2cons(self, prim):
         prim = np.ascontiguousarray(prim, dtype=np.float64)

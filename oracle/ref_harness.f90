@@ -212,4 +212,31 @@ contains
     enddo
   end subroutine ref_prim2cons
 
+  ! species transport tables (what read_idealgas_transport fills, Load_ThermoTransport.f90:129-174)
+  ! on the thermo rows Tmin..Tmax set by ref_setup: mu(nT, ns), k(nT, ns)
+  subroutine ref_transport_setup(nT, mu, k) bind(C, name='ref_transport_setup')
+    integer(c_int), value :: nT
+    real(c_double), intent(in) :: mu(nT, *), k(nT, *)
+    if (allocated(mi_tab)) deallocate(mi_tab, k_tab)
+    allocate(mi_tab(Tmin:Tmax, ns), k_tab(Tmin:Tmax, ns))
+    mi_tab(Tmin:Tmax, 1:ns) = mu(1:nT, 1:ns)
+    k_tab(Tmin:Tmax, 1:ns) = k(1:nT, 1:ns)
+  end subroutine ref_transport_setup
+
+  ! co_k_mi_lam_Wilke (Lib_ThermoTransport.f90:410-423) and f_laminarViscosity (:365-405) per cell
+  subroutine ref_wilke(n, ld, rhoi, rho, T, p, Rtot, mu, lam, mul) bind(C, name='ref_wilke')
+    integer(c_long), value :: n, ld
+    real(c_double), intent(in) :: rhoi(ld, *), rho(*), T(*), p(*), Rtot(*)
+    real(c_double), intent(out) :: mu(*), lam(*), mul(*)
+    real(8) :: r(ns), a, b
+    integer(c_long) :: i
+    do i = 1, n
+      r = rhoi(i, 1:ns)
+      call co_k_mi_lam_Wilke(r, rho(i), T(i), a, b)
+      mu(i) = a
+      lam(i) = b
+      mul(i) = f_laminarViscosity(r, p(i), rho(i), Rtot(i))
+    enddo
+  end subroutine ref_wilke
+
 end module ref_harness

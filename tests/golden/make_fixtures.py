@@ -14,6 +14,9 @@
 * cons2prim_<mech>.npz  prim2cons and cons2prim outputs of the reference (oracle/_ref:
                    Lib_ThermoTransport.f90 compiled by flang) on seeded cells from
                    flint_amd.states.flow_states (seed, start, count, nd stored with them).
+* transport_<mech>.npz  co_k_mi_lam_Wilke and f_laminarViscosity outputs of the reference
+                   (oracle/_ref: Lib_ThermoTransport.f90 compiled by flang) on seeded flow_states
+                   cells, species tables from flint_amd.transport.species_tables (seed stored).
 * wdot_<mech>.npz  chemistry_source outputs of the reference Fortran routines themselves
                    (oracle/_ref: WD.f90, TSR-GP-24.f90, TSR-Rich-31.f90 compiled by flang) on seeded
                    states; the inputs are regenerated from (seed, start, count) by flint_amd.states.
@@ -129,7 +132,25 @@ def cons2prim():
         print(mech, "cons2prim", count, nd)
 
 
+TR_CASES = {"TSR-GP-24": (21, 0, 512, 7), "TSR-Rich-31": (22, 0, 512, 7)}
+
+
+def transport():
+    from flint_amd.transport import species_tables
+    for mech, (seed, start, count, tseed) in TR_CASES.items():
+        md = mech_data(mech)
+        fs = S.flow_states(md.species, md.W, seed, start, count)
+        r = fs["prim"][:md.ns]
+        rho = r.sum(0)
+        Rtot = (r / rho * (8314.51 / md.W[:, None])).sum(0)
+        p = rho * Rtot * fs["T"]
+        mu, lam, mul = O.Reference(md).wilke(species_tables(md, tseed), r, rho, fs["T"], p, Rtot)
+        np.savez_compressed(os.path.join(OUT, f"transport_{mech}.npz"), seed=seed, start=start, count=count,
+                            table_seed=tseed, p=p, Rtot=Rtot, mu=mu, lam=lam, mu_laminar=mul)
+        print(mech, "transport", count)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["tables", "wdot", "cea", "cons2prim", "general"]
+    which = sys.argv[1:] or ["tables", "wdot", "cea", "cons2prim", "general", "transport"]
     for w in which:
         globals()[w]()
