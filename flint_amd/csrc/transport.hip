@@ -1,5 +1,5 @@
 // Mixture transport of a cell by Wilke's rule: co_k_mi_lam_Wilke / co_k_mi_lam_Wilke_expr
-// (Lib_ThermoTransport.f90:410-466) with co_fiij (:350-362), f_molecularWeight (:200-214) and
+// (Lib_ThermoTransport.f90:407-462) with co_fiij (:350-362), f_molecularWeight (:200-214) and
 // f_tabT_expr (:531-542).  One thread per cell.
 //
 // The reference forms every interaction factor
@@ -11,8 +11,9 @@
 //   den_i = sum_j X_j D_ij phi-expanded = (D X)_i + s_i (2Dc (X r))_i + s_i^2 (Dc^2 (X r^2))_i.
 // Every term is positive, so the rewrite costs a few ulps, not cancellation: 3 ns^2 FMAs per cell
 // instead of ns^2 divisions and square roots.  The matrices are read at wave-uniform addresses
-// (scalar loads); the per-cell vectors stay in registers for the compiled species counts, and a
-// generic loop form serves any other count up to FLINT_TRANSPORT_MAXNS.
+// (scalar loads); the per-cell vectors stay in registers for the compiled species counts (the
+// densities are read once), and a generic loop form serves any other count up to
+// FLINT_TRANSPORT_MAXNS.
 #include "flint_device.h"
 #include "flint_transport.h"
 
@@ -63,24 +64,37 @@ __global__ __launch_bounds__(256) void wilke_kernel(TrProg P, long n, long ld, c
   const double* __restrict__ M0 = P.M;
   const double* __restrict__ M1 = P.M + ns * ns;
   const double* __restrict__ M2 = P.M + 2 * ns * ns;
-  // Xi = rhoi * Wmtot / (rho * W) with Wmtot = 1 / inv_wmix
-  const double wm = 1.0 / inv_wmix(c, ld, ns, W);
   const double rc = rho[i];
   double m_acc = 0.0, k_acc = 0.0;
   if constexpr (NS > 0) {
+    // the cell's mole fractions, also in LDS for the row loop's runtime index (each thread reads
+    // only its own column, so no barrier); the densities are read from HBM once
+    __shared__ double sx[NS * 256];
     double u[NS], v[NS], w[NS];
 #pragma unroll
+    for (int j = 0; j < NS; ++j) u[j] = c[(long)j * ld];
+    // Xi = rhoi * Wmtot / (rho * W) with 1 / Wmtot = sum_s rhoi_s / sum(rhoi) / W_s
+    double ra = 0.0, rb = 0.0;
+#pragma unroll
+    for (int j = 0; j < NS; ++j) ra = ra + u[j];
+#pragma unroll
+    for (int j = 0; j < NS; ++j) rb = rb + u[j] / ra / W[j];
+    const double wm = 1.0 / rb;
+#pragma unroll
     for (int j = 0; j < NS; ++j) {
-      const double X = c[(long)j * ld] * wm / (rc * W[j]);
+      const double X = u[j] * wm / (rc * W[j]);
+      sx[j * 256 + threadIdx.x] = X;
       const double2 mk = interp(r0, NS, j, Td);
       const double r = 1.0 / sqrt(mk.x + 1e-20);
       u[j] = X;
       v[j] = X * r;
       w[j] = v[j] * r;
     }
-    // rows one at a time (a runtime loop keeps the scalar-loaded matrix rows from being hoisted
-    // into registers); u, v, w are indexed by the unrolled j only
-#pragma unroll 1
+    // rows four at a time: a runtime loop keeps the scalar-loaded matrix rows from all being
+    // hoisted into registers, and four rows keep enough loads in flight (0.37 ms one row at a
+    // time, 0.30 ms two, 0.28 ms four, per 2^20 24-species cells); u, v, w are indexed by the
+    // unrolled j only
+#pragma unroll 4
     for (int a = 0; a < NS; ++a) {
       double A = 0.0, B = 0.0, C = 0.0;
 #pragma unroll
@@ -92,11 +106,12 @@ __global__ __launch_bounds__(256) void wilke_kernel(TrProg P, long n, long ld, c
       const double2 mk = interp(r0, NS, a, Td);
       const double s = sqrt(mk.x);
       const double inv = 1.0 / fma(s, fma(s, C, B), A);
-      const double Xa = c[(long)a * ld] * wm / (rc * W[a]);  // u[a], recomputed with the same operations
+      const double Xa = sx[a * 256 + threadIdx.x];
       m_acc = fma(Xa * mk.x, inv, m_acc);
       k_acc = fma(Xa * mk.y, inv, k_acc);
     }
   } else {
+    const double wm = 1.0 / inv_wmix(c, ld, ns, W);
     for (int a = 0; a < ns; ++a) {
       double A = 0.0, B = 0.0, C = 0.0;
       for (int j = 0; j < ns; ++j) {

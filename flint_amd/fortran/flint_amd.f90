@@ -107,6 +107,16 @@ module flint_amd_c
       real(c_double), intent(out) :: cons(*)
       integer(c_int) :: rc
     end function
+    function flint_transport_host(m, n, rhoi, rho, T, mu, lam, status) bind(C, name="flint_transport_host") &
+        result(rc)
+      import :: c_ptr, c_long, c_double, c_int
+      type(c_ptr), value :: m
+      integer(c_long), value :: n
+      real(c_double), intent(in) :: rhoi(*), rho(*), T(*)
+      real(c_double), intent(out) :: mu(*), lam(*)
+      type(c_ptr), value :: status
+      integer(c_int) :: rc
+    end function
   end interface
 end module flint_amd_c
 
@@ -131,7 +141,8 @@ module flint_amd
   public :: read_idealgas_thermo, read_chemistry, Assign_Mechanism, chemistry_source, rhs_native, &
             CEA_initialize_global, CEA_solve, free_chemistry_data, chemistry_source_batch, &
             rhs_native_batch, reactor_advance_batch, CEA_solve_batch, flint_error_message, cons2prim, &
-            prim2cons, cons2prim_batch, prim2cons_batch
+            prim2cons, cons2prim_batch, prim2cons_batch, co_k_mi_lam_Wilke, f_laminarViscosity, &
+            transport_batch
 
 contains
 
@@ -320,6 +331,42 @@ contains
                                    c_null_ptr), 'cons2prim')
     prim = q
   end function
+
+  !> Mixture viscosity milam [Pa s] and conductivity klam [W/(m K)] of one cell by Wilke's rule
+  !> (co_k_mi_lam_Wilke, Lib_ThermoTransport.f90:407); the tables come from transport.dat in the
+  !> folder read_chemistry loaded.  NaN when T leaves the transport table.
+  subroutine co_k_mi_lam_Wilke(rhoi, rho, T, milam, klam)
+    real(8), intent(in) :: rhoi(:), rho, T
+    real(8), intent(out) :: milam, klam
+    real(c_double) :: r(size(rhoi)), a(1), b(1), m(1), k(1)
+    call need_mech('co_k_mi_lam_Wilke')
+    r = rhoi
+    a(1) = rho
+    b(1) = T
+    call fail(flint_transport_host(mech, 1_c_long, r, a, b, m, k, c_null_ptr), 'co_k_mi_lam_Wilke')
+    milam = m(1)
+    klam = k(1)
+  end subroutine
+
+  !> Mixture viscosity at T = p / (Rtot rho) (f_laminarViscosity, Lib_ThermoTransport.f90:365).
+  function f_laminarViscosity(rhoi, p, rho, Rtot) result(mil)
+    real(8), intent(in) :: rhoi(:), p, rho, Rtot
+    real(8) :: mil, klam
+    call co_k_mi_lam_Wilke(rhoi, rho, p / (Rtot * rho), mil, klam)
+  end function
+
+  !> co_k_mi_lam_Wilke over n cells in one call: rhoi(n, ns), rho(n), T(n) -> milam(n), klam(n).
+  subroutine transport_batch(rhoi, rho, T, milam, klam, status)
+    real(8), intent(in) :: rhoi(:, :), rho(:), T(:)
+    real(8), intent(out) :: milam(:), klam(:)
+    integer, intent(out), optional, target :: status(:)
+    integer(c_int), allocatable, target :: st(:)
+    call need_mech('transport_batch')
+    allocate(st(size(T)))
+    call fail(flint_transport_host(mech, int(size(T), c_long), rhoi, rho, T, milam, klam, c_loc(st)), &
+              'transport_batch')
+    if (present(status)) status = st
+  end subroutine
 
   !> Conservative variables of one cell from its primitive ones (Lib_ThermoTransport.f90:168).
   function prim2cons(prim) result(cons)

@@ -53,7 +53,8 @@ int flint_mech_create(const char *mech_name, int ns, const double *W, int Tk_min
                       const double *comp, flint_mech **out);
 
 /* Read FLINT's native input folder (phase.txt, composition.txt, chemistry-info.txt, thermo.dat,
- * chemistry-{Arrhenius,Troe,Lindemann}.dat; Tecplot ASCII only, .szplt is not supported) and
+ * chemistry-{Arrhenius,Troe,Lindemann}.dat, and transport.dat when present; Tecplot ASCII only,
+ * .szplt is not supported) and
  * select the compiled kernels for the mechanism named in chemistry-info.txt.
  * Replaces read_idealgas_thermo(folder) + read_chemistry(folder, mech_name) + Assign_Mechanism;
  * prefix plays the role of FLINT_phase_prefix (Lib_ThermoTransport.f90:8). */
@@ -129,8 +130,8 @@ int flint_prim2cons_host(flint_mech *m, long n, int nd, const double *prim, doub
 int flint_transport_setup(flint_mech *m, int Tt_min, int Tt_max, const double *mu_tab, const double *k_tab);
 
 /* Mixture viscosity and thermal conductivity by Wilke's rule over a batch (replaces
- * co_k_mi_lam_Wilke, Lib_ThermoTransport.f90:410-466, with co_fiij :350-362; f_laminarViscosity
- * :365-405 is the same mu with T = p / (Rtot rho)): rhoi_dev[ns][ld], rho_dev[n] (the density the
+ * co_k_mi_lam_Wilke, Lib_ThermoTransport.f90:407-462, with co_fiij :350-362; f_laminarViscosity
+ * :365-404 is the same mu with T = p / (Rtot rho)): rhoi_dev[ns][ld], rho_dev[n] (the density the
  * mole fractions divide by, as the reference's rho argument), T_dev[n] -> mu_dev[n], lam_dev[n].
  * status_dev (may be NULL): 0, or 2 with NaN outputs where T is outside [Tt_min, Tt_max), which
  * the reference does not check.  FLINT_EUNSUPPORTED before flint_transport_setup. */

@@ -1,7 +1,7 @@
 /* Mixture transport of the FLINT reference -- TEST INFRASTRUCTURE ONLY.
  *
- * Restates co_k_mi_lam_Wilke / co_k_mi_lam_Wilke_expr (Lib_ThermoTransport.f90:410-466) and
- * f_laminarViscosity (:365-405) with co_fiij (:350-362), f_molecularWeight (:200-214) and
+ * Restates co_k_mi_lam_Wilke / co_k_mi_lam_Wilke_expr (Lib_ThermoTransport.f90:407-462) and
+ * f_laminarViscosity (:365-404) with co_fiij (:350-362), f_molecularWeight (:200-214) and
  * f_tabT_expr (:531-542), literally: the full ns x ns interaction matrix per cell, in the
  * reference's operation order (gcc on x86-64 emits no FMA without -mfma).  Tables mu_tab, k_tab
  * are [ns][nT] with T fastest, as mi_tab(Tmin:Tmax, ns) in memory; rhoi[s*ld + i].

@@ -223,7 +223,7 @@ contains
     k_tab(Tmin:Tmax, 1:ns) = k(1:nT, 1:ns)
   end subroutine ref_transport_setup
 
-  ! co_k_mi_lam_Wilke (Lib_ThermoTransport.f90:410-423) and f_laminarViscosity (:365-405) per cell
+  ! co_k_mi_lam_Wilke (Lib_ThermoTransport.f90:407-421) and f_laminarViscosity (:365-404) per cell
   subroutine ref_wilke(n, ld, rhoi, rho, T, p, Rtot, mu, lam, mul) bind(C, name='ref_wilke')
     integer(c_long), value :: n, ld
     real(c_double), intent(in) :: rhoi(ld, *), rho(*), T(*), p(*), Rtot(*)
