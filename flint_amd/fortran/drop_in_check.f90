@@ -7,7 +7,8 @@
 !               m, then m lines "T p_bar Y_1 .. Y_ns"        (CEA_solve, UV and HP)
 !               k, then k lines "p_Pa Y_1 .. Y_ns T"         (constant-pressure reactors)
 !               t_end rtol atol                              (reactor integration)
-!               q nd, then q lines "T0 rho_1 .. rho_ns u_1 .. u_nd p"  (prim2cons, cons2prim)
+!               q nd, then q lines "T0 rho_1 .. rho_ns u_1 .. u_nd p"  (prim2cons, cons2prim, and
+!               co_k_mi_lam_Wilke at (rho_k, sum rho_k, T0) when the folder has transport.dat)
 ! results file: blocks "# name rows cols" followed by the rows.
 program drop_in_check
   use flint_amd
@@ -23,6 +24,8 @@ program drop_in_check
   integer :: q, nd, npr
   real(8), allocatable :: T0(:), P0(:, :), C1(:, :), C2(:, :), Q1(:, :), Q2(:, :), Tq(:)
   integer, allocatable :: sq(:)
+  real(8), allocatable :: mu1(:), k1(:), mu2(:), k2(:), rq(:)
+  logical :: has_tr
 
   call get_command_argument(1, folder)
   call get_command_argument(2, fin)
@@ -121,6 +124,19 @@ program drop_in_check
   call cons2prim_batch(C2, T0, Q2, Tq, sq)
   call put(v, 'prim2cons_batch', C2)
   call put(v, 'cons2prim_batch', reshape([Q2, Tq, dble(sq)], [q, npr + 2]))
+
+  ! mixture transport, per cell and batched
+  inquire(file=trim(folder)//'/transport.dat', exist=has_tr)
+  if (has_tr) then
+    allocate(mu1(q), k1(q), mu2(q), k2(q), rq(q))
+    do i = 1, q
+      rq(i) = sum(P0(i, 1:ns))
+      call co_k_mi_lam_Wilke(P0(i, 1:ns), rq(i), T0(i), mu1(i), k1(i))
+    end do
+    call put(v, 'co_k_mi_lam_Wilke', reshape([mu1, k1], [q, 2]))
+    call transport_batch(P0(:, 1:ns), rq, T0, mu2, k2, sq)
+    call put(v, 'transport_batch', reshape([mu2, k2, dble(sq)], [q, 3]))
+  end if
   close(v)
 
   call free_chemistry_data()

@@ -28,9 +28,13 @@ def native(tmp_path_factory, mechs, cuda):
     from flint_amd.tables import write_native
     folder = str(tmp_path_factory.mktemp("native") / NAME)
     print(f"[gpu] write_native {NAME}", flush=True)
-    write_native(NAME, folder)
+    from flint_amd.transport import species_tables
     md = mechs[NAME]
-    return folder, md, Mechanism.load(folder), Mechanism.from_data(md)
+    tb = species_tables(md)
+    write_native(NAME, folder, transport=(tb[2], tb[3]))
+    mm = Mechanism.from_data(md)
+    mm.transport_setup(tb)
+    return folder, md, Mechanism.load(folder), mm
 
 
 def _eq(a, b):
@@ -142,3 +146,7 @@ def test_fortran_drop_in_matches_c_abi(native, tmp_path):
     assert _eq(got["cons2prim"], prim.T)
     assert _eq(got["cons2prim_batch"], np.vstack([prim, Tq[None], sq[None]]).T)
     assert np.all(sq == 0)
+    r = fs["prim"][:ns]
+    mu, lam, stt = mm.transport_host(r, r.sum(0), fs["T0"])
+    assert _eq(got["co_k_mi_lam_Wilke"], np.vstack([mu, lam]).T)
+    assert _eq(got["transport_batch"], np.vstack([mu, lam, stt]).T) and np.all(stt == 0)
