@@ -41,7 +41,6 @@ static int put(flint_general_data* g, const std::vector<T>& v, const T** dst) {
 extern "C" int flint_general_setup(flint_mech* m, int na, int nt, int nl, const double* ni1, const double* ni2,
                                    const double* eps) {
   if (!m || !ni1 || !ni2 || !eps) return flint_fail(FLINT_EARG, "null argument");
-  if (m->is_wd) return flint_fail(FLINT_EUNSUPPORTED, "WD has its own routine (WD.f90); general needs rate tables");
   if (na != m->n_arrh || nt != m->n_troe || nl != m->n_lind)
     return flint_fail(FLINT_EARG, "reaction counts differ from the mechanism's rate tables");
   const int ns = m->ns, nz = ns + 1, nr = na + nt + nl;
@@ -61,11 +60,11 @@ extern "C" int flint_general_setup(flint_mech* m, int na, int nt, int nl, const 
         return flint_fail(FLINT_EARG, "stoichiometric coefficient beyond 64");
       if (a1[s] != 0.0) {
         fs.push_back(s);
-        fe.push_back(k == 0 ? std::nearbyint(a1[s]) : a1[s]);  // nint() for Arrhenius (Lib_Chemistry_wdot.f90:135)
+        fe.push_back(k == 0 ? std::round(a1[s]) : a1[s]);  // nint(): halves away from zero (Lib_Chemistry_wdot.f90:140-141)
       }
       if (a2[s] != 0.0) {
         rs.push_back(s);
-        re.push_back(k == 0 ? std::nearbyint(a2[s]) : a2[s]);
+        re.push_back(k == 0 ? std::round(a2[s]) : a2[s]);
       }
       if (e[s] != 0.0) {
         ms.push_back(s);

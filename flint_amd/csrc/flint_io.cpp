@@ -248,7 +248,7 @@ extern "C" int flint_mech_load(const char* folder, const char* prefix, flint_mec
                              nt ? troe.data() : nullptr, nl, nl ? lind.data() : nullptr, Th_min, Th_max, cp.data(),
                              h.data(), ne, ne ? comp.data() : nullptr, out);
   if (rc) return rc;
-  if (general && !(*out)->is_wd) {
+  if (general) {
     if (int rc2 = flint_general_setup(*out, na, nt, nl, g1.data(), g2.data(), ge.data())) {
       flint_mech_destroy(*out);
       *out = nullptr;

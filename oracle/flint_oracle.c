@@ -96,8 +96,8 @@ void orc_general(const orc_mech *m, int na, int nt, int nl, const double *ni1, c
       double pf = 1.0, pb = 1.0;
       for (int k = 0; k < ns; ++k) {
         if (j < na) {
-          if (a1[k] != 0) pf = pf * powi_r(coi[k], (int)lrint(a1[k]));
-          if (a2[k] != 0) pb = pb * powi_r(coi[k], (int)lrint(a2[k]));
+          if (a1[k] != 0) pf = pf * powi_r(coi[k], (int)lround(a1[k]));
+          if (a2[k] != 0) pb = pb * powi_r(coi[k], (int)lround(a2[k]));
         } else {
           if (a1[k] != 0) pf = pf * pow(coi[k], a1[k]);
           if (a2[k] != 0) pb = pb * pow(coi[k], a2[k]);
@@ -383,12 +383,18 @@ static void jac_core(const orc_mech *m, const double *nu, const double *roi, dou
   }
 }
 
-static void jac_cell(const orc_mech *m, const double *nu, const double *Z, double *F, double *J, double *Jabs) {
+/* Fabs (optional): the scale of F for the normwise comparison, the sum of the absolute values of
+ * the terms each component sums: W_k sum_r |nu_kr| (|prodf_r| + |prodb_r|) for the species and
+ * sum_k |e_k| Fabs_k / |D| for T. */
+static void jac_cell(const orc_mech *m, const double *nu, const double *Z, double *F, double *J, double *Jabs,
+                     double *Fabs) {
   int ns = m->ns, nz = ns + 1;
   double roi[ORC_MAX_NS], dclip[ORC_MAX_NS];
   double T = Z[ns];
   rhs_cell_r(m, Z, F);
   for (int a = 0; a < nz * nz; ++a) { J[a] = 0.0; Jabs[a] = 0.0; }
+  if (Fabs)
+    for (int k = 0; k < nz; ++k) Fabs[k] = 0.0;
   if (F[ns] == -1.0 && (T < m->Th_min || T > m->Th_max || isnan(T))) return;
   for (int k = 0; k < ns; ++k) {
     roi[k] = Z[k] < 0.0 ? 0.0 : Z[k];
@@ -420,6 +426,12 @@ static void jac_cell(const orc_mech *m, const double *nu, const double *Z, doubl
     s_de += (slope(m->h, rowh, ns, k) - Ri) * F[k];
     s_de_abs += fabs((slope(m->h, rowh, ns, k) - Ri) * F[k]);
   }
+  if (Fabs) {
+    double wt[ORC_MAX_NS], pf[ORC_MAX_NR], pb[ORC_MAX_NR], qa = 0.0;
+    wdot_cell_r(m, roi, T, wt, pf, pb, Fabs);
+    for (int k = 0; k < ns; ++k) qa += fabs(e[k]) * Fabs[k];
+    Fabs[ns] = qa / fabs(D);
+  }
   for (int j = 0; j < nz; ++j) {
     double acc = 0.0, acca = 0.0;
     for (int k = 0; k < ns; ++k) { acc += e[k] * J[k * nz + j]; acca += fabs(e[k]) * Jabs[k * nz + j]; }
@@ -433,7 +445,8 @@ static void jac_cell(const orc_mech *m, const double *nu, const double *Z, doubl
   }
 }
 
-void orc_jac(const orc_mech *m, long n, long ld, const double *Z, double *F, double *jac, double *jabs) {
+void orc_jac(const orc_mech *m, long n, long ld, const double *Z, double *F, double *jac, double *jabs,
+             double *fabs_out) {
   int ns = m->ns, nz = ns + 1;
   if (m->is_wd) return; /* WD has fractional orders: no Jacobian program */
   /* nu_kr from the net-production program */
@@ -443,10 +456,12 @@ void orc_jac(const orc_mech *m, long n, long ld, const double *Z, double *F, dou
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < n; ++i) {
     double z[ORC_MAX_NS + 1], f[ORC_MAX_NS + 1];
-    double J[(ORC_MAX_NS + 1) * (ORC_MAX_NS + 1)], Ja[(ORC_MAX_NS + 1) * (ORC_MAX_NS + 1)];
+    double J[(ORC_MAX_NS + 1) * (ORC_MAX_NS + 1)], Ja[(ORC_MAX_NS + 1) * (ORC_MAX_NS + 1)], fa[ORC_MAX_NS + 1];
     for (int k = 0; k < nz; ++k) z[k] = Z[k * ld + i];
-    jac_cell(m, nu, z, f, J, Ja);
+    jac_cell(m, nu, z, f, J, Ja, fabs_out ? fa : 0);
     for (int k = 0; k < nz; ++k) F[k * ld + i] = f[k];
+    if (fabs_out)
+      for (int k = 0; k < nz; ++k) fabs_out[k * ld + i] = fa[k];
     for (int a = 0; a < nz * nz; ++a) {
       jac[(long)a * ld + i] = J[a];
       if (jabs) jabs[(long)a * ld + i] = Ja[a];
@@ -462,11 +477,19 @@ void orc_jac(const orc_mech *m, long n, long ld, const double *Z, double *F, dou
  *   dF_T/dT   = -(sum_k h'_k wdot_k + h_k dwdot_k/dT)/(rho cp) - F_T (sum_k Y_k cp'_k/cp - 1/T)
  * with dwdot_k/dY_j = rho g_j (A_kj - (A Y)_k R_j/R), dwdot_k/dT = b_k - (A rho)_k/T, R = R_mix,
  * and g_j = 1 (the constant-pressure model does not clip Y at 0). */
-static void jac_cp_cell(const orc_mech *m, const double *nu, const double *x, double p, double *F, double *J) {
+/* Ja, Fa (optional): the scales of J and F for the normwise comparison, each formula above with
+ * every term replaced by its absolute value and every inner sum by the sum of |terms|: Aa, ba from
+ * jac_core, W_k sum_r |nu_kr| (|prodf_r| + |prodb_r|) for wdot_k. */
+static void jac_cp_cell(const orc_mech *m, const double *nu, const double *x, double p, double *F, double *J,
+                        double *Fa, double *Ja) {
   int ns = m->ns, nz = ns + 1;
   double T = x[ns];
   rhs_cp_cell_r(m, x, p, F);
   for (int a = 0; a < nz * nz; ++a) J[a] = 0.0;
+  if (Ja)
+    for (int a = 0; a < nz * nz; ++a) Ja[a] = 0.0;
+  if (Fa)
+    for (int k = 0; k < nz; ++k) Fa[k] = 0.0;
   if (T < m->Th_min || T > m->Th_max || isnan(T)) return;
   double Yc[ORC_MAX_NS], g[ORC_MAX_NS], R[ORC_MAX_NS], roi[ORC_MAX_NS], Rmix = 0.0;
   for (int k = 0; k < ns; ++k) {
@@ -511,6 +534,36 @@ static void jac_cp_cell(const orc_mech *m, const double *nu, const double *x, do
   }
   for (int j = 0; j < ns; ++j) J[ns * nz + j] = -colh[j] / (rho * cpm) - FT * g[j] * (cp[j] / cpm - R[j] / Rmix);
   J[ns * nz + ns] = -(s_dhw + colh[ns]) / (rho * cpm) - FT * (sYdcp / cpm - 1.0 / T);
+  if (!Fa && !Ja) return;
+  double wt[ORC_MAX_NS], pf[ORC_MAX_NR], pb[ORC_MAX_NR], wa[ORC_MAX_NS];
+  wdot_cell_r(m, roi, T, wt, pf, pb, wa);
+  double colha[ORC_MAX_NS + 1], s_dhwa = 0.0, Qa = 0.0, sYdcpa = 0.0, cpa = fabs(cpm);
+  for (int j = 0; j < nz; ++j) colha[j] = 0.0;
+  double Jk[ORC_MAX_NS + 1];
+  for (int k = 0; k < ns; ++k) {
+    double AYa = 0.0;
+    for (int j = 0; j < ns; ++j) AYa += Aa[k * ns + j] * fabs(Yc[j]);
+    for (int j = 0; j < ns; ++j) {
+      double dwdYa = rho * (Aa[k * ns + j] + AYa * R[j] / Rmix);
+      Jk[j] = Aa[k * ns + j] + AYa * R[j] / Rmix + (wa[k] / rho) * R[j] / Rmix;
+      colha[j] += fabs(h[k]) * dwdYa;
+    }
+    double dwdTa = ba[k] + rho * AYa / T;
+    Jk[ns] = dwdTa / rho + wa[k] / (rho * T);
+    colha[ns] += fabs(h[k]) * dwdTa;
+    s_dhwa += fabs(dh[k]) * wa[k];
+    Qa += fabs(h[k]) * wa[k];
+    sYdcpa += fabs(Yc[k] * dcp[k]);
+    if (Ja)
+      for (int j = 0; j < nz; ++j) Ja[k * nz + j] = Jk[j];
+    if (Fa) Fa[k] = wa[k] / rho;
+  }
+  double FTa = Qa / (rho * cpa);
+  if (Fa) Fa[ns] = FTa;
+  if (Ja) {
+    for (int j = 0; j < ns; ++j) Ja[ns * nz + j] = colha[j] / (rho * cpa) + FTa * (fabs(cp[j]) / cpa + R[j] / Rmix);
+    Ja[ns * nz + ns] = (s_dhwa + colha[ns]) / (rho * cpa) + FTa * (sYdcpa / cpa + 1.0 / T);
+  }
 }
 
 double *orc_nu_alloc(const orc_mech *m) {
@@ -530,9 +583,9 @@ void orc_rhs_cell(const orc_mech *m, int mode, const double *x, double p, double
 void orc_jac_cell(const orc_mech *m, const double *nu, int mode, const double *x, double p, double *F, double *J) {
   if (mode == 0) {
     double Ja[(ORC_MAX_NS + 1) * (ORC_MAX_NS + 1)];
-    jac_cell(m, nu, x, F, J, Ja);
+    jac_cell(m, nu, x, F, J, Ja, 0);
   } else {
-    jac_cp_cell(m, nu, x, p, F, J);
+    jac_cp_cell(m, nu, x, p, F, J, 0, 0);
   }
 }
 
@@ -547,17 +600,23 @@ void orc_rhs_cp(const orc_mech *m, long n, long ld, const double *X, const doubl
   }
 }
 
-void orc_jac_cp(const orc_mech *m, long n, long ld, const double *X, const double *p, double *F, double *jac) {
+void orc_jac_cp(const orc_mech *m, long n, long ld, const double *X, const double *p, double *F, double *jac,
+                double *fabs_out, double *jabs) {
   int nz = m->ns + 1;
   if (m->is_wd) return;
   double *nu = orc_nu_alloc(m);
 #pragma omp parallel for schedule(static)
   for (long i = 0; i < n; ++i) {
     double z[ORC_MAX_NS + 1], f[ORC_MAX_NS + 1], J[(ORC_MAX_NS + 1) * (ORC_MAX_NS + 1)];
+    double fa[ORC_MAX_NS + 1], Ja[(ORC_MAX_NS + 1) * (ORC_MAX_NS + 1)];
     for (int k = 0; k < nz; ++k) z[k] = X[k * ld + i];
-    jac_cp_cell(m, nu, z, p[i], f, J);
+    jac_cp_cell(m, nu, z, p[i], f, J, fabs_out ? fa : 0, jabs ? Ja : 0);
     for (int k = 0; k < nz; ++k) F[k * ld + i] = f[k];
     for (int a = 0; a < nz * nz; ++a) jac[(long)a * ld + i] = J[a];
+    if (fabs_out)
+      for (int k = 0; k < nz; ++k) fabs_out[k * ld + i] = fa[k];
+    if (jabs)
+      for (int a = 0; a < nz * nz; ++a) jabs[(long)a * ld + i] = Ja[a];
   }
   free(nu);
 }

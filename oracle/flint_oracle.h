@@ -49,13 +49,15 @@ void orc_wd_hrr(const orc_mech *m, long n, long ld, const double *rho, const dou
 /* constant-volume reactor RHS (rhs_native); Z = [rho_1..rho_ns, T] SoA with nz = ns+1 rows */
 void orc_rhs(const orc_mech *m, long n, long ld, const double *Z, double *F);
 /* analytic Jacobian dF/dZ of orc_rhs, jac[(r*nz+c)*ld+i]; jabs = sum of |partial terms| */
-void orc_jac(const orc_mech *m, long n, long ld, const double *Z, double *F, double *jac, double *jabs);
+void orc_jac(const orc_mech *m, long n, long ld, const double *Z, double *F, double *jac, double *jabs,
+             double *fabs_out);
 /* complex-step Jacobian of orc_rhs (column c), step h: d F/d Z_c = Im F(Z + i h e_c)/h */
 void orc_jac_cstep(const orc_mech *m, long n, long ld, const double *Z, double h, double *jac);
 
 /* constant-pressure reactor: X = [Y_1..Y_ns, T] SoA, p[n] in Pa */
 void orc_rhs_cp(const orc_mech *m, long n, long ld, const double *X, const double *p, double *F);
-void orc_jac_cp(const orc_mech *m, long n, long ld, const double *X, const double *p, double *F, double *jac);
+void orc_jac_cp(const orc_mech *m, long n, long ld, const double *X, const double *p, double *F, double *jac,
+                double *fabs_out, double *jabs);
 void orc_jac_cp_cstep(const orc_mech *m, long n, long ld, const double *X, const double *p, double h, double *jac);
 
 /* single-cell helpers used by the BDF oracle (mode 0 = constant volume, 1 = constant pressure) */

@@ -160,15 +160,18 @@ class Oracle:
         self.L.orc_rhs(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(Z), _ptr(F))
         return F
 
-    def jac(self, Z):
-        """Z [nz, n] -> (F [nz, n], J [nz*nz, n] row-major per cell, |J| scale)."""
+    def jac(self, Z, fscale=False):
+        """Z [nz, n] -> (F [nz, n], J [nz*nz, n] row-major per cell, |J| scale) and, with fscale,
+        the scale of F (sum of |terms| of each component) as a fourth result."""
         Z = np.ascontiguousarray(Z, dtype=np.float64)
         nz, n = Z.shape
         F = np.empty_like(Z)
         J = np.empty((nz * nz, n))
         Ja = np.empty((nz * nz, n))
-        self.L.orc_jac(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(Z), _ptr(F), _ptr(J), _ptr(Ja))
-        return F, J, Ja
+        Fa = np.empty_like(Z) if fscale else None
+        self.L.orc_jac(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(Z), _ptr(F), _ptr(J), _ptr(Ja),
+                       None if Fa is None else _ptr(Fa))
+        return (F, J, Ja, Fa) if fscale else (F, J, Ja)
 
     def jac_cstep(self, Z, h=1e-30):
         Z = np.ascontiguousarray(Z, dtype=np.float64)
@@ -185,14 +188,19 @@ class Oracle:
         self.L.orc_rhs_cp(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(X), _ptr(p), _ptr(F))
         return F
 
-    def jac_cp(self, X, p):
+    def jac_cp(self, X, p, scales=False):
+        """Constant-pressure right-hand side and Jacobian: X [nz, n] = (Y_k, T), p [n] ->
+        (F, J) and, with scales, (F, J, |F| scale, |J| scale)."""
         X = np.ascontiguousarray(X, dtype=np.float64)
         p = np.ascontiguousarray(p, dtype=np.float64)
         nz, n = X.shape
         F = np.empty_like(X)
         J = np.empty((nz * nz, n))
-        self.L.orc_jac_cp(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(X), _ptr(p), _ptr(F), _ptr(J))
-        return F, J
+        Fa = np.empty_like(X) if scales else None
+        Ja = np.empty((nz * nz, n)) if scales else None
+        self.L.orc_jac_cp(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(X), _ptr(p), _ptr(F), _ptr(J),
+                          None if Fa is None else _ptr(Fa), None if Ja is None else _ptr(Ja))
+        return (F, J, Fa, Ja) if scales else (F, J)
 
     def jac_cp_cstep(self, X, p, h=1e-30):
         X = np.ascontiguousarray(X, dtype=np.float64)

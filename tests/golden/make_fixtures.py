@@ -115,6 +115,22 @@ def general():
     w = O.Reference(md).general(rho, T)
     np.savez_compressed(os.path.join(OUT, "general_Pelucchi.npz"), rho=rho, T=T, wdot=w)
     print("Pelucchi general", n)
+    general_wd()
+
+
+def general_wd():
+    """WD through the general routine: its tables hold the half-integer stoichiometric coefficients
+    0.5 and 1.5 (CO + 0.5 O2, CH4 + 1.5 O2), which the routine raises to nint() powers."""
+    md = mech_data("WD")
+    seed, start, count = 23, 0, 1024
+    st = S.wd_states(md.species, md.W, seed, start, count)
+    rho = S.soa(st["rho"]).copy()
+    # the blend generator holds no CO; seed it so that CO + 0.5 O2 runs at a nonzero rate
+    ico = md.species.index("CO")
+    rho[ico] = rho.sum(0) * 10.0 ** np.random.default_rng(seed).uniform(-4, -1, count)
+    w = O.Reference(md).general(rho, st["T"])
+    np.savez_compressed(os.path.join(OUT, "general_WD.npz"), rho=rho, T=st["T"], wdot=w)
+    print("WD general", count)
 
 
 C2P_CASES = {"TSR-GP-24": (15, 0, 512, 3), "TSR-Rich-31": (16, 0, 512, 2)}

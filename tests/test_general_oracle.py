@@ -66,6 +66,19 @@ def test_general_oracle_matches_reference_outputs_pelucchi(oracle_mod):
     assert np.array_equal(w, g["wdot"])
 
 
+def test_general_oracle_half_integer_coefficients(oracle_mod):
+    """WD through general: its tables hold the stoichiometric coefficients 0.5 and 1.5, raised to
+    nint() powers (Lib_Chemistry_wdot.f90:140-141), which round halves away from zero
+    (nint(0.5) = 1).  The reference's outputs (tests/golden/general_WD.npz, CO seeded so that
+    CO + 0.5 O2 runs) are matched bit for bit; rounding halves to even (0.5 -> 0) misses them by
+    O(1) normwise."""
+    from flint_amd.mechdata import mech_data
+    g = golden("general_WD.npz")
+    w, sc = oracle_mod.Oracle(mech_data("WD")).general(g["rho"], g["T"])
+    assert np.array_equal(w, g["wdot"])
+    assert np.all(np.abs(g["wdot"]).max(1) > 0)
+
+
 @pytest.mark.skipif(not os.path.exists(REF_LIB), reason="reference build (make -C oracle ref) absent")
 def test_general_oracle_equals_reference_build(oracle_mod, mechs):
     from flint_amd.mechdata import mech_data

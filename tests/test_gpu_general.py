@@ -43,6 +43,26 @@ def test_general_matches_reference_outputs(mechs, oracle_mod, cuda, mech):
     print(f"[gpu] identical to the reference outputs: {np.mean(wg == g['wdot']):.4f} of entries", flush=True)
 
 
+def test_general_half_integer_coefficients(oracle_mod, cuda):
+    """WD's general tables carry the coefficients 0.5 and 1.5, raised to nint() powers
+    (Lib_Chemistry_wdot.f90:140-141, halves away from zero): against the reference's outputs."""
+    import torch
+    from flint_amd.mechdata import mech_data
+    from flint_amd.native import Mechanism
+    md = mech_data("WD")
+    m = Mechanism.from_data(md)
+    m.general_setup(md.general)
+    g = golden("general_WD.npz")
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    n = g["T"].size
+    w = torch.empty((md.ns, n), dtype=torch.float64, device=cuda)
+    print(f"[gpu] general WD n={n}", flush=True)
+    m.source_general(dev(g["rho"]), dev(g["T"]), w)
+    torch.cuda.synchronize()
+    _, sc = oracle_mod.Oracle(md).general(g["rho"], g["T"])
+    assert np.max(np.abs(w.cpu().numpy() - g["wdot"]) / np.maximum(sc, 1e-300)) <= 1e-12
+
+
 def test_general_edges(mechs, oracle_mod, cuda):
     import torch
     from flint_amd.native import Mechanism, FlintError
