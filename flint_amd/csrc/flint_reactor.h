@@ -27,6 +27,7 @@ struct Opts {
   int mode;  // 0 constant volume (rho_k, T), 1 constant pressure (Y_k, T)
   double rtol, atol, t_end;
   int max_steps;
+  int newton;  // Newton policy: 0 as scipy's BDF, 1 as SUNDIALS CVODE (reactor.hip)
 };
 
 }  // namespace flint_reactor

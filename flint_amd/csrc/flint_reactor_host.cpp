@@ -22,6 +22,7 @@ struct flint_extra {
   int teams_per_block = 4;
   int blocks = 0;
   bool tuned = false;
+  int newton = 1;  // Newton policy (flint_reactor_set_newton)
   // captured graph of the last flint_reactor_advance_graph call
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -233,7 +234,7 @@ static int launch(flint_mech* m, long n, long ld, int mode, double* X, const dou
                   double atol, int max_steps, int* stats, int* status, hipStream_t s, int tpb, int blocks) {
   flint_extra* x = m->extra;
   if (int rc = ensure_jac(x, tpb, blocks)) return rc;
-  flint_reactor::Opts O{mode, rtol, atol, t_end, max_steps > 0 ? max_steps : 20000};
+  flint_reactor::Opts O{mode, rtol, atol, t_end, max_steps > 0 ? max_steps : 20000, x->newton};
   hipError_t e = flint_reactor_launch(x->prog, O, n, ld, X, p, stats, status, x->d_counter, tpb, blocks, x->d_jac, s);
   if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("reactor launch: ") + hipGetErrorString(e));
   return FLINT_OK;
@@ -270,6 +271,23 @@ extern "C" int flint_reactor_set_launch(flint_mech* m, int teams_per_block, int 
   m->extra->teams_per_block = teams_per_block;
   m->extra->blocks = blocks;
   m->extra->tuned = true;
+  return FLINT_OK;
+}
+
+extern "C" int flint_reactor_set_newton(flint_mech* m, int policy) {
+  if (!m) return flint_fail(FLINT_EARG, "null mechanism");
+  if (policy != 0 && policy != 1) return flint_fail(FLINT_EARG, "Newton policy must be 0 (scipy) or 1 (CVODE)");
+  if (int rc = ensure_extra(m)) return rc;
+  if (m->extra->newton != policy) {
+    // a captured graph holds the old options
+    flint_extra* x = m->extra;
+    if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
+    if (x->graph) (void)hipGraphDestroy(x->graph);
+    x->gexec = nullptr;
+    x->graph = nullptr;
+    x->gkey.clear();
+  }
+  m->extra->newton = policy;
   return FLINT_OK;
 }
 

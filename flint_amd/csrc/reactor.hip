@@ -3,8 +3,16 @@
 //
 // Method: the variable-order (1..5) quasi-constant step BDF of Shampine & Reichelt ("The MATLAB
 // ODE Suite", SIAM J. Sci. Comput. 18, 1997; kappa = 0), modified Newton with the analytic
-// Jacobian refreshed only on Newton failure, RMS error norm of e/(atol + rtol|y|).  It is the same
-// algorithm, decision for decision, as the CPU checker in oracle/oracle_bdf.c.  FLINT itself
+// Jacobian refreshed only on Newton failure, RMS error norm of e/(atol + rtol|y|).  Two Newton
+// policies (Opts::newton):
+//   0  scipy's: at most 4 iterations, the convergence rate measured within the step, converged
+//      when rate/(1-rate)|dy| < newton_tol; I - cJ refactored whenever h or the order changes;
+//   1  SUNDIALS CVODE's (the BDF behind FLINT's OSlo 'cvode' choice, test-batchF.f90:22-26): at
+//      most 3 iterations, the rate estimate carried over between steps (reset to 1 with a new
+//      matrix, decayed by 0.3 per iteration), converged when |dy| min(1, rate)/(q+1) <= 0.005,
+//      diverged when |dy| > 2|dy_prev|; the factorised I - c_lu J kept while |c/c_lu - 1| <= 0.3
+//      and younger than 20 converged attempts, its corrections scaled by 2/(1 + c/c_lu).
+// Either is the same algorithm, decision for decision, as the CPU checker in oracle/oracle_bdf.c.  FLINT itself
 // integrates through the external OSlo library (test-batchF.f90:81,105), absent from the
 // reference checkout.
 //
@@ -34,6 +42,10 @@ constexpr int MAXO = 5;
 constexpr int NEWTON_MAXITER = 4;
 constexpr int CW = 16;  // production-sum chunk width (the host encoder uses the same value)
 constexpr double MIN_FACTOR = 0.2, MAX_FACTOR = 10.0;
+// Newton policy 1 (CVODE's): iterations, matrix-reuse bounds, rate decay, divergence ratio, and
+// the convergence coefficient (CVODE's 0.1 tightened, see oracle/oracle_bdf.c)
+constexpr int CV_MAXCOR = 3, CV_MSBP = 20;
+constexpr double CV_DGMAX = 0.3, CV_CRDOWN = 0.3, CV_RDIV = 2.0, CV_NLSCOEF = 0.005;
 
 enum { ST_STEPS, ST_REJECT, ST_NFAIL, ST_RHS, ST_JAC, ST_LU, ST_NEWTON, ST_ORDER };
 
@@ -120,6 +132,9 @@ struct Lds {
   const int4* rrec;   // [R*32] reaction records
   const float* effT;  // [ns][nms] third-body efficiencies
   const int* kpart;   // [MAXP][32] chunk slots of each species
+  const int2* cterm;  // [C][CW][32] production-sum chunk terms
+  const double* W;    // [32] molar masses (1 beyond ns)
+  const double* Rk;   // [32] specific gas constants 8314.51 / W_k (0 beyond ns)
   // per team
   double* c;    // [32] concentrations; slot 31 (never a species) holds 1 for empty factors
   double* Mv;   // [nms] third-body sums
@@ -138,33 +153,51 @@ __host__ __device__ inline size_t team_lds_bytes(const Prog& P) {
          sizeof(double);
 }
 
-// Block-shared program bytes (16-byte multiples): records, efficiencies, chunk slots.
-__host__ __device__ inline size_t shared_prog_bytes(const Prog& P) {
-  const size_t rec = (size_t)P.R * 32 * 16;
-  const size_t eff = (((size_t)P.ns * (P.nms > 0 ? P.nms : 1) * 4) + 15) & ~size_t(15);
-  const size_t kp = (((size_t)P.MAXP * 32 * 4) + 15) & ~size_t(15);
-  return rec + eff + kp;
+// Block-shared program (16-byte multiples): records, efficiencies, chunk slots, chunk terms, molar
+// masses and gas constants.  Everything the right-hand side reads besides the state and the
+// tables sits here, so that an evaluation waits on global memory only for its table rows.
+struct SharedLayout {
+  size_t eff, kp, ct, w, total;
+};
+__host__ __device__ inline SharedLayout shared_layout(const Prog& P) {
+  SharedLayout o;
+  o.eff = (size_t)P.R * 32 * 16;
+  o.kp = o.eff + ((((size_t)P.ns * (P.nms > 0 ? P.nms : 1) * 4) + 15) & ~size_t(15));
+  o.ct = o.kp + ((((size_t)P.MAXP * 32 * 4) + 15) & ~size_t(15));
+  o.w = o.ct + (size_t)P.C * CW * TEAM * 8;
+  o.total = o.w + 2 * TEAM * 8;
+  return o;
 }
+__host__ __device__ inline size_t shared_prog_bytes(const Prog& P) { return shared_layout(P).total; }
 
 // Copies the program into the block's LDS (all threads of the block; a block barrier follows).
 __device__ inline void load_shared_prog(const Prog& P, char* base) {
-  const size_t eff_off = (size_t)P.R * 32 * 16;
-  const size_t kp_off = eff_off + ((((size_t)P.ns * (P.nms > 0 ? P.nms : 1) * 4) + 15) & ~size_t(15));
+  const SharedLayout o = shared_layout(P);
   int4* rec = reinterpret_cast<int4*>(base);
   for (int i = threadIdx.x; i < P.R * 32; i += blockDim.x) rec[i] = P.rrec[i];
-  float* eff = reinterpret_cast<float*>(base + eff_off);
+  float* eff = reinterpret_cast<float*>(base + o.eff);
   for (int i = threadIdx.x; i < P.ns * P.nms; i += blockDim.x) eff[i] = P.effT[i];
-  int* kp = reinterpret_cast<int*>(base + kp_off);
+  int* kp = reinterpret_cast<int*>(base + o.kp);
   for (int i = threadIdx.x; i < P.MAXP * 32; i += blockDim.x) kp[i] = P.kpart[i];
+  int2* ct = reinterpret_cast<int2*>(base + o.ct);
+  for (int i = threadIdx.x; i < P.C * CW * TEAM; i += blockDim.x) ct[i] = P.cterm[i];
+  double* w = reinterpret_cast<double*>(base + o.w);
+  for (int i = threadIdx.x; i < TEAM; i += blockDim.x) {
+    const bool sp = i < P.ns;
+    w[i] = sp ? P.W[i] : 1.0;
+    w[TEAM + i] = sp ? 8314.51 / P.W[i] : 0.0;
+  }
 }
 
 __device__ inline Lds carve(char* shared, char* base, const Prog& P) {
   Lds L;
-  const size_t eff_off = (size_t)P.R * 32 * 16;
-  const size_t kp_off = eff_off + ((((size_t)P.ns * (P.nms > 0 ? P.nms : 1) * 4) + 15) & ~size_t(15));
+  const SharedLayout o = shared_layout(P);
   L.rrec = reinterpret_cast<const int4*>(shared);
-  L.effT = reinterpret_cast<const float*>(shared + eff_off);
-  L.kpart = reinterpret_cast<const int*>(shared + kp_off);
+  L.effT = reinterpret_cast<const float*>(shared + o.eff);
+  L.kpart = reinterpret_cast<const int*>(shared + o.kp);
+  L.cterm = reinterpret_cast<const int2*>(shared + o.ct);
+  L.W = reinterpret_cast<const double*>(shared + o.w);
+  L.Rk = L.W + TEAM;
   double* d = reinterpret_cast<double*>(base);
   L.c = d; d += 32;
   L.aux = d; d += 32;
@@ -277,8 +310,8 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   const double T = tbc(x, ns);
   if (!(T >= (double)P.Th_min && T < (double)P.Th_max && T >= (double)P.Tk_min && T < (double)P.Tk_max)) return o;
   const bool sp = lane < ns;
-  const double Wk = sp ? P.W[lane] : 1.0;
-  const double Rk = sp ? 8314.51 / Wk : 0.0;
+  const double Wk = L.W[lane];
+  const double Rk = L.Rk[lane];
   // constant volume clips rho at 0 as rhs_native does (Lib_Chemistry_rhs.f90:37-39); the
   // constant-pressure model keeps Y as it is, so its right-hand side stays smooth through Y = 0
   const bool clip = mode == 0 && x < 0.0;
@@ -338,7 +371,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   for (int c2 = 0; c2 < P.C; c2 += 2) {
     const int c = c2 + half();
     if (c < P.C) {
-      const int2* __restrict__ ct = P.cterm + (long)c * CW * TEAM + lane;
+      const int2* __restrict__ ct = L.cterm + c * CW * TEAM + lane;
       int2 tm[CW];
 #pragma unroll
       for (int t = 0; t < CW; ++t) tm[t] = ct[t * TEAM];
@@ -409,7 +442,7 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
   __threadfence_block();
   tsync();
   double bT = 0.0;
-  const double Wk = sp ? P.W[lane] : 1.0;
+  const double Wk = L.W[lane];
   if (sp && h0) {
     const int t0 = P.wd_ptr[lane], t1 = P.wd_ptr[lane + 1];
 #pragma unroll 2
@@ -440,10 +473,10 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
       bT += coef * Jg[3 * nr + r];
     }
     // species block A_kj = d wdot_k / d rho_j (unclipped), b_k = d wdot_k / dT at fixed rho
-    for (int j = 0; j < ns; ++j) row[j] *= Wk / P.W[j];
+    for (int j = 0; j < ns; ++j) row[j] *= Wk / L.W[j];
   }
   const double bk = Wk * bT;
-  const double Rk = sp ? 8314.51 / Wk : 0.0;
+  const double Rk = L.Rk[lane];
   if (mode == 0) {
     // constant volume: J_kj = g_j A_kj, J_kT = b_k; T row from column sums of e_k J_kj
     if (sp && h0) {
@@ -470,7 +503,7 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
     if (sp && h0) {
       for (int j = 0; j < ns; ++j) AY += row[j] * L.aux[j];
       // stage dwdot_k/dY_j and dwdot_k/dT for the enthalpy-weighted column sums
-      for (int j = 0; j < ns; ++j) row[j] = rho * L.g[j] * (row[j] - AY * (8314.51 / P.W[j]) / Rm);
+      for (int j = 0; j < ns; ++j) row[j] = rho * L.g[j] * (row[j] - AY * L.Rk[j] / Rm);
       row[ns] = bk - rho * AY / T;
     }
     const double s_dhw = tsum(sp ? o.dh * o.w : 0.0);
@@ -487,7 +520,7 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
     }
     tsync();
     if (sp && h0) {
-      for (int j = 0; j < ns; ++j) row[j] = row[j] / rho + L.g[j] * (o.w / rho) * (8314.51 / P.W[j]) / Rm;
+      for (int j = 0; j < ns; ++j) row[j] = row[j] / rho + L.g[j] * (o.w / rho) * L.Rk[j] / Rm;
       row[ns] = row[ns] / rho + o.w / (rho * T);
     }
     if (lane < nz) L.lu[ns * 33 + lane] = jt;
@@ -668,6 +701,10 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
   double t_new = 0.0, ypred = 0.0, psi = 0.0, scale = 0.0, cc = 0.0, ynew = 0.0, d = 0.0;
   double dy_norm_old = -1.0, err_norm = 0.0, safety = 0.9;
   int k_it = 0;
+  // Newton policy 1: coefficient of the factorised matrix, its age, the carried rate estimate
+  const bool cv = O.newton == 1;
+  double c_lu = 0.0, crate = 1.0;
+  int lu_age = 0;
 
   int phase = PH_F0;
   double xe = y;
@@ -721,6 +758,29 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
       int n_iter = k_it + 1;
       if (!o.ok || !isfinite(tsum(o.F))) {
         fail = true;
+      } else if (cv) {
+        const double dy = team_solve(mystep, cc * o.F - psi - d, nz, L) * (2.0 / (1.0 + cc / c_lu));
+        st[ST_NEWTON]++;
+        const double dy_norm = team_rms(dy, scale, nz);
+        if (k_it > 0 && dy_norm > CV_RDIV * dy_norm_old) {
+          fail = true;
+        } else {
+          if (k_it > 0) crate = fmax(CV_CRDOWN * crate, dy_norm / dy_norm_old);
+          ynew += dy;
+          d += dy;
+          if (dy_norm * fmin(1.0, crate) * (1.0 / (order + 1)) <= CV_NLSCOEF) {
+            conv = true;
+          } else {
+            dy_norm_old = dy_norm;
+            ++k_it;
+            if (k_it < CV_MAXCOR) {
+              xe = ynew;
+            } else {
+              fail = true;
+              n_iter = k_it + 1;
+            }
+          }
+        }
       } else {
         const double dy = team_solve(mystep, cc * o.F - psi - d, nz, L);
         st[ST_NEWTON]++;
@@ -754,6 +814,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
           go = GO_HALVE;
         }
       } else if (conv) {
+        if (cv) lu_age++;
         safety = 0.9 * (2 * NEWTON_MAXITER + 1) / (double)(2 * NEWTON_MAXITER + n_iter);
         const double scale2 = atol + rtol * fabs(ynew);
         sc = scale2;
@@ -777,7 +838,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         h_abs *= 0.5;
         change_D(D, order, 0.5);
         n_equal = 0;
-        have_lu = false;
+        if (!cv) have_lu = false;
         go = GO_ATTEMPT;
       } else if (go == GO_ACCEPT) {
         st[ST_STEPS]++;
@@ -811,7 +872,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         h_abs *= factor;
         change_D(D, order, factor);
         n_equal = 0;
-        have_lu = false;
+        if (!cv) have_lu = false;
       } else if (go == GO_STEP) {
         if (!(t < t_end)) {
           st[ST_ORDER] = order;
@@ -834,7 +895,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
           t_new = t_end;
           change_D(D, order, fabs(t_new - t) / h_abs);
           n_equal = 0;
-          have_lu = false;
+          if (!cv) have_lu = false;
         }
         const double h = t_new - t;
         h_abs = fabs(h);
@@ -855,10 +916,14 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         cc = h / alpha_o;
         go = GO_NEWTON;
       } else {  // GO_NEWTON: (re)start the Newton iteration from the predictor
+        if (cv && have_lu && (fabs(cc / c_lu - 1.0) > CV_DGMAX || lu_age >= CV_MSBP)) have_lu = false;
         if (!have_lu) {
           mystep = team_inv(cc, nz, L, Jg);
           st[ST_LU]++;
           have_lu = true;
+          c_lu = cc;
+          lu_age = 0;
+          crate = 1.0;
         }
         ynew = ypred;
         d = 0.0;

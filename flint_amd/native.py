@@ -43,6 +43,7 @@ def lib():
         L.flint_reactor_autotune.argtypes = [vp, l, l, i, vp, vp, d, d, d, C.POINTER(d)]
         L.flint_reactor_set_launch.argtypes = [vp, i, i]
         L.flint_reactor_get_launch.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
+        L.flint_reactor_set_newton.argtypes = [vp, i]
         L.flint_cea_setup.argtypes = [vp, vp, vp]
         L.flint_cea_solve.argtypes = [vp, l, l, i, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.flint_source_host.argtypes = [vp, l, vp, vp, vp]
@@ -245,6 +246,10 @@ class Mechanism:
 
     def set_launch(self, teams_per_block, blocks):
         check(lib().flint_reactor_set_launch(self._h, teams_per_block, blocks))
+
+    def set_newton(self, policy):
+        """Newton policy of the reactor's BDF: 1 as SUNDIALS CVODE (default), 0 as scipy's BDF."""
+        check(lib().flint_reactor_set_newton(self._h, policy))
 
     def get_launch(self):
         a, b = C.c_int(), C.c_int()

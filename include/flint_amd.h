@@ -183,6 +183,13 @@ int flint_reactor_autotune(flint_mech *m, long n, long ld, int mode, const doubl
 int flint_reactor_set_launch(flint_mech *m, int teams_per_block, int blocks);
 int flint_reactor_get_launch(flint_mech *m, int *teams_per_block, int *blocks);
 
+/* Newton policy of the reactor's BDF for this handle (the setup_odesolver solver choice of
+ * test-batchF.f90:81, whose 'cvode' is SUNDIALS CVODE): policy 1 (the default) iterates as CVODE
+ * does (rate estimate carried between steps, I - cJ reused while c changes by at most 30 %,
+ * at most 3 iterations); policy 0 as scipy's BDF (rate measured within the step, I - cJ
+ * refactored at every change of step or order, at most 4 iterations). */
+int flint_reactor_set_newton(flint_mech *m, int policy);
+
 /* Equilibrium data of a mechanism: CEA_initialize_global (Lib_CEA_setup.f90:6-99) for the
  * mechanism's species set.  s = s_tab(Th_min:Th_max, ns) per-mass entropy at 1 bar, column-major
  * like cp/h (read_idealgas_thermo, Load_ThermoTransport.f90:30-85); el_weight[ne] = atomic weights

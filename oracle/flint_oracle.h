@@ -87,6 +87,9 @@ void orc_laminar_viscosity(int ns, const double *W, int Tt_min, int Tt_max, cons
 /* BDF advance of n reactors over [0, t_end] (oracle_bdf.c); stats[i*8..] and status[i] per cell */
 void orc_bdf(const orc_mech *m, long n, long ld, int mode, double *X, const double *p, double rtol, double atol,
              double t_end, int max_steps, int *stats, int *status);
+/* orc_bdf with a Newton policy: 0 as scipy BDF (= orc_bdf), 1 as SUNDIALS CVODE (oracle_bdf.c) */
+void orc_bdf_method(const orc_mech *m, long n, long ld, int mode, double *X, const double *p, double rtol,
+                    double atol, double t_end, int max_steps, int *stats, int *status, int method);
 
 /* CEA-style equilibrium (oracle_cea.c).  A[ne][ns] = species_composition (element rows),
  * elw[ne] = atomic weights of the elements. */

@@ -234,15 +234,17 @@ class Oracle:
                        _ptr(T), _ptr(rho), None if g is None else _ptr(g), _ptr(To), _ptr(Yo), _ptr(it), _ptr(st))
         return To, Yo, it, st
 
-    def bdf(self, X, p=None, mode=0, rtol=1e-6, atol=1e-12, t_end=1e-4, max_steps=20000):
-        """Advance reactors (X [nz, n], mode 0 = (rho, T) constant volume, 1 = (Y, T) constant p)."""
+    def bdf(self, X, p=None, mode=0, rtol=1e-6, atol=1e-12, t_end=1e-4, max_steps=20000, newton=0):
+        """Advance reactors (X [nz, n], mode 0 = (rho, T) constant volume, 1 = (Y, T) constant p);
+        newton = 0 scipy's Newton policy, 1 CVODE's (oracle_bdf.c)."""
         X = np.array(X, dtype=np.float64, order="C", copy=True)
         nz, n = X.shape
         pp = np.ascontiguousarray(p if p is not None else np.zeros(n), dtype=np.float64)
         stats = np.zeros((n, 8), dtype=np.int32)
         status = np.zeros(n, dtype=np.int32)
-        self.L.orc_bdf(C.byref(self.m), C.c_long(n), C.c_long(n), C.c_int(mode), _ptr(X), _ptr(pp), C.c_double(rtol),
-                       C.c_double(atol), C.c_double(t_end), C.c_int(max_steps), _ptr(stats), _ptr(status))
+        self.L.orc_bdf_method(C.byref(self.m), C.c_long(n), C.c_long(n), C.c_int(mode), _ptr(X), _ptr(pp),
+                              C.c_double(rtol), C.c_double(atol), C.c_double(t_end), C.c_int(max_steps), _ptr(stats),
+                              _ptr(status), C.c_int(newton))
         return X, stats, status
 
     def num_threads(self):

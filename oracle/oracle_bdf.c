@@ -129,9 +129,28 @@ static void jac(ctx_t *c, const double *x, double *J) {
   orc_jac_cell(c->m, c->nu, c->mode, x, c->p, F, J);
 }
 
+/* Newton policies of the modified Newton iteration:
+ *   0  as scipy's BDF (Shampine & Reichelt): at most 4 iterations, the convergence rate measured
+ *      within the step (so at least two iterations), converged when rate/(1-rate) |dy| < newton_tol,
+ *      the iteration matrix refactored at every change of h or order;
+ *   1  as SUNDIALS CVODE's BDF (the integrator FLINT's OSlo wraps as 'cvode', test-batchF.f90:22-26):
+ *      at most 3 iterations, the rate estimate carried over from earlier steps (crate, reset to 1
+ *      when the matrix is refactored, decayed by 0.3 per iteration), converged when
+ *      |dy| min(1, crate) C_q <= 0.005 with C_q = 1/(q+1) the error-test constant, diverged when
+ *      |dy| > 2 |dy_prev|; the factorised I - c_lu J is kept while |c/c_lu - 1| <= 0.3 and fewer
+ *      than 20 converged attempts old, its corrections scaled by 2/(1 + c/c_lu).  CVODE's own
+ *      coefficient is 0.1; with it the Newton error enters the error estimate and the Rich-31
+ *      reactors reject 25x more steps, so the iteration is held 20x tighter. */
+#define CV_MAXCOR 3
+#define CV_DGMAX 0.3
+#define CV_MSBP 20
+#define CV_CRDOWN 0.3
+#define CV_RDIV 2.0
+#define CV_NLSCOEF 0.005
+
 /* Returns 0 on success, 1 step size underflow, 2 too many steps, 3 invalid state. */
 static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, double p, double rtol, double atol,
-                    double t_end, int max_steps, int *st) {
+                    double t_end, int max_steps, int *st, int method) {
   const int n = m->ns + 1;
   ctx_t c = {m, nu, mode, p, st};
   double gamma[MAXO + 1], alpha[MAXO + 1], err_c[MAXO + 2];
@@ -166,6 +185,9 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
   jac(&c, y, J);
   int current_jac = 0;
   double t = 0.0;
+  /* method 1: coefficient of the factorised matrix, steps since, carried rate estimate */
+  double c_lu = 0.0, crate = 1.0;
+  int lu_age = 0;
 
   while (t < t_end) {
     if (st[ST_STEPS] >= max_steps) return 2;
@@ -185,7 +207,7 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
         t_new = t_end;
         change_D(D, n, order, fabs(t_new - t) / h_abs);
         n_equal = 0;
-        have_lu = 0;
+        if (method == 0) have_lu = 0;
       }
       double h = t_new - t;
       h_abs = fabs(h);
@@ -201,24 +223,45 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
       int converged = 0, n_iter = 0;
       double cc = h / alpha[order];
       while (!converged) {
+        if (method == 1 && have_lu && (fabs(cc / c_lu - 1.0) > CV_DGMAX || lu_age >= CV_MSBP)) have_lu = 0;
         if (!have_lu) {
           for (int i = 0; i < n; ++i)
             for (int j = 0; j < n; ++j) LU[i * n + j] = (i == j ? 1.0 : 0.0) - cc * J[i * n + j];
           st[ST_LU]++;
           lu_factor(LU, n, piv);
           have_lu = 1;
+          c_lu = cc;
+          lu_age = 0;
+          crate = 1.0;
         }
+        const double corr = method == 1 ? 2.0 / (1.0 + cc / c_lu) : 1.0;
         /* modified Newton on  y - ypred - c f(y) + psi = 0, accumulated correction d */
         for (int i = 0; i < n; ++i) { ynew[i] = ypred[i]; d[i] = 0.0; }
         double dy_norm_old = -1.0;
         converged = 0;
         int k;
-        for (k = 0; k < NEWTON_MAXITER; ++k) {
+        const int maxit = method == 1 ? CV_MAXCOR : NEWTON_MAXITER;
+        for (k = 0; k < maxit; ++k) {
           if (fun(&c, ynew, fv)) break;
           for (int i = 0; i < n; ++i) dy[i] = cc * fv[i] - psi[i] - d[i];
           lu_solve(LU, n, piv, dy);
+          if (method == 1)
+            for (int i = 0; i < n; ++i) dy[i] *= corr;
           st[ST_NEWTON]++;
           double dy_norm = rms(dy, scale, n);
+          if (method == 1) {
+            if (k > 0) {
+              if (dy_norm > CV_RDIV * dy_norm_old) break;
+              crate = fmax(CV_CRDOWN * crate, dy_norm / dy_norm_old);
+            }
+            for (int i = 0; i < n; ++i) { ynew[i] += dy[i]; d[i] += dy[i]; }
+            if (dy_norm * fmin(1.0, crate) * err_c[order] <= CV_NLSCOEF) {
+              converged = 1;
+              break;
+            }
+            dy_norm_old = dy_norm;
+            continue;
+          }
           double rate = -1.0;
           if (dy_norm_old >= 0.0) rate = dy_norm / dy_norm_old;
           if (rate >= 0.0 && (rate >= 1.0 || pow(rate, NEWTON_MAXITER - k) / (1.0 - rate) * dy_norm > newton_tol))
@@ -243,9 +286,10 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
         h_abs *= 0.5;
         change_D(D, n, order, 0.5);
         n_equal = 0;
-        have_lu = 0;
+        if (method == 0) have_lu = 0;
         continue;
       }
+      if (method == 1) lu_age++;
       safety = 0.9 * (2 * NEWTON_MAXITER + 1) / (double)(2 * NEWTON_MAXITER + n_iter);
       for (int i = 0; i < n; ++i) {
         scale[i] = atol + rtol * fabs(ynew[i]);
@@ -295,7 +339,7 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
     h_abs *= factor;
     change_D(D, n, order, factor);
     n_equal = 0;
-    have_lu = 0;
+    if (method == 0) have_lu = 0;
   }
   st[ST_ORDER] = order;
   return 0;
@@ -305,8 +349,8 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
  * constant volume (rhs_native); mode 1: X = (Y_k, T), constant pressure p[i].  stats[i*8 + q]
  * counts steps, rejected steps, Newton failures, RHS, Jacobians, LU, Newton iterations, final
  * order; status[i] is 0 on success. */
-void orc_bdf(const orc_mech *m, long n, long ld, int mode, double *X, const double *p, double rtol, double atol,
-             double t_end, int max_steps, int *stats, int *status) {
+void orc_bdf_method(const orc_mech *m, long n, long ld, int mode, double *X, const double *p, double rtol,
+                    double atol, double t_end, int max_steps, int *stats, int *status, int method) {
   const int nz = m->ns + 1;
   double *nu = orc_nu_alloc(m);
 #pragma omp parallel for schedule(dynamic, 16)
@@ -314,10 +358,15 @@ void orc_bdf(const orc_mech *m, long n, long ld, int mode, double *X, const doub
     double y[NZMAX];
     int st[8] = {0};
     for (int k = 0; k < nz; ++k) y[k] = X[k * ld + i];
-    int rc = bdf_cell(m, nu, mode, y, p ? p[i] : 0.0, rtol, atol, t_end, max_steps, st);
+    int rc = bdf_cell(m, nu, mode, y, p ? p[i] : 0.0, rtol, atol, t_end, max_steps, st, method);
     for (int k = 0; k < nz; ++k) X[k * ld + i] = y[k];
     for (int q = 0; q < 8; ++q) stats[i * 8 + q] = st[q];
     status[i] = rc;
   }
   free(nu);
+}
+
+void orc_bdf(const orc_mech *m, long n, long ld, int mode, double *X, const double *p, double rtol, double atol,
+             double t_end, int max_steps, int *stats, int *status) {
+  orc_bdf_method(m, n, ld, mode, X, p, rtol, atol, t_end, max_steps, stats, status, 0);
 }

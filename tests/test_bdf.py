@@ -105,3 +105,37 @@ def test_bdf_oracle_accuracy_against_tight_radau(oracle_mod, mechs):
         ref = r.y[:, -1]
         assert abs(Xo[-1, i] - ref[-1]) < 0.5          # K
         assert np.max(np.abs(Xo[:-1, i] - ref[:-1])) < 1e-4  # mass fraction
+
+
+def test_cvode_newton_policy_invariants_and_accuracy(oracle_mod, mechs):
+    """Newton policy 1 (CVODE's: carried rate estimate, reused I - cJ) keeps the exact linear
+    invariants, converges to the tight Radau solution at rtol 1e-8 like policy 0, and at the
+    workload's rtol 1e-6 stays within the same error envelope against a rtol 1e-10 solution while
+    evaluating fewer right-hand sides and factorising fewer matrices."""
+    from scipy.integrate import solve_ivp
+    md = mechs["TSR-Rich-31"]
+    o = oracle_mod.Oracle(md)
+    ns = md.ns
+    X, p = _initial(md, 1, 4, 79)
+    Xo, stats, status = o.bdf(X, p, mode=1, rtol=1e-8, atol=1e-14, t_end=1e-4, newton=1)
+    assert np.all(status == 0)
+    assert np.max(np.abs(Xo[:ns].sum(0) - X[:ns].sum(0)) / X[:ns].sum(0)) < 1e-13
+    comp = np.array(md.desc["composition"])
+    e0, e1 = comp @ (X[:ns] / md.W[:, None]), comp @ (Xo[:ns] / md.W[:, None])
+    assert np.max(np.abs(e1 - e0) / np.abs(e0).max(0)) < 1e-12
+    for i in range(X.shape[1]):
+        f, jf = _system(o, 1, p[i])
+        r = solve_ivp(f, (0.0, 1e-4), X[:, i], method="Radau", rtol=1e-11, atol=1e-18, jac=jf)
+        ref = r.y[:, -1]
+        assert abs(Xo[-1, i] - ref[-1]) < 0.5
+        assert np.max(np.abs(Xo[:-1, i] - ref[:-1])) < 1e-4
+    X, p = _initial(md, 1, 96, 80)
+    R, _, rs = o.bdf(X, p, mode=1, rtol=1e-10, atol=1e-16, t_end=1e-4, max_steps=400000)
+    A, sa, ra = o.bdf(X, p, mode=1, rtol=1e-6, atol=1e-12, t_end=1e-4, newton=0)
+    C, sc, rc = o.bdf(X, p, mode=1, rtol=1e-6, atol=1e-12, t_end=1e-4, newton=1)
+    assert np.all(rs == 0) and np.all(ra == 0) and np.all(rc == 0)
+    dA, dC = np.abs(A[-1] - R[-1]), np.abs(C[-1] - R[-1])
+    # the median cell is within 1e-8 of T in both (rtol 1e-6 bounds the local error per step)
+    assert np.median(dC) < 1e-8 * np.median(R[-1]) * 10 and np.median(dA) < 1e-8 * np.median(R[-1]) * 10
+    assert np.quantile(dC, 0.75) <= 10.0 * np.quantile(dA, 0.75) + 1e-6
+    assert sc[:, 3].mean() < 0.85 * sa[:, 3].mean() and sc[:, 5].mean() < 0.7 * sa[:, 5].mean()

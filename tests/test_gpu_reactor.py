@@ -52,9 +52,9 @@ def test_reactor_matches_oracle(gmech, mechs, oracle_mod, cuda, mech, mode):
     else:
         assert np.all(gstatus == 0)
     o = oracle_mod.Oracle(md)
-    A, ast, astatus = o.bdf(X, st["p"], mode=mode)
+    A, ast, astatus = o.bdf(X, st["p"], mode=mode, newton=1)
     Xp = X * (1 + 1e-14 * np.sin(np.arange(X.size)).reshape(X.shape))
-    B, _, bstatus = o.bdf(Xp, st["p"], mode=mode)
+    B, _, bstatus = o.bdf(Xp, st["p"], mode=mode, newton=1)
     ns = md.ns
     ok = (gstatus == 0) & (astatus == 0) & (bstatus == 0)
     assert ok.mean() > 0.9
@@ -138,10 +138,37 @@ def test_reactor_tight_tolerance_pointwise(gmech, mechs, oracle_mod, cuda, mode)
     X = np.vstack([S.soa(st["rho"] if mode == 0 else st["Y"]), st["T"][None, :]])
     print(f"[gpu] reactor tight mode={mode}", flush=True)
     G, gst, gstatus, _ = _run(gmech["TSR-GP-24"], X, st["p"], mode, cuda, t_end=1e-5, rtol=1e-8, atol=1e-14)
-    A, ast, astatus = oracle_mod.Oracle(md).bdf(X, st["p"], mode=mode, rtol=1e-8, atol=1e-14, t_end=1e-5)
+    A, ast, astatus = oracle_mod.Oracle(md).bdf(X, st["p"], mode=mode, rtol=1e-8, atol=1e-14, t_end=1e-5, newton=1)
     ok = (gstatus == 0) & (astatus == 0)
     assert ok.mean() > 0.9
     assert np.median(np.abs(G[-1, ok] - A[-1, ok])) < 1e-4
+
+
+def test_reactor_scipy_newton_policy(mechs, oracle_mod, cuda):
+    """Newton policy 0 (scipy's BDF) on the GPU against the oracle with the same policy: the same
+    statistics as the default policy's test, and the step counts of the two implementations agree."""
+    md = mechs["TSR-Rich-31"]
+    m = Mechanism.from_data(md)
+    m.set_newton(0)
+    n = 128
+    st = S.radical_states(md.species, md.W, 408, 0, n, ignition_half=True)
+    X = np.vstack([S.soa(st["Y"]), st["T"][None, :]])
+    print(f"[gpu] reactor scipy policy n={n}", flush=True)
+    G, gst, gstatus, _ = _run(m, X, st["p"], 1, cuda)
+    o = oracle_mod.Oracle(md)
+    A, ast, astatus = o.bdf(X, st["p"], mode=1, newton=0)
+    Xp = X * (1 + 1e-14 * np.sin(np.arange(X.size)).reshape(X.shape))
+    B, _, bstatus = o.bdf(Xp, st["p"], mode=1, newton=0)
+    assert np.all(gstatus == 0) and np.all(astatus == 0)
+    dGA, dBA = np.abs(G[-1] - A[-1]), np.abs(B[-1] - A[-1])
+    assert np.median(dGA) <= 3.0 * np.median(dBA) + 1e-6
+    assert abs(gst[:, 0].mean() / ast[:, 0].mean() - 1.0) < 0.1
+    # the CVODE policy on the same cells: fewer right-hand sides and factorisations
+    _, cst, cstatus = o.bdf(X, st["p"], mode=1, newton=1)
+    assert np.all(cstatus == 0)
+    assert cst[:, 3].mean() < 0.9 * ast[:, 3].mean() and cst[:, 5].mean() < 0.8 * ast[:, 5].mean()
+    with pytest.raises(Exception):
+        m.set_newton(2)
 
 
 def test_reactor_graph_replay_and_autotune(gmech, mechs, cuda):
