@@ -69,16 +69,26 @@ static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::ve
                       std::vector<int>& jterm) {
   const int nr = hp->nr, ns = hp->ns, nms = hp->nms;
   if (ns > 31) return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports ns <= 31");
+  // Arrhenius and three-body reactions (one code path) fill the first wavefront rounds; the
+  // falloff reactions (Troe, Lindemann: logarithms, powers and divisions) start a round of their
+  // own, so that their code runs in as few rounds as they need.
   std::vector<int> order;
-  for (int kind = 0; kind < 4; ++kind)
+  for (int kind = 0; kind < 2; ++kind)
     for (int r = 0; r < nr; ++r)
       if ((hp->rx_kind[r] & 3) == kind) order.push_back(r);
-  P.R = (nr + 31) / 32;
+  const int n_plain = (int)order.size();
+  if (n_plain < nr)
+    while (order.size() % 64) order.push_back(-1);
+  for (int kind = 2; kind < 4; ++kind)
+    for (int r = 0; r < nr; ++r)
+      if ((hp->rx_kind[r] & 3) == kind) order.push_back(r);
+  (void)n_plain;
+  P.R = ((int)order.size() + 31) / 32;
   rrec.assign((size_t)P.R * 32 * 4, 0);
   std::vector<int> rpack(nr, 0);
   for (int s = 0; s < P.R * 32; ++s) {
     int* rec = &rrec[(size_t)s * 4];
-    if (s >= nr) {
+    if (s >= (int)order.size() || order[s] < 0) {
       rec[0] = 7;
       rec[2] = nr;
       rec[3] = 0x3fffffff;  // every factor slot 31

@@ -345,8 +345,23 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   const int nslot = P.R * TEAM;
   int4 rec = wl < nslot ? L.rrec[wl] : make_int4(7, 0, 0, 0);
   tsync();
-  // third-body sums M_s = sum_j eff_sj c_j, lane s
-  if (lane < P.nms) {
+  // third-body sums M_s = sum_j eff_sj c_j.  Up to 16 sets: set s = wl & 15 in four lane groups
+  // g = wl / 16, group g summing the species j = g (mod 4), the four partial sums then added as
+  // (g0 + g1) + (g2 + g3) (the same value on every lane of set s).  More sets: one lane per set.
+  if (P.nms <= 16) {
+    const int nms = P.nms, g = wl >> 4, sset = wl & 15;
+    double m = 0.0;
+    if (sset < nms) {
+#pragma unroll 2
+      for (int t = 0; t < 8; ++t) {
+        const int j = g + 4 * t;
+        if (j < ns) m = m + (double)L.effT[j * nms + sset] * L.c[j];
+      }
+    }
+    m = m + swz16(m);
+    const double2 hm = halves(m);
+    if (wl < nms) L.Mv[wl] = hm.x + hm.y;
+  } else if (lane < P.nms) {
     const int nms = P.nms;
     double m = 0.0;
 #pragma unroll 8
@@ -644,7 +659,7 @@ __device__ __forceinline__ void change_D(const Dcol D, int order, double factor)
     double prod = 1.0, acc = 0.0;
 #pragma unroll
     for (int j = 1; j <= MAXO; ++j) {
-      prod *= (j - 1 - factor * l) / j;
+      prod *= (j - 1 - factor * l) * (1.0 / j);  // 1/j folds to a constant: no division
       if (j <= order) acc += prod * D[j];
     }
     v[l] = (l <= order) ? acc : 0.0;

@@ -34,12 +34,13 @@ static double rms(const double *v, const double *scale, int n) {
   return sqrt(s / n);
 }
 
-/* R(order, factor): M[0][*] = 1, M[i][j] = (i - 1 - factor j)/i, cumulative product down rows */
+/* R(order, factor): M[0][*] = 1, M[i][j] = (i - 1 - factor j)/i, cumulative product down rows
+ * (the division by i taken as a product with the constant 1/i, as the GPU kernel does) */
 static void compute_R(int order, double factor, double R[MAXO + 1][MAXO + 1]) {
   for (int j = 0; j <= order; ++j) R[0][j] = 1.0;
   for (int i = 1; i <= order; ++i)
     for (int j = 0; j <= order; ++j) {
-      double mij = (j == 0) ? 0.0 : (i - 1 - factor * j) / i;
+      double mij = (j == 0) ? 0.0 : (i - 1 - factor * j) * (1.0 / i);
       R[i][j] = R[i - 1][j] * mij;
     }
 }
