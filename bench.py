@@ -15,10 +15,16 @@ Beside it in the same JSON line:
                velocity components;
   source_general  TSR-GP-24 source terms by the data-driven general routine, 2^20 cells, beside
                the generated explicit routine on the same cells;
-  cpu_baseline the C oracle (same algorithms, OpenMP over cells) on bounded samples of the same
-               workloads on this host's cores.
+  transport    TSR-GP-24 Wilke mixture viscosity and conductivity, 2^20 cells;
+  cpu_baseline the C oracle (same algorithms, OpenMP over cells, release flags, threads pinned) on
+               bounded samples of the same workloads on this host's cores, and on one core.
+Every leg carries a roofline entry: FP64 vector, HBM or BF16 MFMA bound, the achieved rate from
+algorithmic flops or bytes over the HIP-event launch time, and, where profiles/ holds rocprofv3
+counters of the same kernel, the measured traffic and FP64 issue rate.
 
     python bench.py [--steps K] [--warmup W] [--cells N] [--no-cpu] [--only reactor]
+    python bench.py --pmc-summary OUT.json --pmc-csv COUNTERS.csv --kernel REGEX [--copy-bytes B]
+        (summary of a rocprofv3 --pmc run for profiles/)
 """
 import argparse
 import json
@@ -26,14 +32,17 @@ import os
 import sys
 import time
 
+# the CPU baseline's OpenMP threads stay on their cores (set before any OpenMP runtime starts)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM = 8.0e12     # B/s, MI355X HBM3E
-PEAK_FP64 = 78.6e12   # flop/s, MI355X FP64
-# FP64 flops per unit of BDF work (per-cell counters of the kernel): right-hand side, Jacobian,
-# LU of the 32 x 32 Newton matrix (2/3 n^3), one forward/backward solve (2 n^2)
-FLOP_RHS, FLOP_JAC, FLOP_LU, FLOP_SOLVE = 8.0e3, 20.0e3, 2.0 / 3.0 * 32 ** 3, 2.0 * 32 ** 2
+PEAK_FP64 = 78.6e12   # flop/s, MI355X FP64 (vector)
+PEAK_BF16 = 2.5e15    # flop/s, MI355X dense BF16 MFMA
+CPU_BUILD = "gcc -O3 -march=x86-64-v3 -mtune=native -funroll-loops -fopenmp (oracle/libflint_oracle_bench.so)"
 
 
 def parse():
@@ -48,7 +57,115 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="also run a plain device-to-device copy of this many bytes (counter calibration)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the reactor's default launch shape")
+    ap.add_argument("--pmc-summary", default=None, metavar="OUT_JSON",
+                    help="summarise rocprofv3 --pmc counter CSVs (--pmc-csv) of one kernel (--kernel) and exit")
+    ap.add_argument("--pmc-csv", action="append", default=[])
+    ap.add_argument("--kernel", default=None, help="kernel-name regex for --pmc-summary")
+    ap.add_argument("--pmc-cmd", default="", help="the profiled command, recorded in the summary")
     return ap.parse_args()
+
+
+def pmc_summary(args):
+    """Per-dispatch totals of rocprofv3 --pmc counter CSVs for the largest dispatch of one kernel:
+    FP64 lane flops (SQ_INSTS_VALU_FLOPS_FP64 counts the flops of each wave instruction, so 64 x
+    it bounds the lane flops from above: masked lanes count too) and HBM bytes (FETCH_SIZE and
+    WRITE_SIZE in KiB, scaled by what the same counters read for the plain copy of --copy-bytes,
+    whose dispatch is in the same CSV)."""
+    import csv
+    import re
+    per = {}
+    names = {}
+    for path in args.pmc_csv:
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                key = (path, r["Dispatch_Id"])
+                names[key] = r["Kernel_Name"]
+                d = per.setdefault(key, {})
+                d[r["Counter_Name"]] = d.get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+                d.setdefault("_grid", int(r["Grid_Size"]))
+                d.setdefault("_ns", float(r["End_Timestamp"]) - float(r["Start_Timestamp"]))
+    rx = re.compile(args.kernel)
+    out = {"kernel_regex": args.kernel, "command": args.pmc_cmd, "counters": {}}
+    for path in args.pmc_csv:
+        mine = [(k, v) for k, v in per.items() if k[0] == path and rx.search(names[k])]
+        if not mine:
+            continue
+        key, best = max(mine, key=lambda kv: max(x for c, x in kv[1].items() if not c.startswith("_")))
+        out["kernel"] = names[key]
+        out["launch_ms"] = best["_ns"] * 1e-6
+        for c, v in best.items():
+            if not c.startswith("_"):
+                out["counters"][c] = v
+        # copy calibration: the elementwise copy kernel of --copy-bytes in the same run
+        copies = [(k, v) for k, v in per.items() if k[0] == path and "copy" in names[k].lower()]
+        if copies and args.copy_bytes:
+            _, cv = max(copies, key=lambda kv: kv[1].get("FETCH_SIZE", kv[1].get("WRITE_SIZE", 0.0)))
+            kib = args.copy_bytes / 1024.0
+            if "FETCH_SIZE" in cv:
+                out["copy_fetch_scale"] = kib / cv["FETCH_SIZE"]
+            if "WRITE_SIZE" in cv:
+                out["copy_write_scale"] = kib / cv["WRITE_SIZE"]
+    c = out["counters"]
+    if "SQ_INSTS_VALU_FLOPS_FP64" in c:
+        out["fp64_flops_per_launch"] = 64.0 * c["SQ_INSTS_VALU_FLOPS_FP64"]
+        out["fp64_trans_per_launch"] = 64.0 * c.get("SQ_INSTS_VALU_FLOPS_FP64_TRANS", 0.0)
+    if "FETCH_SIZE" in c:
+        out["hbm_fetch_bytes"] = 1024.0 * c["FETCH_SIZE"] * out.get("copy_fetch_scale", 1.0)
+    if "WRITE_SIZE" in c:
+        out["hbm_write_bytes"] = 1024.0 * c["WRITE_SIZE"] * out.get("copy_write_scale", 1.0)
+    if os.path.exists(args.pmc_summary):
+        with open(args.pmc_summary) as f:
+            old = json.load(f)
+        old.update({k: v for k, v in out.items() if k != "counters"})
+        old.setdefault("counters", {}).update(out["counters"])
+        out = old
+    if "hbm_fetch_bytes" in out and "hbm_write_bytes" in out:
+        out["hbm_bytes_per_launch"] = out["hbm_fetch_bytes"] + out["hbm_write_bytes"]
+    with open(args.pmc_summary, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
+
+
+def profile_json(name):
+    p = os.path.join(ROOT, "profiles", name)
+    if os.path.exists(p):
+        with open(p) as f:
+            return json.load(f)
+    return {}
+
+
+def cpu_baseline(o, run, units, seconds, sample, unit="cells/s", one_units=None, run_one=None):
+    """The CPU baseline: run() makes one pass over `units` units of work on all OpenMP threads, timed
+    for about `seconds` after one warm-up pass (the best pass counts); then the same on one thread
+    (run_one over one_units, when given, a smaller sample) for a third of the time."""
+    def rate(fn, u, budget):
+        fn()
+        best, t = float("inf"), time.perf_counter()
+        while True:
+            t1 = time.perf_counter()
+            fn()
+            best = min(best, time.perf_counter() - t1)
+            if time.perf_counter() - t > budget:
+                break
+        return u / best
+    cores = o.num_threads()
+    v = rate(run, units, seconds)
+    o.set_threads(1)
+    try:
+        v1 = rate(run_one or run, one_units or units, seconds / 3.0)
+    finally:
+        o.set_threads(cores)
+    return dict(value=v, unit=unit, cores=cores, value_1core=v1, kind="port", build=CPU_BUILD,
+                threads="OMP_PROC_BIND=%s OMP_PLACES=%s" % (os.environ.get("OMP_PROC_BIND"),
+                                                           os.environ.get("OMP_PLACES")),
+                sample=sample)
+
+
+def roofline_hbm(kernel, bytes_per_launch, ms, traffic=None):
+    a = bytes_per_launch / (ms * 1e-3)
+    return dict(bound="hbm", kernel=kernel, achieved=a / 1e9, peak=PEAK_HBM / 1e9, unit="GB/s", frac=a / PEAK_HBM,
+                traffic=traffic)
 
 
 def timed(stream, fn, reps):
@@ -90,6 +207,15 @@ def reactor_leg(args, dev, stream):
         X.copy_(Xd0)
         mech.advance(X, p, mode=1, t_end=1e-4, rtol=1e-6, atol=1e-12, stats=stats, status=status, stream=stream)
 
+    # launch shape from the project's autotuner: (teams per block, blocks) timed on the first 2^16
+    # cells of the batch, before the timed region
+    tune = None
+    if not args.no_autotune:
+        m = min(n, 1 << 16)
+        t = time.perf_counter()
+        best_ms, shape = mech.autotune(Xd0[:, :m].contiguous(), p[:m].contiguous(), mode=1, t_end=1e-4, rtol=1e-6,
+                                       atol=1e-12)
+        tune = dict(sample_cells=m, best_sample_ms=best_ms, shape=list(shape), seconds=time.perf_counter() - t)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -104,26 +230,31 @@ def reactor_leg(args, dev, stream):
     wall = time.perf_counter() - t0
     ms_step = ms / args.steps
     s = stats.cpu().numpy().astype(np.float64)
-    flops = (s[3] * FLOP_RHS + s[4] * FLOP_JAC + s[5] * FLOP_LU + s[6] * FLOP_SOLVE).sum()
+    from flint_amd.flops import reactor_flops
+    fl = reactor_flops(md.prog, mode=1)
+    flops = fl.total(s[3].sum(), s[4].sum(), s[5].sum(), s[6].sum(), s[0].sum() + s[1].sum())
     res = dict(n=n, ms=ms_step, wall=wall, cells_per_s=n / (ms_step * 1e-3),
                rhs_per_s=s[3].sum() / (ms_step * 1e-3), fails=int((status != 0).sum()),
                steps_mean=float(s[0].mean()), steps_max=int(s[0].max()), rhs_mean=float(s[3].mean()),
                jac_mean=float(s[4].mean()), lu_mean=float(s[5].mean()), newton_mean=float(s[6].mean()),
-               flops=float(flops), launch=list(mech.get_launch()))
+               flops=float(flops), flops_per_unit=fl.__dict__, launch=list(mech.get_launch()), autotune=tune,
+               newton_policy="cvode")
     if not args.no_cpu:
         O = oracle()
-        o = O.Oracle(md)
-        # pilot on 128 cells, then a sample sized to about cpu_seconds of wall time
+        o = O.Oracle(md, release=True)
+        # pilot on 128 cells, then samples sized to about cpu_seconds per pass on all threads and
+        # a third of that on one thread
         t = time.perf_counter()
-        o.bdf(X0[:, :128], st["p"][:128], mode=1, rtol=1e-6, atol=1e-12, t_end=1e-4)
+        o.bdf(X0[:, :128], st["p"][:128], mode=1, rtol=1e-6, atol=1e-12, t_end=1e-4, newton=1)
         pilot = time.perf_counter() - t
-        ncpu = int(min(n, max(128, 128 * args.cpu_seconds / max(pilot, 1e-3))))
-        t = time.perf_counter()
-        o.bdf(X0[:, :ncpu], st["p"][:ncpu], mode=1, rtol=1e-6, atol=1e-12, t_end=1e-4)
-        dt = time.perf_counter() - t
-        res["cpu"] = dict(value=ncpu / dt, unit="cells/s", cores=o.num_threads(), kind="port",
-                          sample=f"first {ncpu} cells of the same workload (every second one near ignition), "
-                                 f"C BDF oracle (same method, OpenMP over cells), wall {dt:.1f} s")
+        ncpu = int(min(n, max(128, 128 * 0.5 * args.cpu_seconds / max(pilot, 1e-3))))
+        n1 = int(max(32, ncpu / max(1, o.num_threads()) / 3))
+        res["cpu"] = cpu_baseline(
+            o, lambda: o.bdf(X0[:, :ncpu], st["p"][:ncpu], mode=1, rtol=1e-6, atol=1e-12, t_end=1e-4, newton=1),
+            ncpu, 0.5 * args.cpu_seconds,
+            f"first {ncpu} cells (1 core: {n1}) of the same workload (every second one near ignition), C BDF "
+            f"oracle (same method and Newton policy, OpenMP over cells)", one_units=n1,
+            run_one=lambda: o.bdf(X0[:, :n1], st["p"][:n1], mode=1, rtol=1e-6, atol=1e-12, t_end=1e-4, newton=1))
     return res
 
 
@@ -142,6 +273,17 @@ def mlp_leg(args, dev, stream):
     X = torch.from_numpy(np.vstack([S.soa(st["Y"]), st["T"][None, :]])).to(dev)
     p = torch.from_numpy(st["p"]).to(dev)
     out = {}
+    # the network alone: one forward pass over the 2^18 cells (BF16 MFMA, FP32 accumulation)
+    net = sur.net
+    Xf = torch.randn((n, net.dims[0]), dtype=torch.float32, device=dev)
+    Yf = net.forward(Xf, stream=stream)
+    ms_f = timed(stream, lambda: net.forward(Xf, Yf, stream=stream), 10)
+    fl = 2.0 * n * sum(a * b for a, b in zip(net.dims[:-1], net.dims[1:]))
+    out["network"] = dict(dims=net.dims, cells=n, forward_ms=ms_f, flops=fl,
+                          roofline=dict(bound="mfma", kernel="flint_mlp forward (v_mfma_f32_32x32x16_bf16)",
+                                        achieved=fl / (ms_f * 1e-3) / 1e12, peak=PEAK_BF16 / 1e12,
+                                        unit="TFLOP/s", frac=fl / (ms_f * 1e-3) / PEAK_BF16))
+    del Xf, Yf
     for phase, t0 in (("t0", 0.0), ("mid", 5e-5)):
         if t0 > 0:
             mech.advance(X, p, mode=1, t_end=t0, stream=stream)
@@ -228,10 +370,8 @@ def omega_leg(args, dev, stream):
     res["rich31"] = dict(n=nr_, ms=ms_r, cells_per_s=nr_ / (ms_r * 1e-3), bytes_per_cell=bpr,
                          achieved_GBps=bpr * nr_ / (ms_r * 1e-3) / 1e9)
     del Zr, Fr, Jr
-    p = os.path.join(ROOT, "profiles", "rhs_jac_tsr_gp_24_1M.traffic.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            res["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    res["traffic"] = profile_json("rhs_jac_tsr_gp_24_1M.traffic.json").get("hbm_bytes_per_launch")
+    res["roofline"] = roofline_hbm("rhs_jac_kernel<true> (TSR-GP-24)", bpc * n, ms, res["traffic"])
     # WD source terms + heat release, 65,536 cells
     mwd = mech_data("WD")
     wd = Mechanism.from_data(mwd)
@@ -245,27 +385,17 @@ def omega_leg(args, dev, stream):
     ms_wd = timed(stream, lambda: wd.source(rho, T, w, q, stream=stream), 200)
     res_wd = dict(n=1 << 16, ms=ms_wd, cells_per_s=(1 << 16) / (ms_wd * 1e-3), bytes_per_cell=96,
                   achieved_GBps=96 * (1 << 16) / (ms_wd * 1e-3) / 1e9)
+    res_wd["roofline"] = roofline_hbm("wdot_hrr_kernel (WD)", 96 * (1 << 16), ms_wd)
     if not args.no_cpu:
         O = oracle()
-        o = O.Oracle(md)
+        o = O.Oracle(md, release=True)
         Zs = Z[:, :1 << 16].copy()
-        o.jac(Zs[:, :1024])
-        reps, t = 0, time.perf_counter()
-        while time.perf_counter() - t < args.cpu_seconds:
-            o.jac(Zs)
-            reps += 1
-        dt = time.perf_counter() - t
-        res["cpu"] = dict(value=reps * Zs.shape[1] / dt, unit="cells/s", cores=o.num_threads(), kind="port",
-                          sample=f"{reps} passes over the first {Zs.shape[1]} cells, C oracle, wall {dt:.1f} s")
-        ow = O.Oracle(mwd)
+        res["cpu"] = cpu_baseline(o, lambda: o.jac(Zs), Zs.shape[1], 0.4 * args.cpu_seconds,
+                                  f"passes over the first {Zs.shape[1]} cells, C oracle (rates, Jacobian)")
+        ow = O.Oracle(mwd, release=True)
         rs, Ts = S.soa(sw["rho"]), sw["T"]
-        reps, t = 0, time.perf_counter()
-        while time.perf_counter() - t < 0.3 * args.cpu_seconds:
-            ow.wd_hrr(rs, Ts)
-            reps += 1
-        dt = time.perf_counter() - t
-        res_wd["cpu"] = dict(value=reps * Ts.size / dt, unit="cells/s", cores=o.num_threads(), kind="port",
-                             sample=f"{reps} passes over the 65,536 cells, C oracle, wall {dt:.1f} s")
+        res_wd["cpu"] = cpu_baseline(ow, lambda: ow.wd_hrr(rs, Ts), Ts.size, 0.2 * args.cpu_seconds,
+                                     "passes over the 65,536 cells, C oracle")
     return res, res_wd
 
 
@@ -297,18 +427,22 @@ def cea_leg(args, dev, stream):
                warm_start=dict(ms=ms2, states_per_s=n / (ms2 * 1e-3), iters_mean=float(it2.float().mean()),
                                iters_max=int(it2.max()), failed=int((s2 != 0).sum()),
                                max_abs_dT_vs_default=float((T2 - T1)[ok].abs().max())))
+    pj = profile_json("cea_kernel_tsr_gp_24_262144.fp64.json")
+    if pj.get("fp64_flops_per_launch"):
+        a = pj["fp64_flops_per_launch"] / (ms1 * 1e-3)
+        res["roofline"] = dict(bound="fp64_valu", kernel="cea_kernel (default start)", achieved=a / 1e12,
+                               peak=PEAK_FP64 / 1e12, unit="TFLOP/s", frac=a / PEAK_FP64, traffic=None,
+                               note="flops per launch from rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 "
+                                    "(profiles/cea_kernel_tsr_gp_24_262144.fp64.json)")
     if not args.no_cpu:
         O = oracle()
-        o = O.Oracle(md)
+        o = O.Oracle(md, release=True)
         Yc = S.soa(st["Y"])
-        reps, t = 0, time.perf_counter()
-        while time.perf_counter() - t < 0.3 * args.cpu_seconds:
-            o.cea(st["p_bar"], st["T0"], Yc, hp=True)
-            reps += 1
-        dt = time.perf_counter() - t
-        res["cpu"] = dict(value=reps * n / dt, unit="states/s", cores=o.num_threads(), kind="port",
-                          sample=f"{reps} passes over the same {n} states, C CEA oracle (OpenMP over states), "
-                                 f"wall {dt:.1f} s")
+        m = 1 << 16
+        Ym, pm, Tm = np.ascontiguousarray(Yc[:, :m]), st["p_bar"][:m].copy(), st["T0"][:m].copy()
+        res["cpu"] = cpu_baseline(o, lambda: o.cea(pm, Tm, Ym, hp=True), m, 0.3 * args.cpu_seconds,
+                                  f"passes over the first {m} states, C CEA oracle (OpenMP over states)",
+                                  unit="states/s")
     return res
 
 
@@ -339,24 +473,16 @@ def thermo_leg(args, dev, stream):
     res = dict(n=n, nd=nd, ms=ms, cells_per_s=n / (ms * 1e-3), bytes_per_cell=bpc,
                achieved_GBps=bpc * n / (ms * 1e-3) / 1e9, iters_mean=float(it.float().mean()),
                max_rel_dp=float((prim[-1] / q[-1] - 1.0).abs().max()))
-    tp = os.path.join(ROOT, "profiles", "cons2prim_tsr_gp_24_1M.traffic.json")
-    if os.path.exists(tp):
-        with open(tp) as f:
-            res["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    res["traffic"] = profile_json("cons2prim_tsr_gp_24_1M.traffic.json").get("hbm_bytes_per_launch")
+    res["roofline"] = roofline_hbm("cons2prim_kernel (TSR-GP-24)", bpc * n, ms, res["traffic"])
     if not args.no_cpu:
         O = oracle()
-        o = O.Oracle(md)
+        o = O.Oracle(md, release=True)
         m = 1 << 16
         cs = cons[:, :m].cpu().numpy()
         t0 = fs["T0"][:m].copy()
-        reps, t = 0, time.perf_counter()
-        while time.perf_counter() - t < 0.3 * args.cpu_seconds:
-            o.cons2prim(cs, t0)
-            reps += 1
-        dt = time.perf_counter() - t
-        res["cpu"] = dict(value=reps * m / dt, unit="cells/s", cores=o.num_threads(), kind="port",
-                          sample=f"{reps} passes over the first {m} cells, C oracle (OpenMP over cells), "
-                                 f"wall {dt:.1f} s")
+        res["cpu"] = cpu_baseline(o, lambda: o.cons2prim(cs, t0), m, 0.2 * args.cpu_seconds,
+                                  f"passes over the first {m} cells, C oracle (OpenMP over cells)")
     return res
 
 
@@ -390,18 +516,18 @@ def transport_leg(args, dev, stream):
     fpc = 6 * ns * ns + 20 * ns  # 3 FMAs per species pair, ~20 flops per species (X, r, interpolation)
     res = dict(n=n, ms=ms, cells_per_s=n / (ms * 1e-3), bytes_per_cell=bpc, flops_per_cell=fpc,
                achieved_GBps=bpc * n / (ms * 1e-3) / 1e9, achieved_TFLOPs=fpc * n / (ms * 1e-3) / 1e12)
+    res["roofline"] = dict(bound="fp64_valu", kernel="wilke_kernel<24>", achieved=res["achieved_TFLOPs"],
+                           peak=PEAK_FP64 / 1e12, unit="TFLOP/s", frac=res["achieved_TFLOPs"] * 1e12 / PEAK_FP64,
+                           traffic=profile_json("wilke_tsr_gp_24_1M.traffic.json").get("hbm_bytes_per_launch"),
+                           hbm_frac=res["achieved_GBps"] * 1e9 / PEAK_HBM)
     if not args.no_cpu:
         O = oracle()
-        o = O.Oracle(md)
+        o = O.Oracle(md, release=True)
         m = 1 << 14
-        reps, t = 0, time.perf_counter()
-        while time.perf_counter() - t < 0.3 * args.cpu_seconds:
-            o.wilke(tb, r_h[:, :m], rho_h[:m], fs["T"][:m])
-            reps += 1
-        dt = time.perf_counter() - t
-        res["cpu"] = dict(value=reps * m / dt, unit="cells/s", cores=o.num_threads(), kind="port",
-                          sample=f"{reps} passes over the first {m} cells, C oracle (the reference's ns^2 "
-                                 f"form, OpenMP over cells), wall {dt:.1f} s")
+        rs, rhs_, Ts = np.ascontiguousarray(r_h[:, :m]), rho_h[:m].copy(), fs["T"][:m].copy()
+        res["cpu"] = cpu_baseline(o, lambda: o.wilke(tb, rs, rhs_, Ts), m, 0.2 * args.cpu_seconds,
+                                  f"passes over the first {m} cells, C oracle (the reference's ns^2 form, "
+                                  f"OpenMP over cells)")
     return res
 
 
@@ -428,24 +554,23 @@ def general_leg(args, dev, stream):
     res = dict(n=n, ms=ms_g, cells_per_s=n / (ms_g * 1e-3), bytes_per_cell=bpc,
                achieved_GBps=bpc * n / (ms_g * 1e-3) / 1e9, explicit_ms=ms_e,
                explicit_cells_per_s=n / (ms_e * 1e-3))
+    res["roofline"] = roofline_hbm("general_kernel (TSR-GP-24)", bpc * n, ms_g,
+                                   profile_json("general_tsr_gp_24_1M.traffic.json").get("hbm_bytes_per_launch"))
     if not args.no_cpu:
         O = oracle()
-        o = O.Oracle(md)
+        o = O.Oracle(md, release=True)
         m = 1 << 14
         rs, Ts = S.soa(st["rho"][:m]), st["T"][:m].copy()
-        reps, t = 0, time.perf_counter()
-        while time.perf_counter() - t < 0.3 * args.cpu_seconds:
-            o.general(rs, Ts)
-            reps += 1
-        dt = time.perf_counter() - t
-        res["cpu"] = dict(value=reps * m / dt, unit="cells/s", cores=o.num_threads(), kind="port",
-                          sample=f"{reps} passes over the first {m} cells, C oracle (OpenMP over cells), "
-                                 f"wall {dt:.1f} s")
+        res["cpu"] = cpu_baseline(o, lambda: o.general(rs, Ts), m, 0.2 * args.cpu_seconds,
+                                  f"passes over the first {m} cells, C oracle (OpenMP over cells)")
     return res
 
 
 def main():
     args = parse()
+    if args.pmc_summary:
+        pmc_summary(args)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (cells shard with no "
               "exchange; see the sharding paragraph of the design notes). Use --gpus 1.", file=sys.stderr)
@@ -498,18 +623,31 @@ def main():
     if rx:
         achieved = rx["flops"] / (rx["ms"] * 1e-3)
         out["config"].update({k: rx[k] for k in ("fails", "steps_mean", "steps_max", "rhs_mean", "jac_mean",
-                                                 "lu_mean", "newton_mean", "launch", "wall")})
+                                                 "lu_mean", "newton_mean", "launch", "autotune", "newton_policy",
+                                                 "wall")})
         out["rhs_evals_per_s"] = rx["rhs_per_s"]
-        traffic = None
-        tp = os.path.join(ROOT, "profiles", "bdf_kernel_rich31_262144.traffic.json")
-        if os.path.exists(tp) and args.cells == 1 << 18:
-            with open(tp) as f:
-                traffic = json.load(f).get("hbm_bytes_per_launch")
-        out["roofline"] = {"bound": "mfma", "kernel": "flint_reactor::bdf_kernel",
+        # counter profiles of this kernel and workload; used only while they describe the kernel
+        # being timed (their recorded launch time within 10 % of this run's)
+        tj = profile_json("bdf_kernel_rich31_262144.traffic.json")
+        fj = profile_json("bdf_kernel_rich31_262144.fp64.json")
+        same = lambda j: args.cells == 1 << 18 and j.get("launch_ms") and abs(j["launch_ms"] / rx["ms"] - 1) < 0.1
+        traffic = tj.get("hbm_bytes_per_launch") if same(tj) else None
+        out["roofline"] = {"bound": "fp64_valu", "kernel": "flint_reactor::bdf_kernel",
                            "achieved": achieved / 1e12, "peak": PEAK_FP64 / 1e12, "unit": "TFLOP/s",
                            "frac": achieved / PEAK_FP64, "traffic": traffic,
-                           "note": "FP64 vector work (no MFMA): flops from the kernel's per-cell counters x "
-                                   "8e3/RHS, 2e4/Jacobian, 2/3*32^3/LU, 2*32^2/solve; peak = MI355X FP64"}
+                           "flops_per_launch": rx["flops"],
+                           "note": "FP64 vector work (no MFMA). achieved = algorithmic flops (static counts per "
+                                   "right-hand side, Jacobian, Gauss-Jordan inverse, Newton correction and step "
+                                   "from the rate program, flint_amd/flops.py, times the kernel's per-cell "
+                                   "counters) / HIP-event launch time; peak = MI355X FP64 vector"}
+        if same(fj) and fj.get("fp64_flops_per_launch"):
+            ca = fj["fp64_flops_per_launch"] / (rx["ms"] * 1e-3)
+            out["roofline"].update(counter_achieved=ca / 1e12, counter_frac=ca / PEAK_FP64,
+                                   counter_flops_per_launch=fj["fp64_flops_per_launch"],
+                                   counter_note="rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes of the same launch "
+                                                "(profiles/bdf_kernel_rich31_262144.fp64.json): the FP64 work the "
+                                                "SIMDs issue, both half-wavefronts of a cell and masked lanes "
+                                                "included")
         if "cpu" in rx:
             out["cpu_baseline"] = rx["cpu"]
     out.update(extra)

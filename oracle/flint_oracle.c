@@ -35,6 +35,15 @@ int orc_num_threads(void) {
 #endif
 }
 
+/* threads of the following parallel loops (the CPU baseline's 1-core and N-core figures) */
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
 /* ---- real instantiation ---- */
 #define REAL double
 #define SFX(x) x##_r

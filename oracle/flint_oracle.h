@@ -108,6 +108,7 @@ void orc_cea(const orc_mech *m, const orc_cea_data *c, long n, long ld, int hp, 
              const double *rho, const double *guess, double *T_out, double *Y_out, int *iters, int *status);
 
 int orc_num_threads(void);
+void orc_set_threads(int n);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,7 @@ from flint_amd.mechanism import flatten
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libflint_oracle.so")
+BENCH_LIB = os.path.join(HERE, "libflint_oracle_bench.so")  # release flags, bench.py's CPU baseline
 REF_LIB = os.path.join(HERE, "_ref", "libflint_ref.so")
 
 _P = C.c_void_p
@@ -43,6 +44,7 @@ def build():
 
 
 _lib = None
+_bench_lib = None
 
 
 def lib():
@@ -54,6 +56,15 @@ def lib():
     return _lib
 
 
+def bench_lib():
+    global _bench_lib
+    if _bench_lib is None:
+        if not os.path.exists(BENCH_LIB):
+            build()
+        _bench_lib = C.CDLL(BENCH_LIB)
+    return _bench_lib
+
+
 def _ptr(a):
     return a.ctypes.data_as(_P)
 
@@ -61,7 +72,9 @@ def _ptr(a):
 class Oracle:
     """The C restatement bound to one mechanism's program and tables (a MechData)."""
 
-    def __init__(self, md):
+    def __init__(self, md, release=False):
+        """release: the baseline build at release flags (libflint_oracle_bench.so) instead of the
+        checker build."""
         self.md = md
         self.ns = md.ns
         self._keep = flatten(md.prog)
@@ -75,7 +88,7 @@ class Oracle:
         m.Tk_min, m.Tk_max, m.Th_min, m.Th_max = md.Tk_min, md.Tk_max, md.Th_min, md.Th_max
         m.is_wd = int(md.is_wd)
         self.m = m
-        self.L = lib()
+        self.L = bench_lib() if release else lib()
 
     def wdot(self, rho, T):
         """rho [ns, n] SoA, T [n] -> (wdot [ns, n], scale [ns, n])."""
@@ -249,6 +262,9 @@ class Oracle:
 
     def num_threads(self):
         return self.L.orc_num_threads()
+
+    def set_threads(self, n):
+        self.L.orc_set_threads(C.c_int(n))
 
 
 class Reference:
