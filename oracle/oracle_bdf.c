@@ -34,22 +34,23 @@ static double rms(const double *v, const double *scale, int n) {
   return sqrt(s / n);
 }
 
-/* R(order, factor): M[0][*] = 1, M[i][j] = (i - 1 - factor j)/i, cumulative product down rows
- * (the division by i taken as a product with the constant 1/i, as the GPU kernel does) */
-static void compute_R(int order, double factor, double R[MAXO + 1][MAXO + 1]) {
+/* R(order, factor): M[0][*] = 1, M[i][j] = (i - 1 - factor j)/i, cumulative product down rows.
+ * rcp: the division by i taken as a product with the constant 1/i, as the GPU kernel does (Newton
+ * policy 1); scipy divides (policy 0, whose step sequence is pinned to scipy's). */
+static void compute_R(int order, double factor, double R[MAXO + 1][MAXO + 1], int rcp) {
   for (int j = 0; j <= order; ++j) R[0][j] = 1.0;
   for (int i = 1; i <= order; ++i)
     for (int j = 0; j <= order; ++j) {
-      double mij = (j == 0) ? 0.0 : (i - 1 - factor * j) * (1.0 / i);
+      double mij = (j == 0) ? 0.0 : (rcp ? (i - 1 - factor * j) * (1.0 / i) : (i - 1 - factor * j) / i);
       R[i][j] = R[i - 1][j] * mij;
     }
 }
 
 /* D[0..order] <- (R U)^T D[0..order] : rescales the difference array to step factor*h */
-static void change_D(double D[MAXO + 3][NZMAX], int n, int order, double factor) {
+static void change_D(double D[MAXO + 3][NZMAX], int n, int order, double factor, int rcp) {
   double R[MAXO + 1][MAXO + 1], U[MAXO + 1][MAXO + 1], RU[MAXO + 1][MAXO + 1];
-  compute_R(order, factor, R);
-  compute_R(order, 1.0, U);
+  compute_R(order, factor, R, rcp);
+  compute_R(order, 1.0, U, rcp);
   for (int i = 0; i <= order; ++i)
     for (int j = 0; j <= order; ++j) {
       double s = 0.0;
@@ -194,7 +195,7 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
     if (st[ST_STEPS] >= max_steps) return 2;
     double min_step = 10.0 * fabs(nextafter(t, INFINITY) - t);
     if (h_abs < min_step) {
-      change_D(D, n, order, min_step / h_abs);
+      change_D(D, n, order, min_step / h_abs, method);
       h_abs = min_step;
       n_equal = 0;
     }
@@ -206,7 +207,7 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
       t_new = t + h_abs;
       if (t_new - t_end > 0) {
         t_new = t_end;
-        change_D(D, n, order, fabs(t_new - t) / h_abs);
+        change_D(D, n, order, fabs(t_new - t) / h_abs, method);
         n_equal = 0;
         if (method == 0) have_lu = 0;
       }
@@ -285,7 +286,7 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
       if (!converged) {
         st[ST_NFAIL]++;
         h_abs *= 0.5;
-        change_D(D, n, order, 0.5);
+        change_D(D, n, order, 0.5, method);
         n_equal = 0;
         if (method == 0) have_lu = 0;
         continue;
@@ -301,7 +302,7 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
         st[ST_REJECT]++;
         double factor = fmax(MIN_FACTOR, safety * pow(err_norm, -1.0 / (order + 1)));
         h_abs *= factor;
-        change_D(D, n, order, factor);
+        change_D(D, n, order, factor, method);
         n_equal = 0;
       } else {
         accepted = 1;
@@ -338,7 +339,7 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
     order += which - 1;
     double factor = fmin(MAX_FACTOR, safety * best);
     h_abs *= factor;
-    change_D(D, n, order, factor);
+    change_D(D, n, order, factor, method);
     n_equal = 0;
     if (method == 0) have_lu = 0;
   }
