@@ -133,6 +133,7 @@ extern "C" int flint_mech_create(const char* mech_name, int ns, const double* W,
   std::copy(m->h_tabh.begin() + (size_t)(nTh - 1) * 2 * ns, m->h_tabh.begin() + (size_t)nTh * 2 * ns,
             m->h_tabh.begin() + (size_t)nTh * 2 * ns);
   int rc = flint_mech_upload(m);
+  if (!rc) rc = flint_reactor_prepare(m);
   if (rc) {
     flint_mech_destroy(m);
     return rc;

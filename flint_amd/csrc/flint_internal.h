@@ -44,6 +44,7 @@ std::string flint_mech_id(const std::string& name);
 int flint_mech_upload(flint_mech* m);
 int flint_stage(flint_mech* m, size_t doubles);
 void flint_mech_release_extra(flint_mech* m);
+int flint_reactor_prepare(flint_mech* m);  // reactor program of a generated mechanism (no-op otherwise)
 void flint_mech_release_cea(flint_mech* m);
 void flint_mech_release_sur(flint_mech* m);
 void flint_mech_release_general(flint_mech* m);

@@ -38,6 +38,13 @@ hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reacto
                                 double* X, const double* pres, int* stats, int* status, int* counter,
                                 int teams_per_block, int blocks, double* jac_scratch, hipStream_t s);
 
+// Allocation-free reactor launches (flint_reactor_advance_noalloc): the per-team Jacobian slots
+// and the work counter live in a file-scope device array of reactor.hip with room for
+// FLINT_SCRATCH_TEAMS teams.  flint_reactor_scratch returns that array's address on the current
+// device (looked up once per device) as (counter, Jacobian slots).
+constexpr int FLINT_SCRATCH_TEAMS = 2048;
+hipError_t flint_reactor_scratch(int** counter, double** jac_slots);
+
 hipError_t flint_reactor_jac_launch(const flint_reactor::Prog& P, int mode, long n, long ld, const double* X,
                                     const double* pres, double* F, double* J, int teams_per_block, int blocks,
                                     double* jac_scratch, hipStream_t s);

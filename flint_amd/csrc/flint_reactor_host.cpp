@@ -161,8 +161,7 @@ static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::ve
   return FLINT_OK;
 }
 
-static int ensure_extra(flint_mech* m) {
-  if (m->extra) return FLINT_OK;
+static int build_extra(flint_mech* m) {
   if (m->is_wd || !m->ops || !m->ops->prog)
     return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel needs a generated mechanism (not WD)");
   if (m->ns + 1 > 32) return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports ns + 1 <= 32");
@@ -207,6 +206,20 @@ static int ensure_extra(flint_mech* m) {
   x->blocks = prop.multiProcessorCount * std::max(1, std::min(per_cu, 8 / x->teams_per_block));
   HIPCHK(flint_reactor_set_max_lds((int)std::min<size_t>(lds_cu, 160 * 1024)));
   return FLINT_OK;
+}
+
+// The reactor program of the handle, built on first use; a failed build leaves no partial state.
+static int ensure_extra(flint_mech* m) {
+  if (m->extra) return FLINT_OK;
+  const int rc = build_extra(m);
+  if (rc) flint_mech_release_extra(m);
+  return rc;
+}
+
+int flint_reactor_prepare(flint_mech* m) {
+  if (m->is_wd || !m->ops || !m->ops->prog) return FLINT_OK;
+  const int rc = ensure_extra(m);
+  return rc == FLINT_EUNSUPPORTED ? FLINT_OK : rc;  // the other entry points stay usable
 }
 
 static int check_args(const flint_mech* m, long n, long ld, int mode, const double* X, const double* p, double t_end,
@@ -257,6 +270,32 @@ extern "C" int flint_reactor_advance(flint_mech* m, long n, long ld, int mode, d
   if (int rc = ensure_extra(m)) return rc;
   return launch(m, n, ld, mode, X, p, t_end, rtol, atol, max_steps, stats, status, (hipStream_t)stream,
                 m->extra->teams_per_block, m->extra->blocks);
+}
+
+// No counterpart in the reference: flint_reactor_advance for callers that cannot allocate device
+// memory (code under graph capture, serving loops).  The Jacobian slots and the work counter come
+// from the library's static device scratch (flint_reactor_scratch); the handle's program was
+// uploaded when the handle was created.  Nothing here calls hipMalloc or hipFree.
+extern "C" int flint_reactor_advance_noalloc(flint_mech* m, long n, long ld, int mode, double* X, const double* p,
+                                             double t_end, double rtol, double atol, int max_steps, int* stats,
+                                             int* status, void* stream) {
+  if (int rc = check_args(m, n, ld, mode, X, p, t_end, rtol, atol)) return rc;
+  if (n == 0) return FLINT_OK;
+  flint_extra* x = m->extra;
+  if (!x) return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel needs a generated mechanism (not WD)");
+  const long teams = (long)x->teams_per_block * x->blocks;
+  if (teams > FLINT_SCRATCH_TEAMS)
+    return flint_fail(FLINT_EARG, "launch shape of " + std::to_string(teams) + " teams exceeds the " +
+                                      std::to_string(FLINT_SCRATCH_TEAMS) +
+                                      " Jacobian slots of the allocation-free scratch (flint_reactor_set_launch)");
+  int* counter = nullptr;
+  double* slots = nullptr;
+  HIPCHK(flint_reactor_scratch(&counter, &slots));
+  flint_reactor::Opts O{mode, rtol, atol, t_end, max_steps > 0 ? max_steps : 20000, x->newton};
+  hipError_t e = flint_reactor_launch(x->prog, O, n, ld, X, p, stats, status, counter, x->teams_per_block, x->blocks,
+                                      slots, (hipStream_t)stream);
+  if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("reactor launch: ") + hipGetErrorString(e));
+  return FLINT_OK;
 }
 
 extern "C" int flint_reactor_jacobian(flint_mech* m, long n, long ld, int mode, const double* X, const double* p,

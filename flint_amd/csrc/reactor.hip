@@ -31,6 +31,9 @@
 //   instead of one per term.
 //   Modes: 0 = constant volume, state (rho_k, T), rhs_native (Lib_Chemistry_rhs.f90:15-52);
 //          1 = constant pressure, state (Y_k, T) at fixed p.
+#include <mutex>
+#include <vector>
+
 #include "flint_device.h"
 #include "flint_reactor.h"
 
@@ -1051,6 +1054,31 @@ hipError_t flint_reactor_set_max_lds(int bytes) {
   if (e2 != hipSuccess) return e2;
   return hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::jac_kernel<false>),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+}
+
+// Scratch of the allocation-free launches: 64 ints of work counter (256 bytes), then
+// FLINT_SCRATCH_TEAMS Jacobian slots of 32 x 33 doubles (17.3 MB).  Each GPU holds its own copy
+// for the life of the process; every allocation-free launch on a GPU shares it, so two such
+// launches that overlap on one GPU race.
+static __device__ __align__(256) double scratch[32 + (size_t)FLINT_SCRATCH_TEAMS * 32 * 33];
+
+hipError_t flint_reactor_scratch(int** counter, double** jac_slots) {
+  static std::mutex mu;
+  static std::vector<double*> addr;  // per device; the host-side name of `scratch` is no address
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> g(mu);
+  if ((int)addr.size() <= dev) addr.resize(dev + 1, nullptr);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    addr[dev] = static_cast<double*>(p);
+  }
+  *counter = reinterpret_cast<int*>(addr[dev]);
+  *jac_slots = addr[dev] + 32;
+  return hipSuccess;
 }
 
 size_t flint_reactor_team_bytes(const flint_reactor::Prog& P) { return flint_reactor::team_lds_bytes(P); }

@@ -39,6 +39,7 @@ def lib():
         L.flint_rhs.argtypes = [vp, l, l, vp, vp, vp, vp, vp]
         L.flint_reactor_advance.argtypes = [vp, l, l, i, vp, vp, d, d, d, i, vp, vp, vp]
         L.flint_reactor_advance_graph.argtypes = [vp, l, l, i, vp, vp, d, d, d, i, vp, vp, vp]
+        L.flint_reactor_advance_noalloc.argtypes = [vp, l, l, i, vp, vp, d, d, d, i, vp, vp, vp]
         L.flint_reactor_jacobian.argtypes = [vp, l, l, i, vp, vp, vp, vp, vp]
         L.flint_reactor_autotune.argtypes = [vp, l, l, i, vp, vp, d, d, d, C.POINTER(d)]
         L.flint_reactor_set_launch.argtypes = [vp, i, i]

@@ -169,6 +169,17 @@ int flint_reactor_advance_graph(flint_mech *m, long n, long ld, int mode, double
                                 double t_end, double rtol, double atol, int max_steps, int *stats_dev, int *status_dev,
                                 void *stream);
 
+/* No counterpart in the reference: flint_reactor_advance for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernel and results; it allocates nothing: the per-team Jacobian slots and the work
+ * counter live in a static device array of the library (room for 2048 teams, 17.3 MB per GPU,
+ * held until the process ends), and the handle's program was uploaded by flint_mech_create.  A
+ * launch shape (flint_reactor_set_launch) of more than 2048 teams is rejected with FLINT_EARG.
+ * All calls on one GPU share that array: calls that overlap in time on one GPU race. */
+int flint_reactor_advance_noalloc(flint_mech *m, long n, long ld, int mode, double *X_dev, const double *p_dev,
+                                  double t_end, double rtol, double atol, int max_steps, int *stats_dev,
+                                  int *status_dev, void *stream);
+
 /* Host-memory convenience for the reactor batch (the time loop of test-batchF.f90:99-108, which
  * advances one reactor through OSlo's run_odesolver with rhs_native): copies X[(ns+1)][n] and p[n] in,
  * runs flint_reactor_advance with ld = n on the library's own stream, copies X and status[n] (may
