@@ -132,6 +132,11 @@ extern "C" int flint_mech_create(const char* mech_name, int ns, const double* W,
     }
   std::copy(m->h_tabh.begin() + (size_t)(nTh - 1) * 2 * ns, m->h_tabh.begin() + (size_t)nTh * 2 * ns,
             m->h_tabh.begin() + (size_t)nTh * 2 * ns);
+  m->t_cp.assign(cp, cp + (size_t)ns * nTh);
+  m->t_h.assign(h, h + (size_t)ns * nTh);
+  m->t_s.assign((size_t)ns * nTh, 0.0);
+  m->t_dcpi.assign((size_t)ns * nTh, 0.0);
+  for (int k = 0; k < ns; ++k) m->species.push_back("S" + std::to_string(k + 1));
   int rc = flint_mech_upload(m);
   if (!rc) rc = flint_reactor_prepare(m);
   if (rc) {
@@ -169,6 +174,40 @@ extern "C" void flint_mech_destroy(flint_mech* m) {
   if (m->d_stage) (void)hipFree(m->d_stage);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
+}
+
+extern "C" int flint_mech_thermo(const flint_mech* m, double* W, double* cp, double* h, double* s, double* dcpi) {
+  if (!m) return flint_fail(FLINT_EARG, "null mechanism");
+  if (W) std::copy(m->W.begin(), m->W.end(), W);
+  if (cp) std::copy(m->t_cp.begin(), m->t_cp.end(), cp);
+  if (h) std::copy(m->t_h.begin(), m->t_h.end(), h);
+  if (s) std::copy(m->t_s.begin(), m->t_s.end(), s);
+  if (dcpi) std::copy(m->t_dcpi.begin(), m->t_dcpi.end(), dcpi);
+  return FLINT_OK;
+}
+
+extern "C" int flint_mech_names(const flint_mech* m, int kind, char* buf, int len, int* count) {
+  if (!m) return flint_fail(FLINT_EARG, "null mechanism");
+  if (kind != 0 && kind != 1) return flint_fail(FLINT_EARG, "kind must be 0 (species) or 1 (elements)");
+  const std::vector<std::string>& v = kind == 0 ? m->species : m->elements;
+  if (count) *count = (int)v.size();
+  if (!buf) return FLINT_OK;
+  if (len <= 0) return flint_fail(FLINT_EARG, "need len > 0");
+  for (size_t i = 0; i < v.size(); ++i) {
+    char* d = buf + i * (size_t)len;
+    std::fill(d, d + len, ' ');
+    std::copy(v[i].begin(), v[i].begin() + std::min<size_t>(v[i].size(), (size_t)len), d);
+  }
+  return FLINT_OK;
+}
+
+extern "C" int flint_mech_composition(const flint_mech* m, double* comp, double* el_weight) {
+  if (!m) return flint_fail(FLINT_EARG, "null mechanism");
+  if (comp) std::copy(m->comp.begin(), m->comp.end(), comp);
+  if (el_weight) {
+    for (size_t e = 0; e < (size_t)m->ne; ++e) el_weight[e] = e < m->el_weight.size() ? m->el_weight[e] : 0.0;
+  }
+  return FLINT_OK;
 }
 
 extern "C" int flint_mech_query(const flint_mech* m, int* ns, int* Tk_min, int* Tk_max, int* Th_min, int* Th_max) {
@@ -283,6 +322,23 @@ extern "C" int flint_source_host(flint_mech* m, long n, const double* rho, const
   HIPCHK(hipMemcpyAsync(d_rho, rho, ns * n * sizeof(double), hipMemcpyHostToDevice, m->stream));
   HIPCHK(hipMemcpyAsync(d_T, T, n * sizeof(double), hipMemcpyHostToDevice, m->stream));
   if (int rc = flint_source(m, n, n, d_rho, d_T, d_w, nullptr, nullptr, m->stream)) return rc;
+  HIPCHK(hipMemcpyAsync(wdot, d_w, ns * n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  HIPCHK(hipStreamSynchronize(m->stream));
+  return FLINT_OK;
+}
+
+extern "C" int flint_source_general_host(flint_mech* m, long n, const double* rho, const double* T, double* wdot) {
+  if (int rc = check_batch(m, n, n)) return rc;
+  if (n == 0) return FLINT_OK;
+  std::lock_guard<std::mutex> g(m->mu);
+  const size_t ns = m->ns;
+  if (int rc = flint_stage(m, (2 * ns + 1) * (size_t)n)) return rc;
+  double* d_rho = m->d_stage;
+  double* d_T = d_rho + ns * n;
+  double* d_w = d_T + n;
+  HIPCHK(hipMemcpyAsync(d_rho, rho, ns * n * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  HIPCHK(hipMemcpyAsync(d_T, T, n * sizeof(double), hipMemcpyHostToDevice, m->stream));
+  if (int rc = flint_source_general(m, n, n, d_rho, d_T, d_w, nullptr, m->stream)) return rc;
   HIPCHK(hipMemcpyAsync(wdot, d_w, ns * n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
   HIPCHK(hipStreamSynchronize(m->stream));
   return FLINT_OK;

@@ -24,6 +24,11 @@ struct flint_mech {
   const flint_mech_ops* ops = nullptr;
   std::vector<double> W, comp;       // comp[ne*ns], species_composition(ne, ns) column-major
   std::vector<double> h_tabk, h_tabh;  // host copies of the device tables
+  // what FLINT's loaders leave in FLINT_Lib_Thermodynamic (Load_ThermoTransport.f90:30-126):
+  // species and element names, per-species thermo tables (Th_min..Th_max) x ns column-major,
+  // and the element weights of CEA_initialize_global (0 for an unknown symbol)
+  std::vector<std::string> species, elements;
+  std::vector<double> t_cp, t_h, t_s, t_dcpi, el_weight;
   double* d_tabk = nullptr;
   double* d_tabh = nullptr;
   double* d_W = nullptr;

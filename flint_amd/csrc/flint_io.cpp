@@ -146,18 +146,20 @@ extern "C" int flint_mech_load(const char* folder, const char* prefix, flint_mec
   const int start = Th_min;
   if (Th_min == 1) Th_min = 0;  // Load_ThermoTransport.f90:70,80-85
   const int nTh = Th_max - Th_min + 1;
-  std::vector<double> cp((size_t)ns * nTh), h((size_t)ns * nTh), sv((size_t)ns * nTh);
+  std::vector<double> cp((size_t)ns * nTh), h((size_t)ns * nTh), sv((size_t)ns * nTh), dc((size_t)ns * nTh);
   for (int k = 0; k < ns; ++k) {
     const auto& v = th.vars[k];
     for (int i = 0; i < th.npts; ++i) {
       cp[(size_t)k * nTh + (start - Th_min) + i] = v[(size_t)0 * th.npts + i];
       h[(size_t)k * nTh + (start - Th_min) + i] = v[(size_t)1 * th.npts + i];
       sv[(size_t)k * nTh + (start - Th_min) + i] = v[(size_t)2 * th.npts + i];
+      dc[(size_t)k * nTh + (start - Th_min) + i] = v[(size_t)3 * th.npts + i];
     }
     if (start != Th_min) {
       cp[(size_t)k * nTh] = v[0];
       h[(size_t)k * nTh] = 0.0;
       sv[(size_t)k * nTh] = v[(size_t)2 * th.npts];
+      dc[(size_t)k * nTh] = v[(size_t)3 * th.npts];
     }
   }
   // chemistry-info.txt
@@ -248,6 +250,11 @@ extern "C" int flint_mech_load(const char* folder, const char* prefix, flint_mec
                              nt ? troe.data() : nullptr, nl, nl ? lind.data() : nullptr, Th_min, Th_max, cp.data(),
                              h.data(), ne, ne ? comp.data() : nullptr, out);
   if (rc) return rc;
+  (*out)->species = names;
+  (*out)->elements = elements;
+  (*out)->t_s = sv;
+  (*out)->t_dcpi = dc;
+  for (const auto& e : elements) (*out)->el_weight.push_back(flint_atomic_weight(e.c_str()));
   if (general) {
     if (int rc2 = flint_general_setup(*out, na, nt, nl, g1.data(), g2.data(), ge.data())) {
       flint_mech_destroy(*out);

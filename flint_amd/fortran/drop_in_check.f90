@@ -1,5 +1,8 @@
-! Calls every procedure of module flint_amd the way a FLINT caller does and writes the results
-! at full precision, so that they can be compared with the same calls made through the C ABI.
+! A FLINT caller, written against FLINT's own module names and call sequence: the use lines of
+! src/test/Fortran/test-CEA.f90:1-5 and test-wdot.f90:2-6, read_idealgas_thermo / read_chemistry /
+! Assign_Mechanism / Chemistry_Source as test-wdot.f90:27-51 calls them, CEA_initialize_global /
+! CEA_solve as test-CEA.f90:19-32, and the deallocations of test-CEA.f90:46-50.  It writes every
+! result at full precision, so that the same calls made through the C ABI can be compared.
 !
 !   drop_in_check <native input folder> <states file> <results file>
 !
@@ -9,16 +12,26 @@
 !               t_end rtol atol                              (reactor integration)
 !               q nd, then q lines "T0 rho_1 .. rho_ns u_1 .. u_nd p"  (prim2cons, cons2prim, and
 !               co_k_mi_lam_Wilke at (rho_k, sum rho_k, T0) when the folder has transport.dat)
-! results file: blocks "# name rows cols" followed by the rows.
+! results file: blocks "# name rows cols" followed by the rows; "# species_names ns" followed by
+!               the names.
 program drop_in_check
+  use FLINT_Lib_Thermodynamic
+  use FLINT_Load_ThermoTransport
+  use FLINT_CEA_setup
+  use FLINT_CEA_solver
+  use FLINT_Load_chemistry
+  use FLINT_Lib_Chemistry_data
+  use FLINT_Lib_Chemistry_wdot
+  use FLINT_Lib_Chemistry_rhs
+  use FLINT_CEA_data, only: Vol
   use flint_amd
   implicit none
   character(len=512) :: folder, fin, fout
-  character(len=64) :: mech_name
+  character(len=32) :: mech_name
   integer :: err, n, m, k, i, u, v, nz
-  real(8), allocatable :: T(:), rho(:, :), w1(:, :), w2(:, :), F1(:, :), F2(:, :), J(:, :, :)
+  real(8), allocatable :: T(:), rho(:, :), w1(:, :), w2(:, :), w3(:, :), F1(:, :), F2(:, :), J(:, :, :)
   real(8), allocatable :: Tc(:), pc(:), Yc(:, :), To(:), Yo(:, :), Tb(:), Yb(:, :)
-  real(8), allocatable :: pr(:), X(:, :), row(:)
+  real(8), allocatable :: pr(:), X(:, :), row(:), Rm(:, :)
   integer, allocatable :: st(:)
   real(8) :: t_end, rtol, atol
   integer :: q, nd, npr
@@ -33,7 +46,7 @@ program drop_in_check
 
   err = read_idealgas_thermo(trim(folder))
   if (err /= 0) stop 2
-  err = read_chemistry(folder=trim(folder), mech_name=mech_name)
+  err = read_chemistry( folder=trim(folder), mech_name=mech_name )
   if (err /= 0) stop 3
   call Assign_Mechanism(mech_name)
   nz = ns + 1
@@ -75,17 +88,44 @@ program drop_in_check
 
   open(newunit=v, file=trim(fout), status='replace', action='write')
 
+  ! the module arrays FLINT's loaders fill, as callers read them
+  write(v, '(A,1X,A,1X,I0)') '#', 'species_names', ns
+  do i = 1, ns
+    write(v, '(A)') trim(species_names(i))
+  end do
+  call put(v, 'wm_tab', reshape([wm_tab, Ri_tab], [ns, 2]))
+  call put(v, 'thermo_rows', reshape([cp_tab(Tmin, :), cp_tab(1000, :), cp_tab(Tmax, :), h_tab(1000, :), &
+                                      s_tab(1000, :), dcpi_tab(1000, :)], [ns, 6]))
+  call put(v, 'composition', transpose(species_composition))
+  allocate(Rm(n, 3))
+  do i = 1, n
+    Rm(i, 1) = f_Rtot(rho(i, :))
+    Rm(i, 2) = f_cp(rho(i, :), T(i), sum(rho(i, :)))
+    Rm(i, 3) = H0i(1, T(i))
+  end do
+  call put(v, 'f_Rtot_f_cp_H0i', Rm)
+
   ! per-cell calls, as a CFD code makes them
   allocate(w1(n, ns), F1(n, nz))
   do i = 1, n
     row(1:ns) = rho(i, :)
-    call chemistry_source(row(1:ns), T(i), w1(i, :))
+    call Chemistry_Source(row(1:ns), T(i), w1(i, :))
     row(1:ns) = rho(i, :)
     row(nz) = T(i)
     call rhs_native(nz, 0d0, row(1:nz), F1(i, :))
   end do
   call put(v, 'chemistry_source', w1)
   call put(v, 'rhs_native', F1)
+
+  ! an unknown mechanism name: FLINT warns and falls back to the general routine
+  call Assign_Mechanism('nemo')
+  allocate(w3(n, ns))
+  do i = 1, n
+    row(1:ns) = rho(i, :)
+    call chemistry_source(row(1:ns), T(i), w3(i, :))
+  end do
+  call put(v, 'general', w3)
+  call Assign_Mechanism(mech_name)
 
   ! the same cells in one call each
   allocate(w2(n, ns), F2(n, nz), J(n, nz, nz))
@@ -99,6 +139,7 @@ program drop_in_check
 
   ! equilibrium, one state at a time (UV, FLINT's default) and batched (HP)
   call CEA_initialize_global()
+  call put(v, 'el_weight', reshape(el_weight, [ne, 1]))
   allocate(To(m), Yo(m, ns), Tb(m), Yb(m, ns), st(max(m, k)))
   do i = 1, m
     call CEA_solve(pc(i), Tc(i), Yc(i, :), To(i), Yo(i, :))
@@ -139,6 +180,11 @@ program drop_in_check
   end if
   close(v)
 
+  ! the caller owns the thermodynamic arrays (test-CEA.f90:46-50, test-wdot.f90:93-95)
+  deallocate(wm_tab); deallocate(Ri_tab)
+  deallocate(h_tab); deallocate(cp_tab); deallocate(dcpi_tab); deallocate(s_tab)
+  deallocate(species_names); deallocate(elements_names)
+  deallocate(species_composition); deallocate(el_weight)
   call free_chemistry_data()
 
 contains

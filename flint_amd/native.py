@@ -51,6 +51,7 @@ def lib():
         L.flint_rhs_host.argtypes = [vp, l, vp, vp, vp]
         L.flint_general_setup.argtypes = [vp, i, i, i, vp, vp, vp]
         L.flint_source_general.argtypes = [vp, l, l, vp, vp, vp, vp, vp]
+        L.flint_source_general_host.argtypes = [vp, l, vp, vp, vp]
         L.flint_cons2prim.argtypes = [vp, l, l, i] + [vp] * 7
         L.flint_prim2cons.argtypes = [vp, l, l, i, vp, vp, vp, vp]
         L.flint_cons2prim_host.argtypes = [vp, l, i] + [vp] * 5
@@ -288,6 +289,14 @@ class Mechanism:
         T = np.ascontiguousarray(T, dtype=np.float64)
         w = np.empty_like(rho)
         check(lib().flint_source_host(self._h, T.size, _p(rho), _p(T), _p(w)))
+        return w
+
+    def source_general_host(self, rho, T):
+        """The general routine over host arrays rho [ns, n], T [n] (flint_source_general_host)."""
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        w = np.empty_like(rho)
+        check(lib().flint_source_general_host(self._h, T.size, _p(rho), _p(T), _p(w)))
         return w
 
     def rhs_host(self, Z, jac=False):

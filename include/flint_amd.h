@@ -65,6 +65,23 @@ void flint_mech_destroy(flint_mech *m);
 /* ns, and the table ranges [Tk_min, Tk_max] (kinetics) and [Th_min, Th_max] (thermo). */
 int flint_mech_query(const flint_mech *m, int *ns, int *Tk_min, int *Tk_max, int *Th_min, int *Th_max);
 
+/* What FLINT's loaders leave in FLINT_Lib_Thermodynamic's module arrays (read_idealgas_thermo and
+ * read_idealgas_composition, Load_ThermoTransport.f90:30-126), for callers that read them:
+ *   flint_mech_thermo       W[ns] (wm_tab) and the thermo tables cp, h, s, dcpi as
+ *                           (Th_max - Th_min + 1) x ns column-major arrays (cp_tab(Tmin:Tmax, ns)
+ *                           ...); any pointer may be NULL; s and dcpi are 0 for a handle made by
+ *                           flint_mech_create (it is given cp and h only);
+ *   flint_mech_names        kind 0 species names, kind 1 element names (species_names,
+ *                           elements_names), blank-padded into count x len characters; *count
+ *                           receives the number (buf may be NULL to query it);
+ *   flint_mech_composition  species_composition(ne, ns) column-major into comp[ne*ns], and the
+ *                           element weights CEA_initialize_global takes from its table by symbol
+ *                           (Lib_CEA_setup.f90:157-170) into el_weight[ne] (0 for an unknown
+ *                           symbol); either may be NULL. */
+int flint_mech_thermo(const flint_mech *m, double *W, double *cp, double *h, double *s, double *dcpi);
+int flint_mech_names(const flint_mech *m, int kind, char *buf, int len, int *count);
+int flint_mech_composition(const flint_mech *m, double *comp, double *el_weight);
+
 /* chemistry_source over a batch (Lib_Chemistry_wdot.f90:6-20 procedure pointer, e.g.
  * TSR-GP-24.f90:4-415): rho_dev[ns][ld], T_dev[n] -> wdot_dev[ns][ld] in kg/m^3/s.
  * hrr_dev (may be NULL) receives the heat release rate -sum_k h_k(T) wdot_k [W/m^3].
@@ -86,6 +103,11 @@ int flint_general_setup(flint_mech *m, int na, int nt, int nl, const double *ni1
                         const double *eps);
 int flint_source_general(const flint_mech *m, long n, long ld, const double *rho_dev, const double *T_dev,
                          double *wdot_dev, int *status_dev, void *stream);
+
+/* Host-memory form of flint_source_general (the general routine of Lib_Chemistry_wdot.f90:96-221
+ * that FLINT's Assign_Mechanism falls back to): rho[ns][n], T[n] in, wdot[ns][n] out, staged
+ * through the handle's buffer on its own stream, synchronous. */
+int flint_source_general_host(flint_mech *m, long n, const double *rho, const double *T, double *wdot);
 
 /* rhs_native over a batch (Lib_Chemistry_rhs.f90:15-52): Z_dev[ns+1][ld] = (rho_1..rho_ns, T)
  * -> F_dev[ns+1][ld] = (wdot, dT/dt) of the constant-volume reactor; F = -1 where T is outside the

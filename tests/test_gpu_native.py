@@ -4,10 +4,12 @@
   composition.txt, chemistry-info.txt and Tecplot tables, Load_ThermoTransport.f90:7-126,
   Load_Chemistry.f90:7-164) gives a mechanism whose source terms, Jacobians, equilibria and
   reactor steps are bit-identical to the one built from the same tables in memory;
-* the Fortran module (flint_amd/fortran/flint_amd.f90) called the way a FLINT caller calls
-  read_idealgas_thermo / read_chemistry / Assign_Mechanism / chemistry_source / rhs_native /
-  CEA_solve / cons2prim / prim2cons returns, per cell, exactly what the C ABI returns for the
-  same cells."""
+* the Fortran drop-in (flint_amd/fortran/flint_amd.f90: FLINT's modules FLINT_Lib_Thermodynamic,
+  FLINT_Load_ThermoTransport, FLINT_Load_Chemistry, FLINT_Lib_Chemistry_data,
+  FLINT_Lib_Chemistry_wdot, FLINT_Lib_Chemistry_rhs, FLINT_CEA_setup, FLINT_CEA_solver), used by a
+  program with FLINT's own use lines and call sequence, fills the module arrays a FLINT caller
+  reads and deallocates, falls back to the general routine for an unknown mechanism name with
+  FLINT's warning, and returns, per cell, exactly what the C ABI returns for the same cells."""
 import os
 import subprocess
 
@@ -79,17 +81,18 @@ def test_loaded_folder_matches_in_memory_tables(native, cuda):
 
 
 def _blocks(path):
-    out, name, rows = {}, None, []
+    """'# name rows cols' numeric blocks and '# name count' blocks of text lines."""
+    out, name, rows, text = {}, None, [], False
     with open(path) as f:
         for line in f:
             if line.startswith("#"):
                 if name:
-                    out[name] = np.array(rows)
-                _, name, _, _ = line.split()
-                rows = []
+                    out[name] = rows if text else np.array(rows)
+                head = line.split()
+                name, text, rows = head[1], len(head) == 3, []
             else:
-                rows.append([float(v) for v in line.split()])
-    out[name] = np.array(rows)
+                rows.append(line.strip() if text else [float(v) for v in line.split()])
+    out[name] = rows if text else np.array(rows)
     return out
 
 
@@ -124,9 +127,30 @@ def test_fortran_drop_in_matches_c_abi(native, tmp_path):
     r = subprocess.run([EXE, folder, str(fin), str(fout)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     got = _blocks(fout)
-    # the same cells through the C ABI (host-memory entry points, in-memory tables)
+    # FLINT_Lib_Thermodynamic's arrays as the caller reads them: names, W, R_k, thermo tables over
+    # Tmin:Tmax, composition, and the accessors over them
+    assert got["species_names"] == list(md.species)
+    assert _eq(got["wm_tab"], np.stack([md.W, 8314.51 / md.W], 1))
+    r0 = lambda T0: T0 - md.Th_min
+    th = got["thermo_rows"]
+    assert _eq(th[:, 0], md.cp[0]) and _eq(th[:, 1], md.cp[r0(1000)]) and _eq(th[:, 2], md.cp[-1])
+    assert _eq(th[:, 3], md.h[r0(1000)]) and _eq(th[:, 4], md.s[r0(1000)]) and np.all(np.isfinite(th[:, 5]))
+    assert _eq(got["composition"], np.asarray(md.composition).T)
     rho, T = S.soa(st["rho"]), st["T"]
+    Rk = 8314.51 / md.W
+    Ti = T.astype(int)
+    fr = (T - Ti)[:, None]
+    cpT = md.cp[Ti - md.Th_min] + (md.cp[Ti + 1 - md.Th_min] - md.cp[Ti - md.Th_min]) * fr
+    h1 = md.h[Ti - md.Th_min, 0] + (md.h[Ti + 1 - md.Th_min, 0] - md.h[Ti - md.Th_min, 0]) * fr[:, 0]
+    ref = np.stack([(Rk[:, None] * rho / rho.sum(0)).sum(0), (rho / rho.sum(0) * cpT.T).sum(0), h1], 1)
+    assert np.allclose(got["f_Rtot_f_cp_H0i"], ref, rtol=1e-14, atol=0)
+    from flint_amd.elements import element_weights
+    assert _eq(got["el_weight"][:, 0], element_weights(md.elements))
+    # the same cells through the C ABI (host-memory entry points, in-memory tables)
     w = mm.source_host(rho, T)
+    mm.general_setup(md.general)
+    assert _eq(got["general"], mm.source_general_host(rho, T).T)
+    assert "defaulting to the general procedure" in r.stdout
     F, J = mm.rhs_host(np.vstack([rho, T[None]]), jac=True)
     F1 = np.stack([mm.rhs_host(np.concatenate([rho[:, i], T[i:i + 1]])[:, None])[:, 0] for i in range(n)], 1)
     assert _eq(got["chemistry_source"], w.T) and _eq(got["chemistry_source_batch"], w.T)
