@@ -427,6 +427,22 @@ def cea_leg(args, dev, stream):
                warm_start=dict(ms=ms2, states_per_s=n / (ms2 * 1e-3), iters_mean=float(it2.float().mean()),
                                iters_max=int(it2.max()), failed=int((s2 != 0).sum()),
                                max_abs_dT_vs_default=float((T2 - T1)[ok].abs().max())))
+    # the tightened convergence tests (flint_cea_set_tolerance(1e-10)): both starts meet the
+    # workload's 1e-6 K / 1e-10 bar against each other
+    try:
+        mech.cea_tolerance(1e-10)
+        run1(), run2()
+        mt1, mt2 = timed(stream, run1, 5), timed(stream, run2, 5)
+    finally:
+        mech.cea_tolerance(0.0)
+    ok = (s1 == 0) & (s2 == 0)
+    res["tight_1e-10"] = dict(
+        default=dict(ms=mt1, states_per_s=n / (mt1 * 1e-3), iters_mean=float(it1.float().mean()),
+                     iters_max=int(it1.max()), failed=int((s1 != 0).sum())),
+        warm_start=dict(ms=mt2, states_per_s=n / (mt2 * 1e-3), iters_mean=float(it2.float().mean()),
+                        iters_max=int(it2.max()), failed=int((s2 != 0).sum())),
+        speedup=mt1 / mt2, max_abs_dT_warm_vs_default=float((T2 - T1)[ok].abs().max()),
+        max_rel_dY_warm_vs_default=float(((Y2 - Y1).abs() / Y1.abs().max(0).values)[:, ok].max()))
     pj = profile_json("cea_kernel_tsr_gp_24_262144.fp64.json")
     if pj.get("fp64_flops_per_launch"):
         a = pj["fp64_flops_per_launch"] / (ms1 * 1e-3)

@@ -13,6 +13,7 @@ struct CeaProg {
   int redo;  // nonzero: solve only the states whose status is nonzero, from the default guess, adding
              // to their iteration counts (the second pass of a warm-started batch)
   int max_iter;        // iterations before status 1 (50 in the reference, Lib_CEA_solver.f90)
+  double tol;          // 0: the reference's convergence tests; > 0: every test at tol (tightened mode)
   const int* idx;      // non-null: thread t solves state idx[t], for t < *count (a compacted redo list)
   const int* count;
 };

@@ -21,7 +21,7 @@ struct flint_cea_data {
   double *G = nullptr, *scal = nullptr;
   int* st = nullptr;
   int* redo_idx = nullptr;  // [ld + 1]: compacted list of the states to solve again, then its length
-  int warm_iters = 5;       // iteration cap of the warm-started pass (flint_cea_warm_options)
+  int warm_iters = 6;       // iteration cap of the warm-started pass (flint_cea_warm_options)
   int cus = 0;              // compute units of the device (grid of the fused warm-start kernel)
   bool warm_fused = true;   // features, network and start in one kernel where the network fits
   void* wpk = nullptr;      // the network's weights packed for that kernel ...
@@ -91,6 +91,15 @@ extern "C" int flint_cea_setup(flint_mech* m, const double* s, const double* el_
   P.Th_min = m->Th_min;
   P.Th_max = m->Th_max;
   P.max_iter = 50;
+  P.tol = 0.0;
+  return FLINT_OK;
+}
+
+extern "C" int flint_cea_set_tolerance(flint_mech* m, double tol) {
+  if (!m) return flint_fail(FLINT_EARG, "null mechanism");
+  if (!m->cea) return flint_fail(FLINT_EARG, "flint_cea_setup has not been called for this mechanism");
+  if (!(tol >= 0.0) || tol > 1e-4) return flint_fail(FLINT_EARG, "tolerance must be 0 (the reference's tests) or in (0, 1e-4]");
+  m->cea->P.tol = tol;
   return FLINT_OK;
 }
 

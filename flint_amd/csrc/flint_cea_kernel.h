@@ -12,6 +12,9 @@
 // Arithmetic follows the reference's operation order with contraction off (scoped to each
 // function); only exp/log differ from the host's in the last bit.
 //
+// Tightened mode (P.tol > 0, no counterpart in the reference): the relative-correction, |dln T|
+// and element-residual tests of Lib_CEA_solver.f90:345-364 all at tol, so that the converged state
+// does not depend on the start (the learned warm start and the default guess agree to round-off).
 // Status per state: 0 converged, 1 more than P.max_iter iterations (the reference's maxitn = 50,
 // Lib_CEA_solver.f90:68,318), 2 singular Newton matrix after the reference's retries (its
 // component switch, Lib_CEA_solver.f90:725-848, is not taken), 3 temperature outside the thermo
@@ -180,7 +183,7 @@ __global__ __launch_bounds__(64) void cea_kernel(CeaProg P, long n, long ld, con
       double bigb = 0.0;
 #pragma unroll
       for (int k = 0; k < NE; ++k) bigb = fmax(bigb, fabs(B0[k]));
-      Bcheck = bigb * .000001;
+      Bcheck = P.tol > 0.0 ? bigb * P.tol : bigb * .000001;
     }
   }
 
@@ -345,9 +348,9 @@ __global__ __launch_bounds__(64) void cea_kernel(CeaProg P, long n, long ld, con
           rc = 1;
           break;
         }
-        if (fabs(xq1 * Enn / Totn) > (double)0.5E-5f) continue;
-        if (dmax / Totn > 0.5e-5) continue;
-        if (fabs(dlnt) > 1.e-04) continue;
+        if (fabs(xq1 * Enn / Totn) > (P.tol > 0.0 ? P.tol : (double)0.5E-5f)) continue;
+        if (dmax / Totn > (P.tol > 0.0 ? P.tol : 0.5e-5)) continue;
+        if (fabs(dlnt) > (P.tol > 0.0 ? P.tol : 1.e-04)) continue;
         bool again = false;
 #pragma unroll
         for (int i = 0; i < NE; ++i)

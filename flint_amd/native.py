@@ -63,6 +63,7 @@ def lib():
         L.flint_cea_solve_host.argtypes = [vp, l, i, vp, vp, vp, vp, vp, vp]
         L.flint_cea_solve_warm.argtypes = [vp, vp, l, l, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.flint_cea_warm_options.argtypes = [vp, i, i]
+        L.flint_cea_set_tolerance.argtypes = [vp, d]
         _lib = L
     return _lib
 
@@ -88,6 +89,29 @@ def _stream(stream):
     if isinstance(stream, int):
         return C.c_void_p(stream)
     return C.c_void_p(stream.cuda_stream)
+
+
+def _chk(t, name, ld, rows=None, dtype="float64", optional=False):
+    """A device array the kernels read or write: [rows, ld] (structure of arrays with leading
+    dimension ld), or, with rows None, a vector of at least ld entries (callers pass the cell
+    count n: the kernels index vectors by cell); a contiguous torch tensor of the given dtype on
+    the GPU.  Anything else would make the kernel read or write past the allocation."""
+    if t is None:
+        if optional:
+            return
+        raise FlintError(f"{name}: array required")
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise FlintError(f"{name}: must be a torch tensor on the GPU")
+    if t.dtype != getattr(torch, dtype):
+        raise FlintError(f"{name}: dtype {t.dtype}, need {dtype}")
+    if not t.is_contiguous():
+        raise FlintError(f"{name}: must be contiguous")
+    if rows is None:
+        if t.dim() != 1 or t.shape[0] < ld:
+            raise FlintError(f"{name}: shape {tuple(t.shape)}, need [>= {ld}]")
+    elif t.dim() != 2 or t.shape[0] != rows or t.shape[1] != ld:
+        raise FlintError(f"{name}: shape {tuple(t.shape)}, need [{rows}, {ld}]")
 
 
 def _fortran(a):
@@ -157,11 +181,15 @@ class Mechanism:
     def source(self, rho, T, wdot, hrr=None, status=None, n=None, stream=None):
         ld = rho.shape[-1]
         n = ld if n is None else n
+        _chk(rho, "rho", ld, self.ns), _chk(T, "T", n), _chk(wdot, "wdot", ld, self.ns)
+        _chk(hrr, "hrr", n, optional=True), _chk(status, "status", n, dtype="int32", optional=True)
         check(lib().flint_source(self._h, n, ld, _p(rho), _p(T), _p(wdot), _p(hrr), _p(status), _stream(stream)))
 
     def rhs(self, Z, F, J=None, status=None, n=None, stream=None):
         ld = Z.shape[-1]
         n = ld if n is None else n
+        _chk(Z, "Z", ld, self.nz), _chk(F, "F", ld, self.nz), _chk(J, "J", ld, self.nz * self.nz, optional=True)
+        _chk(status, "status", n, dtype="int32", optional=True)
         check(lib().flint_rhs(self._h, n, ld, _p(Z), _p(F), _p(J), _p(status), _stream(stream)))
 
     # ---- the data-driven general routine ----
@@ -176,6 +204,8 @@ class Mechanism:
         """rho [ns, ld], T [ld] -> wdot [ns, ld] with the general routine (flint_source_general)."""
         ld = rho.shape[-1]
         n = ld if n is None else n
+        _chk(rho, "rho", ld, self.ns), _chk(T, "T", n), _chk(wdot, "wdot", ld, self.ns)
+        _chk(status, "status", n, dtype="int32", optional=True)
         check(lib().flint_source_general(self._h, n, ld, _p(rho), _p(T), _p(wdot), _p(status), _stream(stream)))
 
     # ---- conservative <-> primitive variables ----
@@ -185,6 +215,9 @@ class Mechanism:
         ld = cons.shape[-1]
         n = ld if n is None else n
         nd = cons.shape[0] - self.ns - 1
+        _chk(cons, "cons", ld, cons.shape[0]), _chk(T, "T", n), _chk(prim, "prim", ld, cons.shape[0])
+        _chk(T_out, "T_out", n, optional=True), _chk(iters, "iters", n, dtype="int32", optional=True)
+        _chk(status, "status", n, dtype="int32", optional=True)
         check(lib().flint_cons2prim(self._h, n, ld, nd, _p(cons), _p(T), _p(prim), _p(T_out), _p(iters), _p(status),
                                     _stream(stream)))
 
@@ -193,6 +226,8 @@ class Mechanism:
         ld = prim.shape[-1]
         n = ld if n is None else n
         nd = prim.shape[0] - self.ns - 1
+        _chk(prim, "prim", ld, prim.shape[0]), _chk(cons, "cons", ld, prim.shape[0])
+        _chk(status, "status", n, dtype="int32", optional=True)
         check(lib().flint_prim2cons(self._h, n, ld, nd, _p(prim), _p(cons), _p(status), _stream(stream)))
 
     # ---- mixture transport (Wilke) ----
@@ -210,6 +245,8 @@ class Mechanism:
         """rhoi [ns, ld], rho [ld], T [ld] (device) -> mu, lam [ld] by Wilke's rule (flint_transport)."""
         ld = rhoi.shape[-1]
         n = ld if n is None else n
+        _chk(rhoi, "rhoi", ld, self.ns), _chk(rho, "rho", n), _chk(T, "T", n), _chk(mu, "mu", n)
+        _chk(lam, "lam", n), _chk(status, "status", n, dtype="int32", optional=True)
         check(lib().flint_transport(self._h, n, ld, _p(rhoi), _p(rho), _p(T), _p(mu), _p(lam), _p(status),
                                     _stream(stream)))
 
@@ -230,6 +267,9 @@ class Mechanism:
         """Advance X [ns+1, ld] (torch, device) in place from 0 to t_end."""
         ld = X.shape[-1]
         n = ld if n is None else n
+        _chk(X, "X", ld, self.nz), _chk(p, "p", n, optional=mode == 0)
+        _chk(stats, "stats", ld, 8, dtype="int32", optional=True)
+        _chk(status, "status", n, dtype="int32", optional=True)
         fn = lib().flint_reactor_advance_graph if graph else lib().flint_reactor_advance
         check(fn(self._h, n, ld, mode, _p(X), _p(p), t_end, rtol, atol, max_steps, _p(stats), _p(status),
                  _stream(stream)))
@@ -238,10 +278,13 @@ class Mechanism:
         """F [ns+1, ld] and J [(ns+1)^2, ld] of the reactor model at X [ns+1, ld] (torch, device)."""
         ld = X.shape[-1]
         n = ld if n is None else n
+        _chk(X, "X", ld, self.nz), _chk(p, "p", n, optional=mode == 0), _chk(F, "F", ld, self.nz)
+        _chk(J, "J", ld, self.nz * self.nz, optional=True)
         check(lib().flint_reactor_jacobian(self._h, n, ld, mode, _p(X), _p(p), _p(F), _p(J), _stream(stream)))
 
     def autotune(self, X, p=None, mode=0, t_end=1e-4, rtol=1e-6, atol=1e-12):
         ld = X.shape[-1]
+        _chk(X, "X", ld, self.nz), _chk(p, "p", ld, optional=mode == 0)
         best = C.c_double()
         check(lib().flint_reactor_autotune(self._h, ld, ld, mode, _p(X), _p(p), t_end, rtol, atol, C.byref(best)))
         return best.value, self.get_launch()
@@ -264,6 +307,9 @@ class Mechanism:
         """CEA_solve over a batch (torch, device): T [ld], rho [ns, ld] -> T_out [ld], Y_out [ns, ld]."""
         ld = T.shape[-1]
         n = ld if n is None else n
+        _chk(T, "T", n), _chk(rho, "rho", ld, self.ns), _chk(T_out, "T_out", n), _chk(Y_out, "Y_out", ld, self.ns)
+        _chk(p, "p", n, optional=not hp), _chk(guess, "guess", ld, self.nz, optional=True)
+        _chk(iters, "iters", n, dtype="int32", optional=True), _chk(status, "status", n, dtype="int32", optional=True)
         check(lib().flint_cea_solve(self._h, n, ld, int(hp), _p(p), _p(T), _p(rho), _p(guess), _p(T_out), _p(Y_out),
                                     _p(iters), _p(status), _stream(stream)))
 
@@ -278,7 +324,12 @@ class Mechanism:
         check(lib().flint_cea_solve_warm(self._h, net._h, n, ld, _p(p), _p(T), _p(rho), _p(sc), _p(T_out), _p(Y_out),
                                          _p(iters), _p(status), _stream(stream)))
 
-    def warm_options(self, max_iter=5, fused=True):
+    def cea_tolerance(self, tol):
+        """Convergence tests of the equilibrium solves: 0 the reference's, > 0 all at tol
+        (flint_cea_set_tolerance)."""
+        check(lib().flint_cea_set_tolerance(self._h, float(tol)))
+
+    def warm_options(self, max_iter=6, fused=True):
         """Iteration cap of the warm-started pass of equilibrium_warm, and whether its features,
         network and Newton start run as one kernel (flint_cea_warm_options)."""
         check(lib().flint_cea_warm_options(self._h, int(max_iter), int(bool(fused))))

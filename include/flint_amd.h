@@ -230,6 +230,14 @@ int flint_reactor_set_newton(flint_mech *m, int policy);
  * Atmwt table by symbol, Lib_CEA_setup.f90:157-170).  flint_mech_load calls this itself. */
 int flint_cea_setup(flint_mech *m, const double *s, const double *el_weight);
 
+/* Convergence tests of the equilibrium solves of this handle (flint_cea_solve, _host, _warm):
+ * tol = 0 (the default) keeps the reference's (Lib_CEA_solver.f90:345-364: relative corrections
+ * <= 0.5e-5, |dln T| <= 1e-4, element residual <= 1e-6 max b0), whose converged state depends on
+ * the starting point by up to ~1e-4 K; tol in (0, 1e-4] applies tol to all of them (no
+ * counterpart in the reference), at about one more Newton iteration for tol = 1e-10, so that any
+ * start lands on the same equilibrium to round-off. */
+int flint_cea_set_tolerance(flint_mech *m, double tol);
+
 /* CEA_solve over a batch (Lib_CEA_solver.f90:6-856, with CEA_initialize_local per state):
  *   hp = 1  HP: fixed pressure p_dev[n] (bar) and the enthalpy of the input mixture at T_dev[n]
  *           (the reference with Vol = .false.);
@@ -257,7 +265,7 @@ int flint_cea_solve_host(flint_mech *m, long n, int hp, const double *p, const d
  * ns+1 outputs, see flint_mlp_create below) maps the standardised (Y0_k, h0, ln p) of each state
  * (h0 = sum_k Y0_k h_k(T0)) to the standardised Newton start (T, ln n_1..ln n_ns); scaling (host,
  * 4 ns + 6 doubles) = in_mean[ns+2], in_scale[ns+2], out_mean[ns+1], out_scale[ns+1].  The batch is
- * solved as flint_cea_solve with hp = 1 from those starts, held to 5 iterations (see
+ * solved as flint_cea_solve with hp = 1 from those starts, held to 6 iterations (see
  * flint_cea_warm_options), and the states that do not converge within them are solved again from
  * the default start, with the reference's 50-iteration limit, in a second pass over just those
  * states (iters_dev then counts both), so the results are those of the default path wherever it
@@ -269,7 +277,7 @@ int flint_cea_solve_warm(flint_mech *m, flint_mlp *net, long n, long ld, const d
                          int *iters_dev, int *status_dev, void *stream);
 
 /* Options of flint_cea_solve_warm (no reference counterpart), held until the next
- * flint_cea_setup: max_iter (1..50, default 5) caps the warm-started pass, so that the states
+ * flint_cea_setup: max_iter (1..50, default 6) caps the warm-started pass, so that the states
  * that converge slowly from their predicted start move to the second pass (a wavefront runs as
  * many iterations as its slowest state); fused (default 1)
  * computes the features, a 3-layer network (<= 32 inputs, hidden widths 128 and 128, <= 32

@@ -92,11 +92,13 @@ void orc_bdf_method(const orc_mech *m, long n, long ld, int mode, double *X, con
                     double atol, double t_end, int max_steps, int *stats, int *status, int method);
 
 /* CEA-style equilibrium (oracle_cea.c).  A[ne][ns] = species_composition (element rows),
- * elw[ne] = atomic weights of the elements. */
+ * elw[ne] = atomic weights of the elements; tol = 0: the reference's convergence tests, tol > 0:
+ * the tightened mode (every correction test and the element residual at tol, see oracle_cea.c). */
 typedef struct orc_cea_data {
   int ne;
   const double *A;
   const double *elw;
+  double tol;
 } orc_cea_data;
 /* one state: hp = 1 HP (fixed h, p in bar), 0 UV (fixed e, v); rho_in[ns] = amounts (kg or kg/m^3);
  * guess = NULL or [T, ln n_1..ln n_ns]; returns status (0 ok, 1 no convergence, 2 unsupported

@@ -36,7 +36,7 @@ class _OrcMech(C.Structure):
 
 
 class _OrcCea(C.Structure):
-    _fields_ = [("ne", C.c_int), ("A", _P), ("elw", _P)]
+    _fields_ = [("ne", C.c_int), ("A", _P), ("elw", _P), ("tol", C.c_double)]
 
 
 def build():
@@ -223,10 +223,11 @@ class Oracle:
         self.L.orc_jac_cp_cstep(C.byref(self.m), C.c_long(n), C.c_long(n), _ptr(X), _ptr(p), C.c_double(h), _ptr(J))
         return J
 
-    def cea(self, p, T, rho, hp=True, guess=None):
+    def cea(self, p, T, rho, hp=True, guess=None, tol=0.0):
         """Equilibrium of n states (orc_cea): p [n] (bar for HP), T [n] (the input mixture's
         temperature, also the first iterate), rho [ns, n] amounts; guess [ns+1, n] = (T, ln n_k) or
-        None.  Returns (T_eq [n], Y_eq [ns, n], iterations [n], status [n])."""
+        None; tol = 0 the reference's convergence tests, > 0 the tightened mode.  Returns
+        (T_eq [n], Y_eq [ns, n], iterations [n], status [n])."""
         from flint_amd.elements import element_weights
         rho = np.ascontiguousarray(rho, dtype=np.float64)
         ns, n = rho.shape
@@ -239,6 +240,7 @@ class Oracle:
         p = np.ascontiguousarray(p, dtype=np.float64)
         T = np.ascontiguousarray(T, dtype=np.float64)
         g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+        self._cea[0].tol = float(tol)
         To = np.empty(n)
         Yo = np.empty((ns, n))
         it = np.empty(n, dtype=np.int32)

@@ -15,6 +15,11 @@
  * Lib_CEA_solver.f90:725-848, which also rewrites the reference's global element matrix) are
  * reported with status 2 instead of being taken.
  *
+ * Tightened mode (c->tol > 0, no counterpart in the reference): the convergence tests of
+ * Lib_CEA_solver.f90:345-364 (relative corrections of n and of the element potentials <= 0.5e-5,
+ * |dln T| <= 1e-4, element residual <= 1e-6 max b0) all at tol, so that the converged state no
+ * longer depends on the starting point (a warm start and the default guess land within round-off
+ * of each other); everything else is the reference's iteration.
  * Status: 0 converged, 1 more than maxitn = 50 iterations (the reference leaves T_out unset),
  * 2 unsupported branch, 3 temperature outside the thermo table. */
 #include <math.h>
@@ -125,7 +130,8 @@ int orc_cea_cell(const orc_mech *m, const orc_cea_data *c, int hp, double p_in, 
     for (int j = 0; j < nlm; ++j) B0[j] = B0[j] / Wp;
   double bigb = 0.0;
   for (int j = 0; j < nlm; ++j) bigb = fmax(bigb, fabs(B0[j]));
-  const double Bcheck = bigb * .000001;
+  /* tightened mode: the element residual relative to the largest b0 at tol as well */
+  const double Bcheck = c->tol > 0.0 ? bigb * c->tol : bigb * .000001;
 
   /* CEA_solve: initial estimates (Lib_CEA_solver.f90:45-62) */
   double Pp = p_in, Tt = T_in;
@@ -272,12 +278,12 @@ int orc_cea_cell(const orc_mech *m, const orc_cea_data *c, int hp, double p_in, 
       /* convergence tests (Lib_CEA_solver.f90:311-368) */
       if (numb > maxitn) { *iters = numb; return 1; }
       sum = (X[Iq1] * Enn / Totn);
-      if (fabs(sum) > (double)0.5E-5f) continue;
+      if (fabs(sum) > (c->tol > 0.0 ? c->tol : (double)0.5E-5f)) continue;
       int again = 0;
       for (int j = 0; j < ng && !again; ++j)
-        if (fabs(Deln[j]) * En[j] / Totn > 0.5e-5) again = 1;
+        if (fabs(Deln[j]) * En[j] / Totn > (c->tol > 0.0 ? c->tol : 0.5e-5)) again = 1;
       if (again) continue;
-      if (fabs(dlnt) > 1.e-04) continue;
+      if (fabs(dlnt) > (c->tol > 0.0 ? c->tol : 1.e-04)) continue;
       for (int i = 0; i < nlm && !again; ++i) {
         if (fabs(B0[i]) >= 1.e-06) {
           double s = 0.0;

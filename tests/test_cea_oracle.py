@@ -105,3 +105,24 @@ def test_cea_warm_start_from_solution(gp24):
     assert np.all(status2 == 0)
     assert np.max(np.abs(T2 - T)) < 1e-2
     assert it2.mean() < 0.5 * it.mean()
+
+
+def test_cea_tight_mode_is_independent_of_the_start(oracle_mod, mechs):
+    """The tightened mode (tol 1e-10, every convergence test of Lib_CEA_solver.f90:345-364 at tol):
+    solves from the default guess and from a start 5 % off in T and 0.1 off in every ln n_k meet
+    to 1e-8 K and 1e-12 in composition, where the reference's tests leave them 1e-7 K or more
+    apart; tol = 0 is the reference's solver (its golden outputs, test above)."""
+    md = mechs["TSR-GP-24"]
+    o = oracle_mod.Oracle(md)
+    st = S.cea_states(md.species, md.W, 31, 0, 1024)
+    Y0 = S.soa(st["Y"])
+    spread = {}
+    for tol in (0.0, 1e-10):
+        T1, Y1, i1, s1 = o.cea(st["p_bar"], st["T0"], Y0, hp=True, tol=tol)
+        g = np.vstack([1.05 * T1[None], np.log(np.maximum(Y1 / md.W[:, None], 1e-300)) + 0.1])
+        T2, Y2, i2, s2 = o.cea(st["p_bar"], st["T0"], Y0, hp=True, guess=g, tol=tol)
+        ok = (s1 == 0) & (s2 == 0)
+        assert ok.mean() > 0.99
+        spread[tol] = (np.max(np.abs(T2 - T1)[ok]), np.max((np.abs(Y2 - Y1) / np.abs(Y1).max(0))[:, ok]))
+    assert spread[1e-10][0] < 1e-8 and spread[1e-10][1] < 1e-12
+    assert spread[0.0][0] > 1e-7

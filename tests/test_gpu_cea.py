@@ -134,9 +134,52 @@ def test_cea_learned_warm_start(gp24, cuda):
     assert int((s1 != 0).sum()) == 0 and int((s2 != 0).sum()) == 0
     dT = (T2 - T1).abs().cpu().numpy()
     dY = ((Y2 - Y1).abs() / Y1.abs().max(0).values).max(0).values.cpu().numpy()
-    assert dT.max() < 1e-2 and np.quantile(dT, 0.99) < 1e-4
+    # with the reference's convergence tests (corrections <= 0.5e-5, |dln T| <= 1e-4) the converged
+    # state depends on the start: a regression bound near the measured spread (3e-4 K), not the
+    # workload's 1e-6 K bar, which the tightened mode below meets
+    assert dT.max() < 1e-3 and np.quantile(dT, 0.99) < 1e-4
     assert dY.max() < 1e-5
     assert it2.float().mean().item() < 0.5 * it1.float().mean().item()
+
+
+def test_cea_tight_mode_meets_the_bar(gp24, oracle_mod, cuda):
+    """Tightened convergence tests (flint_cea_set_tolerance(1e-10)): the default-guess solve and
+    the learned warm start both land within 1e-6 K and 1e-10 (relative to the largest mass
+    fraction) of the oracle's tightened solve from the default guess, and the warm start still
+    needs fewer Newton iterations."""
+    import torch
+    from flint_amd.warmstart import WarmStart, DEFAULT_MODEL
+    md, mech = gp24
+    ws = WarmStart.load(DEFAULT_MODEL)
+    n = 4096
+    st = S.cea_states(md.species, md.W, 26, 0, n)
+    Yh = S.soa(st["Y"])
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(cuda)
+    Y0, T0, p = dev(Yh), dev(st["T0"]), dev(st["p_bar"])
+    out = []
+    print("[gpu] cea tightened mode", flush=True)
+    try:
+        mech.cea_tolerance(1e-10)
+        for warm in (False, True):
+            T, Y = torch.empty_like(T0), torch.empty_like(Y0)
+            it, s = torch.empty(n, dtype=torch.int32, device=cuda), torch.empty(n, dtype=torch.int32, device=cuda)
+            if warm:
+                ws.solve(mech, md, Y0, T0, p, T, Y, iters=it, status=s)
+            else:
+                mech.equilibrium(T0, Y0, T, Y, p=p, hp=True, iters=it, status=s)
+            torch.cuda.synchronize()
+            out.append([x.cpu().numpy() for x in (T, Y, it, s)])
+    finally:
+        mech.cea_tolerance(0.0)
+    Tr, Yr, itr, sr = oracle_mod.Oracle(md).cea(st["p_bar"], st["T0"], Yh, hp=True, tol=1e-10)
+    assert np.all(sr == 0)
+    for T, Y, it, s in out:
+        assert np.all(s == 0)
+        assert np.max(np.abs(T - Tr)) <= 1e-6
+        assert np.max(np.abs(Y - Yr) / np.abs(Yr).max(0)) <= 1e-10
+    assert out[1][2].mean() < 0.6 * out[0][2].mean()
+    # the default-start solve takes the oracle's iteration count
+    assert np.mean(out[0][2] == itr) > 0.99
 
 
 def test_cea_warm_start_iteration_cap(gp24, cuda):
@@ -183,8 +226,8 @@ def test_cea_warm_start_iteration_cap(gp24, cuda):
 def test_cea_fused_warm_start_matches_layered(gp24, cuda):
     """The one-kernel warm start (features, MFMA network with LDS-resident weights, Newton start)
     against the same network run layer by layer (flint_mlp_forward between the feature and start
-    kernels): both feed the same solver, so the equilibria agree to the solver's tolerances and
-    the iteration counts are those of (nearly) the same starts."""
+    kernels): both feed the same solver (in the tightened mode), so the equilibria agree to 1e-6 K
+    and 1e-10 and the iteration counts are those of (nearly) the same starts."""
     import torch
     from flint_amd.warmstart import WarmStart, DEFAULT_MODEL
     md, mech = gp24
@@ -196,6 +239,7 @@ def test_cea_fused_warm_start_matches_layered(gp24, cuda):
     out = {}
     print("[gpu] cea fused vs layered warm start", flush=True)
     try:
+        mech.cea_tolerance(1e-10)
         for fused in (True, False):
             mech.warm_options(fused=fused)
             T, Y = torch.empty_like(T0), torch.empty_like(Y0)
@@ -205,11 +249,12 @@ def test_cea_fused_warm_start_matches_layered(gp24, cuda):
             out[fused] = [x.cpu().numpy() for x in (T, Y, it, s)]
     finally:
         mech.warm_options()
+        mech.cea_tolerance(0.0)
     (Tf, Yf, itf, sf), (Tl, Yl, itl, sl) = out[True], out[False]
     assert np.all(sf == 0) and np.all(sl == 0)
-    dT = np.abs(Tf - Tl)
-    assert dT.max() < 1e-2 and np.quantile(dT, 0.99) < 1e-4
-    assert np.max(np.abs(Yf - Yl) / np.abs(Yl).max(0)) < 1e-5
+    # under the tightened tests both starts reach the same equilibrium to the workload's bar
+    assert np.max(np.abs(Tf - Tl)) <= 1e-6
+    assert np.max(np.abs(Yf - Yl) / np.abs(Yl).max(0)) <= 1e-10
     assert np.mean(itf == itl) > 0.95 and abs(itf.mean() - itl.mean()) < 0.05 * itl.mean()
     print(f"[gpu] identical iteration counts {np.mean(itf == itl):.4f}, identical T {np.mean(Tf == Tl):.4f}",
           flush=True)
