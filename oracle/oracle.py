@@ -272,10 +272,10 @@ class Oracle:
 class Reference:
     """The reference Fortran modules (oracle/_ref), loaded with one mechanism's tables."""
 
-    def __init__(self, md):
-        if not os.path.exists(REF_LIB):
-            raise FileNotFoundError(REF_LIB)
-        self.L = C.CDLL(REF_LIB)
+    def __init__(self, md, lib_path=REF_LIB):
+        if not os.path.exists(lib_path):
+            raise FileNotFoundError(lib_path)
+        self.L = C.CDLL(lib_path)
         self.md = md
         ns = md.ns
         nTk = md.kf.shape[0]
