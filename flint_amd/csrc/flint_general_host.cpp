@@ -45,42 +45,99 @@ extern "C" int flint_general_setup(flint_mech* m, int na, int nt, int nl, const 
     return flint_fail(FLINT_EARG, "reaction counts differ from the mechanism's rate tables");
   const int ns = m->ns, nz = ns + 1, nr = na + nt + nl;
   if (ns > 64) return flint_fail(FLINT_EUNSUPPORTED, "more than 64 species");
-  std::vector<int> kind(nr), fp(1, 0), fs, rp(1, 0), rs, mp(1, 0), ms, op(1, 0), os;
-  std::vector<double> fe, re, me, ow;
+  std::vector<flint_general::Hdr> hdr(nr);
+  std::vector<flint_general::Ent> ent;
+  auto factor_entry = [](int s, double x, bool arrhenius) {
+    // nint() for Arrhenius (halves away from zero, Lib_Chemistry_wdot.f90:140-141); a real exponent
+    // of a fall-off reaction keeps pow unless it is integer-valued
+    const double e = arrhenius ? std::round(x) : x;
+    return flint_general::Ent{e, s, e == std::rint(e) ? (int)e : flint_general::kRealExp};
+  };
   for (int j = 0; j < nr; ++j) {
     const double* a1 = ni1 + (size_t)j * nz;
     const double* a2 = ni2 + (size_t)j * nz;
     const double* e = eps + (size_t)j * nz;
     const int k = j < na ? 0 : (j < na + nt ? 1 : 2);
-    kind[j] = k | (a1[ns] > 0.0 ? 4 : 0);
     for (int s = 0; s < ns; ++s) {
       if (!std::isfinite(a1[s]) || !std::isfinite(a2[s]) || !std::isfinite(e[s]))
         return flint_fail(FLINT_EARG, "non-finite table entry");
       if (std::fabs(a1[s]) > 64.0 || std::fabs(a2[s]) > 64.0)
         return flint_fail(FLINT_EARG, "stoichiometric coefficient beyond 64");
-      if (a1[s] != 0.0) {
-        fs.push_back(s);
-        fe.push_back(k == 0 ? std::round(a1[s]) : a1[s]);  // nint(): halves away from zero (Lib_Chemistry_wdot.f90:140-141)
-      }
-      if (a2[s] != 0.0) {
-        rs.push_back(s);
-        re.push_back(k == 0 ? std::round(a2[s]) : a2[s]);
-      }
-      if (e[s] != 0.0) {
-        ms.push_back(s);
-        me.push_back(e[s]);
-      }
-      const double d = a2[s] - a1[s];
-      if (d != 0.0) {
-        os.push_back(s);
-        ow.push_back(m->W[s] * d);
-      }
     }
-    fp.push_back((int)fs.size());
-    rp.push_back((int)rs.size());
-    mp.push_back((int)ms.size());
-    op.push_back((int)os.size());
+    flint_general::Hdr& h = hdr[j];
+    h.kind = k | (a1[ns] > 0.0 ? 4 : 0);
+    h.col = k == 0 ? 2 * j : (k == 1 ? 2 * na + 4 * (j - na) : 2 * na + 4 * nt + 3 * (j - na - nt));
+    h.pad = 0;
+    h.m0 = (int)ent.size();
+    for (int s = 0; s < ns; ++s)
+      if (e[s] != 0.0) ent.push_back({e[s], s, 0});
+    h.f0 = (int)ent.size();
+    for (int s = 0; s < ns; ++s)
+      if (a1[s] != 0.0) ent.push_back(factor_entry(s, a1[s], k == 0));
+    h.r0 = (int)ent.size();
+    for (int s = 0; s < ns; ++s)
+      if (a2[s] != 0.0) ent.push_back(factor_entry(s, a2[s], k == 0));
+    h.o0 = (int)ent.size();
+    for (int s = 0; s < ns; ++s) {
+      const double d = a2[s] - a1[s];
+      if (d != 0.0) ent.push_back({m->W[s] * d, s, 0});
+    }
+    h.end = (int)ent.size();
   }
+  // compact records and the distinct efficiency sets (equal rows give bit-identical sums)
+  std::vector<flint_general::Rec> rec(nr);
+  std::vector<int> set_ptr(1, 0), set_first;
+  std::vector<flint_general::Ent> set_ent;
+  for (int j = 0; j < nr; ++j) {
+    const flint_general::Hdr& h = hdr[j];
+    flint_general::Rec& R = rec[j];
+    R = flint_general::Rec{};
+    R.kind = h.kind;
+    R.col = h.col;
+    R.mset = -1;
+    const int nf = h.r0 - h.f0, nrv = h.o0 - h.r0, no = h.end - h.o0;
+    bool fits = nf <= 4 && nrv <= 4 && no <= 8;
+    for (int t = h.f0; t < h.o0 && fits; ++t) fits = ent[t].e >= 1 && ent[t].e <= 255;
+    if (fits && (h.kind & 4)) {
+      int found = -1;
+      for (size_t q = 0; q + 1 < set_ptr.size() && found < 0; ++q) {
+        const int a = set_ptr[q], n1 = set_ptr[q + 1] - a;
+        if (n1 != h.f0 - h.m0) continue;
+        bool same = true;
+        for (int t = 0; t < n1 && same; ++t)
+          same = set_ent[a + t].sp == ent[h.m0 + t].sp && set_ent[a + t].v == ent[h.m0 + t].v;
+        if (same) found = (int)q;
+      }
+      if (found < 0 && (int)set_ptr.size() - 1 < flint_general::kMaxSets) {
+        set_ent.insert(set_ent.end(), ent.begin() + h.m0, ent.begin() + h.f0);
+        set_ptr.push_back((int)set_ent.size());
+        found = (int)set_ptr.size() - 2;
+      }
+      if (found < 0) fits = false;
+      R.mset = found;
+    }
+    if (!fits) {
+      R.kind |= 8;
+      continue;
+    }
+    R.cnt = nf | nrv << 4 | no << 8;
+    for (int q = 0; q < nf; ++q) {
+      R.fsp |= (unsigned)ent[h.f0 + q].sp << (8 * q);
+      R.fexp |= (unsigned)ent[h.f0 + q].e << (8 * q);
+    }
+    for (int q = 0; q < nrv; ++q) {
+      R.rsp |= (unsigned)ent[h.r0 + q].sp << (8 * q);
+      R.rexp |= (unsigned)ent[h.r0 + q].e << (8 * q);
+    }
+    for (int q = 0; q < no; ++q) {
+      (q < 4 ? R.osp0 : R.osp1) |= (unsigned)ent[h.o0 + q].sp << (8 * (q & 3));
+      R.wd[q] = ent[h.o0 + q].v;
+    }
+  }
+  if (set_ent.empty()) set_ent.push_back({0.0, 0, 0});
+  std::vector<int> cols(nr + 2, 0);
+  for (int j = 0; j < nr; ++j) cols[j] = rec[j].col;
+  rec.push_back(flint_general::Rec{});  // read ahead of the last reaction, never used
   flint_mech_release_general(m);
   auto* g = new flint_general_data();
   m->gen = g;
@@ -92,19 +149,13 @@ extern "C" int flint_general_setup(flint_mech* m, int na, int nt, int nl, const 
   P.nl = nl;
   P.wk = m->wk;
   int rc = 0;
-  rc |= put(g, kind, &P.kind);
-  rc |= put(g, fp, &P.f_ptr);
-  rc |= put(g, fs, &P.f_sp);
-  rc |= put(g, fe, &P.f_exp);
-  rc |= put(g, rp, &P.r_ptr);
-  rc |= put(g, rs, &P.r_sp);
-  rc |= put(g, re, &P.r_exp);
-  rc |= put(g, mp, &P.m_ptr);
-  rc |= put(g, ms, &P.m_sp);
-  rc |= put(g, me, &P.m_eps);
-  rc |= put(g, op, &P.o_ptr);
-  rc |= put(g, os, &P.o_sp);
-  rc |= put(g, ow, &P.o_wd);
+  rc |= put(g, hdr, &P.hdr);
+  rc |= put(g, ent, &P.ent);
+  rc |= put(g, rec, &P.rec);
+  rc |= put(g, cols, &P.col);
+  P.nsets = (int)set_ptr.size() - 1;
+  rc |= put(g, set_ptr, &P.set_ptr);
+  rc |= put(g, set_ent, &P.set_ent);
   if (rc) {
     flint_mech_release_general(m);
     return rc;
