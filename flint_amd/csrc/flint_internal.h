@@ -49,6 +49,13 @@ std::string flint_mech_id(const std::string& name);
 int flint_mech_upload(flint_mech* m);
 int flint_stage(flint_mech* m, size_t doubles);
 void flint_mech_release_extra(flint_mech* m);
+// The implicit-step check of predicted states (verify_kernel): cells with flag 0 whose
+// backward-Euler (and, F0 = F(X) given, trapezoidal) Newton corrections over dt are within
+// (tol_y, tol_t) take Xp into X; the others get flag 1 and are appended to idx (count must start
+// at zero).
+int flint_reactor_verify(flint_mech* m, long n, long ld, double* X, const double* Xp, long ldp, const double* F0,
+                         long ldf, const double* p, double dt, double tol_y, double tol_t, int* flag, int* idx,
+                         int* count, double* dmax, void* stream);
 int flint_reactor_prepare(flint_mech* m);  // reactor program of a generated mechanism (no-op otherwise)
 void flint_mech_release_cea(flint_mech* m);
 void flint_mech_release_sur(flint_mech* m);

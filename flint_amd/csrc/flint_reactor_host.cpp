@@ -313,6 +313,24 @@ extern "C" int flint_reactor_jacobian(flint_mech* m, long n, long ld, int mode, 
   return FLINT_OK;
 }
 
+int flint_reactor_verify(flint_mech* m, long n, long ld, double* X, const double* Xp, long ldp, const double* F0,
+                         long ldf, const double* p, double dt, double tol_y, double tol_t, int* flag, int* idx,
+                         int* count, double* dmax, void* stream) {
+  if (int rc = check_args(m, n, ld, 1, X, p, dt, 1.0, 1.0)) return rc;
+  if (n == 0) return FLINT_OK;
+  if (!Xp || !flag || !idx || !count || ldp < n) return flint_fail(FLINT_EARG, "null or short check buffers");
+  if (int rc = ensure_extra(m)) return rc;
+  flint_extra* x = m->extra;
+  const int tpb = x->teams_per_block;
+  const int blocks = (int)std::min<long>(x->blocks, (n + tpb - 1) / tpb);
+  if (int rc = ensure_jac(x, tpb, blocks)) return rc;
+  if (F0 && ldf < n) return flint_fail(FLINT_EARG, "short right-hand side buffer");
+  hipError_t e = flint_reactor_verify_launch(x->prog, n, ld, X, Xp, ldp, F0, ldf, p, dt, tol_y, tol_t, flag, idx,
+                                             count, dmax, tpb, blocks, x->d_jac, (hipStream_t)stream);
+  if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("reactor check launch: ") + hipGetErrorString(e));
+  return FLINT_OK;
+}
+
 extern "C" int flint_reactor_set_launch(flint_mech* m, int teams_per_block, int blocks) {
   if (!m) return flint_fail(FLINT_EARG, "null mechanism");
   if (int rc = ensure_extra(m)) return rc;

@@ -21,8 +21,9 @@ struct SurProg {
 
 hipError_t flint_sur_features(const flint_sur::SurProg& P, long n, long ld, const double* X, const double* p, float* F,
                               hipStream_t s);
-hipError_t flint_sur_apply(const flint_sur::SurProg& P, long n, long ld, double* X, const float* O, int* flag, int* idx,
-                           int* count, hipStream_t s);
+hipError_t flint_sur_apply(const flint_sur::SurProg& P, long n, long ld, const double* X, double* Xp, const float* O,
+                           int* flag, hipStream_t s);
+hipError_t flint_sur_zero(int* p, int n, hipStream_t s);
 hipError_t flint_sur_gather(int dir, int nrow, long ld, double* X, long nf, long ldg, double* Xg, const int* idx,
                             const double* p, double* pg, hipStream_t s);
 hipError_t flint_sur_scatter_int(int nrow, long ld, int* dst, long nf, long ldg, const int* src, const int* idx,

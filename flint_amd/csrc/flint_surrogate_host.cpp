@@ -15,7 +15,7 @@ struct flint_sur_data {
   long cap = 0;
   float *F = nullptr, *O = nullptr;
   int *flag = nullptr, *idx = nullptr, *count = nullptr, *stg = nullptr, *sg = nullptr;
-  double *Xg = nullptr, *pg = nullptr, *scal = nullptr, *comp = nullptr, *elw = nullptr, *Fr = nullptr;
+  double *Xg = nullptr, *pg = nullptr, *scal = nullptr, *comp = nullptr, *elw = nullptr, *Fr = nullptr, *Xp = nullptr;
   std::vector<void*> all;
 };
 
@@ -62,6 +62,7 @@ static int ensure(flint_mech* m, long n) {
   rc |= grab(d, &d->comp, (size_t)ne * ns);
   rc |= grab(d, &d->elw, ne);
   rc |= grab(d, &d->Fr, (size_t)(ns + 1) * n);
+  rc |= grab(d, &d->Xp, (size_t)(ns + 1) * n);
   if (rc) {
     flint_mech_release_sur(m);
     return rc;
@@ -77,9 +78,9 @@ static int ensure(flint_mech* m, long n) {
 extern "C" int flint_reactor_advance_mlp(flint_mech* m, flint_mlp* net, long n, long ld, double* X, const double* p,
                                          double t_end, double rtol, double atol, const double* scaling,
                                          const double* el_weight, double ymin, double yfloor, double tfloor,
-                                         double max_dT, double max_residual, double max_T_mismatch, int* fallback,
-                                         long* n_fallback,
-                                         int* stats, int* status, void* stream) {
+                                         double max_dT, double max_residual, double max_T_mismatch,
+                                         double max_newton_dY, double max_newton_dT, int* fallback, long* n_fallback,
+                                         double* check, int* stats, int* status, void* stream) {
   if (!m || !net || !scaling || !el_weight) return flint_fail(FLINT_EARG, "null argument");
   if (n < 0 || ld < n) return flint_fail(FLINT_EARG, "need 0 <= n <= ld");
   if (n_fallback) *n_fallback = 0;
@@ -91,7 +92,8 @@ extern "C" int flint_reactor_advance_mlp(flint_mech* m, flint_mlp* net, long n, 
   if (net->dims.front() != ns + 2 || net->dims.back() != ns + 1)
     return flint_fail(FLINT_EARG, "network widths must be ns+2 inputs and ns+1 outputs");
   if (!(t_end > 0.0) || !(rtol > 0.0) || !(atol > 0.0) || !(ymin > 0.0) || !(yfloor >= 0.0) || !(tfloor >= 0.0) ||
-      !(max_dT >= 0.0) || !(max_residual >= 0.0) || !(max_T_mismatch >= 0.0))
+      !(max_dT >= 0.0) || !(max_residual >= 0.0) || !(max_T_mismatch >= 0.0) || !(max_newton_dY >= 0.0) ||
+      !(max_newton_dT >= 0.0))
     return flint_fail(FLINT_EARG, "need t_end, rtol, atol, ymin > 0 and non-negative floors and thresholds");
   if (int rc = ensure(m, ld)) return rc;
   flint_sur_data* d = m->sur;
@@ -105,7 +107,12 @@ extern "C" int flint_reactor_advance_mlp(flint_mech* m, flint_mlp* net, long n, 
   HIPCHK(flint_sur_features(P, n, ld, X, p, d->F, s));
   if (int rc = flint_mlp_forward(net, n, d->F, d->O, s)) return rc;
   int* flag = fallback ? fallback : d->flag;
-  HIPCHK(flint_sur_apply(P, n, ld, X, d->O, flag, d->idx, d->count, s));
+  HIPCHK(flint_sur_apply(P, n, ld, X, d->Xp, d->O, flag, s));
+  // the FP64 judge: one implicit-step Newton correction from each prediction
+  HIPCHK(flint_sur_zero(d->count, 1, s));
+  if (int rc = flint_reactor_verify(m, n, ld, X, d->Xp, ld, d->Fr, ld, p, t_end, max_newton_dY, max_newton_dT, flag,
+                                    d->idx, d->count, check, stream))
+    return rc;
   int nf = 0;
   HIPCHK(hipMemcpyAsync(&nf, d->count, sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHK(hipStreamSynchronize(s));

@@ -1043,6 +1043,74 @@ __global__ __launch_bounds__(512) void jac_kernel(Prog P, int mode, long n, long
   }
 }
 
+// The check of a predicted step Xp over dt from X: Newton corrections from the prediction, with F
+// and J at Xp, towards the backward-Euler step and (F0 = F(X) given) the trapezoidal step,
+//   delta_BE = -(I - dt J)^-1 (Xp - X - dt F(Xp)),
+//   delta_TR = -(I - dt/2 J)^-1 (Xp - X - dt/2 (F0 + F(Xp))).
+// A cell passes when |delta_k| <= tol_y for every species and |delta_T| <= tol_t in both.  Stiff
+// modes enter delta_BE damped by 1 / (1 - dt lambda), so a prediction on the slow manifold passes
+// it; the second-order trapezoidal step rejects a prediction that merely agrees with backward
+// Euler's own first-order error.  A passing cell takes Xp into X; a failing one, or one flagged on
+// entry, keeps X, gets flag 1 and is appended to idx.  dmax (optional): the larger of the two
+// corrections' max |delta| over species and |delta_T| of each checked cell.
+__global__ __launch_bounds__(512) void verify_kernel(Prog P, long n, long ld, double* __restrict__ X,
+                                                     const double* __restrict__ Xp, long ldp,
+                                                     const double* __restrict__ F0, long ldf,
+                                                     const double* __restrict__ pres, double dt, double tol_y,
+                                                     double tol_t, int* __restrict__ flag, int* __restrict__ idx,
+                                                     int* __restrict__ count, double* __restrict__ dmax,
+                                                     unsigned team_bytes, double* __restrict__ jac_scratch) {
+  extern __shared__ __align__(16) char smem[];
+  const int lane = threadIdx.x & (TEAM - 1);
+  const bool h0 = half() == 0;
+  const int team = threadIdx.x / WAVE;
+  const int tpb = blockDim.x / WAVE;
+  const size_t sh = shared_prog_bytes(P);
+  load_shared_prog(P, smem);
+  __syncthreads();
+  const Lds L = carve(smem, smem + sh + (size_t)team * team_bytes, P);
+  double* __restrict__ Jg = jac_scratch + ((size_t)blockIdx.x * tpb + team) * (TEAM * 33);
+  const int nz = P.ns + 1;
+  const long stride = (long)gridDim.x * tpb;
+  for (long cell = (long)blockIdx.x * tpb + team; cell < n; cell += stride) {
+    bool pass = flag[cell] == 0;
+    if (pass) {
+      const double y0 = lane < nz ? X[(long)lane * ld + cell] : 0.0;
+      const double yp = lane < nz ? Xp[(long)lane * ldp + cell] : 0.0;
+      const RhsOut o = team_rhs(P, 1, pres[cell], yp, L, true, Jg);
+      pass = o.ok;
+      if (pass) {
+        team_jac(P, 1, yp, L, o, Jg);
+        const double G = lane < nz ? yp - y0 - dt * o.F : 0.0;
+        const int ms = team_inv(dt, nz, L, Jg);
+        double d = fabs(team_solve(ms, -G, nz, L));
+        if (F0) {
+          const double f0 = lane < nz ? F0[(long)lane * ldf + cell] : 0.0;
+          const double G2 = lane < nz ? yp - y0 - 0.5 * dt * (f0 + o.F) : 0.0;
+          const int ms2 = team_inv(0.5 * dt, nz, L, Jg);
+          d = fmax(d, fabs(team_solve(ms2, -G2, nz, L)));
+        }
+        double ey = lane < P.ns ? d : 0.0, et = lane == P.ns ? d : 0.0;
+        const double bad = tsum((lane < nz && !isfinite(d)) ? 1.0 : 0.0);
+        int l1 = lane, l2 = lane;
+        targmax(ey, l1);
+        targmax(et, l2);
+        pass = bad == 0.0 && ey <= tol_y && et <= tol_t;
+        if (dmax && lane == 0) {
+          dmax[cell] = bad == 0.0 ? ey : INFINITY;
+          dmax[ld + cell] = bad == 0.0 ? et : INFINITY;
+        }
+        if (pass && h0 && lane < nz) X[(long)lane * ld + cell] = yp;
+      }
+      tsync();
+    }
+    if (!pass && threadIdx.x % WAVE == 0) {
+      flag[cell] = 1;
+      idx[atomicAdd(count, 1)] = (int)cell;
+    }
+  }
+}
+
 }  // namespace flint_reactor
 
 hipError_t flint_reactor_set_max_lds(int bytes) {
@@ -1052,7 +1120,10 @@ hipError_t flint_reactor_set_max_lds(int bytes) {
   const hipError_t e2 = hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::jac_kernel<true>),
                                             hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
   if (e2 != hipSuccess) return e2;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::jac_kernel<false>),
+  const hipError_t e3 = hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::jac_kernel<false>),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
+  if (e3 != hipSuccess) return e3;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(&flint_reactor::verify_kernel),
                              hipFuncAttributeMaxDynamicSharedMemorySize, bytes);
 }
 
@@ -1091,6 +1162,17 @@ hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reacto
   hipLaunchKernelGGL(flint_reactor::zero_counter, dim3(1), dim3(64), 0, s, counter);
   hipLaunchKernelGGL(flint_reactor::bdf_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::WAVE),
                      flint_reactor::shared_prog_bytes(P) + (size_t)tb * teams_per_block, s, P, O, n, ld, X, pres, stats, status, counter, tb, jac_scratch);
+  return hipGetLastError();
+}
+
+hipError_t flint_reactor_verify_launch(const flint_reactor::Prog& P, long n, long ld, double* X, const double* Xp,
+                                       long ldp, const double* F0, long ldf, const double* pres, double dt,
+                                       double tol_y, double tol_t, int* flag, int* idx, int* count, double* dmax,
+                                       int teams_per_block, int blocks, double* jac_scratch, hipStream_t s) {
+  const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P) + 15) & ~size_t(15));
+  hipLaunchKernelGGL(flint_reactor::verify_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::WAVE),
+                     flint_reactor::shared_prog_bytes(P) + (size_t)tb * teams_per_block, s, P, n, ld, X, Xp, ldp,
+                     F0, ldf, pres, dt, tol_y, tol_t, flag, idx, count, dmax, tb, jac_scratch);
   return hipGetLastError();
 }
 

@@ -55,10 +55,13 @@ __device__ __forceinline__ bool hmix(const SurProg& P, const double (&Y)[NS], do
   return true;
 }
 
+// The network's increments applied to X (read only): the prediction, projected onto the conserved
+// element and mass totals, with T from the conserved enthalpy, goes to Xp; flag[c] = 1 marks a
+// cell that fails the plausibility tests (|dT|, raw element residual, T from enthalpy).
 template <int NS>
-__global__ __launch_bounds__(128) void apply_kernel(SurProg P, long n, long ld, double* __restrict__ X,
-                                                    const float* __restrict__ O, int* __restrict__ flag,
-                                                    int* __restrict__ idx, int* __restrict__ count) {
+__global__ __launch_bounds__(128) void apply_kernel(SurProg P, long n, long ld, const double* __restrict__ X,
+                                                    double* __restrict__ Xp, const float* __restrict__ O,
+                                                    int* __restrict__ flag) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const int ne = P.ne;
@@ -219,15 +222,11 @@ __global__ __launch_bounds__(128) void apply_kernel(SurProg P, long n, long ld, 
       if (!conv || !(fabs(T - (T0 + dT)) <= P.max_Tmis)) fall = true;
     }
   }
-  if (fall) {
-    flag[c] = 1;
-    idx[atomicAdd(count, 1)] = (int)c;
-    return;
-  }
-  flag[c] = 0;
+  flag[c] = fall ? 1 : 0;
+  if (fall) return;
 #pragma unroll
-  for (int k = 0; k < NS; ++k) X[(long)k * ld + c] = Y[k];
-  X[(long)NS * ld + c] = T;
+  for (int k = 0; k < NS; ++k) Xp[(long)k * ld + c] = Y[k];
+  Xp[(long)NS * ld + c] = T;
 }
 
 __global__ void zero_int(int* p, int n) {
@@ -273,16 +272,20 @@ hipError_t flint_sur_features(const flint_sur::SurProg& P, long n, long ld, cons
   return hipGetLastError();
 }
 
-hipError_t flint_sur_apply(const flint_sur::SurProg& P, long n, long ld, double* X, const float* O, int* flag, int* idx,
-                           int* count, hipStream_t s) {
-  hipLaunchKernelGGL(flint_sur::zero_int, dim3(1), dim3(64), 0, s, count, 1);
+hipError_t flint_sur_apply(const flint_sur::SurProg& P, long n, long ld, const double* X, double* Xp, const float* O,
+                           int* flag, hipStream_t s) {
   const unsigned b = (unsigned)((n + 127) / 128);
   if (P.ns == 31)
-    hipLaunchKernelGGL(flint_sur::apply_kernel<31>, dim3(b), dim3(128), 0, s, P, n, ld, X, O, flag, idx, count);
+    hipLaunchKernelGGL(flint_sur::apply_kernel<31>, dim3(b), dim3(128), 0, s, P, n, ld, X, Xp, O, flag);
   else if (P.ns == 24)
-    hipLaunchKernelGGL(flint_sur::apply_kernel<24>, dim3(b), dim3(128), 0, s, P, n, ld, X, O, flag, idx, count);
+    hipLaunchKernelGGL(flint_sur::apply_kernel<24>, dim3(b), dim3(128), 0, s, P, n, ld, X, Xp, O, flag);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t flint_sur_zero(int* p, int n, hipStream_t s) {
+  hipLaunchKernelGGL(flint_sur::zero_int, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, p, n);
   return hipGetLastError();
 }
 

@@ -5,7 +5,11 @@ pairs over a fixed step dt from seeded states; an MFMA MLP (flint_amd.mlp) learn
 map (log max(Y_k, ymin), T, log p) -> o, the increments in units of the local rate,
 o_k = dY_k / (dt |dY_k/dt| + yfloor) and o_T = dT / (dt |dT/dt| + tfloor) with dX/dt at the old state;
 flint_reactor_advance_mlp applies it with the BDF as per-cell fallback and exact
-element/mass/enthalpy conservation for the accepted cells.
+element/mass/enthalpy conservation for the accepted cells.  Each prediction is judged in FP64
+before it is accepted: the Newton corrections from it towards the backward-Euler and the
+trapezoidal step over dt (F and J at the prediction) must stay within max_newton_dY and
+max_newton_dT, half the step's accuracy bars (1e-4 in Y, 1 K); the raw element residual only
+screens out implausible outputs.
 
     python -m flint_amd.surrogate --cells 262144 --steps 16 --out flint_amd/data/surrogate_rich31.npz
 """
@@ -27,6 +31,9 @@ YFLOOR = 1e-9     # increment scale floors: mass fraction, K
 TFLOOR = 1e-3
 OCLIP = 50.0      # |o| clip of the training targets
 HIDDEN = (256, 256, 256)
+MAX_RESIDUAL = 1e-4     # plausibility of the raw prediction (element and mass residual)
+MAX_NEWTON_DY = 5e-5    # the FP64 judge: implicit-step Newton correction bounds (half the bars)
+MAX_NEWTON_DT = 0.5
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "data", "surrogate_rich31.npz")
 
@@ -87,11 +94,13 @@ class Surrogate:
     """A trained network with its feature scalings and acceptance thresholds."""
 
     def __init__(self, net, xs, ys, el_weight, dt, ymin=YMIN, yfloor=YFLOOR, tfloor=TFLOOR, max_dT=50.0,
-                 max_residual=1e-7, max_T_mismatch=1.0, meta=None):
+                 max_residual=MAX_RESIDUAL, max_T_mismatch=1.0, max_newton_dY=MAX_NEWTON_DY,
+                 max_newton_dT=MAX_NEWTON_DT, meta=None):
         self.net, self.xs, self.ys = net, xs, ys
         self.el_weight = np.ascontiguousarray(el_weight, dtype=np.float64)
         self.dt, self.ymin, self.yfloor, self.tfloor = dt, ymin, yfloor, tfloor
         self.max_dT, self.max_residual, self.max_T_mismatch = max_dT, max_residual, max_T_mismatch
+        self.max_newton_dY, self.max_newton_dT = max_newton_dY, max_newton_dT
         self.meta = meta or {}
 
     @property
@@ -111,19 +120,22 @@ class Surrogate:
         z = np.load(path, allow_pickle=False)
         net = MLP(z["dims"].tolist())
         net.set_params(z["params"])
+        # the acceptance thresholds are those of this module (the judge replaced the tight residual
+        # gate the file may carry)
         return cls(net, Scaler(z["x_mean"], z["x_scale"]), Scaler(z["y_mean"], z["y_scale"]), z["el_weight"],
                    float(z["dt"]), float(z["ymin"]), float(z["yfloor"]), float(z["tfloor"]), float(z["max_dT"]),
-                   float(z["max_residual"]), float(z["max_T_mismatch"]), json.loads(str(z["meta"])))
+                   meta=json.loads(str(z["meta"])))
 
-    def advance(self, mech, X, p, t_end=None, rtol=1e-6, atol=1e-12, fallback=None, stats=None, status=None, n=None,
-                stream=None):
-        """One learned step of X [ns+1, ld] (torch, device, in place); returns the fallback count."""
+    def advance(self, mech, X, p, t_end=None, rtol=1e-6, atol=1e-12, fallback=None, check_out=None, stats=None,
+                status=None, n=None, stream=None):
+        """One learned step of X [ns+1, ld] (torch, device, in place); returns the fallback count.
+        check_out [2, ld] (FP64, optional) receives the judge's max |delta Y_k| and |delta T|."""
         L = lib()
         if not getattr(L, "_sur_bound", False):
             vp, l = C.c_void_p, C.c_long
             d = C.c_double
-            L.flint_reactor_advance_mlp.argtypes = [vp, vp, l, l, vp, vp, d, d, d, vp, vp, d, d, d, d, d, d, vp,
-                                                    C.POINTER(C.c_long), vp, vp, vp]
+            L.flint_reactor_advance_mlp.argtypes = [vp, vp, l, l, vp, vp, d, d, d, vp, vp, d, d, d, d, d, d, d, d, vp,
+                                                    C.POINTER(C.c_long), vp, vp, vp, vp]
             L._sur_bound = True
         ld = X.shape[-1]
         n = ld if n is None else n
@@ -132,9 +144,8 @@ class Surrogate:
         check(L.flint_reactor_advance_mlp(mech._h, self.net._h, n, ld, _p(X), _p(p), self.dt if t_end is None else t_end,
                                           rtol, atol, _p(sc), _p(self.el_weight), self.ymin, self.yfloor,
                                           self.tfloor, self.max_dT, self.max_residual, self.max_T_mismatch,
-                                          _p(fallback), C.byref(nf),
-                                          _p(stats), _p(status),
-                                          _stream(stream)))
+                                          self.max_newton_dY, self.max_newton_dT, _p(fallback), C.byref(nf),
+                                          _p(check_out), _p(stats), _p(status), _stream(stream)))
         return nf.value
 
 

@@ -316,14 +316,22 @@ int flint_mlp_train_step(flint_mlp *m, long n, const float *X_dev, const float *
  * than max_residual or predicts |dT| > max_dT; an accepted prediction is projected onto the old
  * element amounts and total mass (exactly, to round-off) and T solved from the old mixture
  * enthalpy, and the cell still falls back when that T differs from the predicted T + dT by more
- * than max_T_mismatch.  fallback_dev [ld] (may be NULL) receives 1 for the cells that took the BDF,
- * *n_fallback their number, stats_dev/status_dev as flint_reactor_advance (zero for accepted
- * cells).  Synchronises the stream once (to size the fallback launch).  Needs ns = 24 or 31. */
+ * than max_T_mismatch.  Every prediction that passes is then judged in FP64 by the Newton
+ * corrections over t_end from the projected prediction Xp (F and J at Xp) towards the
+ * backward-Euler and the trapezoidal step,
+ * delta_BE = -(I - t_end J)^-1 (Xp - X - t_end F(Xp)),
+ * delta_TR = -(I - t_end/2 J)^-1 (Xp - X - t_end/2 (F(X) + F(Xp))),
+ * and the cell is accepted only when |delta_k| <= max_newton_dY for every species and
+ * |delta_T| <= max_newton_dT in both.  fallback_dev [ld] (may be NULL) receives 1 for the cells
+ * that took the BDF, *n_fallback their number, check_dev [2][ld] (may be NULL) the larger
+ * correction's max_k |delta_k| and |delta_T| of each judged cell, stats_dev/status_dev
+ * as flint_reactor_advance (zero for accepted cells).  Synchronises the stream once (to size the
+ * fallback launch).  Needs ns = 24 or 31. */
 int flint_reactor_advance_mlp(flint_mech *m, flint_mlp *net, long n, long ld, double *X_dev, const double *p_dev,
                               double t_end, double rtol, double atol, const double *scaling, const double *el_weight,
                               double ymin, double yfloor, double tfloor, double max_dT, double max_residual,
-                              double max_T_mismatch, int *fallback_dev, long *n_fallback, int *stats_dev,
-                              int *status_dev, void *stream);
+                              double max_T_mismatch, double max_newton_dY, double max_newton_dT, int *fallback_dev,
+                              long *n_fallback, double *check_dev, int *stats_dev, int *status_dev, void *stream);
 
 #ifdef __cplusplus
 }

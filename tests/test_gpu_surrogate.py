@@ -5,7 +5,11 @@ flint_amd/data/surrogate_rich31.npz.
   on the same cells;
 * accepted cells keep element amounts, total mass and mixture enthalpy of the old state to
   round-off (the projection and the enthalpy solve);
-* accepted cells stay within max |dY| <= 1e-4 and max |dT| <= 1 K of the plain FP64 BDF step."""
+* accepted cells stay within max |dY| <= 1e-4 and max |dT| <= 1 K of the plain FP64 BDF step, and
+  each passed the FP64 judge (the backward-Euler and trapezoidal Newton corrections from the
+  prediction within the thresholds);
+* mid-trajectory, on 2^16 cells, the learned step is faster than the plain step (measured
+  1.75x at 2^18 cells)."""
 import numpy as np
 import pytest
 
@@ -49,17 +53,19 @@ def test_surrogate_step_conservation_and_fallback(setup, cuda):
     fb = torch.full((n,), -1, dtype=torch.int32, device=cuda)
     sm = torch.full((n,), -1, dtype=torch.int32, device=cuda)
     print("[gpu] surrogate step", flush=True)
-    nf = sur.advance(mech, Xm, p, fallback=fb, status=sm)
+    chk = torch.full((2, n), -1.0, dtype=torch.float64, device=cuda)
+    nf = sur.advance(mech, Xm, p, fallback=fb, check_out=chk, status=sm)
     torch.cuda.synchronize()
     X0, Xp, Xm = X.cpu().numpy(), Xp.cpu().numpy(), Xm.cpu().numpy()
-    fb, sm, sp = fb.cpu().numpy(), sm.cpu().numpy(), sp.cpu().numpy()
+    fb, sm, sp, chk = fb.cpu().numpy(), sm.cpu().numpy(), sp.cpu().numpy(), chk.cpu().numpy()
     assert set(np.unique(fb)) <= {0, 1} and nf == int(fb.sum())
     f = fb == 1
     # fallback cells: the plain BDF kernel on the same cells
     assert np.array_equal(Xm[:, f], Xp[:, f]) and np.array_equal(sm[f], sp[f])
     a = ~f
-    assert a.sum() > 0.1 * n
+    assert a.sum() > 0.5 * n
     assert np.all(sm[a] == 0)
+    assert np.all(chk[0, a] <= sur.max_newton_dY) and np.all(chk[1, a] <= sur.max_newton_dT)
     comp = np.asarray(md.composition, dtype=np.float64)
     W = np.asarray(md.W)
     b0, b1 = comp @ (X0[:ns, a] / W[:, None]), comp @ (Xm[:ns, a] / W[:, None])
@@ -68,8 +74,8 @@ def test_surrogate_step_conservation_and_fallback(setup, cuda):
     h0, h1 = _mix_h(md, X0[:ns, a], X0[ns, a]), _mix_h(md, Xm[:ns, a], Xm[ns, a])
     assert np.max(np.abs(h1 - h0) / np.maximum(np.abs(h0), 1e5)) < 1e-9
     # accuracy against the plain FP64 step: the learned step's bars, max |dY| <= 1e-4 and
-    # max |dT| <= 1 K over the accepted cells (the element-residual gate of 1e-7 keeps them;
-    # at 2^18 cells it measured 2.6e-5 and 0.16 K)
+    # max |dT| <= 1 K over the accepted cells (the judge's thresholds, half the bars, keep them;
+    # at 2^18 cells the worst accepted cells measured 5.0e-5 and 0.49 K)
     assert np.max(np.abs(Xm[ns, a] - Xp[ns, a])) <= 1.0
     assert np.max(np.abs(Xm[:ns, a] - Xp[:ns, a])) <= 1e-4
 
@@ -98,3 +104,42 @@ def test_surrogate_rejects_mismatched_network(setup, cuda):
     bad = Surrogate(MLP([10, 16, 5]), sur.xs, sur.ys, sur.el_weight, sur.dt)
     with pytest.raises(FlintError):
         bad.advance(mech, X.clone(), p)
+
+
+def test_surrogate_gain_mid_trajectory(mechs, cuda):
+    """2^16 cells at t = 5e-5 s: the learned step (network, judge and BDF fallback, timed as one)
+    against the plain BDF step over the same dt, with every accepted cell within the bars."""
+    import time
+    import torch
+    from flint_amd.native import Mechanism
+    from flint_amd.surrogate import Surrogate, DEFAULT_MODEL
+    md = mechs["TSR-Rich-31"]
+    mech = Mechanism.from_data(md)
+    sur = Surrogate.load(DEFAULT_MODEL)
+    ns, n = md.ns, 1 << 16
+    st = S.radical_states(md.species, md.W, 4048, 0, n, ignition_half=True)
+    X = torch.from_numpy(np.vstack([S.soa(st["Y"]), st["T"][None, :]])).to(cuda)
+    p = torch.from_numpy(st["p"]).to(cuda)
+    print(f"[gpu] surrogate gain n={n}: plain BDF to t = 5e-5", flush=True)
+    mech.advance(X, p, mode=1, t_end=5e-5)
+    w = X[:, :1024].clone()
+    sur.advance(mech, w, p[:1024].clone())  # first-call set-up outside the timing
+
+    def timed(f):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        f()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t
+
+    Xp, Xm = X.clone(), X.clone()
+    fb = torch.zeros(n, dtype=torch.int32, device=cuda)
+    t_plain = timed(lambda: mech.advance(Xp, p, mode=1, t_end=sur.dt))
+    t_mlp = timed(lambda: sur.advance(mech, Xm, p, fallback=fb))
+    acc = fb == 0
+    dY = float((Xm[:ns, acc] - Xp[:ns, acc]).abs().max())
+    dT = float((Xm[ns, acc] - Xp[ns, acc]).abs().max())
+    print(f"[gpu] plain {t_plain * 1e3:.1f} ms, learned {t_mlp * 1e3:.1f} ms ({t_plain / t_mlp:.2f}x), "
+          f"accepted {float(acc.float().mean()):.3f}, max |dY| {dY:.2e}, max |dT| {dT:.3f} K", flush=True)
+    assert dY <= 1e-4 and dT <= 1.0
+    assert t_plain / t_mlp >= 1.25
