@@ -6,12 +6,13 @@ namespace flint_sur {
 
 struct SurProg {
   int ns, ne;
+  int nin;             // network inputs: ns + 2 (state) or 2 ns + 3 (state and rates)
   const double* comp;  // [ne][ns] atoms of element e in species k
   const double* W;     // [ns] molecular weights
   const double* elw;   // [ne] atomic weights (element mass fractions for the residual test)
   const double* tabh;  // [nT+1][ns][2] (cp, h) per mass
   int Th_min, Th_max;
-  const double* scal;  // in_mean[ns+2], in_scale[ns+2], out_mean[ns+1], out_scale[ns+1]
+  const double* scal;  // in_mean[nin], in_scale[nin], out_mean[ns+1], out_scale[ns+1]
   const double* F;     // [ns+1][ldF] right-hand side dX/dt at the old state
   long ldF;
   double dt, ymin, yfloor, tfloor, max_dT, max_res, max_Tmis;

@@ -49,7 +49,7 @@ static int ensure(flint_mech* m, long n) {
   m->sur = d;
   const int ns = m->ns, ne = m->ne;
   int rc = 0;
-  rc |= grab(d, &d->F, (size_t)n * (ns + 2));
+  rc |= grab(d, &d->F, (size_t)n * (2 * ns + 3));
   rc |= grab(d, &d->O, (size_t)n * (ns + 1));
   rc |= grab(d, &d->flag, n);
   rc |= grab(d, &d->idx, n);
@@ -58,7 +58,7 @@ static int ensure(flint_mech* m, long n) {
   rc |= grab(d, &d->sg, n);
   rc |= grab(d, &d->Xg, (size_t)(ns + 1) * n);
   rc |= grab(d, &d->pg, n);
-  rc |= grab(d, &d->scal, (size_t)4 * (ns + 2));
+  rc |= grab(d, &d->scal, (size_t)2 * (2 * ns + 3) + 2 * (ns + 1));
   rc |= grab(d, &d->comp, (size_t)ne * ns);
   rc |= grab(d, &d->elw, ne);
   rc |= grab(d, &d->Fr, (size_t)(ns + 1) * n);
@@ -89,8 +89,9 @@ extern "C" int flint_reactor_advance_mlp(flint_mech* m, flint_mlp* net, long n, 
   const int ns = m->ns, ne = m->ne;
   if (ns != 24 && ns != 31) return flint_fail(FLINT_EUNSUPPORTED, "the learned step supports the generated mechanisms");
   if (ne < 1 || ne > 4 || m->comp.empty()) return flint_fail(FLINT_EARG, "the learned step needs 1..4 elements");
-  if (net->dims.front() != ns + 2 || net->dims.back() != ns + 1)
-    return flint_fail(FLINT_EARG, "network widths must be ns+2 inputs and ns+1 outputs");
+  const int nin = net->dims.front();
+  if ((nin != ns + 2 && nin != 2 * ns + 3) || net->dims.back() != ns + 1)
+    return flint_fail(FLINT_EARG, "network widths must be ns+2 or 2ns+3 inputs and ns+1 outputs");
   if (!(t_end > 0.0) || !(rtol > 0.0) || !(atol > 0.0) || !(ymin > 0.0) || !(yfloor >= 0.0) || !(tfloor >= 0.0) ||
       !(max_dT >= 0.0) || !(max_residual >= 0.0) || !(max_T_mismatch >= 0.0) || !(max_newton_dY >= 0.0) ||
       !(max_newton_dT >= 0.0))
@@ -98,12 +99,12 @@ extern "C" int flint_reactor_advance_mlp(flint_mech* m, flint_mlp* net, long n, 
   if (int rc = ensure(m, ld)) return rc;
   flint_sur_data* d = m->sur;
   hipStream_t s = (hipStream_t)stream;
-  HIPCHK(hipMemcpyAsync(d->scal, scaling, (size_t)(4 * ns + 6) * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemcpyAsync(d->scal, scaling, (size_t)(2 * nin + 2 * (ns + 1)) * sizeof(double), hipMemcpyHostToDevice,
+                        s));
   HIPCHK(hipMemcpyAsync(d->elw, el_weight, (size_t)ne * sizeof(double), hipMemcpyHostToDevice, s));
   if (int rc = flint_reactor_jacobian(m, n, ld, 1, X, p, d->Fr, nullptr, stream)) return rc;
-  flint_sur::SurProg P{ns,      ne,          d->comp, m->d_W, d->elw, m->d_tabh, m->Th_min, m->Th_max, d->scal,
-                       d->Fr,   ld,          t_end,   ymin,   yfloor, tfloor,    max_dT,    max_residual,
-                       max_T_mismatch};
+  flint_sur::SurProg P{ns,    ne,   nin,    d->comp, m->d_W, d->elw,       m->d_tabh,   m->Th_min, m->Th_max, d->scal,
+                       d->Fr, ld,   t_end,  ymin,    yfloor, tfloor, max_dT, max_residual, max_T_mismatch};
   HIPCHK(flint_sur_features(P, n, ld, X, p, d->F, s));
   if (int rc = flint_mlp_forward(net, n, d->F, d->O, s)) return rc;
   int* flag = fallback ? fallback : d->flag;

@@ -1,7 +1,9 @@
 // Learned constant-pressure reactor step with the plain BDF as fallback (no counterpart in the
 // FLINT reference).
 //
-// Per cell (one thread): features (log max(Y_k, ymin), T, log p) standardised -> MLP (mlp.hip)
+// Per cell (one thread): features (log max(Y_k, ymin), T, log p) standardised, and for a network
+// of 2 ns + 3 inputs also the rate features asinh(dt dY_k/dt / yfloor), asinh(dt dT/dt / tfloor)
+// with the right-hand side at the old state -> MLP (mlp.hip)
 // -> o de-standardised, the increments in units of the local rate: dY_k = o_k (dt |dY_k/dt| + yfloor),
 // dT = o_T (dt |dT/dt| + tfloor), with the right-hand side at the old state, so that a cell whose
 // chemistry is nearly frozen gets a nearly zero increment whatever the network's relative error.
@@ -22,18 +24,27 @@ namespace flint_sur {
 
 constexpr int MAXE = 4;
 
-template <int NS>
+template <int NS, bool RATES>
 __global__ __launch_bounds__(256) void features_kernel(SurProg P, long n, long ld, const double* __restrict__ X,
                                                        const double* __restrict__ pres, float* __restrict__ F) {
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
+  constexpr int NIN = RATES ? 2 * NS + 3 : NS + 2;
   const double* __restrict__ mu = P.scal;
-  const double* __restrict__ sd = P.scal + (NS + 2);
-  float* __restrict__ f = F + c * (NS + 2);
+  const double* __restrict__ sd = P.scal + NIN;
+  float* __restrict__ f = F + c * NIN;
 #pragma unroll
   for (int k = 0; k < NS; ++k) f[k] = (float)((log(fmax(X[(long)k * ld + c], P.ymin)) - mu[k]) / sd[k]);
   f[NS] = (float)((X[(long)NS * ld + c] - mu[NS]) / sd[NS]);
   f[NS + 1] = (float)((log(pres[c]) - mu[NS + 1]) / sd[NS + 1]);
+  if (RATES) {
+#pragma unroll
+    for (int k = 0; k <= NS; ++k) {
+      const double fl = k < NS ? P.yfloor : P.tfloor;
+      const double r = asinh(P.dt * P.F[(long)k * P.ldF + c] / fl);
+      f[NS + 2 + k] = (float)((r - mu[NS + 2 + k]) / sd[NS + 2 + k]);
+    }
+  }
 }
 
 // mixture enthalpy sum_k Y_k h_k(T) and its slope within the table bin
@@ -65,7 +76,7 @@ __global__ __launch_bounds__(128) void apply_kernel(SurProg P, long n, long ld, 
   const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
   const int ne = P.ne;
-  const double* __restrict__ om = P.scal + 2 * (NS + 2);
+  const double* __restrict__ om = P.scal + 2 * P.nin;
   const double* __restrict__ os = om + (NS + 1);
   double Y0[NS], Y[NS];
   const double T0 = X[(long)NS * ld + c];
@@ -263,10 +274,16 @@ __global__ void scatter_int(int nrow, long ld, int* __restrict__ dst, long nf, l
 hipError_t flint_sur_features(const flint_sur::SurProg& P, long n, long ld, const double* X, const double* p, float* F,
                               hipStream_t s) {
   const unsigned b = (unsigned)((n + 255) / 256);
-  if (P.ns == 31)
-    hipLaunchKernelGGL(flint_sur::features_kernel<31>, dim3(b), dim3(256), 0, s, P, n, ld, X, p, F);
+  const bool rates = P.nin == 2 * P.ns + 3;
+  if (!rates && P.nin != P.ns + 2) return hipErrorInvalidValue;
+  if (P.ns == 31 && rates)
+    hipLaunchKernelGGL((flint_sur::features_kernel<31, true>), dim3(b), dim3(256), 0, s, P, n, ld, X, p, F);
+  else if (P.ns == 31)
+    hipLaunchKernelGGL((flint_sur::features_kernel<31, false>), dim3(b), dim3(256), 0, s, P, n, ld, X, p, F);
+  else if (P.ns == 24 && rates)
+    hipLaunchKernelGGL((flint_sur::features_kernel<24, true>), dim3(b), dim3(256), 0, s, P, n, ld, X, p, F);
   else if (P.ns == 24)
-    hipLaunchKernelGGL(flint_sur::features_kernel<24>, dim3(b), dim3(256), 0, s, P, n, ld, X, p, F);
+    hipLaunchKernelGGL((flint_sur::features_kernel<24, false>), dim3(b), dim3(256), 0, s, P, n, ld, X, p, F);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();

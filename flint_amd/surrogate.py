@@ -2,7 +2,8 @@
 
 The plain batched BDF (flint_reactor_advance, constant pressure) generates (state, increment)
 pairs over a fixed step dt from seeded states; an MFMA MLP (flint_amd.mlp) learns the standardised
-map (log max(Y_k, ymin), T, log p) -> o, the increments in units of the local rate,
+map (log max(Y_k, ymin), T, log p, and the rate features asinh(dt dY_k/dt / yfloor),
+asinh(dt dT/dt / tfloor)) -> o, the increments in units of the local rate,
 o_k = dY_k / (dt |dY_k/dt| + yfloor) and o_T = dT / (dt |dT/dt| + tfloor) with dX/dt at the old state;
 flint_reactor_advance_mlp applies it with the BDF as per-cell fallback and exact
 element/mass/enthalpy conservation for the accepted cells.  Each prediction is judged in FP64
@@ -11,7 +12,7 @@ trapezoidal step over dt (F and J at the prediction) must stay within max_newton
 max_newton_dT, half the step's accuracy bars (1e-4 in Y, 1 K); the raw element residual only
 screens out implausible outputs.
 
-    python -m flint_amd.surrogate --cells 262144 --steps 16 --out flint_amd/data/surrogate_rich31.npz
+    python -m flint_amd.surrogate --cells 262144 --steps 16 --epochs 60 --out flint_amd/data/surrogate_rich31.npz
 """
 import argparse
 import ctypes as C
@@ -38,11 +39,18 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_MODEL = os.path.join(HERE, "data", "surrogate_rich31.npz")
 
 
-def features(X, p, ymin=YMIN):
-    """X [ns+1, n] (Y_k, T), p [n] (torch, FP64) -> [n, ns+2]."""
+def features(X, p, ymin=YMIN, F=None, dt=None, yfloor=YFLOOR, tfloor=TFLOOR):
+    """X [ns+1, n] (Y_k, T), p [n] (torch, FP64) -> [n, ns+2]; with the right-hand side F [ns+1, n]
+    at X and the step dt also the rate features asinh(dt F_k / yfloor), asinh(dt F_T / tfloor)
+    -> [n, 2 ns + 3] (the device features kernel computes the same)."""
     import torch
     ns = X.shape[0] - 1
-    return torch.cat([torch.log(torch.clamp(X[:ns], min=ymin)).T, X[ns:].T, torch.log(p)[:, None]], 1)
+    cols = [torch.log(torch.clamp(X[:ns], min=ymin)).T, X[ns:].T, torch.log(p)[:, None]]
+    if F is not None:
+        fl = torch.full((ns + 1, 1), yfloor, dtype=F.dtype, device=F.device)
+        fl[ns] = tfloor
+        cols.append(torch.asinh(dt * F / fl).T)
+    return torch.cat(cols, 1)
 
 
 def rate_scale(F, dt, yfloor=YFLOOR, tfloor=TFLOOR):
@@ -64,9 +72,9 @@ def torch_clamp(x, c):
     return torch.clamp(x, min=-c, max=c)
 
 
-def generate_pairs(mech, md, n_cells, n_steps, dt, seed, rtol=1e-6, atol=1e-12, log=None):
-    """(features, increments) [N, ns+2], [N, ns+1] (FP64, device) along BDF trajectories of
-    n_cells seeded states (half near ignition) over n_steps steps of dt."""
+def generate_pairs(mech, md, n_cells, n_steps, dt, seed, rtol=1e-6, atol=1e-12, log=None, rates=False):
+    """(features, increments) [N, ns+2 (2 ns + 3 with rates)], [N, ns+1] (FP64, device) along BDF
+    trajectories of n_cells seeded states (half near ignition) over n_steps steps of dt."""
     import torch
     dev = torch.device("cuda:0")
     st = S.radical_states(md.species, md.W, seed, 0, n_cells, ignition_half=True)
@@ -83,7 +91,7 @@ def generate_pairs(mech, md, n_cells, n_steps, dt, seed, rtol=1e-6, atol=1e-12, 
         mech.advance(X, p, mode=1, t_end=dt, rtol=rtol, atol=atol, status=status)
         torch.cuda.synchronize()
         alive &= status == 0
-        Fs.append(features(X0[:, alive], p[alive]))
+        Fs.append(features(X0[:, alive], p[alive], F=F[:, alive] if rates else None, dt=dt))
         Ds.append(increments(X0[:, alive], X[:, alive], F[:, alive], dt))
         if log:
             log(f"step {s + 1}/{n_steps}: {int(alive.sum())} cells ok, {time.time() - t:.1f} s")
@@ -149,10 +157,11 @@ class Surrogate:
         return nf.value
 
 
-def train(mech, md, n_cells=1 << 16, n_steps=16, dt=1e-5, seed=7, epochs=30, batch=4096, lr=1e-3, log=print):
+def train(mech, md, n_cells=1 << 16, n_steps=16, dt=1e-5, seed=7, epochs=30, batch=4096, lr=1e-3, log=print,
+          rates=False):
     import torch
     t0 = time.time()
-    Fx, Dy = generate_pairs(mech, md, n_cells, n_steps, dt, seed, log=log)
+    Fx, Dy = generate_pairs(mech, md, n_cells, n_steps, dt, seed, log=log, rates=rates)
     t_gen = time.time() - t0
     xs = Scaler.fit(Fx.cpu().numpy())
     ys = Scaler.fit(Dy.cpu().numpy())
@@ -160,13 +169,13 @@ def train(mech, md, n_cells=1 << 16, n_steps=16, dt=1e-5, seed=7, epochs=30, bat
     Xn = ((Fx - torch.from_numpy(xs.mean).to(dev)) / torch.from_numpy(xs.scale).to(dev)).float().contiguous()
     Yn = ((Dy - torch.from_numpy(ys.mean).to(dev)) / torch.from_numpy(ys.scale).to(dev)).float().contiguous()
     ns = md.ns
-    net = MLP([ns + 2, *HIDDEN, ns + 1], seed=seed)
+    net = MLP([Fx.shape[1], *HIDDEN, ns + 1], seed=seed)
     t1 = time.time()
     hist = net.fit(Xn, Yn, epochs=epochs, batch=batch, lr=lr, seed=seed, log=log)
     torch.cuda.synchronize()
     t_train = time.time() - t1
     meta = dict(pairs=int(Fx.shape[0]), cells=n_cells, steps=n_steps, seed=seed, epochs=epochs, batch=batch, lr=lr,
-                loss=hist, t_generate=t_gen, t_train=t_train)
+                rates=rates, loss=hist, t_generate=t_gen, t_train=t_train)
     log(f"{Fx.shape[0]} pairs in {t_gen:.1f} s, trained in {t_train:.1f} s, final loss {hist[-1]:.3e}")
     return Surrogate(net, xs, ys, element_weights(md.elements), dt, meta=meta)
 
@@ -184,11 +193,12 @@ def main():
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--out", default=DEFAULT_MODEL)
+    ap.add_argument("--state-only", action="store_true", help="inputs (log Y, T, log p) without the rate features")
     a = ap.parse_args()
     md = mech_data(a.mech)
     mech = Mechanism.from_data(md)
     log = lambda s: print(s, flush=True)
-    sur = train(mech, md, a.cells, a.steps, a.dt, a.seed, a.epochs, lr=a.lr, log=log)
+    sur = train(mech, md, a.cells, a.steps, a.dt, a.seed, a.epochs, lr=a.lr, log=log, rates=not a.state_only)
     sur.save(a.out)
     log(f"saved {a.out}")
 

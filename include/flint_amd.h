@@ -306,11 +306,13 @@ int flint_mlp_train_step(flint_mlp *m, long n, const float *X_dev, const float *
 
 /* Learned constant-pressure reactor step with the batched BDF as fallback (no reference
  * counterpart).  X_dev = (Y_1..Y_ns, T) [ns+1][ld] and p_dev [n] (Pa) as in flint_reactor_advance
- * mode 1; net maps the standardised features (log max(Y_k, ymin), T, log p) [ns+2] to the
- * standardised increments over t_end in units of the local rate, o_k = dY_k / (t_end |dY_k/dt| +
- * yfloor) and o_T = dT / (t_end |dT/dt| + tfloor) with dX/dt at the old state (the right-hand side
- * of flint_reactor_jacobian mode 1) [ns+1].  scaling (host, 4 ns + 6 doubles) =
- * in_mean[ns+2], in_scale[ns+2], out_mean[ns+1], out_scale[ns+1]; el_weight (host, [ne]) =
+ * mode 1; net maps the standardised features (log max(Y_k, ymin), T, log p) [ns+2], or with
+ * 2 ns + 3 inputs those and the rate features asinh(t_end dY_k/dt / yfloor), asinh(t_end dT/dt /
+ * tfloor), to the standardised increments over t_end in units of the local rate,
+ * o_k = dY_k / (t_end |dY_k/dt| + yfloor) and o_T = dT / (t_end |dT/dt| + tfloor) with dX/dt at
+ * the old state (the right-hand side of flint_reactor_jacobian mode 1) [ns+1].  scaling (host,
+ * 2 nin + 2 ns + 2 doubles, nin the input width) = in_mean[nin], in_scale[nin], out_mean[ns+1],
+ * out_scale[ns+1]; el_weight (host, [ne]) =
  * atomic weights of the elements.  A cell falls back to flint_reactor_advance(mode 1, t_end,
  * rtol, atol) when its raw prediction changes an element mass fraction or the total mass by more
  * than max_residual or predicts |dT| > max_dT; an accepted prediction is projected onto the old
