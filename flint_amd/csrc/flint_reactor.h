@@ -45,10 +45,12 @@ hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reacto
 constexpr int FLINT_SCRATCH_TEAMS = 2048;
 hipError_t flint_reactor_scratch(int** counter, double** jac_slots);
 
+hipError_t flint_reactor_fill_int(int* p, long n, int v, hipStream_t s);
 hipError_t flint_reactor_verify_launch(const flint_reactor::Prog& P, long n, long ld, double* X, const double* Xp,
                                        long ldp, const double* F0, long ldf, const double* pres, double dt,
-                                       double tol_y, double tol_t, int* flag, int* idx, int* count, double* dmax,
-                                       int teams_per_block, int blocks, double* jac_scratch, hipStream_t s);
+                                       double tol_y, double tol_t, int commit, int* flag, int* idx, int* count,
+                                       double* dmax, int teams_per_block, int blocks, double* jac_scratch,
+                                       hipStream_t s);
 hipError_t flint_reactor_jac_launch(const flint_reactor::Prog& P, int mode, long n, long ld, const double* X,
                                     const double* pres, double* F, double* J, int teams_per_block, int blocks,
                                     double* jac_scratch, hipStream_t s);

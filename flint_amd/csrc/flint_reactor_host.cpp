@@ -23,6 +23,8 @@ struct flint_extra {
   int blocks = 0;
   bool tuned = false;
   int newton = 1;  // Newton policy (flint_reactor_set_newton)
+  double* d_chkF = nullptr;  // right-hand side at the old states for flint_reactor_check_step
+  size_t chk_cap = 0;
   // captured graph of the last flint_reactor_advance_graph call
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -43,6 +45,7 @@ void flint_mech_release_extra(flint_mech* m) {
   for (void* p : x->bufs) (void)hipFree(p);
   if (x->d_counter) (void)hipFree(x->d_counter);
   if (x->d_jac) (void)hipFree(x->d_jac);
+  if (x->d_chkF) (void)hipFree(x->d_chkF);
   delete x;
   m->extra = nullptr;
 }
@@ -325,8 +328,39 @@ int flint_reactor_verify(flint_mech* m, long n, long ld, double* X, const double
   const int blocks = (int)std::min<long>(x->blocks, (n + tpb - 1) / tpb);
   if (int rc = ensure_jac(x, tpb, blocks)) return rc;
   if (F0 && ldf < n) return flint_fail(FLINT_EARG, "short right-hand side buffer");
-  hipError_t e = flint_reactor_verify_launch(x->prog, n, ld, X, Xp, ldp, F0, ldf, p, dt, tol_y, tol_t, flag, idx,
+  hipError_t e = flint_reactor_verify_launch(x->prog, n, ld, X, Xp, ldp, F0, ldf, p, dt, tol_y, tol_t, 1, flag, idx,
                                              count, dmax, tpb, blocks, x->d_jac, (hipStream_t)stream);
+  if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("reactor check launch: ") + hipGetErrorString(e));
+  return FLINT_OK;
+}
+
+extern "C" int flint_reactor_check_step(flint_mech* m, long n, long ld, const double* X, const double* Xp,
+                                        const double* p, double dt, double tol_y, double tol_t, int* flag,
+                                        double* check, void* stream) {
+  if (int rc = check_args(m, n, ld, 1, X, p, dt, 1.0, 1.0)) return rc;
+  if (n == 0) return FLINT_OK;
+  if (!Xp || !flag) return flint_fail(FLINT_EARG, "null candidate state or flag array");
+  if (!(tol_y >= 0.0) || !(tol_t >= 0.0)) return flint_fail(FLINT_EARG, "need non-negative tolerances");
+  if (int rc = ensure_extra(m)) return rc;
+  flint_extra* x = m->extra;
+  const int tpb = x->teams_per_block;
+  const int blocks = (int)std::min<long>(x->blocks, (n + tpb - 1) / tpb);
+  if (int rc = ensure_jac(x, tpb, blocks)) return rc;
+  const size_t need = (size_t)(m->ns + 1) * ld;
+  if (x->chk_cap < need) {
+    if (x->d_chkF) HIPCHK(hipFree(x->d_chkF));
+    x->d_chkF = nullptr;
+    x->chk_cap = 0;
+    HIPCHK(hipMalloc(&x->d_chkF, need * sizeof(double)));
+    x->chk_cap = need;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  // F at the old states (the trapezoidal correction), then every cell judged (flag cleared by the kernel)
+  hipError_t e = flint_reactor_jac_launch(x->prog, 1, n, ld, X, p, x->d_chkF, nullptr, tpb, blocks, x->d_jac, s);
+  if (e == hipSuccess) e = flint_reactor_fill_int(flag, n, 0, s);
+  if (e == hipSuccess)
+    e = flint_reactor_verify_launch(x->prog, n, ld, const_cast<double*>(X), Xp, ld, x->d_chkF, ld, p, dt, tol_y,
+                                    tol_t, 0, flag, nullptr, nullptr, check, tpb, blocks, x->d_jac, s);
   if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("reactor check launch: ") + hipGetErrorString(e));
   return FLINT_OK;
 }

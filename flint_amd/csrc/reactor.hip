@@ -956,6 +956,11 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
   }
 }
 
+__global__ void fill_int(int* p, long n, int v) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 __global__ void zero_counter(int* c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = 0;
 }
@@ -1051,15 +1056,17 @@ __global__ __launch_bounds__(512) void jac_kernel(Prog P, int mode, long n, long
 // modes enter delta_BE damped by 1 / (1 - dt lambda), so a prediction on the slow manifold passes
 // it; the second-order trapezoidal step rejects a prediction that merely agrees with backward
 // Euler's own first-order error.  A passing cell takes Xp into X; a failing one, or one flagged on
-// entry, keeps X, gets flag 1 and is appended to idx.  dmax (optional): the larger of the two
-// corrections' max |delta| over species and |delta_T| of each checked cell.
+// entry, keeps X, gets flag 1 and is appended to idx (idx may be null: nothing is listed; commit
+// 0: X is never written).  dmax (optional): the larger of the two corrections' max |delta| over
+// species and |delta_T| of each checked cell.
 __global__ __launch_bounds__(512) void verify_kernel(Prog P, long n, long ld, double* __restrict__ X,
                                                      const double* __restrict__ Xp, long ldp,
                                                      const double* __restrict__ F0, long ldf,
                                                      const double* __restrict__ pres, double dt, double tol_y,
-                                                     double tol_t, int* __restrict__ flag, int* __restrict__ idx,
-                                                     int* __restrict__ count, double* __restrict__ dmax,
-                                                     unsigned team_bytes, double* __restrict__ jac_scratch) {
+                                                     double tol_t, int commit, int* __restrict__ flag,
+                                                     int* __restrict__ idx, int* __restrict__ count,
+                                                     double* __restrict__ dmax, unsigned team_bytes,
+                                                     double* __restrict__ jac_scratch) {
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x & (TEAM - 1);
   const bool h0 = half() == 0;
@@ -1100,13 +1107,13 @@ __global__ __launch_bounds__(512) void verify_kernel(Prog P, long n, long ld, do
           dmax[cell] = bad == 0.0 ? ey : INFINITY;
           dmax[ld + cell] = bad == 0.0 ? et : INFINITY;
         }
-        if (pass && h0 && lane < nz) X[(long)lane * ld + cell] = yp;
+        if (pass && commit && h0 && lane < nz) X[(long)lane * ld + cell] = yp;
       }
       tsync();
     }
-    if (!pass && threadIdx.x % WAVE == 0) {
-      flag[cell] = 1;
-      idx[atomicAdd(count, 1)] = (int)cell;
+    if (threadIdx.x % WAVE == 0) {
+      flag[cell] = pass ? 0 : 1;
+      if (!pass && idx) idx[atomicAdd(count, 1)] = (int)cell;
     }
   }
 }
@@ -1165,14 +1172,21 @@ hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reacto
   return hipGetLastError();
 }
 
+hipError_t flint_reactor_fill_int(int* p, long n, int v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(flint_reactor::fill_int, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
 hipError_t flint_reactor_verify_launch(const flint_reactor::Prog& P, long n, long ld, double* X, const double* Xp,
                                        long ldp, const double* F0, long ldf, const double* pres, double dt,
-                                       double tol_y, double tol_t, int* flag, int* idx, int* count, double* dmax,
-                                       int teams_per_block, int blocks, double* jac_scratch, hipStream_t s) {
+                                       double tol_y, double tol_t, int commit, int* flag, int* idx, int* count,
+                                       double* dmax, int teams_per_block, int blocks, double* jac_scratch,
+                                       hipStream_t s) {
   const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P) + 15) & ~size_t(15));
   hipLaunchKernelGGL(flint_reactor::verify_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::WAVE),
                      flint_reactor::shared_prog_bytes(P) + (size_t)tb * teams_per_block, s, P, n, ld, X, Xp, ldp,
-                     F0, ldf, pres, dt, tol_y, tol_t, flag, idx, count, dmax, tb, jac_scratch);
+                     F0, ldf, pres, dt, tol_y, tol_t, commit, flag, idx, count, dmax, tb, jac_scratch);
   return hipGetLastError();
 }
 

@@ -41,6 +41,7 @@ def lib():
         L.flint_reactor_advance_graph.argtypes = [vp, l, l, i, vp, vp, d, d, d, i, vp, vp, vp]
         L.flint_reactor_advance_noalloc.argtypes = [vp, l, l, i, vp, vp, d, d, d, i, vp, vp, vp]
         L.flint_reactor_jacobian.argtypes = [vp, l, l, i, vp, vp, vp, vp, vp]
+        L.flint_reactor_check_step.argtypes = [vp, l, l, vp, vp, vp, d, d, d, vp, vp, vp]
         L.flint_reactor_autotune.argtypes = [vp, l, l, i, vp, vp, d, d, d, C.POINTER(d)]
         L.flint_reactor_set_launch.argtypes = [vp, i, i]
         L.flint_reactor_get_launch.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
@@ -281,6 +282,18 @@ class Mechanism:
         _chk(X, "X", ld, self.nz), _chk(p, "p", n, optional=mode == 0), _chk(F, "F", ld, self.nz)
         _chk(J, "J", ld, self.nz * self.nz, optional=True)
         check(lib().flint_reactor_jacobian(self._h, n, ld, mode, _p(X), _p(p), _p(F), _p(J), _stream(stream)))
+
+    def check_step(self, X, Xp, p, dt, flag, tol_y=5e-5, tol_t=0.5, check_out=None, n=None, stream=None):
+        """The FP64 judge of a candidate constant-pressure step X -> Xp over dt ([ns+1, ld] each):
+        flag [>= n] int32 gets 0 where the backward-Euler and trapezoidal Newton corrections from Xp
+        stay within (tol_y, tol_t), else 1; check_out [2, ld] the larger correction's max |dY_k|
+        and |dT|."""
+        ld = X.shape[-1]
+        n = ld if n is None else n
+        _chk(X, "X", ld, self.nz), _chk(Xp, "Xp", ld, self.nz), _chk(p, "p", n)
+        _chk(flag, "flag", n, dtype="int32"), _chk(check_out, "check_out", ld, 2, optional=True)
+        check(lib().flint_reactor_check_step(self._h, n, ld, _p(X), _p(Xp), _p(p), dt, tol_y, tol_t, _p(flag),
+                                             _p(check_out), _stream(stream)))
 
     def autotune(self, X, p=None, mode=0, t_end=1e-4, rtol=1e-6, atol=1e-12):
         ld = X.shape[-1]

@@ -304,6 +304,19 @@ int flint_mlp_gradients(flint_mlp *m, long n, const float *X_dev, const float *T
 int flint_mlp_train_step(flint_mlp *m, long n, const float *X_dev, const float *T_dev, float lr, float *loss_dev,
                          void *stream);
 
+/* The FP64 judge of the learned step as a building block, for any predictor of a
+ * constant-pressure step (no reference counterpart).  For each cell, a candidate state Xp_dev
+ * [(ns+1)][ld] after a step of dt from X_dev (both (Y_1..Y_ns, T), p_dev [n] in Pa): the Newton
+ * corrections from Xp towards the backward-Euler and the trapezoidal step, with F and J at Xp,
+ * delta_BE = -(I - dt J)^-1 (Xp - X - dt F(Xp)) and
+ * delta_TR = -(I - dt/2 J)^-1 (Xp - X - dt/2 (F(X) + F(Xp))).  flag_dev [n] receives 0 when
+ * both stay within tol_y in every Y_k and tol_t in T, else 1 (also for T outside the tables);
+ * check_dev [2][ld] (may be NULL) the larger correction's max_k |delta_k| and |delta_T|.  X_dev
+ * and Xp_dev are not changed.  Needs ns + 1 <= 32. */
+int flint_reactor_check_step(flint_mech *m, long n, long ld, const double *X_dev, const double *Xp_dev,
+                             const double *p_dev, double dt, double tol_y, double tol_t, int *flag_dev,
+                             double *check_dev, void *stream);
+
 /* Learned constant-pressure reactor step with the batched BDF as fallback (no reference
  * counterpart).  X_dev = (Y_1..Y_ns, T) [ns+1][ld] and p_dev [n] (Pa) as in flint_reactor_advance
  * mode 1; net maps the standardised features (log max(Y_k, ymin), T, log p) [ns+2], or with

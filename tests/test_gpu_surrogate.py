@@ -9,7 +9,9 @@ flint_amd/data/surrogate_rich31.npz.
   each passed the FP64 judge (the backward-Euler and trapezoidal Newton corrections from the
   prediction within the thresholds);
 * mid-trajectory, on 2^16 cells, the learned step is faster than the plain step (measured
-  1.75x at 2^18 cells)."""
+  1.79x at 2^18 cells);
+* flint_reactor_check_step, the judge on its own, passes the plain step and rejects a perturbed
+  one without changing either state."""
 import numpy as np
 import pytest
 
@@ -143,3 +145,32 @@ def test_surrogate_gain_mid_trajectory(mechs, cuda):
           f"accepted {float(acc.float().mean()):.3f}, max |dY| {dY:.2e}, max |dT| {dT:.3f} K", flush=True)
     assert dY <= 1e-4 and dT <= 1.0
     assert t_plain / t_mlp >= 1.25
+
+
+def test_check_step_judges_candidate_states(setup, cuda):
+    """flint_reactor_check_step, the learned step's FP64 judge for any predictor: mid-trajectory
+    the plain BDF's own step passes for most cells, the same step with one species moved by 1e-3
+    fails for every cell, and neither state is changed."""
+    import torch
+    md, mech, sur, X, p = setup
+    ns, n = md.ns, X.shape[1]
+    Xt = X.clone()
+    mech.advance(Xt, p, mode=1, t_end=sur.dt)
+    X0, Xt0 = X.clone(), Xt.clone()
+    flag = torch.full((n,), -1, dtype=torch.int32, device=cuda)
+    chk = torch.full((2, n), -1.0, dtype=torch.float64, device=cuda)
+    print("[gpu] check_step on the plain step", flush=True)
+    mech.check_step(X, Xt, p, sur.dt, flag, 5e-5, 0.5, check_out=chk)
+    torch.cuda.synchronize()
+    f, c = flag.cpu().numpy(), chk.cpu().numpy()
+    assert set(np.unique(f)) <= {0, 1}
+    assert (f == 0).mean() > 0.8
+    assert np.all((c[0] <= 5e-5) & (c[1] <= 0.5) == (f == 0))
+    assert torch.equal(X, X0) and torch.equal(Xt, Xt0)
+    bad = Xt.clone()
+    bad[3] += 1e-3
+    mech.check_step(X, bad, p, sur.dt, flag, 5e-5, 0.5, check_out=chk)
+    torch.cuda.synchronize()
+    assert int(flag.sum()) == n
+    assert float(chk[0].min()) > 5e-5
+
