@@ -127,6 +127,108 @@ __global__ __launch_bounds__(64) void gemm_fwd_kernel(int M, int N, int K, const
   }
 }
 
+// Forward product with the weights staged in LDS, for K <= 256, N % 32 == 0 and N <= 256
+// (NT = N / 32): each block copies WT [N][K] into LDS once, zero-filled to a multiple of 16 columns
+// and padded by 16 bytes per row (so that the 32 rows a fragment read touches start in different
+// banks); each wavefront then computes 32 x N output strips over a grid-stride loop.  All A
+// fragments of a strip (up to 16 k-steps) are loaded into registers before its MFMAs, so the loads
+// overlap each other and the other wavefronts' MFMAs; every B fragment comes from LDS.  AL: K % 8
+// == 0, one 16-byte load per A fragment (otherwise element loads, zero beyond K).  The same k-steps,
+// zero tail and MFMA order per output tile as gemm_kernel and gemm_fwd_kernel (bit-identical
+// results), the same epilogues.
+template <bool AL>
+__device__ __forceinline__ bf16x8 a_fragment(const __bf16* __restrict__ A, int M, int K, int am, int k) {
+  bf16x8 v = {};
+  if (am >= M) return v;
+  if (AL) {
+    if (k < K) v = *reinterpret_cast<const bf16x8*>(A + (long)am * K + k);
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      if (k + e < K) v[e] = A[(long)am * K + k + e];
+  }
+  return v;
+}
+
+template <int EPI, int NT, bool AL>
+__global__ __launch_bounds__(512) void gemm_fwd_lds_kernel(int M, int N, int K, const __bf16* __restrict__ A,
+                                                          const __bf16* __restrict__ WT,
+                                                          const float* __restrict__ bias, float* __restrict__ O32,
+                                                          __bf16* __restrict__ O16, long ldo) {
+  constexpr int KSMAX = 16;  // k-steps of 16: K <= 256
+  extern __shared__ __align__(16) char smem_raw[];
+  __bf16* ws = reinterpret_cast<__bf16*>(smem_raw);
+  const int K16 = (K + 15) & ~15, KP = K16 + 8, ks = K16 / 16;
+  if (AL) {
+    const int kc = K16 / 8, kv = K / 8;
+    for (int i = threadIdx.x; i < N * kc; i += blockDim.x) {
+      const int r = i / kc, c = i - r * kc;
+      bf16x8 v = {};
+      if (c < kv) v = *reinterpret_cast<const bf16x8*>(WT + (long)r * K + 8 * c);
+      *reinterpret_cast<bf16x8*>(ws + r * KP + 8 * c) = v;
+    }
+  } else {
+    for (int i = threadIdx.x; i < N * K16; i += blockDim.x) {
+      const int r = i / K16, c = i - r * K16;
+      ws[r * KP + c] = c < K ? WT[(long)r * K + c] : (__bf16)0.0f;
+    }
+  }
+  __syncthreads();
+  const int l = threadIdx.x & 63, nw = blockDim.x >> 6;
+  const int kk = 8 * (l >> 5);
+  const int tiles = (M + 31) / 32;
+  for (int tile = blockIdx.x * nw + (threadIdx.x >> 6); tile < tiles; tile += gridDim.x * nw) {
+    const int m0 = tile * 32, am = m0 + (l & 31);
+    bf16x8 af[KSMAX];
+#pragma unroll
+    for (int q = 0; q < KSMAX; ++q)
+      if (q < ks) af[q] = a_fragment<AL>(A, M, K, am, 16 * q + kk);
+    // the column tiles in groups of at most 4 (64 accumulator registers), reusing the A fragments
+    constexpr int G = NT < 4 ? NT : 4;
+#pragma unroll 1
+    for (int t0 = 0; t0 < NT; t0 += G) {
+    f32x16 acc[G];
+#pragma unroll
+    for (int t = 0; t < G; ++t) acc[t] = f32x16{};
+#pragma unroll
+    for (int q = 0; q < KSMAX; ++q) {
+      if (q < ks) {
+#pragma unroll
+        for (int t = 0; t < G; ++t) {
+          const bf16x8 b = *reinterpret_cast<const bf16x8*>(ws + (32 * (t0 + t) + (l & 31)) * KP + 16 * q + kk);
+          acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[q], b, acc[t], 0, 0, 0);
+        }
+      }
+    }
+    // the bias of this lane's columns is loaded once, before the stores: a load waited for between
+    // stores would wait for the stores as well
+    float bv[G];
+#pragma unroll
+    for (int t = 0; t < G; ++t) bv[t] = bias ? bias[32 * (t0 + t) + (l & 31)] : 0.0f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int m = m0 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+      if (m >= M) continue;
+      const long o0 = (long)m * ldo + 32 * t0 + (l & 31);
+#pragma unroll
+      for (int t = 0; t < G; ++t) {
+        const long o = o0 + 32 * t;
+        float v = acc[t][r];
+        if (EPI == EPI_LINEAR) {
+          if (bias) v += bv[t];
+          if (O32) O32[o] = v;
+          if (O16) O16[o] = (__bf16)v;
+        } else {
+          v += bv[t];
+          if (O32) O32[o] = v;
+          O16[o] = (__bf16)gelu(v);
+        }
+      }
+    }
+    }
+  }
+}
+
 // WT_l [dout][din] from W_l [din][dout] (bf16), all layers
 __global__ void transpose_w16(int din, int dout, const __bf16* __restrict__ w, __bf16* __restrict__ wt) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -211,9 +313,55 @@ hipError_t gemm(int M, int N, int K, const __bf16* A, long sam, long sak, const 
 
 inline unsigned nb(long n, int b) { return (unsigned)((n + b - 1) / b); }
 
+// blocks of the LDS-staged forward product: as many as fit on the GPU at once (LDS-limited), at
+// most one per 8 output strips
+template <int EPI, int NT, bool AL>
+hipError_t gemm_fwd_lds(int M, int K, const __bf16* A, const __bf16* WT, const float* bias, float* O32, __bf16* O16,
+                        long ldo, hipStream_t s) {
+  constexpr int N = 32 * NT;
+  const size_t lds = (size_t)N * (((K + 15) & ~15) + 8) * sizeof(__bf16);
+  static int cus = 0;
+  static size_t lds_set = 0;
+  if (!cus) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+  }
+  if (lds > lds_set) {
+    const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&gemm_fwd_lds_kernel<EPI, NT, AL>),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    lds_set = lds;
+  }
+  const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / lds));
+  const long strips = (M + 31) / 32;
+  const int blocks = (int)std::max<long>(1, std::min<long>((long)cus * per_cu, (strips + 7) / 8));
+  hipLaunchKernelGGL((gemm_fwd_lds_kernel<EPI, NT, AL>), dim3(blocks), dim3(512), lds, s, M, N, K, A, WT, bias, O32,
+                     O16, ldo);
+  return hipGetLastError();
+}
+
+template <int EPI, bool AL>
+hipError_t gemm_fwd_lds_n(int M, int N, int K, const __bf16* A, const __bf16* WT, const float* bias, float* O32,
+                          __bf16* O16, long ldo, hipStream_t s) {
+  switch (N / 32) {
+    case 1: return gemm_fwd_lds<EPI, 1, AL>(M, K, A, WT, bias, O32, O16, ldo, s);
+    case 2: return gemm_fwd_lds<EPI, 2, AL>(M, K, A, WT, bias, O32, O16, ldo, s);
+    case 4: return gemm_fwd_lds<EPI, 4, AL>(M, K, A, WT, bias, O32, O16, ldo, s);
+    default: return gemm_fwd_lds<EPI, 8, AL>(M, K, A, WT, bias, O32, O16, ldo, s);
+  }
+}
+
+// true when the LDS-staged forward product takes the shape
+inline bool lds_shape(int N, int K, long ldo) {
+  return K >= 1 && K <= 256 && (N == 32 || N == 64 || N == 128 || N == 256) && ldo >= N;
+}
+
 template <int EPI>
 hipError_t gemm_fwd(int M, int N, int K, const __bf16* A, const __bf16* WT, const float* bias, float* O32,
                     __bf16* O16, long ldo, hipStream_t s) {
+  if (lds_shape(N, K, ldo)) return gemm_fwd_lds_n<EPI, true>(M, N, K, A, WT, bias, O32, O16, ldo, s);
   if (N <= 32) {
     hipLaunchKernelGGL((gemm_fwd_kernel<EPI, 1>), dim3(nb(M, 32), 1), dim3(64), 0, s, M, N, K, A, WT, bias, O32, O16, ldo);
   } else if (N <= 64) {
@@ -303,7 +451,14 @@ int flint_mlp::forward(long n, float* out, hipStream_t s, bool keep) {
     const float* b = w32 + b_off[l];
     const bool fast = din % 8 == 0;
     hipError_t e;
-    if (l + 1 < L)
+    if (!fast && lds_shape(dout, din, dout)) {
+      // K not a multiple of 8 (the input layer): element loads of A, weights zero-filled in LDS
+      if (l + 1 < L)
+        e = gemm_fwd_lds_n<EPI_GELU, false>((int)n, dout, din, a16[l], WT, b, keep ? z32[l] : nullptr, a16[l + 1],
+                                            dout, s);
+      else
+        e = gemm_fwd_lds_n<EPI_LINEAR, false>((int)n, dout, din, a16[l], WT, b, out ? out : z32[l], nullptr, dout, s);
+    } else if (l + 1 < L)
       e = fast ? gemm_fwd<EPI_GELU>((int)n, dout, din, a16[l], WT, b, keep ? z32[l] : nullptr, a16[l + 1], dout, s)
                : gemm<EPI_GELU>((int)n, dout, din, a16[l], din, 1, W, dout, 1, b, nullptr, keep ? z32[l] : nullptr,
                                 a16[l + 1], dout, s);
