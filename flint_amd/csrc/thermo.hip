@@ -8,8 +8,8 @@
 // temperature f_tab recomputes from the pressure, T' = p / (R rho), for the enthalpy.  Layout:
 // cons[k*ld + i], k = (rho_1..rho_ns, rho u_1..rho u_nd, rho E); prim[k*ld + i] =
 // (rho_1..rho_ns, u_1..u_nd, p).  The table lookups are gathers of one row pair per cell and
-// species; the sums over species reread the cell's densities (L1/L2 hits), so the kernel keeps
-// no per-species arrays in registers and runs for any species count.
+// species.  Up to 32 species cons2prim keeps the cell's densities in registers for the Newton
+// loop's sums; beyond that it rereads them and runs for any species count.
 #include "flint_device.h"
 #include "flint_thermo.h"
 
@@ -84,6 +84,30 @@ __device__ __forceinline__ bool energy_E(const Tab& t, const double* __restrict_
   return true;
 }
 
+// The species loops of cons2prim over the cell's densities: NSM > 0 holds them in registers (read
+// once from memory with non-temporal loads, so that the state stream does not evict the table rows
+// the Newton loop gathers), every loop unrolled over NSM with static indices; NSM = 0 rereads them
+// from memory and serves any species count.
+template <int NSM>
+struct Dens {
+  double r[NSM > 0 ? NSM : 1];
+  const double* __restrict__ c;
+  long ld;
+  __device__ __forceinline__ double operator()(int s) const { return NSM > 0 ? r[s] : c[(long)s * ld]; }
+};
+
+template <int NSM, class F>
+__device__ __forceinline__ void for_species(int ns, F&& f) {
+  if constexpr (NSM > 0) {
+#pragma unroll
+    for (int s = 0; s < NSM; ++s)
+      if (s < ns) f(s);
+  } else {
+    for (int s = 0; s < ns; ++s) f(s);
+  }
+}
+
+template <int NSM>
 __global__ __launch_bounds__(256) void cons2prim_kernel(Tab t, const double* __restrict__ W, long n, long ld, int nd,
                                                         const double* __restrict__ cons, const double* __restrict__ Tin,
                                                         double* __restrict__ prim, double* __restrict__ Tout,
@@ -96,10 +120,19 @@ __global__ __launch_bounds__(256) void cons2prim_kernel(Tab t, const double* __r
   if (i >= n) return;
   const int ns = t.ns, np = ns + nd + 1;
   const double* __restrict__ c = cons + i;
-  double rho, Rgas;
-  rho_R(sRi, ns, c, ld, rho, Rgas);
+  Dens<NSM> den;
+  den.c = c;
+  den.ld = ld;
+  if constexpr (NSM > 0) {
+#pragma unroll
+    for (int s = 0; s < NSM; ++s) den.r[s] = s < ns ? __builtin_nontemporal_load(c + (long)s * ld) : 0.0;
+  }
+  // sum(r) and f_Rtot(r), in the order of rho_R
+  double rho = 0.0, Rgas = 0.0;
+  for_species<NSM>(ns, [&](int s) { rho = rho + den(s); });
+  for_species<NSM>(ns, [&](int s) { Rgas = Rgas + sRi[s] * den(s) / rho; });
   // outputs bypass L2 (streaming stores), which keeps the table rows the Newton loop gathers there
-  for (int s = 0; s < ns; ++s) stream_store(prim + (long)s * ld + i, c[(long)s * ld]);
+  for_species<NSM>(ns, [&](int s) { stream_store(prim + (long)s * ld + i, den(s)); });
   for (int d = 0; d < nd; ++d) stream_store(prim + (long)(ns + d) * ld + i, c[(long)(ns + d) * ld] / rho);
   const double vel = norm2(c + (long)ns * ld, ld, nd, rho, true);
   const double energy = c[(long)(np - 1) * ld] / rho - 0.5 * vel * vel;
@@ -121,20 +154,20 @@ __global__ __launch_bounds__(256) void cons2prim_kernel(Tab t, const double* __r
     const double* __restrict__ r2 = t.tabh + (long)(Ti2 - t.Th_min) * (2 * ns);
     double cp = 0.0, ent = 0.0;
     if (Ti2 == Ti) {
-      for (int s = 0; s < ns; ++s) {
+      for_species<NSM>(ns, [&](int s) {
         const double2 a = ld2(r0 + 2 * s), b = ld2(r0 + 2 * ns + 2 * s);
-        const double r = c[(long)s * ld];
+        const double r = den(s);
         cp = cp + r / rho * (a.x + (b.x - a.x) * Td);
         ent = ent + (a.y + (b.y - a.y) * Td2) * r / rho;
-      }
+      });
     } else {
-      for (int s = 0; s < ns; ++s) {
+      for_species<NSM>(ns, [&](int s) {
         const double2 a = ld2(r0 + 2 * s), b = ld2(r0 + 2 * ns + 2 * s);
         const double2 a2 = ld2(r2 + 2 * s), b2 = ld2(r2 + 2 * ns + 2 * s);
-        const double r = c[(long)s * ld];
+        const double r = den(s);
         cp = cp + r / rho * (a.x + (b.x - a.x) * Td);
         ent = ent + (a2.y + (b2.y - a2.y) * Td2) * r / rho;
-      }
+      });
     }
     const double e_it = ent - p_it / rho;
     const double cv = cp - Rgas;
@@ -178,8 +211,15 @@ hipError_t flint_thermo_cons2prim(const double* tabh, int Th_min, int Th_max, in
   if (n <= 0) return hipSuccess;
   if (ns < 1 || ns > FLINT_THERMO_MAXNS) return hipErrorInvalidValue;
   const flint_thermo_k::Tab t{tabh, Th_min, Th_max, ns};
-  hipLaunchKernelGGL(flint_thermo_k::cons2prim_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t, W, n, ld,
-                     nd, cons, T, prim, T_out, iters, status);
+if (t.ns <= 16)
+    hipLaunchKernelGGL(flint_thermo_k::cons2prim_kernel<16>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t, W,
+                       n, ld, nd, cons, T, prim, T_out, iters, status);
+  else if (t.ns <= 32)
+    hipLaunchKernelGGL(flint_thermo_k::cons2prim_kernel<32>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t, W,
+                       n, ld, nd, cons, T, prim, T_out, iters, status);
+  else
+    hipLaunchKernelGGL(flint_thermo_k::cons2prim_kernel<0>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, t, W,
+                       n, ld, nd, cons, T, prim, T_out, iters, status);
   return hipGetLastError();
 }
 
