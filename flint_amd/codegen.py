@@ -380,11 +380,11 @@ def emit(prog):
     a("  if (i >= n) return;")
     a("  double r[NS], F[NZ];")
     a("#pragma unroll")
-    a("  for (int k = 0; k < NS; ++k) r[k] = rho[k * ld + i];")
-    a("  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, T[i], F, nullptr,")
+    a("  for (int k = 0; k < NS; ++k) r[k] = stream_load(rho + k * ld + i);")
+    a("  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, stream_load(T + i), F, nullptr,")
     a("      NullSink{}, NullOut{});")
     a("#pragma unroll")
-    a("  for (int k = 0; k < NS; ++k) wdot[k * ld + i] = F[k];")
+    a("  for (int k = 0; k < NS; ++k) stream_store(wdot + k * ld + i, F[k]);")
     a("  if (status) status[i] = st;")
     a("}")
     a("")
@@ -402,19 +402,19 @@ def emit(prog):
     a("  if (i >= n) return;")
     a("  double r[NS], F[NZ];")
     a("#pragma unroll")
-    a("  for (int k = 0; k < NS; ++k) r[k] = Z[k * ld + i];")
-    a("  const double T = Z[NS * ld + i];")
+    a("  for (int k = 0; k < NS; ++k) r[k] = stream_load(Z + k * ld + i);")
+    a("  const double T = stream_load(Z + NS * ld + i);")
     a("  double* __restrict__ Ji = J + i;")
     a("  __shared__ double smem[JAC ? NSLOT * JAC_BLOCK : 1];")
     a("  const int st = eval<JAC, true, JAC_BLOCK>(tabk, Tk_min, Tk_max, tabh, Th_min, Th_max, r, T, F, smem + threadIdx.x,")
     a("      [&](int row, int col, double v) { stream_store(Ji + (long)(row * NZ + col) * ld, v); },")
     a("      [&](const double (&Fv)[NZ]) {")
     a("#pragma unroll")
-    a("        for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = Fv[k];")
+    a("        for (int k = 0; k < NZ; ++k) stream_store(Fo + k * ld + i, Fv[k]);")
     a("      });")
     a("  if (!JAC) {")
     a("#pragma unroll")
-    a("    for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = F[k];")
+    a("    for (int k = 0; k < NZ; ++k) stream_store(Fo + k * ld + i, F[k]);")
     a("  }")
     a("  if (status) status[i] = st;")
     a("}")

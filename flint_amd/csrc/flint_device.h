@@ -13,6 +13,9 @@ __device__ __forceinline__ double2 ld2(const double* __restrict__ p) {
 // Output that is written once and not read again by the kernel: non-temporal vector store,
 // so the multi-GB Jacobian stream does not evict the rate tables from L2.
 __device__ __forceinline__ void stream_store(double* p, double v) { __builtin_nontemporal_store(v, p); }
+// Input that is read once: non-temporal load, so the state stream does not evict the tables the
+// kernel gathers from L2.
+__device__ __forceinline__ double stream_load(const double* p) { return __builtin_nontemporal_load(p); }
 
 // Returns p unchanged but opaque to the optimiser, so loads through it are issued where they are
 // written instead of being merged with earlier loads of the same table row (which would keep

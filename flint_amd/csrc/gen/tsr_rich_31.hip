@@ -13699,11 +13699,11 @@ __global__ __launch_bounds__(256) void wdot_kernel(long n, long ld, const double
   if (i >= n) return;
   double r[NS], F[NZ];
 #pragma unroll
-  for (int k = 0; k < NS; ++k) r[k] = rho[k * ld + i];
-  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, T[i], F, nullptr,
+  for (int k = 0; k < NS; ++k) r[k] = stream_load(rho + k * ld + i);
+  const int st = eval<false, false, 1>(tabk, Tk_min, Tk_max, nullptr, 0, 0, r, stream_load(T + i), F, nullptr,
       NullSink{}, NullOut{});
 #pragma unroll
-  for (int k = 0; k < NS; ++k) wdot[k * ld + i] = F[k];
+  for (int k = 0; k < NS; ++k) stream_store(wdot + k * ld + i, F[k]);
   if (status) status[i] = st;
 }
 
@@ -13719,19 +13719,19 @@ __global__ __launch_bounds__(JAC_BLOCK) void rhs_jac_kernel(long n, long ld, con
   if (i >= n) return;
   double r[NS], F[NZ];
 #pragma unroll
-  for (int k = 0; k < NS; ++k) r[k] = Z[k * ld + i];
-  const double T = Z[NS * ld + i];
+  for (int k = 0; k < NS; ++k) r[k] = stream_load(Z + k * ld + i);
+  const double T = stream_load(Z + NS * ld + i);
   double* __restrict__ Ji = J + i;
   __shared__ double smem[JAC ? NSLOT * JAC_BLOCK : 1];
   const int st = eval<JAC, true, JAC_BLOCK>(tabk, Tk_min, Tk_max, tabh, Th_min, Th_max, r, T, F, smem + threadIdx.x,
       [&](int row, int col, double v) { stream_store(Ji + (long)(row * NZ + col) * ld, v); },
       [&](const double (&Fv)[NZ]) {
 #pragma unroll
-        for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = Fv[k];
+        for (int k = 0; k < NZ; ++k) stream_store(Fo + k * ld + i, Fv[k]);
       });
   if (!JAC) {
 #pragma unroll
-    for (int k = 0; k < NZ; ++k) Fo[k * ld + i] = F[k];
+    for (int k = 0; k < NZ; ++k) stream_store(Fo + k * ld + i, F[k]);
   }
   if (status) status[i] = st;
 }

@@ -262,14 +262,14 @@ __global__ __launch_bounds__(BLOCK) void general_reg_kernel(GenProg P, const dou
   double* om = smem + threadIdx.x;
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    c0[k] = k < ns ? rho[(long)k * ld + i] / W[k] : 0.0;
-    if (NCH > 1) c1[k] = k + 16 < ns ? rho[(long)(k + 16) * ld + i] / W[k + 16] : 0.0;
+    c0[k] = k < ns ? stream_load(rho + (long)k * ld + i) / W[k] : 0.0;
+    if (NCH > 1) c1[k] = k + 16 < ns ? stream_load(rho + (long)(k + 16) * ld + i) / W[k + 16] : 0.0;
   }
   for (int k = 0; k < ns; ++k) om[k * BLOCK] = 0.0;
   RegCell<NCH> S{c0, c1, om};
-  const Row R = table_row(P, tabk, Tk_min, Tk_max, T[i]);
+  const Row R = table_row(P, tabk, Tk_min, Tk_max, stream_load(T + i));
   reactions(P, S, R.r0, R.r1, R.Td);
-  for (int k = 0; k < ns; ++k) wdot[(long)k * ld + i] = om[k * BLOCK];
+  for (int k = 0; k < ns; ++k) stream_store(wdot + (long)k * ld + i, om[k * BLOCK]);
   if (status) status[i] = R.st;
 }
 
@@ -285,13 +285,13 @@ __global__ __launch_bounds__(BLOCK) void general_lds_kernel(GenProg P, const dou
   double* co = smem + (long)ns * BLOCK + threadIdx.x;
   double* om = smem + threadIdx.x;
   for (int k = 0; k < ns; ++k) {
-    co[k * BLOCK] = rho[(long)k * ld + i] / W[k];
+    co[k * BLOCK] = stream_load(rho + (long)k * ld + i) / W[k];
     om[k * BLOCK] = 0.0;
   }
   LdsCell S{co, om};
-  const Row R = table_row(P, tabk, Tk_min, Tk_max, T[i]);
+  const Row R = table_row(P, tabk, Tk_min, Tk_max, stream_load(T + i));
   reactions(P, S, R.r0, R.r1, R.Td);
-  for (int k = 0; k < ns; ++k) wdot[(long)k * ld + i] = om[k * BLOCK];
+  for (int k = 0; k < ns; ++k) stream_store(wdot + (long)k * ld + i, om[k * BLOCK]);
   if (status) status[i] = R.st;
 }
 
