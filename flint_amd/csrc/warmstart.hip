@@ -109,6 +109,9 @@ __global__ __launch_bounds__(512) void fused_kernel(long n, long ld, int ns, con
   constexpr int N1 = H1T * KS1 * 64, N2 = H2T * KS2 * 64, N3 = KS3 * 64;
   __shared__ bf16x8 wl[N1 + N2 + N3];
   __shared__ float sb1[H1], sb2[H2], sb3[32];
+  // the output scaling in LDS as well: global loads between the stores of G would each wait for the
+  // stores before them (one memory counter for both on this GPU)
+  __shared__ double som[32], sos[32];
   const bf16x8* w1 = wl;
   const bf16x8* w2 = wl + N1;
   const bf16x8* w3 = wl + N1 + N2;
@@ -116,14 +119,16 @@ __global__ __launch_bounds__(512) void fused_kernel(long n, long ld, int ns, con
   for (int q = threadIdx.x; q < H1; q += blockDim.x) sb1[q] = b1[q];
   for (int q = threadIdx.x; q < H2; q += blockDim.x) sb2[q] = b2[q];
   for (int q = threadIdx.x; q < 32; q += blockDim.x) sb3[q] = q < dL ? b3[q] : 0.0f;
+  for (int q = threadIdx.x; q < 32; q += blockDim.x) {
+    som[q] = q < dL ? scal[2 * (ns + 2) + q] : 0.0;
+    sos[q] = q < dL ? scal[2 * (ns + 2) + (ns + 1) + q] : 0.0;
+  }
   __syncthreads();
 
   const int lane = threadIdx.x & 63, h = lane >> 5;
   const int wpb = blockDim.x >> 6;
   const double* __restrict__ imean = scal;
   const double* __restrict__ iscale = scal + (ns + 2);
-  const double* __restrict__ omean = scal + 2 * (ns + 2);
-  const double* __restrict__ oscale = omean + (ns + 1);
   for (long tile = (long)blockIdx.x * wpb + (threadIdx.x >> 6); tile * 32 < n; tile += (long)gridDim.x * wpb) {
     const long c = tile * 32 + (lane & 31);
     const bool ok = c < n;
@@ -203,7 +208,7 @@ __global__ __launch_bounds__(512) void fused_kernel(long n, long ld, int ns, con
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int j = 8 * (r >> 2) + 4 * h + (r & 3);
-        if (j < dL) G[(long)j * ld + c] = (double)(o[r] + sb3[j + z]) * oscale[j + z] + omean[j + z];
+        if (j < dL) G[(long)j * ld + c] = (double)(o[r] + sb3[j + z]) * sos[j + z] + som[j + z];
       }
     }
   }
