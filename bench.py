@@ -287,8 +287,18 @@ def mlp_leg(args, dev, stream):
     for phase, t0 in (("t0", 0.0), ("mid", 5e-5)):
         if t0 > 0:
             mech.advance(X, p, mode=1, t_end=t0, stream=stream)
-        Xp, Xm = X.clone(), X.clone()
-        ms_plain = timed(stream, lambda: mech.advance(Xp, p, mode=1, t_end=sur.dt, stream=stream), 1)
+        Xp, Xm, Xq = X.clone(), X.clone(), X.clone()
+        # the plain step in the caller's order and with the cells handed out by descending |dT/dt|
+        # (the learned step orders its fallback cells that way); the faster is the comparison
+        ms_given = timed(stream, lambda: mech.advance(Xp, p, mode=1, t_end=sur.dt, stream=stream), 1)
+        mech.set_order(1)
+        try:
+            mech.advance(X.clone(), p, mode=1, t_end=1e-9, stream=stream)  # the order's buffers, untimed
+            ms_sorted = timed(stream, lambda: mech.advance(Xq, p, mode=1, t_end=sur.dt, stream=stream), 1)
+        finally:
+            mech.set_order(0)
+        assert torch.equal(Xp, Xq)
+        ms_plain = min(ms_given, ms_sorted)
         fb = torch.zeros(n, dtype=torch.int32, device=dev)
         nf = [0]
 
@@ -298,7 +308,8 @@ def mlp_leg(args, dev, stream):
         ms_mlp = timed(stream, run, 1)
         acc = fb == 0
         r = dict(t_start=t0, dt=sur.dt, plain_cells_per_s=n / (ms_plain * 1e-3), mlp_cells_per_s=n / (ms_mlp * 1e-3),
-                 speedup=ms_plain / ms_mlp, fallback_rate=nf[0] / n)
+                 speedup=ms_plain / ms_mlp, fallback_rate=nf[0] / n, plain_ms_given_order=ms_given,
+                 plain_ms_by_dTdt=ms_sorted, mlp_ms=ms_mlp)
         if bool(acc.any()):
             dY = (Xm[:ns, acc] - Xp[:ns, acc]).abs().max(0).values
             dT = (Xm[ns, acc] - Xp[ns, acc]).abs()

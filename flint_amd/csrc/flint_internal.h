@@ -56,6 +56,10 @@ void flint_mech_release_extra(flint_mech* m);
 int flint_reactor_verify(flint_mech* m, long n, long ld, double* X, const double* Xp, long ldp, const double* F0,
                          long ldf, const double* p, double dt, double tol_y, double tol_t, int* flag, int* idx,
                          int* count, double* dmax, void* stream);
+// flint_reactor_advance with the cells taken in the order given (whatever the handle's work-order
+// policy): for callers that have ordered the cells themselves.
+int flint_reactor_advance_as_given(flint_mech* m, long n, long ld, int mode, double* X, const double* p, double t_end,
+                                   double rtol, double atol, int max_steps, int* stats, int* status, void* stream);
 int flint_reactor_prepare(flint_mech* m);  // reactor program of a generated mechanism (no-op otherwise)
 void flint_mech_release_cea(flint_mech* m);
 void flint_mech_release_sur(flint_mech* m);

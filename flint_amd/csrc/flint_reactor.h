@@ -36,7 +36,11 @@ size_t flint_reactor_team_bytes(const flint_reactor::Prog& P);
 size_t flint_reactor_shared_bytes(const flint_reactor::Prog& P);  // block-shared program in LDS
 hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reactor::Opts& O, long n, long ld,
                                 double* X, const double* pres, int* stats, int* status, int* counter,
-                                int teams_per_block, int blocks, double* jac_scratch, hipStream_t s);
+                                int teams_per_block, int blocks, double* jac_scratch, const int* order, hipStream_t s);
+// Work order by descending |key[cell]| (cells idx[0..n) or, with idx null, 0..n-1) into out[n]; hist
+// is scratch of FLINT_ORDER_BUCKETS ints.  Kernels only (no memset node: graph-safe).
+constexpr int FLINT_ORDER_BUCKETS = 4096;
+hipError_t flint_reactor_order(long n, const double* key, const int* idx, int* out, int* hist, hipStream_t s);
 
 // Allocation-free reactor launches (flint_reactor_advance_noalloc): the per-team Jacobian slots
 // and the work counter live in a file-scope device array of reactor.hip with room for

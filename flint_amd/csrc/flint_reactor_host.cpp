@@ -23,8 +23,11 @@ struct flint_extra {
   int blocks = 0;
   bool tuned = false;
   int newton = 1;  // Newton policy (flint_reactor_set_newton)
-  double* d_chkF = nullptr;  // right-hand side at the old states for flint_reactor_check_step
+  double* d_chkF = nullptr;  // right-hand side at the old states (flint_reactor_check_step, work order)
   size_t chk_cap = 0;
+  int order = 0;              // work order policy (flint_reactor_set_order)
+  int* d_order = nullptr;     // [order_cap] cell permutation, then FLINT_ORDER_BUCKETS class counts
+  size_t order_cap = 0;
   // captured graph of the last flint_reactor_advance_graph call
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -46,6 +49,7 @@ void flint_mech_release_extra(flint_mech* m) {
   if (x->d_counter) (void)hipFree(x->d_counter);
   if (x->d_jac) (void)hipFree(x->d_jac);
   if (x->d_chkF) (void)hipFree(x->d_chkF);
+  if (x->d_order) (void)hipFree(x->d_order);
   delete x;
   m->extra = nullptr;
 }
@@ -256,18 +260,91 @@ static int ensure_jac(flint_extra* x, int tpb, int blocks) {
   return FLINT_OK;
 }
 
+// Right-hand-side buffer [(ns+1) x ld] (d_chkF) for the trapezoidal check and the work order.
+static int ensure_rhs_buf(flint_mech* m, long ld) {
+  flint_extra* x = m->extra;
+  const size_t need = (size_t)(m->ns + 1) * ld;
+  if (x->chk_cap < need) {
+    if (x->gexec) (void)hipGraphExecDestroy(x->gexec);  // a captured graph may hold the old buffer
+    if (x->graph) (void)hipGraphDestroy(x->graph);
+    x->gexec = nullptr;
+    x->graph = nullptr;
+    x->gkey.clear();
+    if (x->d_chkF) HIPCHK(hipFree(x->d_chkF));
+    x->d_chkF = nullptr;
+    x->chk_cap = 0;
+    HIPCHK(hipMalloc(&x->d_chkF, need * sizeof(double)));
+    x->chk_cap = need;
+  }
+  return FLINT_OK;
+}
+
+// Buffers of the work order for n cells of leading dimension ld (allocates; not under capture).
+static int ensure_order(flint_mech* m, long n, long ld) {
+  flint_extra* x = m->extra;
+  if (int rc = ensure_rhs_buf(m, ld)) return rc;
+  if (x->order_cap < (size_t)n) {
+    if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
+    if (x->graph) (void)hipGraphDestroy(x->graph);
+    x->gexec = nullptr;
+    x->graph = nullptr;
+    x->gkey.clear();
+    if (x->d_order) HIPCHK(hipFree(x->d_order));
+    x->d_order = nullptr;
+    x->order_cap = 0;
+    HIPCHK(hipMalloc(&x->d_order, ((size_t)n + FLINT_ORDER_BUCKETS) * sizeof(int)));
+    x->order_cap = (size_t)n;
+  }
+  return FLINT_OK;
+}
+
+// Enqueues the work order of policy 1: dX/dt at the initial states, then the cells by descending
+// |dT/dt| (the stiffest, costliest cells first).  Returns the permutation (null for policy 0).
+static int enqueue_order(flint_mech* m, long n, long ld, int mode, const double* X, const double* p, hipStream_t s,
+                         const int** order) {
+  flint_extra* x = m->extra;
+  *order = nullptr;
+  if (x->order == 0) return FLINT_OK;
+  const int tpb = x->teams_per_block;
+  const int blocks = (int)std::min<long>(x->blocks, (n + tpb - 1) / tpb);
+  hipError_t e = flint_reactor_jac_launch(x->prog, mode, n, ld, X, p, x->d_chkF, nullptr, tpb, blocks, x->d_jac, s);
+  if (e == hipSuccess)
+    e = flint_reactor_order(n, x->d_chkF + (size_t)m->ns * ld, nullptr, x->d_order, x->d_order + x->order_cap, s);
+  if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("work order: ") + hipGetErrorString(e));
+  *order = x->d_order;
+  return FLINT_OK;
+}
+
 static int launch(flint_mech* m, long n, long ld, int mode, double* X, const double* p, double t_end, double rtol,
-                  double atol, int max_steps, int* stats, int* status, hipStream_t s, int tpb, int blocks) {
+                  double atol, int max_steps, int* stats, int* status, hipStream_t s, int tpb, int blocks,
+                  const int* order = nullptr) {
   flint_extra* x = m->extra;
   if (int rc = ensure_jac(x, tpb, blocks)) return rc;
   flint_reactor::Opts O{mode, rtol, atol, t_end, max_steps > 0 ? max_steps : 20000, x->newton};
-  hipError_t e = flint_reactor_launch(x->prog, O, n, ld, X, p, stats, status, x->d_counter, tpb, blocks, x->d_jac, s);
+  hipError_t e =
+      flint_reactor_launch(x->prog, O, n, ld, X, p, stats, status, x->d_counter, tpb, blocks, x->d_jac, order, s);
   if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("reactor launch: ") + hipGetErrorString(e));
   return FLINT_OK;
 }
 
 extern "C" int flint_reactor_advance(flint_mech* m, long n, long ld, int mode, double* X, const double* p, double t_end,
                                      double rtol, double atol, int max_steps, int* stats, int* status, void* stream) {
+  if (int rc = check_args(m, n, ld, mode, X, p, t_end, rtol, atol)) return rc;
+  if (n == 0) return FLINT_OK;
+  if (int rc = ensure_extra(m)) return rc;
+  flint_extra* x = m->extra;
+  const int* order = nullptr;
+  if (x->order) {
+    if (int rc = ensure_jac(x, x->teams_per_block, x->blocks)) return rc;
+    if (int rc = ensure_order(m, n, ld)) return rc;
+    if (int rc = enqueue_order(m, n, ld, mode, X, p, (hipStream_t)stream, &order)) return rc;
+  }
+  return launch(m, n, ld, mode, X, p, t_end, rtol, atol, max_steps, stats, status, (hipStream_t)stream,
+                x->teams_per_block, x->blocks, order);
+}
+
+int flint_reactor_advance_as_given(flint_mech* m, long n, long ld, int mode, double* X, const double* p, double t_end,
+                                   double rtol, double atol, int max_steps, int* stats, int* status, void* stream) {
   if (int rc = check_args(m, n, ld, mode, X, p, t_end, rtol, atol)) return rc;
   if (n == 0) return FLINT_OK;
   if (int rc = ensure_extra(m)) return rc;
@@ -296,7 +373,7 @@ extern "C" int flint_reactor_advance_noalloc(flint_mech* m, long n, long ld, int
   HIPCHK(flint_reactor_scratch(&counter, &slots));
   flint_reactor::Opts O{mode, rtol, atol, t_end, max_steps > 0 ? max_steps : 20000, x->newton};
   hipError_t e = flint_reactor_launch(x->prog, O, n, ld, X, p, stats, status, counter, x->teams_per_block, x->blocks,
-                                      slots, (hipStream_t)stream);
+                                      slots, nullptr, (hipStream_t)stream);
   if (e != hipSuccess) return flint_fail(FLINT_EHIP, std::string("reactor launch: ") + hipGetErrorString(e));
   return FLINT_OK;
 }
@@ -346,14 +423,7 @@ extern "C" int flint_reactor_check_step(flint_mech* m, long n, long ld, const do
   const int tpb = x->teams_per_block;
   const int blocks = (int)std::min<long>(x->blocks, (n + tpb - 1) / tpb);
   if (int rc = ensure_jac(x, tpb, blocks)) return rc;
-  const size_t need = (size_t)(m->ns + 1) * ld;
-  if (x->chk_cap < need) {
-    if (x->d_chkF) HIPCHK(hipFree(x->d_chkF));
-    x->d_chkF = nullptr;
-    x->chk_cap = 0;
-    HIPCHK(hipMalloc(&x->d_chkF, need * sizeof(double)));
-    x->chk_cap = need;
-  }
+  if (int rc = ensure_rhs_buf(m, ld)) return rc;
   hipStream_t s = (hipStream_t)stream;
   // F at the old states (the trapezoidal correction), then every cell judged (flag cleared by the kernel)
   hipError_t e = flint_reactor_jac_launch(x->prog, 1, n, ld, X, p, x->d_chkF, nullptr, tpb, blocks, x->d_jac, s);
@@ -389,6 +459,22 @@ extern "C" int flint_reactor_set_newton(flint_mech* m, int policy) {
     x->gkey.clear();
   }
   m->extra->newton = policy;
+  return FLINT_OK;
+}
+
+extern "C" int flint_reactor_set_order(flint_mech* m, int policy) {
+  if (!m) return flint_fail(FLINT_EARG, "null mechanism");
+  if (policy != 0 && policy != 1) return flint_fail(FLINT_EARG, "work order must be 0 (as given) or 1 (by |dT/dt|)");
+  if (int rc = ensure_extra(m)) return rc;
+  flint_extra* x = m->extra;
+  if (x->order != policy) {
+    if (x->gexec) (void)hipGraphExecDestroy(x->gexec);  // a captured graph holds the old work order
+    if (x->graph) (void)hipGraphDestroy(x->graph);
+    x->gexec = nullptr;
+    x->graph = nullptr;
+    x->gkey.clear();
+  }
+  x->order = policy;
   return FLINT_OK;
 }
 
@@ -478,12 +564,14 @@ extern "C" int flint_reactor_advance_graph(flint_mech* m, long n, long ld, int m
     int max_steps;
     int* stats;
     int* status;
-    int tpb, blocks;
-  } key{n, ld, mode, X, p, t_end, rtol, atol, max_steps, stats, status, x->teams_per_block, x->blocks};
+    int tpb, blocks, order;
+  } key{n, ld, mode, X, p, t_end, rtol, atol, max_steps, stats, status, x->teams_per_block, x->blocks, x->order};
   std::vector<char> kb(reinterpret_cast<char*>(&key), reinterpret_cast<char*>(&key) + sizeof key);
   hipStream_t s = (hipStream_t)stream;
   if (!x->gexec || kb != x->gkey) {
     if (int rc = ensure_jac(x, x->teams_per_block, x->blocks)) return rc;
+    if (x->order)
+      if (int rc = ensure_order(m, n, ld)) return rc;
     if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
     if (x->graph) (void)hipGraphDestroy(x->graph);
     x->gexec = nullptr;
@@ -491,7 +579,11 @@ extern "C" int flint_reactor_advance_graph(flint_mech* m, long n, long ld, int m
     hipStream_t cs;
     HIPCHK(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
     HIPCHK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-    int rc = launch(m, n, ld, mode, X, p, t_end, rtol, atol, max_steps, stats, status, cs, x->teams_per_block, x->blocks);
+    const int* order = nullptr;
+    int rc = enqueue_order(m, n, ld, mode, X, p, cs, &order);
+    if (!rc)
+      rc = launch(m, n, ld, mode, X, p, t_end, rtol, atol, max_steps, stats, status, cs, x->teams_per_block,
+                  x->blocks, order);
     hipGraph_t g = nullptr;
     hipError_t ee = hipStreamEndCapture(cs, &g);
     (void)hipStreamDestroy(cs);

@@ -9,12 +9,13 @@
 #include "../../include/flint_amd.h"
 #include "flint_internal.h"
 #include "flint_mlp.h"
+#include "flint_reactor.h"
 #include "flint_surrogate.h"
 
 struct flint_sur_data {
   long cap = 0;
   float *F = nullptr, *O = nullptr;
-  int *flag = nullptr, *idx = nullptr, *count = nullptr, *stg = nullptr, *sg = nullptr;
+  int *flag = nullptr, *idx = nullptr, *count = nullptr, *stg = nullptr, *sg = nullptr, *ord = nullptr;
   double *Xg = nullptr, *pg = nullptr, *scal = nullptr, *comp = nullptr, *elw = nullptr, *Fr = nullptr, *Xp = nullptr;
   std::vector<void*> all;
 };
@@ -53,6 +54,7 @@ static int ensure(flint_mech* m, long n) {
   rc |= grab(d, &d->O, (size_t)n * (ns + 1));
   rc |= grab(d, &d->flag, n);
   rc |= grab(d, &d->idx, n);
+  rc |= grab(d, &d->ord, (size_t)n + FLINT_ORDER_BUCKETS);  // fallback work order, then class counts
   rc |= grab(d, &d->count, 64);
   rc |= grab(d, &d->stg, (size_t)8 * n);
   rc |= grab(d, &d->sg, n);
@@ -121,13 +123,16 @@ extern "C" int flint_reactor_advance_mlp(flint_mech* m, flint_mlp* net, long n, 
   if (stats)
     for (int r = 0; r < 8; ++r) HIPCHK(hipMemsetAsync(stats + (size_t)r * ld, 0, (size_t)n * sizeof(int), s));
   if (nf > 0) {
-    HIPCHK(flint_sur_gather(0, ns + 1, ld, X, nf, nf, d->Xg, d->idx, p, d->pg, s));
-    if (int rc = flint_reactor_advance(m, nf, nf, 1, d->Xg, d->pg, t_end, rtol, atol, 0, stats ? d->stg : nullptr,
-                                       d->sg, stream))
+    // the fallback cells by descending |dT/dt| at the old state: the igniting cells, which take the
+    // most BDF steps, start first, so the launch does not wait on one of them started last
+    HIPCHK(flint_reactor_order(nf, d->Fr + (size_t)ns * ld, d->idx, d->ord, d->ord + nf, s));
+    HIPCHK(flint_sur_gather(0, ns + 1, ld, X, nf, nf, d->Xg, d->ord, p, d->pg, s));
+    if (int rc = flint_reactor_advance_as_given(m, nf, nf, 1, d->Xg, d->pg, t_end, rtol, atol, 0,
+                                                stats ? d->stg : nullptr, d->sg, stream))
       return rc;
-    HIPCHK(flint_sur_gather(1, ns + 1, ld, X, nf, nf, d->Xg, d->idx, nullptr, nullptr, s));
-    if (status) HIPCHK(flint_sur_scatter_int(1, n, status, nf, nf, d->sg, d->idx, s));
-    if (stats) HIPCHK(flint_sur_scatter_int(8, ld, stats, nf, nf, d->stg, d->idx, s));
+    HIPCHK(flint_sur_gather(1, ns + 1, ld, X, nf, nf, d->Xg, d->ord, nullptr, nullptr, s));
+    if (status) HIPCHK(flint_sur_scatter_int(1, n, status, nf, nf, d->sg, d->ord, s));
+    if (stats) HIPCHK(flint_sur_scatter_int(8, ld, stats, nf, nf, d->stg, d->ord, s));
   }
   if (n_fallback) *n_fallback = nf;
   return FLINT_OK;

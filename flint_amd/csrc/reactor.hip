@@ -965,13 +965,61 @@ __global__ void zero_counter(int* c) {
   if (threadIdx.x == 0 && blockIdx.x == 0) c[0] = 0;
 }
 
+// Work order of the persistent kernel: cells by descending |key| (|dT/dt| at the initial state),
+// a counting sort over ORDER_BUCKETS classes of the key's binary exponent and leading mantissa
+// bit (a factor of sqrt(2) per class).  Cells are independent, so the order changes no result;
+// handing the costliest cells out first shortens the tail that one slow cell would otherwise set
+// (longest-processing-time-first).  Within a class the order follows the atomics.
+constexpr int ORDER_BUCKETS = 4096;
+__device__ __forceinline__ int order_bucket(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v));  // sign bit 0
+  return ORDER_BUCKETS - 1 - (int)(b >> 51);
+}
+__global__ void order_hist(long n, const double* __restrict__ key, const int* __restrict__ idx, int* __restrict__ hist) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) atomicAdd(&hist[order_bucket(key[idx ? idx[i] : i])], 1);
+}
+// exclusive prefix sum of the class counts in place (one block of 1024 threads, 4 classes each)
+__global__ __launch_bounds__(1024) void order_scan(int* __restrict__ hist) {
+  __shared__ int s[1024];
+  const int t = threadIdx.x;
+  int v[4], sum = 0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    v[j] = hist[4 * t + j];
+    sum += v[j];
+  }
+  s[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int x = t >= off ? s[t - off] : 0;
+    __syncthreads();
+    s[t] += x;
+    __syncthreads();
+  }
+  int run = s[t] - sum;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    hist[4 * t + j] = run;
+    run += v[j];
+  }
+}
+__global__ void order_scatter(long n, const double* __restrict__ key, const int* __restrict__ idx,
+                              int* __restrict__ hist, int* __restrict__ out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int cell = idx ? idx[i] : (int)i;
+  out[atomicAdd(&hist[order_bucket(key[cell])], 1)] = cell;
+}
+
 // Two waves per SIMD: the per-team LDS (LU factors) allows about seven resident waves per CU, and
 // at two waves the register budget costs only a small spill (measured faster than one wave at 400
 // registers, and than three waves).
 __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) void bdf_kernel(Prog P, Opts O, long n, long ld, double* __restrict__ X,
                                                   const double* __restrict__ pres, int* __restrict__ stats,
                                                   int* __restrict__ status, int* __restrict__ counter,
-                                                  unsigned team_bytes, double* __restrict__ jac_scratch) {
+                                                  unsigned team_bytes, double* __restrict__ jac_scratch,
+                                                  const int* __restrict__ order) {
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x & (TEAM - 1);
   const bool h0 = half() == 0;
@@ -987,6 +1035,7 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
     if ((threadIdx.x & (WAVE - 1)) == 0) cell = atomicAdd(counter, 1);
     cell = __builtin_amdgcn_readlane(cell, 0);
     if (cell >= n) break;
+    if (order) cell = order[cell];  // the i-th cell handed out (a permutation of 0..n-1)
     double y = lane < nz ? X[(long)lane * ld + cell] : 0.0;
     const double p = O.mode == 1 ? pres[cell] : 0.0;
     int st[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1164,11 +1213,22 @@ size_t flint_reactor_shared_bytes(const flint_reactor::Prog& P) { return flint_r
 
 hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reactor::Opts& O, long n, long ld,
                                 double* X, const double* pres, int* stats, int* status, int* counter, int teams_per_block,
-                                int blocks, double* jac_scratch, hipStream_t s) {
+                                int blocks, double* jac_scratch, const int* order, hipStream_t s) {
   const unsigned tb = (unsigned)((flint_reactor::team_lds_bytes(P) + 15) & ~size_t(15));
   hipLaunchKernelGGL(flint_reactor::zero_counter, dim3(1), dim3(64), 0, s, counter);
   hipLaunchKernelGGL(flint_reactor::bdf_kernel, dim3(blocks), dim3(teams_per_block * flint_reactor::WAVE),
-                     flint_reactor::shared_prog_bytes(P) + (size_t)tb * teams_per_block, s, P, O, n, ld, X, pres, stats, status, counter, tb, jac_scratch);
+                     flint_reactor::shared_prog_bytes(P) + (size_t)tb * teams_per_block, s, P, O, n, ld, X, pres, stats, status, counter, tb, jac_scratch, order);
+  return hipGetLastError();
+}
+
+hipError_t flint_reactor_order(long n, const double* key, const int* idx, int* out, int* hist, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  const unsigned g = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(flint_reactor::fill_int, dim3(flint_reactor::ORDER_BUCKETS / 256), dim3(256), 0, s, hist,
+                     (long)flint_reactor::ORDER_BUCKETS, 0);
+  hipLaunchKernelGGL(flint_reactor::order_hist, dim3(g), dim3(256), 0, s, n, key, idx, hist);
+  hipLaunchKernelGGL(flint_reactor::order_scan, dim3(1), dim3(1024), 0, s, hist);
+  hipLaunchKernelGGL(flint_reactor::order_scatter, dim3(g), dim3(256), 0, s, n, key, idx, hist, out);
   return hipGetLastError();
 }
 

@@ -46,6 +46,7 @@ def lib():
         L.flint_reactor_set_launch.argtypes = [vp, i, i]
         L.flint_reactor_get_launch.argtypes = [vp, C.POINTER(i), C.POINTER(i)]
         L.flint_reactor_set_newton.argtypes = [vp, i]
+        L.flint_reactor_set_order.argtypes = [vp, i]
         L.flint_cea_setup.argtypes = [vp, vp, vp]
         L.flint_cea_solve.argtypes = [vp, l, l, i, vp, vp, vp, vp, vp, vp, vp, vp, vp]
         L.flint_source_host.argtypes = [vp, l, vp, vp, vp]
@@ -308,6 +309,11 @@ class Mechanism:
     def set_newton(self, policy):
         """Newton policy of the reactor's BDF: 1 as SUNDIALS CVODE (default), 0 as scipy's BDF."""
         check(lib().flint_reactor_set_newton(self._h, policy))
+
+    def set_order(self, policy):
+        """Work order of advance: 0 the caller's order, 1 by descending |dT/dt| at the initial
+        state (flint_reactor_set_order; results are identical)."""
+        check(lib().flint_reactor_set_order(self._h, policy))
 
     def get_launch(self):
         a, b = C.c_int(), C.c_int()

@@ -223,6 +223,16 @@ int flint_reactor_get_launch(flint_mech *m, int *teams_per_block, int *blocks);
  * refactored at every change of step or order, at most 4 iterations). */
 int flint_reactor_set_newton(flint_mech *m, int policy);
 
+/* Work order of flint_reactor_advance and flint_reactor_advance_graph for this handle (no
+ * reference counterpart: FLINT advances one reactor at a time, test-batchF.f90:99-108).  Policy 0
+ * (the default) hands the cells to the persistent kernel's teams in the caller's order; policy 1
+ * evaluates dX/dt at the initial states and hands them out by descending |dT/dt|, so that the
+ * costliest (igniting) cells start first and no late slow cell sets the launch's tail.  Each cell is
+ * integrated independently: results are identical under either policy.  Policy 1 allocates a
+ * right-hand-side buffer and a permutation on first use; flint_reactor_advance_noalloc keeps the
+ * caller's order. */
+int flint_reactor_set_order(flint_mech *m, int policy);
+
 /* Equilibrium data of a mechanism: CEA_initialize_global (Lib_CEA_setup.f90:6-99) for the
  * mechanism's species set.  s = s_tab(Th_min:Th_max, ns) per-mass entropy at 1 bar, column-major
  * like cp/h (read_idealgas_thermo, Load_ThermoTransport.f90:30-85); el_weight[ne] = atomic weights

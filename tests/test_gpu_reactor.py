@@ -210,6 +210,37 @@ def test_reactor_graph_replay_and_autotune(gmech, mechs, cuda):
     assert torch.allclose(outs[0], ref, rtol=0, atol=0) or float((outs[0] - ref).abs().max()) < 1e-6 * float(ref.abs().max())
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_reactor_work_order_changes_no_result(gmech, mechs, cuda, mode):
+    """Work order 1 (cells handed out by descending |dT/dt|) gives the same states, statistics and
+    statuses bit for bit as the caller's order, directly and through the captured graph."""
+    import torch
+    md = mechs["TSR-Rich-31"]
+    m = gmech["TSR-Rich-31"]
+    n = 300
+    st = S.radical_states(md.species, md.W, 406, 0, n, ignition_half=True)
+    X = np.vstack([S.soa(st["rho"] if mode == 0 else st["Y"]), st["T"][None, :]])
+    print(f"[gpu] reactor work order mode={mode}", flush=True)
+    ref = _run(m, X, st["p"], mode, cuda, t_end=2e-5)
+    try:
+        m.set_order(1)
+        got = _run(m, X, st["p"], mode, cuda, t_end=2e-5)
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b)
+        ld = n + 5
+        pd = torch.from_numpy(np.ascontiguousarray(st["p"])).to(cuda)
+        for rep in range(2):
+            Xd = torch.zeros((X.shape[0], ld), dtype=torch.float64, device=cuda)
+            Xd[:, :n] = torch.from_numpy(X).to(cuda)
+            m.advance(Xd, pd, mode=mode, t_end=2e-5, n=n, graph=True)
+            torch.cuda.synchronize()
+            assert np.array_equal(Xd[:, :n].cpu().numpy(), ref[0])
+        with pytest.raises(Exception):
+            m.set_order(2)
+    finally:
+        m.set_order(0)
+
+
 def test_reactor_rejects_bad_arguments(gmech, cuda):
     import torch
     from flint_amd.native import FlintError
