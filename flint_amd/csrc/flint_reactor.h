@@ -7,7 +7,6 @@ namespace flint_reactor {
 // Rate program in device memory (built from the host program a generated mechanism registers).
 struct Prog {
   int ns, nr, nms;
-  const int* wd_ptr;      // [ns+1] net-production terms of species k (in reaction order)
   // The program encoded for the integrator (flint_reactor_host.cpp encode_rhs, reactor.hip):
   int R;                  // reaction rounds: slot 32*round + lane
   const int4* rrec;       // [R*32] {kind | (mset + 1) << 3 (kind 7: empty), column, reaction, factor slots}
@@ -15,7 +14,12 @@ struct Prog {
   int C, CW, MAXP;        // production-sum chunk rounds, chunk width, most chunks of one species
   const int2* cterm;      // [C][CW][32] {reaction (nr: empty), coefficient bits} of chunk (round, lane)
   const int* kpart;       // [MAXP][32] partial-sum slots (32*round + lane) of species k in order, -1 none
-  const int4* jterm;      // [nterm] terms of wd_ptr order: {reaction, coefficient bits, mset, factor slots}
+  const int4* rprod;      // [R*32][2] net-production terms of the slot's reaction: {species (5 bits each),
+                          // count | (dq/dM slot + 1) << 3, coefficient bits x 6}
+  int nmr;                // reactions with a concentration M (the dq/dM slots)
+  int ntb;                // entries of tbe
+  const int4* tbe;        // [ntb] terms whose reaction has M, by species: {species, coefficient bits, mset, dq slot}
+  const int* tbptr;       // [ns+1] first tbe entry of species k
   const double* W;        // [ns]
   const double* tabk;
   int Tk_min, Tk_max, wk;

@@ -71,9 +71,12 @@ static int upload(flint_extra* x, const T* src, size_t count, const T** dst) {
 //  * third-body efficiencies as a dense [ns][nms] table;
 //  * the net production sum of species k split into chunks of at most CW terms in reaction order,
 //    chunks spread over (round, lane) slots, and per species the list of its chunk slots.
+//  * for the Jacobian, each slot's net-production terms (species, coefficient; at most six) and the
+//    index of its dq/dM slot, and per species the list of its terms whose reaction has a
+//    third-body or falloff concentration M.
 static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::vector<int>& rrec,
                       std::vector<float>& effT, std::vector<int>& cterm, std::vector<int>& kpart,
-                      std::vector<int>& jterm) {
+                      std::vector<int>& rprod, std::vector<int>& tbe, std::vector<int>& tbptr) {
   const int nr = hp->nr, ns = hp->ns, nms = hp->nms;
   if (ns > 31) return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports ns <= 31");
   // Arrhenius and three-body reactions (one code path) fill the first wavefront rounds; the
@@ -92,7 +95,6 @@ static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::ve
   (void)n_plain;
   P.R = ((int)order.size() + 31) / 32;
   rrec.assign((size_t)P.R * 32 * 4, 0);
-  std::vector<int> rpack(nr, 0);
   for (int s = 0; s < P.R * 32; ++s) {
     int* rec = &rrec[(size_t)s * 4];
     if (s >= (int)order.size() || order[s] < 0) {
@@ -119,7 +121,6 @@ static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::ve
       for (int f = 0; f < 3; ++f) packed |= (unsigned)slots[f] << (5 * (3 * side + f));
     }
     rec[3] = (int)packed;
-    rpack[r] = (int)packed;
   }
   const int nmsp = std::max(nms, 1);
   effT.assign((size_t)ns * nmsp, 0.0f);
@@ -151,15 +152,48 @@ static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::ve
     }
     slots_of[k].push_back(round * 32 + lane);
   }
-  jterm.assign((size_t)std::max(hp->nterm, 1) * 4, 0);
-  for (int t = 0; t < hp->nterm; ++t) {
-    const int r = hp->wd_rx[t];
-    const float coef = (float)hp->wd_coef[t];
-    jterm[(size_t)t * 4 + 0] = r;
-    std::memcpy(&jterm[(size_t)t * 4 + 1], &coef, sizeof coef);
-    jterm[(size_t)t * 4 + 2] = hp->rx_mset[r];
-    jterm[(size_t)t * 4 + 3] = rpack[r];
+  // dq/dM slots: the reactions with a concentration M, in reaction order
+  std::vector<int> dqslot(nr, -1);
+  P.nmr = 0;
+  for (int r = 0; r < nr; ++r)
+    if (hp->rx_mset[r] >= 0) dqslot[r] = P.nmr++;
+  // per reaction its net-production terms (species order), as the Jacobian scatters them
+  std::vector<std::vector<std::pair<int, float>>> terms_of(nr);
+  for (int k = 0; k < ns; ++k)
+    for (int t = hp->wd_ptr[k]; t < hp->wd_ptr[k + 1]; ++t)
+      terms_of[hp->wd_rx[t]].push_back({k, (float)hp->wd_coef[t]});
+  rprod.assign((size_t)P.R * 32 * 8, 0);
+  for (int s = 0; s < P.R * 32; ++s) {
+    int* pr = &rprod[(size_t)s * 8];
+    if (s >= (int)order.size() || order[s] < 0) continue;  // empty slot: no terms
+    const int r = order[s];
+    const auto& tl = terms_of[r];
+    if (tl.size() > 6) return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports at most 6 species per reaction");
+    unsigned sk = 0;
+    for (size_t i = 0; i < tl.size(); ++i) {
+      sk |= (unsigned)tl[i].first << (5 * i);
+      std::memcpy(&pr[2 + i], &tl[i].second, sizeof(float));
+    }
+    pr[0] = (int)sk;
+    pr[1] = (int)tl.size() | ((dqslot[r] + 1) << 3);
   }
+  // per species its terms with M (species order, reaction order within), for the third-body columns
+  tbptr.assign((size_t)ns + 1, 0);
+  tbe.clear();
+  for (int k = 0; k < ns; ++k) {
+    tbptr[k] = (int)(tbe.size() / 4);
+    for (int t = hp->wd_ptr[k]; t < hp->wd_ptr[k + 1]; ++t) {
+      const int r = hp->wd_rx[t];
+      if (hp->rx_mset[r] < 0) continue;
+      const float coef = (float)hp->wd_coef[t];
+      int cb;
+      std::memcpy(&cb, &coef, sizeof cb);
+      tbe.insert(tbe.end(), {k, cb, hp->rx_mset[r], dqslot[r]});
+    }
+  }
+  tbptr[ns] = (int)(tbe.size() / 4);
+  P.ntb = tbptr[ns];
+  if (tbe.empty()) tbe.assign(4, 0);
   P.MAXP = 1;
   for (int k = 0; k < ns; ++k) P.MAXP = std::max<int>(P.MAXP, (int)slots_of[k].size());
   kpart.assign((size_t)P.MAXP * 32, -1);
@@ -172,8 +206,6 @@ static int build_extra(flint_mech* m) {
   if (m->is_wd || !m->ops || !m->ops->prog)
     return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel needs a generated mechanism (not WD)");
   if (m->ns + 1 > 32) return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports ns + 1 <= 32");
-  if (4 * m->ops->prog->nr > 32 * 33)
-    return flint_fail(FLINT_EUNSUPPORTED, "the reactor kernel supports at most 264 reactions");
   const flint_host_prog* hp = m->ops->prog;
   auto* x = new flint_extra();
   m->extra = x;
@@ -183,15 +215,16 @@ static int build_extra(flint_mech* m) {
   P.nms = hp->nms;
   std::vector<int> rrec, kpart;
   std::vector<float> effT;
-  std::vector<int> cterm, jterm;
-  if (int rc = encode_rhs(hp, P, rrec, effT, cterm, kpart, jterm)) return rc;
+  std::vector<int> cterm, rprod, tbe, tbptr;
+  if (int rc = encode_rhs(hp, P, rrec, effT, cterm, kpart, rprod, tbe, tbptr)) return rc;
   int rc = 0;
   rc |= upload(x, reinterpret_cast<const int4*>(rrec.data()), rrec.size() / 4, &P.rrec);
   rc |= upload(x, effT.data(), effT.size(), &P.effT);
   rc |= upload(x, reinterpret_cast<const int2*>(cterm.data()), cterm.size() / 2, &P.cterm);
   rc |= upload(x, kpart.data(), kpart.size(), &P.kpart);
-  rc |= upload(x, reinterpret_cast<const int4*>(jterm.data()), jterm.size() / 4, &P.jterm);
-  rc |= upload(x, hp->wd_ptr, (size_t)hp->ns + 1, &P.wd_ptr);
+  rc |= upload(x, reinterpret_cast<const int4*>(rprod.data()), rprod.size() / 4, &P.rprod);
+  rc |= upload(x, reinterpret_cast<const int4*>(tbe.data()), tbe.size() / 4, &P.tbe);
+  rc |= upload(x, tbptr.data(), tbptr.size(), &P.tbptr);
   if (rc) return rc;
   P.W = m->d_W;
   P.tabk = m->d_tabk;

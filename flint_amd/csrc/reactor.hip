@@ -18,8 +18,8 @@
 //
 // Layout of one team (lanes l = 0..31 of a half wavefront):
 //   lane k < ns owns species k, lane ns owns T: the state component and the 8-row BDF difference
-//   column D[.][k] live in VGPRs and LDS; row k of the Jacobian sits in a per-team global slot and row k
-//   of the inverse of I - cJ in LDS (stride 33, conflict-free).  The inverse comes from an
+//   column D[.][k] live in VGPRs and LDS; row k of the Jacobian is accumulated in LDS, kept in a
+//   per-team global slot, and row k of the inverse of I - cJ sits in LDS (stride 33, conflict-free).  The inverse comes from an
 //   in-place Gauss-Jordan elimination with partial pivoting (team argmax, pivot row read as an LDS
 //   broadcast), so that each Newton correction is one matrix-vector product instead of two
 //   32-step triangular sweeps.
@@ -138,6 +138,9 @@ struct Lds {
   const int2* cterm;  // [C][CW][32] production-sum chunk terms
   const double* W;    // [32] molar masses (1 beyond ns)
   const double* Rk;   // [32] specific gas constants 8314.51 / W_k (0 beyond ns)
+  const int4* rprod;  // [R*32][2] net-production terms of each slot's reaction (Jacobian scatter)
+  const int4* tbe;    // [ntb] terms with a concentration M, by species
+  const int* tbptr;   // [ns+1]
   // per team
   double* c;    // [32] concentrations; slot 31 (never a species) holds 1 for empty factors
   double* Mv;   // [nms] third-body sums
@@ -149,10 +152,12 @@ struct Lds {
   double* vx;   // [32] its solution
   double* D;    // [8][32] BDF difference array (row j, component k); kept in LDS to spare registers
   double* lu;   // [32][33] inverse of I - cJ (row = lane, padded); Jacobian rows while one is built
+  double* dq;   // [nmr] dq/dM of the reactions with a concentration M (while a Jacobian is built)
 };
 
 __host__ __device__ inline size_t team_lds_bytes(const Prog& P) {
-  return (size_t)(32 * 5 + 8 * 32 + (P.nms > 0 ? P.nms : 1) + (size_t)P.nr + 1 + (size_t)P.C * 32 + 32 * 33) *
+  return (size_t)(32 * 5 + 8 * 32 + (P.nms > 0 ? P.nms : 1) + (size_t)P.nr + 1 + (size_t)P.C * 32 + 32 * 33 +
+                  (P.nmr > 0 ? P.nmr : 1)) *
          sizeof(double);
 }
 
@@ -160,7 +165,7 @@ __host__ __device__ inline size_t team_lds_bytes(const Prog& P) {
 // masses and gas constants.  Everything the right-hand side reads besides the state and the
 // tables sits here, so that an evaluation waits on global memory only for its table rows.
 struct SharedLayout {
-  size_t eff, kp, ct, w, total;
+  size_t eff, kp, ct, w, rp, tb, tbp, total;
 };
 __host__ __device__ inline SharedLayout shared_layout(const Prog& P) {
   SharedLayout o;
@@ -168,7 +173,10 @@ __host__ __device__ inline SharedLayout shared_layout(const Prog& P) {
   o.kp = o.eff + ((((size_t)P.ns * (P.nms > 0 ? P.nms : 1) * 4) + 15) & ~size_t(15));
   o.ct = o.kp + ((((size_t)P.MAXP * 32 * 4) + 15) & ~size_t(15));
   o.w = o.ct + (size_t)P.C * CW * TEAM * 8;
-  o.total = o.w + 2 * TEAM * 8;
+  o.rp = o.w + 2 * TEAM * 8;
+  o.tb = o.rp + (size_t)P.R * 32 * 32;
+  o.tbp = o.tb + (size_t)(P.ntb > 0 ? P.ntb : 1) * 16;
+  o.total = o.tbp + ((((size_t)P.ns + 1) * 4 + 15) & ~size_t(15));
   return o;
 }
 __host__ __device__ inline size_t shared_prog_bytes(const Prog& P) { return shared_layout(P).total; }
@@ -190,6 +198,12 @@ __device__ inline void load_shared_prog(const Prog& P, char* base) {
     w[i] = sp ? P.W[i] : 1.0;
     w[TEAM + i] = sp ? 8314.51 / P.W[i] : 0.0;
   }
+  int4* rp = reinterpret_cast<int4*>(base + o.rp);
+  for (int i = threadIdx.x; i < P.R * 32 * 2; i += blockDim.x) rp[i] = P.rprod[i];
+  int4* tb = reinterpret_cast<int4*>(base + o.tb);
+  for (int i = threadIdx.x; i < (P.ntb > 0 ? P.ntb : 1); i += blockDim.x) tb[i] = P.tbe[i];
+  int* tbp = reinterpret_cast<int*>(base + o.tbp);
+  for (int i = threadIdx.x; i <= P.ns; i += blockDim.x) tbp[i] = P.tbptr[i];
 }
 
 __device__ inline Lds carve(char* shared, char* base, const Prog& P) {
@@ -201,6 +215,9 @@ __device__ inline Lds carve(char* shared, char* base, const Prog& P) {
   L.cterm = reinterpret_cast<const int2*>(shared + o.ct);
   L.W = reinterpret_cast<const double*>(shared + o.w);
   L.Rk = L.W + TEAM;
+  L.rprod = reinterpret_cast<const int4*>(shared + o.rp);
+  L.tbe = reinterpret_cast<const int4*>(shared + o.tb);
+  L.tbptr = reinterpret_cast<const int*>(shared + o.tbp);
   double* d = reinterpret_cast<double*>(base);
   L.c = d; d += 32;
   L.aux = d; d += 32;
@@ -211,7 +228,8 @@ __device__ inline Lds carve(char* shared, char* base, const Prog& P) {
   L.Mv = d; d += (P.nms > 0 ? P.nms : 1);
   L.q = d; d += P.nr + 1;
   L.part = d; d += P.C * 32;
-  L.lu = d;
+  L.lu = d; d += 32 * 33;
+  L.dq = d;
   return L;
 }
 
@@ -223,11 +241,18 @@ struct RhsOut {
   double Rmix, rho_cv, cpm; // team-uniform sums
 };
 
+// No-return add to a Jacobian entry in the team's LDS (ds_add_f64).
+__device__ __forceinline__ void jadd(double* p, double v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+}
+
 // Rates of progress of the reaction in record rec (kinds: 0 Arrhenius, 1 three-body, 2 Troe,
-// 3 Lindemann), Lib_Chemistry_falloff.f90:12-80 for the falloff forms.
+// 3 Lindemann), Lib_Chemistry_falloff.f90:12-80 for the falloff forms.  With JAC the reaction's
+// derivatives are scattered into the Jacobian rows in L.lu (zeroed by the caller): slot is the
+// record's index, which also indexes its net-production terms L.rprod.
 template <bool JAC>
 __device__ __forceinline__ void reaction(const int4 rec, const double* __restrict__ rk0, const double* __restrict__ rk1,
-                                         double Td, const Lds& L, double* __restrict__ jx, int nr) {
+                                         double Td, const Lds& L, int slot) {
   const int kind = rec.x & 7;
   const int ms = (rec.x >> 3) - 1;
   const int col = rec.y;
@@ -294,18 +319,43 @@ __device__ __forceinline__ void reaction(const int4 rec, const double* __restric
   }
   L.q[r] = k1 * Pf - k2 * Pb;
   if (JAC) {
-    jx[r] = k1;
-    jx[nr + r] = k2;
-    jx[2 * nr + r] = dqM;
-    jx[3 * nr + r] = dk1T * Pf - dk2T * Pb;
+    // dq/dc at each factor slot (a repeated species gets one term per slot, the power rule; slot
+    // 31 is the constant 1 and has no column), times each net-production coefficient nu_kr, added
+    // to row k.  Several reactions of one round may add to one entry, hence the atomic adds.
+    // dq/dT goes to the row's spare column 32, dq/dM to the reaction's slot in L.dq.
+    const int a0 = fs & 31, a1 = (fs >> 5) & 31, a2 = (fs >> 10) & 31;
+    const int b0 = (fs >> 15) & 31, b1 = (fs >> 20) & 31, b2 = (fs >> 25) & 31;
+    const double ca0 = L.c[a0], ca1 = L.c[a1], ca2 = L.c[a2];
+    const double cb0 = L.c[b0], cb1 = L.c[b1], cb2 = L.c[b2];
+    const double da0 = k1 * (ca1 * ca2), da1 = k1 * (ca0 * ca2), da2 = k1 * (ca0 * ca1);
+    const double db0 = k2 * (cb1 * cb2), db1 = k2 * (cb0 * cb2), db2 = k2 * (cb0 * cb1);
+    const double dqT = dk1T * Pf - dk2T * Pb;
+    const int4 p0 = L.rprod[2 * slot], p1 = L.rprod[2 * slot + 1];
+    const int cnt = p0.y & 7, dqs = (p0.y >> 3) - 1;
+    if (dqs >= 0) L.dq[dqs] = dqM;
+    const int cbits[6] = {p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+      if (t < cnt) {
+        const int k = ((unsigned)p0.x >> (5 * t)) & 31;
+        const double nu = (double)__int_as_float(cbits[t]);
+        double* row = L.lu + k * 33;
+        if (a0 != 31) jadd(row + a0, nu * da0);
+        if (a1 != 31) jadd(row + a1, nu * da1);
+        if (a2 != 31) jadd(row + a2, nu * da2);
+        if (b0 != 31) jadd(row + b0, -(nu * db0));
+        if (b1 != 31) jadd(row + b1, -(nu * db1));
+        if (b2 != 31) jadd(row + b2, -(nu * db2));
+        jadd(row + 32, nu * dqT);
+      }
+    }
   }
 }
 
 // Team right-hand side; o.ok is false (team-uniform) when T leaves the tables.  With jac the
-// per-reaction values the Jacobian needs are staged in LDS as well.  Kept out of line: it is called
-// from five places of the integrator and inlining every copy would multiply its register demand.
-__device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, double x, const Lds& L, bool jac,
-                                          double* __restrict__ jx) {
+// reactions' derivatives are accumulated in the Jacobian rows in LDS as well (team_jac finishes
+// them); the rows replace the inverse of I - cJ there, which a new Jacobian always renews.
+__device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, double x, const Lds& L, bool jac) {
   RhsOut o;
   o.ok = false;
   const int lane = threadIdx.x & (TEAM - 1);
@@ -332,6 +382,10 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   }
   L.c[lane] = sp ? roi / Wk : (lane == TEAM - 1 ? 1.0 : 0.0);
   if (lane == 0) L.q[P.nr] = 0.0;
+  if (jac) {
+    const int wl0 = threadIdx.x & (WAVE - 1);
+    for (int i = wl0; i < 32 * 33; i += WAVE) L.lu[i] = 0.0;
+  }
   const int T_i = (int)T;
   const double Td = T - (double)T_i;
   const double* __restrict__ rk0 = P.tabk + (long)(T_i - P.Tk_min) * P.wk;
@@ -379,8 +433,8 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
     const int nxt = (i + 1) * WAVE + wl;
     rec = nxt < nslot ? L.rrec[nxt] : make_int4(7, 0, 0, 0);
     if ((cur.x & 7) != 7) {
-      if (jac) reaction<true>(cur, rk0, rk1, Td, L, jx, P.nr);
-      else reaction<false>(cur, rk0, rk1, Td, L, jx, P.nr);
+      if (jac) reaction<true>(cur, rk0, rk1, Td, L, i * WAVE + wl);
+      else reaction<false>(cur, rk0, rk1, Td, L, 0);
     }
   }
   tsync();
@@ -436,60 +490,48 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   return o;
 }
 
-// Jacobian of the team's cell, after team_rhs with jac (which left the per-reaction effective
-// coefficients k_f, k_b, dq/dM and dq/dT in the team's global slot Jg: [4][nr]).  Row k is
-// accumulated by lane k in LDS (the inverse's area: a new Jacobian always gets a new inverse) from
-// the species' production terms in reaction order, then stored to Jg (row k = lane k, stride 33).
+// Jacobian of the team's cell, after team_rhs with jac (which left the reactions' dq/dc terms,
+// times nu_kr, in row k of L.lu, their dq/dT sums in column 32, and dq/dM of the reactions with a
+// concentration M in L.dq).  The third-body columns are added here: row k gets
+// sum over its terms with M of nu_kr dq/dM eff_j, column j by lane j, rows 2i and 2i + 1 by the two
+// halves.  Then row k (lane k) is scaled and completed, the temperature row formed from column
+// sums, and the rows stored to the team's global slot Jg (row k = lane k, stride 33).
 __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, const Lds& L, const RhsOut o,
                                          double* __restrict__ Jg) {
   const int lane = threadIdx.x & (TEAM - 1);
-  const int ns = P.ns, nz = ns + 1, nr = P.nr;
+  const int ns = P.ns, nz = ns + 1;
   const bool sp = lane < ns;
   const double T = tbc(x, ns);
   const double FT = tbc(o.F, ns);
-  // rows are built by the lower half (the upper half joins the team sums with identical values)
+  // rows are completed by the lower half (the upper half joins the team sums with identical values)
   const bool h0 = half() == 0;
   double* row = L.lu + lane * 33;
-  if (h0) {
-#pragma unroll
-    for (int j = 0; j < TEAM; ++j) row[j] = 0.0;
-  }
   L.g[lane] = o.g;
   if (mode == 1) L.aux[lane] = o.xc;
-  // the reaction coefficients in Jg were stored by other lanes of this wavefront
-  __threadfence_block();
+  tsync();
+  {
+    const int h = half();
+#pragma unroll 1
+    for (int i = 0; 2 * i < ns; ++i) {
+      const int k = 2 * i + h;
+      const int e0 = k < ns ? L.tbptr[k] : 0, e1 = k < ns ? L.tbptr[k + 1] : 0;
+      if (e1 > e0) {
+        double acc = 0.0;
+#pragma unroll 2
+        for (int e = e0; e < e1; ++e) {
+          const int4 te = L.tbe[e];
+          const double Dv = (double)__int_as_float(te.y) * L.dq[te.w];
+          if (sp) acc += Dv * (double)L.effT[lane * P.nms + te.z];
+        }
+        if (sp) L.lu[k * 33 + lane] += acc;
+      }
+    }
+  }
   tsync();
   double bT = 0.0;
   const double Wk = L.W[lane];
   if (sp && h0) {
-    const int t0 = P.wd_ptr[lane], t1 = P.wd_ptr[lane + 1];
-#pragma unroll 2
-    for (int t = t0; t < t1; ++t) {
-      const int4 tr = P.jterm[t];
-      const int r = tr.x;
-      const double coef = (double)__int_as_float(tr.y);
-      const int ms = tr.z;
-      const unsigned fs = (unsigned)tr.w;
-      const double kf = Jg[r], kb = Jg[nr + r];
-      // d(k c_a c_b c_c)/dc at each factor slot; a repeated species gets one term per slot
-      // (the power rule), slot 31 is the constant 1 and has no column
-      const int a0 = fs & 31, a1 = (fs >> 5) & 31, a2 = (fs >> 10) & 31;
-      const int b0 = (fs >> 15) & 31, b1 = (fs >> 20) & 31, b2 = (fs >> 25) & 31;
-      const double ca0 = L.c[a0], ca1 = L.c[a1], ca2 = L.c[a2];
-      const double cb0 = L.c[b0], cb1 = L.c[b1], cb2 = L.c[b2];
-      const double f = coef * kf, g = coef * kb;
-      if (a0 != 31) row[a0] += f * (ca1 * ca2);
-      if (a1 != 31) row[a1] += f * (ca0 * ca2);
-      if (a2 != 31) row[a2] += f * (ca0 * ca1);
-      if (b0 != 31) row[b0] -= g * (cb1 * cb2);
-      if (b1 != 31) row[b1] -= g * (cb0 * cb2);
-      if (b2 != 31) row[b2] -= g * (cb0 * cb1);
-      if (ms >= 0) {
-        const double Dv = coef * Jg[2 * nr + r];
-        for (int j = 0; j < ns; ++j) row[j] += Dv * (double)L.effT[j * P.nms + ms];
-      }
-      bT += coef * Jg[3 * nr + r];
-    }
+    bT = row[32];
     // species block A_kj = d wdot_k / d rho_j (unclipped), b_k = d wdot_k / dT at fixed rho
     for (int j = 0; j < ns; ++j) row[j] *= Wk / L.W[j];
   }
@@ -544,9 +586,7 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
     if (lane < nz) L.lu[ns * 33 + lane] = jt;
     tsync();
   }
-  // the rows go to the global slot, which the inverse reads row by row (same lane); every lane's
-  // loads of the reaction coefficients there have completed first
-  __threadfence_block();
+  // the rows go to the global slot, where the inverse reads them row by row (the same lane)
   if (h0) {
     double* __restrict__ jrow = Jg + lane * 33;
 #pragma unroll
@@ -634,6 +674,82 @@ __device__ __forceinline__ double team_solve(int mystep, double b, int nz, const
   const double x = L.vx[lane];
   tsync();
   return x;
+}
+
+// Two solves, (I - c1 J) x1 = b1 and (I - c2 J) x2 = b2, by Gauss-Jordan elimination of the
+// augmented systems with partial pivoting: the same steps, register layout and rotation as
+// team_inv, with each row's right-hand side carried along (both halves hold it) and no inverse
+// kept.  The two eliminations are independent and interleaved step by step, so that each step's
+// cross-lane latencies (argmax, pivot-row publication) are paid once for both.  Row p_k, the pivot
+// of step k, ends as x_k = beta_{p_k}.  The pivot rows are published in L.vx and L.vb, their
+// right-hand sides in L.g[0..1].  For lanes < nz, x1 and x2 receive this lane's components.
+__device__ __forceinline__ void team_solve2(double c1, double b1, double c2, double b2, int nz, const Lds& L,
+                                            const double* __restrict__ Jg, double& x1, double& x2) {
+  constexpr int HC = TEAM / 2;
+  const int lane = threadIdx.x & (TEAM - 1);
+  const int h = half(), c0 = HC * h;
+  const double* __restrict__ jrow = Jg + lane * 33;
+  double r[HC], q[HC];
+#pragma unroll
+  for (int j = 0; j < HC; ++j) {
+    const double jv = (lane < nz && c0 + j < nz) ? jrow[c0 + j] : 0.0;
+    const double id = lane == c0 + j ? 1.0 : 0.0;
+    r[j] = id - c1 * jv;
+    q[j] = id - c2 * jv;
+  }
+  double be1 = lane < nz ? b1 : 0.0, be2 = lane < nz ? b2 : 0.0;
+  double* __restrict__ pb1 = L.vx;
+  double* __restrict__ pb2 = L.vb;
+  int my1 = -1, my2 = -1;
+#pragma unroll 1
+  for (int k = 0; k < nz; ++k) {
+    const int hk = k / HC;
+    const double2 hv1 = halves(r[0]), hv2 = halves(q[0]);
+    const double rk1 = hk == 0 ? hv1.x : hv1.y, rk2 = hk == 0 ? hv2.x : hv2.y;
+    double best1 = my1 < 0 ? fabs(rk1) : -1.0, best2 = my2 < 0 ? fabs(rk2) : -1.0;
+    int bl1 = lane, bl2 = lane;
+    targmax(best1, bl1);
+    targmax(best2, bl2);
+    const bool me1 = lane == bl1, me2 = lane == bl2;
+    if (me1) {
+      my1 = k;
+#pragma unroll
+      for (int j = 0; j < HC; ++j) pb1[c0 + j] = r[j];
+      if (h == 0) L.g[0] = be1;
+    }
+    if (me2) {
+      my2 = k;
+#pragma unroll
+      for (int j = 0; j < HC; ++j) pb2[c0 + j] = q[j];
+      if (h == 0) L.g[1] = be2;
+    }
+    tsync();
+    const double inv1 = 1.0 / pb1[HC * hk], inv2 = 1.0 / pb2[HC * hk];
+    const double pbe1 = L.g[0], pbe2 = L.g[1];
+    const double keep1 = me1 ? 0.0 : 1.0, fm1 = me1 ? -1.0 : rk1;
+    const double keep2 = me2 ? 0.0 : 1.0, fm2 = me2 ? -1.0 : rk2;
+    be1 = keep1 * be1 - fm1 * (pbe1 * inv1);
+    be2 = keep2 * be2 - fm2 * (pbe2 * inv2);
+    const double nk1 = me1 ? inv1 : -rk1 * inv1, nk2 = me2 ? inv2 : -rk2 * inv2;
+    const double r0 = h == hk ? nk1 : keep1 * r[0] - fm1 * (pb1[c0] * inv1);
+    const double q0 = h == hk ? nk2 : keep2 * q[0] - fm2 * (pb2[c0] * inv2);
+#pragma unroll
+    for (int j = 0; j < HC - 1; ++j) {
+      r[j] = keep1 * r[j + 1] - fm1 * (pb1[c0 + j + 1] * inv1);
+      q[j] = keep2 * q[j + 1] - fm2 * (pb2[c0 + j + 1] * inv2);
+    }
+    r[HC - 1] = r0;
+    q[HC - 1] = q0;
+    tsync();
+  }
+  if (h == 0 && lane < nz) {  // (a row that never pivoted, only in a singular system, keeps its slot)
+    pb1[my1 < 0 ? lane : my1] = be1;
+    pb2[my2 < 0 ? lane : my2] = be2;
+  }
+  tsync();
+  x1 = pb1[lane];
+  x2 = pb2[lane];
+  tsync();
 }
 
 __device__ __forceinline__ double team_rms(double v, double scale, int nz) {
@@ -728,7 +844,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
   double xe = y;
   bool wj = false;
   while (true) {
-    const RhsOut o = team_rhs(P, O.mode, p, xe, L, wj, Jg);
+    const RhsOut o = team_rhs(P, O.mode, p, xe, L, wj);
     if (wj) {
       if (o.ok) {
         team_jac(P, O.mode, xe, L, o, Jg);
@@ -1074,7 +1190,7 @@ __global__ __launch_bounds__(512) void jac_kernel(Prog P, int mode, long n, long
   for (long cell = (long)blockIdx.x * tpb + team; cell < n; cell += stride) {
     const double y = lane < nz ? X[(long)lane * ld + cell] : 0.0;
     const double p = mode == 1 ? pres[cell] : 0.0;
-    const RhsOut o = team_rhs(P, mode, p, y, L, JAC, Jg);
+    const RhsOut o = team_rhs(P, mode, p, y, L, JAC);
     if (!o.ok) {
       if (h0 && lane < nz) {
         F[(long)lane * ld + cell] = -1.0;
@@ -1133,19 +1249,17 @@ __global__ __launch_bounds__(512) void verify_kernel(Prog P, long n, long ld, do
     if (pass) {
       const double y0 = lane < nz ? X[(long)lane * ld + cell] : 0.0;
       const double yp = lane < nz ? Xp[(long)lane * ldp + cell] : 0.0;
-      const RhsOut o = team_rhs(P, 1, pres[cell], yp, L, true, Jg);
+      const RhsOut o = team_rhs(P, 1, pres[cell], yp, L, true);
       pass = o.ok;
       if (pass) {
         team_jac(P, 1, yp, L, o, Jg);
         const double G = lane < nz ? yp - y0 - dt * o.F : 0.0;
-        const int ms = team_inv(dt, nz, L, Jg);
-        double d = fabs(team_solve(ms, -G, nz, L));
-        if (F0) {
-          const double f0 = lane < nz ? F0[(long)lane * ldf + cell] : 0.0;
-          const double G2 = lane < nz ? yp - y0 - 0.5 * dt * (f0 + o.F) : 0.0;
-          const int ms2 = team_inv(0.5 * dt, nz, L, Jg);
-          d = fmax(d, fabs(team_solve(ms2, -G2, nz, L)));
-        }
+        const double f0 = (F0 && lane < nz) ? F0[(long)lane * ldf + cell] : 0.0;
+        const double G2 = lane < nz ? yp - y0 - 0.5 * dt * (f0 + o.F) : 0.0;
+        double d1, d2;
+        team_solve2(dt, -G, 0.5 * dt, -G2, nz, L, Jg, d1, d2);
+        double d = fabs(d1);
+        if (F0) d = fmax(d, fabs(d2));
         double ey = lane < P.ns ? d : 0.0, et = lane == P.ns ? d : 0.0;
         const double bad = tsum((lane < nz && !isfinite(d)) ? 1.0 : 0.0);
         int l1 = lane, l2 = lane;

@@ -115,10 +115,11 @@ __global__ __launch_bounds__(128) void apply_kernel(SurProg P, long n, long ld, 
   bool fall = !(fabs(dT) <= P.max_dT) || !(res <= P.max_res);
   double T = T0 + dT;
   // weighted least-norm correction onto C Y = C Y0 (ne element rows and the mass row), applied
-  // twice: the second pass removes what the first left of the residual (the normal matrix is
-  // ill-conditioned when an element is nearly absent)
+  // three times: each pass removes what the previous one left of the residual (the normal matrix
+  // is ill-conditioned when an element is nearly absent, and two passes left up to 1e-12 of total
+  // mass)
 #pragma unroll 1
-  for (int pass = 0; pass < 2 && !fall; ++pass) {
+  for (int pass = 0; pass < 3 && !fall; ++pass) {
     if (pass > 0) {
 #pragma unroll
       for (int e = 0; e < MAXE; ++e) {

@@ -167,6 +167,27 @@ def test_check_step_judges_candidate_states(setup, cuda):
     assert (f == 0).mean() > 0.8
     assert np.all((c[0] <= 5e-5) & (c[1] <= 0.5) == (f == 0))
     assert torch.equal(X, X0) and torch.equal(Xt, Xt0)
+    # the corrections themselves, against numpy's solve with F and J of the same reactor model
+    # (flint_reactor_jacobian) on 256 cells
+    m = 256
+    nz = ns + 1
+    F0 = torch.empty((nz, n), dtype=torch.float64, device=cuda)
+    Fp = torch.empty_like(F0)
+    Jp = torch.empty((nz * nz, n), dtype=torch.float64, device=cuda)
+    mech.reactor_jacobian(X, F0, None, p=p, mode=1)
+    mech.reactor_jacobian(Xt, Fp, Jp, p=p, mode=1)
+    torch.cuda.synchronize()
+    x0, xt = X0.cpu().numpy()[:, :m], Xt0.cpu().numpy()[:, :m]
+    f0, fp, jp = F0.cpu().numpy()[:, :m], Fp.cpu().numpy()[:, :m], Jp.cpu().numpy()[:, :m]
+    dt = sur.dt
+    for i in range(m):
+        Ji = jp[:, i].reshape(nz, nz)
+        dbe = -np.linalg.solve(np.eye(nz) - dt * Ji, xt[:, i] - x0[:, i] - dt * fp[:, i])
+        dtr = -np.linalg.solve(np.eye(nz) - 0.5 * dt * Ji, xt[:, i] - x0[:, i] - 0.5 * dt * (f0[:, i] + fp[:, i]))
+        ey = max(np.abs(dbe[:ns]).max(), np.abs(dtr[:ns]).max())
+        et = max(abs(dbe[ns]), abs(dtr[ns]))
+        assert abs(c[0, i] - ey) <= 1e-9 * ey + 1e-16
+        assert abs(c[1, i] - et) <= 1e-9 * et + 1e-12
     bad = Xt.clone()
     bad[3] += 1e-3
     mech.check_step(X, bad, p, sur.dt, flag, 5e-5, 0.5, check_out=chk)
