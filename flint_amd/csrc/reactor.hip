@@ -107,6 +107,23 @@ __device__ __forceinline__ void targmax(double& best, int& bl) {
 #undef FLINT_ARGMAX_LEVEL
 }
 
+// Pivot search of the eliminations: the team lane with the largest |v| among the lanes with
+// eligible set, as one 32-bit unsigned maximum.  The key keeps |v|'s exponent and its 15 leading
+// mantissa bits (v's high word with the low five bits cleared) above 32 - lane, so that values
+// within a factor 1 + 2^-15 of each other count as equal and go to the lower lane; an ineligible
+// lane has key 0.  Every lane ends with the same key, and the decoded lane.
+__device__ __forceinline__ int tpivot(double v, bool eligible) {
+  const int lane = threadIdx.x & 31;
+  const unsigned hi = (unsigned)__double2hiint(v) & 0x7fffffffu;
+  unsigned key = eligible ? (hi & ~31u) + (unsigned)(32 - lane) : 0u;
+  key = max(key, (unsigned)dppi<DPP_X1>((int)key));
+  key = max(key, (unsigned)dppi<DPP_X2>((int)key));
+  key = max(key, (unsigned)dppi<DPP_HMIR>((int)key));
+  key = max(key, (unsigned)dppi<DPP_MIR>((int)key));
+  key = max(key, (unsigned)__builtin_amdgcn_ds_swizzle((int)key, 0x401F));
+  return 32 - (int)(key & 31u) - ((key & 31u) == 0u ? 32 : 0);
+}
+
 // Both halves of the wavefront hold the same component values (component lane k = lane k and
 // lane 32 + k) and run the same control flow; they split the reaction, chunk, elimination and
 // solve work.  half(): which half this lane is in.
@@ -597,8 +614,9 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
 
 // Inverse of A = I - c J by Gauss-Jordan elimination with partial pivoting and no row exchanges.
 // Row i lives in registers of component lane i: columns 0..15 in the lower half, 16..31 in the
-// upper half.  Step k takes the unused row p with the largest |a_pk| (team argmax; ties to the
-// lower lane), row p is published in LDS and
+// upper half.  Step k takes the unused row p with the largest |a_pk| (tpivot: ties, and values
+// within 2^-15 of each other, to the lower lane), row p is published in LDS already scaled by
+// 1 / a_pk (its column-k slot carrying 1 / a_pk itself), and
 //   row p:        a_pj <- a_pj / a_pk (j != k),   a_pk <- 1 / a_pk
 //   rows i != p:  a_ij <- a_ij - a_ik a_pj (j != k, a_pj scaled),   a_ik <- -a_ik / a_pk
 // (the column-k storage takes over the identity column p, as in the classic in-place scheme).
@@ -624,24 +642,24 @@ __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const do
     const int hk = k / HC;
     const double2 hv = halves(r[0]);  // column k is register 0 of half hk
     const double rk = hk == 0 ? hv.x : hv.y;
-    double best = mystep < 0 ? fabs(rk) : -1.0;
-    int bl = lane;
-    targmax(best, bl);
-    const bool me = lane == bl;
+    double iv = 1.0 / rk;  // used by the pivot row only; computed by all while the search runs
+    asm volatile("" : "+v"(iv));  // (kept ahead of the search rather than sunk into the pivot branch)
+    const bool me = lane == tpivot(rk, mystep < 0);
     if (me) {
       mystep = k;
+      pb[c0] = h == hk ? iv : r[0] * iv;
 #pragma unroll
-      for (int j = 0; j < HC; ++j) pb[c0 + j] = r[j];
+      for (int j = 1; j < HC; ++j) pb[c0 + j] = r[j] * iv;
     }
     tsync();
-    const double inv = 1.0 / pb[HC * hk];
-    // pivot lane: a_pj * inv; other lanes: a_ij - a_ik (a_pj * inv); one fused form for both,
+    const double inv = pb[HC * hk];
+    // pivot lane: a_pj / a_pk; other lanes: a_ij - a_ik (a_pj / a_pk); one fused form for both,
     // rotating by one register on the way
     const double keep = me ? 0.0 : 1.0, fm = me ? -1.0 : rk;
     const double nk = me ? inv : -rk * inv;
-    const double r0 = h == hk ? nk : keep * r[0] - fm * (pb[c0] * inv);
+    const double r0 = h == hk ? nk : keep * r[0] - fm * pb[c0];
 #pragma unroll
-    for (int j = 0; j < HC - 1; ++j) r[j] = keep * r[j + 1] - fm * (pb[c0 + j + 1] * inv);
+    for (int j = 0; j < HC - 1; ++j) r[j] = keep * r[j + 1] - fm * pb[c0 + j + 1];
     r[HC - 1] = r0;
     tsync();
   }
@@ -679,10 +697,13 @@ __device__ __forceinline__ double team_solve(int mystep, double b, int nz, const
 // Two solves, (I - c1 J) x1 = b1 and (I - c2 J) x2 = b2, by Gauss-Jordan elimination of the
 // augmented systems with partial pivoting: the same steps, register layout and rotation as
 // team_inv, with each row's right-hand side carried along (both halves hold it) and no inverse
-// kept.  The two eliminations are independent and interleaved step by step, so that each step's
-// cross-lane latencies (argmax, pivot-row publication) are paid once for both.  Row p_k, the pivot
-// of step k, ends as x_k = beta_{p_k}.  The pivot rows are published in L.vx and L.vb, their
-// right-hand sides in L.g[0..1].  For lanes < nz, x1 and x2 receive this lane's components.
+// kept.  The pivot row is left unscaled (it stands for the classic scaled row times a_pk, a factor
+// the later steps carry along linearly and the end divides out), which spares one product per
+// element and step.  The two eliminations are independent and interleaved step by step, so that
+// each step's cross-lane latencies (pivot search, pivot-row publication) are paid once for both.
+// Row p_k, the pivot of step k, ends as x_k = beta_{p_k} / a_{p_k k}.  The pivot rows are
+// published in L.vx and L.vb, their right-hand sides in L.g[0..1].  For lanes < nz, x1 and x2
+// receive this lane's components.
 __device__ __forceinline__ void team_solve2(double c1, double b1, double c2, double b2, int nz, const Lds& L,
                                             const double* __restrict__ Jg, double& x1, double& x2) {
   constexpr int HC = TEAM / 2;
@@ -701,50 +722,49 @@ __device__ __forceinline__ void team_solve2(double c1, double b1, double c2, dou
   double* __restrict__ pb1 = L.vx;
   double* __restrict__ pb2 = L.vb;
   int my1 = -1, my2 = -1;
+  double myiv1 = 1.0, myiv2 = 1.0;
 #pragma unroll 1
   for (int k = 0; k < nz; ++k) {
     const int hk = k / HC;
     const double2 hv1 = halves(r[0]), hv2 = halves(q[0]);
     const double rk1 = hk == 0 ? hv1.x : hv1.y, rk2 = hk == 0 ? hv2.x : hv2.y;
-    double best1 = my1 < 0 ? fabs(rk1) : -1.0, best2 = my2 < 0 ? fabs(rk2) : -1.0;
-    int bl1 = lane, bl2 = lane;
-    targmax(best1, bl1);
-    targmax(best2, bl2);
-    const bool me1 = lane == bl1, me2 = lane == bl2;
+    double iv1 = 1.0 / rk1, iv2 = 1.0 / rk2;
+    asm volatile("" : "+v"(iv1), "+v"(iv2));
+    const bool me1 = lane == tpivot(rk1, my1 < 0), me2 = lane == tpivot(rk2, my2 < 0);
     if (me1) {
       my1 = k;
+      myiv1 = iv1;
+      pb1[c0] = h == hk ? iv1 : r[0];
 #pragma unroll
-      for (int j = 0; j < HC; ++j) pb1[c0 + j] = r[j];
+      for (int j = 1; j < HC; ++j) pb1[c0 + j] = r[j];
       if (h == 0) L.g[0] = be1;
     }
     if (me2) {
       my2 = k;
+      myiv2 = iv2;
+      pb2[c0] = h == hk ? iv2 : q[0];
 #pragma unroll
-      for (int j = 0; j < HC; ++j) pb2[c0 + j] = q[j];
+      for (int j = 1; j < HC; ++j) pb2[c0 + j] = q[j];
       if (h == 0) L.g[1] = be2;
     }
     tsync();
-    const double inv1 = 1.0 / pb1[HC * hk], inv2 = 1.0 / pb2[HC * hk];
-    const double pbe1 = L.g[0], pbe2 = L.g[1];
-    const double keep1 = me1 ? 0.0 : 1.0, fm1 = me1 ? -1.0 : rk1;
-    const double keep2 = me2 ? 0.0 : 1.0, fm2 = me2 ? -1.0 : rk2;
-    be1 = keep1 * be1 - fm1 * (pbe1 * inv1);
-    be2 = keep2 * be2 - fm2 * (pbe2 * inv2);
-    const double nk1 = me1 ? inv1 : -rk1 * inv1, nk2 = me2 ? inv2 : -rk2 * inv2;
-    const double r0 = h == hk ? nk1 : keep1 * r[0] - fm1 * (pb1[c0] * inv1);
-    const double q0 = h == hk ? nk2 : keep2 * q[0] - fm2 * (pb2[c0] * inv2);
+    const double f1 = me1 ? 0.0 : rk1 * pb1[HC * hk], f2 = me2 ? 0.0 : rk2 * pb2[HC * hk];
+    be1 = be1 - f1 * L.g[0];
+    be2 = be2 - f2 * L.g[1];
+    const double r0 = h == hk ? (me1 ? 1.0 : -f1) : r[0] - f1 * pb1[c0];
+    const double q0 = h == hk ? (me2 ? 1.0 : -f2) : q[0] - f2 * pb2[c0];
 #pragma unroll
     for (int j = 0; j < HC - 1; ++j) {
-      r[j] = keep1 * r[j + 1] - fm1 * (pb1[c0 + j + 1] * inv1);
-      q[j] = keep2 * q[j + 1] - fm2 * (pb2[c0 + j + 1] * inv2);
+      r[j] = r[j + 1] - f1 * pb1[c0 + j + 1];
+      q[j] = q[j + 1] - f2 * pb2[c0 + j + 1];
     }
     r[HC - 1] = r0;
     q[HC - 1] = q0;
     tsync();
   }
   if (h == 0 && lane < nz) {  // (a row that never pivoted, only in a singular system, keeps its slot)
-    pb1[my1 < 0 ? lane : my1] = be1;
-    pb2[my2 < 0 ? lane : my2] = be2;
+    pb1[my1 < 0 ? lane : my1] = be1 * myiv1;
+    pb2[my2 < 0 ? lane : my2] = be2 * myiv2;
   }
   tsync();
   x1 = pb1[lane];
@@ -1091,9 +1111,23 @@ __device__ __forceinline__ int order_bucket(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(v));  // sign bit 0
   return ORDER_BUCKETS - 1 - (int)(b >> 51);
 }
-__global__ void order_hist(long n, const double* __restrict__ key, const int* __restrict__ idx, int* __restrict__ hist) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) atomicAdd(&hist[order_bucket(key[idx ? idx[i] : i])], 1);
+// Each block counts its chunk of ORDER_CHUNK cells in LDS first: cells of similar |key| (all of
+// them, at t = 0) would otherwise serialise on one global counter.
+constexpr int ORDER_CHUNK = 4096, ORDER_THREADS = 256;
+__device__ __forceinline__ void order_count(long n, const double* __restrict__ key, const int* __restrict__ idx,
+                                            int* lh) {
+  for (int b = threadIdx.x; b < ORDER_BUCKETS; b += ORDER_THREADS) lh[b] = 0;
+  __syncthreads();
+  const long i0 = (long)blockIdx.x * ORDER_CHUNK, i1 = i0 + ORDER_CHUNK < n ? i0 + ORDER_CHUNK : n;
+  for (long i = i0 + threadIdx.x; i < i1; i += ORDER_THREADS) atomicAdd(&lh[order_bucket(key[idx ? idx[i] : i])], 1);
+  __syncthreads();
+}
+__global__ __launch_bounds__(ORDER_THREADS) void order_hist(long n, const double* __restrict__ key,
+                                                            const int* __restrict__ idx, int* __restrict__ hist) {
+  __shared__ int lh[ORDER_BUCKETS];
+  order_count(n, key, idx, lh);
+  for (int b = threadIdx.x; b < ORDER_BUCKETS; b += ORDER_THREADS)
+    if (lh[b]) atomicAdd(&hist[b], lh[b]);
 }
 // exclusive prefix sum of the class counts in place (one block of 1024 threads, 4 classes each)
 __global__ __launch_bounds__(1024) void order_scan(int* __restrict__ hist) {
@@ -1120,12 +1154,21 @@ __global__ __launch_bounds__(1024) void order_scan(int* __restrict__ hist) {
     run += v[j];
   }
 }
-__global__ void order_scatter(long n, const double* __restrict__ key, const int* __restrict__ idx,
-                              int* __restrict__ hist, int* __restrict__ out) {
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int cell = idx ? idx[i] : (int)i;
-  out[atomicAdd(&hist[order_bucket(key[cell])], 1)] = cell;
+// The block reserves a range of each class's positions (hist holds the classes' next free
+// positions), then hands them out to its cells.
+__global__ __launch_bounds__(ORDER_THREADS) void order_scatter(long n, const double* __restrict__ key,
+                                                               const int* __restrict__ idx, int* __restrict__ hist,
+                                                               int* __restrict__ out) {
+  __shared__ int lh[ORDER_BUCKETS];
+  order_count(n, key, idx, lh);
+  for (int b = threadIdx.x; b < ORDER_BUCKETS; b += ORDER_THREADS)
+    if (lh[b]) lh[b] = atomicAdd(&hist[b], lh[b]);
+  __syncthreads();
+  const long i0 = (long)blockIdx.x * ORDER_CHUNK, i1 = i0 + ORDER_CHUNK < n ? i0 + ORDER_CHUNK : n;
+  for (long i = i0 + threadIdx.x; i < i1; i += ORDER_THREADS) {
+    const int cell = idx ? idx[i] : (int)i;
+    out[atomicAdd(&lh[order_bucket(key[cell])], 1)] = cell;
+  }
 }
 
 // Two waves per SIMD: the per-team LDS (LU factors) allows about seven resident waves per CU, and
@@ -1337,12 +1380,13 @@ hipError_t flint_reactor_launch(const flint_reactor::Prog& P, const flint_reacto
 
 hipError_t flint_reactor_order(long n, const double* key, const int* idx, int* out, int* hist, hipStream_t s) {
   if (n <= 0) return hipSuccess;
-  const unsigned g = (unsigned)((n + 255) / 256);
+  const unsigned g = (unsigned)((n + flint_reactor::ORDER_CHUNK - 1) / flint_reactor::ORDER_CHUNK);
+  const unsigned t = flint_reactor::ORDER_THREADS;
   hipLaunchKernelGGL(flint_reactor::fill_int, dim3(flint_reactor::ORDER_BUCKETS / 256), dim3(256), 0, s, hist,
                      (long)flint_reactor::ORDER_BUCKETS, 0);
-  hipLaunchKernelGGL(flint_reactor::order_hist, dim3(g), dim3(256), 0, s, n, key, idx, hist);
+  hipLaunchKernelGGL(flint_reactor::order_hist, dim3(g), dim3(t), 0, s, n, key, idx, hist);
   hipLaunchKernelGGL(flint_reactor::order_scan, dim3(1), dim3(1024), 0, s, hist);
-  hipLaunchKernelGGL(flint_reactor::order_scatter, dim3(g), dim3(256), 0, s, n, key, idx, hist, out);
+  hipLaunchKernelGGL(flint_reactor::order_scatter, dim3(g), dim3(t), 0, s, n, key, idx, hist, out);
   return hipGetLastError();
 }
 
