@@ -179,7 +179,7 @@ __host__ __device__ inline size_t team_lds_bytes(const Prog& P) {
 }
 
 // Block-shared program (16-byte multiples): records, efficiencies, chunk slots, chunk terms, molar
-// masses and gas constants.  Everything the right-hand side reads besides the state and the
+// masses and gas constants, and the Jacobian's term lists.  Everything the right-hand side reads besides the state and the
 // tables sits here, so that an evaluation waits on global memory only for its table rows.
 struct SharedLayout {
   size_t eff, kp, ct, w, rp, tb, tbp, total;
@@ -545,20 +545,23 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
     }
   }
   tsync();
+  // each row loop below touches only this lane's row and reads per-column values kept elsewhere
+  // (restrict-qualified within its block, so that its loads need not wait for its stores)
   double bT = 0.0;
   const double Wk = L.W[lane];
-  if (sp && h0) {
-    bT = row[32];
-    // species block A_kj = d wdot_k / d rho_j (unclipped), b_k = d wdot_k / dT at fixed rho
-    for (int j = 0; j < ns; ++j) row[j] *= Wk / L.W[j];
-  }
-  const double bk = Wk * bT;
   const double Rk = L.Rk[lane];
   if (mode == 0) {
-    // constant volume: J_kj = g_j A_kj, J_kT = b_k; T row from column sums of e_k J_kj
+    // constant volume: species block A_kj = d wdot_k / d rho_j (unclipped) = (W_k / W_j) times the
+    // concentration derivative, J_kj = g_j A_kj, J_kT = b_k = W_k d wdot_k / dT at fixed rho; the
+    // T row from column sums of e_k J_kj
     if (sp && h0) {
-      for (int j = 0; j < ns; ++j) row[j] *= L.g[j];
-      row[ns] = bk;
+      double* __restrict__ rw = row;
+      const double* __restrict__ gj = L.g;
+      const double* __restrict__ Wj = L.W;
+      bT = rw[32];
+#pragma unroll 4
+      for (int j = 0; j < ns; ++j) rw[j] = rw[j] * (Wk / Wj[j]) * gj[j];
+      rw[ns] = Wk * bT;
     }
     const double s_de = tsum(sp ? (o.dh - Rk) * o.w : 0.0);
     const double s_dcv = tsum(sp ? o.xc * o.dcp : 0.0);
@@ -576,12 +579,26 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
   } else {
     // constant pressure (Y, T): see oracle/flint_oracle.c jac_cp_cell for the formulas
     const double Rm = o.Rmix, rho = o.rho;
+    const double iRm = 1.0 / Rm, irho = 1.0 / rho;
     double AY = 0.0;
     if (sp && h0) {
-      for (int j = 0; j < ns; ++j) AY += row[j] * L.aux[j];
+      double* __restrict__ rw = row;
+      const double* __restrict__ gj = L.g;
+      const double* __restrict__ Rj = L.Rk;
+      const double* __restrict__ Wj = L.W;
+      const double* __restrict__ Yj = L.aux;
+      bT = rw[32];
+#pragma unroll 4
+      for (int j = 0; j < ns; ++j) {
+        const double a = rw[j] * (Wk / Wj[j]);
+        rw[j] = a;
+        AY += a * Yj[j];
+      }
       // stage dwdot_k/dY_j and dwdot_k/dT for the enthalpy-weighted column sums
-      for (int j = 0; j < ns; ++j) row[j] = rho * L.g[j] * (row[j] - AY * L.Rk[j] / Rm);
-      row[ns] = bk - rho * AY / T;
+      const double t = AY * iRm;
+#pragma unroll 4
+      for (int j = 0; j < ns; ++j) rw[j] = rho * gj[j] * (rw[j] - t * Rj[j]);
+      rw[ns] = Wk * bT - rho * AY / T;
     }
     const double s_dhw = tsum(sp ? o.dh * o.w : 0.0);
     const double sYdcp = tsum(sp ? o.xc * o.dcp : 0.0);
@@ -597,8 +614,13 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
     }
     tsync();
     if (sp && h0) {
-      for (int j = 0; j < ns; ++j) row[j] = row[j] / rho + L.g[j] * (o.w / rho) * L.Rk[j] / Rm;
-      row[ns] = row[ns] / rho + o.w / (rho * T);
+      double* __restrict__ rw = row;
+      const double* __restrict__ gj = L.g;
+      const double* __restrict__ Rj = L.Rk;
+      const double u = o.w * irho * iRm;
+#pragma unroll 4
+      for (int j = 0; j < ns; ++j) rw[j] = rw[j] * irho + gj[j] * u * Rj[j];
+      rw[ns] = rw[ns] * irho + o.w / (rho * T);
     }
     if (lane < nz) L.lu[ns * 33 + lane] = jt;
     tsync();
@@ -694,81 +716,60 @@ __device__ __forceinline__ double team_solve(int mystep, double b, int nz, const
   return x;
 }
 
-// Two solves, (I - c1 J) x1 = b1 and (I - c2 J) x2 = b2, by Gauss-Jordan elimination of the
-// augmented systems with partial pivoting: the same steps, register layout and rotation as
-// team_inv, with each row's right-hand side carried along (both halves hold it) and no inverse
-// kept.  The pivot row is left unscaled (it stands for the classic scaled row times a_pk, a factor
-// the later steps carry along linearly and the end divides out), which spares one product per
-// element and step.  The two eliminations are independent and interleaved step by step, so that
-// each step's cross-lane latencies (pivot search, pivot-row publication) are paid once for both.
-// Row p_k, the pivot of step k, ends as x_k = beta_{p_k} / a_{p_k k}.  The pivot rows are
-// published in L.vx and L.vb, their right-hand sides in L.g[0..1].  For lanes < nz, x1 and x2
-// receive this lane's components.
+// Two solves, (I - c1 J) x1 = b1 and (I - c2 J) x2 = b2: system 1 in the lower half of the
+// wavefront, system 2 in the upper half, each lane holding one full row (32 columns in registers,
+// rotated by one per step so that column k is always register 0 and every index stays static).
+// Gauss-Jordan elimination of the augmented systems with partial pivoting, each half on its own:
+// the pivot search, the publication of the pivot row and the row updates all stay within a half,
+// so one instruction stream serves both systems.  The pivot row is left unscaled (it stands for
+// the classic scaled row times a_pk, a factor the later steps carry along linearly and the end
+// divides out), and no inverse is kept: row p_k, the pivot of step k, ends as
+// x_k = beta_{p_k} / a_{p_k k}.  The pivot rows go through L.vx (system 1) and L.vb (system 2),
+// their right-hand sides through L.g[0..1].  Every lane receives x1 and x2 of its component.
 __device__ __forceinline__ void team_solve2(double c1, double b1, double c2, double b2, int nz, const Lds& L,
                                             const double* __restrict__ Jg, double& x1, double& x2) {
-  constexpr int HC = TEAM / 2;
   const int lane = threadIdx.x & (TEAM - 1);
-  const int h = half(), c0 = HC * h;
+  const int h = half();
+  const double c = h == 0 ? c1 : c2;
   const double* __restrict__ jrow = Jg + lane * 33;
-  double r[HC], q[HC];
+  double r[TEAM];
 #pragma unroll
-  for (int j = 0; j < HC; ++j) {
-    const double jv = (lane < nz && c0 + j < nz) ? jrow[c0 + j] : 0.0;
-    const double id = lane == c0 + j ? 1.0 : 0.0;
-    r[j] = id - c1 * jv;
-    q[j] = id - c2 * jv;
+  for (int j = 0; j < TEAM; ++j) {
+    const double jv = (lane < nz && j < nz) ? jrow[j] : 0.0;
+    r[j] = (lane == j ? 1.0 : 0.0) - c * jv;
   }
-  double be1 = lane < nz ? b1 : 0.0, be2 = lane < nz ? b2 : 0.0;
-  double* __restrict__ pb1 = L.vx;
-  double* __restrict__ pb2 = L.vb;
-  int my1 = -1, my2 = -1;
-  double myiv1 = 1.0, myiv2 = 1.0;
+  double be = lane < nz ? (h == 0 ? b1 : b2) : 0.0;
+  double* __restrict__ pb = h == 0 ? L.vx : L.vb;
+  double* __restrict__ pbe = L.g + h;
+  int my = -1;
+  double myiv = 1.0;
 #pragma unroll 1
   for (int k = 0; k < nz; ++k) {
-    const int hk = k / HC;
-    const double2 hv1 = halves(r[0]), hv2 = halves(q[0]);
-    const double rk1 = hk == 0 ? hv1.x : hv1.y, rk2 = hk == 0 ? hv2.x : hv2.y;
-    double iv1 = 1.0 / rk1, iv2 = 1.0 / rk2;
-    asm volatile("" : "+v"(iv1), "+v"(iv2));
-    const bool me1 = lane == tpivot(rk1, my1 < 0), me2 = lane == tpivot(rk2, my2 < 0);
-    if (me1) {
-      my1 = k;
-      myiv1 = iv1;
-      pb1[c0] = h == hk ? iv1 : r[0];
+    const double rk = r[0];
+    double iv = 1.0 / rk;  // used by the pivot row only; computed by all while the search runs
+    asm volatile("" : "+v"(iv));
+    const bool me = lane == tpivot(rk, my < 0);
+    if (me) {
+      my = k;
+      myiv = iv;
+      pb[0] = iv;
 #pragma unroll
-      for (int j = 1; j < HC; ++j) pb1[c0 + j] = r[j];
-      if (h == 0) L.g[0] = be1;
-    }
-    if (me2) {
-      my2 = k;
-      myiv2 = iv2;
-      pb2[c0] = h == hk ? iv2 : q[0];
-#pragma unroll
-      for (int j = 1; j < HC; ++j) pb2[c0 + j] = q[j];
-      if (h == 0) L.g[1] = be2;
+      for (int j = 1; j < TEAM; ++j) pb[j] = r[j];
+      *pbe = be;
     }
     tsync();
-    const double f1 = me1 ? 0.0 : rk1 * pb1[HC * hk], f2 = me2 ? 0.0 : rk2 * pb2[HC * hk];
-    be1 = be1 - f1 * L.g[0];
-    be2 = be2 - f2 * L.g[1];
-    const double r0 = h == hk ? (me1 ? 1.0 : -f1) : r[0] - f1 * pb1[c0];
-    const double q0 = h == hk ? (me2 ? 1.0 : -f2) : q[0] - f2 * pb2[c0];
+    // f = a_ik / a_pk (0 on the pivot row, which keeps its values); column k itself is not needed
+    const double f = me ? 0.0 : rk * pb[0];
+    be = be - f * *pbe;
 #pragma unroll
-    for (int j = 0; j < HC - 1; ++j) {
-      r[j] = r[j + 1] - f1 * pb1[c0 + j + 1];
-      q[j] = q[j + 1] - f2 * pb2[c0 + j + 1];
-    }
-    r[HC - 1] = r0;
-    q[HC - 1] = q0;
+    for (int j = 0; j < TEAM - 1; ++j) r[j] = r[j + 1] - f * pb[j + 1];
+    r[TEAM - 1] = 0.0;
     tsync();
   }
-  if (h == 0 && lane < nz) {  // (a row that never pivoted, only in a singular system, keeps its slot)
-    pb1[my1 < 0 ? lane : my1] = be1 * myiv1;
-    pb2[my2 < 0 ? lane : my2] = be2 * myiv2;
-  }
+  if (lane < nz) pb[my < 0 ? lane : my] = be * myiv;  // (a row that never pivoted: singular system)
   tsync();
-  x1 = pb1[lane];
-  x2 = pb2[lane];
+  x1 = L.vx[lane];
+  x2 = L.vb[lane];
   tsync();
 }
 
