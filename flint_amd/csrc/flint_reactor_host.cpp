@@ -520,7 +520,9 @@ extern "C" int flint_reactor_get_launch(flint_mech* m, int* teams_per_block, int
 }
 
 // Autotuner: times every candidate launch shape on a copy of the sample (the caller's data is not
-// modified) and keeps the fastest for this handle.
+// modified) and keeps the fastest for this handle.  The sample's cells are handed out by descending
+// |dT/dt| in every candidate launch (the order of policy 1): a sample holds fewer cells per team
+// than the full batch, and in the caller's order one stiff cell started late would set its time.
 extern "C" int flint_reactor_autotune(flint_mech* m, long n, long ld, int mode, const double* X, const double* p,
                                       double t_end, double rtol, double atol, double* best_ms) {
   if (int rc = check_args(m, n, ld, mode, X, p, t_end, rtol, atol)) return rc;
@@ -535,32 +537,48 @@ extern "C" int flint_reactor_autotune(flint_mech* m, long n, long ld, int mode, 
                                          160 * 1024);
   const int nz = m->ns + 1;
   double* work = nullptr;
-  HIPCHK(hipMalloc(&work, (size_t)nz * n * sizeof(double)));
-  hipStream_t s;
-  HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0));
-  HIPCHK(hipEventCreate(&e1));
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = FLINT_OK;
+  // records the first failure; every later step is skipped and the resources are released below
+  auto hip = [&rc](hipError_t e, const char* what) {
+    if (e != hipSuccess && rc == FLINT_OK) rc = flint_fail(FLINT_EHIP, std::string("autotune: ") + what + ": " + hipGetErrorString(e));
+    return rc == FLINT_OK;
+  };
+  auto copy_sample = [&]() {
+    return hip(hipMemcpy2DAsync(work, n * sizeof(double), X, ld * sizeof(double), n * sizeof(double), nz,
+                                hipMemcpyDeviceToDevice, s),
+               "sample copy");
+  };
   float best = 1e30f;
   int best_tpb = x->teams_per_block, best_blocks = x->blocks;
+  const int* order = nullptr;
+  if (hip(hipMalloc(&work, (size_t)nz * n * sizeof(double)), "hipMalloc") &&
+      hip(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream") && hip(hipEventCreate(&e0), "event") &&
+      hip(hipEventCreate(&e1), "event")) {
+    rc = ensure_jac(x, x->teams_per_block, x->blocks);
+    if (!rc) rc = ensure_order(m, n, n);
+    if (!rc && copy_sample()) {
+      const int keep = x->order;
+      x->order = 1;
+      rc = enqueue_order(m, n, n, mode, work, p, s, &order);
+      x->order = keep;
+    }
+  }
   for (int tpb : {1, 2, 4, 8}) {
     const int fit = (int)(lds_cu / (flint_reactor_shared_bytes(x->prog) + tb * tpb));
-    if (fit < 1) continue;
     for (int per_cu : {1, 2, 3, 4, 6, 8}) {
-      if (per_cu > fit || per_cu * tpb > 8) continue;
+      if (rc || fit < 1 || per_cu > fit || per_cu * tpb > 8) continue;
       const int blocks = cus * per_cu;
-      int rc = 0;
-      for (int rep = 0; rep < 2 && !rc; ++rep) {
-        HIPCHK(hipMemcpy2DAsync(work, n * sizeof(double), X, ld * sizeof(double), n * sizeof(double), nz,
-                                hipMemcpyDeviceToDevice, s));
-        HIPCHK(hipEventRecord(e0, s));
-        rc = launch(m, n, n, mode, work, p, t_end, rtol, atol, 0, nullptr, nullptr, s, tpb, blocks);
-        HIPCHK(hipEventRecord(e1, s));
-        HIPCHK(hipStreamSynchronize(s));
+      int lrc = 0;
+      for (int rep = 0; rep < 2 && !lrc && !rc; ++rep) {
+        if (!copy_sample() || !hip(hipEventRecord(e0, s), "event")) break;
+        lrc = launch(m, n, n, mode, work, p, t_end, rtol, atol, 0, nullptr, nullptr, s, tpb, blocks, order);
+        hip(hipEventRecord(e1, s), "event");
+        hip(hipStreamSynchronize(s), "synchronize");
       }
-      if (rc) continue;
       float ms = 0.f;
-      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      if (lrc || rc || !hip(hipEventElapsedTime(&ms, e0, e1), "timing")) continue;  // a shape that fails to launch is skipped
       if (ms < best) {
         best = ms;
         best_tpb = tpb;
@@ -568,10 +586,11 @@ extern "C" int flint_reactor_autotune(flint_mech* m, long n, long ld, int mode, 
       }
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipStreamDestroy(s);
-  (void)hipFree(work);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (work) (void)hipFree(work);
+  if (rc) return rc;
   x->teams_per_block = best_tpb;
   x->blocks = best_blocks;
   x->tuned = true;
