@@ -82,17 +82,36 @@ static int encode_rhs(const flint_host_prog* hp, flint_reactor::Prog& P, std::ve
   // Arrhenius and three-body reactions (one code path) fill the first wavefront rounds; the
   // falloff reactions (Troe, Lindemann: logarithms, powers and divisions) start a round of their
   // own, so that their code runs in as few rounds as they need.
+  // Within each group the reactions with the most net-production terms and factors come first, so
+  // that the rounds after the first carry fewer of the Jacobian's scatter iterations (a round runs
+  // as many as its busiest reaction needs).  The slot order changes no rate: each reaction's rate
+  // goes to its own index, and the production sums run in reaction order.
+  std::vector<int> nterm_of(nr, 0), nfac_of(nr, 0);
+  for (int t = 0; t < hp->wd_ptr[ns]; ++t) nterm_of[hp->wd_rx[t]]++;
+  for (int r = 0; r < nr; ++r)
+    for (int f = 0; f < 8; ++f)
+      if (hp->rx_fac[r * 8 + f] >= 0) nfac_of[r] += hp->rx_fac[r * 8 + f] >> 8;
+  auto busier = [&](int a, int b) {
+    if (nterm_of[a] != nterm_of[b]) return nterm_of[a] > nterm_of[b];
+    if (nfac_of[a] != nfac_of[b]) return nfac_of[a] > nfac_of[b];
+    return a < b;
+  };
   std::vector<int> order;
   for (int kind = 0; kind < 2; ++kind)
     for (int r = 0; r < nr; ++r)
       if ((hp->rx_kind[r] & 3) == kind) order.push_back(r);
-  const int n_plain = (int)order.size();
-  if (n_plain < nr)
+  std::sort(order.begin(), order.end(), busier);
+  const size_t n_plain = order.size();
+  if ((int)n_plain < nr)
     while (order.size() % 64) order.push_back(-1);
+  const size_t f0 = order.size();
   for (int kind = 2; kind < 4; ++kind)
     for (int r = 0; r < nr; ++r)
       if ((hp->rx_kind[r] & 3) == kind) order.push_back(r);
-  (void)n_plain;
+  std::sort(order.begin() + f0, order.end(), [&](int a, int b) {
+    if ((hp->rx_kind[a] & 3) != (hp->rx_kind[b] & 3)) return (hp->rx_kind[a] & 3) < (hp->rx_kind[b] & 3);
+    return busier(a, b);
+  });
   P.R = ((int)order.size() + 31) / 32;
   rrec.assign((size_t)P.R * 32 * 4, 0);
   for (int s = 0; s < P.R * 32; ++s) {
