@@ -155,6 +155,7 @@ struct Lds {
   const int2* cterm;  // [C][CW][32] production-sum chunk terms
   const double* W;    // [32] molar masses (1 beyond ns)
   const double* Rk;   // [32] specific gas constants 8314.51 / W_k (0 beyond ns)
+  const double* rcp;  // [32] 1/k for k >= 1, and 1/(ns+1) at 0
   const int4* rprod;  // [R*32][2] net-production terms of each slot's reaction (Jacobian scatter)
   const int4* tbe;    // [ntb] terms with a concentration M, by species
   const int* tbptr;   // [ns+1]
@@ -170,16 +171,17 @@ struct Lds {
   double* D;    // [8][32] BDF difference array (row j, component k); kept in LDS to spare registers
   double* lu;   // [32][33] inverse of I - cJ (row = lane, padded); Jacobian rows while one is built
   double* dq;   // [nmr] dq/dM of the reactions with a concentration M (while a Jacobian is built)
+  double* sx;   // [8] team scalars: [0] the Newton correction scale of the current matrix
 };
 
 __host__ __device__ inline size_t team_lds_bytes(const Prog& P) {
   return (size_t)(32 * 5 + 8 * 32 + (P.nms > 0 ? P.nms : 1) + (size_t)P.nr + 1 + (size_t)P.C * 32 + 32 * 33 +
-                  (P.nmr > 0 ? P.nmr : 1)) *
+                  (P.nmr > 0 ? P.nmr : 1) + 8) *
          sizeof(double);
 }
 
 // Block-shared program (16-byte multiples): records, efficiencies, chunk slots, chunk terms, molar
-// masses and gas constants, and the Jacobian's term lists.  Everything the right-hand side reads besides the state and the
+// masses, gas constants and small reciprocals, and the Jacobian's term lists.  Everything the right-hand side reads besides the state and the
 // tables sits here, so that an evaluation waits on global memory only for its table rows.
 struct SharedLayout {
   size_t eff, kp, ct, w, rp, tb, tbp, total;
@@ -190,7 +192,7 @@ __host__ __device__ inline SharedLayout shared_layout(const Prog& P) {
   o.kp = o.eff + ((((size_t)P.ns * (P.nms > 0 ? P.nms : 1) * 4) + 15) & ~size_t(15));
   o.ct = o.kp + ((((size_t)P.MAXP * 32 * 4) + 15) & ~size_t(15));
   o.w = o.ct + (size_t)P.C * CW * TEAM * 8;
-  o.rp = o.w + 2 * TEAM * 8;
+  o.rp = o.w + 3 * TEAM * 8;
   o.tb = o.rp + (size_t)P.R * 32 * 32;
   o.tbp = o.tb + (size_t)(P.ntb > 0 ? P.ntb : 1) * 16;
   o.total = o.tbp + ((((size_t)P.ns + 1) * 4 + 15) & ~size_t(15));
@@ -214,6 +216,8 @@ __device__ inline void load_shared_prog(const Prog& P, char* base) {
     const bool sp = i < P.ns;
     w[i] = sp ? P.W[i] : 1.0;
     w[TEAM + i] = sp ? 8314.51 / P.W[i] : 0.0;
+    // reciprocals the step control multiplies by: 1/k (slot k >= 1) and 1/nz (slot 0)
+    w[2 * TEAM + i] = i == 0 ? 1.0 / (P.ns + 1) : 1.0 / i;
   }
   int4* rp = reinterpret_cast<int4*>(base + o.rp);
   for (int i = threadIdx.x; i < P.R * 32 * 2; i += blockDim.x) rp[i] = P.rprod[i];
@@ -232,6 +236,7 @@ __device__ inline Lds carve(char* shared, char* base, const Prog& P) {
   L.cterm = reinterpret_cast<const int2*>(shared + o.ct);
   L.W = reinterpret_cast<const double*>(shared + o.w);
   L.Rk = L.W + TEAM;
+  L.rcp = L.W + 2 * TEAM;
   L.rprod = reinterpret_cast<const int4*>(shared + o.rp);
   L.tbe = reinterpret_cast<const int4*>(shared + o.tb);
   L.tbptr = reinterpret_cast<const int*>(shared + o.tbp);
@@ -246,7 +251,8 @@ __device__ inline Lds carve(char* shared, char* base, const Prog& P) {
   L.q = d; d += P.nr + 1;
   L.part = d; d += P.C * 32;
   L.lu = d; d += 32 * 33;
-  L.dq = d;
+  L.dq = d; d += (P.nmr > 0 ? P.nmr : 1);
+  L.sx = d;
   return L;
 }
 
@@ -773,10 +779,12 @@ __device__ __forceinline__ void team_solve2(double c1, double b1, double c2, dou
   tsync();
 }
 
-__device__ __forceinline__ double team_rms(double v, double scale, int nz) {
+// RMS over the team's nz components of v times the reciprocal weight iscale (products by
+// reciprocals instead of divisions: the step control runs this several times per step).
+__device__ __forceinline__ double team_rms(double v, double iscale, int nz, const Lds& L) {
   const int lane = threadIdx.x & (TEAM - 1);
-  const double q = lane < nz ? v / scale : 0.0;
-  return sqrt(tsum(q * q) / nz);
+  const double q = lane < nz ? v * iscale : 0.0;
+  return sqrt(tsum(q * q) * L.rcp[0]);
 }
 
 // This lane's column of the BDF difference array in LDS (both halves hold the same values).
@@ -843,7 +851,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
   enum { PH_F0, PH_F1, PH_J0, PH_NEWTON, PH_JREF };
   enum { GO_EVAL, GO_STEP, GO_ATTEMPT, GO_NEWTON, GO_HALVE, GO_ACCEPT };
   // start-up values
-  double f0 = 0.0, sc = 0.0, h0 = 0.0, d1 = 0.0;
+  double f0 = 0.0, isc = 0.0, h0 = 0.0, d1 = 0.0;  // isc: 1 / the error weight of the last accepted state
   // method state
   const Dcol D{L.D + lane};
 #pragma unroll
@@ -853,7 +861,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
   int mystep = lane;
   double t = 0.0, h_abs = 0.0, min_step = 0.0;
   // current step attempt and Newton iteration
-  double t_new = 0.0, ypred = 0.0, psi = 0.0, scale = 0.0, cc = 0.0, ynew = 0.0, d = 0.0;
+  double t_new = 0.0, ypred = 0.0, psi = 0.0, iscale = 0.0, cc = 0.0, ynew = 0.0, d = 0.0;
   double dy_norm_old = -1.0, err_norm = 0.0, safety = 0.9;
   int k_it = 0;
   // Newton policy 1: coefficient of the factorised matrix, its age, the carried rate estimate
@@ -878,16 +886,16 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
     if (phase == PH_F0) {
       if (!o.ok) return 3;
       f0 = o.F;
-      sc = atol + fabs(y) * rtol;
-      const double d0 = team_rms(y, sc, nz);
-      d1 = team_rms(f0, sc, nz);
+      isc = 1.0 / (atol + fabs(y) * rtol);
+      const double d0 = team_rms(y, isc, nz, L);
+      d1 = team_rms(f0, isc, nz, L);
       h0 = (d0 < 1e-5 || d1 < 1e-5) ? 1e-6 : 0.01 * d0 / d1;
       h0 = fmin(h0, t_end);
       xe = y + h0 * f0;
       phase = PH_F1;
     } else if (phase == PH_F1) {
       if (!o.ok) return 3;
-      const double d2 = team_rms(o.F - f0, sc, nz) / h0;
+      const double d2 = team_rms(o.F - f0, isc, nz, L) / h0;
       const double h1 = (d1 <= 1e-15 && d2 <= 1e-15) ? fmax(1e-6, h0 * 1e-3) : sqrt(0.01 / fmax(d1, d2));
       h_abs = fmin(100.0 * h0, fmin(h1, t_end));
       D[0] = y;
@@ -914,9 +922,9 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
       if (!o.ok || !isfinite(tsum(o.F))) {
         fail = true;
       } else if (cv) {
-        const double dy = team_solve(mystep, cc * o.F - psi - d, nz, L) * (2.0 / (1.0 + cc / c_lu));
+        const double dy = team_solve(mystep, cc * o.F - psi - d, nz, L) * L.sx[0];
         st[ST_NEWTON]++;
-        const double dy_norm = team_rms(dy, scale, nz);
+        const double dy_norm = team_rms(dy, iscale, nz, L);
         if (k_it > 0 && dy_norm > CV_RDIV * dy_norm_old) {
           fail = true;
         } else {
@@ -939,7 +947,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
       } else {
         const double dy = team_solve(mystep, cc * o.F - psi - d, nz, L);
         st[ST_NEWTON]++;
-        const double dy_norm = team_rms(dy, scale, nz);
+        const double dy_norm = team_rms(dy, iscale, nz, L);
         const double rate = dy_norm_old >= 0.0 ? dy_norm / dy_norm_old : -1.0;
         if (rate >= 0.0 && (rate >= 1.0 || pow(rate, (double)(NEWTON_MAXITER - k_it)) / (1.0 - rate) * dy_norm > newton_tol)) {
           fail = true;
@@ -971,9 +979,8 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
       } else if (conv) {
         if (cv) lu_age++;
         safety = 0.9 * (2 * NEWTON_MAXITER + 1) / (double)(2 * NEWTON_MAXITER + n_iter);
-        const double scale2 = atol + rtol * fabs(ynew);
-        sc = scale2;
-        err_norm = team_rms(d / (order + 1), scale2, nz);
+        isc = 1.0 / (atol + rtol * fabs(ynew));
+        err_norm = team_rms(d * L.rcp[order + 1], isc, nz, L);
         if (err_norm > 1.0) {
           st[ST_REJECT]++;
           const double factor = fmax(MIN_FACTOR, safety * pow(err_norm, -1.0 / (order + 1)));
@@ -1012,8 +1019,8 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         go = GO_STEP;
         if (n_equal < order + 1) continue;
         const double Dm = D[order], Dp = D[order + 2];
-        const double em = order > 1 ? team_rms(Dm / order, sc, nz) : INFINITY;
-        const double ep = order < MAXO ? team_rms(Dp / (order + 2), sc, nz) : INFINITY;
+        const double em = order > 1 ? team_rms(Dm * L.rcp[order], isc, nz, L) : INFINITY;
+        const double ep = order < MAXO ? team_rms(Dp * L.rcp[order + 2], isc, nz, L) : INFINITY;
         const double norms[3] = {em, err_norm, ep};
         double best = -1.0;
         int which = 1;
@@ -1067,7 +1074,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         for (int j = 1; j <= MAXO; ++j)
           if (j == order) alpha_o = gamma[j];
         psi /= alpha_o;
-        scale = atol + rtol * fabs(ypred);
+        iscale = 1.0 / (atol + rtol * fabs(ypred));
         cc = h / alpha_o;
         go = GO_NEWTON;
       } else {  // GO_NEWTON: (re)start the Newton iteration from the predictor
@@ -1080,6 +1087,8 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
           lu_age = 0;
           crate = 1.0;
         }
+        // CVODE's correction scale for a matrix factored at c_lu, fixed for this attempt's iterations
+        if (cv) L.sx[0] = 2.0 / (1.0 + cc / c_lu);
         ynew = ypred;
         d = 0.0;
         dy_norm_old = -1.0;
