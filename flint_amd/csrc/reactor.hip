@@ -156,6 +156,7 @@ struct Lds {
   const double* W;    // [32] molar masses (1 beyond ns)
   const double* Rk;   // [32] specific gas constants 8314.51 / W_k (0 beyond ns)
   const double* rcp;  // [32] 1/k for k >= 1, and 1/(ns+1) at 0
+  const double* Wi;   // [32] 1 / W_k (1 beyond ns)
   const int4* rprod;  // [R*32][2] net-production terms of each slot's reaction (Jacobian scatter)
   const int4* tbe;    // [ntb] terms with a concentration M, by species
   const int* tbptr;   // [ns+1]
@@ -192,7 +193,7 @@ __host__ __device__ inline SharedLayout shared_layout(const Prog& P) {
   o.kp = o.eff + ((((size_t)P.ns * (P.nms > 0 ? P.nms : 1) * 4) + 15) & ~size_t(15));
   o.ct = o.kp + ((((size_t)P.MAXP * 32 * 4) + 15) & ~size_t(15));
   o.w = o.ct + (size_t)P.C * CW * TEAM * 8;
-  o.rp = o.w + 3 * TEAM * 8;
+  o.rp = o.w + 4 * TEAM * 8;
   o.tb = o.rp + (size_t)P.R * 32 * 32;
   o.tbp = o.tb + (size_t)(P.ntb > 0 ? P.ntb : 1) * 16;
   o.total = o.tbp + ((((size_t)P.ns + 1) * 4 + 15) & ~size_t(15));
@@ -218,6 +219,7 @@ __device__ inline void load_shared_prog(const Prog& P, char* base) {
     w[TEAM + i] = sp ? 8314.51 / P.W[i] : 0.0;
     // reciprocals the step control multiplies by: 1/k (slot k >= 1) and 1/nz (slot 0)
     w[2 * TEAM + i] = i == 0 ? 1.0 / (P.ns + 1) : 1.0 / i;
+    w[3 * TEAM + i] = sp ? 1.0 / P.W[i] : 1.0;
   }
   int4* rp = reinterpret_cast<int4*>(base + o.rp);
   for (int i = threadIdx.x; i < P.R * 32 * 2; i += blockDim.x) rp[i] = P.rprod[i];
@@ -237,6 +239,7 @@ __device__ inline Lds carve(char* shared, char* base, const Prog& P) {
   L.W = reinterpret_cast<const double*>(shared + o.w);
   L.Rk = L.W + TEAM;
   L.rcp = L.W + 2 * TEAM;
+  L.Wi = L.W + 3 * TEAM;
   L.rprod = reinterpret_cast<const int4*>(shared + o.rp);
   L.tbe = reinterpret_cast<const int4*>(shared + o.tb);
   L.tbptr = reinterpret_cast<const int*>(shared + o.tbp);
@@ -403,7 +406,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
     o.rho = p / (o.Rmix * T);
     roi = o.rho * o.xc;
   }
-  L.c[lane] = sp ? roi / Wk : (lane == TEAM - 1 ? 1.0 : 0.0);
+  L.c[lane] = sp ? roi * L.Wi[lane] : (lane == TEAM - 1 ? 1.0 : 0.0);
   if (lane == 0) L.q[P.nr] = 0.0;
   if (jac) {
     const int wl0 = threadIdx.x & (WAVE - 1);
@@ -505,8 +508,9 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   } else {
     const double Q = tsum(o.h * w);
     o.cpm = tsum(o.xc * o.cp);
-    FT = -Q / (o.rho * o.cpm);
-    o.F = sp ? w / o.rho : 0.0;
+    const double irho = 1.0 / o.rho;
+    FT = -Q * irho / o.cpm;
+    o.F = sp ? w * irho : 0.0;
   }
   if (lane == ns) o.F = FT;
   o.ok = true;
@@ -779,6 +783,47 @@ __device__ __forceinline__ void team_solve2(double c1, double b1, double c2, dou
   tsync();
 }
 
+// Step-size factor x^e of the controller (x >= 0, -1 <= e < 0): x = m 2^k with m in
+// [sqrt(1/2), sqrt(2)), log2 x = k + 2 atanh(s) / ln 2 with s = (m - 1)/(m + 1) (|s| <= 0.172, the
+// series to s^13), then 2^t = 2^n e^(f ln 2) with n = rint(t), |f| <= 1/2 (the series to f^10):
+// about 1e-12 relative, which is more than a step-size factor needs, in about 100 instructions
+// against about 250 for an FP64 pow.  Every operation is an IEEE double operation written out
+// (explicit fma), so the CPU BDF computes the same factors bit for bit.  x = 0 gives +inf,
+// x = +inf gives 0, as pow does.
+__device__ __forceinline__ double step_pow(double x, double e) {
+  if (x == 0.0) return INFINITY;
+  if (!(x <= 1.7976931348623157e308)) return x == INFINITY ? 0.0 : x;
+  int k;
+  double m = frexp(x, &k);
+  if (m < 0.70710678118654752) {
+    m = m * 2.0;
+    k = k - 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+  double p = 1.0 / 13.0;
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  p = fma(p, s2, 1.0);
+  const double l2 = fma(2.0 * s * p, 1.4426950408889634, (double)k);
+  const double t = fmin(fmax(e * l2, -1074.0), 1024.0);
+  const double n = rint(t), y = (t - n) * 0.6931471805599453;
+  double q = 1.0 / 3628800.0;
+  q = fma(q, y, 1.0 / 362880.0);
+  q = fma(q, y, 1.0 / 40320.0);
+  q = fma(q, y, 1.0 / 5040.0);
+  q = fma(q, y, 1.0 / 720.0);
+  q = fma(q, y, 1.0 / 120.0);
+  q = fma(q, y, 1.0 / 24.0);
+  q = fma(q, y, 1.0 / 6.0);
+  q = fma(q, y, 0.5);
+  q = fma(q, y, 1.0);
+  q = fma(q, y, 1.0);
+  return ldexp(q, (int)n);
+}
+
 // RMS over the team's nz components of v times the reciprocal weight iscale (products by
 // reciprocals instead of divisions: the step control runs this several times per step).
 __device__ __forceinline__ double team_rms(double v, double iscale, int nz, const Lds& L) {
@@ -983,7 +1028,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         err_norm = team_rms(d * L.rcp[order + 1], isc, nz, L);
         if (err_norm > 1.0) {
           st[ST_REJECT]++;
-          const double factor = fmax(MIN_FACTOR, safety * pow(err_norm, -1.0 / (order + 1)));
+          const double factor = fmax(MIN_FACTOR, safety * step_pow(err_norm, -L.rcp[order + 1]));
           h_abs *= factor;
           change_D(D, order, factor);
           n_equal = 0;
@@ -1026,7 +1071,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         int which = 1;
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
-          const double fct = norms[q] == 0.0 ? INFINITY : pow(norms[q], -1.0 / (order + q));
+          const double fct = norms[q] == 0.0 ? INFINITY : step_pow(norms[q], -L.rcp[order + q]);
           if (fct > best) { best = fct; which = q; }
         }
         order += which - 1;

@@ -25,6 +25,46 @@
 
 enum { ST_STEPS, ST_REJECT, ST_NFAIL, ST_RHS, ST_JAC, ST_LU, ST_NEWTON, ST_ORDER };
 
+/* Step-size factor x^e of the controller.  method 0 takes scipy's pow.  method 1 (Newton policy 1)
+ * takes the GPU kernel's evaluation, the same IEEE double operations in the same order (explicit
+ * fma): x = m 2^k with m in [sqrt(1/2), sqrt(2)), log2 x = k + 2 atanh(s) / ln 2 with
+ * s = (m - 1)/(m + 1) (series to s^13), 2^t = 2^n e^(f ln 2) with n = rint(t) (series to f^10);
+ * about 1e-12 relative, and the kernel's factors bit for bit. */
+static double step_pow(double x, double e, int method) {
+  if (!method) return pow(x, e);
+  if (x == 0.0) return INFINITY;
+  if (!(x <= DBL_MAX)) return x == INFINITY ? 0.0 : x;
+  int k;
+  double m = frexp(x, &k);
+  if (m < 0.70710678118654752) {
+    m = m * 2.0;
+    k = k - 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+  double p = 1.0 / 13.0;
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  p = fma(p, s2, 1.0);
+  const double l2 = fma(2.0 * s * p, 1.4426950408889634, (double)k);
+  const double t = fmin(fmax(e * l2, -1074.0), 1024.0);
+  const double n = rint(t), y = (t - n) * 0.6931471805599453;
+  double q = 1.0 / 3628800.0;
+  q = fma(q, y, 1.0 / 362880.0);
+  q = fma(q, y, 1.0 / 40320.0);
+  q = fma(q, y, 1.0 / 5040.0);
+  q = fma(q, y, 1.0 / 720.0);
+  q = fma(q, y, 1.0 / 120.0);
+  q = fma(q, y, 1.0 / 24.0);
+  q = fma(q, y, 1.0 / 6.0);
+  q = fma(q, y, 0.5);
+  q = fma(q, y, 1.0);
+  q = fma(q, y, 1.0);
+  return ldexp(q, (int)n);
+}
+
 static double rms(const double *v, const double *scale, int n) {
   double s = 0.0;
   for (int i = 0; i < n; ++i) {
@@ -300,7 +340,7 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
       err_norm = rms(dy, scale, n);
       if (err_norm > 1.0) {
         st[ST_REJECT]++;
-        double factor = fmax(MIN_FACTOR, safety * pow(err_norm, -1.0 / (order + 1)));
+        double factor = fmax(MIN_FACTOR, safety * step_pow(err_norm, -1.0 / (order + 1), method));
         h_abs *= factor;
         change_D(D, n, order, factor, method);
         n_equal = 0;
@@ -333,7 +373,7 @@ static int bdf_cell(const orc_mech *m, const double *nu, int mode, double *y, do
     double norms[3] = {em, err_norm, ep}, best = -1.0;
     int which = 1;
     for (int q = 0; q < 3; ++q) {
-      double fct = (norms[q] == 0.0) ? INFINITY : pow(norms[q], -1.0 / (order + q));
+      double fct = (norms[q] == 0.0) ? INFINITY : step_pow(norms[q], -1.0 / (order + q), method);
       if (fct > best) { best = fct; which = q; }
     }
     order += which - 1;
