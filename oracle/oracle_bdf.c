@@ -29,7 +29,8 @@ enum { ST_STEPS, ST_REJECT, ST_NFAIL, ST_RHS, ST_JAC, ST_LU, ST_NEWTON, ST_ORDER
  * takes the GPU kernel's evaluation, the same IEEE double operations in the same order (explicit
  * fma): x = m 2^k with m in [sqrt(1/2), sqrt(2)), log2 x = k + 2 atanh(s) / ln 2 with
  * s = (m - 1)/(m + 1) (series to s^13), 2^t = 2^n e^(f ln 2) with n = rint(t) (series to f^10);
- * about 1e-12 relative, and the kernel's factors bit for bit. */
+ * about 1e-12 relative, and the kernel's factors bit for bit.  Under method 0 the oracle stays on
+ * scipy's pow while the kernel keeps its series, so the two differ there by about 1e-12. */
 static double step_pow(double x, double e, int method) {
   if (!method) return pow(x, e);
   if (x == 0.0) return INFINITY;
