@@ -380,8 +380,11 @@ __device__ __forceinline__ void reaction(const int4 rec, const double* __restric
 
 // Team right-hand side; o.ok is false (team-uniform) when T leaves the tables.  With jac the
 // reactions' derivatives are accumulated in the Jacobian rows in LDS as well (team_jac finishes
-// them); the rows replace the inverse of I - cJ there, which a new Jacobian always renews.
-__device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, double x, const Lds& L, bool jac) {
+// them); the rows replace the inverse of I - cJ there, which a new Jacobian always renews.  jac is
+// a template argument so that the plain evaluation, nearly all of the integrator's, is compiled
+// apart from the scatter and does not carry its register needs.
+template <bool jac>
+__device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, double x, const Lds& L) {
   RhsOut o;
   o.ok = false;
   const int lane = threadIdx.x & (TEAM - 1);
@@ -874,14 +877,19 @@ __device__ __forceinline__ void change_D(const Dcol D, int order, double factor)
   }
 }
 
+// A decision of the step control, taken as the first lane's.  Every lane computes the same bits
+// (team sums are bit-identical on all lanes and both halves hold the same values), but the compiler
+// cannot know it: with the value read from one lane the branch is a scalar branch with no exec-mask
+// save and restore, and the state it sets (order, phase, counters) lives in scalar registers,
+// which leaves vector registers to the evaluations.  Only called with the whole wavefront active.
+__device__ __forceinline__ bool unib(bool b) { return __builtin_amdgcn_readfirstlane((int)b) != 0; }
+
 // One cell; mirrors bdf_cell of oracle/oracle_bdf.c decision for decision.  Returns 0 ok, 1 step
 // underflow, 2 too many steps, 3 invalid state.
 //
-// The integrator is written as a state machine around a single right-hand-side evaluation (and a
+// The integrator is written as a state machine around one right-hand-side evaluation site (and a
 // single Jacobian, inverse and solve site): every place where the method needs f(x) sets x and the
-// phase that consumes the result.  One copy of the evaluation keeps the kernel's code and register
-// footprint small, and two teams of a wavefront that both need an evaluation, whatever their
-// phase, run it together.
+// phase that consumes the result, which keeps the kernel's code small.
 __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p, double& y, int (&st)[8], const Lds& L,
                                           double* __restrict__ Jg) {
   const int lane = threadIdx.x & (TEAM - 1);
@@ -918,7 +926,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
   double xe = y;
   bool wj = false;
   while (true) {
-    const RhsOut o = team_rhs(P, O.mode, p, xe, L, wj);
+    const RhsOut o = wj ? team_rhs<true>(P, O.mode, p, xe, L) : team_rhs<false>(P, O.mode, p, xe, L);
     if (wj) {
       if (o.ok) {
         team_jac(P, O.mode, xe, L, o, Jg);
@@ -964,19 +972,19 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
     } else {  // PH_NEWTON: f at the current iterate ynew
       bool conv = false, fail = false;
       int n_iter = k_it + 1;
-      if (!o.ok || !isfinite(tsum(o.F))) {
+      if (!o.ok || unib(!isfinite(tsum(o.F)))) {
         fail = true;
       } else if (cv) {
         const double dy = team_solve(mystep, cc * o.F - psi - d, nz, L) * L.sx[0];
         st[ST_NEWTON]++;
         const double dy_norm = team_rms(dy, iscale, nz, L);
-        if (k_it > 0 && dy_norm > CV_RDIV * dy_norm_old) {
+        if (k_it > 0 && unib(dy_norm > CV_RDIV * dy_norm_old)) {
           fail = true;
         } else {
           if (k_it > 0) crate = fmax(CV_CRDOWN * crate, dy_norm / dy_norm_old);
           ynew += dy;
           d += dy;
-          if (dy_norm * fmin(1.0, crate) * (1.0 / (order + 1)) <= CV_NLSCOEF) {
+          if (unib(dy_norm * fmin(1.0, crate) * (1.0 / (order + 1)) <= CV_NLSCOEF)) {
             conv = true;
           } else {
             dy_norm_old = dy_norm;
@@ -994,12 +1002,12 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         st[ST_NEWTON]++;
         const double dy_norm = team_rms(dy, iscale, nz, L);
         const double rate = dy_norm_old >= 0.0 ? dy_norm / dy_norm_old : -1.0;
-        if (rate >= 0.0 && (rate >= 1.0 || pow(rate, (double)(NEWTON_MAXITER - k_it)) / (1.0 - rate) * dy_norm > newton_tol)) {
+        if (unib(rate >= 0.0 && (rate >= 1.0 || pow(rate, (double)(NEWTON_MAXITER - k_it)) / (1.0 - rate) * dy_norm > newton_tol))) {
           fail = true;
         } else {
           ynew += dy;
           d += dy;
-          if (dy_norm == 0.0 || (rate >= 0.0 && rate / (1.0 - rate) * dy_norm < newton_tol)) {
+          if (unib(dy_norm == 0.0 || (rate >= 0.0 && rate / (1.0 - rate) * dy_norm < newton_tol))) {
             conv = true;
           } else {
             dy_norm_old = dy_norm;
@@ -1026,7 +1034,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         safety = 0.9 * (2 * NEWTON_MAXITER + 1) / (double)(2 * NEWTON_MAXITER + n_iter);
         isc = 1.0 / (atol + rtol * fabs(ynew));
         err_norm = team_rms(d * L.rcp[order + 1], isc, nz, L);
-        if (err_norm > 1.0) {
+        if (unib(err_norm > 1.0)) {
           st[ST_REJECT]++;
           const double factor = fmax(MIN_FACTOR, safety * step_pow(err_norm, -L.rcp[order + 1]));
           h_abs *= factor;
@@ -1072,7 +1080,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
 #pragma unroll
         for (int q = 0; q < 3; ++q) {
           const double fct = norms[q] == 0.0 ? INFINITY : step_pow(norms[q], -L.rcp[order + q]);
-          if (fct > best) { best = fct; which = q; }
+          if (unib(fct > best)) { best = fct; which = q; }
         }
         order += which - 1;
         const double factor = fmin(MAX_FACTOR, safety * best);
@@ -1081,13 +1089,13 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         n_equal = 0;
         if (!cv) have_lu = false;
       } else if (go == GO_STEP) {
-        if (!(t < t_end)) {
+        if (unib(!(t < t_end))) {
           st[ST_ORDER] = order;
           return 0;
         }
         if (st[ST_STEPS] >= O.max_steps) return 2;
         min_step = 10.0 * fabs(nextafter(t, INFINITY) - t);
-        if (h_abs < min_step) {
+        if (unib(h_abs < min_step)) {
           change_D(D, order, min_step / h_abs);
           h_abs = min_step;
           n_equal = 0;
@@ -1096,9 +1104,9 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         err_norm = 0.0;
         go = GO_ATTEMPT;
       } else if (go == GO_ATTEMPT) {
-        if (h_abs < min_step) return 1;
+        if (unib(h_abs < min_step)) return 1;
         t_new = t + h_abs;
-        if (t_new - t_end > 0) {
+        if (unib(t_new - t_end > 0)) {
           t_new = t_end;
           change_D(D, order, fabs(t_new - t) / h_abs);
           n_equal = 0;
@@ -1123,7 +1131,7 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         cc = h / alpha_o;
         go = GO_NEWTON;
       } else {  // GO_NEWTON: (re)start the Newton iteration from the predictor
-        if (cv && have_lu && (fabs(cc / c_lu - 1.0) > CV_DGMAX || lu_age >= CV_MSBP)) have_lu = false;
+        if (cv && have_lu && (unib(fabs(cc / c_lu - 1.0) > CV_DGMAX) || lu_age >= CV_MSBP)) have_lu = false;
         if (!have_lu) {
           mystep = team_inv(cc, nz, L, Jg);
           st[ST_LU]++;
@@ -1237,7 +1245,9 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2, 2))) voi
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x & (TEAM - 1);
   const bool h0 = half() == 0;
-  const int team = threadIdx.x / WAVE;
+  // the wavefront's team index, in a scalar register: the team's LDS addresses derived from it are
+  // scalars too (the compiler cannot see that threadIdx.x / 64 is uniform)
+  const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);
   const size_t sh = shared_prog_bytes(P);
   load_shared_prog(P, smem);
   __syncthreads();
@@ -1288,7 +1298,7 @@ __global__ __launch_bounds__(512) void jac_kernel(Prog P, int mode, long n, long
   for (long cell = (long)blockIdx.x * tpb + team; cell < n; cell += stride) {
     const double y = lane < nz ? X[(long)lane * ld + cell] : 0.0;
     const double p = mode == 1 ? pres[cell] : 0.0;
-    const RhsOut o = team_rhs(P, mode, p, y, L, JAC);
+    const RhsOut o = team_rhs<JAC>(P, mode, p, y, L);
     if (!o.ok) {
       if (h0 && lane < nz) {
         F[(long)lane * ld + cell] = -1.0;
@@ -1347,7 +1357,7 @@ __global__ __launch_bounds__(512) void verify_kernel(Prog P, long n, long ld, do
     if (pass) {
       const double y0 = lane < nz ? X[(long)lane * ld + cell] : 0.0;
       const double yp = lane < nz ? Xp[(long)lane * ldp + cell] : 0.0;
-      const RhsOut o = team_rhs(P, 1, pres[cell], yp, L, true);
+      const RhsOut o = team_rhs<true>(P, 1, pres[cell], yp, L);
       pass = o.ok;
       if (pass) {
         team_jac(P, 1, yp, L, o, Jg);
