@@ -1286,7 +1286,7 @@ __global__ __launch_bounds__(512) void jac_kernel(Prog P, int mode, long n, long
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x & (TEAM - 1);
   const bool h0 = half() == 0;
-  const int team = threadIdx.x / WAVE;
+  const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);  // scalar: see bdf_kernel
   const int tpb = blockDim.x / WAVE;
   const size_t sh = shared_prog_bytes(P);
   load_shared_prog(P, smem);
@@ -1343,7 +1343,7 @@ __global__ __launch_bounds__(512) void verify_kernel(Prog P, long n, long ld, do
   extern __shared__ __align__(16) char smem[];
   const int lane = threadIdx.x & (TEAM - 1);
   const bool h0 = half() == 0;
-  const int team = threadIdx.x / WAVE;
+  const int team = __builtin_amdgcn_readfirstlane(threadIdx.x / WAVE);  // scalar: see bdf_kernel
   const int tpb = blockDim.x / WAVE;
   const size_t sh = shared_prog_bytes(P);
   load_shared_prog(P, smem);
