@@ -74,12 +74,13 @@ __device__ __forceinline__ double swz16(double v) {
 constexpr int DPP_X1 = 0xB1, DPP_X2 = 0x4E, DPP_HMIR = 0x141, DPP_MIR = 0x140;
 
 // Team sum, bit-identical on every lane: each level adds the same two rounded values (a + b ==
-// b + a exactly).  The register barrier rounds v first; without it the compiler may contract a
+// b + a exactly).  The register barrier (an empty asm the compiler cannot see through, not volatile
+// so that it does not bound the instruction scheduler) rounds v first; without it the compiler may contract a
 // caller's product into the first addition, fma(x_i, y_i, partner's x_j y_j), which differs between
 // partner lanes, and step sizes, orders and branch decisions would then diverge inside one team.
 // Every call site runs with all 32 lanes of the team active.
 __device__ __forceinline__ double tsum(double v) {
-  asm volatile("" : "+v"(v));
+  asm("" : "+v"(v));
   v += dpp<DPP_X1>(v);
   v += dpp<DPP_X2>(v);
   v += dpp<DPP_HMIR>(v);
@@ -392,7 +393,6 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   const double T = tbc(x, ns);
   if (!(T >= (double)P.Th_min && T < (double)P.Th_max && T >= (double)P.Tk_min && T < (double)P.Tk_max)) return o;
   const bool sp = lane < ns;
-  const double Wk = L.W[lane];
   const double Rk = L.Rk[lane];
   // constant volume clips rho at 0 as rhs_native does (Lib_Chemistry_rhs.f90:37-39); the
   // constant-pressure model keeps Y as it is, so its right-hand side stays smooth through Y = 0
@@ -419,14 +419,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   const double Td = T - (double)T_i;
   const double* __restrict__ rk0 = P.tabk + (long)(T_i - P.Tk_min) * P.wk;
   const double* __restrict__ rk1 = rk0 + P.wk;
-  // thermo rows and the first reaction record do not depend on the sums below: issue them now
-  const double* __restrict__ rh0 = P.tabh + (long)(T_i - P.Th_min) * (2 * ns);
-  const double* __restrict__ rh1 = rh0 + 2 * ns;
-  double2 th0 = make_double2(0.0, 0.0), th1 = th0;
-  if (sp) {
-    th0 = ld2(rh0 + 2 * lane);
-    th1 = ld2(rh1 + 2 * lane);
-  }
+  // the first reaction record does not depend on the sums below: issue it now
   const int wl = threadIdx.x & (WAVE - 1);
   const int nslot = P.R * TEAM;
   int4 rec = wl < nslot ? L.rrec[wl] : make_int4(7, 0, 0, 0);
@@ -482,6 +475,15 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
       L.part[c * TEAM + lane] = acc;
     }
   }
+  // thermo rows, only needed at the end: loaded here, their latency hidden by the production sums
+  // (held through the reaction rounds they would cost those rounds registers)
+  const double* __restrict__ rh0 = P.tabh + (long)(T_i - P.Th_min) * (2 * ns);
+  const double* __restrict__ rh1 = rh0 + 2 * ns;
+  double2 th0 = make_double2(0.0, 0.0), th1 = th0;
+  if (sp) {
+    th0 = ld2(rh0 + 2 * lane);
+    th1 = ld2(rh1 + 2 * lane);
+  }
   tsync();
   double w = 0.0;
   if (sp) {
@@ -492,7 +494,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
       if (slot < 0) break;
       acc = acc + L.part[slot];
     }
-    w = Wk * acc;
+    w = L.W[lane] * acc;
   }
   o.w = w;
   o.cp = o.h = o.dcp = o.dh = 0.0;
