@@ -488,11 +488,24 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   double w = 0.0;
   if (sp) {
     double acc = 0.0;
+    if (P.MAXP <= 8) {
+      // all slots read at once, then all partial sums (slot -1 reads slot 0 and is not added):
+      // two LDS round trips instead of two per chunk, the same additions in the same order
+      int sl[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) sl[u] = u < P.MAXP ? L.kpart[u * TEAM + lane] : -1;
+      double pv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) pv[u] = u < P.MAXP ? L.part[max(sl[u], 0)] : 0.0;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) acc = sl[u] >= 0 ? acc + pv[u] : acc;
+    } else {
 #pragma unroll 1
-    for (int i = 0; i < P.MAXP; ++i) {
-      const int slot = L.kpart[i * TEAM + lane];
-      if (slot < 0) break;
-      acc = acc + L.part[slot];
+      for (int i = 0; i < P.MAXP; ++i) {
+        const int slot = L.kpart[i * TEAM + lane];
+        if (slot < 0) break;
+        acc = acc + L.part[slot];
+      }
     }
     w = L.W[lane] * acc;
   }
