@@ -88,6 +88,8 @@ void orc_laminar_viscosity(int ns, const double *W, int Tt_min, int Tt_max, cons
 void orc_bdf(const orc_mech *m, long n, long ld, int mode, double *X, const double *p, double rtol, double atol,
              double t_end, int max_steps, int *stats, int *status);
 /* orc_bdf with a Newton policy: 0 as scipy BDF (= orc_bdf), 1 as SUNDIALS CVODE (oracle_bdf.c) */
+/* The BDF's step-size factor x^e: method 0 pow, method 1 the GPU kernel's FP64 series. */
+double orc_step_pow(double x, double e, int method);
 void orc_bdf_method(const orc_mech *m, long n, long ld, int mode, double *X, const double *p, double rtol,
                     double atol, double t_end, int max_steps, int *stats, int *status, int method);
 

@@ -9,7 +9,8 @@
  * when Newton fails, error control in the RMS norm of e / (atol + rtol |y|), and the step/order
  * selection of that paper.  The GPU path must reproduce this algorithm; the oracle is pinned by
  * tight reference solutions (scipy Radau at rtol 1e-12, tests/test_bdf.py) and by invariants
- * (mass, elements, and internal energy / enthalpy). */
+ * (mass, elements, and internal energy / enthalpy).  The step-size factor series of Newton
+ * policy 1 (step_pow) is pinned against libm's pow (tests/test_bdf.py). */
 #include <float.h>
 #include <math.h>
 #include <stdlib.h>
@@ -65,6 +66,9 @@ static double step_pow(double x, double e, int method) {
   q = fma(q, y, 1.0);
   return ldexp(q, (int)n);
 }
+
+/* exported for tests/test_bdf.py, which pins the series against pow */
+double orc_step_pow(double x, double e, int method) { return step_pow(x, e, method); }
 
 static double rms(const double *v, const double *scale, int n) {
   double s = 0.0;

@@ -139,3 +139,28 @@ def test_cvode_newton_policy_invariants_and_accuracy(oracle_mod, mechs):
     assert np.median(dC) < 1e-8 * np.median(R[-1]) * 10 and np.median(dA) < 1e-8 * np.median(R[-1]) * 10
     assert np.quantile(dC, 0.75) <= 10.0 * np.quantile(dA, 0.75) + 1e-6
     assert sc[:, 3].mean() < 0.85 * sa[:, 3].mean() and sc[:, 5].mean() < 0.7 * sa[:, 5].mean()
+
+
+def test_step_factor_series_against_pow(oracle_mod):
+    """The step-size factor x^e that the CVODE-policy BDF (GPU kernel and oracle alike) evaluates
+    by its FP64 log2/exp2 series, against libm's pow: within 1e-12 relative over the whole range
+    of error norms (x from 1e-300 to 1e300, e = -1/k for the orders' k = 1..7), exact at x = 1,
+    and pow's values at x = 0 (inf) and x = inf (0).  The series reaches 7.6e-13; dropping its
+    last logarithm term gives 1.8e-11 and its last exponential term 9.4e-12 (earlier terms far
+    more), so the bar catches any dropped or wrong term."""
+    import ctypes as C
+    L = oracle_mod.lib()
+    f = L.orc_step_pow
+    f.restype = C.c_double
+    f.argtypes = [C.c_double, C.c_double, C.c_int]
+    rng = np.random.default_rng(7)
+    xs = np.concatenate([10.0 ** rng.uniform(-300, 300, 4000), rng.uniform(0.5, 2.0, 2000),
+                         2.0 ** np.arange(-60, 61), [np.sqrt(0.5), np.sqrt(2.0), 1.0 - 2**-52, 1.0 + 2**-52]])
+    for k in range(1, 8):
+        e = -1.0 / k
+        got = np.array([f(x, e, 1) for x in xs])
+        ref = np.power(xs, e)
+        assert np.max(np.abs(got / ref - 1.0)) < 1e-12
+        assert f(1.0, e, 1) == 1.0
+        assert f(0.0, e, 1) == np.inf and f(np.inf, e, 1) == 0.0
+        assert f(0.25, e, 0) == 0.25 ** e  # method 0 is pow itself
