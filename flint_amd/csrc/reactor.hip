@@ -148,6 +148,16 @@ __device__ __forceinline__ void tsync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// The lane's index in the wavefront, read where it is used.  threadIdx.x is live through the whole
+// kernel, so lane-derived LDS addresses computed from it are loop invariants that the compiler
+// hoists out of the integrator's loop and then has to keep: across an evaluation they spilled to
+// scratch.  A fresh read (an opaque v_mbcnt pair) makes them cheap values computed again per call.
+__device__ __forceinline__ int wave_lane() {
+  int l;
+  asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+  return l;
+}
+
 struct Lds {
   // block-shared copy of the program (read-only after the kernel's first barrier)
   const int4* rrec;   // [R*32] reaction records
@@ -388,7 +398,8 @@ template <bool jac>
 __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, double x, const Lds& L) {
   RhsOut o;
   o.ok = false;
-  const int lane = threadIdx.x & (TEAM - 1);
+  const int wlf = wave_lane();
+  const int lane = wlf & (TEAM - 1);
   const int ns = P.ns;
   const double T = tbc(x, ns);
   if (!(T >= (double)P.Th_min && T < (double)P.Th_max && T >= (double)P.Tk_min && T < (double)P.Tk_max)) return o;
@@ -412,7 +423,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   L.c[lane] = sp ? roi * L.Wi[lane] : (lane == TEAM - 1 ? 1.0 : 0.0);
   if (lane == 0) L.q[P.nr] = 0.0;
   if (jac) {
-    const int wl0 = threadIdx.x & (WAVE - 1);
+    const int wl0 = wlf;
     for (int i = wl0; i < 32 * 33; i += WAVE) L.lu[i] = 0.0;
   }
   const int T_i = (int)T;
@@ -420,7 +431,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   const double* __restrict__ rk0 = P.tabk + (long)(T_i - P.Tk_min) * P.wk;
   const double* __restrict__ rk1 = rk0 + P.wk;
   // the first reaction record does not depend on the sums below: issue it now
-  const int wl = threadIdx.x & (WAVE - 1);
+  const int wl = wlf;
   const int nslot = P.R * TEAM;
   int4 rec = wl < nslot ? L.rrec[wl] : make_int4(7, 0, 0, 0);
   tsync();
@@ -463,7 +474,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   // net production: chunk sums in reaction order, then the chunks of species k in order
 #pragma unroll 1
   for (int c2 = 0; c2 < P.C; c2 += 2) {
-    const int c = c2 + half();
+    const int c = c2 + ((wlf >> 5) & 1);
     if (c < P.C) {
       const int2* __restrict__ ct = L.cterm + c * CW * TEAM + lane;
       int2 tm[CW];
@@ -678,8 +689,9 @@ __device__ __forceinline__ void team_jac(const Prog& P, int mode, double x, cons
 // the solves.  Rows and columns beyond nz are the identity and take no part.  Returns step(lane).
 __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const double* __restrict__ Jg) {
   constexpr int HC = TEAM / 2;  // columns per half
-  const int lane = threadIdx.x & (TEAM - 1);
-  const int h = half(), c0 = HC * h;
+  const int wlf = wave_lane();
+  const int lane = wlf & (TEAM - 1);
+  const int h = (wlf >> 5) & 1, c0 = HC * h;
   const double* __restrict__ jrow = Jg + lane * 33;
   double r[HC];
 #pragma unroll
