@@ -283,13 +283,42 @@ __device__ __forceinline__ void jadd(double* p, double v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
 }
 
+// The kinetics-table values record rec reads: rows T_i (a0, b0) and T_i + 1 (a1, b1), columns col
+// to col + 3 as its kind uses them.  Loaded ahead for the first round of rates, whose latency then
+// overlaps the third-body sums.
+struct Tab {
+  double2 a0, a1, b0, b1;
+};
+__device__ __forceinline__ Tab tab_load(const int4 rec, const double* __restrict__ rk0, const double* __restrict__ rk1) {
+  Tab t;
+  const int kind = rec.x & 7;
+  const int col = rec.y;
+  t.a0 = t.a1 = t.b0 = t.b1 = make_double2(0.0, 0.0);
+  if (kind <= 1) {
+    t.a0 = ld2(rk0 + col);
+    t.a1 = ld2(rk1 + col);
+  } else if (kind == 2) {
+    t.a0 = ld2(rk0 + col);
+    t.b0 = ld2(rk0 + col + 2);
+    t.a1 = ld2(rk1 + col);
+    t.b1 = ld2(rk1 + col + 2);
+  } else if (kind == 3) {
+    t.a0 = make_double2(rk0[col], rk0[col + 1]);
+    t.b0.x = rk0[col + 2];
+    t.a1 = make_double2(rk1[col], rk1[col + 1]);
+    t.b1.x = rk1[col + 2];
+  }
+  return t;
+}
+
 // Rates of progress of the reaction in record rec (kinds: 0 Arrhenius, 1 three-body, 2 Troe,
 // 3 Lindemann), Lib_Chemistry_falloff.f90:12-80 for the falloff forms.  With JAC the reaction's
 // derivatives are scattered into the Jacobian rows in L.lu (zeroed by the caller): slot is the
-// record's index, which also indexes its net-production terms L.rprod.
-template <bool JAC>
+// record's index, which also indexes its net-production terms L.rprod.  With PRE the table values
+// come from tv (tab_load) instead of being loaded here.
+template <bool JAC, bool PRE = false>
 __device__ __forceinline__ void reaction(const int4 rec, const double* __restrict__ rk0, const double* __restrict__ rk1,
-                                         double Td, const Lds& L, int slot) {
+                                         double Td, const Lds& L, int slot, const Tab tv = Tab{}) {
   const int kind = rec.x & 7;
   const int ms = (rec.x >> 3) - 1;
   const int col = rec.y;
@@ -300,7 +329,7 @@ __device__ __forceinline__ void reaction(const int4 rec, const double* __restric
   const double Pb = L.c[(fs >> 15) & 31] * L.c[(fs >> 20) & 31] * L.c[(fs >> 25) & 31];
   double k1, k2, dk1T = 0.0, dk2T = 0.0, dqM = 0.0;
   if (kind <= 1) {
-    const double2 a0 = ld2(rk0 + col), a1 = ld2(rk1 + col);
+    const double2 a0 = PRE ? tv.a0 : ld2(rk0 + col), a1 = PRE ? tv.a1 : ld2(rk1 + col);
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double Mf = kind == 1 ? M : 1.0;
     k1 = kf * Mf;
@@ -314,15 +343,19 @@ __device__ __forceinline__ void reaction(const int4 rec, const double* __restric
     const bool troe = kind == 2;
     double kinf, k0, kc, Fc = 1.0, skinf, sk0, skc, sFc = 0.0;
     if (troe) {
-      const double2 x0 = ld2(rk0 + col), y0 = ld2(rk0 + col + 2), x1 = ld2(rk1 + col), y1 = ld2(rk1 + col + 2);
+      const double2 x0 = PRE ? tv.a0 : ld2(rk0 + col), y0 = PRE ? tv.b0 : ld2(rk0 + col + 2);
+      const double2 x1 = PRE ? tv.a1 : ld2(rk1 + col), y1 = PRE ? tv.b1 : ld2(rk1 + col + 2);
       kinf = x0.x + (x1.x - x0.x) * Td; skinf = x1.x - x0.x;
       k0 = x0.y + (x1.y - x0.y) * Td; sk0 = x1.y - x0.y;
       kc = y0.x + (y1.x - y0.x) * Td; skc = y1.x - y0.x;
       Fc = y0.y + (y1.y - y0.y) * Td; sFc = y1.y - y0.y;
     } else {
-      kinf = rk0[col] + (rk1[col] - rk0[col]) * Td; skinf = rk1[col] - rk0[col];
-      k0 = rk0[col + 1] + (rk1[col + 1] - rk0[col + 1]) * Td; sk0 = rk1[col + 1] - rk0[col + 1];
-      kc = rk0[col + 2] + (rk1[col + 2] - rk0[col + 2]) * Td; skc = rk1[col + 2] - rk0[col + 2];
+      const double u0 = PRE ? tv.a0.x : rk0[col], u1 = PRE ? tv.a1.x : rk1[col];
+      const double v0 = PRE ? tv.a0.y : rk0[col + 1], v1 = PRE ? tv.a1.y : rk1[col + 1];
+      const double w0 = PRE ? tv.b0.x : rk0[col + 2], w1 = PRE ? tv.b1.x : rk1[col + 2];
+      kinf = u0 + (u1 - u0) * Td; skinf = u1 - u0;
+      k0 = v0 + (v1 - v0) * Td; sk0 = v1 - v0;
+      kc = w0 + (w1 - w0) * Td; skc = w1 - w0;
     }
     const double Pr = k0 * M / kinf;
     const double gg = Pr / (1.0 + Pr);
@@ -430,10 +463,13 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   const double Td = T - (double)T_i;
   const double* __restrict__ rk0 = P.tabk + (long)(T_i - P.Tk_min) * P.wk;
   const double* __restrict__ rk1 = rk0 + P.wk;
-  // the first reaction record does not depend on the sums below: issue it now
+  // the first round's records (and, for a plain evaluation, their table values) do not depend on
+  // the sums below: issue them now
   const int wl = wlf;
   const int nslot = P.R * TEAM;
   int4 rec = wl < nslot ? L.rrec[wl] : make_int4(7, 0, 0, 0);
+  Tab tv0;
+  if (!jac) tv0 = tab_load(rec, rk0, rk1);  // the first round's table values, ahead
   tsync();
   // third-body sums M_s = sum_j eff_sj c_j.  Up to 16 sets: set s = wl & 15 in four lane groups
   // g = wl / 16, group g summing the species j = g (mod 4), the four partial sums then added as
@@ -460,8 +496,15 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   }
   tsync();
   // rates of progress, one reaction per lane and round; the next round's record is loaded ahead
+  int i0 = 0;
+  if (!jac) {  // the first round from the values loaded ahead
+    const int4 cur = rec;
+    rec = WAVE + wl < nslot ? L.rrec[WAVE + wl] : make_int4(7, 0, 0, 0);
+    if ((cur.x & 7) != 7) reaction<false, true>(cur, rk0, rk1, Td, L, 0, tv0);
+    i0 = 1;
+  }
 #pragma unroll 1
-  for (int i = 0; i * WAVE < nslot; ++i) {
+  for (int i = i0; i * WAVE < nslot; ++i) {
     const int4 cur = rec;
     const int nxt = (i + 1) * WAVE + wl;
     rec = nxt < nslot ? L.rrec[nxt] : make_int4(7, 0, 0, 0);
