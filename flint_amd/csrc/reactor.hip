@@ -54,8 +54,9 @@ enum { ST_STEPS, ST_REJECT, ST_NFAIL, ST_RHS, ST_JAC, ST_LU, ST_NEWTON, ST_ORDER
 
 // Cross-lane exchanges inside a 32-lane team without the LDS crossbar where possible: DPP quad
 // permutations (lane ^ 1, lane ^ 2) and row mirrors (lane i <-> 7 - i within 8, i <-> 15 - i within
-// 16), and a bit-mode swizzle for lane ^ 16 (all verified on gfx950).  Each level pairs disjoint
-// groups, so five levels combine all 32 lanes.
+// 16), and for lane ^ 16 a permlane16 swap (sums, maxima) or a bit-mode swizzle (where the partner
+// must be told apart), all verified on gfx950.  Each level pairs disjoint groups, so five levels
+// combine all 32 lanes.
 template <int CTRL>
 __device__ __forceinline__ double dpp(double v) {
   const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
@@ -71,6 +72,15 @@ __device__ __forceinline__ double swz16(double v) {
   const int hi = __builtin_amdgcn_ds_swizzle(__double2hiint(v), 0x401F);
   return __hiloint2double(hi, lo);
 }
+// (this lane's value, lane ^ 16's value) in some order, the same two values on both lanes: a
+// v_permlane16_swap of v with itself (lanes of even 16-lane rows receive their partner in the
+// second value, lanes of odd rows in the first; verified on gfx950).  A VALU exchange, where the
+// swizzle goes through the LDS crossbar; sums and maxima need no order.
+__device__ __forceinline__ double2 pair16(double v) {
+  const auto a = __builtin_amdgcn_permlane16_swap((unsigned)__double2loint(v), (unsigned)__double2loint(v), false, false);
+  const auto b = __builtin_amdgcn_permlane16_swap((unsigned)__double2hiint(v), (unsigned)__double2hiint(v), false, false);
+  return make_double2(__hiloint2double((int)b[0], (int)a[0]), __hiloint2double((int)b[1], (int)a[1]));
+}
 constexpr int DPP_X1 = 0xB1, DPP_X2 = 0x4E, DPP_HMIR = 0x141, DPP_MIR = 0x140;
 
 // Team sum, bit-identical on every lane: each level adds the same two rounded values (a + b ==
@@ -85,8 +95,8 @@ __device__ __forceinline__ double tsum(double v) {
   v += dpp<DPP_X2>(v);
   v += dpp<DPP_HMIR>(v);
   v += dpp<DPP_MIR>(v);
-  v += swz16(v);
-  return v;
+  const double2 q = pair16(v);
+  return q.x + q.y;  // v + (lane ^ 16's v), in either order the same bits
 }
 
 // Team argmax of (best, lane), ties to the lower lane: the same pairing levels as tsum, with
@@ -121,7 +131,8 @@ __device__ __forceinline__ int tpivot(double v, bool eligible) {
   key = max(key, (unsigned)dppi<DPP_X2>((int)key));
   key = max(key, (unsigned)dppi<DPP_HMIR>((int)key));
   key = max(key, (unsigned)dppi<DPP_MIR>((int)key));
-  key = max(key, (unsigned)__builtin_amdgcn_ds_swizzle((int)key, 0x401F));
+  const auto kp = __builtin_amdgcn_permlane16_swap(key, key, false, false);
+  key = max((unsigned)kp[0], (unsigned)kp[1]);
   return 32 - (int)(key & 31u) - ((key & 31u) == 0u ? 32 : 0);
 }
 
@@ -484,7 +495,8 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
         if (j < ns) m = m + (double)L.effT[j * nms + sset] * L.c[j];
       }
     }
-    m = m + swz16(m);
+    const double2 mq = pair16(m);
+    m = mq.x + mq.y;
     const double2 hm = halves(m);
     if (wl < nms) L.Mv[wl] = hm.x + hm.y;
   } else if (lane < P.nms) {
