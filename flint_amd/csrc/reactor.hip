@@ -489,10 +489,20 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
     const int nms = P.nms, g = wl >> 4, sset = wl & 15;
     double m = 0.0;
     if (sset < nms) {
-#pragma unroll 2
+      // all sixteen operands read first (the efficiency row index clamped into the table), the
+      // species guard a select: guarded loads would wait one after another
+      float ev[8];
+      double cv8[8];
+#pragma unroll
       for (int t = 0; t < 8; ++t) {
         const int j = g + 4 * t;
-        if (j < ns) m = m + (double)L.effT[j * nms + sset] * L.c[j];
+        ev[t] = L.effT[min(j, ns - 1) * nms + sset];
+        cv8[t] = L.c[j];
+      }
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const double u = m + (double)ev[t] * cv8[t];
+        m = g + 4 * t < ns ? u : m;
       }
     }
     const double2 mq = pair16(m);
@@ -749,9 +759,12 @@ __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const do
   const int h = (wlf >> 5) & 1, c0 = HC * h;
   const double* __restrict__ jrow = Jg + lane * 33;
   double r[HC];
+  double jv[HC];  // the slot row read unguarded (it is always in bounds), the guard a select
+#pragma unroll
+  for (int j = 0; j < HC; ++j) jv[j] = jrow[c0 + j];
 #pragma unroll
   for (int j = 0; j < HC; ++j)
-    r[j] = (lane == c0 + j ? 1.0 : 0.0) - ((lane < nz && c0 + j < nz) ? c * jrow[c0 + j] : 0.0);
+    r[j] = (lane == c0 + j ? 1.0 : 0.0) - ((lane < nz && c0 + j < nz) ? c * jv[j] : 0.0);
   double* __restrict__ pb = L.vx;  // the pivot row of the current step, rotated like the registers
   int mystep = -1;
 #pragma unroll 1
@@ -798,10 +811,20 @@ __device__ __forceinline__ double team_solve(int mystep, double b, int nz, const
   const double* arow = L.lu + lane * 33;
   L.vb[mystep] = b;
   tsync();
+  // all operands read first and the guard applied as a select: a guarded load becomes a branch
+  // with its own wait, and sixteen of them would run one after another
+  double av[HC], bv[HC];
+#pragma unroll
+  for (int m = 0; m < HC; ++m) {
+    av[m] = arow[c0 + m];
+    bv[m] = L.vb[c0 + m];
+  }
   double yp = 0.0;
 #pragma unroll
-  for (int m = 0; m < HC; ++m)
-    if (c0 + m < nz) yp = yp + arow[c0 + m] * L.vb[c0 + m];
+  for (int m = 0; m < HC; ++m) {
+    const double t = yp + av[m] * bv[m];
+    yp = c0 + m < nz ? t : yp;
+  }
   const double2 yv = halves(yp);
   const double y = lane < nz ? yv.x + yv.y : b;
   L.vx[mystep] = y;
@@ -829,8 +852,10 @@ __device__ __forceinline__ void team_solve2(double c1, double b1, double c2, dou
   const double* __restrict__ jrow = Jg + lane * 33;
   double r[TEAM];
 #pragma unroll
+  for (int j = 0; j < TEAM; ++j) r[j] = jrow[j];  // unguarded (always in bounds), guard below
+#pragma unroll
   for (int j = 0; j < TEAM; ++j) {
-    const double jv = (lane < nz && j < nz) ? jrow[j] : 0.0;
+    const double jv = (lane < nz && j < nz) ? r[j] : 0.0;
     r[j] = (lane == j ? 1.0 : 0.0) - c * jv;
   }
   double be = lane < nz ? (h == 0 ? b1 : b2) : 0.0;
@@ -931,6 +956,9 @@ struct Dcol {
 // an absolute error of eps |y| in every difference, which later step-size increases amplify by
 // factor^j into a visible loss of mass after a long run of step halvings.
 __device__ __forceinline__ void change_D(const Dcol D, int order, double factor) {
+  double dv[MAXO + 1];  // every row read at once (in bounds), the order's guard a select
+#pragma unroll
+  for (int j = 1; j <= MAXO; ++j) dv[j] = D[j];
   double v[MAXO + 1];
 #pragma unroll
   for (int l = 1; l <= MAXO; ++l) {
@@ -938,7 +966,8 @@ __device__ __forceinline__ void change_D(const Dcol D, int order, double factor)
 #pragma unroll
     for (int j = 1; j <= MAXO; ++j) {
       prod *= (j - 1 - factor * l) * (1.0 / j);  // 1/j folds to a constant: no division
-      if (j <= order) acc += prod * D[j];
+      const double t = acc + prod * dv[j];
+      acc = j <= order ? t : acc;
     }
     v[l] = (l <= order) ? acc : 0.0;
   }
@@ -1198,12 +1227,19 @@ __device__ __forceinline__ int integrate(const Prog& P, const Opts& O, double p,
         h_abs = fabs(h);
         ypred = 0.0;
         psi = 0.0;
+        double dv[MAXO + 1];  // every row read at once, the order's guard a select
 #pragma unroll
-        for (int j = 0; j <= MAXO; ++j)
-          if (j <= order) ypred += D[j];
+        for (int j = 0; j <= MAXO; ++j) dv[j] = D[j];
 #pragma unroll
-        for (int j = 1; j <= MAXO; ++j)
-          if (j <= order) psi += D[j] * gamma[j];
+        for (int j = 0; j <= MAXO; ++j) {
+          const double t = ypred + dv[j];
+          ypred = j <= order ? t : ypred;
+        }
+#pragma unroll
+        for (int j = 1; j <= MAXO; ++j) {
+          const double t = psi + dv[j] * gamma[j];
+          psi = j <= order ? t : psi;
+        }
         double alpha_o = 0.0;
 #pragma unroll
         for (int j = 1; j <= MAXO; ++j)
