@@ -433,6 +433,13 @@ __device__ __forceinline__ void reaction(const int4 rec, const double* __restric
   }
 }
 
+// Record s of the program, or an empty one (kind 7) past the end: the read unguarded (index
+// clamped into the table), the guard a select.
+__device__ __forceinline__ int4 rec_at(const Lds& L, int s, int nslot) {
+  const int4 r = L.rrec[min(s, nslot - 1)];
+  return s < nslot ? r : make_int4(7, 0, 0, 0);
+}
+
 // Team right-hand side; o.ok is false (team-uniform) when T leaves the tables.  With jac the
 // reactions' derivatives are accumulated in the Jacobian rows in LDS as well (team_jac finishes
 // them); the rows replace the inverse of I - cJ there, which a new Jacobian always renews.  jac is
@@ -464,7 +471,8 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
     o.rho = p / (o.Rmix * T);
     roi = o.rho * o.xc;
   }
-  L.c[lane] = sp ? roi * L.Wi[lane] : (lane == TEAM - 1 ? 1.0 : 0.0);
+  const double wi = L.Wi[lane];  // read unguarded (32 entries), used under the guard
+  L.c[lane] = sp ? roi * wi : (lane == TEAM - 1 ? 1.0 : 0.0);
   if (lane == 0) L.q[P.nr] = 0.0;
   if (jac) {
     const int wl0 = wlf;
@@ -478,7 +486,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   // the sums below: issue them now
   const int wl = wlf;
   const int nslot = P.R * TEAM;
-  int4 rec = wl < nslot ? L.rrec[wl] : make_int4(7, 0, 0, 0);
+  int4 rec = rec_at(L, wl, nslot);
   Tab tv0;
   if (!jac) tv0 = tab_load(rec, rk0, rk1);  // the first round's table values, ahead
   tsync();
@@ -521,7 +529,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   int i0 = 0;
   if (!jac) {  // the first round from the values loaded ahead
     const int4 cur = rec;
-    rec = WAVE + wl < nslot ? L.rrec[WAVE + wl] : make_int4(7, 0, 0, 0);
+    rec = rec_at(L, WAVE + wl, nslot);
     if ((cur.x & 7) != 7) reaction<false, true>(cur, rk0, rk1, Td, L, 0, tv0);
     i0 = 1;
   }
@@ -529,7 +537,7 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
   for (int i = i0; i * WAVE < nslot; ++i) {
     const int4 cur = rec;
     const int nxt = (i + 1) * WAVE + wl;
-    rec = nxt < nslot ? L.rrec[nxt] : make_int4(7, 0, 0, 0);
+    rec = rec_at(L, nxt, nslot);
     if ((cur.x & 7) != 7) {
       if (jac) reaction<true>(cur, rk0, rk1, Td, L, i * WAVE + wl);
       else reaction<false>(cur, rk0, rk1, Td, L, 0);
@@ -782,14 +790,19 @@ __device__ __forceinline__ int team_inv(double c, int nz, const Lds& L, const do
       for (int j = 1; j < HC; ++j) pb[c0 + j] = r[j] * iv;
     }
     tsync();
+    // the pivot row's half read whole before the update: read as the update consumes it, the
+    // compiler reuses a few registers and waits for every pair of values
+    double pv[HC];
+#pragma unroll
+    for (int j = 0; j < HC; ++j) pv[j] = pb[c0 + j];
     const double inv = pb[HC * hk];
     // pivot lane: a_pj / a_pk; other lanes: a_ij - a_ik (a_pj / a_pk); one fused form for both,
     // rotating by one register on the way
     const double keep = me ? 0.0 : 1.0, fm = me ? -1.0 : rk;
     const double nk = me ? inv : -rk * inv;
-    const double r0 = h == hk ? nk : keep * r[0] - fm * pb[c0];
+    const double r0 = h == hk ? nk : keep * r[0] - fm * pv[0];
 #pragma unroll
-    for (int j = 0; j < HC - 1; ++j) r[j] = keep * r[j + 1] - fm * pb[c0 + j + 1];
+    for (int j = 0; j < HC - 1; ++j) r[j] = keep * r[j + 1] - fm * pv[j + 1];
     r[HC - 1] = r0;
     tsync();
   }
