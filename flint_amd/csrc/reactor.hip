@@ -553,9 +553,13 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
       int2 tm[CW];
 #pragma unroll
       for (int t = 0; t < CW; ++t) tm[t] = ct[t * TEAM];
+      // every rate gathered before the sum: gathered inside it, each one is waited for in turn
+      double qv[CW];
+#pragma unroll
+      for (int t = 0; t < CW; ++t) qv[t] = L.q[tm[t].x];
       double acc = 0.0;
 #pragma unroll
-      for (int t = 0; t < CW; ++t) acc = acc + (double)__int_as_float(tm[t].y) * L.q[tm[t].x];
+      for (int t = 0; t < CW; ++t) acc = acc + (double)__int_as_float(tm[t].y) * qv[t];
       L.part[c * TEAM + lane] = acc;
     }
   }
@@ -577,10 +581,13 @@ __device__ __forceinline__ RhsOut team_rhs(const Prog& P, int mode, double p, do
       // two LDS round trips instead of two per chunk, the same additions in the same order
       int sl[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) sl[u] = u < P.MAXP ? L.kpart[u * TEAM + lane] : -1;
+      for (int u = 0; u < 8; ++u) {
+        const int v = L.kpart[min(u, P.MAXP - 1) * TEAM + lane];  // clamped into the table
+        sl[u] = u < P.MAXP ? v : -1;
+      }
       double pv[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) pv[u] = u < P.MAXP ? L.part[max(sl[u], 0)] : 0.0;
+      for (int u = 0; u < 8; ++u) pv[u] = L.part[max(sl[u], 0)];
 #pragma unroll
       for (int u = 0; u < 8; ++u) acc = sl[u] >= 0 ? acc + pv[u] : acc;
     } else {
@@ -894,8 +901,17 @@ __device__ __forceinline__ void team_solve2(double c1, double b1, double c2, dou
     // f = a_ik / a_pk (0 on the pivot row, which keeps its values); column k itself is not needed
     const double f = me ? 0.0 : rk * pb[0];
     be = be - f * *pbe;
+    // the pivot row read eight values at a time ahead of their use (one at a time, each read is
+    // waited for; all 32 at once do not fit beside the rows)
 #pragma unroll
-    for (int j = 0; j < TEAM - 1; ++j) r[j] = r[j + 1] - f * pb[j + 1];
+    for (int j0 = 0; j0 < TEAM; j0 += 8) {
+      double pv[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) pv[u] = pb[min(j0 + u + 1, TEAM - 1)];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (j0 + u < TEAM - 1) r[j0 + u] = r[j0 + u + 1] - f * pv[u];
+    }
     r[TEAM - 1] = 0.0;
     tsync();
   }
