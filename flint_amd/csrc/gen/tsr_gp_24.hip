@@ -145,8 +145,24 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   double w23;
   // JAC: dT-derivative accumulators
   [[maybe_unused]] double dT[NS] = {};
+  const double2 A0_1 = ld2(rk0 + 0), A1_1 = ld2(rk1 + 0);
+  const double2 A0_2 = ld2(rk0 + 2), A1_2 = ld2(rk1 + 2);
+  const double2 A0_3 = ld2(rk0 + 4), A1_3 = ld2(rk1 + 4);
+  const double2 A0_4 = ld2(rk0 + 6), A1_4 = ld2(rk1 + 6);
+  const double2 A0_5 = ld2(rk0 + 8), A1_5 = ld2(rk1 + 8);
+  const double2 A0_6 = ld2(rk0 + 10), A1_6 = ld2(rk1 + 10);
+  const double2 A0_7 = ld2(rk0 + 12), A1_7 = ld2(rk1 + 12);
+  const double2 A0_8 = ld2(rk0 + 14), A1_8 = ld2(rk1 + 14);
+  const double2 A0_9 = ld2(rk0 + 16), A1_9 = ld2(rk1 + 16);
+  const double2 A0_10 = ld2(rk0 + 18), A1_10 = ld2(rk1 + 18);
+  const double2 A0_11 = ld2(rk0 + 20), A1_11 = ld2(rk1 + 20);
+  const double2 A0_12 = ld2(rk0 + 22), A1_12 = ld2(rk1 + 22);
+  const double2 A0_13 = ld2(rk0 + 24), A1_13 = ld2(rk1 + 24);
+  const double2 A0_14 = ld2(rk0 + 26), A1_14 = ld2(rk1 + 26);
+  const double2 A0_15 = ld2(rk0 + 28), A1_15 = ld2(rk1 + 28);
+  const double2 A0_16 = ld2(rk0 + 30), A1_16 = ld2(rk1 + 30);
   {  // reaction 1: 2 O + M <=> O2 + M
-    const double2 a0 = ld2(rk0 + 0), a1 = ld2(rk1 + 0);
+    const double2 a0 = A0_1, a1 = A1_1;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c9*c9)*M0;
     const double pb = kb*(c18)*M0;
@@ -162,7 +178,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = 1.0 * q;
   }
   {  // reaction 2: H + O + M <=> OH + M
-    const double2 a0 = ld2(rk0 + 2), a1 = ld2(rk1 + 2);
+    const double2 a0 = A0_2, a1 = A1_2;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c9)*M1;
     const double pb = kb*(c3)*M1;
@@ -180,7 +196,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w9 = w9 + -1.0 * q;
   }
   {  // reaction 3: H2 + O <=> H + OH
-    const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
+    const double2 a0 = A0_3, a1 = A1_3;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[43 * SST] = kf; scr[44 * SST] = kb; }
     const double pf = kf*(c4)*(c9);
@@ -200,7 +216,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w9 = w9 + -1.0 * q;
   }
   {  // reaction 4: HO2 + O <=> O2 + OH
-    const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
+    const double2 a0 = A0_4, a1 = A1_4;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[45 * SST] = kf; scr[46 * SST] = kb; }
     const double pf = kf*(c0)*(c9);
@@ -220,7 +236,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + 1.0 * q;
   }
   {  // reaction 5: H2O2 + O <=> HO2 + OH
-    const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
+    const double2 a0 = A0_5, a1 = A1_5;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[47 * SST] = kf; scr[48 * SST] = kb; }
     const double pf = kf*(c1)*(c9);
@@ -240,7 +256,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w9 = w9 + -1.0 * q;
   }
   {  // reaction 6: CH2 + O <=> H + HCO
-    const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
+    const double2 a0 = A0_6, a1 = A1_6;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[49 * SST] = kf; scr[50 * SST] = kb; }
     const double pf = kf*(c19)*(c9);
@@ -260,7 +276,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = -1.0 * q;
   }
   {  // reaction 7: CH2(S) + O <=> CO + H2
-    const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
+    const double2 a0 = A0_7, a1 = A1_7;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[51 * SST] = kf; scr[52 * SST] = kb; }
     const double pf = kf*(c7)*(c9);
@@ -280,7 +296,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w9 = w9 + -1.0 * q;
   }
   {  // reaction 8: CH2(S) + O <=> H + HCO
-    const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
+    const double2 a0 = A0_8, a1 = A1_8;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[53 * SST] = kf; scr[54 * SST] = kb; }
     const double pf = kf*(c7)*(c9);
@@ -300,7 +316,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w9 = w9 + -1.0 * q;
   }
   {  // reaction 9: CH3 + O <=> CH2O + H
-    const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
+    const double2 a0 = A0_9, a1 = A1_9;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[55 * SST] = kf; scr[56 * SST] = kb; }
     const double pf = kf*(c11)*(c9);
@@ -320,7 +336,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = 1.0 * q;
   }
   {  // reaction 10: CH4 + O <=> CH3 + OH
-    const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
+    const double2 a0 = A0_10, a1 = A1_10;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[57 * SST] = kf; scr[58 * SST] = kb; }
     const double pf = kf*(c21)*(c9);
@@ -340,7 +356,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = -1.0 * q;
   }
   {  // reaction 11: CO + O + M <=> CO2 + M
-    const double2 a0 = ld2(rk0 + 20), a1 = ld2(rk1 + 20);
+    const double2 a0 = A0_11, a1 = A1_11;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c9)*M2;
     const double pb = kb*(c23)*M2;
@@ -358,7 +374,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = 1.0 * q;
   }
   {  // reaction 12: HCO + O <=> CO + OH
-    const double2 a0 = ld2(rk0 + 22), a1 = ld2(rk1 + 22);
+    const double2 a0 = A0_12, a1 = A1_12;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[59 * SST] = kf; scr[60 * SST] = kb; }
     const double pf = kf*(c5)*(c9);
@@ -378,7 +394,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w9 = w9 + -1.0 * q;
   }
   {  // reaction 13: HCO + O <=> CO2 + H
-    const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
+    const double2 a0 = A0_13, a1 = A1_13;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[61 * SST] = kf; scr[62 * SST] = kb; }
     const double pf = kf*(c5)*(c9);
@@ -398,7 +414,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + 1.0 * q;
   }
   {  // reaction 14: CH2O + O <=> HCO + OH
-    const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
+    const double2 a0 = A0_14, a1 = A1_14;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[63 * SST] = kf; scr[64 * SST] = kb; }
     const double pf = kf*(c13)*(c9);
@@ -418,7 +434,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + -1.0 * q;
   }
   {  // reaction 15: CH3O + O <=> CH2O + OH
-    const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
+    const double2 a0 = A0_15, a1 = A1_15;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[65 * SST] = kf; scr[66 * SST] = kb; }
     const double pf = kf*(c6)*(c9);
@@ -438,7 +454,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + 1.0 * q;
   }
   {  // reaction 16: C2H2 + O <=> C2H + OH
-    const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
+    const double2 a0 = A0_16, a1 = A1_16;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[67 * SST] = kf; scr[68 * SST] = kb; }
     const double pf = kf*(c12)*(c9);
@@ -457,8 +473,24 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w12 = -1.0 * q;
     w15 = 1.0 * q;
   }
+  const double2 A0_17 = ld2(rk0 + 32), A1_17 = ld2(rk1 + 32);
+  const double2 A0_18 = ld2(rk0 + 34), A1_18 = ld2(rk1 + 34);
+  const double2 A0_19 = ld2(rk0 + 36), A1_19 = ld2(rk1 + 36);
+  const double2 A0_20 = ld2(rk0 + 38), A1_20 = ld2(rk1 + 38);
+  const double2 A0_21 = ld2(rk0 + 40), A1_21 = ld2(rk1 + 40);
+  const double2 A0_22 = ld2(rk0 + 42), A1_22 = ld2(rk1 + 42);
+  const double2 A0_23 = ld2(rk0 + 44), A1_23 = ld2(rk1 + 44);
+  const double2 A0_24 = ld2(rk0 + 46), A1_24 = ld2(rk1 + 46);
+  const double2 A0_25 = ld2(rk0 + 48), A1_25 = ld2(rk1 + 48);
+  const double2 A0_26 = ld2(rk0 + 50), A1_26 = ld2(rk1 + 50);
+  const double2 A0_27 = ld2(rk0 + 52), A1_27 = ld2(rk1 + 52);
+  const double2 A0_28 = ld2(rk0 + 54), A1_28 = ld2(rk1 + 54);
+  const double2 A0_29 = ld2(rk0 + 56), A1_29 = ld2(rk1 + 56);
+  const double2 A0_30 = ld2(rk0 + 58), A1_30 = ld2(rk1 + 58);
+  const double2 A0_31 = ld2(rk0 + 60), A1_31 = ld2(rk1 + 60);
+  const double2 A0_32 = ld2(rk0 + 62), A1_32 = ld2(rk1 + 62);
   {  // reaction 17: C2H2 + O <=> CH2 + CO
-    const double2 a0 = ld2(rk0 + 32), a1 = ld2(rk1 + 32);
+    const double2 a0 = A0_17, a1 = A1_17;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[69 * SST] = kf; scr[70 * SST] = kb; }
     const double pf = kf*(c12)*(c9);
@@ -478,7 +510,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
   }
   {  // reaction 18: CO + O2 <=> CO2 + O
-    const double2 a0 = ld2(rk0 + 34), a1 = ld2(rk1 + 34);
+    const double2 a0 = A0_18, a1 = A1_18;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[71 * SST] = kf; scr[72 * SST] = kb; }
     const double pf = kf*(c2)*(c18);
@@ -498,7 +530,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + 1.0 * q;
   }
   {  // reaction 19: CH2O + O2 <=> HCO + HO2
-    const double2 a0 = ld2(rk0 + 36), a1 = ld2(rk1 + 36);
+    const double2 a0 = A0_19, a1 = A1_19;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[73 * SST] = kf; scr[74 * SST] = kb; }
     const double pf = kf*(c13)*(c18);
@@ -518,7 +550,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 20: H + O2 + M <=> HO2 + M
-    const double2 a0 = ld2(rk0 + 38), a1 = ld2(rk1 + 38);
+    const double2 a0 = A0_20, a1 = A1_20;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c18)*M3;
     const double pb = kb*(c0)*M3;
@@ -536,7 +568,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 21: H + 2 O2 <=> HO2 + O2
-    const double2 a0 = ld2(rk0 + 40), a1 = ld2(rk1 + 40);
+    const double2 a0 = A0_21, a1 = A1_21;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c18)*M4;
     const double pb = kb*(c0)*M4;
@@ -554,7 +586,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 22: H + H2O + O2 <=> H2O + HO2
-    const double2 a0 = ld2(rk0 + 42), a1 = ld2(rk1 + 42);
+    const double2 a0 = A0_22, a1 = A1_22;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c18)*M5;
     const double pb = kb*(c0)*M5;
@@ -572,7 +604,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 23: H + O2 <=> O + OH
-    const double2 a0 = ld2(rk0 + 44), a1 = ld2(rk1 + 44);
+    const double2 a0 = A0_23, a1 = A1_23;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[75 * SST] = kf; scr[76 * SST] = kb; }
     const double pf = kf*(c8)*(c18);
@@ -592,7 +624,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 24: 2 H + M <=> H2 + M
-    const double2 a0 = ld2(rk0 + 46), a1 = ld2(rk1 + 46);
+    const double2 a0 = A0_24, a1 = A1_24;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8*c8)*M6;
     const double pb = kb*(c4)*M6;
@@ -608,7 +640,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w8 = w8 + -2.0 * q;
   }
   {  // reaction 25: 2 H + H2 <=> 2 H2
-    const double2 a0 = ld2(rk0 + 48), a1 = ld2(rk1 + 48);
+    const double2 a0 = A0_25, a1 = A1_25;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8*c8)*M7;
     const double pb = kb*(c4)*M7;
@@ -624,7 +656,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w8 = w8 + -2.0 * q;
   }
   {  // reaction 26: 2 H + H2O <=> H2 + H2O
-    const double2 a0 = ld2(rk0 + 50), a1 = ld2(rk1 + 50);
+    const double2 a0 = A0_26, a1 = A1_26;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8*c8)*M5;
     const double pb = kb*(c4)*M5;
@@ -640,7 +672,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w8 = w8 + -2.0 * q;
   }
   {  // reaction 27: CO2 + 2 H <=> CO2 + H2
-    const double2 a0 = ld2(rk0 + 52), a1 = ld2(rk1 + 52);
+    const double2 a0 = A0_27, a1 = A1_27;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8*c8)*M8;
     const double pb = kb*(c4)*M8;
@@ -656,7 +688,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w8 = w8 + -2.0 * q;
   }
   {  // reaction 28: H + OH + M <=> H2O + M
-    const double2 a0 = ld2(rk0 + 54), a1 = ld2(rk1 + 54);
+    const double2 a0 = A0_28, a1 = A1_28;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c3)*M9;
     const double pb = kb*(c16)*M9;
@@ -674,7 +706,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = 1.0 * q;
   }
   {  // reaction 29: H + HO2 <=> H2O + O
-    const double2 a0 = ld2(rk0 + 56), a1 = ld2(rk1 + 56);
+    const double2 a0 = A0_29, a1 = A1_29;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[77 * SST] = kf; scr[78 * SST] = kb; }
     const double pf = kf*(c8)*(c0);
@@ -694,7 +726,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + 1.0 * q;
   }
   {  // reaction 30: H + HO2 <=> H2 + O2
-    const double2 a0 = ld2(rk0 + 58), a1 = ld2(rk1 + 58);
+    const double2 a0 = A0_30, a1 = A1_30;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c0);
     const double pb = kb*(c4)*(c18);
@@ -713,7 +745,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + 1.0 * q;
   }
   {  // reaction 31: H + HO2 <=> 2 OH
-    const double2 a0 = ld2(rk0 + 60), a1 = ld2(rk1 + 60);
+    const double2 a0 = A0_31, a1 = A1_31;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c0);
     const double pb = kb*(c3*c3);
@@ -730,7 +762,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w8 = w8 + -1.0 * q;
   }
   {  // reaction 32: H + H2O2 <=> H2 + HO2
-    const double2 a0 = ld2(rk0 + 62), a1 = ld2(rk1 + 62);
+    const double2 a0 = A0_32, a1 = A1_32;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c1);
     const double pb = kb*(c4)*(c0);
@@ -748,8 +780,17 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w4 = w4 + 1.0 * q;
     w8 = w8 + -1.0 * q;
   }
+  const double2 A0_33 = ld2(rk0 + 64), A1_33 = ld2(rk1 + 64);
+  const double2 A0_34 = ld2(rk0 + 66), A1_34 = ld2(rk1 + 66);
+  const double2 A0_35 = ld2(rk0 + 68), A1_35 = ld2(rk1 + 68);
+  const double2 A0_36 = ld2(rk0 + 70), A1_36 = ld2(rk1 + 70);
+  const double2 A0_37 = ld2(rk0 + 72), A1_37 = ld2(rk1 + 72);
+  const double2 A0_38 = ld2(rk0 + 74), A1_38 = ld2(rk1 + 74);
+  const double2 A0_39 = ld2(rk0 + 76), A1_39 = ld2(rk1 + 76);
+  const double2 A0_40 = ld2(rk0 + 78), A1_40 = ld2(rk1 + 78);
+  const double2 A0_41 = ld2(rk0 + 80), A1_41 = ld2(rk1 + 80);
   {  // reaction 33: H + H2O2 <=> H2O + OH
-    const double2 a0 = ld2(rk0 + 64), a1 = ld2(rk1 + 64);
+    const double2 a0 = A0_33, a1 = A1_33;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c1);
     const double pb = kb*(c16)*(c3);
@@ -846,7 +887,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + 1.0 * q;
   }
   {  // reaction 36: CH4 + H <=> CH3 + H2
-    const double2 a0 = ld2(rk0 + 66), a1 = ld2(rk1 + 66);
+    const double2 a0 = A0_34, a1 = A1_34;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c21)*(c8);
     const double pb = kb*(c11)*(c4);
@@ -904,7 +945,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + 1.0 * q;
   }
   {  // reaction 38: H + HCO <=> CO + H2
-    const double2 a0 = ld2(rk0 + 68), a1 = ld2(rk1 + 68);
+    const double2 a0 = A0_35, a1 = A1_35;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c5);
     const double pb = kb*(c2)*(c4);
@@ -962,7 +1003,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + -1.0 * q;
   }
   {  // reaction 40: CH2O + H <=> H2 + HCO
-    const double2 a0 = ld2(rk0 + 70), a1 = ld2(rk1 + 70);
+    const double2 a0 = A0_36, a1 = A1_36;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c13)*(c8);
     const double pb = kb*(c4)*(c5);
@@ -981,7 +1022,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + -1.0 * q;
   }
   {  // reaction 41: CH3O + H <=> CH2O + H2
-    const double2 a0 = ld2(rk0 + 72), a1 = ld2(rk1 + 72);
+    const double2 a0 = A0_37, a1 = A1_37;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c8);
     const double pb = kb*(c13)*(c4);
@@ -1000,7 +1041,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + 1.0 * q;
   }
   {  // reaction 42: CH3O + H <=> CH3 + OH
-    const double2 a0 = ld2(rk0 + 74), a1 = ld2(rk1 + 74);
+    const double2 a0 = A0_38, a1 = A1_38;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c8);
     const double pb = kb*(c11)*(c3);
@@ -1019,7 +1060,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w11 = w11 + 1.0 * q;
   }
   {  // reaction 43: CH3O + H <=> CH2(S) + H2O
-    const double2 a0 = ld2(rk0 + 76), a1 = ld2(rk1 + 76);
+    const double2 a0 = A0_39, a1 = A1_39;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c8);
     const double pb = kb*(c7)*(c16);
@@ -1116,7 +1157,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + 1.0 * q;
   }
   {  // reaction 46: H2 + OH <=> H + H2O
-    const double2 a0 = ld2(rk0 + 78), a1 = ld2(rk1 + 78);
+    const double2 a0 = A0_40, a1 = A1_40;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c4)*(c3);
     const double pb = kb*(c8)*(c16);
@@ -1172,7 +1213,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w3 = w3 + -2.0 * q;
   }
   {  // reaction 48: 2 OH <=> H2O + O
-    const double2 a0 = ld2(rk0 + 80), a1 = ld2(rk1 + 80);
+    const double2 a0 = A0_41, a1 = A1_41;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c3*c3);
     const double pb = kb*(c16)*(c9);
@@ -1188,8 +1229,24 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w9 = w9 + 1.0 * q;
     w16 = w16 + 1.0 * q;
   }
+  const double2 A0_42 = ld2(rk0 + 82), A1_42 = ld2(rk1 + 82);
+  const double2 A0_43 = ld2(rk0 + 84), A1_43 = ld2(rk1 + 84);
+  const double2 A0_44 = ld2(rk0 + 86), A1_44 = ld2(rk1 + 86);
+  const double2 A0_45 = ld2(rk0 + 88), A1_45 = ld2(rk1 + 88);
+  const double2 A0_46 = ld2(rk0 + 90), A1_46 = ld2(rk1 + 90);
+  const double2 A0_47 = ld2(rk0 + 92), A1_47 = ld2(rk1 + 92);
+  const double2 A0_48 = ld2(rk0 + 94), A1_48 = ld2(rk1 + 94);
+  const double2 A0_49 = ld2(rk0 + 96), A1_49 = ld2(rk1 + 96);
+  const double2 A0_50 = ld2(rk0 + 98), A1_50 = ld2(rk1 + 98);
+  const double2 A0_51 = ld2(rk0 + 100), A1_51 = ld2(rk1 + 100);
+  const double2 A0_52 = ld2(rk0 + 102), A1_52 = ld2(rk1 + 102);
+  const double2 A0_53 = ld2(rk0 + 104), A1_53 = ld2(rk1 + 104);
+  const double2 A0_54 = ld2(rk0 + 106), A1_54 = ld2(rk1 + 106);
+  const double2 A0_55 = ld2(rk0 + 108), A1_55 = ld2(rk1 + 108);
+  const double2 A0_56 = ld2(rk0 + 110), A1_56 = ld2(rk1 + 110);
+  const double2 A0_57 = ld2(rk0 + 112), A1_57 = ld2(rk1 + 112);
   {  // reaction 49: HO2 + OH <=> H2O + O2
-    const double2 a0 = ld2(rk0 + 82), a1 = ld2(rk1 + 82);
+    const double2 a0 = A0_42, a1 = A1_42;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c0)*(c3);
     const double pb = kb*(c16)*(c18);
@@ -1208,7 +1265,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + 1.0 * q;
   }
   {  // reaction 50: H2O2 + OH <=> H2O + HO2
-    const double2 a0 = ld2(rk0 + 84), a1 = ld2(rk1 + 84);
+    const double2 a0 = A0_43, a1 = A1_43;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c1)*(c3);
     const double pb = kb*(c16)*(c0);
@@ -1227,7 +1284,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + 1.0 * q;
   }
   {  // reaction 51: H2O2 + OH <=> H2O + HO2
-    const double2 a0 = ld2(rk0 + 86), a1 = ld2(rk1 + 86);
+    const double2 a0 = A0_44, a1 = A1_44;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c1)*(c3);
     const double pb = kb*(c16)*(c0);
@@ -1246,7 +1303,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + 1.0 * q;
   }
   {  // reaction 52: CH2 + OH <=> CH2O + H
-    const double2 a0 = ld2(rk0 + 88), a1 = ld2(rk1 + 88);
+    const double2 a0 = A0_45, a1 = A1_45;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c19)*(c3);
     const double pb = kb*(c13)*(c8);
@@ -1265,7 +1322,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + -1.0 * q;
   }
   {  // reaction 53: CH2(S) + OH <=> CH2O + H
-    const double2 a0 = ld2(rk0 + 90), a1 = ld2(rk1 + 90);
+    const double2 a0 = A0_46, a1 = A1_46;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c7)*(c3);
     const double pb = kb*(c13)*(c8);
@@ -1284,7 +1341,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + 1.0 * q;
   }
   {  // reaction 54: CH3 + OH <=> CH2 + H2O
-    const double2 a0 = ld2(rk0 + 92), a1 = ld2(rk1 + 92);
+    const double2 a0 = A0_47, a1 = A1_47;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c3);
     const double pb = kb*(c19)*(c16);
@@ -1303,7 +1360,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
   }
   {  // reaction 55: CH3 + OH <=> CH2(S) + H2O
-    const double2 a0 = ld2(rk0 + 94), a1 = ld2(rk1 + 94);
+    const double2 a0 = A0_48, a1 = A1_48;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c3);
     const double pb = kb*(c7)*(c16);
@@ -1322,7 +1379,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + 1.0 * q;
   }
   {  // reaction 56: CH4 + OH <=> CH3 + H2O
-    const double2 a0 = ld2(rk0 + 96), a1 = ld2(rk1 + 96);
+    const double2 a0 = A0_49, a1 = A1_49;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c21)*(c3);
     const double pb = kb*(c11)*(c16);
@@ -1341,7 +1398,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + -1.0 * q;
   }
   {  // reaction 57: CO + OH <=> CO2 + H
-    const double2 a0 = ld2(rk0 + 98), a1 = ld2(rk1 + 98);
+    const double2 a0 = A0_50, a1 = A1_50;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c3);
     const double pb = kb*(c23)*(c8);
@@ -1360,7 +1417,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + 1.0 * q;
   }
   {  // reaction 58: HCO + OH <=> CO + H2O
-    const double2 a0 = ld2(rk0 + 100), a1 = ld2(rk1 + 100);
+    const double2 a0 = A0_51, a1 = A1_51;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c5)*(c3);
     const double pb = kb*(c2)*(c16);
@@ -1379,7 +1436,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + 1.0 * q;
   }
   {  // reaction 59: CH2O + OH <=> H2O + HCO
-    const double2 a0 = ld2(rk0 + 102), a1 = ld2(rk1 + 102);
+    const double2 a0 = A0_52, a1 = A1_52;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c13)*(c3);
     const double pb = kb*(c16)*(c5);
@@ -1398,7 +1455,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + 1.0 * q;
   }
   {  // reaction 60: CH3O + OH <=> CH2O + H2O
-    const double2 a0 = ld2(rk0 + 104), a1 = ld2(rk1 + 104);
+    const double2 a0 = A0_53, a1 = A1_53;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c3);
     const double pb = kb*(c13)*(c16);
@@ -1417,7 +1474,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + 1.0 * q;
   }
   {  // reaction 61: C2H2 + OH <=> C2H + H2O
-    const double2 a0 = ld2(rk0 + 106), a1 = ld2(rk1 + 106);
+    const double2 a0 = A0_54, a1 = A1_54;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c3);
     const double pb = kb*(c15)*(c16);
@@ -1436,7 +1493,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + 1.0 * q;
   }
   {  // reaction 62: C2H2 + OH <=> CH3 + CO
-    const double2 a0 = ld2(rk0 + 108), a1 = ld2(rk1 + 108);
+    const double2 a0 = A0_55, a1 = A1_55;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c3);
     const double pb = kb*(c11)*(c2);
@@ -1455,7 +1512,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w12 = w12 + -1.0 * q;
   }
   {  // reaction 63: 2 HO2 <=> H2O2 + O2
-    const double2 a0 = ld2(rk0 + 110), a1 = ld2(rk1 + 110);
+    const double2 a0 = A0_56, a1 = A1_56;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c0*c0);
     const double pb = kb*(c1)*(c18);
@@ -1472,7 +1529,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + 1.0 * q;
   }
   {  // reaction 64: 2 HO2 <=> H2O2 + O2
-    const double2 a0 = ld2(rk0 + 112), a1 = ld2(rk1 + 112);
+    const double2 a0 = A0_57, a1 = A1_57;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c0*c0);
     const double pb = kb*(c1)*(c18);
@@ -1488,8 +1545,24 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w1 = w1 + 1.0 * q;
     w18 = w18 + 1.0 * q;
   }
+  const double2 A0_58 = ld2(rk0 + 114), A1_58 = ld2(rk1 + 114);
+  const double2 A0_59 = ld2(rk0 + 116), A1_59 = ld2(rk1 + 116);
+  const double2 A0_60 = ld2(rk0 + 118), A1_60 = ld2(rk1 + 118);
+  const double2 A0_61 = ld2(rk0 + 120), A1_61 = ld2(rk1 + 120);
+  const double2 A0_62 = ld2(rk0 + 122), A1_62 = ld2(rk1 + 122);
+  const double2 A0_63 = ld2(rk0 + 124), A1_63 = ld2(rk1 + 124);
+  const double2 A0_64 = ld2(rk0 + 126), A1_64 = ld2(rk1 + 126);
+  const double2 A0_65 = ld2(rk0 + 128), A1_65 = ld2(rk1 + 128);
+  const double2 A0_66 = ld2(rk0 + 130), A1_66 = ld2(rk1 + 130);
+  const double2 A0_67 = ld2(rk0 + 132), A1_67 = ld2(rk1 + 132);
+  const double2 A0_68 = ld2(rk0 + 134), A1_68 = ld2(rk1 + 134);
+  const double2 A0_69 = ld2(rk0 + 136), A1_69 = ld2(rk1 + 136);
+  const double2 A0_70 = ld2(rk0 + 138), A1_70 = ld2(rk1 + 138);
+  const double2 A0_71 = ld2(rk0 + 140), A1_71 = ld2(rk1 + 140);
+  const double2 A0_72 = ld2(rk0 + 142), A1_72 = ld2(rk1 + 142);
+  const double2 A0_73 = ld2(rk0 + 144), A1_73 = ld2(rk1 + 144);
   {  // reaction 65: CH2 + HO2 <=> CH2O + OH
-    const double2 a0 = ld2(rk0 + 114), a1 = ld2(rk1 + 114);
+    const double2 a0 = A0_58, a1 = A1_58;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c19)*(c0);
     const double pb = kb*(c13)*(c3);
@@ -1508,7 +1581,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + -1.0 * q;
   }
   {  // reaction 66: CH3 + HO2 <=> CH4 + O2
-    const double2 a0 = ld2(rk0 + 116), a1 = ld2(rk1 + 116);
+    const double2 a0 = A0_59, a1 = A1_59;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c0);
     const double pb = kb*(c21)*(c18);
@@ -1527,7 +1600,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + 1.0 * q;
   }
   {  // reaction 67: CH3 + HO2 <=> CH3O + OH
-    const double2 a0 = ld2(rk0 + 118), a1 = ld2(rk1 + 118);
+    const double2 a0 = A0_60, a1 = A1_60;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c0);
     const double pb = kb*(c6)*(c3);
@@ -1546,7 +1619,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w11 = w11 + -1.0 * q;
   }
   {  // reaction 68: CO + HO2 <=> CO2 + OH
-    const double2 a0 = ld2(rk0 + 120), a1 = ld2(rk1 + 120);
+    const double2 a0 = A0_61, a1 = A1_61;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c0);
     const double pb = kb*(c23)*(c3);
@@ -1565,7 +1638,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + 1.0 * q;
   }
   {  // reaction 69: CH2O + HO2 <=> H2O2 + HCO
-    const double2 a0 = ld2(rk0 + 122), a1 = ld2(rk1 + 122);
+    const double2 a0 = A0_62, a1 = A1_62;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c13)*(c0);
     const double pb = kb*(c1)*(c5);
@@ -1584,7 +1657,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + -1.0 * q;
   }
   {  // reaction 70: CH2 + O2 <=> HCO + OH
-    const double2 a0 = ld2(rk0 + 124), a1 = ld2(rk1 + 124);
+    const double2 a0 = A0_63, a1 = A1_63;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c19)*(c18);
     const double pb = kb*(c5)*(c3);
@@ -1603,7 +1676,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + -1.0 * q;
   }
   {  // reaction 71: CH2 + H2 <=> CH3 + H
-    const double2 a0 = ld2(rk0 + 126), a1 = ld2(rk1 + 126);
+    const double2 a0 = A0_64, a1 = A1_64;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c19)*(c4);
     const double pb = kb*(c11)*(c8);
@@ -1622,7 +1695,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + -1.0 * q;
   }
   {  // reaction 72: 2 CH2 <=> C2H2 + H2
-    const double2 a0 = ld2(rk0 + 128), a1 = ld2(rk1 + 128);
+    const double2 a0 = A0_65, a1 = A1_65;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c19*c19);
     const double pb = kb*(c12)*(c4);
@@ -1639,7 +1712,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + -2.0 * q;
   }
   {  // reaction 73: CH2 + CH4 <=> 2 CH3
-    const double2 a0 = ld2(rk0 + 130), a1 = ld2(rk1 + 130);
+    const double2 a0 = A0_66, a1 = A1_66;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c19)*(c21);
     const double pb = kb*(c11*c11);
@@ -1656,7 +1729,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + -1.0 * q;
   }
   {  // reaction 74: CH2(S) + O2 <=> CO + H + OH
-    const double2 a0 = ld2(rk0 + 132), a1 = ld2(rk1 + 132);
+    const double2 a0 = A0_67, a1 = A1_67;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[41 * SST] = kf; scr[42 * SST] = kb; }
     const double pf = kf*(c7)*(c18);
@@ -1678,7 +1751,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 75: CH2(S) + O2 <=> CO + H2O
-    const double2 a0 = ld2(rk0 + 134), a1 = ld2(rk1 + 134);
+    const double2 a0 = A0_68, a1 = A1_68;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c7)*(c18);
     const double pb = kb*(c2)*(c16);
@@ -1697,7 +1770,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 76: CH2(S) + H2 <=> CH3 + H
-    const double2 a0 = ld2(rk0 + 136), a1 = ld2(rk1 + 136);
+    const double2 a0 = A0_69, a1 = A1_69;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c7)*(c4);
     const double pb = kb*(c11)*(c8);
@@ -1716,7 +1789,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w11 = w11 + 1.0 * q;
   }
   {  // reaction 77: CH2(S) + H2O <=> CH2 + H2O
-    const double2 a0 = ld2(rk0 + 138), a1 = ld2(rk1 + 138);
+    const double2 a0 = A0_70, a1 = A1_70;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c7)*(c16);
     const double pb = kb*(c19)*(c16);
@@ -1733,7 +1806,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
   }
   {  // reaction 78: CH2(S) + CH4 <=> 2 CH3
-    const double2 a0 = ld2(rk0 + 140), a1 = ld2(rk1 + 140);
+    const double2 a0 = A0_71, a1 = A1_71;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c7)*(c21);
     const double pb = kb*(c11*c11);
@@ -1750,7 +1823,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + -1.0 * q;
   }
   {  // reaction 79: CH2(S) + CO <=> CH2 + CO
-    const double2 a0 = ld2(rk0 + 142), a1 = ld2(rk1 + 142);
+    const double2 a0 = A0_72, a1 = A1_72;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c7)*(c2);
     const double pb = kb*(c19)*(c2);
@@ -1767,7 +1840,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
   }
   {  // reaction 80: CH2(S) + CO2 <=> CH2 + CO2
-    const double2 a0 = ld2(rk0 + 144), a1 = ld2(rk1 + 144);
+    const double2 a0 = A0_73, a1 = A1_73;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c7)*(c23);
     const double pb = kb*(c19)*(c23);
@@ -1783,8 +1856,23 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
     w23 = w23 + 0.0 * q;
   }
+  const double2 A0_74 = ld2(rk0 + 146), A1_74 = ld2(rk1 + 146);
+  const double2 A0_75 = ld2(rk0 + 148), A1_75 = ld2(rk1 + 148);
+  const double2 A0_76 = ld2(rk0 + 150), A1_76 = ld2(rk1 + 150);
+  const double2 A0_77 = ld2(rk0 + 152), A1_77 = ld2(rk1 + 152);
+  const double2 A0_78 = ld2(rk0 + 154), A1_78 = ld2(rk1 + 154);
+  const double2 A0_79 = ld2(rk0 + 156), A1_79 = ld2(rk1 + 156);
+  const double2 A0_80 = ld2(rk0 + 158), A1_80 = ld2(rk1 + 158);
+  const double2 A0_81 = ld2(rk0 + 160), A1_81 = ld2(rk1 + 160);
+  const double2 A0_82 = ld2(rk0 + 162), A1_82 = ld2(rk1 + 162);
+  const double2 A0_83 = ld2(rk0 + 164), A1_83 = ld2(rk1 + 164);
+  const double2 A0_84 = ld2(rk0 + 166), A1_84 = ld2(rk1 + 166);
+  const double2 A0_85 = ld2(rk0 + 168), A1_85 = ld2(rk1 + 168);
+  const double2 A0_86 = ld2(rk0 + 170), A1_86 = ld2(rk1 + 170);
+  const double2 A0_87 = ld2(rk0 + 172), A1_87 = ld2(rk1 + 172);
+  const double2 A0_88 = ld2(rk0 + 174), A1_88 = ld2(rk1 + 174);
   {  // reaction 81: CH2(S) + CO2 <=> CH2O + CO
-    const double2 a0 = ld2(rk0 + 146), a1 = ld2(rk1 + 146);
+    const double2 a0 = A0_74, a1 = A1_74;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c7)*(c23);
     const double pb = kb*(c13)*(c2);
@@ -1803,7 +1891,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + -1.0 * q;
   }
   {  // reaction 82: CH3 + O2 <=> CH3O + O
-    const double2 a0 = ld2(rk0 + 148), a1 = ld2(rk1 + 148);
+    const double2 a0 = A0_75, a1 = A1_75;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c18);
     const double pb = kb*(c6)*(c9);
@@ -1822,7 +1910,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 83: CH3 + O2 <=> CH2O + OH
-    const double2 a0 = ld2(rk0 + 150), a1 = ld2(rk1 + 150);
+    const double2 a0 = A0_76, a1 = A1_76;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c18);
     const double pb = kb*(c13)*(c3);
@@ -1841,7 +1929,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 84: CH3 + H2O2 <=> CH4 + HO2
-    const double2 a0 = ld2(rk0 + 152), a1 = ld2(rk1 + 152);
+    const double2 a0 = A0_77, a1 = A1_77;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c1);
     const double pb = kb*(c21)*(c0);
@@ -1897,7 +1985,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w20 = 1.0 * q;
   }
   {  // reaction 86: CH3 + HCO <=> CH4 + CO
-    const double2 a0 = ld2(rk0 + 154), a1 = ld2(rk1 + 154);
+    const double2 a0 = A0_78, a1 = A1_78;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c5);
     const double pb = kb*(c21)*(c2);
@@ -1916,7 +2004,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + 1.0 * q;
   }
   {  // reaction 87: CH2O + CH3 <=> CH4 + HCO
-    const double2 a0 = ld2(rk0 + 156), a1 = ld2(rk1 + 156);
+    const double2 a0 = A0_79, a1 = A1_79;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c13)*(c11);
     const double pb = kb*(c21)*(c5);
@@ -1935,7 +2023,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + 1.0 * q;
   }
   {  // reaction 88: H2O + HCO <=> CO + H + H2O
-    const double2 a0 = ld2(rk0 + 158), a1 = ld2(rk1 + 158);
+    const double2 a0 = A0_80, a1 = A1_80;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c5)*M5;
     const double pb = kb*(c2)*(c8)*M5;
@@ -1953,7 +2041,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w8 = w8 + 1.0 * q;
   }
   {  // reaction 89: HCO + M <=> CO + H + M
-    const double2 a0 = ld2(rk0 + 160), a1 = ld2(rk1 + 160);
+    const double2 a0 = A0_81, a1 = A1_81;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c5)*M10;
     const double pb = kb*(c2)*(c8)*M10;
@@ -1971,7 +2059,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w8 = w8 + 1.0 * q;
   }
   {  // reaction 90: HCO + O2 <=> CO + HO2
-    const double2 a0 = ld2(rk0 + 162), a1 = ld2(rk1 + 162);
+    const double2 a0 = A0_82, a1 = A1_82;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c5)*(c18);
     const double pb = kb*(c2)*(c0);
@@ -1990,7 +2078,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 91: CH3O + O2 <=> CH2O + HO2
-    const double2 a0 = ld2(rk0 + 164), a1 = ld2(rk1 + 164);
+    const double2 a0 = A0_83, a1 = A1_83;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c18);
     const double pb = kb*(c13)*(c0);
@@ -2009,7 +2097,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 92: C2H + O2 <=> CO + HCO
-    const double2 a0 = ld2(rk0 + 166), a1 = ld2(rk1 + 166);
+    const double2 a0 = A0_84, a1 = A1_84;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c15)*(c18);
     const double pb = kb*(c2)*(c5);
@@ -2028,7 +2116,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 93: C2H + H2 <=> C2H2 + H
-    const double2 a0 = ld2(rk0 + 168), a1 = ld2(rk1 + 168);
+    const double2 a0 = A0_85, a1 = A1_85;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c15)*(c4);
     const double pb = kb*(c12)*(c8);
@@ -2047,7 +2135,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w15 = w15 + -1.0 * q;
   }
   {  // reaction 94: CH3 + OH <=> CH2O + H2
-    const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
+    const double2 a0 = A0_86, a1 = A1_86;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c3);
     const double pb = kb*(c13)*(c4);
@@ -2066,7 +2154,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + 1.0 * q;
   }
   {  // reaction 95: CH3 + O2 <=> CH3O2
-    const double2 a0 = ld2(rk0 + 172), a1 = ld2(rk1 + 172);
+    const double2 a0 = A0_87, a1 = A1_87;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c18);
     const double pb = kb*(c17);
@@ -2083,7 +2171,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 96: CH3O + HO2 <=> CH2O + H2O2
-    const double2 a0 = ld2(rk0 + 174), a1 = ld2(rk1 + 174);
+    const double2 a0 = A0_88, a1 = A1_88;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c0);
     const double pb = kb*(c13)*(c1);
@@ -2101,8 +2189,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w6 = w6 + -1.0 * q;
     w13 = w13 + 1.0 * q;
   }
+  const double2 A0_89 = ld2(rk0 + 176), A1_89 = ld2(rk1 + 176);
+  const double2 A0_90 = ld2(rk0 + 178), A1_90 = ld2(rk1 + 178);
+  const double2 A0_91 = ld2(rk0 + 180), A1_91 = ld2(rk1 + 180);
+  const double2 A0_92 = ld2(rk0 + 182), A1_92 = ld2(rk1 + 182);
+  const double2 A0_93 = ld2(rk0 + 184), A1_93 = ld2(rk1 + 184);
+  const double2 A0_94 = ld2(rk0 + 186), A1_94 = ld2(rk1 + 186);
+  const double2 A0_95 = ld2(rk0 + 188), A1_95 = ld2(rk1 + 188);
+  const double2 A0_96 = ld2(rk0 + 190), A1_96 = ld2(rk1 + 190);
+  const double2 A0_97 = ld2(rk0 + 192), A1_97 = ld2(rk1 + 192);
+  const double2 A0_98 = ld2(rk0 + 194), A1_98 = ld2(rk1 + 194);
+  const double2 A0_99 = ld2(rk0 + 196), A1_99 = ld2(rk1 + 196);
+  const double2 A0_100 = ld2(rk0 + 198), A1_100 = ld2(rk1 + 198);
+  const double2 A0_101 = ld2(rk0 + 200), A1_101 = ld2(rk1 + 200);
+  const double2 A0_102 = ld2(rk0 + 202), A1_102 = ld2(rk1 + 202);
   {  // reaction 97: CH3 + CH3O <=> CH2O + CH4
-    const double2 a0 = ld2(rk0 + 176), a1 = ld2(rk1 + 176);
+    const double2 a0 = A0_89, a1 = A1_89;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c6);
     const double pb = kb*(c13)*(c21);
@@ -2121,7 +2223,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + 1.0 * q;
   }
   {  // reaction 98: CH3 + CH3O2 <=> 2 CH3O
-    const double2 a0 = ld2(rk0 + 178), a1 = ld2(rk1 + 178);
+    const double2 a0 = A0_90, a1 = A1_90;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c17);
     const double pb = kb*(c6*c6);
@@ -2138,7 +2240,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + -1.0 * q;
   }
   {  // reaction 99: C2H2 + O2 => 2 HCO
-    const double2 a0 = ld2(rk0 + 180), a1 = ld2(rk1 + 180);
+    const double2 a0 = A0_91, a1 = A1_91;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c18);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2155,7 +2257,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 100: 2 HCO => C2H2 + O2
-    const double2 a0 = ld2(rk0 + 182), a1 = ld2(rk1 + 182);
+    const double2 a0 = A0_92, a1 = A1_92;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c5*c5);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2172,7 +2274,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + 1.0 * q;
   }
   {  // reaction 101: CH3O2 + HO2 => CH3O + O2 + OH
-    const double2 a0 = ld2(rk0 + 184), a1 = ld2(rk1 + 184);
+    const double2 a0 = A0_93, a1 = A1_93;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c17)*(c0);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2193,7 +2295,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + 1.0 * q;
   }
   {  // reaction 102: 2 CH3O2 => 2 CH3O + O2
-    const double2 a0 = ld2(rk0 + 186), a1 = ld2(rk1 + 186);
+    const double2 a0 = A0_94, a1 = A1_94;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c17*c17);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2210,7 +2312,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + 1.0 * q;
   }
   {  // reaction 103: C3H3 + O => C2H + CH2O
-    const double2 a0 = ld2(rk0 + 188), a1 = ld2(rk1 + 188);
+    const double2 a0 = A0_95, a1 = A1_95;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c14)*(c9);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2229,7 +2331,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w15 = w15 + 1.0 * q;
   }
   {  // reaction 104: C2H + CH2O => C3H3 + O
-    const double2 a0 = ld2(rk0 + 190), a1 = ld2(rk1 + 190);
+    const double2 a0 = A0_96, a1 = A1_96;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c15)*(c13);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2248,7 +2350,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w15 = w15 + -1.0 * q;
   }
   {  // reaction 105: C3H3 + OH => C3H2 + H2O
-    const double2 a0 = ld2(rk0 + 192), a1 = ld2(rk1 + 192);
+    const double2 a0 = A0_97, a1 = A1_97;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c14)*(c3);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2267,7 +2369,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + 1.0 * q;
   }
   {  // reaction 106: C3H2 + H2O => C3H3 + OH
-    const double2 a0 = ld2(rk0 + 194), a1 = ld2(rk1 + 194);
+    const double2 a0 = A0_98, a1 = A1_98;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c10)*(c16);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2286,7 +2388,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + -1.0 * q;
   }
   {  // reaction 107: HCO2 + M => CO2 + H + M
-    const double2 a0 = ld2(rk0 + 196), a1 = ld2(rk1 + 196);
+    const double2 a0 = A0_99, a1 = A1_99;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c22)*M11;
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2304,7 +2406,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + 1.0 * q;
   }
   {  // reaction 108: CO2 + H + M => HCO2 + M
-    const double2 a0 = ld2(rk0 + 198), a1 = ld2(rk1 + 198);
+    const double2 a0 = A0_100, a1 = A1_100;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c23)*(c8)*M11;
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2322,7 +2424,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + -1.0 * q;
   }
   {  // reaction 109: HCO2 + M => CO + OH + M
-    const double2 a0 = ld2(rk0 + 200), a1 = ld2(rk1 + 200);
+    const double2 a0 = A0_101, a1 = A1_101;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c22)*M11;
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2340,7 +2442,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w22 = w22 + -1.0 * q;
   }
   {  // reaction 110: CO + OH + M => HCO2 + M
-    const double2 a0 = ld2(rk0 + 202), a1 = ld2(rk1 + 202);
+    const double2 a0 = A0_102, a1 = A1_102;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c3)*M11;
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2590,6 +2692,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_103 = ld2(rk0 + 38), A1_103 = ld2(rk1 + 38);
+    const double2 A0_104 = ld2(rk0 + 40), A1_104 = ld2(rk1 + 40);
+    const double2 A0_105 = ld2(rk0 + 42), A1_105 = ld2(rk1 + 42);
+    const double2 A0_106 = ld2(rk0 + 58), A1_106 = ld2(rk1 + 58);
+    const double2 A0_107 = ld2(rk0 + 60), A1_107 = ld2(rk1 + 60);
+    const double2 A0_108 = ld2(rk0 + 62), A1_108 = ld2(rk1 + 62);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -2628,19 +2736,19 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc13 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 38), a1 = ld2(rk1 + 38);
+    { const double2 a0 = A0_103, a1 = A1_103;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M3 * 1.0 + scr[3 * SST];
       jc0 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 40), a1 = ld2(rk1 + 40);
+    { const double2 a0 = A0_104, a1 = A1_104;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M4 * 1.0;
       jc0 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 42), a1 = ld2(rk1 + 42);
+    { const double2 a0 = A0_105, a1 = A1_105;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M5 * 1.0;
       jc0 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -2664,20 +2772,20 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 58), a1 = ld2(rk1 + 58);
+    { const double2 a0 = A0_106, a1 = A1_106;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc0 += -1.0 * dq;
       jc4 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 60), a1 = ld2(rk1 + 60);
+    { const double2 a0 = A0_107, a1 = A1_107;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc0 += -1.0 * dq;
       jc3 += 2.0 * dq;
       jc8 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 62), a1 = ld2(rk1 + 62);
+    { const double2 a0 = A0_108, a1 = A1_108;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
@@ -2690,6 +2798,16 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
+    const double2 A0_109 = ld2(rk0 + 82), A1_109 = ld2(rk1 + 82);
+    const double2 A0_110 = ld2(rk0 + 84), A1_110 = ld2(rk1 + 84);
+    const double2 A0_111 = ld2(rk0 + 86), A1_111 = ld2(rk1 + 86);
+    const double2 A0_112 = ld2(rk0 + 110), A1_112 = ld2(rk1 + 110);
+    const double2 A0_113 = ld2(rk0 + 112), A1_113 = ld2(rk1 + 112);
+    const double2 A0_114 = ld2(rk0 + 114), A1_114 = ld2(rk1 + 114);
+    const double2 A0_115 = ld2(rk0 + 116), A1_115 = ld2(rk1 + 116);
+    const double2 A0_116 = ld2(rk0 + 118), A1_116 = ld2(rk1 + 118);
+    const double2 A0_117 = ld2(rk0 + 120), A1_117 = ld2(rk1 + 120);
+    const double2 A0_118 = ld2(rk0 + 122), A1_118 = ld2(rk1 + 122);
     {
       const double dq = scr[14 * SST];
       jc8 += -1.0 * dq;
@@ -2725,75 +2843,80 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 82), a1 = ld2(rk1 + 82);
+    { const double2 a0 = A0_109, a1 = A1_109;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc0 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 84), a1 = ld2(rk1 + 84);
+    { const double2 a0 = A0_110, a1 = A1_110;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 86), a1 = ld2(rk1 + 86);
+    { const double2 a0 = A0_111, a1 = A1_111;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 110), a1 = ld2(rk1 + 110);
+    { const double2 a0 = A0_112, a1 = A1_112;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c0;
       jc0 += -2.0 * dq;
       jc1 += 1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 112), a1 = ld2(rk1 + 112);
+    { const double2 a0 = A0_113, a1 = A1_113;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c0;
       jc0 += -2.0 * dq;
       jc1 += 1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 114), a1 = ld2(rk1 + 114);
+    { const double2 a0 = A0_114, a1 = A1_114;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc13 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 116), a1 = ld2(rk1 + 116);
+    { const double2 a0 = A0_115, a1 = A1_115;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc0 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 118), a1 = ld2(rk1 + 118);
+    { const double2 a0 = A0_116, a1 = A1_116;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc6 += 1.0 * dq;
       jc11 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 120), a1 = ld2(rk1 + 120);
+    { const double2 a0 = A0_117, a1 = A1_117;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc0 += -1.0 * dq;
       jc2 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 122), a1 = ld2(rk1 + 122);
+    { const double2 a0 = A0_118, a1 = A1_118;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c13;
       jc0 += -1.0 * dq;
       jc1 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 152), a1 = ld2(rk1 + 152);
+    const double2 A0_119 = ld2(rk0 + 152), A1_119 = ld2(rk1 + 152);
+    const double2 A0_120 = ld2(rk0 + 162), A1_120 = ld2(rk1 + 162);
+    const double2 A0_121 = ld2(rk0 + 164), A1_121 = ld2(rk1 + 164);
+    const double2 A0_122 = ld2(rk0 + 174), A1_122 = ld2(rk1 + 174);
+    const double2 A0_123 = ld2(rk0 + 184), A1_123 = ld2(rk1 + 184);
+    { const double2 a0 = A0_119, a1 = A1_119;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c21;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
@@ -2811,28 +2934,28 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 162), a1 = ld2(rk1 + 162);
+    { const double2 a0 = A0_120, a1 = A1_120;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc0 += 1.0 * dq;
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 164), a1 = ld2(rk1 + 164);
+    { const double2 a0 = A0_121, a1 = A1_121;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
       jc0 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 174), a1 = ld2(rk1 + 174);
+    { const double2 a0 = A0_122, a1 = A1_122;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc0 += -1.0 * dq;
       jc1 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 184), a1 = ld2(rk1 + 184);
+    { const double2 a0 = A0_123, a1 = A1_123;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c17;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
@@ -2916,6 +3039,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_124 = ld2(rk0 + 62), A1_124 = ld2(rk1 + 62);
+    const double2 A0_125 = ld2(rk0 + 64), A1_125 = ld2(rk1 + 64);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -2957,14 +3082,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 62), a1 = ld2(rk1 + 62);
+    { const double2 a0 = A0_124, a1 = A1_124;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc4 += 1.0 * dq;
       jc8 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 64), a1 = ld2(rk1 + 64);
+    { const double2 a0 = A0_125, a1 = A1_125;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc1 += -1.0 * dq;
       jc3 += 1.0 * dq;
@@ -3012,40 +3137,47 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 84), a1 = ld2(rk1 + 84);
+    const double2 A0_126 = ld2(rk0 + 84), A1_126 = ld2(rk1 + 84);
+    const double2 A0_127 = ld2(rk0 + 86), A1_127 = ld2(rk1 + 86);
+    const double2 A0_128 = ld2(rk0 + 110), A1_128 = ld2(rk1 + 110);
+    const double2 A0_129 = ld2(rk0 + 112), A1_129 = ld2(rk1 + 112);
+    const double2 A0_130 = ld2(rk0 + 122), A1_130 = ld2(rk1 + 122);
+    const double2 A0_131 = ld2(rk0 + 152), A1_131 = ld2(rk1 + 152);
+    const double2 A0_132 = ld2(rk0 + 174), A1_132 = ld2(rk1 + 174);
+    { const double2 a0 = A0_126, a1 = A1_126;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 86), a1 = ld2(rk1 + 86);
+    { const double2 a0 = A0_127, a1 = A1_127;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 110), a1 = ld2(rk1 + 110);
+    { const double2 a0 = A0_128, a1 = A1_128;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc0 += -2.0 * dq;
       jc1 += 1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 112), a1 = ld2(rk1 + 112);
+    { const double2 a0 = A0_129, a1 = A1_129;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc0 += -2.0 * dq;
       jc1 += 1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 122), a1 = ld2(rk1 + 122);
+    { const double2 a0 = A0_130, a1 = A1_130;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c5;
       jc0 += -1.0 * dq;
       jc1 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 152), a1 = ld2(rk1 + 152);
+    { const double2 a0 = A0_131, a1 = A1_131;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
@@ -3063,7 +3195,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 174), a1 = ld2(rk1 + 174);
+    { const double2 a0 = A0_132, a1 = A1_132;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
       jc0 += -1.0 * dq;
       jc1 += 1.0 * dq;
@@ -3147,6 +3279,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_133 = ld2(rk0 + 20), A1_133 = ld2(rk1 + 20);
+    const double2 A0_134 = ld2(rk0 + 68), A1_134 = ld2(rk1 + 68);
     {
       const double dq = 1.75 * scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -3165,7 +3299,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 20), a1 = ld2(rk1 + 20);
+    { const double2 a0 = A0_133, a1 = A1_133;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M2 * c9 + 1.5 * scr[2 * SST];
       jc2 += -1.0 * dq;
       jc9 += -1.0 * dq;
@@ -3227,7 +3361,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 68), a1 = ld2(rk1 + 68);
+    { const double2 a0 = A0_134, a1 = A1_134;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
@@ -3246,6 +3380,18 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc15 += -1.0 * dq;
     }
+    const double2 A0_135 = ld2(rk0 + 98), A1_135 = ld2(rk1 + 98);
+    const double2 A0_136 = ld2(rk0 + 100), A1_136 = ld2(rk1 + 100);
+    const double2 A0_137 = ld2(rk0 + 108), A1_137 = ld2(rk1 + 108);
+    const double2 A0_138 = ld2(rk0 + 120), A1_138 = ld2(rk1 + 120);
+    const double2 A0_139 = ld2(rk0 + 134), A1_139 = ld2(rk1 + 134);
+    const double2 A0_140 = ld2(rk0 + 142), A1_140 = ld2(rk1 + 142);
+    const double2 A0_141 = ld2(rk0 + 146), A1_141 = ld2(rk1 + 146);
+    const double2 A0_142 = ld2(rk0 + 154), A1_142 = ld2(rk1 + 154);
+    const double2 A0_143 = ld2(rk0 + 158), A1_143 = ld2(rk1 + 158);
+    const double2 A0_144 = ld2(rk0 + 160), A1_144 = ld2(rk1 + 160);
+    const double2 A0_145 = ld2(rk0 + 162), A1_145 = ld2(rk1 + 162);
+    const double2 A0_146 = ld2(rk0 + 166), A1_146 = ld2(rk1 + 166);
     {
       const double dq = scr[27 * SST] * c4 + 1.5 * scr[26 * SST];
       jc2 += -1.0 * dq;
@@ -3257,28 +3403,28 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 98), a1 = ld2(rk1 + 98);
+    { const double2 a0 = A0_135, a1 = A1_135;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc2 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 100), a1 = ld2(rk1 + 100);
+    { const double2 a0 = A0_136, a1 = A1_136;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc5 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 108), a1 = ld2(rk1 + 108);
+    { const double2 a0 = A0_137, a1 = A1_137;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c11;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc11 += 1.0 * dq;
       jc12 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 120), a1 = ld2(rk1 + 120);
+    { const double2 a0 = A0_138, a1 = A1_138;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
       jc0 += -1.0 * dq;
       jc2 += -1.0 * dq;
@@ -3293,20 +3439,20 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 134), a1 = ld2(rk1 + 134);
+    { const double2 a0 = A0_139, a1 = A1_139;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 142), a1 = ld2(rk1 + 142);
+    { const double2 a0 = A0_140, a1 = A1_140;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7 - (a0.y + (a1.y - a0.y) * Td) * c19;
       jc2 += 0.0 * dq;
       jc7 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 146), a1 = ld2(rk1 + 146);
+    { const double2 a0 = A0_141, a1 = A1_141;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -3318,39 +3464,40 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -2.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 154), a1 = ld2(rk1 + 154);
+    { const double2 a0 = A0_142, a1 = A1_142;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c21;
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 158), a1 = ld2(rk1 + 158);
+    { const double2 a0 = A0_143, a1 = A1_143;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M5 * c8;
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 160), a1 = ld2(rk1 + 160);
+    { const double2 a0 = A0_144, a1 = A1_144;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M10 * c8 + 1.5 * scr[36 * SST];
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 162), a1 = ld2(rk1 + 162);
+    { const double2 a0 = A0_145, a1 = A1_145;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c0;
       jc0 += 1.0 * dq;
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 166), a1 = ld2(rk1 + 166);
+    { const double2 a0 = A0_146, a1 = A1_146;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c5;
       jc2 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
+    const double2 A0_147 = ld2(rk0 + 202), A1_147 = ld2(rk1 + 202);
     {
       const double dq = scr[37 * SST];
       jc8 += 1.0 * dq;
@@ -3369,7 +3516,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc3 += 1.0 * dq;
       jc22 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 202), a1 = ld2(rk1 + 202);
+    { const double2 a0 = A0_147, a1 = A1_147;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M11 * c3 + scr[40 * SST];
       jc2 += -1.0 * dq;
       jc3 += -1.0 * dq;
@@ -3430,12 +3577,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_148 = ld2(rk0 + 2), A1_148 = ld2(rk1 + 2);
+    const double2 A0_149 = ld2(rk0 + 54), A1_149 = ld2(rk1 + 54);
+    const double2 A0_150 = ld2(rk0 + 60), A1_150 = ld2(rk1 + 60);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 2), a1 = ld2(rk1 + 2);
+    { const double2 a0 = A0_148, a1 = A1_148;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M1 * 1.0 + scr[1 * SST];
       jc3 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -3521,19 +3671,28 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += 1.0 * dq;
       jc8 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 54), a1 = ld2(rk1 + 54);
+    { const double2 a0 = A0_149, a1 = A1_149;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M9 * c8 + scr[10 * SST];
       jc3 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 60), a1 = ld2(rk1 + 60);
+    { const double2 a0 = A0_150, a1 = A1_150;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * 2.0*c3;
       jc0 += -1.0 * dq;
       jc3 += 2.0 * dq;
       jc8 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 64), a1 = ld2(rk1 + 64);
+    const double2 A0_151 = ld2(rk0 + 64), A1_151 = ld2(rk1 + 64);
+    const double2 A0_152 = ld2(rk0 + 74), A1_152 = ld2(rk1 + 74);
+    const double2 A0_153 = ld2(rk0 + 78), A1_153 = ld2(rk1 + 78);
+    const double2 A0_154 = ld2(rk0 + 80), A1_154 = ld2(rk1 + 80);
+    const double2 A0_155 = ld2(rk0 + 82), A1_155 = ld2(rk1 + 82);
+    const double2 A0_156 = ld2(rk0 + 84), A1_156 = ld2(rk1 + 84);
+    const double2 A0_157 = ld2(rk0 + 86), A1_157 = ld2(rk1 + 86);
+    const double2 A0_158 = ld2(rk0 + 88), A1_158 = ld2(rk1 + 88);
+    const double2 A0_159 = ld2(rk0 + 90), A1_159 = ld2(rk1 + 90);
+    { const double2 a0 = A0_151, a1 = A1_151;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc1 += -1.0 * dq;
       jc3 += 1.0 * dq;
@@ -3564,7 +3723,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 74), a1 = ld2(rk1 + 74);
+    { const double2 a0 = A0_152, a1 = A1_152;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c11;
       jc3 += 1.0 * dq;
       jc6 += -1.0 * dq;
@@ -3583,7 +3742,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 78), a1 = ld2(rk1 + 78);
+    { const double2 a0 = A0_153, a1 = A1_153;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c4;
       jc3 += -1.0 * dq;
       jc4 += -1.0 * dq;
@@ -3595,132 +3754,146 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 80), a1 = ld2(rk1 + 80);
+    { const double2 a0 = A0_154, a1 = A1_154;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c3;
       jc3 += -2.0 * dq;
       jc9 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 82), a1 = ld2(rk1 + 82);
+    { const double2 a0 = A0_155, a1 = A1_155;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
       jc0 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 84), a1 = ld2(rk1 + 84);
+    { const double2 a0 = A0_156, a1 = A1_156;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c1;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 86), a1 = ld2(rk1 + 86);
+    { const double2 a0 = A0_157, a1 = A1_157;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c1;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 88), a1 = ld2(rk1 + 88);
+    { const double2 a0 = A0_158, a1 = A1_158;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
       jc3 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc13 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 90), a1 = ld2(rk1 + 90);
+    { const double2 a0 = A0_159, a1 = A1_159;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc3 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 92), a1 = ld2(rk1 + 92);
+    const double2 A0_160 = ld2(rk0 + 92), A1_160 = ld2(rk1 + 92);
+    const double2 A0_161 = ld2(rk0 + 94), A1_161 = ld2(rk1 + 94);
+    const double2 A0_162 = ld2(rk0 + 96), A1_162 = ld2(rk1 + 96);
+    const double2 A0_163 = ld2(rk0 + 98), A1_163 = ld2(rk1 + 98);
+    const double2 A0_164 = ld2(rk0 + 100), A1_164 = ld2(rk1 + 100);
+    const double2 A0_165 = ld2(rk0 + 102), A1_165 = ld2(rk1 + 102);
+    const double2 A0_166 = ld2(rk0 + 104), A1_166 = ld2(rk1 + 104);
+    const double2 A0_167 = ld2(rk0 + 106), A1_167 = ld2(rk1 + 106);
+    const double2 A0_168 = ld2(rk0 + 108), A1_168 = ld2(rk1 + 108);
+    const double2 A0_169 = ld2(rk0 + 114), A1_169 = ld2(rk1 + 114);
+    const double2 A0_170 = ld2(rk0 + 118), A1_170 = ld2(rk1 + 118);
+    const double2 A0_171 = ld2(rk0 + 120), A1_171 = ld2(rk1 + 120);
+    const double2 A0_172 = ld2(rk0 + 124), A1_172 = ld2(rk1 + 124);
+    const double2 A0_173 = ld2(rk0 + 150), A1_173 = ld2(rk1 + 150);
+    { const double2 a0 = A0_160, a1 = A1_160;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc3 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 94), a1 = ld2(rk1 + 94);
+    { const double2 a0 = A0_161, a1 = A1_161;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc3 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 96), a1 = ld2(rk1 + 96);
+    { const double2 a0 = A0_162, a1 = A1_162;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c21;
       jc3 += -1.0 * dq;
       jc11 += 1.0 * dq;
       jc16 += 1.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 98), a1 = ld2(rk1 + 98);
+    { const double2 a0 = A0_163, a1 = A1_163;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 100), a1 = ld2(rk1 + 100);
+    { const double2 a0 = A0_164, a1 = A1_164;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c5;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc5 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 102), a1 = ld2(rk1 + 102);
+    { const double2 a0 = A0_165, a1 = A1_165;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c13;
       jc3 += -1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 104), a1 = ld2(rk1 + 104);
+    { const double2 a0 = A0_166, a1 = A1_166;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc3 += -1.0 * dq;
       jc6 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 106), a1 = ld2(rk1 + 106);
+    { const double2 a0 = A0_167, a1 = A1_167;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc3 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 108), a1 = ld2(rk1 + 108);
+    { const double2 a0 = A0_168, a1 = A1_168;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc11 += 1.0 * dq;
       jc12 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 114), a1 = ld2(rk1 + 114);
+    { const double2 a0 = A0_169, a1 = A1_169;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc13 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 118), a1 = ld2(rk1 + 118);
+    { const double2 a0 = A0_170, a1 = A1_170;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c6;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc6 += 1.0 * dq;
       jc11 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 120), a1 = ld2(rk1 + 120);
+    { const double2 a0 = A0_171, a1 = A1_171;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c23;
       jc0 += -1.0 * dq;
       jc2 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 124), a1 = ld2(rk1 + 124);
+    { const double2 a0 = A0_172, a1 = A1_172;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c5;
       jc3 += 1.0 * dq;
       jc5 += 1.0 * dq;
@@ -3735,7 +3908,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 150), a1 = ld2(rk1 + 150);
+    { const double2 a0 = A0_173, a1 = A1_173;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
       jc3 += 1.0 * dq;
       jc11 += -1.0 * dq;
@@ -3747,20 +3920,23 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -2.0 * dq;
       jc20 += 1.0 * dq;
     }
+    const double2 A0_174 = ld2(rk0 + 170), A1_174 = ld2(rk1 + 170);
+    const double2 A0_175 = ld2(rk0 + 192), A1_175 = ld2(rk1 + 192);
+    const double2 A0_176 = ld2(rk0 + 202), A1_176 = ld2(rk1 + 202);
     {
       const double dq = scr[36 * SST];
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
+    { const double2 a0 = A0_174, a1 = A1_174;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc3 += -1.0 * dq;
       jc4 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 192), a1 = ld2(rk1 + 192);
+    { const double2 a0 = A0_175, a1 = A1_175;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c14;
       jc3 += -1.0 * dq;
       jc10 += 1.0 * dq;
@@ -3785,7 +3961,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc3 += 1.0 * dq;
       jc22 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 202), a1 = ld2(rk1 + 202);
+    { const double2 a0 = A0_176, a1 = A1_176;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M11 * c2 + scr[40 * SST];
       jc2 += -1.0 * dq;
       jc3 += -1.0 * dq;
@@ -3844,6 +4020,13 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_177 = ld2(rk0 + 46), A1_177 = ld2(rk1 + 46);
+    const double2 A0_178 = ld2(rk0 + 48), A1_178 = ld2(rk1 + 48);
+    const double2 A0_179 = ld2(rk0 + 50), A1_179 = ld2(rk1 + 50);
+    const double2 A0_180 = ld2(rk0 + 52), A1_180 = ld2(rk1 + 52);
+    const double2 A0_181 = ld2(rk0 + 58), A1_181 = ld2(rk1 + 58);
+    const double2 A0_182 = ld2(rk0 + 62), A1_182 = ld2(rk1 + 62);
+    const double2 A0_183 = ld2(rk0 + 66), A1_183 = ld2(rk1 + 66);
     {
       const double dq = 2.4000000953674316 * scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -3881,22 +4064,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 46), a1 = ld2(rk1 + 46);
+    { const double2 a0 = A0_177, a1 = A1_177;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M6 * 1.0;
       jc4 += 1.0 * dq;
       jc8 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 48), a1 = ld2(rk1 + 48);
+    { const double2 a0 = A0_178, a1 = A1_178;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M7 * 1.0 + scr[7 * SST];
       jc4 += 1.0 * dq;
       jc8 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 50), a1 = ld2(rk1 + 50);
+    { const double2 a0 = A0_179, a1 = A1_179;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M5 * 1.0;
       jc4 += 1.0 * dq;
       jc8 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 52), a1 = ld2(rk1 + 52);
+    { const double2 a0 = A0_180, a1 = A1_180;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M8 * 1.0;
       jc4 += 1.0 * dq;
       jc8 += -2.0 * dq;
@@ -3907,14 +4090,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 58), a1 = ld2(rk1 + 58);
+    { const double2 a0 = A0_181, a1 = A1_181;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc0 += -1.0 * dq;
       jc4 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 62), a1 = ld2(rk1 + 62);
+    { const double2 a0 = A0_182, a1 = A1_182;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c0;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
@@ -3933,20 +4116,29 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 66), a1 = ld2(rk1 + 66);
+    { const double2 a0 = A0_183, a1 = A1_183;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c11;
       jc4 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc11 += 1.0 * dq;
       jc21 += -1.0 * dq;
     }
+    const double2 A0_184 = ld2(rk0 + 68), A1_184 = ld2(rk1 + 68);
+    const double2 A0_185 = ld2(rk0 + 70), A1_185 = ld2(rk1 + 70);
+    const double2 A0_186 = ld2(rk0 + 72), A1_186 = ld2(rk1 + 72);
+    const double2 A0_187 = ld2(rk0 + 78), A1_187 = ld2(rk1 + 78);
+    const double2 A0_188 = ld2(rk0 + 126), A1_188 = ld2(rk1 + 126);
+    const double2 A0_189 = ld2(rk0 + 128), A1_189 = ld2(rk1 + 128);
+    const double2 A0_190 = ld2(rk0 + 136), A1_190 = ld2(rk1 + 136);
+    const double2 A0_191 = ld2(rk0 + 168), A1_191 = ld2(rk1 + 168);
+    const double2 A0_192 = ld2(rk0 + 170), A1_192 = ld2(rk1 + 170);
     {
       const double dq = 2.0 * scr[17 * SST];
       jc5 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 68), a1 = ld2(rk1 + 68);
+    { const double2 a0 = A0_184, a1 = A1_184;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
@@ -3959,14 +4151,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 70), a1 = ld2(rk1 + 70);
+    { const double2 a0 = A0_185, a1 = A1_185;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c5;
       jc4 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 72), a1 = ld2(rk1 + 72);
+    { const double2 a0 = A0_186, a1 = A1_186;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
       jc4 += 1.0 * dq;
       jc6 += -1.0 * dq;
@@ -3985,7 +4177,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 78), a1 = ld2(rk1 + 78);
+    { const double2 a0 = A0_187, a1 = A1_187;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc4 += -1.0 * dq;
@@ -3997,20 +4189,20 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 126), a1 = ld2(rk1 + 126);
+    { const double2 a0 = A0_188, a1 = A1_188;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc11 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 128), a1 = ld2(rk1 + 128);
+    { const double2 a0 = A0_189, a1 = A1_189;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c12;
       jc4 += 1.0 * dq;
       jc12 += 1.0 * dq;
       jc19 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 136), a1 = ld2(rk1 + 136);
+    { const double2 a0 = A0_190, a1 = A1_190;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc4 += -1.0 * dq;
       jc7 += -1.0 * dq;
@@ -4028,14 +4220,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 168), a1 = ld2(rk1 + 168);
+    { const double2 a0 = A0_191, a1 = A1_191;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc12 += 1.0 * dq;
       jc15 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
+    { const double2 a0 = A0_192, a1 = A1_192;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
       jc3 += -1.0 * dq;
       jc4 += 1.0 * dq;
@@ -4119,6 +4311,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_193 = ld2(rk0 + 68), A1_193 = ld2(rk1 + 68);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -4213,20 +4406,31 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 68), a1 = ld2(rk1 + 68);
+    { const double2 a0 = A0_193, a1 = A1_193;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc8 += -1.0 * dq;
     }
+    const double2 A0_194 = ld2(rk0 + 70), A1_194 = ld2(rk1 + 70);
+    const double2 A0_195 = ld2(rk0 + 100), A1_195 = ld2(rk1 + 100);
+    const double2 A0_196 = ld2(rk0 + 102), A1_196 = ld2(rk1 + 102);
+    const double2 A0_197 = ld2(rk0 + 122), A1_197 = ld2(rk1 + 122);
+    const double2 A0_198 = ld2(rk0 + 124), A1_198 = ld2(rk1 + 124);
+    const double2 A0_199 = ld2(rk0 + 154), A1_199 = ld2(rk1 + 154);
+    const double2 A0_200 = ld2(rk0 + 156), A1_200 = ld2(rk1 + 156);
+    const double2 A0_201 = ld2(rk0 + 158), A1_201 = ld2(rk1 + 158);
+    const double2 A0_202 = ld2(rk0 + 160), A1_202 = ld2(rk1 + 160);
+    const double2 A0_203 = ld2(rk0 + 162), A1_203 = ld2(rk1 + 162);
+    const double2 A0_204 = ld2(rk0 + 166), A1_204 = ld2(rk1 + 166);
     {
       const double dq = scr[20 * SST];
       jc6 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 70), a1 = ld2(rk1 + 70);
+    { const double2 a0 = A0_194, a1 = A1_194;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc4 += 1.0 * dq;
       jc5 += 1.0 * dq;
@@ -4250,28 +4454,28 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 100), a1 = ld2(rk1 + 100);
+    { const double2 a0 = A0_195, a1 = A1_195;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc5 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 102), a1 = ld2(rk1 + 102);
+    { const double2 a0 = A0_196, a1 = A1_196;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc3 += -1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 122), a1 = ld2(rk1 + 122);
+    { const double2 a0 = A0_197, a1 = A1_197;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c1;
       jc0 += -1.0 * dq;
       jc1 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 124), a1 = ld2(rk1 + 124);
+    { const double2 a0 = A0_198, a1 = A1_198;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc3 += 1.0 * dq;
       jc5 += 1.0 * dq;
@@ -4283,47 +4487,48 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -2.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 154), a1 = ld2(rk1 + 154);
+    { const double2 a0 = A0_199, a1 = A1_199;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 156), a1 = ld2(rk1 + 156);
+    { const double2 a0 = A0_200, a1 = A1_200;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c21;
       jc5 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc13 += -1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 158), a1 = ld2(rk1 + 158);
+    { const double2 a0 = A0_201, a1 = A1_201;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M5 * 1.0;
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 160), a1 = ld2(rk1 + 160);
+    { const double2 a0 = A0_202, a1 = A1_202;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M10 * 1.0 + scr[36 * SST];
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 162), a1 = ld2(rk1 + 162);
+    { const double2 a0 = A0_203, a1 = A1_203;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc0 += 1.0 * dq;
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 166), a1 = ld2(rk1 + 166);
+    { const double2 a0 = A0_204, a1 = A1_204;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 182), a1 = ld2(rk1 + 182);
+    const double2 A0_205 = ld2(rk0 + 182), A1_205 = ld2(rk1 + 182);
+    { const double2 a0 = A0_205, a1 = A1_205;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c5;
       jc5 += -2.0 * dq;
       jc12 += 1.0 * dq;
@@ -4407,6 +4612,9 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_206 = ld2(rk0 + 72), A1_206 = ld2(rk1 + 72);
+    const double2 A0_207 = ld2(rk0 + 74), A1_207 = ld2(rk1 + 74);
+    const double2 A0_208 = ld2(rk0 + 76), A1_208 = ld2(rk1 + 76);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -4472,21 +4680,21 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 72), a1 = ld2(rk1 + 72);
+    { const double2 a0 = A0_206, a1 = A1_206;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc4 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 74), a1 = ld2(rk1 + 74);
+    { const double2 a0 = A0_207, a1 = A1_207;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc3 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc11 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 76), a1 = ld2(rk1 + 76);
+    { const double2 a0 = A0_208, a1 = A1_208;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc6 += -1.0 * dq;
       jc7 += 1.0 * dq;
@@ -4505,26 +4713,33 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
+    const double2 A0_209 = ld2(rk0 + 104), A1_209 = ld2(rk1 + 104);
+    const double2 A0_210 = ld2(rk0 + 118), A1_210 = ld2(rk1 + 118);
+    const double2 A0_211 = ld2(rk0 + 148), A1_211 = ld2(rk1 + 148);
+    const double2 A0_212 = ld2(rk0 + 164), A1_212 = ld2(rk1 + 164);
+    const double2 A0_213 = ld2(rk0 + 174), A1_213 = ld2(rk1 + 174);
+    const double2 A0_214 = ld2(rk0 + 176), A1_214 = ld2(rk1 + 176);
+    const double2 A0_215 = ld2(rk0 + 178), A1_215 = ld2(rk1 + 178);
     {
       const double dq = scr[29 * SST];
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 104), a1 = ld2(rk1 + 104);
+    { const double2 a0 = A0_209, a1 = A1_209;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc6 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 118), a1 = ld2(rk1 + 118);
+    { const double2 a0 = A0_210, a1 = A1_210;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc6 += 1.0 * dq;
       jc11 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 148), a1 = ld2(rk1 + 148);
+    { const double2 a0 = A0_211, a1 = A1_211;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc6 += 1.0 * dq;
       jc9 += 1.0 * dq;
@@ -4542,28 +4757,28 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 164), a1 = ld2(rk1 + 164);
+    { const double2 a0 = A0_212, a1 = A1_212;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc0 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 174), a1 = ld2(rk1 + 174);
+    { const double2 a0 = A0_213, a1 = A1_213;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
       jc0 += -1.0 * dq;
       jc1 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 176), a1 = ld2(rk1 + 176);
+    { const double2 a0 = A0_214, a1 = A1_214;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc6 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 178), a1 = ld2(rk1 + 178);
+    { const double2 a0 = A0_215, a1 = A1_215;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * 2.0*c6;
       jc6 += 2.0 * dq;
       jc11 += -1.0 * dq;
@@ -4646,6 +4861,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_216 = ld2(rk0 + 76), A1_216 = ld2(rk1 + 76);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -4718,7 +4934,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 76), a1 = ld2(rk1 + 76);
+    { const double2 a0 = A0_216, a1 = A1_216;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc6 += -1.0 * dq;
       jc7 += 1.0 * dq;
@@ -4742,14 +4958,23 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 90), a1 = ld2(rk1 + 90);
+    const double2 A0_217 = ld2(rk0 + 90), A1_217 = ld2(rk1 + 90);
+    const double2 A0_218 = ld2(rk0 + 94), A1_218 = ld2(rk1 + 94);
+    const double2 A0_219 = ld2(rk0 + 134), A1_219 = ld2(rk1 + 134);
+    const double2 A0_220 = ld2(rk0 + 136), A1_220 = ld2(rk1 + 136);
+    const double2 A0_221 = ld2(rk0 + 138), A1_221 = ld2(rk1 + 138);
+    const double2 A0_222 = ld2(rk0 + 140), A1_222 = ld2(rk1 + 140);
+    const double2 A0_223 = ld2(rk0 + 142), A1_223 = ld2(rk1 + 142);
+    const double2 A0_224 = ld2(rk0 + 144), A1_224 = ld2(rk1 + 144);
+    const double2 A0_225 = ld2(rk0 + 146), A1_225 = ld2(rk1 + 146);
+    { const double2 a0 = A0_217, a1 = A1_217;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 94), a1 = ld2(rk1 + 94);
+    { const double2 a0 = A0_218, a1 = A1_218;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc3 += -1.0 * dq;
       jc7 += 1.0 * dq;
@@ -4764,45 +4989,45 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 134), a1 = ld2(rk1 + 134);
+    { const double2 a0 = A0_219, a1 = A1_219;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 136), a1 = ld2(rk1 + 136);
+    { const double2 a0 = A0_220, a1 = A1_220;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c4;
       jc4 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc11 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 138), a1 = ld2(rk1 + 138);
+    { const double2 a0 = A0_221, a1 = A1_221;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
       jc7 += -1.0 * dq;
       jc16 += 0.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 140), a1 = ld2(rk1 + 140);
+    { const double2 a0 = A0_222, a1 = A1_222;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c21;
       jc7 += -1.0 * dq;
       jc11 += 2.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 142), a1 = ld2(rk1 + 142);
+    { const double2 a0 = A0_223, a1 = A1_223;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += 0.0 * dq;
       jc7 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 144), a1 = ld2(rk1 + 144);
+    { const double2 a0 = A0_224, a1 = A1_224;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c23;
       jc7 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc23 += 0.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 146), a1 = ld2(rk1 + 146);
+    { const double2 a0 = A0_225, a1 = A1_225;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c23;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -4897,12 +5122,20 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_226 = ld2(rk0 + 2), A1_226 = ld2(rk1 + 2);
+    const double2 A0_227 = ld2(rk0 + 38), A1_227 = ld2(rk1 + 38);
+    const double2 A0_228 = ld2(rk0 + 40), A1_228 = ld2(rk1 + 40);
+    const double2 A0_229 = ld2(rk0 + 42), A1_229 = ld2(rk1 + 42);
+    const double2 A0_230 = ld2(rk0 + 46), A1_230 = ld2(rk1 + 46);
+    const double2 A0_231 = ld2(rk0 + 48), A1_231 = ld2(rk1 + 48);
+    const double2 A0_232 = ld2(rk0 + 50), A1_232 = ld2(rk1 + 50);
+    const double2 A0_233 = ld2(rk0 + 52), A1_233 = ld2(rk1 + 52);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 2), a1 = ld2(rk1 + 2);
+    { const double2 a0 = A0_226, a1 = A1_226;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M1 * c9 + scr[1 * SST];
       jc3 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -4949,19 +5182,19 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 38), a1 = ld2(rk1 + 38);
+    { const double2 a0 = A0_227, a1 = A1_227;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M3 * c18 + scr[3 * SST];
       jc0 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 40), a1 = ld2(rk1 + 40);
+    { const double2 a0 = A0_228, a1 = A1_228;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M4 * c18;
       jc0 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 42), a1 = ld2(rk1 + 42);
+    { const double2 a0 = A0_229, a1 = A1_229;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M5 * c18;
       jc0 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -4974,27 +5207,38 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 46), a1 = ld2(rk1 + 46);
+    { const double2 a0 = A0_230, a1 = A1_230;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M6 * 2.0*c8 + scr[6 * SST];
       jc4 += 1.0 * dq;
       jc8 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 48), a1 = ld2(rk1 + 48);
+    { const double2 a0 = A0_231, a1 = A1_231;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M7 * 2.0*c8;
       jc4 += 1.0 * dq;
       jc8 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 50), a1 = ld2(rk1 + 50);
+    { const double2 a0 = A0_232, a1 = A1_232;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M5 * 2.0*c8;
       jc4 += 1.0 * dq;
       jc8 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 52), a1 = ld2(rk1 + 52);
+    { const double2 a0 = A0_233, a1 = A1_233;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M8 * 2.0*c8;
       jc4 += 1.0 * dq;
       jc8 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 54), a1 = ld2(rk1 + 54);
+    const double2 A0_234 = ld2(rk0 + 54), A1_234 = ld2(rk1 + 54);
+    const double2 A0_235 = ld2(rk0 + 58), A1_235 = ld2(rk1 + 58);
+    const double2 A0_236 = ld2(rk0 + 60), A1_236 = ld2(rk1 + 60);
+    const double2 A0_237 = ld2(rk0 + 62), A1_237 = ld2(rk1 + 62);
+    const double2 A0_238 = ld2(rk0 + 64), A1_238 = ld2(rk1 + 64);
+    const double2 A0_239 = ld2(rk0 + 66), A1_239 = ld2(rk1 + 66);
+    const double2 A0_240 = ld2(rk0 + 68), A1_240 = ld2(rk1 + 68);
+    const double2 A0_241 = ld2(rk0 + 70), A1_241 = ld2(rk1 + 70);
+    const double2 A0_242 = ld2(rk0 + 72), A1_242 = ld2(rk1 + 72);
+    const double2 A0_243 = ld2(rk0 + 74), A1_243 = ld2(rk1 + 74);
+    const double2 A0_244 = ld2(rk0 + 76), A1_244 = ld2(rk1 + 76);
+    { const double2 a0 = A0_234, a1 = A1_234;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M9 * c3 + scr[10 * SST];
       jc3 += -1.0 * dq;
       jc8 += -1.0 * dq;
@@ -5007,27 +5251,27 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 58), a1 = ld2(rk1 + 58);
+    { const double2 a0 = A0_235, a1 = A1_235;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
       jc0 += -1.0 * dq;
       jc4 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 60), a1 = ld2(rk1 + 60);
+    { const double2 a0 = A0_236, a1 = A1_236;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
       jc0 += -1.0 * dq;
       jc3 += 2.0 * dq;
       jc8 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 62), a1 = ld2(rk1 + 62);
+    { const double2 a0 = A0_237, a1 = A1_237;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c1;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc4 += 1.0 * dq;
       jc8 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 64), a1 = ld2(rk1 + 64);
+    { const double2 a0 = A0_238, a1 = A1_238;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c1;
       jc1 += -1.0 * dq;
       jc3 += 1.0 * dq;
@@ -5046,7 +5290,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 66), a1 = ld2(rk1 + 66);
+    { const double2 a0 = A0_239, a1 = A1_239;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c21;
       jc4 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -5059,7 +5303,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 68), a1 = ld2(rk1 + 68);
+    { const double2 a0 = A0_240, a1 = A1_240;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c5;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
@@ -5072,34 +5316,44 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 70), a1 = ld2(rk1 + 70);
+    { const double2 a0 = A0_241, a1 = A1_241;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c13;
       jc4 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 72), a1 = ld2(rk1 + 72);
+    { const double2 a0 = A0_242, a1 = A1_242;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc4 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 74), a1 = ld2(rk1 + 74);
+    { const double2 a0 = A0_243, a1 = A1_243;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc3 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc11 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 76), a1 = ld2(rk1 + 76);
+    { const double2 a0 = A0_244, a1 = A1_244;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc6 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
+    const double2 A0_245 = ld2(rk0 + 78), A1_245 = ld2(rk1 + 78);
+    const double2 A0_246 = ld2(rk0 + 88), A1_246 = ld2(rk1 + 88);
+    const double2 A0_247 = ld2(rk0 + 90), A1_247 = ld2(rk1 + 90);
+    const double2 A0_248 = ld2(rk0 + 98), A1_248 = ld2(rk1 + 98);
+    const double2 A0_249 = ld2(rk0 + 126), A1_249 = ld2(rk1 + 126);
+    const double2 A0_250 = ld2(rk0 + 136), A1_250 = ld2(rk1 + 136);
+    const double2 A0_251 = ld2(rk0 + 158), A1_251 = ld2(rk1 + 158);
+    const double2 A0_252 = ld2(rk0 + 160), A1_252 = ld2(rk1 + 160);
+    const double2 A0_253 = ld2(rk0 + 168), A1_253 = ld2(rk1 + 168);
+    const double2 A0_254 = ld2(rk0 + 198), A1_254 = ld2(rk1 + 198);
     {
       const double dq = scr[24 * SST] * c15 + scr[23 * SST];
       jc8 += -1.0 * dq;
@@ -5112,7 +5366,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 78), a1 = ld2(rk1 + 78);
+    { const double2 a0 = A0_245, a1 = A1_245;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc3 += -1.0 * dq;
       jc4 += -1.0 * dq;
@@ -5124,28 +5378,28 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 88), a1 = ld2(rk1 + 88);
+    { const double2 a0 = A0_246, a1 = A1_246;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
       jc3 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc13 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 90), a1 = ld2(rk1 + 90);
+    { const double2 a0 = A0_247, a1 = A1_247;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
       jc3 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 98), a1 = ld2(rk1 + 98);
+    { const double2 a0 = A0_248, a1 = A1_248;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c23;
       jc2 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 126), a1 = ld2(rk1 + 126);
+    { const double2 a0 = A0_249, a1 = A1_249;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c11;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
@@ -5160,7 +5414,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 136), a1 = ld2(rk1 + 136);
+    { const double2 a0 = A0_250, a1 = A1_250;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c11;
       jc4 += -1.0 * dq;
       jc7 += -1.0 * dq;
@@ -5172,19 +5426,19 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -2.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 158), a1 = ld2(rk1 + 158);
+    { const double2 a0 = A0_251, a1 = A1_251;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M5 * c2;
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 160), a1 = ld2(rk1 + 160);
+    { const double2 a0 = A0_252, a1 = A1_252;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M10 * c2 + scr[36 * SST];
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 168), a1 = ld2(rk1 + 168);
+    { const double2 a0 = A0_253, a1 = A1_253;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c12;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
@@ -5197,7 +5451,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 198), a1 = ld2(rk1 + 198);
+    { const double2 a0 = A0_254, a1 = A1_254;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M11 * c23 + scr[38 * SST];
       jc8 += -1.0 * dq;
       jc22 += 1.0 * dq;
@@ -5269,12 +5523,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
-    { const double2 a0 = ld2(rk0 + 0), a1 = ld2(rk1 + 0);
+    const double2 A0_255 = ld2(rk0 + 0), A1_255 = ld2(rk1 + 0);
+    const double2 A0_256 = ld2(rk0 + 2), A1_256 = ld2(rk1 + 2);
+    const double2 A0_257 = ld2(rk0 + 20), A1_257 = ld2(rk1 + 20);
+    { const double2 a0 = A0_255, a1 = A1_255;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M0 * 2.0*c9 + scr[0 * SST];
       jc9 += -2.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 2), a1 = ld2(rk1 + 2);
+    { const double2 a0 = A0_256, a1 = A1_256;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M1 * c8 + scr[1 * SST];
       jc3 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -5336,7 +5593,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 20), a1 = ld2(rk1 + 20);
+    { const double2 a0 = A0_257, a1 = A1_257;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M2 * c2 + scr[2 * SST];
       jc2 += -1.0 * dq;
       jc9 += -1.0 * dq;
@@ -5377,6 +5634,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += -1.0 * dq;
       jc15 += 1.0 * dq;
     }
+    const double2 A0_258 = ld2(rk0 + 80), A1_258 = ld2(rk1 + 80);
+    const double2 A0_259 = ld2(rk0 + 148), A1_259 = ld2(rk1 + 148);
     {
       const double dq = scr[69 * SST] * c12;
       jc2 += 1.0 * dq;
@@ -5463,19 +5722,20 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 80), a1 = ld2(rk1 + 80);
+    { const double2 a0 = A0_258, a1 = A1_258;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc3 += -2.0 * dq;
       jc9 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 148), a1 = ld2(rk1 + 148);
+    { const double2 a0 = A0_259, a1 = A1_259;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c6;
       jc6 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
+    const double2 A0_260 = ld2(rk0 + 188), A1_260 = ld2(rk1 + 188);
     {
       const double dq = scr[32 * SST];
       jc11 += -2.0 * dq;
@@ -5487,7 +5747,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 188), a1 = ld2(rk1 + 188);
+    { const double2 a0 = A0_260, a1 = A1_260;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c14;
       jc9 += -1.0 * dq;
       jc13 += 1.0 * dq;
@@ -5572,6 +5832,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_261 = ld2(rk0 + 194), A1_261 = ld2(rk1 + 194);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -5658,7 +5919,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 194), a1 = ld2(rk1 + 194);
+    { const double2 a0 = A0_261, a1 = A1_261;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
       jc3 += 1.0 * dq;
       jc10 += -1.0 * dq;
@@ -5743,6 +6004,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_262 = ld2(rk0 + 66), A1_262 = ld2(rk1 + 66);
+    const double2 A0_263 = ld2(rk0 + 74), A1_263 = ld2(rk1 + 74);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -5803,7 +6066,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 66), a1 = ld2(rk1 + 66);
+    { const double2 a0 = A0_262, a1 = A1_262;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc4 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -5822,7 +6085,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 74), a1 = ld2(rk1 + 74);
+    { const double2 a0 = A0_263, a1 = A1_263;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc3 += 1.0 * dq;
       jc6 += -1.0 * dq;
@@ -5841,94 +6104,108 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
+    const double2 A0_264 = ld2(rk0 + 92), A1_264 = ld2(rk1 + 92);
+    const double2 A0_265 = ld2(rk0 + 94), A1_265 = ld2(rk1 + 94);
+    const double2 A0_266 = ld2(rk0 + 96), A1_266 = ld2(rk1 + 96);
+    const double2 A0_267 = ld2(rk0 + 108), A1_267 = ld2(rk1 + 108);
+    const double2 A0_268 = ld2(rk0 + 116), A1_268 = ld2(rk1 + 116);
+    const double2 A0_269 = ld2(rk0 + 118), A1_269 = ld2(rk1 + 118);
+    const double2 A0_270 = ld2(rk0 + 126), A1_270 = ld2(rk1 + 126);
+    const double2 A0_271 = ld2(rk0 + 130), A1_271 = ld2(rk1 + 130);
+    const double2 A0_272 = ld2(rk0 + 136), A1_272 = ld2(rk1 + 136);
+    const double2 A0_273 = ld2(rk0 + 140), A1_273 = ld2(rk1 + 140);
+    const double2 A0_274 = ld2(rk0 + 148), A1_274 = ld2(rk1 + 148);
+    const double2 A0_275 = ld2(rk0 + 150), A1_275 = ld2(rk1 + 150);
+    const double2 A0_276 = ld2(rk0 + 152), A1_276 = ld2(rk1 + 152);
+    const double2 A0_277 = ld2(rk0 + 154), A1_277 = ld2(rk1 + 154);
     {
       const double dq = scr[29 * SST];
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 92), a1 = ld2(rk1 + 92);
+    { const double2 a0 = A0_264, a1 = A1_264;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 94), a1 = ld2(rk1 + 94);
+    { const double2 a0 = A0_265, a1 = A1_265;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 96), a1 = ld2(rk1 + 96);
+    { const double2 a0 = A0_266, a1 = A1_266;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc3 += -1.0 * dq;
       jc11 += 1.0 * dq;
       jc16 += 1.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 108), a1 = ld2(rk1 + 108);
+    { const double2 a0 = A0_267, a1 = A1_267;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc11 += 1.0 * dq;
       jc12 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 116), a1 = ld2(rk1 + 116);
+    { const double2 a0 = A0_268, a1 = A1_268;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
       jc0 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 118), a1 = ld2(rk1 + 118);
+    { const double2 a0 = A0_269, a1 = A1_269;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc6 += 1.0 * dq;
       jc11 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 126), a1 = ld2(rk1 + 126);
+    { const double2 a0 = A0_270, a1 = A1_270;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc11 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 130), a1 = ld2(rk1 + 130);
+    { const double2 a0 = A0_271, a1 = A1_271;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * 2.0*c11;
       jc11 += 2.0 * dq;
       jc19 += -1.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 136), a1 = ld2(rk1 + 136);
+    { const double2 a0 = A0_272, a1 = A1_272;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
       jc4 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc11 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 140), a1 = ld2(rk1 + 140);
+    { const double2 a0 = A0_273, a1 = A1_273;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * 2.0*c11;
       jc7 += -1.0 * dq;
       jc11 += 2.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 148), a1 = ld2(rk1 + 148);
+    { const double2 a0 = A0_274, a1 = A1_274;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc6 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 150), a1 = ld2(rk1 + 150);
+    { const double2 a0 = A0_275, a1 = A1_275;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 152), a1 = ld2(rk1 + 152);
+    { const double2 a0 = A0_276, a1 = A1_276;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c1;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
@@ -5940,14 +6217,19 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -2.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 154), a1 = ld2(rk1 + 154);
+    { const double2 a0 = A0_277, a1 = A1_277;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c5;
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 156), a1 = ld2(rk1 + 156);
+    const double2 A0_278 = ld2(rk0 + 156), A1_278 = ld2(rk1 + 156);
+    const double2 A0_279 = ld2(rk0 + 170), A1_279 = ld2(rk1 + 170);
+    const double2 A0_280 = ld2(rk0 + 172), A1_280 = ld2(rk1 + 172);
+    const double2 A0_281 = ld2(rk0 + 176), A1_281 = ld2(rk1 + 176);
+    const double2 A0_282 = ld2(rk0 + 178), A1_282 = ld2(rk1 + 178);
+    { const double2 a0 = A0_278, a1 = A1_278;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c13;
       jc5 += 1.0 * dq;
       jc11 += -1.0 * dq;
@@ -5960,27 +6242,27 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
+    { const double2 a0 = A0_279, a1 = A1_279;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc4 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 172), a1 = ld2(rk1 + 172);
+    { const double2 a0 = A0_280, a1 = A1_280;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc11 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 176), a1 = ld2(rk1 + 176);
+    { const double2 a0 = A0_281, a1 = A1_281;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc6 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 178), a1 = ld2(rk1 + 178);
+    { const double2 a0 = A0_282, a1 = A1_282;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c17;
       jc6 += 2.0 * dq;
       jc11 += -1.0 * dq;
@@ -6062,6 +6344,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_283 = ld2(rk0 + 106), A1_283 = ld2(rk1 + 106);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -6151,21 +6434,25 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 106), a1 = ld2(rk1 + 106);
+    { const double2 a0 = A0_283, a1 = A1_283;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 108), a1 = ld2(rk1 + 108);
+    const double2 A0_284 = ld2(rk0 + 108), A1_284 = ld2(rk1 + 108);
+    const double2 A0_285 = ld2(rk0 + 128), A1_285 = ld2(rk1 + 128);
+    const double2 A0_286 = ld2(rk0 + 168), A1_286 = ld2(rk1 + 168);
+    const double2 A0_287 = ld2(rk0 + 180), A1_287 = ld2(rk1 + 180);
+    { const double2 a0 = A0_284, a1 = A1_284;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc11 += 1.0 * dq;
       jc12 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 128), a1 = ld2(rk1 + 128);
+    { const double2 a0 = A0_285, a1 = A1_285;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc4 += 1.0 * dq;
       jc12 += 1.0 * dq;
@@ -6182,14 +6469,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 168), a1 = ld2(rk1 + 168);
+    { const double2 a0 = A0_286, a1 = A1_286;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc12 += 1.0 * dq;
       jc15 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 180), a1 = ld2(rk1 + 180);
+    { const double2 a0 = A0_287, a1 = A1_287;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc5 += 2.0 * dq;
       jc12 += -1.0 * dq;
@@ -6273,6 +6560,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_288 = ld2(rk0 + 70), A1_288 = ld2(rk1 + 70);
+    const double2 A0_289 = ld2(rk0 + 72), A1_289 = ld2(rk1 + 72);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -6359,20 +6648,31 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 70), a1 = ld2(rk1 + 70);
+    { const double2 a0 = A0_288, a1 = A1_288;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc4 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 72), a1 = ld2(rk1 + 72);
+    { const double2 a0 = A0_289, a1 = A1_289;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc4 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
+    const double2 A0_290 = ld2(rk0 + 88), A1_290 = ld2(rk1 + 88);
+    const double2 A0_291 = ld2(rk0 + 90), A1_291 = ld2(rk1 + 90);
+    const double2 A0_292 = ld2(rk0 + 102), A1_292 = ld2(rk1 + 102);
+    const double2 A0_293 = ld2(rk0 + 104), A1_293 = ld2(rk1 + 104);
+    const double2 A0_294 = ld2(rk0 + 114), A1_294 = ld2(rk1 + 114);
+    const double2 A0_295 = ld2(rk0 + 122), A1_295 = ld2(rk1 + 122);
+    const double2 A0_296 = ld2(rk0 + 146), A1_296 = ld2(rk1 + 146);
+    const double2 A0_297 = ld2(rk0 + 150), A1_297 = ld2(rk1 + 150);
+    const double2 A0_298 = ld2(rk0 + 156), A1_298 = ld2(rk1 + 156);
+    const double2 A0_299 = ld2(rk0 + 164), A1_299 = ld2(rk1 + 164);
+    const double2 A0_300 = ld2(rk0 + 170), A1_300 = ld2(rk1 + 170);
     {
       const double dq = scr[23 * SST];
       jc8 += -1.0 * dq;
@@ -6390,56 +6690,56 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 88), a1 = ld2(rk1 + 88);
+    { const double2 a0 = A0_290, a1 = A1_290;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
       jc3 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc13 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 90), a1 = ld2(rk1 + 90);
+    { const double2 a0 = A0_291, a1 = A1_291;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
       jc3 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 102), a1 = ld2(rk1 + 102);
+    { const double2 a0 = A0_292, a1 = A1_292;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 104), a1 = ld2(rk1 + 104);
+    { const double2 a0 = A0_293, a1 = A1_293;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc3 += -1.0 * dq;
       jc6 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 114), a1 = ld2(rk1 + 114);
+    { const double2 a0 = A0_294, a1 = A1_294;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc13 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 122), a1 = ld2(rk1 + 122);
+    { const double2 a0 = A0_295, a1 = A1_295;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
       jc0 += -1.0 * dq;
       jc1 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 146), a1 = ld2(rk1 + 146);
+    { const double2 a0 = A0_296, a1 = A1_296;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc23 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 150), a1 = ld2(rk1 + 150);
+    { const double2 a0 = A0_297, a1 = A1_297;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc3 += 1.0 * dq;
       jc11 += -1.0 * dq;
@@ -6451,7 +6751,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -2.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 156), a1 = ld2(rk1 + 156);
+    { const double2 a0 = A0_298, a1 = A1_298;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc5 += 1.0 * dq;
       jc11 += -1.0 * dq;
@@ -6464,35 +6764,38 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 164), a1 = ld2(rk1 + 164);
+    { const double2 a0 = A0_299, a1 = A1_299;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c0;
       jc0 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
+    { const double2 a0 = A0_300, a1 = A1_300;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc3 += -1.0 * dq;
       jc4 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 174), a1 = ld2(rk1 + 174);
+    const double2 A0_301 = ld2(rk0 + 174), A1_301 = ld2(rk1 + 174);
+    const double2 A0_302 = ld2(rk0 + 176), A1_302 = ld2(rk1 + 176);
+    const double2 A0_303 = ld2(rk0 + 190), A1_303 = ld2(rk1 + 190);
+    { const double2 a0 = A0_301, a1 = A1_301;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c1;
       jc0 += -1.0 * dq;
       jc1 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 176), a1 = ld2(rk1 + 176);
+    { const double2 a0 = A0_302, a1 = A1_302;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c21;
       jc6 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 190), a1 = ld2(rk1 + 190);
+    { const double2 a0 = A0_303, a1 = A1_303;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc9 += 1.0 * dq;
       jc13 += -1.0 * dq;
@@ -6577,6 +6880,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_304 = ld2(rk0 + 188), A1_304 = ld2(rk1 + 188);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -6663,14 +6967,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 188), a1 = ld2(rk1 + 188);
+    { const double2 a0 = A0_304, a1 = A1_304;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
       jc9 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc14 += -1.0 * dq;
       jc15 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 192), a1 = ld2(rk1 + 192);
+    const double2 A0_305 = ld2(rk0 + 192), A1_305 = ld2(rk1 + 192);
+    { const double2 a0 = A0_305, a1 = A1_305;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc10 += 1.0 * dq;
@@ -6754,6 +7059,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_306 = ld2(rk0 + 106), A1_306 = ld2(rk1 + 106);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -6836,7 +7142,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 106), a1 = ld2(rk1 + 106);
+    { const double2 a0 = A0_306, a1 = A1_306;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc3 += -1.0 * dq;
       jc12 += -1.0 * dq;
@@ -6848,27 +7154,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -2.0 * dq;
       jc20 += 1.0 * dq;
     }
+    const double2 A0_307 = ld2(rk0 + 166), A1_307 = ld2(rk1 + 166);
+    const double2 A0_308 = ld2(rk0 + 168), A1_308 = ld2(rk1 + 168);
+    const double2 A0_309 = ld2(rk0 + 190), A1_309 = ld2(rk1 + 190);
     {
       const double dq = scr[36 * SST];
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 166), a1 = ld2(rk1 + 166);
+    { const double2 a0 = A0_307, a1 = A1_307;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc2 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 168), a1 = ld2(rk1 + 168);
+    { const double2 a0 = A0_308, a1 = A1_308;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c4;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc12 += 1.0 * dq;
       jc15 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 190), a1 = ld2(rk1 + 190);
+    { const double2 a0 = A0_309, a1 = A1_309;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c13;
       jc9 += 1.0 * dq;
       jc13 += -1.0 * dq;
@@ -6954,6 +7263,10 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_310 = ld2(rk0 + 54), A1_310 = ld2(rk1 + 54);
+    const double2 A0_311 = ld2(rk0 + 64), A1_311 = ld2(rk1 + 64);
+    const double2 A0_312 = ld2(rk0 + 76), A1_312 = ld2(rk1 + 76);
+    const double2 A0_313 = ld2(rk0 + 78), A1_313 = ld2(rk1 + 78);
     {
       const double dq = 15.399999618530273 * scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -6982,7 +7295,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += 1.0 * dq;
       jc8 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 54), a1 = ld2(rk1 + 54);
+    { const double2 a0 = A0_310, a1 = A1_310;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M9 * 1.0 + 3.6500000953674316 * scr[10 * SST];
       jc3 += -1.0 * dq;
       jc8 += -1.0 * dq;
@@ -6995,7 +7308,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 64), a1 = ld2(rk1 + 64);
+    { const double2 a0 = A0_311, a1 = A1_311;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc1 += -1.0 * dq;
       jc3 += 1.0 * dq;
@@ -7026,7 +7339,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 76), a1 = ld2(rk1 + 76);
+    { const double2 a0 = A0_312, a1 = A1_312;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c7;
       jc6 += -1.0 * dq;
       jc7 += 1.0 * dq;
@@ -7045,102 +7358,115 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 78), a1 = ld2(rk1 + 78);
+    { const double2 a0 = A0_313, a1 = A1_313;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
       jc3 += -1.0 * dq;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
+    const double2 A0_314 = ld2(rk0 + 80), A1_314 = ld2(rk1 + 80);
+    const double2 A0_315 = ld2(rk0 + 82), A1_315 = ld2(rk1 + 82);
+    const double2 A0_316 = ld2(rk0 + 84), A1_316 = ld2(rk1 + 84);
+    const double2 A0_317 = ld2(rk0 + 86), A1_317 = ld2(rk1 + 86);
+    const double2 A0_318 = ld2(rk0 + 92), A1_318 = ld2(rk1 + 92);
+    const double2 A0_319 = ld2(rk0 + 94), A1_319 = ld2(rk1 + 94);
+    const double2 A0_320 = ld2(rk0 + 96), A1_320 = ld2(rk1 + 96);
+    const double2 A0_321 = ld2(rk0 + 100), A1_321 = ld2(rk1 + 100);
+    const double2 A0_322 = ld2(rk0 + 102), A1_322 = ld2(rk1 + 102);
+    const double2 A0_323 = ld2(rk0 + 104), A1_323 = ld2(rk1 + 104);
+    const double2 A0_324 = ld2(rk0 + 106), A1_324 = ld2(rk1 + 106);
+    const double2 A0_325 = ld2(rk0 + 134), A1_325 = ld2(rk1 + 134);
+    const double2 A0_326 = ld2(rk0 + 138), A1_326 = ld2(rk1 + 138);
     {
       const double dq = 6.0 * scr[29 * SST];
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 80), a1 = ld2(rk1 + 80);
+    { const double2 a0 = A0_314, a1 = A1_314;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc3 += -2.0 * dq;
       jc9 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 82), a1 = ld2(rk1 + 82);
+    { const double2 a0 = A0_315, a1 = A1_315;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc0 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 84), a1 = ld2(rk1 + 84);
+    { const double2 a0 = A0_316, a1 = A1_316;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c0;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 86), a1 = ld2(rk1 + 86);
+    { const double2 a0 = A0_317, a1 = A1_317;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c0;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 92), a1 = ld2(rk1 + 92);
+    { const double2 a0 = A0_318, a1 = A1_318;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc3 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 94), a1 = ld2(rk1 + 94);
+    { const double2 a0 = A0_319, a1 = A1_319;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c7;
       jc3 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 96), a1 = ld2(rk1 + 96);
+    { const double2 a0 = A0_320, a1 = A1_320;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c11;
       jc3 += -1.0 * dq;
       jc11 += 1.0 * dq;
       jc16 += 1.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 100), a1 = ld2(rk1 + 100);
+    { const double2 a0 = A0_321, a1 = A1_321;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc5 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 102), a1 = ld2(rk1 + 102);
+    { const double2 a0 = A0_322, a1 = A1_322;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c5;
       jc3 += -1.0 * dq;
       jc5 += 1.0 * dq;
       jc13 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 104), a1 = ld2(rk1 + 104);
+    { const double2 a0 = A0_323, a1 = A1_323;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
       jc3 += -1.0 * dq;
       jc6 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 106), a1 = ld2(rk1 + 106);
+    { const double2 a0 = A0_324, a1 = A1_324;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c15;
       jc3 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 134), a1 = ld2(rk1 + 134);
+    { const double2 a0 = A0_325, a1 = A1_325;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 138), a1 = ld2(rk1 + 138);
+    { const double2 a0 = A0_326, a1 = A1_326;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7 - (a0.y + (a1.y - a0.y) * Td) * c19;
       jc7 += -1.0 * dq;
       jc16 += 0.0 * dq;
@@ -7157,7 +7483,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 194), a1 = ld2(rk1 + 194);
+    const double2 A0_327 = ld2(rk0 + 194), A1_327 = ld2(rk1 + 194);
+    { const double2 a0 = A0_327, a1 = A1_327;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c10;
       jc3 += 1.0 * dq;
       jc10 += -1.0 * dq;
@@ -7241,6 +7568,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_328 = ld2(rk0 + 172), A1_328 = ld2(rk1 + 172);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -7327,19 +7655,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 172), a1 = ld2(rk1 + 172);
+    { const double2 a0 = A0_328, a1 = A1_328;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * 1.0;
       jc11 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 178), a1 = ld2(rk1 + 178);
+    const double2 A0_329 = ld2(rk0 + 178), A1_329 = ld2(rk1 + 178);
+    const double2 A0_330 = ld2(rk0 + 184), A1_330 = ld2(rk1 + 184);
+    const double2 A0_331 = ld2(rk0 + 186), A1_331 = ld2(rk1 + 186);
+    { const double2 a0 = A0_329, a1 = A1_329;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc6 += 2.0 * dq;
       jc11 += -1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 184), a1 = ld2(rk1 + 184);
+    { const double2 a0 = A0_330, a1 = A1_330;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
@@ -7347,7 +7678,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc17 += -1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 186), a1 = ld2(rk1 + 186);
+    { const double2 a0 = A0_331, a1 = A1_331;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c17;
       jc6 += 2.0 * dq;
       jc17 += -2.0 * dq;
@@ -7431,7 +7762,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
-    { const double2 a0 = ld2(rk0 + 0), a1 = ld2(rk1 + 0);
+    const double2 A0_332 = ld2(rk0 + 0), A1_332 = ld2(rk1 + 0);
+    const double2 A0_333 = ld2(rk0 + 38), A1_333 = ld2(rk1 + 38);
+    const double2 A0_334 = ld2(rk0 + 40), A1_334 = ld2(rk1 + 40);
+    const double2 A0_335 = ld2(rk0 + 42), A1_335 = ld2(rk1 + 42);
+    const double2 A0_336 = ld2(rk0 + 58), A1_336 = ld2(rk1 + 58);
+    { const double2 a0 = A0_332, a1 = A1_332;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M0 * 1.0 + scr[0 * SST];
       jc9 += -2.0 * dq;
       jc18 += 1.0 * dq;
@@ -7469,19 +7805,19 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc13 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 38), a1 = ld2(rk1 + 38);
+    { const double2 a0 = A0_333, a1 = A1_333;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M3 * c8;
       jc0 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 40), a1 = ld2(rk1 + 40);
+    { const double2 a0 = A0_334, a1 = A1_334;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M4 * c8 + scr[4 * SST];
       jc0 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 42), a1 = ld2(rk1 + 42);
+    { const double2 a0 = A0_335, a1 = A1_335;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M5 * c8;
       jc0 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -7505,7 +7841,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc16 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 58), a1 = ld2(rk1 + 58);
+    { const double2 a0 = A0_336, a1 = A1_336;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc0 += -1.0 * dq;
       jc4 += 1.0 * dq;
@@ -7530,6 +7866,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc13 += 1.0 * dq;
     }
+    const double2 A0_337 = ld2(rk0 + 82), A1_337 = ld2(rk1 + 82);
+    const double2 A0_338 = ld2(rk0 + 110), A1_338 = ld2(rk1 + 110);
+    const double2 A0_339 = ld2(rk0 + 112), A1_339 = ld2(rk1 + 112);
+    const double2 A0_340 = ld2(rk0 + 116), A1_340 = ld2(rk1 + 116);
+    const double2 A0_341 = ld2(rk0 + 124), A1_341 = ld2(rk1 + 124);
+    const double2 A0_342 = ld2(rk0 + 134), A1_342 = ld2(rk1 + 134);
+    const double2 A0_343 = ld2(rk0 + 148), A1_343 = ld2(rk1 + 148);
+    const double2 A0_344 = ld2(rk0 + 150), A1_344 = ld2(rk1 + 150);
+    const double2 A0_345 = ld2(rk0 + 162), A1_345 = ld2(rk1 + 162);
     {
       const double dq = scr[20 * SST];
       jc6 += 1.0 * dq;
@@ -7553,33 +7898,33 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 82), a1 = ld2(rk1 + 82);
+    { const double2 a0 = A0_337, a1 = A1_337;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc0 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 110), a1 = ld2(rk1 + 110);
+    { const double2 a0 = A0_338, a1 = A1_338;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c1;
       jc0 += -2.0 * dq;
       jc1 += 1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 112), a1 = ld2(rk1 + 112);
+    { const double2 a0 = A0_339, a1 = A1_339;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c1;
       jc0 += -2.0 * dq;
       jc1 += 1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 116), a1 = ld2(rk1 + 116);
+    { const double2 a0 = A0_340, a1 = A1_340;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c21;
       jc0 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 124), a1 = ld2(rk1 + 124);
+    { const double2 a0 = A0_341, a1 = A1_341;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
       jc3 += 1.0 * dq;
       jc5 += 1.0 * dq;
@@ -7594,21 +7939,21 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 134), a1 = ld2(rk1 + 134);
+    { const double2 a0 = A0_342, a1 = A1_342;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 148), a1 = ld2(rk1 + 148);
+    { const double2 a0 = A0_343, a1 = A1_343;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc6 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 150), a1 = ld2(rk1 + 150);
+    { const double2 a0 = A0_344, a1 = A1_344;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc3 += 1.0 * dq;
       jc11 += -1.0 * dq;
@@ -7626,34 +7971,38 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 162), a1 = ld2(rk1 + 162);
+    { const double2 a0 = A0_345, a1 = A1_345;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c5;
       jc0 += 1.0 * dq;
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 164), a1 = ld2(rk1 + 164);
+    const double2 A0_346 = ld2(rk0 + 164), A1_346 = ld2(rk1 + 164);
+    const double2 A0_347 = ld2(rk0 + 166), A1_347 = ld2(rk1 + 166);
+    const double2 A0_348 = ld2(rk0 + 172), A1_348 = ld2(rk1 + 172);
+    const double2 A0_349 = ld2(rk0 + 180), A1_349 = ld2(rk1 + 180);
+    { const double2 a0 = A0_346, a1 = A1_346;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc0 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc13 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 166), a1 = ld2(rk1 + 166);
+    { const double2 a0 = A0_347, a1 = A1_347;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc2 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 172), a1 = ld2(rk1 + 172);
+    { const double2 a0 = A0_348, a1 = A1_348;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc11 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 180), a1 = ld2(rk1 + 180);
+    { const double2 a0 = A0_349, a1 = A1_349;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc5 += 2.0 * dq;
       jc12 += -1.0 * dq;
@@ -7736,6 +8085,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_350 = ld2(rk0 + 88), A1_350 = ld2(rk1 + 88);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -7825,66 +8175,75 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 88), a1 = ld2(rk1 + 88);
+    { const double2 a0 = A0_350, a1 = A1_350;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc13 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 92), a1 = ld2(rk1 + 92);
+    const double2 A0_351 = ld2(rk0 + 92), A1_351 = ld2(rk1 + 92);
+    const double2 A0_352 = ld2(rk0 + 114), A1_352 = ld2(rk1 + 114);
+    const double2 A0_353 = ld2(rk0 + 124), A1_353 = ld2(rk1 + 124);
+    const double2 A0_354 = ld2(rk0 + 126), A1_354 = ld2(rk1 + 126);
+    const double2 A0_355 = ld2(rk0 + 128), A1_355 = ld2(rk1 + 128);
+    const double2 A0_356 = ld2(rk0 + 130), A1_356 = ld2(rk1 + 130);
+    const double2 A0_357 = ld2(rk0 + 138), A1_357 = ld2(rk1 + 138);
+    const double2 A0_358 = ld2(rk0 + 142), A1_358 = ld2(rk1 + 142);
+    const double2 A0_359 = ld2(rk0 + 144), A1_359 = ld2(rk1 + 144);
+    { const double2 a0 = A0_351, a1 = A1_351;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc3 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc16 += 1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 114), a1 = ld2(rk1 + 114);
+    { const double2 a0 = A0_352, a1 = A1_352;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c0;
       jc0 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc13 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 124), a1 = ld2(rk1 + 124);
+    { const double2 a0 = A0_353, a1 = A1_353;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc5 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 126), a1 = ld2(rk1 + 126);
+    { const double2 a0 = A0_354, a1 = A1_354;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c4;
       jc4 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc11 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 128), a1 = ld2(rk1 + 128);
+    { const double2 a0 = A0_355, a1 = A1_355;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c19;
       jc4 += 1.0 * dq;
       jc12 += 1.0 * dq;
       jc19 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 130), a1 = ld2(rk1 + 130);
+    { const double2 a0 = A0_356, a1 = A1_356;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c21;
       jc11 += 2.0 * dq;
       jc19 += -1.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 138), a1 = ld2(rk1 + 138);
+    { const double2 a0 = A0_357, a1 = A1_357;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc7 += -1.0 * dq;
       jc16 += 0.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 142), a1 = ld2(rk1 + 142);
+    { const double2 a0 = A0_358, a1 = A1_358;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 0.0 * dq;
       jc7 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 144), a1 = ld2(rk1 + 144);
+    { const double2 a0 = A0_359, a1 = A1_359;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c23;
       jc7 += -1.0 * dq;
       jc19 += 1.0 * dq;
@@ -8140,6 +8499,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_360 = ld2(rk0 + 66), A1_360 = ld2(rk1 + 66);
+    const double2 A0_361 = ld2(rk0 + 96), A1_361 = ld2(rk1 + 96);
     {
       const double dq = 2.0 * scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -8193,7 +8554,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 66), a1 = ld2(rk1 + 66);
+    { const double2 a0 = A0_360, a1 = A1_360;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc4 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -8229,33 +8590,40 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 96), a1 = ld2(rk1 + 96);
+    { const double2 a0 = A0_361, a1 = A1_361;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc11 += 1.0 * dq;
       jc16 += 1.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 116), a1 = ld2(rk1 + 116);
+    const double2 A0_362 = ld2(rk0 + 116), A1_362 = ld2(rk1 + 116);
+    const double2 A0_363 = ld2(rk0 + 130), A1_363 = ld2(rk1 + 130);
+    const double2 A0_364 = ld2(rk0 + 140), A1_364 = ld2(rk1 + 140);
+    const double2 A0_365 = ld2(rk0 + 152), A1_365 = ld2(rk1 + 152);
+    const double2 A0_366 = ld2(rk0 + 154), A1_366 = ld2(rk1 + 154);
+    const double2 A0_367 = ld2(rk0 + 156), A1_367 = ld2(rk1 + 156);
+    const double2 A0_368 = ld2(rk0 + 176), A1_368 = ld2(rk1 + 176);
+    { const double2 a0 = A0_362, a1 = A1_362;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc0 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 130), a1 = ld2(rk1 + 130);
+    { const double2 a0 = A0_363, a1 = A1_363;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
       jc11 += 2.0 * dq;
       jc19 += -1.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 140), a1 = ld2(rk1 + 140);
+    { const double2 a0 = A0_364, a1 = A1_364;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc7 += -1.0 * dq;
       jc11 += 2.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 152), a1 = ld2(rk1 + 152);
+    { const double2 a0 = A0_365, a1 = A1_365;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c0;
       jc0 += 1.0 * dq;
       jc1 += -1.0 * dq;
@@ -8267,14 +8635,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += -2.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 154), a1 = ld2(rk1 + 154);
+    { const double2 a0 = A0_366, a1 = A1_366;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc5 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc21 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 156), a1 = ld2(rk1 + 156);
+    { const double2 a0 = A0_367, a1 = A1_367;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c5;
       jc5 += 1.0 * dq;
       jc11 += -1.0 * dq;
@@ -8287,7 +8655,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 176), a1 = ld2(rk1 + 176);
+    { const double2 a0 = A0_368, a1 = A1_368;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c13;
       jc6 += -1.0 * dq;
       jc11 += -1.0 * dq;
@@ -8370,6 +8738,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_369 = ld2(rk0 + 196), A1_369 = ld2(rk1 + 196);
     {
       const double dq = scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -8456,19 +8825,20 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc5 += -1.0 * dq;
       jc8 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 196), a1 = ld2(rk1 + 196);
+    { const double2 a0 = A0_369, a1 = A1_369;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M11 * 1.0 + scr[37 * SST];
       jc8 += 1.0 * dq;
       jc22 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
+    const double2 A0_370 = ld2(rk0 + 200), A1_370 = ld2(rk1 + 200);
     {
       const double dq = scr[38 * SST];
       jc8 += -1.0 * dq;
       jc22 += 1.0 * dq;
       jc23 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 200), a1 = ld2(rk1 + 200);
+    { const double2 a0 = A0_370, a1 = A1_370;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M11 * 1.0 + scr[39 * SST];
       jc2 += 1.0 * dq;
       jc3 += 1.0 * dq;
@@ -8533,6 +8903,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc21 = 0.0;
     double jc22 = 0.0;
     double jc23 = 0.0;
+    const double2 A0_371 = ld2(rk0 + 20), A1_371 = ld2(rk1 + 20);
+    const double2 A0_372 = ld2(rk0 + 98), A1_372 = ld2(rk1 + 98);
     {
       const double dq = 3.5999999046325684 * scr[0 * SST];
       jc9 += -2.0 * dq;
@@ -8544,7 +8916,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += -1.0 * dq;
       jc9 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 20), a1 = ld2(rk1 + 20);
+    { const double2 a0 = A0_371, a1 = A1_371;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M2 * 1.0 + 3.5 * scr[2 * SST];
       jc2 += -1.0 * dq;
       jc9 += -1.0 * dq;
@@ -8622,27 +8994,31 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc1 += 1.0 * dq;
       jc3 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 98), a1 = ld2(rk1 + 98);
+    { const double2 a0 = A0_372, a1 = A1_372;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
       jc2 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc8 += 1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 120), a1 = ld2(rk1 + 120);
+    const double2 A0_373 = ld2(rk0 + 120), A1_373 = ld2(rk1 + 120);
+    const double2 A0_374 = ld2(rk0 + 144), A1_374 = ld2(rk1 + 144);
+    const double2 A0_375 = ld2(rk0 + 146), A1_375 = ld2(rk1 + 146);
+    const double2 A0_376 = ld2(rk0 + 198), A1_376 = ld2(rk1 + 198);
+    { const double2 a0 = A0_373, a1 = A1_373;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc0 += -1.0 * dq;
       jc2 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 144), a1 = ld2(rk1 + 144);
+    { const double2 a0 = A0_374, a1 = A1_374;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7 - (a0.y + (a1.y - a0.y) * Td) * c19;
       jc7 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc23 += 0.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 146), a1 = ld2(rk1 + 146);
+    { const double2 a0 = A0_375, a1 = A1_375;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -8666,7 +9042,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc22 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 198), a1 = ld2(rk1 + 198);
+    { const double2 a0 = A0_376, a1 = A1_376;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M11 * c8 + scr[38 * SST];
       jc8 += -1.0 * dq;
       jc22 += 1.0 * dq;

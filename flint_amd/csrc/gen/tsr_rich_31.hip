@@ -173,8 +173,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
   double w30;
   // JAC: dT-derivative accumulators
   [[maybe_unused]] double dT[NS] = {};
+  const double2 A0_1 = ld2(rk0 + 0), A1_1 = ld2(rk1 + 0);
+  const double2 A0_2 = ld2(rk0 + 2), A1_2 = ld2(rk1 + 2);
+  const double2 A0_3 = ld2(rk0 + 4), A1_3 = ld2(rk1 + 4);
+  const double2 A0_4 = ld2(rk0 + 6), A1_4 = ld2(rk1 + 6);
   {  // reaction 1: 2 O + M <=> O2 + M
-    const double2 a0 = ld2(rk0 + 0), a1 = ld2(rk1 + 0);
+    const double2 a0 = A0_1, a1 = A1_1;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c7*c7)*M0;
     const double pb = kb*(c15)*M0;
@@ -190,7 +194,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w15 = 1.0 * q;
   }
   {  // reaction 2: H + O + M <=> OH + M
-    const double2 a0 = ld2(rk0 + 2), a1 = ld2(rk1 + 2);
+    const double2 a0 = A0_2, a1 = A1_2;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c7)*M1;
     const double pb = kb*(c27)*M1;
@@ -208,7 +212,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = 1.0 * q;
   }
   {  // reaction 3: H2 + O <=> H + OH
-    const double2 a0 = ld2(rk0 + 4), a1 = ld2(rk1 + 4);
+    const double2 a0 = A0_3, a1 = A1_3;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[53 * SST] = kf; scr[54 * SST] = kb; }
     const double pf = kf*(c19)*(c7);
@@ -228,7 +232,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 4: HO2 + O <=> O2 + OH
-    const double2 a0 = ld2(rk0 + 6), a1 = ld2(rk1 + 6);
+    const double2 a0 = A0_4, a1 = A1_4;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[55 * SST] = kf; scr[56 * SST] = kb; }
     const double pf = kf*(c24)*(c7);
@@ -247,8 +251,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = -1.0 * q;
     w27 = w27 + 1.0 * q;
   }
+  const double2 A0_5 = ld2(rk0 + 8), A1_5 = ld2(rk1 + 8);
+  const double2 A0_6 = ld2(rk0 + 10), A1_6 = ld2(rk1 + 10);
+  const double2 A0_7 = ld2(rk0 + 12), A1_7 = ld2(rk1 + 12);
+  const double2 A0_8 = ld2(rk0 + 14), A1_8 = ld2(rk1 + 14);
   {  // reaction 5: H2O2 + O <=> HO2 + OH
-    const double2 a0 = ld2(rk0 + 8), a1 = ld2(rk1 + 8);
+    const double2 a0 = A0_5, a1 = A1_5;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[57 * SST] = kf; scr[58 * SST] = kb; }
     const double pf = kf*(c8)*(c7);
@@ -268,7 +276,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 6: CH + O <=> CO + H
-    const double2 a0 = ld2(rk0 + 10), a1 = ld2(rk1 + 10);
+    const double2 a0 = A0_6, a1 = A1_6;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[59 * SST] = kf; scr[60 * SST] = kb; }
     const double pf = kf*(c26)*(c7);
@@ -288,7 +296,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w26 = -1.0 * q;
   }
   {  // reaction 7: CH2 + O <=> H + HCO
-    const double2 a0 = ld2(rk0 + 12), a1 = ld2(rk1 + 12);
+    const double2 a0 = A0_7, a1 = A1_7;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[61 * SST] = kf; scr[62 * SST] = kb; }
     const double pf = kf*(c29)*(c7);
@@ -308,7 +316,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = -1.0 * q;
   }
   {  // reaction 8: CH2(S) + O <=> CO + H2
-    const double2 a0 = ld2(rk0 + 14), a1 = ld2(rk1 + 14);
+    const double2 a0 = A0_8, a1 = A1_8;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[63 * SST] = kf; scr[64 * SST] = kb; }
     const double pf = kf*(c16)*(c7);
@@ -327,8 +335,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = -1.0 * q;
     w19 = w19 + 1.0 * q;
   }
+  const double2 A0_9 = ld2(rk0 + 16), A1_9 = ld2(rk1 + 16);
+  const double2 A0_10 = ld2(rk0 + 18), A1_10 = ld2(rk1 + 18);
+  const double2 A0_11 = ld2(rk0 + 20), A1_11 = ld2(rk1 + 20);
+  const double2 A0_12 = ld2(rk0 + 22), A1_12 = ld2(rk1 + 22);
   {  // reaction 9: CH2(S) + O <=> H + HCO
-    const double2 a0 = ld2(rk0 + 16), a1 = ld2(rk1 + 16);
+    const double2 a0 = A0_9, a1 = A1_9;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[65 * SST] = kf; scr[66 * SST] = kb; }
     const double pf = kf*(c16)*(c7);
@@ -348,7 +360,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + 1.0 * q;
   }
   {  // reaction 10: CH3 + O <=> CH2O + H
-    const double2 a0 = ld2(rk0 + 18), a1 = ld2(rk1 + 18);
+    const double2 a0 = A0_10, a1 = A1_10;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[67 * SST] = kf; scr[68 * SST] = kb; }
     const double pf = kf*(c18)*(c7);
@@ -368,7 +380,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = -1.0 * q;
   }
   {  // reaction 11: CH4 + O <=> CH3 + OH
-    const double2 a0 = ld2(rk0 + 20), a1 = ld2(rk1 + 20);
+    const double2 a0 = A0_11, a1 = A1_11;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[69 * SST] = kf; scr[70 * SST] = kb; }
     const double pf = kf*(c3)*(c7);
@@ -388,7 +400,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 12: CO + O + M <=> CO2 + M
-    const double2 a0 = ld2(rk0 + 22), a1 = ld2(rk1 + 22);
+    const double2 a0 = A0_12, a1 = A1_12;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c9)*(c7)*M2;
     const double pb = kb*(c30)*M2;
@@ -405,8 +417,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w9 = w9 + -1.0 * q;
     w30 = 1.0 * q;
   }
+  const double2 A0_13 = ld2(rk0 + 24), A1_13 = ld2(rk1 + 24);
+  const double2 A0_14 = ld2(rk0 + 26), A1_14 = ld2(rk1 + 26);
+  const double2 A0_15 = ld2(rk0 + 28), A1_15 = ld2(rk1 + 28);
+  const double2 A0_16 = ld2(rk0 + 30), A1_16 = ld2(rk1 + 30);
   {  // reaction 13: HCO + O <=> CO + OH
-    const double2 a0 = ld2(rk0 + 24), a1 = ld2(rk1 + 24);
+    const double2 a0 = A0_13, a1 = A1_13;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[71 * SST] = kf; scr[72 * SST] = kb; }
     const double pf = kf*(c17)*(c7);
@@ -426,7 +442,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 14: HCO + O <=> CO2 + H
-    const double2 a0 = ld2(rk0 + 26), a1 = ld2(rk1 + 26);
+    const double2 a0 = A0_14, a1 = A1_14;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[73 * SST] = kf; scr[74 * SST] = kb; }
     const double pf = kf*(c17)*(c7);
@@ -446,7 +462,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w30 = w30 + 1.0 * q;
   }
   {  // reaction 15: CH2O + O <=> HCO + OH
-    const double2 a0 = ld2(rk0 + 28), a1 = ld2(rk1 + 28);
+    const double2 a0 = A0_15, a1 = A1_15;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[75 * SST] = kf; scr[76 * SST] = kb; }
     const double pf = kf*(c10)*(c7);
@@ -466,7 +482,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 16: C2H2 + O <=> CH2 + CO
-    const double2 a0 = ld2(rk0 + 30), a1 = ld2(rk1 + 30);
+    const double2 a0 = A0_16, a1 = A1_16;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[77 * SST] = kf; scr[78 * SST] = kb; }
     const double pf = kf*(c4)*(c7);
@@ -485,8 +501,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w9 = w9 + 1.0 * q;
     w29 = w29 + 1.0 * q;
   }
+  const double2 A0_17 = ld2(rk0 + 32), A1_17 = ld2(rk1 + 32);
+  const double2 A0_18 = ld2(rk0 + 34), A1_18 = ld2(rk1 + 34);
+  const double2 A0_19 = ld2(rk0 + 36), A1_19 = ld2(rk1 + 36);
+  const double2 A0_20 = ld2(rk0 + 38), A1_20 = ld2(rk1 + 38);
   {  // reaction 17: C2H3 + O <=> CH2CO + H
-    const double2 a0 = ld2(rk0 + 32), a1 = ld2(rk1 + 32);
+    const double2 a0 = A0_17, a1 = A1_17;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c7);
     const double pb = kb*(c12)*(c2);
@@ -505,7 +525,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w12 = 1.0 * q;
   }
   {  // reaction 18: C2H4 + O <=> CH3 + HCO
-    const double2 a0 = ld2(rk0 + 34), a1 = ld2(rk1 + 34);
+    const double2 a0 = A0_18, a1 = A1_18;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c7);
     const double pb = kb*(c18)*(c17);
@@ -524,7 +544,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = -1.0 * q;
   }
   {  // reaction 19: C2H5 + O <=> CH2O + CH3
-    const double2 a0 = ld2(rk0 + 36), a1 = ld2(rk1 + 36);
+    const double2 a0 = A0_19, a1 = A1_19;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c28)*(c7);
     const double pb = kb*(c10)*(c18);
@@ -543,7 +563,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = -1.0 * q;
   }
   {  // reaction 20: C2H6 + O <=> C2H5 + OH
-    const double2 a0 = ld2(rk0 + 38), a1 = ld2(rk1 + 38);
+    const double2 a0 = A0_20, a1 = A1_20;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c7);
     const double pb = kb*(c28)*(c27);
@@ -561,8 +581,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
     w28 = w28 + 1.0 * q;
   }
+  const double2 A0_21 = ld2(rk0 + 40), A1_21 = ld2(rk1 + 40);
+  const double2 A0_22 = ld2(rk0 + 42), A1_22 = ld2(rk1 + 42);
+  const double2 A0_23 = ld2(rk0 + 44), A1_23 = ld2(rk1 + 44);
+  const double2 A0_24 = ld2(rk0 + 46), A1_24 = ld2(rk1 + 46);
   {  // reaction 21: CH2CO + O <=> CH2 + CO2
-    const double2 a0 = ld2(rk0 + 40), a1 = ld2(rk1 + 40);
+    const double2 a0 = A0_21, a1 = A1_21;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c7);
     const double pb = kb*(c29)*(c30);
@@ -581,7 +605,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w30 = w30 + 1.0 * q;
   }
   {  // reaction 22: CO + O2 <=> CO2 + O
-    const double2 a0 = ld2(rk0 + 42), a1 = ld2(rk1 + 42);
+    const double2 a0 = A0_22, a1 = A1_22;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c9)*(c15);
     const double pb = kb*(c30)*(c7);
@@ -600,7 +624,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w30 = w30 + 1.0 * q;
   }
   {  // reaction 23: CH2O + O2 <=> HCO + HO2
-    const double2 a0 = ld2(rk0 + 44), a1 = ld2(rk1 + 44);
+    const double2 a0 = A0_23, a1 = A1_23;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c10)*(c15);
     const double pb = kb*(c17)*(c24);
@@ -619,7 +643,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + 1.0 * q;
   }
   {  // reaction 24: H + O2 + M <=> HO2 + M
-    const double2 a0 = ld2(rk0 + 46), a1 = ld2(rk1 + 46);
+    const double2 a0 = A0_24, a1 = A1_24;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c15)*M3;
     const double pb = kb*(c24)*M3;
@@ -636,8 +660,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w15 = w15 + -1.0 * q;
     w24 = w24 + 1.0 * q;
   }
+  const double2 A0_25 = ld2(rk0 + 48), A1_25 = ld2(rk1 + 48);
+  const double2 A0_26 = ld2(rk0 + 50), A1_26 = ld2(rk1 + 50);
+  const double2 A0_27 = ld2(rk0 + 52), A1_27 = ld2(rk1 + 52);
+  const double2 A0_28 = ld2(rk0 + 54), A1_28 = ld2(rk1 + 54);
   {  // reaction 25: H + O2 + O2 <=> HO2 + O2
-    const double2 a0 = ld2(rk0 + 48), a1 = ld2(rk1 + 48);
+    const double2 a0 = A0_25, a1 = A1_25;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c15)*M4;
     const double pb = kb*(c24)*M4;
@@ -655,7 +683,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + 1.0 * q;
   }
   {  // reaction 26: H + O2 + H2O <=> HO2 + H2O
-    const double2 a0 = ld2(rk0 + 50), a1 = ld2(rk1 + 50);
+    const double2 a0 = A0_26, a1 = A1_26;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c15)*M5;
     const double pb = kb*(c24)*M5;
@@ -673,7 +701,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + 1.0 * q;
   }
   {  // reaction 27: H + O2 <=> O + OH
-    const double2 a0 = ld2(rk0 + 52), a1 = ld2(rk1 + 52);
+    const double2 a0 = A0_27, a1 = A1_27;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c15);
     const double pb = kb*(c7)*(c27);
@@ -692,7 +720,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 28: 2 H + M <=> H2 + M
-    const double2 a0 = ld2(rk0 + 54), a1 = ld2(rk1 + 54);
+    const double2 a0 = A0_28, a1 = A1_28;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2*c2)*M6;
     const double pb = kb*(c19)*M6;
@@ -707,8 +735,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w2 = w2 + -2.0 * q;
     w19 = w19 + 1.0 * q;
   }
+  const double2 A0_29 = ld2(rk0 + 56), A1_29 = ld2(rk1 + 56);
+  const double2 A0_30 = ld2(rk0 + 58), A1_30 = ld2(rk1 + 58);
+  const double2 A0_31 = ld2(rk0 + 60), A1_31 = ld2(rk1 + 60);
+  const double2 A0_32 = ld2(rk0 + 62), A1_32 = ld2(rk1 + 62);
   {  // reaction 29: 2 H + H2 <=> H2 + H2
-    const double2 a0 = ld2(rk0 + 56), a1 = ld2(rk1 + 56);
+    const double2 a0 = A0_29, a1 = A1_29;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2*c2)*M7;
     const double pb = kb*(c19)*M7;
@@ -724,7 +756,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
   }
   {  // reaction 30: 2 H + H2O <=> H2 + H2O
-    const double2 a0 = ld2(rk0 + 58), a1 = ld2(rk1 + 58);
+    const double2 a0 = A0_30, a1 = A1_30;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2*c2)*M5;
     const double pb = kb*(c19)*M5;
@@ -740,7 +772,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
   }
   {  // reaction 31: 2 H + CO2 <=> H2 + CO2
-    const double2 a0 = ld2(rk0 + 60), a1 = ld2(rk1 + 60);
+    const double2 a0 = A0_31, a1 = A1_31;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2*c2)*M8;
     const double pb = kb*(c19)*M8;
@@ -756,7 +788,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
   }
   {  // reaction 32: H + OH + M <=> H2O + M
-    const double2 a0 = ld2(rk0 + 62), a1 = ld2(rk1 + 62);
+    const double2 a0 = A0_32, a1 = A1_32;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c27)*M9;
     const double pb = kb*(c14)*M9;
@@ -773,8 +805,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w14 = 1.0 * q;
     w27 = w27 + -1.0 * q;
   }
+  const double2 A0_33 = ld2(rk0 + 64), A1_33 = ld2(rk1 + 64);
+  const double2 A0_34 = ld2(rk0 + 66), A1_34 = ld2(rk1 + 66);
+  const double2 A0_35 = ld2(rk0 + 68), A1_35 = ld2(rk1 + 68);
+  const double2 A0_36 = ld2(rk0 + 70), A1_36 = ld2(rk1 + 70);
   {  // reaction 33: H + HO2 <=> H2O + O
-    const double2 a0 = ld2(rk0 + 64), a1 = ld2(rk1 + 64);
+    const double2 a0 = A0_33, a1 = A1_33;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c24);
     const double pb = kb*(c14)*(c7);
@@ -793,7 +829,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + -1.0 * q;
   }
   {  // reaction 34: H + HO2 <=> H2 + O2
-    const double2 a0 = ld2(rk0 + 66), a1 = ld2(rk1 + 66);
+    const double2 a0 = A0_34, a1 = A1_34;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c24);
     const double pb = kb*(c19)*(c15);
@@ -812,7 +848,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + -1.0 * q;
   }
   {  // reaction 35: H + HO2 <=> 2 OH
-    const double2 a0 = ld2(rk0 + 68), a1 = ld2(rk1 + 68);
+    const double2 a0 = A0_35, a1 = A1_35;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c24);
     const double pb = kb*(c27*c27);
@@ -829,7 +865,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 2.0 * q;
   }
   {  // reaction 36: H + H2O2 <=> H2 + HO2
-    const double2 a0 = ld2(rk0 + 70), a1 = ld2(rk1 + 70);
+    const double2 a0 = A0_36, a1 = A1_36;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c8);
     const double pb = kb*(c19)*(c24);
@@ -847,8 +883,10 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
     w24 = w24 + 1.0 * q;
   }
+  const double2 A0_37 = ld2(rk0 + 72), A1_37 = ld2(rk1 + 72);
+  const double2 A0_38 = ld2(rk0 + 74), A1_38 = ld2(rk1 + 74);
   {  // reaction 37: H + H2O2 <=> H2O + OH
-    const double2 a0 = ld2(rk0 + 72), a1 = ld2(rk1 + 72);
+    const double2 a0 = A0_37, a1 = A1_37;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c8);
     const double pb = kb*(c14)*(c27);
@@ -906,7 +944,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + -1.0 * q;
   }
   {  // reaction 39: CH2(S) + H <=> CH + H2
-    const double2 a0 = ld2(rk0 + 74), a1 = ld2(rk1 + 74);
+    const double2 a0 = A0_38, a1 = A1_38;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c16)*(c2);
     const double pb = kb*(c26)*(c19);
@@ -963,8 +1001,11 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w3 = w3 + 1.0 * q;
     w18 = w18 + -1.0 * q;
   }
+  const double2 A0_39 = ld2(rk0 + 76), A1_39 = ld2(rk1 + 76);
+  const double2 A0_40 = ld2(rk0 + 78), A1_40 = ld2(rk1 + 78);
+  const double2 A0_41 = ld2(rk0 + 80), A1_41 = ld2(rk1 + 80);
   {  // reaction 41: CH4 + H <=> CH3 + H2
-    const double2 a0 = ld2(rk0 + 76), a1 = ld2(rk1 + 76);
+    const double2 a0 = A0_39, a1 = A1_39;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c3)*(c2);
     const double pb = kb*(c18)*(c19);
@@ -1022,7 +1063,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + -1.0 * q;
   }
   {  // reaction 43: H + HCO <=> CO + H2
-    const double2 a0 = ld2(rk0 + 78), a1 = ld2(rk1 + 78);
+    const double2 a0 = A0_40, a1 = A1_40;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c2)*(c17);
     const double pb = kb*(c9)*(c19);
@@ -1041,7 +1082,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
   }
   {  // reaction 44: CH2O + H <=> H2 + HCO
-    const double2 a0 = ld2(rk0 + 80), a1 = ld2(rk1 + 80);
+    const double2 a0 = A0_41, a1 = A1_41;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c10)*(c2);
     const double pb = kb*(c19)*(c17);
@@ -1059,6 +1100,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + 1.0 * q;
     w19 = w19 + 1.0 * q;
   }
+  const double2 A0_42 = ld2(rk0 + 82), A1_42 = ld2(rk1 + 82);
   {  // reaction 45: C2H2 + H (+M) <=> C2H3 (+M)
     const double2 x0 = ld2(rk0 + 382), y0 = ld2(rk0 + 384);
     const double2 x1 = ld2(rk1 + 382), y1 = ld2(rk1 + 384);
@@ -1138,7 +1180,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + 1.0 * q;
   }
   {  // reaction 47: C2H3 + H <=> C2H2 + H2
-    const double2 a0 = ld2(rk0 + 82), a1 = ld2(rk1 + 82);
+    const double2 a0 = A0_42, a1 = A1_42;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c2);
     const double pb = kb*(c4)*(c19);
@@ -1195,8 +1237,11 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + -1.0 * q;
     w28 = w28 + 1.0 * q;
   }
+  const double2 A0_43 = ld2(rk0 + 84), A1_43 = ld2(rk1 + 84);
+  const double2 A0_44 = ld2(rk0 + 86), A1_44 = ld2(rk1 + 86);
+  const double2 A0_45 = ld2(rk0 + 88), A1_45 = ld2(rk1 + 88);
   {  // reaction 49: C2H4 + H <=> C2H3 + H2
-    const double2 a0 = ld2(rk0 + 84), a1 = ld2(rk1 + 84);
+    const double2 a0 = A0_43, a1 = A1_43;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c2);
     const double pb = kb*(c6)*(c19);
@@ -1254,7 +1299,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + -1.0 * q;
   }
   {  // reaction 51: C2H5 + H <=> C2H4 + H2
-    const double2 a0 = ld2(rk0 + 86), a1 = ld2(rk1 + 86);
+    const double2 a0 = A0_44, a1 = A1_44;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c28)*(c2);
     const double pb = kb*(c25)*(c19);
@@ -1273,7 +1318,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + -1.0 * q;
   }
   {  // reaction 52: C2H6 + H <=> C2H5 + H2
-    const double2 a0 = ld2(rk0 + 88), a1 = ld2(rk1 + 88);
+    const double2 a0 = A0_45, a1 = A1_45;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c2);
     const double pb = kb*(c28)*(c19);
@@ -1291,8 +1336,10 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
     w28 = w28 + 1.0 * q;
   }
+  const double2 A0_46 = ld2(rk0 + 90), A1_46 = ld2(rk1 + 90);
+  const double2 A0_47 = ld2(rk0 + 92), A1_47 = ld2(rk1 + 92);
   {  // reaction 53: CH2CO + H <=> CH3 + CO
-    const double2 a0 = ld2(rk0 + 90), a1 = ld2(rk1 + 90);
+    const double2 a0 = A0_46, a1 = A1_46;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c2);
     const double pb = kb*(c18)*(c9);
@@ -1350,7 +1397,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + -1.0 * q;
   }
   {  // reaction 55: H2 + OH <=> H + H2O
-    const double2 a0 = ld2(rk0 + 92), a1 = ld2(rk1 + 92);
+    const double2 a0 = A0_47, a1 = A1_47;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c19)*(c27);
     const double pb = kb*(c2)*(c14);
@@ -1405,8 +1452,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w8 = w8 + 1.0 * q;
     w27 = w27 + -2.0 * q;
   }
+  const double2 A0_48 = ld2(rk0 + 94), A1_48 = ld2(rk1 + 94);
+  const double2 A0_49 = ld2(rk0 + 96), A1_49 = ld2(rk1 + 96);
+  const double2 A0_50 = ld2(rk0 + 98), A1_50 = ld2(rk1 + 98);
+  const double2 A0_51 = ld2(rk0 + 100), A1_51 = ld2(rk1 + 100);
   {  // reaction 57: 2 OH <=> H2O + O
-    const double2 a0 = ld2(rk0 + 94), a1 = ld2(rk1 + 94);
+    const double2 a0 = A0_48, a1 = A1_48;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c27*c27);
     const double pb = kb*(c14)*(c7);
@@ -1423,7 +1474,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -2.0 * q;
   }
   {  // reaction 58: HO2 + OH <=> H2O + O2
-    const double2 a0 = ld2(rk0 + 96), a1 = ld2(rk1 + 96);
+    const double2 a0 = A0_49, a1 = A1_49;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c24)*(c27);
     const double pb = kb*(c14)*(c15);
@@ -1442,7 +1493,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 59: H2O2 + OH <=> H2O + HO2
-    const double2 a0 = ld2(rk0 + 98), a1 = ld2(rk1 + 98);
+    const double2 a0 = A0_50, a1 = A1_50;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c27);
     const double pb = kb*(c14)*(c24);
@@ -1461,7 +1512,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 60: H2O2 + OH <=> H2O + HO2
-    const double2 a0 = ld2(rk0 + 100), a1 = ld2(rk1 + 100);
+    const double2 a0 = A0_51, a1 = A1_51;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c8)*(c27);
     const double pb = kb*(c14)*(c24);
@@ -1479,8 +1530,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + 1.0 * q;
     w27 = w27 + -1.0 * q;
   }
+  const double2 A0_52 = ld2(rk0 + 102), A1_52 = ld2(rk1 + 102);
+  const double2 A0_53 = ld2(rk0 + 104), A1_53 = ld2(rk1 + 104);
+  const double2 A0_54 = ld2(rk0 + 106), A1_54 = ld2(rk1 + 106);
+  const double2 A0_55 = ld2(rk0 + 108), A1_55 = ld2(rk1 + 108);
   {  // reaction 61: CH + OH <=> H + HCO
-    const double2 a0 = ld2(rk0 + 102), a1 = ld2(rk1 + 102);
+    const double2 a0 = A0_52, a1 = A1_52;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c26)*(c27);
     const double pb = kb*(c2)*(c17);
@@ -1499,7 +1554,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 62: CH2 + OH <=> CH2O + H
-    const double2 a0 = ld2(rk0 + 104), a1 = ld2(rk1 + 104);
+    const double2 a0 = A0_53, a1 = A1_53;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c29)*(c27);
     const double pb = kb*(c10)*(c2);
@@ -1518,7 +1573,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + -1.0 * q;
   }
   {  // reaction 63: CH2 + OH <=> CH + H2O
-    const double2 a0 = ld2(rk0 + 106), a1 = ld2(rk1 + 106);
+    const double2 a0 = A0_54, a1 = A1_54;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c29)*(c27);
     const double pb = kb*(c26)*(c14);
@@ -1537,7 +1592,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + -1.0 * q;
   }
   {  // reaction 64: CH2(S) + OH <=> CH2O + H
-    const double2 a0 = ld2(rk0 + 108), a1 = ld2(rk1 + 108);
+    const double2 a0 = A0_55, a1 = A1_55;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c16)*(c27);
     const double pb = kb*(c10)*(c2);
@@ -1555,8 +1610,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + -1.0 * q;
     w27 = w27 + -1.0 * q;
   }
+  const double2 A0_56 = ld2(rk0 + 110), A1_56 = ld2(rk1 + 110);
+  const double2 A0_57 = ld2(rk0 + 112), A1_57 = ld2(rk1 + 112);
+  const double2 A0_58 = ld2(rk0 + 114), A1_58 = ld2(rk1 + 114);
+  const double2 A0_59 = ld2(rk0 + 116), A1_59 = ld2(rk1 + 116);
   {  // reaction 65: CH3 + OH <=> CH2 + H2O
-    const double2 a0 = ld2(rk0 + 110), a1 = ld2(rk1 + 110);
+    const double2 a0 = A0_56, a1 = A1_56;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18)*(c27);
     const double pb = kb*(c29)*(c14);
@@ -1575,7 +1634,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + 1.0 * q;
   }
   {  // reaction 66: CH3 + OH <=> CH2(S) + H2O
-    const double2 a0 = ld2(rk0 + 112), a1 = ld2(rk1 + 112);
+    const double2 a0 = A0_57, a1 = A1_57;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18)*(c27);
     const double pb = kb*(c16)*(c14);
@@ -1594,7 +1653,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 67: CH4 + OH <=> CH3 + H2O
-    const double2 a0 = ld2(rk0 + 114), a1 = ld2(rk1 + 114);
+    const double2 a0 = A0_58, a1 = A1_58;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c3)*(c27);
     const double pb = kb*(c18)*(c14);
@@ -1613,7 +1672,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 68: CO + OH <=> CO2 + H
-    const double2 a0 = ld2(rk0 + 116), a1 = ld2(rk1 + 116);
+    const double2 a0 = A0_59, a1 = A1_59;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c9)*(c27);
     const double pb = kb*(c30)*(c2);
@@ -1631,8 +1690,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
     w30 = w30 + 1.0 * q;
   }
+  const double2 A0_60 = ld2(rk0 + 118), A1_60 = ld2(rk1 + 118);
+  const double2 A0_61 = ld2(rk0 + 120), A1_61 = ld2(rk1 + 120);
+  const double2 A0_62 = ld2(rk0 + 122), A1_62 = ld2(rk1 + 122);
+  const double2 A0_63 = ld2(rk0 + 124), A1_63 = ld2(rk1 + 124);
   {  // reaction 69: HCO + OH <=> CO + H2O
-    const double2 a0 = ld2(rk0 + 118), a1 = ld2(rk1 + 118);
+    const double2 a0 = A0_60, a1 = A1_60;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c17)*(c27);
     const double pb = kb*(c9)*(c14);
@@ -1651,7 +1714,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 70: CH2O + OH <=> H2O + HCO
-    const double2 a0 = ld2(rk0 + 120), a1 = ld2(rk1 + 120);
+    const double2 a0 = A0_61, a1 = A1_61;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c10)*(c27);
     const double pb = kb*(c14)*(c17);
@@ -1670,7 +1733,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 71: C2H2 + OH <=> CH2CO + H
-    const double2 a0 = ld2(rk0 + 122), a1 = ld2(rk1 + 122);
+    const double2 a0 = A0_62, a1 = A1_62;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c4)*(c27);
     const double pb = kb*(c12)*(c2);
@@ -1689,7 +1752,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 72: C2H2 + OH <=> CH3 + CO
-    const double2 a0 = ld2(rk0 + 124), a1 = ld2(rk1 + 124);
+    const double2 a0 = A0_63, a1 = A1_63;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c4)*(c27);
     const double pb = kb*(c18)*(c9);
@@ -1707,8 +1770,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + 1.0 * q;
     w27 = w27 + -1.0 * q;
   }
+  const double2 A0_64 = ld2(rk0 + 126), A1_64 = ld2(rk1 + 126);
+  const double2 A0_65 = ld2(rk0 + 128), A1_65 = ld2(rk1 + 128);
+  const double2 A0_66 = ld2(rk0 + 130), A1_66 = ld2(rk1 + 130);
+  const double2 A0_67 = ld2(rk0 + 132), A1_67 = ld2(rk1 + 132);
   {  // reaction 73: C2H3 + OH <=> C2H2 + H2O
-    const double2 a0 = ld2(rk0 + 126), a1 = ld2(rk1 + 126);
+    const double2 a0 = A0_64, a1 = A1_64;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c27);
     const double pb = kb*(c4)*(c14);
@@ -1727,7 +1794,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 74: C2H4 + OH <=> C2H3 + H2O
-    const double2 a0 = ld2(rk0 + 128), a1 = ld2(rk1 + 128);
+    const double2 a0 = A0_65, a1 = A1_65;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c27);
     const double pb = kb*(c6)*(c14);
@@ -1746,7 +1813,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 75: C2H6 + OH <=> C2H5 + H2O
-    const double2 a0 = ld2(rk0 + 130), a1 = ld2(rk1 + 130);
+    const double2 a0 = A0_66, a1 = A1_66;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c27);
     const double pb = kb*(c28)*(c14);
@@ -1765,7 +1832,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + 1.0 * q;
   }
   {  // reaction 76: 2 HO2 <=> H2O2 + O2
-    const double2 a0 = ld2(rk0 + 132), a1 = ld2(rk1 + 132);
+    const double2 a0 = A0_67, a1 = A1_67;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c24*c24);
     const double pb = kb*(c8)*(c15);
@@ -1781,8 +1848,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w15 = w15 + 1.0 * q;
     w24 = w24 + -2.0 * q;
   }
+  const double2 A0_68 = ld2(rk0 + 134), A1_68 = ld2(rk1 + 134);
+  const double2 A0_69 = ld2(rk0 + 136), A1_69 = ld2(rk1 + 136);
+  const double2 A0_70 = ld2(rk0 + 138), A1_70 = ld2(rk1 + 138);
+  const double2 A0_71 = ld2(rk0 + 140), A1_71 = ld2(rk1 + 140);
   {  // reaction 77: 2 HO2 <=> H2O2 + O2
-    const double2 a0 = ld2(rk0 + 134), a1 = ld2(rk1 + 134);
+    const double2 a0 = A0_68, a1 = A1_68;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c24*c24);
     const double pb = kb*(c8)*(c15);
@@ -1799,7 +1870,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + -2.0 * q;
   }
   {  // reaction 78: CH2 + HO2 <=> CH2O + OH
-    const double2 a0 = ld2(rk0 + 136), a1 = ld2(rk1 + 136);
+    const double2 a0 = A0_69, a1 = A1_69;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c29)*(c24);
     const double pb = kb*(c10)*(c27);
@@ -1818,7 +1889,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + -1.0 * q;
   }
   {  // reaction 79: CH3 + HO2 <=> CH4 + O2
-    const double2 a0 = ld2(rk0 + 138), a1 = ld2(rk1 + 138);
+    const double2 a0 = A0_70, a1 = A1_70;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18)*(c24);
     const double pb = kb*(c3)*(c15);
@@ -1837,7 +1908,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + -1.0 * q;
   }
   {  // reaction 80: CO + HO2 <=> CO2 + OH
-    const double2 a0 = ld2(rk0 + 140), a1 = ld2(rk1 + 140);
+    const double2 a0 = A0_71, a1 = A1_71;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c9)*(c24);
     const double pb = kb*(c30)*(c27);
@@ -1855,8 +1926,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
     w30 = w30 + 1.0 * q;
   }
+  const double2 A0_72 = ld2(rk0 + 142), A1_72 = ld2(rk1 + 142);
+  const double2 A0_73 = ld2(rk0 + 144), A1_73 = ld2(rk1 + 144);
+  const double2 A0_74 = ld2(rk0 + 146), A1_74 = ld2(rk1 + 146);
+  const double2 A0_75 = ld2(rk0 + 148), A1_75 = ld2(rk1 + 148);
   {  // reaction 81: CH2O + HO2 <=> H2O2 + HCO
-    const double2 a0 = ld2(rk0 + 142), a1 = ld2(rk1 + 142);
+    const double2 a0 = A0_72, a1 = A1_72;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c10)*(c24);
     const double pb = kb*(c8)*(c17);
@@ -1875,7 +1950,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + -1.0 * q;
   }
   {  // reaction 82: CH + O2 <=> HCO + O
-    const double2 a0 = ld2(rk0 + 144), a1 = ld2(rk1 + 144);
+    const double2 a0 = A0_73, a1 = A1_73;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c26)*(c15);
     const double pb = kb*(c17)*(c7);
@@ -1894,7 +1969,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w26 = w26 + -1.0 * q;
   }
   {  // reaction 83: CH + H2 <=> CH2 + H
-    const double2 a0 = ld2(rk0 + 146), a1 = ld2(rk1 + 146);
+    const double2 a0 = A0_74, a1 = A1_74;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c26)*(c19);
     const double pb = kb*(c29)*(c2);
@@ -1913,7 +1988,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + 1.0 * q;
   }
   {  // reaction 84: CH + H2O <=> CH2O + H
-    const double2 a0 = ld2(rk0 + 148), a1 = ld2(rk1 + 148);
+    const double2 a0 = A0_75, a1 = A1_75;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c26)*(c14);
     const double pb = kb*(c10)*(c2);
@@ -1931,8 +2006,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w14 = w14 + -1.0 * q;
     w26 = w26 + -1.0 * q;
   }
+  const double2 A0_76 = ld2(rk0 + 150), A1_76 = ld2(rk1 + 150);
+  const double2 A0_77 = ld2(rk0 + 152), A1_77 = ld2(rk1 + 152);
+  const double2 A0_78 = ld2(rk0 + 154), A1_78 = ld2(rk1 + 154);
+  const double2 A0_79 = ld2(rk0 + 156), A1_79 = ld2(rk1 + 156);
   {  // reaction 85: CH + CH2 <=> C2H2 + H
-    const double2 a0 = ld2(rk0 + 150), a1 = ld2(rk1 + 150);
+    const double2 a0 = A0_76, a1 = A1_76;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c26)*(c29);
     const double pb = kb*(c4)*(c2);
@@ -1951,7 +2030,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + -1.0 * q;
   }
   {  // reaction 86: CH + CH3 <=> C2H3 + H
-    const double2 a0 = ld2(rk0 + 152), a1 = ld2(rk1 + 152);
+    const double2 a0 = A0_77, a1 = A1_77;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c26)*(c18);
     const double pb = kb*(c6)*(c2);
@@ -1970,7 +2049,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w26 = w26 + -1.0 * q;
   }
   {  // reaction 87: CH + CH4 <=> C2H4 + H
-    const double2 a0 = ld2(rk0 + 154), a1 = ld2(rk1 + 154);
+    const double2 a0 = A0_78, a1 = A1_78;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c26)*(c3);
     const double pb = kb*(c25)*(c2);
@@ -1989,7 +2068,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w26 = w26 + -1.0 * q;
   }
   {  // reaction 88: CH + CO2 <=> CO + HCO
-    const double2 a0 = ld2(rk0 + 156), a1 = ld2(rk1 + 156);
+    const double2 a0 = A0_79, a1 = A1_79;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c26)*(c30);
     const double pb = kb*(c9)*(c17);
@@ -2007,8 +2086,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w26 = w26 + -1.0 * q;
     w30 = w30 + -1.0 * q;
   }
+  const double2 A0_80 = ld2(rk0 + 158), A1_80 = ld2(rk1 + 158);
+  const double2 A0_81 = ld2(rk0 + 160), A1_81 = ld2(rk1 + 160);
+  const double2 A0_82 = ld2(rk0 + 162), A1_82 = ld2(rk1 + 162);
+  const double2 A0_83 = ld2(rk0 + 164), A1_83 = ld2(rk1 + 164);
   {  // reaction 89: CH + CH2O <=> CH2CO + H
-    const double2 a0 = ld2(rk0 + 158), a1 = ld2(rk1 + 158);
+    const double2 a0 = A0_80, a1 = A1_80;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c26)*(c10);
     const double pb = kb*(c12)*(c2);
@@ -2027,7 +2110,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w26 = w26 + -1.0 * q;
   }
   {  // reaction 90: CH2 + O2 <=> HCO + OH
-    const double2 a0 = ld2(rk0 + 160), a1 = ld2(rk1 + 160);
+    const double2 a0 = A0_81, a1 = A1_81;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c29)*(c15);
     const double pb = kb*(c17)*(c27);
@@ -2046,7 +2129,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + -1.0 * q;
   }
   {  // reaction 91: CH2 + H2 <=> CH3 + H
-    const double2 a0 = ld2(rk0 + 162), a1 = ld2(rk1 + 162);
+    const double2 a0 = A0_82, a1 = A1_82;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c29)*(c19);
     const double pb = kb*(c18)*(c2);
@@ -2065,7 +2148,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + -1.0 * q;
   }
   {  // reaction 92: 2 CH2 <=> C2H2 + H2
-    const double2 a0 = ld2(rk0 + 164), a1 = ld2(rk1 + 164);
+    const double2 a0 = A0_83, a1 = A1_83;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c29*c29);
     const double pb = kb*(c4)*(c19);
@@ -2081,8 +2164,11 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
     w29 = w29 + -2.0 * q;
   }
+  const double2 A0_84 = ld2(rk0 + 166), A1_84 = ld2(rk1 + 166);
+  const double2 A0_85 = ld2(rk0 + 168), A1_85 = ld2(rk1 + 168);
+  const double2 A0_86 = ld2(rk0 + 170), A1_86 = ld2(rk1 + 170);
   {  // reaction 93: CH2 + CH3 <=> C2H4 + H
-    const double2 a0 = ld2(rk0 + 166), a1 = ld2(rk1 + 166);
+    const double2 a0 = A0_84, a1 = A1_84;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c29)*(c18);
     const double pb = kb*(c25)*(c2);
@@ -2101,7 +2187,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + -1.0 * q;
   }
   {  // reaction 94: CH2 + CH4 <=> 2 CH3
-    const double2 a0 = ld2(rk0 + 168), a1 = ld2(rk1 + 168);
+    const double2 a0 = A0_85, a1 = A1_85;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c29)*(c3);
     const double pb = kb*(c18*c18);
@@ -2157,7 +2243,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + -1.0 * q;
   }
   {  // reaction 96: CH2(S) + O2 <=> CO + H + OH
-    const double2 a0 = ld2(rk0 + 170), a1 = ld2(rk1 + 170);
+    const double2 a0 = A0_86, a1 = A1_86;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     if (JAC) { scr[51 * SST] = kf; scr[52 * SST] = kb; }
     const double pf = kf*(c16)*(c15);
@@ -2178,8 +2264,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + -1.0 * q;
     w27 = w27 + 1.0 * q;
   }
+  const double2 A0_87 = ld2(rk0 + 172), A1_87 = ld2(rk1 + 172);
+  const double2 A0_88 = ld2(rk0 + 174), A1_88 = ld2(rk1 + 174);
+  const double2 A0_89 = ld2(rk0 + 176), A1_89 = ld2(rk1 + 176);
+  const double2 A0_90 = ld2(rk0 + 178), A1_90 = ld2(rk1 + 178);
   {  // reaction 97: CH2(S) + O2 <=> CO + H2O
-    const double2 a0 = ld2(rk0 + 172), a1 = ld2(rk1 + 172);
+    const double2 a0 = A0_87, a1 = A1_87;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c16)*(c15);
     const double pb = kb*(c9)*(c14);
@@ -2198,7 +2288,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + -1.0 * q;
   }
   {  // reaction 98: CH2(S) + H2 <=> CH3 + H
-    const double2 a0 = ld2(rk0 + 174), a1 = ld2(rk1 + 174);
+    const double2 a0 = A0_88, a1 = A1_88;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c16)*(c19);
     const double pb = kb*(c18)*(c2);
@@ -2217,7 +2307,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + -1.0 * q;
   }
   {  // reaction 99: CH2(S) + H2O <=> CH2 + H2O
-    const double2 a0 = ld2(rk0 + 176), a1 = ld2(rk1 + 176);
+    const double2 a0 = A0_89, a1 = A1_89;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c16)*(c14);
     const double pb = kb*(c29)*(c14);
@@ -2234,7 +2324,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + 1.0 * q;
   }
   {  // reaction 100: CH2(S) + CH3 <=> C2H4 + H
-    const double2 a0 = ld2(rk0 + 178), a1 = ld2(rk1 + 178);
+    const double2 a0 = A0_90, a1 = A1_90;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c16)*(c18);
     const double pb = kb*(c25)*(c2);
@@ -2252,8 +2342,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
     w25 = w25 + 1.0 * q;
   }
+  const double2 A0_91 = ld2(rk0 + 180), A1_91 = ld2(rk1 + 180);
+  const double2 A0_92 = ld2(rk0 + 182), A1_92 = ld2(rk1 + 182);
+  const double2 A0_93 = ld2(rk0 + 184), A1_93 = ld2(rk1 + 184);
+  const double2 A0_94 = ld2(rk0 + 186), A1_94 = ld2(rk1 + 186);
   {  // reaction 101: CH2(S) + CH4 <=> 2 CH3
-    const double2 a0 = ld2(rk0 + 180), a1 = ld2(rk1 + 180);
+    const double2 a0 = A0_91, a1 = A1_91;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c16)*(c3);
     const double pb = kb*(c18*c18);
@@ -2270,7 +2364,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + 2.0 * q;
   }
   {  // reaction 102: CH2(S) + CO <=> CH2 + CO
-    const double2 a0 = ld2(rk0 + 182), a1 = ld2(rk1 + 182);
+    const double2 a0 = A0_92, a1 = A1_92;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c16)*(c9);
     const double pb = kb*(c29)*(c9);
@@ -2287,7 +2381,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + 1.0 * q;
   }
   {  // reaction 103: CH2(S) + CO2 <=> CH2 + CO2
-    const double2 a0 = ld2(rk0 + 184), a1 = ld2(rk1 + 184);
+    const double2 a0 = A0_93, a1 = A1_93;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c16)*(c30);
     const double pb = kb*(c29)*(c30);
@@ -2304,7 +2398,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w30 = w30 + 0.0 * q;
   }
   {  // reaction 104: CH2(S) + CO2 <=> CH2O + CO
-    const double2 a0 = ld2(rk0 + 186), a1 = ld2(rk1 + 186);
+    const double2 a0 = A0_94, a1 = A1_94;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c16)*(c30);
     const double pb = kb*(c10)*(c9);
@@ -2322,8 +2416,11 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w16 = w16 + -1.0 * q;
     w30 = w30 + -1.0 * q;
   }
+  const double2 A0_95 = ld2(rk0 + 188), A1_95 = ld2(rk1 + 188);
+  const double2 A0_96 = ld2(rk0 + 190), A1_96 = ld2(rk1 + 190);
+  const double2 A0_97 = ld2(rk0 + 192), A1_97 = ld2(rk1 + 192);
   {  // reaction 105: C2H6 + CH2(S) <=> C2H5 + CH3
-    const double2 a0 = ld2(rk0 + 188), a1 = ld2(rk1 + 188);
+    const double2 a0 = A0_95, a1 = A1_95;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c16);
     const double pb = kb*(c28)*(c18);
@@ -2342,7 +2439,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + 1.0 * q;
   }
   {  // reaction 106: CH3 + O2 <=> CH2O + OH
-    const double2 a0 = ld2(rk0 + 190), a1 = ld2(rk1 + 190);
+    const double2 a0 = A0_96, a1 = A1_96;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18)*(c15);
     const double pb = kb*(c10)*(c27);
@@ -2361,7 +2458,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 107: CH3 + H2O2 <=> CH4 + HO2
-    const double2 a0 = ld2(rk0 + 192), a1 = ld2(rk1 + 192);
+    const double2 a0 = A0_97, a1 = A1_97;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18)*(c8);
     const double pb = kb*(c3)*(c24);
@@ -2416,8 +2513,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w11 = w11 + 1.0 * q;
     w18 = w18 + -2.0 * q;
   }
+  const double2 A0_98 = ld2(rk0 + 194), A1_98 = ld2(rk1 + 194);
+  const double2 A0_99 = ld2(rk0 + 196), A1_99 = ld2(rk1 + 196);
+  const double2 A0_100 = ld2(rk0 + 198), A1_100 = ld2(rk1 + 198);
+  const double2 A0_101 = ld2(rk0 + 200), A1_101 = ld2(rk1 + 200);
   {  // reaction 109: 2 CH3 <=> C2H5 + H
-    const double2 a0 = ld2(rk0 + 194), a1 = ld2(rk1 + 194);
+    const double2 a0 = A0_98, a1 = A1_98;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18*c18);
     const double pb = kb*(c28)*(c2);
@@ -2434,7 +2535,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + 1.0 * q;
   }
   {  // reaction 110: CH3 + HCO <=> CH4 + CO
-    const double2 a0 = ld2(rk0 + 196), a1 = ld2(rk1 + 196);
+    const double2 a0 = A0_99, a1 = A1_99;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18)*(c17);
     const double pb = kb*(c3)*(c9);
@@ -2453,7 +2554,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 111: CH2O + CH3 <=> CH4 + HCO
-    const double2 a0 = ld2(rk0 + 198), a1 = ld2(rk1 + 198);
+    const double2 a0 = A0_100, a1 = A1_100;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c10)*(c18);
     const double pb = kb*(c3)*(c17);
@@ -2472,7 +2573,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 112: C2H4 + CH3 <=> C2H3 + CH4
-    const double2 a0 = ld2(rk0 + 200), a1 = ld2(rk1 + 200);
+    const double2 a0 = A0_101, a1 = A1_101;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c18);
     const double pb = kb*(c6)*(c3);
@@ -2490,8 +2591,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
     w25 = w25 + -1.0 * q;
   }
+  const double2 A0_102 = ld2(rk0 + 202), A1_102 = ld2(rk1 + 202);
+  const double2 A0_103 = ld2(rk0 + 204), A1_103 = ld2(rk1 + 204);
+  const double2 A0_104 = ld2(rk0 + 206), A1_104 = ld2(rk1 + 206);
+  const double2 A0_105 = ld2(rk0 + 208), A1_105 = ld2(rk1 + 208);
   {  // reaction 113: C2H6 + CH3 <=> C2H5 + CH4
-    const double2 a0 = ld2(rk0 + 202), a1 = ld2(rk1 + 202);
+    const double2 a0 = A0_102, a1 = A1_102;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c18);
     const double pb = kb*(c28)*(c3);
@@ -2510,7 +2615,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + 1.0 * q;
   }
   {  // reaction 114: HCO + H2O <=> CO + H + H2O
-    const double2 a0 = ld2(rk0 + 204), a1 = ld2(rk1 + 204);
+    const double2 a0 = A0_103, a1 = A1_103;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c17)*M5;
     const double pb = kb*(c9)*(c2)*M5;
@@ -2528,7 +2633,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + -1.0 * q;
   }
   {  // reaction 115: HCO + M <=> CO + H + M
-    const double2 a0 = ld2(rk0 + 206), a1 = ld2(rk1 + 206);
+    const double2 a0 = A0_104, a1 = A1_104;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c17)*M10;
     const double pb = kb*(c9)*(c2)*M10;
@@ -2546,7 +2651,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + -1.0 * q;
   }
   {  // reaction 116: HCO + O2 <=> CO + HO2
-    const double2 a0 = ld2(rk0 + 208), a1 = ld2(rk1 + 208);
+    const double2 a0 = A0_105, a1 = A1_105;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c17)*(c15);
     const double pb = kb*(c9)*(c24);
@@ -2564,8 +2669,11 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + -1.0 * q;
     w24 = w24 + 1.0 * q;
   }
+  const double2 A0_106 = ld2(rk0 + 210), A1_106 = ld2(rk1 + 210);
+  const double2 A0_107 = ld2(rk0 + 212), A1_107 = ld2(rk1 + 212);
+  const double2 A0_108 = ld2(rk0 + 214), A1_108 = ld2(rk1 + 214);
   {  // reaction 117: C2H3 + O2 <=> CH2O + HCO
-    const double2 a0 = ld2(rk0 + 210), a1 = ld2(rk1 + 210);
+    const double2 a0 = A0_106, a1 = A1_106;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c15);
     const double pb = kb*(c10)*(c17);
@@ -2623,7 +2731,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + -1.0 * q;
   }
   {  // reaction 119: C2H5 + O2 <=> C2H4 + HO2
-    const double2 a0 = ld2(rk0 + 212), a1 = ld2(rk1 + 212);
+    const double2 a0 = A0_107, a1 = A1_107;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c28)*(c15);
     const double pb = kb*(c25)*(c24);
@@ -2642,7 +2750,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + -1.0 * q;
   }
   {  // reaction 120: CH3 + OH <=> CH2O + H2
-    const double2 a0 = ld2(rk0 + 214), a1 = ld2(rk1 + 214);
+    const double2 a0 = A0_108, a1 = A1_108;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18)*(c27);
     const double pb = kb*(c10)*(c19);
@@ -2660,8 +2768,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w19 = w19 + 1.0 * q;
     w27 = w27 + -1.0 * q;
   }
+  const double2 A0_109 = ld2(rk0 + 216), A1_109 = ld2(rk1 + 216);
+  const double2 A0_110 = ld2(rk0 + 218), A1_110 = ld2(rk1 + 218);
+  const double2 A0_111 = ld2(rk0 + 220), A1_111 = ld2(rk1 + 220);
+  const double2 A0_112 = ld2(rk0 + 222), A1_112 = ld2(rk1 + 222);
   {  // reaction 121: C2H3 + C2H5 => 2 C2H4
-    const double2 a0 = ld2(rk0 + 216), a1 = ld2(rk1 + 216);
+    const double2 a0 = A0_109, a1 = A1_109;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c28);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2678,7 +2790,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + -1.0 * q;
   }
   {  // reaction 122: 2 C2H4 => C2H3 + C2H5
-    const double2 a0 = ld2(rk0 + 218), a1 = ld2(rk1 + 218);
+    const double2 a0 = A0_110, a1 = A1_110;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25*c25);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2695,7 +2807,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + 1.0 * q;
   }
   {  // reaction 123: C2H4 + O => C2H3 + OH
-    const double2 a0 = ld2(rk0 + 220), a1 = ld2(rk1 + 220);
+    const double2 a0 = A0_111, a1 = A1_111;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c7);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2714,7 +2826,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 124: C2H3 + OH => C2H4 + O
-    const double2 a0 = ld2(rk0 + 222), a1 = ld2(rk1 + 222);
+    const double2 a0 = A0_112, a1 = A1_112;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c27);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2732,8 +2844,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + 1.0 * q;
     w27 = w27 + -1.0 * q;
   }
+  const double2 A0_113 = ld2(rk0 + 224), A1_113 = ld2(rk1 + 224);
+  const double2 A0_114 = ld2(rk0 + 226), A1_114 = ld2(rk1 + 226);
+  const double2 A0_115 = ld2(rk0 + 228), A1_115 = ld2(rk1 + 228);
+  const double2 A0_116 = ld2(rk0 + 230), A1_116 = ld2(rk1 + 230);
   {  // reaction 125: C2H2 + O2 => 2 HCO
-    const double2 a0 = ld2(rk0 + 224), a1 = ld2(rk1 + 224);
+    const double2 a0 = A0_113, a1 = A1_113;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c4)*(c15);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2750,7 +2866,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + 2.0 * q;
   }
   {  // reaction 126: 2 HCO => C2H2 + O2
-    const double2 a0 = ld2(rk0 + 226), a1 = ld2(rk1 + 226);
+    const double2 a0 = A0_114, a1 = A1_114;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c17*c17);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2767,7 +2883,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + -2.0 * q;
   }
   {  // reaction 127: CH + O2 => CO + OH
-    const double2 a0 = ld2(rk0 + 228), a1 = ld2(rk1 + 228);
+    const double2 a0 = A0_115, a1 = A1_115;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c26)*(c15);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2786,7 +2902,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 128: CO + OH => CH + O2
-    const double2 a0 = ld2(rk0 + 230), a1 = ld2(rk1 + 230);
+    const double2 a0 = A0_116, a1 = A1_116;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c9)*(c27);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2804,8 +2920,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w26 = w26 + 1.0 * q;
     w27 = w27 + -1.0 * q;
   }
+  const double2 A0_117 = ld2(rk0 + 232), A1_117 = ld2(rk1 + 232);
+  const double2 A0_118 = ld2(rk0 + 234), A1_118 = ld2(rk1 + 234);
+  const double2 A0_119 = ld2(rk0 + 236), A1_119 = ld2(rk1 + 236);
+  const double2 A0_120 = ld2(rk0 + 238), A1_120 = ld2(rk1 + 238);
   {  // reaction 129: C2H3 + O2 => C2H2 + HO2
-    const double2 a0 = ld2(rk0 + 232), a1 = ld2(rk1 + 232);
+    const double2 a0 = A0_117, a1 = A1_117;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c15);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2824,7 +2944,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + 1.0 * q;
   }
   {  // reaction 130: C2H2 + HO2 => C2H3 + O2
-    const double2 a0 = ld2(rk0 + 234), a1 = ld2(rk1 + 234);
+    const double2 a0 = A0_118, a1 = A1_118;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c4)*(c24);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2843,7 +2963,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + -1.0 * q;
   }
   {  // reaction 131: CH2CO + O => 2 HCO
-    const double2 a0 = ld2(rk0 + 236), a1 = ld2(rk1 + 236);
+    const double2 a0 = A0_119, a1 = A1_119;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c7);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2860,7 +2980,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + 2.0 * q;
   }
   {  // reaction 132: 2 HCO => CH2CO + O
-    const double2 a0 = ld2(rk0 + 238), a1 = ld2(rk1 + 238);
+    const double2 a0 = A0_120, a1 = A1_120;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c17*c17);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2876,8 +2996,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w12 = w12 + 1.0 * q;
     w17 = w17 + -2.0 * q;
   }
+  const double2 A0_121 = ld2(rk0 + 240), A1_121 = ld2(rk1 + 240);
+  const double2 A0_122 = ld2(rk0 + 242), A1_122 = ld2(rk1 + 242);
+  const double2 A0_123 = ld2(rk0 + 244), A1_123 = ld2(rk1 + 244);
+  const double2 A0_124 = ld2(rk0 + 246), A1_124 = ld2(rk1 + 246);
   {  // reaction 133: CH2CO + OH => CH2O + HCO
-    const double2 a0 = ld2(rk0 + 240), a1 = ld2(rk1 + 240);
+    const double2 a0 = A0_121, a1 = A1_121;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c27);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2896,7 +3020,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 134: CH2O + HCO => CH2CO + OH
-    const double2 a0 = ld2(rk0 + 242), a1 = ld2(rk1 + 242);
+    const double2 a0 = A0_122, a1 = A1_122;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c10)*(c17);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2915,7 +3039,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 135: C2H6 + O2 => C2H5 + HO2
-    const double2 a0 = ld2(rk0 + 244), a1 = ld2(rk1 + 244);
+    const double2 a0 = A0_123, a1 = A1_123;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c15);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2934,7 +3058,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + 1.0 * q;
   }
   {  // reaction 136: C2H5 + HO2 => C2H6 + O2
-    const double2 a0 = ld2(rk0 + 246), a1 = ld2(rk1 + 246);
+    const double2 a0 = A0_124, a1 = A1_124;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c28)*(c24);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2952,8 +3076,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + -1.0 * q;
     w28 = w28 + -1.0 * q;
   }
+  const double2 A0_125 = ld2(rk0 + 248), A1_125 = ld2(rk1 + 248);
+  const double2 A0_126 = ld2(rk0 + 250), A1_126 = ld2(rk1 + 250);
+  const double2 A0_127 = ld2(rk0 + 252), A1_127 = ld2(rk1 + 252);
+  const double2 A0_128 = ld2(rk0 + 254), A1_128 = ld2(rk1 + 254);
   {  // reaction 137: C2H6 + HO2 => C2H5 + H2O2
-    const double2 a0 = ld2(rk0 + 248), a1 = ld2(rk1 + 248);
+    const double2 a0 = A0_125, a1 = A1_125;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c24);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2972,7 +3100,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + 1.0 * q;
   }
   {  // reaction 138: C2H5 + H2O2 => C2H6 + HO2
-    const double2 a0 = ld2(rk0 + 250), a1 = ld2(rk1 + 250);
+    const double2 a0 = A0_126, a1 = A1_126;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c28)*(c8);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -2991,7 +3119,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + -1.0 * q;
   }
   {  // reaction 139: C2H4 + C2H6 => 2 C2H5
-    const double2 a0 = ld2(rk0 + 252), a1 = ld2(rk1 + 252);
+    const double2 a0 = A0_127, a1 = A1_127;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c11);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3008,7 +3136,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + 2.0 * q;
   }
   {  // reaction 140: 2 C2H5 => C2H4 + C2H6
-    const double2 a0 = ld2(rk0 + 254), a1 = ld2(rk1 + 254);
+    const double2 a0 = A0_128, a1 = A1_128;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c28*c28);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3024,8 +3152,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + 1.0 * q;
     w28 = w28 + -2.0 * q;
   }
+  const double2 A0_129 = ld2(rk0 + 256), A1_129 = ld2(rk1 + 256);
+  const double2 A0_130 = ld2(rk0 + 258), A1_130 = ld2(rk1 + 258);
+  const double2 A0_131 = ld2(rk0 + 260), A1_131 = ld2(rk1 + 260);
+  const double2 A0_132 = ld2(rk0 + 262), A1_132 = ld2(rk1 + 262);
   {  // reaction 141: C2H3 + CH3 => C2H2 + CH4
-    const double2 a0 = ld2(rk0 + 256), a1 = ld2(rk1 + 256);
+    const double2 a0 = A0_129, a1 = A1_129;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c18);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3044,7 +3176,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
   }
   {  // reaction 142: C2H2 + CH4 => C2H3 + CH3
-    const double2 a0 = ld2(rk0 + 258), a1 = ld2(rk1 + 258);
+    const double2 a0 = A0_130, a1 = A1_130;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c4)*(c3);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3063,7 +3195,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + 1.0 * q;
   }
   {  // reaction 143: C2H5 + CH3 => C2H4 + CH4
-    const double2 a0 = ld2(rk0 + 260), a1 = ld2(rk1 + 260);
+    const double2 a0 = A0_131, a1 = A1_131;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c28)*(c18);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3082,7 +3214,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + -1.0 * q;
   }
   {  // reaction 144: C2H4 + CH4 => C2H5 + CH3
-    const double2 a0 = ld2(rk0 + 262), a1 = ld2(rk1 + 262);
+    const double2 a0 = A0_132, a1 = A1_132;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c3);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3100,8 +3232,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + -1.0 * q;
     w28 = w28 + 1.0 * q;
   }
+  const double2 A0_133 = ld2(rk0 + 264), A1_133 = ld2(rk1 + 264);
+  const double2 A0_134 = ld2(rk0 + 266), A1_134 = ld2(rk1 + 266);
+  const double2 A0_135 = ld2(rk0 + 268), A1_135 = ld2(rk1 + 268);
+  const double2 A0_136 = ld2(rk0 + 270), A1_136 = ld2(rk1 + 270);
   {  // reaction 145: CH3CO + M => CH3 + CO + M
-    const double2 a0 = ld2(rk0 + 264), a1 = ld2(rk1 + 264);
+    const double2 a0 = A0_133, a1 = A1_133;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c20)*M11;
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3119,7 +3255,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w20 = -1.0 * q;
   }
   {  // reaction 146: CH3 + CO + M => CH3CO + M
-    const double2 a0 = ld2(rk0 + 266), a1 = ld2(rk1 + 266);
+    const double2 a0 = A0_134, a1 = A1_134;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18)*(c9)*M11;
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3137,7 +3273,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w20 = w20 + 1.0 * q;
   }
   {  // reaction 147: C2H4 + O2 => C2H3 + HO2
-    const double2 a0 = ld2(rk0 + 268), a1 = ld2(rk1 + 268);
+    const double2 a0 = A0_135, a1 = A1_135;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c15);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3156,7 +3292,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + -1.0 * q;
   }
   {  // reaction 148: C2H3 + HO2 => C2H4 + O2
-    const double2 a0 = ld2(rk0 + 270), a1 = ld2(rk1 + 270);
+    const double2 a0 = A0_136, a1 = A1_136;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c24);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3174,8 +3310,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + -1.0 * q;
     w25 = w25 + 1.0 * q;
   }
+  const double2 A0_137 = ld2(rk0 + 272), A1_137 = ld2(rk1 + 272);
+  const double2 A0_138 = ld2(rk0 + 274), A1_138 = ld2(rk1 + 274);
+  const double2 A0_139 = ld2(rk0 + 276), A1_139 = ld2(rk1 + 276);
+  const double2 A0_140 = ld2(rk0 + 278), A1_140 = ld2(rk1 + 278);
   {  // reaction 149: C2H4 + HO2 => C2H3 + H2O2
-    const double2 a0 = ld2(rk0 + 272), a1 = ld2(rk1 + 272);
+    const double2 a0 = A0_137, a1 = A1_137;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c24);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3194,7 +3334,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + -1.0 * q;
   }
   {  // reaction 150: C2H3 + H2O2 => C2H4 + HO2
-    const double2 a0 = ld2(rk0 + 274), a1 = ld2(rk1 + 274);
+    const double2 a0 = A0_138, a1 = A1_138;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c6)*(c8);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3213,7 +3353,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + 1.0 * q;
   }
   {  // reaction 151: IC3H7 => C2H4 + CH3
-    const double2 a0 = ld2(rk0 + 276), a1 = ld2(rk1 + 276);
+    const double2 a0 = A0_139, a1 = A1_139;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c13);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3230,7 +3370,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + 1.0 * q;
   }
   {  // reaction 152: C2H4 + CH3 => IC3H7
-    const double2 a0 = ld2(rk0 + 278), a1 = ld2(rk1 + 278);
+    const double2 a0 = A0_140, a1 = A1_140;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c18);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3246,8 +3386,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + -1.0 * q;
     w25 = w25 + -1.0 * q;
   }
+  const double2 A0_141 = ld2(rk0 + 280), A1_141 = ld2(rk1 + 280);
+  const double2 A0_142 = ld2(rk0 + 282), A1_142 = ld2(rk1 + 282);
+  const double2 A0_143 = ld2(rk0 + 284), A1_143 = ld2(rk1 + 284);
+  const double2 A0_144 = ld2(rk0 + 286), A1_144 = ld2(rk1 + 286);
   {  // reaction 153: CH3CO + O => CH2CO + OH
-    const double2 a0 = ld2(rk0 + 280), a1 = ld2(rk1 + 280);
+    const double2 a0 = A0_141, a1 = A1_141;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c20)*(c7);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3266,7 +3410,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 154: CH2CO + OH => CH3CO + O
-    const double2 a0 = ld2(rk0 + 282), a1 = ld2(rk1 + 282);
+    const double2 a0 = A0_142, a1 = A1_142;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c27);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3285,7 +3429,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 155: CH3 + CH3CO => CH2CO + CH4
-    const double2 a0 = ld2(rk0 + 284), a1 = ld2(rk1 + 284);
+    const double2 a0 = A0_143, a1 = A1_143;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18)*(c20);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3304,7 +3448,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w20 = w20 + -1.0 * q;
   }
   {  // reaction 156: CH2CO + CH4 => CH3 + CH3CO
-    const double2 a0 = ld2(rk0 + 286), a1 = ld2(rk1 + 286);
+    const double2 a0 = A0_144, a1 = A1_144;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c3);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3322,8 +3466,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w18 = w18 + 1.0 * q;
     w20 = w20 + 1.0 * q;
   }
+  const double2 A0_145 = ld2(rk0 + 288), A1_145 = ld2(rk1 + 288);
+  const double2 A0_146 = ld2(rk0 + 290), A1_146 = ld2(rk1 + 290);
+  const double2 A0_147 = ld2(rk0 + 292), A1_147 = ld2(rk1 + 292);
+  const double2 A0_148 = ld2(rk0 + 294), A1_148 = ld2(rk1 + 294);
   {  // reaction 157: C2H4 + O => CH2CHO + H
-    const double2 a0 = ld2(rk0 + 288), a1 = ld2(rk1 + 288);
+    const double2 a0 = A0_145, a1 = A1_145;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c7);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3342,7 +3490,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + -1.0 * q;
   }
   {  // reaction 158: CH2CHO + H => C2H4 + O
-    const double2 a0 = ld2(rk0 + 290), a1 = ld2(rk1 + 290);
+    const double2 a0 = A0_146, a1 = A1_146;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c1)*(c2);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3361,7 +3509,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + 1.0 * q;
   }
   {  // reaction 159: C2H6 + CH => C2H5 + CH2
-    const double2 a0 = ld2(rk0 + 292), a1 = ld2(rk1 + 292);
+    const double2 a0 = A0_147, a1 = A1_147;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c26);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3380,7 +3528,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + 1.0 * q;
   }
   {  // reaction 160: C2H5 + CH2 => C2H6 + CH
-    const double2 a0 = ld2(rk0 + 294), a1 = ld2(rk1 + 294);
+    const double2 a0 = A0_148, a1 = A1_148;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c28)*(c29);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3398,8 +3546,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w28 = w28 + -1.0 * q;
     w29 = w29 + -1.0 * q;
   }
+  const double2 A0_149 = ld2(rk0 + 296), A1_149 = ld2(rk1 + 296);
+  const double2 A0_150 = ld2(rk0 + 298), A1_150 = ld2(rk1 + 298);
+  const double2 A0_151 = ld2(rk0 + 300), A1_151 = ld2(rk1 + 300);
+  const double2 A0_152 = ld2(rk0 + 302), A1_152 = ld2(rk1 + 302);
   {  // reaction 161: C2H6 + CH2 => C2H5 + CH3
-    const double2 a0 = ld2(rk0 + 296), a1 = ld2(rk1 + 296);
+    const double2 a0 = A0_149, a1 = A1_149;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c11)*(c29);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3418,7 +3570,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + -1.0 * q;
   }
   {  // reaction 162: C2H5 + CH3 => C2H6 + CH2
-    const double2 a0 = ld2(rk0 + 298), a1 = ld2(rk1 + 298);
+    const double2 a0 = A0_150, a1 = A1_150;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c28)*(c18);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3437,7 +3589,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w29 = w29 + 1.0 * q;
   }
   {  // reaction 163: C2H4 + OH => CH2O + CH3
-    const double2 a0 = ld2(rk0 + 300), a1 = ld2(rk1 + 300);
+    const double2 a0 = A0_151, a1 = A1_151;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c27);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3456,7 +3608,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 164: CH2O + CH3 => C2H4 + OH
-    const double2 a0 = ld2(rk0 + 302), a1 = ld2(rk1 + 302);
+    const double2 a0 = A0_152, a1 = A1_152;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c10)*(c18);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3474,8 +3626,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + 1.0 * q;
     w27 = w27 + 1.0 * q;
   }
+  const double2 A0_153 = ld2(rk0 + 304), A1_153 = ld2(rk1 + 304);
+  const double2 A0_154 = ld2(rk0 + 306), A1_154 = ld2(rk1 + 306);
+  const double2 A0_155 = ld2(rk0 + 308), A1_155 = ld2(rk1 + 308);
+  const double2 A0_156 = ld2(rk0 + 310), A1_156 = ld2(rk1 + 310);
   {  // reaction 165: C2H4O1-2 => CH3 + HCO
-    const double2 a0 = ld2(rk0 + 304), a1 = ld2(rk1 + 304);
+    const double2 a0 = A0_153, a1 = A1_153;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c22);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3492,7 +3648,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w22 = -1.0 * q;
   }
   {  // reaction 166: CH3 + HCO => C2H4O1-2
-    const double2 a0 = ld2(rk0 + 306), a1 = ld2(rk1 + 306);
+    const double2 a0 = A0_154, a1 = A1_154;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18)*(c17);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3509,7 +3665,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w22 = w22 + 1.0 * q;
   }
   {  // reaction 167: C2H4O1-2 + OH => C2H3O1-2 + H2O
-    const double2 a0 = ld2(rk0 + 308), a1 = ld2(rk1 + 308);
+    const double2 a0 = A0_155, a1 = A1_155;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c22)*(c27);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3528,7 +3684,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 168: C2H3O1-2 + H2O => C2H4O1-2 + OH
-    const double2 a0 = ld2(rk0 + 310), a1 = ld2(rk1 + 310);
+    const double2 a0 = A0_156, a1 = A1_156;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c23)*(c14);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3546,8 +3702,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + -1.0 * q;
     w27 = w27 + 1.0 * q;
   }
+  const double2 A0_157 = ld2(rk0 + 312), A1_157 = ld2(rk1 + 312);
+  const double2 A0_158 = ld2(rk0 + 314), A1_158 = ld2(rk1 + 314);
+  const double2 A0_159 = ld2(rk0 + 316), A1_159 = ld2(rk1 + 316);
+  const double2 A0_160 = ld2(rk0 + 318), A1_160 = ld2(rk1 + 318);
   {  // reaction 169: C2H4O1-2 + H => C2H3O1-2 + H2
-    const double2 a0 = ld2(rk0 + 312), a1 = ld2(rk1 + 312);
+    const double2 a0 = A0_157, a1 = A1_157;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c22)*(c2);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3566,7 +3726,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + 1.0 * q;
   }
   {  // reaction 170: C2H3O1-2 + H2 => C2H4O1-2 + H
-    const double2 a0 = ld2(rk0 + 314), a1 = ld2(rk1 + 314);
+    const double2 a0 = A0_158, a1 = A1_158;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c23)*(c19);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3585,7 +3745,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + -1.0 * q;
   }
   {  // reaction 171: C2H4O1-2 + HO2 => C2H3O1-2 + H2O2
-    const double2 a0 = ld2(rk0 + 316), a1 = ld2(rk1 + 316);
+    const double2 a0 = A0_159, a1 = A1_159;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c22)*(c24);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3604,7 +3764,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + -1.0 * q;
   }
   {  // reaction 172: C2H3O1-2 + H2O2 => C2H4O1-2 + HO2
-    const double2 a0 = ld2(rk0 + 318), a1 = ld2(rk1 + 318);
+    const double2 a0 = A0_160, a1 = A1_160;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c23)*(c8);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3622,8 +3782,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + -1.0 * q;
     w24 = w24 + 1.0 * q;
   }
+  const double2 A0_161 = ld2(rk0 + 320), A1_161 = ld2(rk1 + 320);
+  const double2 A0_162 = ld2(rk0 + 322), A1_162 = ld2(rk1 + 322);
+  const double2 A0_163 = ld2(rk0 + 324), A1_163 = ld2(rk1 + 324);
+  const double2 A0_164 = ld2(rk0 + 326), A1_164 = ld2(rk1 + 326);
   {  // reaction 173: C2H4O1-2 + CH3 => C2H3O1-2 + CH4
-    const double2 a0 = ld2(rk0 + 320), a1 = ld2(rk1 + 320);
+    const double2 a0 = A0_161, a1 = A1_161;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c22)*(c18);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3642,7 +3806,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + 1.0 * q;
   }
   {  // reaction 174: C2H3O1-2 + CH4 => C2H4O1-2 + CH3
-    const double2 a0 = ld2(rk0 + 322), a1 = ld2(rk1 + 322);
+    const double2 a0 = A0_162, a1 = A1_162;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c23)*(c3);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3661,7 +3825,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + -1.0 * q;
   }
   {  // reaction 175: C3H6O1-3 => C2H4 + CH2O
-    const double2 a0 = ld2(rk0 + 324), a1 = ld2(rk1 + 324);
+    const double2 a0 = A0_163, a1 = A1_163;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c21);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3678,7 +3842,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + 1.0 * q;
   }
   {  // reaction 176: C2H4 + CH2O => C3H6O1-3
-    const double2 a0 = ld2(rk0 + 326), a1 = ld2(rk1 + 326);
+    const double2 a0 = A0_164, a1 = A1_164;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c10);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3694,8 +3858,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + 1.0 * q;
     w25 = w25 + -1.0 * q;
   }
+  const double2 A0_165 = ld2(rk0 + 328), A1_165 = ld2(rk1 + 328);
+  const double2 A0_166 = ld2(rk0 + 330), A1_166 = ld2(rk1 + 330);
+  const double2 A0_167 = ld2(rk0 + 332), A1_167 = ld2(rk1 + 332);
+  const double2 A0_168 = ld2(rk0 + 334), A1_168 = ld2(rk1 + 334);
   {  // reaction 177: C3H6O1-3 + OH => C2H3 + CH2O + H2O
-    const double2 a0 = ld2(rk0 + 328), a1 = ld2(rk1 + 328);
+    const double2 a0 = A0_165, a1 = A1_165;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c21)*(c27);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3716,7 +3884,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
   }
   {  // reaction 178: C3H6O1-3 + O => C2H3 + CH2O + OH
-    const double2 a0 = ld2(rk0 + 330), a1 = ld2(rk1 + 330);
+    const double2 a0 = A0_166, a1 = A1_166;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c21)*(c7);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3737,7 +3905,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + 1.0 * q;
   }
   {  // reaction 179: C3H6O1-3 + H => C2H3 + CH2O + H2
-    const double2 a0 = ld2(rk0 + 332), a1 = ld2(rk1 + 332);
+    const double2 a0 = A0_167, a1 = A1_167;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c21)*(c2);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3758,7 +3926,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + -1.0 * q;
   }
   {  // reaction 180: C3H6O1-3 + HO2 => C2H3 + CH2O + H2O2
-    const double2 a0 = ld2(rk0 + 334), a1 = ld2(rk1 + 334);
+    const double2 a0 = A0_168, a1 = A1_168;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c21)*(c24);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3778,8 +3946,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + -1.0 * q;
     w24 = w24 + -1.0 * q;
   }
+  const double2 A0_169 = ld2(rk0 + 336), A1_169 = ld2(rk1 + 336);
+  const double2 A0_170 = ld2(rk0 + 338), A1_170 = ld2(rk1 + 338);
+  const double2 A0_171 = ld2(rk0 + 340), A1_171 = ld2(rk1 + 340);
+  const double2 A0_172 = ld2(rk0 + 342), A1_172 = ld2(rk1 + 342);
   {  // reaction 181: C3H6O1-3 + CH3 => C2H3 + CH2O + CH4
-    const double2 a0 = ld2(rk0 + 336), a1 = ld2(rk1 + 336);
+    const double2 a0 = A0_169, a1 = A1_169;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c21)*(c18);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3800,7 +3972,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w21 = w21 + -1.0 * q;
   }
   {  // reaction 182: C3H3 + O2 => CH2CO + HCO
-    const double2 a0 = ld2(rk0 + 338), a1 = ld2(rk1 + 338);
+    const double2 a0 = A0_170, a1 = A1_170;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c5)*(c15);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3819,7 +3991,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + 1.0 * q;
   }
   {  // reaction 183: CH2CO + HCO => C3H3 + O2
-    const double2 a0 = ld2(rk0 + 340), a1 = ld2(rk1 + 340);
+    const double2 a0 = A0_171, a1 = A1_171;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c17);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3838,7 +4010,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w17 = w17 + -1.0 * q;
   }
   {  // reaction 184: C2H3O1-2 => CH3CO
-    const double2 a0 = ld2(rk0 + 342), a1 = ld2(rk1 + 342);
+    const double2 a0 = A0_172, a1 = A1_172;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c23);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3852,8 +4024,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w20 = w20 + 1.0 * q;
     w23 = w23 + -1.0 * q;
   }
+  const double2 A0_173 = ld2(rk0 + 344), A1_173 = ld2(rk1 + 344);
+  const double2 A0_174 = ld2(rk0 + 346), A1_174 = ld2(rk1 + 346);
+  const double2 A0_175 = ld2(rk0 + 348), A1_175 = ld2(rk1 + 348);
+  const double2 A0_176 = ld2(rk0 + 350), A1_176 = ld2(rk1 + 350);
   {  // reaction 185: CH3CO => C2H3O1-2
-    const double2 a0 = ld2(rk0 + 344), a1 = ld2(rk1 + 344);
+    const double2 a0 = A0_173, a1 = A1_173;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c20);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3868,7 +4044,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + 1.0 * q;
   }
   {  // reaction 186: C2H3O1-2 => CH2CHO
-    const double2 a0 = ld2(rk0 + 346), a1 = ld2(rk1 + 346);
+    const double2 a0 = A0_174, a1 = A1_174;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c23);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3883,7 +4059,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + -1.0 * q;
   }
   {  // reaction 187: CH2CHO => C2H3O1-2
-    const double2 a0 = ld2(rk0 + 348), a1 = ld2(rk1 + 348);
+    const double2 a0 = A0_175, a1 = A1_175;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c1);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3898,7 +4074,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w23 = w23 + 1.0 * q;
   }
   {  // reaction 188: CH2CHO => CH2CO + H
-    const double2 a0 = ld2(rk0 + 350), a1 = ld2(rk1 + 350);
+    const double2 a0 = A0_176, a1 = A1_176;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c1);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3914,8 +4090,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w2 = w2 + 1.0 * q;
     w12 = w12 + 1.0 * q;
   }
+  const double2 A0_177 = ld2(rk0 + 352), A1_177 = ld2(rk1 + 352);
+  const double2 A0_178 = ld2(rk0 + 354), A1_178 = ld2(rk1 + 354);
+  const double2 A0_179 = ld2(rk0 + 356), A1_179 = ld2(rk1 + 356);
+  const double2 A0_180 = ld2(rk0 + 358), A1_180 = ld2(rk1 + 358);
   {  // reaction 189: CH2CO + H => CH2CHO
-    const double2 a0 = ld2(rk0 + 352), a1 = ld2(rk1 + 352);
+    const double2 a0 = A0_177, a1 = A1_177;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c2);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3932,7 +4112,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w12 = w12 + -1.0 * q;
   }
   {  // reaction 190: CH2CHO + O2 => CH2CO + HO2
-    const double2 a0 = ld2(rk0 + 354), a1 = ld2(rk1 + 354);
+    const double2 a0 = A0_178, a1 = A1_178;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c1)*(c15);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3951,7 +4131,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + 1.0 * q;
   }
   {  // reaction 191: CH2CO + HO2 => CH2CHO + O2
-    const double2 a0 = ld2(rk0 + 356), a1 = ld2(rk1 + 356);
+    const double2 a0 = A0_179, a1 = A1_179;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c24);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3970,7 +4150,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w24 = w24 + -1.0 * q;
   }
   {  // reaction 192: CH2CO + OH => CH3 + CO2
-    const double2 a0 = ld2(rk0 + 358), a1 = ld2(rk1 + 358);
+    const double2 a0 = A0_180, a1 = A1_180;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c12)*(c27);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -3988,8 +4168,12 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w27 = w27 + -1.0 * q;
     w30 = w30 + 1.0 * q;
   }
+  const double2 A0_181 = ld2(rk0 + 360), A1_181 = ld2(rk1 + 360);
+  const double2 A0_182 = ld2(rk0 + 362), A1_182 = ld2(rk1 + 362);
+  const double2 A0_183 = ld2(rk0 + 364), A1_183 = ld2(rk1 + 364);
+  const double2 A0_184 = ld2(rk0 + 366), A1_184 = ld2(rk1 + 366);
   {  // reaction 193: CH3 + CO2 => CH2CO + OH
-    const double2 a0 = ld2(rk0 + 360), a1 = ld2(rk1 + 360);
+    const double2 a0 = A0_181, a1 = A1_181;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c18)*(c30);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -4008,7 +4192,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w30 = w30 + -1.0 * q;
   }
   {  // reaction 194: IC3H7CO => CO + IC3H7
-    const double2 a0 = ld2(rk0 + 362), a1 = ld2(rk1 + 362);
+    const double2 a0 = A0_182, a1 = A1_182;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c0);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -4025,7 +4209,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + 1.0 * q;
   }
   {  // reaction 195: CO + IC3H7 => IC3H7CO
-    const double2 a0 = ld2(rk0 + 364), a1 = ld2(rk1 + 364);
+    const double2 a0 = A0_183, a1 = A1_183;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c9)*(c13);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -4042,7 +4226,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w13 = w13 + -1.0 * q;
   }
   {  // reaction 196: C2H4 + HO2 => C2H4O1-2 + OH
-    const double2 a0 = ld2(rk0 + 366), a1 = ld2(rk1 + 366);
+    const double2 a0 = A0_184, a1 = A1_184;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c25)*(c24);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -4060,8 +4244,9 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     w25 = w25 + -1.0 * q;
     w27 = w27 + 1.0 * q;
   }
+  const double2 A0_185 = ld2(rk0 + 368), A1_185 = ld2(rk1 + 368);
   {  // reaction 197: C2H4O1-2 + OH => C2H4 + HO2
-    const double2 a0 = ld2(rk0 + 368), a1 = ld2(rk1 + 368);
+    const double2 a0 = A0_185, a1 = A1_185;
     const double kf = a0.x + (a1.x - a0.x) * Td, kb = a0.y + (a1.y - a0.y) * Td;
     const double pf = kf*(c22)*(c27);
     const double pb = 0.0;  // irreversible: kb table is identically 0
@@ -4487,13 +4672,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_186 = ld2(rk0 + 362), A1_186 = ld2(rk1 + 362);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 362), a1 = ld2(rk1 + 362);
+    { const double2 a0 = A0_186, a1 = A1_186;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 1.0;
       jc0 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -4678,31 +4864,35 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_187 = ld2(rk0 + 290), A1_187 = ld2(rk1 + 290);
+    const double2 A0_188 = ld2(rk0 + 348), A1_188 = ld2(rk1 + 348);
+    const double2 A0_189 = ld2(rk0 + 350), A1_189 = ld2(rk1 + 350);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 290), a1 = ld2(rk1 + 290);
+    { const double2 a0 = A0_187, a1 = A1_187;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc1 += -1.0 * dq;
       jc2 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 348), a1 = ld2(rk1 + 348);
+    { const double2 a0 = A0_188, a1 = A1_188;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 1.0;
       jc1 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 350), a1 = ld2(rk1 + 350);
+    { const double2 a0 = A0_189, a1 = A1_189;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 1.0;
       jc1 += -1.0 * dq;
       jc2 += 1.0 * dq;
       jc12 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 354), a1 = ld2(rk1 + 354);
+    const double2 A0_190 = ld2(rk0 + 354), A1_190 = ld2(rk1 + 354);
+    { const double2 a0 = A0_190, a1 = A1_190;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc1 += -1.0 * dq;
       jc12 += 1.0 * dq;
@@ -4776,12 +4966,13 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc28 = 0.0;
     double jc29 = 0.0;
     double jc30 = 0.0;
+    const double2 A0_191 = ld2(rk0 + 2), A1_191 = ld2(rk1 + 2);
     {
       const double dq = scr[0 * SST];
       jc7 += -2.0 * dq;
       jc15 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 2), a1 = ld2(rk1 + 2);
+    { const double2 a0 = A0_191, a1 = A1_191;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M1 * c7 + scr[1 * SST];
       jc2 += -1.0 * dq;
       jc7 += -1.0 * dq;
@@ -4828,6 +5019,9 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
+    const double2 A0_192 = ld2(rk0 + 32), A1_192 = ld2(rk1 + 32);
+    const double2 A0_193 = ld2(rk0 + 46), A1_193 = ld2(rk1 + 46);
+    const double2 A0_194 = ld2(rk0 + 48), A1_194 = ld2(rk1 + 48);
     {
       const double dq = -scr[74 * SST] * c30;
       jc2 += 1.0 * dq;
@@ -4835,105 +5029,119 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc17 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 32), a1 = ld2(rk1 + 32);
+    { const double2 a0 = A0_192, a1 = A1_192;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c12;
       jc2 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc12 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 46), a1 = ld2(rk1 + 46);
+    { const double2 a0 = A0_193, a1 = A1_193;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M3 * c15 + scr[3 * SST];
       jc2 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 48), a1 = ld2(rk1 + 48);
+    { const double2 a0 = A0_194, a1 = A1_194;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M4 * c15;
       jc2 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 50), a1 = ld2(rk1 + 50);
+    const double2 A0_195 = ld2(rk0 + 50), A1_195 = ld2(rk1 + 50);
+    const double2 A0_196 = ld2(rk0 + 52), A1_196 = ld2(rk1 + 52);
+    const double2 A0_197 = ld2(rk0 + 54), A1_197 = ld2(rk1 + 54);
+    const double2 A0_198 = ld2(rk0 + 56), A1_198 = ld2(rk1 + 56);
+    { const double2 a0 = A0_195, a1 = A1_195;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M5 * c15;
       jc2 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 52), a1 = ld2(rk1 + 52);
+    { const double2 a0 = A0_196, a1 = A1_196;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc2 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 54), a1 = ld2(rk1 + 54);
+    { const double2 a0 = A0_197, a1 = A1_197;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M6 * 2.0*c2 + scr[6 * SST];
       jc2 += -2.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 56), a1 = ld2(rk1 + 56);
+    { const double2 a0 = A0_198, a1 = A1_198;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M7 * 2.0*c2;
       jc2 += -2.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 58), a1 = ld2(rk1 + 58);
+    const double2 A0_199 = ld2(rk0 + 58), A1_199 = ld2(rk1 + 58);
+    const double2 A0_200 = ld2(rk0 + 60), A1_200 = ld2(rk1 + 60);
+    const double2 A0_201 = ld2(rk0 + 62), A1_201 = ld2(rk1 + 62);
+    const double2 A0_202 = ld2(rk0 + 64), A1_202 = ld2(rk1 + 64);
+    { const double2 a0 = A0_199, a1 = A1_199;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M5 * 2.0*c2;
       jc2 += -2.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 60), a1 = ld2(rk1 + 60);
+    { const double2 a0 = A0_200, a1 = A1_200;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M8 * 2.0*c2;
       jc2 += -2.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 62), a1 = ld2(rk1 + 62);
+    { const double2 a0 = A0_201, a1 = A1_201;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M9 * c27 + scr[10 * SST];
       jc2 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 64), a1 = ld2(rk1 + 64);
+    { const double2 a0 = A0_202, a1 = A1_202;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc2 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 66), a1 = ld2(rk1 + 66);
+    const double2 A0_203 = ld2(rk0 + 66), A1_203 = ld2(rk1 + 66);
+    const double2 A0_204 = ld2(rk0 + 68), A1_204 = ld2(rk1 + 68);
+    const double2 A0_205 = ld2(rk0 + 70), A1_205 = ld2(rk1 + 70);
+    const double2 A0_206 = ld2(rk0 + 72), A1_206 = ld2(rk1 + 72);
+    { const double2 a0 = A0_203, a1 = A1_203;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc2 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc19 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 68), a1 = ld2(rk1 + 68);
+    { const double2 a0 = A0_204, a1 = A1_204;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc2 += -1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 70), a1 = ld2(rk1 + 70);
+    { const double2 a0 = A0_205, a1 = A1_205;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc2 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 72), a1 = ld2(rk1 + 72);
+    { const double2 a0 = A0_206, a1 = A1_206;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc2 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
+    const double2 A0_207 = ld2(rk0 + 74), A1_207 = ld2(rk1 + 74);
+    const double2 A0_208 = ld2(rk0 + 76), A1_208 = ld2(rk1 + 76);
     {
       const double dq = scr[12 * SST] * c29 + scr[11 * SST];
       jc2 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 74), a1 = ld2(rk1 + 74);
+    { const double2 a0 = A0_207, a1 = A1_207;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
       jc2 += -1.0 * dq;
       jc16 += -1.0 * dq;
@@ -4946,27 +5154,29 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc3 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 76), a1 = ld2(rk1 + 76);
+    { const double2 a0 = A0_208, a1 = A1_208;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc2 += -1.0 * dq;
       jc3 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc19 += 1.0 * dq;
     }
+    const double2 A0_209 = ld2(rk0 + 78), A1_209 = ld2(rk1 + 78);
+    const double2 A0_210 = ld2(rk0 + 80), A1_210 = ld2(rk1 + 80);
     {
       const double dq = scr[18 * SST] * c17 + scr[17 * SST];
       jc2 += -1.0 * dq;
       jc10 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 78), a1 = ld2(rk1 + 78);
+    { const double2 a0 = A0_209, a1 = A1_209;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c17;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 80), a1 = ld2(rk1 + 80);
+    { const double2 a0 = A0_210, a1 = A1_210;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c10;
       jc2 += -1.0 * dq;
       jc10 += -1.0 * dq;
@@ -4979,13 +5189,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += -1.0 * dq;
       jc6 += 1.0 * dq;
     }
+    const double2 A0_211 = ld2(rk0 + 82), A1_211 = ld2(rk1 + 82);
+    const double2 A0_212 = ld2(rk0 + 84), A1_212 = ld2(rk1 + 84);
     {
       const double dq = scr[24 * SST] * c6 + scr[23 * SST];
       jc2 += -1.0 * dq;
       jc6 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 82), a1 = ld2(rk1 + 82);
+    { const double2 a0 = A0_211, a1 = A1_211;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc2 += -1.0 * dq;
       jc4 += 1.0 * dq;
@@ -4998,47 +5210,52 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc25 += -1.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 84), a1 = ld2(rk1 + 84);
+    { const double2 a0 = A0_212, a1 = A1_212;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc2 += -1.0 * dq;
       jc6 += 1.0 * dq;
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
+    const double2 A0_213 = ld2(rk0 + 86), A1_213 = ld2(rk1 + 86);
+    const double2 A0_214 = ld2(rk0 + 88), A1_214 = ld2(rk1 + 88);
+    const double2 A0_215 = ld2(rk0 + 90), A1_215 = ld2(rk1 + 90);
     {
       const double dq = scr[30 * SST] * c28 + scr[29 * SST];
       jc2 += -1.0 * dq;
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 86), a1 = ld2(rk1 + 86);
+    { const double2 a0 = A0_213, a1 = A1_213;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c28;
       jc2 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc25 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 88), a1 = ld2(rk1 + 88);
+    { const double2 a0 = A0_214, a1 = A1_214;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc2 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 90), a1 = ld2(rk1 + 90);
+    { const double2 a0 = A0_215, a1 = A1_215;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc12 += -1.0 * dq;
       jc18 += 1.0 * dq;
     }
+    const double2 A0_216 = ld2(rk0 + 92), A1_216 = ld2(rk1 + 92);
+    const double2 A0_217 = ld2(rk0 + 102), A1_217 = ld2(rk1 + 102);
     {
       const double dq = scr[32 * SST];
       jc9 += -1.0 * dq;
       jc10 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 92), a1 = ld2(rk1 + 92);
+    { const double2 a0 = A0_216, a1 = A1_216;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc2 += 1.0 * dq;
       jc14 += 1.0 * dq;
@@ -5050,97 +5267,111 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 102), a1 = ld2(rk1 + 102);
+    { const double2 a0 = A0_217, a1 = A1_217;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
       jc2 += 1.0 * dq;
       jc17 += 1.0 * dq;
       jc26 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 104), a1 = ld2(rk1 + 104);
+    const double2 A0_218 = ld2(rk0 + 104), A1_218 = ld2(rk1 + 104);
+    const double2 A0_219 = ld2(rk0 + 108), A1_219 = ld2(rk1 + 108);
+    const double2 A0_220 = ld2(rk0 + 116), A1_220 = ld2(rk1 + 116);
+    const double2 A0_221 = ld2(rk0 + 122), A1_221 = ld2(rk1 + 122);
+    { const double2 a0 = A0_218, a1 = A1_218;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c10;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 108), a1 = ld2(rk1 + 108);
+    { const double2 a0 = A0_219, a1 = A1_219;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c10;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 116), a1 = ld2(rk1 + 116);
+    { const double2 a0 = A0_220, a1 = A1_220;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c30;
       jc2 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc27 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 122), a1 = ld2(rk1 + 122);
+    { const double2 a0 = A0_221, a1 = A1_221;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c12;
       jc2 += 1.0 * dq;
       jc4 += -1.0 * dq;
       jc12 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 146), a1 = ld2(rk1 + 146);
+    const double2 A0_222 = ld2(rk0 + 146), A1_222 = ld2(rk1 + 146);
+    const double2 A0_223 = ld2(rk0 + 148), A1_223 = ld2(rk1 + 148);
+    const double2 A0_224 = ld2(rk0 + 150), A1_224 = ld2(rk1 + 150);
+    const double2 A0_225 = ld2(rk0 + 152), A1_225 = ld2(rk1 + 152);
+    { const double2 a0 = A0_222, a1 = A1_222;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c29;
       jc2 += 1.0 * dq;
       jc19 += -1.0 * dq;
       jc26 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 148), a1 = ld2(rk1 + 148);
+    { const double2 a0 = A0_223, a1 = A1_223;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c10;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc14 += -1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 150), a1 = ld2(rk1 + 150);
+    { const double2 a0 = A0_224, a1 = A1_224;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
       jc26 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 152), a1 = ld2(rk1 + 152);
+    { const double2 a0 = A0_225, a1 = A1_225;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c6;
       jc2 += 1.0 * dq;
       jc6 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 154), a1 = ld2(rk1 + 154);
+    const double2 A0_226 = ld2(rk0 + 154), A1_226 = ld2(rk1 + 154);
+    const double2 A0_227 = ld2(rk0 + 158), A1_227 = ld2(rk1 + 158);
+    const double2 A0_228 = ld2(rk0 + 162), A1_228 = ld2(rk1 + 162);
+    const double2 A0_229 = ld2(rk0 + 166), A1_229 = ld2(rk1 + 166);
+    { const double2 a0 = A0_226, a1 = A1_226;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c25;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc25 += 1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 158), a1 = ld2(rk1 + 158);
+    { const double2 a0 = A0_227, a1 = A1_227;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c12;
       jc2 += 1.0 * dq;
       jc10 += -1.0 * dq;
       jc12 += 1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 162), a1 = ld2(rk1 + 162);
+    { const double2 a0 = A0_228, a1 = A1_228;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc2 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc19 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 166), a1 = ld2(rk1 + 166);
+    { const double2 a0 = A0_229, a1 = A1_229;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c25;
       jc2 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
+    const double2 A0_230 = ld2(rk0 + 174), A1_230 = ld2(rk1 + 174);
+    const double2 A0_231 = ld2(rk0 + 178), A1_231 = ld2(rk1 + 178);
     {
       const double dq = scr[38 * SST];
       jc9 += -1.0 * dq;
@@ -5155,43 +5386,47 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc16 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 174), a1 = ld2(rk1 + 174);
+    { const double2 a0 = A0_230, a1 = A1_230;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc2 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 178), a1 = ld2(rk1 + 178);
+    { const double2 a0 = A0_231, a1 = A1_231;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c25;
       jc2 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
+    const double2 A0_232 = ld2(rk0 + 194), A1_232 = ld2(rk1 + 194);
+    const double2 A0_233 = ld2(rk0 + 204), A1_233 = ld2(rk1 + 204);
+    const double2 A0_234 = ld2(rk0 + 206), A1_234 = ld2(rk1 + 206);
     {
       const double dq = scr[41 * SST];
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 194), a1 = ld2(rk1 + 194);
+    { const double2 a0 = A0_232, a1 = A1_232;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c28;
       jc2 += 1.0 * dq;
       jc18 += -2.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 204), a1 = ld2(rk1 + 204);
+    { const double2 a0 = A0_233, a1 = A1_233;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M5 * c9;
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 206), a1 = ld2(rk1 + 206);
+    { const double2 a0 = A0_234, a1 = A1_234;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M10 * c9 + scr[45 * SST];
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
+    const double2 A0_235 = ld2(rk0 + 290), A1_235 = ld2(rk1 + 290);
     {
       const double dq = scr[46 * SST];
       jc4 += 1.0 * dq;
@@ -5210,21 +5445,24 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 290), a1 = ld2(rk1 + 290);
+    { const double2 a0 = A0_235, a1 = A1_235;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c1;
       jc1 += -1.0 * dq;
       jc2 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 312), a1 = ld2(rk1 + 312);
+    const double2 A0_236 = ld2(rk0 + 312), A1_236 = ld2(rk1 + 312);
+    const double2 A0_237 = ld2(rk0 + 332), A1_237 = ld2(rk1 + 332);
+    const double2 A0_238 = ld2(rk0 + 352), A1_238 = ld2(rk1 + 352);
+    { const double2 a0 = A0_236, a1 = A1_236;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c22;
       jc2 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc22 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 332), a1 = ld2(rk1 + 332);
+    { const double2 a0 = A0_237, a1 = A1_237;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c21;
       jc2 += -1.0 * dq;
       jc6 += 1.0 * dq;
@@ -5232,7 +5470,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 352), a1 = ld2(rk1 + 352);
+    { const double2 a0 = A0_238, a1 = A1_238;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc1 += 1.0 * dq;
       jc2 += -1.0 * dq;
@@ -5350,13 +5588,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
+    const double2 A0_239 = ld2(rk0 + 76), A1_239 = ld2(rk1 + 76);
     {
       const double dq = -scr[16 * SST] * 1.0 + 2.0 * scr[14 * SST];
       jc2 += -1.0 * dq;
       jc3 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 76), a1 = ld2(rk1 + 76);
+    { const double2 a0 = A0_239, a1 = A1_239;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc3 += -1.0 * dq;
@@ -5399,33 +5638,39 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc10 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
+    const double2 A0_240 = ld2(rk0 + 114), A1_240 = ld2(rk1 + 114);
+    const double2 A0_241 = ld2(rk0 + 138), A1_241 = ld2(rk1 + 138);
+    const double2 A0_242 = ld2(rk0 + 154), A1_242 = ld2(rk1 + 154);
     {
       const double dq = 2.0 * scr[35 * SST];
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 114), a1 = ld2(rk1 + 114);
+    { const double2 a0 = A0_240, a1 = A1_240;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc3 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 138), a1 = ld2(rk1 + 138);
+    { const double2 a0 = A0_241, a1 = A1_241;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c15;
       jc3 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 154), a1 = ld2(rk1 + 154);
+    { const double2 a0 = A0_242, a1 = A1_242;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc25 += 1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 168), a1 = ld2(rk1 + 168);
+    const double2 A0_243 = ld2(rk0 + 168), A1_243 = ld2(rk1 + 168);
+    const double2 A0_244 = ld2(rk0 + 180), A1_244 = ld2(rk1 + 180);
+    const double2 A0_245 = ld2(rk0 + 192), A1_245 = ld2(rk1 + 192);
+    { const double2 a0 = A0_243, a1 = A1_243;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c29;
       jc3 += -1.0 * dq;
       jc18 += 2.0 * dq;
@@ -5437,46 +5682,51 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 180), a1 = ld2(rk1 + 180);
+    { const double2 a0 = A0_244, a1 = A1_244;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
       jc3 += -1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += 2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 192), a1 = ld2(rk1 + 192);
+    { const double2 a0 = A0_245, a1 = A1_245;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c24;
       jc3 += 1.0 * dq;
       jc8 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
+    const double2 A0_246 = ld2(rk0 + 196), A1_246 = ld2(rk1 + 196);
+    const double2 A0_247 = ld2(rk0 + 198), A1_247 = ld2(rk1 + 198);
+    const double2 A0_248 = ld2(rk0 + 200), A1_248 = ld2(rk1 + 200);
     {
       const double dq = 2.0 * scr[41 * SST];
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 196), a1 = ld2(rk1 + 196);
+    { const double2 a0 = A0_246, a1 = A1_246;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc3 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 198), a1 = ld2(rk1 + 198);
+    { const double2 a0 = A0_247, a1 = A1_247;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
       jc3 += 1.0 * dq;
       jc10 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 200), a1 = ld2(rk1 + 200);
+    { const double2 a0 = A0_248, a1 = A1_248;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c6;
       jc3 += 1.0 * dq;
       jc6 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 202), a1 = ld2(rk1 + 202);
+    const double2 A0_249 = ld2(rk0 + 202), A1_249 = ld2(rk1 + 202);
+    const double2 A0_250 = ld2(rk0 + 258), A1_250 = ld2(rk1 + 258);
+    { const double2 a0 = A0_249, a1 = A1_249;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c28;
       jc3 += 1.0 * dq;
       jc11 += -1.0 * dq;
@@ -5495,14 +5745,16 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 258), a1 = ld2(rk1 + 258);
+    { const double2 a0 = A0_250, a1 = A1_250;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c4;
       jc3 += -1.0 * dq;
       jc4 += -1.0 * dq;
       jc6 += 1.0 * dq;
       jc18 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 262), a1 = ld2(rk1 + 262);
+    const double2 A0_251 = ld2(rk0 + 262), A1_251 = ld2(rk1 + 262);
+    const double2 A0_252 = ld2(rk0 + 286), A1_252 = ld2(rk1 + 286);
+    { const double2 a0 = A0_251, a1 = A1_251;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc3 += -1.0 * dq;
       jc18 += 1.0 * dq;
@@ -5521,14 +5773,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 286), a1 = ld2(rk1 + 286);
+    { const double2 a0 = A0_252, a1 = A1_252;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc3 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 322), a1 = ld2(rk1 + 322);
+    const double2 A0_253 = ld2(rk0 + 322), A1_253 = ld2(rk1 + 322);
+    { const double2 a0 = A0_253, a1 = A1_253;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c23;
       jc3 += -1.0 * dq;
       jc18 += 1.0 * dq;
@@ -5668,7 +5921,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc6 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 82), a1 = ld2(rk1 + 82);
+    const double2 A0_254 = ld2(rk0 + 82), A1_254 = ld2(rk1 + 82);
+    { const double2 a0 = A0_254, a1 = A1_254;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc2 += -1.0 * dq;
       jc4 += 1.0 * dq;
@@ -5693,40 +5947,45 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc10 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
+    const double2 A0_255 = ld2(rk0 + 122), A1_255 = ld2(rk1 + 122);
+    const double2 A0_256 = ld2(rk0 + 124), A1_256 = ld2(rk1 + 124);
+    const double2 A0_257 = ld2(rk0 + 126), A1_257 = ld2(rk1 + 126);
     {
       const double dq = scr[35 * SST];
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 122), a1 = ld2(rk1 + 122);
+    { const double2 a0 = A0_255, a1 = A1_255;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc2 += 1.0 * dq;
       jc4 += -1.0 * dq;
       jc12 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 124), a1 = ld2(rk1 + 124);
+    { const double2 a0 = A0_256, a1 = A1_256;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc4 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 126), a1 = ld2(rk1 + 126);
+    { const double2 a0 = A0_257, a1 = A1_257;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc4 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 150), a1 = ld2(rk1 + 150);
+    const double2 A0_258 = ld2(rk0 + 150), A1_258 = ld2(rk1 + 150);
+    const double2 A0_259 = ld2(rk0 + 164), A1_259 = ld2(rk1 + 164);
+    { const double2 a0 = A0_258, a1 = A1_258;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
       jc26 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 164), a1 = ld2(rk1 + 164);
+    { const double2 a0 = A0_259, a1 = A1_259;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc4 += 1.0 * dq;
       jc19 += 1.0 * dq;
@@ -5743,6 +6002,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
+    const double2 A0_260 = ld2(rk0 + 224), A1_260 = ld2(rk1 + 224);
+    const double2 A0_261 = ld2(rk0 + 234), A1_261 = ld2(rk1 + 234);
     {
       const double dq = scr[45 * SST];
       jc2 += 1.0 * dq;
@@ -5755,20 +6016,21 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 224), a1 = ld2(rk1 + 224);
+    { const double2 a0 = A0_260, a1 = A1_260;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc4 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc17 += 2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 234), a1 = ld2(rk1 + 234);
+    { const double2 a0 = A0_261, a1 = A1_261;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc4 += -1.0 * dq;
       jc6 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 258), a1 = ld2(rk1 + 258);
+    const double2 A0_262 = ld2(rk0 + 258), A1_262 = ld2(rk1 + 258);
+    { const double2 a0 = A0_262, a1 = A1_262;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc4 += -1.0 * dq;
@@ -5965,13 +6227,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_263 = ld2(rk0 + 338), A1_263 = ld2(rk1 + 338);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 338), a1 = ld2(rk1 + 338);
+    { const double2 a0 = A0_263, a1 = A1_263;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc5 += -1.0 * dq;
       jc12 += 1.0 * dq;
@@ -6040,6 +6303,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc28 = 0.0;
     double jc29 = 0.0;
     double jc30 = 0.0;
+    const double2 A0_264 = ld2(rk0 + 32), A1_264 = ld2(rk1 + 32);
     {
       const double dq = scr[0 * SST];
       jc7 += -2.0 * dq;
@@ -6057,7 +6321,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 32), a1 = ld2(rk1 + 32);
+    { const double2 a0 = A0_264, a1 = A1_264;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc2 += 1.0 * dq;
       jc6 += -1.0 * dq;
@@ -6111,7 +6375,9 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc6 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 82), a1 = ld2(rk1 + 82);
+    const double2 A0_265 = ld2(rk0 + 82), A1_265 = ld2(rk1 + 82);
+    const double2 A0_266 = ld2(rk0 + 84), A1_266 = ld2(rk1 + 84);
+    { const double2 a0 = A0_265, a1 = A1_265;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc4 += 1.0 * dq;
@@ -6124,7 +6390,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc25 += -1.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 84), a1 = ld2(rk1 + 84);
+    { const double2 a0 = A0_266, a1 = A1_266;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc2 += -1.0 * dq;
       jc6 += 1.0 * dq;
@@ -6137,6 +6403,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
+    const double2 A0_267 = ld2(rk0 + 126), A1_267 = ld2(rk1 + 126);
+    const double2 A0_268 = ld2(rk0 + 128), A1_268 = ld2(rk1 + 128);
     {
       const double dq = scr[32 * SST];
       jc9 += -1.0 * dq;
@@ -6148,21 +6416,23 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 126), a1 = ld2(rk1 + 126);
+    { const double2 a0 = A0_267, a1 = A1_267;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc4 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 128), a1 = ld2(rk1 + 128);
+    { const double2 a0 = A0_268, a1 = A1_268;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc6 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc25 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 152), a1 = ld2(rk1 + 152);
+    const double2 A0_269 = ld2(rk0 + 152), A1_269 = ld2(rk1 + 152);
+    const double2 A0_270 = ld2(rk0 + 200), A1_270 = ld2(rk1 + 200);
+    { const double2 a0 = A0_269, a1 = A1_269;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc6 += 1.0 * dq;
@@ -6180,20 +6450,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 200), a1 = ld2(rk1 + 200);
+    { const double2 a0 = A0_270, a1 = A1_270;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc3 += 1.0 * dq;
       jc6 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += -1.0 * dq;
     }
+    const double2 A0_271 = ld2(rk0 + 210), A1_271 = ld2(rk1 + 210);
+    const double2 A0_272 = ld2(rk0 + 216), A1_272 = ld2(rk1 + 216);
     {
       const double dq = scr[45 * SST];
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 210), a1 = ld2(rk1 + 210);
+    { const double2 a0 = A0_271, a1 = A1_271;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc6 += -1.0 * dq;
       jc10 += 1.0 * dq;
@@ -6206,27 +6478,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 216), a1 = ld2(rk1 + 216);
+    { const double2 a0 = A0_272, a1 = A1_272;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c28;
       jc6 += -1.0 * dq;
       jc25 += 2.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 222), a1 = ld2(rk1 + 222);
+    const double2 A0_273 = ld2(rk0 + 222), A1_273 = ld2(rk1 + 222);
+    const double2 A0_274 = ld2(rk0 + 232), A1_274 = ld2(rk1 + 232);
+    const double2 A0_275 = ld2(rk0 + 256), A1_275 = ld2(rk1 + 256);
+    { const double2 a0 = A0_273, a1 = A1_273;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc6 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc25 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 232), a1 = ld2(rk1 + 232);
+    { const double2 a0 = A0_274, a1 = A1_274;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc4 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 256), a1 = ld2(rk1 + 256);
+    { const double2 a0 = A0_275, a1 = A1_275;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc4 += 1.0 * dq;
@@ -6239,20 +6514,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_276 = ld2(rk0 + 270), A1_276 = ld2(rk1 + 270);
+    const double2 A0_277 = ld2(rk0 + 274), A1_277 = ld2(rk1 + 274);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 270), a1 = ld2(rk1 + 270);
+    { const double2 a0 = A0_276, a1 = A1_276;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc6 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 274), a1 = ld2(rk1 + 274);
+    { const double2 a0 = A0_277, a1 = A1_277;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc6 += -1.0 * dq;
       jc8 += -1.0 * dq;
@@ -6324,12 +6601,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc28 = 0.0;
     double jc29 = 0.0;
     double jc30 = 0.0;
-    { const double2 a0 = ld2(rk0 + 0), a1 = ld2(rk1 + 0);
+    const double2 A0_278 = ld2(rk0 + 0), A1_278 = ld2(rk1 + 0);
+    const double2 A0_279 = ld2(rk0 + 2), A1_279 = ld2(rk1 + 2);
+    { const double2 a0 = A0_278, a1 = A1_278;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M0 * 2.0*c7 + scr[0 * SST];
       jc7 += -2.0 * dq;
       jc15 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 2), a1 = ld2(rk1 + 2);
+    { const double2 a0 = A0_279, a1 = A1_279;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M1 * c2 + scr[1 * SST];
       jc2 += -1.0 * dq;
       jc7 += -1.0 * dq;
@@ -6377,6 +6656,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc16 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
+    const double2 A0_280 = ld2(rk0 + 22), A1_280 = ld2(rk1 + 22);
     {
       const double dq = scr[65 * SST] * c16;
       jc2 += 1.0 * dq;
@@ -6398,7 +6678,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 22), a1 = ld2(rk1 + 22);
+    { const double2 a0 = A0_280, a1 = A1_280;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M2 * c9 + scr[2 * SST];
       jc7 += -1.0 * dq;
       jc9 += -1.0 * dq;
@@ -6432,42 +6712,49 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += 1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 32), a1 = ld2(rk1 + 32);
+    const double2 A0_281 = ld2(rk0 + 32), A1_281 = ld2(rk1 + 32);
+    const double2 A0_282 = ld2(rk0 + 34), A1_282 = ld2(rk1 + 34);
+    const double2 A0_283 = ld2(rk0 + 36), A1_283 = ld2(rk1 + 36);
+    const double2 A0_284 = ld2(rk0 + 38), A1_284 = ld2(rk1 + 38);
+    { const double2 a0 = A0_281, a1 = A1_281;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc2 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc12 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 34), a1 = ld2(rk1 + 34);
+    { const double2 a0 = A0_282, a1 = A1_282;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc7 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 36), a1 = ld2(rk1 + 36);
+    { const double2 a0 = A0_283, a1 = A1_283;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c28;
       jc7 += -1.0 * dq;
       jc10 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 38), a1 = ld2(rk1 + 38);
+    { const double2 a0 = A0_284, a1 = A1_284;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc7 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc27 += 1.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 40), a1 = ld2(rk1 + 40);
+    const double2 A0_285 = ld2(rk0 + 40), A1_285 = ld2(rk1 + 40);
+    const double2 A0_286 = ld2(rk0 + 42), A1_286 = ld2(rk1 + 42);
+    const double2 A0_287 = ld2(rk0 + 52), A1_287 = ld2(rk1 + 52);
+    { const double2 a0 = A0_285, a1 = A1_285;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc7 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc29 += 1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 42), a1 = ld2(rk1 + 42);
+    { const double2 a0 = A0_286, a1 = A1_286;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c30;
       jc7 += 1.0 * dq;
       jc9 += -1.0 * dq;
@@ -6480,13 +6767,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 52), a1 = ld2(rk1 + 52);
+    { const double2 a0 = A0_287, a1 = A1_287;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
       jc2 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
+    const double2 A0_288 = ld2(rk0 + 64), A1_288 = ld2(rk1 + 64);
     {
       const double dq = scr[6 * SST];
       jc2 += -2.0 * dq;
@@ -6498,7 +6786,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 64), a1 = ld2(rk1 + 64);
+    { const double2 a0 = A0_288, a1 = A1_288;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc2 += -1.0 * dq;
       jc7 += 1.0 * dq;
@@ -6558,13 +6846,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 94), a1 = ld2(rk1 + 94);
+    const double2 A0_289 = ld2(rk0 + 94), A1_289 = ld2(rk1 + 94);
+    const double2 A0_290 = ld2(rk0 + 144), A1_290 = ld2(rk1 + 144);
+    { const double2 a0 = A0_289, a1 = A1_289;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc7 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 144), a1 = ld2(rk1 + 144);
+    { const double2 a0 = A0_290, a1 = A1_290;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
       jc7 += 1.0 * dq;
       jc15 += -1.0 * dq;
@@ -6582,6 +6872,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
+    const double2 A0_291 = ld2(rk0 + 220), A1_291 = ld2(rk1 + 220);
+    const double2 A0_292 = ld2(rk0 + 236), A1_292 = ld2(rk1 + 236);
     {
       const double dq = scr[45 * SST];
       jc2 += 1.0 * dq;
@@ -6594,19 +6886,21 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 220), a1 = ld2(rk1 + 220);
+    { const double2 a0 = A0_291, a1 = A1_291;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc6 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc25 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 236), a1 = ld2(rk1 + 236);
+    { const double2 a0 = A0_292, a1 = A1_292;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc7 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc17 += 2.0 * dq;
     }
+    const double2 A0_293 = ld2(rk0 + 280), A1_293 = ld2(rk1 + 280);
+    const double2 A0_294 = ld2(rk0 + 288), A1_294 = ld2(rk1 + 288);
     {
       const double dq = scr[49 * SST];
       jc9 += 1.0 * dq;
@@ -6619,21 +6913,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 280), a1 = ld2(rk1 + 280);
+    { const double2 a0 = A0_293, a1 = A1_293;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c20;
       jc7 += -1.0 * dq;
       jc12 += 1.0 * dq;
       jc20 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 288), a1 = ld2(rk1 + 288);
+    { const double2 a0 = A0_294, a1 = A1_294;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc1 += 1.0 * dq;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 330), a1 = ld2(rk1 + 330);
+    const double2 A0_295 = ld2(rk0 + 330), A1_295 = ld2(rk1 + 330);
+    { const double2 a0 = A0_295, a1 = A1_295;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c21;
       jc6 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -6728,6 +7023,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
+    const double2 A0_296 = ld2(rk0 + 70), A1_296 = ld2(rk1 + 70);
     {
       const double dq = scr[3 * SST];
       jc2 += -1.0 * dq;
@@ -6745,14 +7041,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 70), a1 = ld2(rk1 + 70);
+    { const double2 a0 = A0_296, a1 = A1_296;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 72), a1 = ld2(rk1 + 72);
+    const double2 A0_297 = ld2(rk0 + 72), A1_297 = ld2(rk1 + 72);
+    { const double2 a0 = A0_297, a1 = A1_297;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc8 += -1.0 * dq;
@@ -6801,6 +7098,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
+    const double2 A0_298 = ld2(rk0 + 98), A1_298 = ld2(rk1 + 98);
+    const double2 A0_299 = ld2(rk0 + 100), A1_299 = ld2(rk1 + 100);
     {
       const double dq = scr[32 * SST];
       jc9 += -1.0 * dq;
@@ -6812,33 +7111,36 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 98), a1 = ld2(rk1 + 98);
+    { const double2 a0 = A0_298, a1 = A1_298;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc8 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc24 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 100), a1 = ld2(rk1 + 100);
+    { const double2 a0 = A0_299, a1 = A1_299;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc8 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc24 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 132), a1 = ld2(rk1 + 132);
+    const double2 A0_300 = ld2(rk0 + 132), A1_300 = ld2(rk1 + 132);
+    const double2 A0_301 = ld2(rk0 + 134), A1_301 = ld2(rk1 + 134);
+    const double2 A0_302 = ld2(rk0 + 142), A1_302 = ld2(rk1 + 142);
+    { const double2 a0 = A0_300, a1 = A1_300;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c15;
       jc8 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 134), a1 = ld2(rk1 + 134);
+    { const double2 a0 = A0_301, a1 = A1_301;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c15;
       jc8 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 142), a1 = ld2(rk1 + 142);
+    { const double2 a0 = A0_302, a1 = A1_302;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
       jc8 += 1.0 * dq;
       jc10 += -1.0 * dq;
@@ -6851,7 +7153,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 192), a1 = ld2(rk1 + 192);
+    const double2 A0_303 = ld2(rk0 + 192), A1_303 = ld2(rk1 + 192);
+    { const double2 a0 = A0_303, a1 = A1_303;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -6875,7 +7178,9 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 250), a1 = ld2(rk1 + 250);
+    const double2 A0_304 = ld2(rk0 + 250), A1_304 = ld2(rk1 + 250);
+    const double2 A0_305 = ld2(rk0 + 274), A1_305 = ld2(rk1 + 274);
+    { const double2 a0 = A0_304, a1 = A1_304;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c28;
       jc8 += -1.0 * dq;
       jc11 += 1.0 * dq;
@@ -6894,14 +7199,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 274), a1 = ld2(rk1 + 274);
+    { const double2 a0 = A0_305, a1 = A1_305;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc6 += -1.0 * dq;
       jc8 += -1.0 * dq;
       jc24 += 1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 318), a1 = ld2(rk1 + 318);
+    const double2 A0_306 = ld2(rk0 + 318), A1_306 = ld2(rk1 + 318);
+    { const double2 a0 = A0_306, a1 = A1_306;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c23;
       jc8 += -1.0 * dq;
       jc22 += 1.0 * dq;
@@ -6998,7 +7304,9 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc16 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 22), a1 = ld2(rk1 + 22);
+    const double2 A0_307 = ld2(rk0 + 22), A1_307 = ld2(rk1 + 22);
+    const double2 A0_308 = ld2(rk0 + 42), A1_308 = ld2(rk1 + 42);
+    { const double2 a0 = A0_307, a1 = A1_307;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M2 * c7 + 1.5 * scr[2 * SST];
       jc7 += -1.0 * dq;
       jc9 += -1.0 * dq;
@@ -7018,7 +7326,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += 1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 42), a1 = ld2(rk1 + 42);
+    { const double2 a0 = A0_308, a1 = A1_308;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc7 += 1.0 * dq;
       jc9 += -1.0 * dq;
@@ -7048,6 +7356,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
+    const double2 A0_309 = ld2(rk0 + 78), A1_309 = ld2(rk1 + 78);
     {
       const double dq = 1.5 * scr[14 * SST];
       jc2 += -1.0 * dq;
@@ -7060,7 +7369,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc10 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 78), a1 = ld2(rk1 + 78);
+    { const double2 a0 = A0_309, a1 = A1_309;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -7073,6 +7382,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += -1.0 * dq;
       jc6 += 1.0 * dq;
     }
+    const double2 A0_310 = ld2(rk0 + 90), A1_310 = ld2(rk1 + 90);
     {
       const double dq = 1.5 * scr[23 * SST];
       jc2 += -1.0 * dq;
@@ -7091,13 +7401,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 90), a1 = ld2(rk1 + 90);
+    { const double2 a0 = A0_310, a1 = A1_310;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc12 += -1.0 * dq;
       jc18 += 1.0 * dq;
     }
+    const double2 A0_311 = ld2(rk0 + 116), A1_311 = ld2(rk1 + 116);
+    const double2 A0_312 = ld2(rk0 + 118), A1_312 = ld2(rk1 + 118);
     {
       const double dq = scr[33 * SST] * c19 + 1.5 * scr[32 * SST];
       jc9 += -1.0 * dq;
@@ -7109,35 +7421,38 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 116), a1 = ld2(rk1 + 116);
+    { const double2 a0 = A0_311, a1 = A1_311;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc2 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc27 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 118), a1 = ld2(rk1 + 118);
+    { const double2 a0 = A0_312, a1 = A1_312;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc9 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 124), a1 = ld2(rk1 + 124);
+    const double2 A0_313 = ld2(rk0 + 124), A1_313 = ld2(rk1 + 124);
+    const double2 A0_314 = ld2(rk0 + 140), A1_314 = ld2(rk1 + 140);
+    const double2 A0_315 = ld2(rk0 + 156), A1_315 = ld2(rk1 + 156);
+    { const double2 a0 = A0_313, a1 = A1_313;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc4 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 140), a1 = ld2(rk1 + 140);
+    { const double2 a0 = A0_314, a1 = A1_314;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc9 += -1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 156), a1 = ld2(rk1 + 156);
+    { const double2 a0 = A0_315, a1 = A1_315;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
       jc9 += 1.0 * dq;
       jc17 += 1.0 * dq;
@@ -7150,6 +7465,9 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
+    const double2 A0_316 = ld2(rk0 + 172), A1_316 = ld2(rk1 + 172);
+    const double2 A0_317 = ld2(rk0 + 182), A1_317 = ld2(rk1 + 182);
+    const double2 A0_318 = ld2(rk0 + 186), A1_318 = ld2(rk1 + 186);
     {
       const double dq = -scr[52 * SST] * c2*c27;
       jc2 += 1.0 * dq;
@@ -7158,51 +7476,56 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc16 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 172), a1 = ld2(rk1 + 172);
+    { const double2 a0 = A0_316, a1 = A1_316;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc9 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc16 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 182), a1 = ld2(rk1 + 182);
+    { const double2 a0 = A0_317, a1 = A1_317;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16 - (a0.y + (a1.y - a0.y) * Td) * c29;
       jc9 += 0.0 * dq;
       jc16 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 186), a1 = ld2(rk1 + 186);
+    { const double2 a0 = A0_318, a1 = A1_318;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c10;
       jc9 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc30 += -1.0 * dq;
     }
+    const double2 A0_319 = ld2(rk0 + 196), A1_319 = ld2(rk1 + 196);
+    const double2 A0_320 = ld2(rk0 + 204), A1_320 = ld2(rk1 + 204);
+    const double2 A0_321 = ld2(rk0 + 206), A1_321 = ld2(rk1 + 206);
     {
       const double dq = 1.5 * scr[41 * SST];
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 196), a1 = ld2(rk1 + 196);
+    { const double2 a0 = A0_319, a1 = A1_319;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc3 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 204), a1 = ld2(rk1 + 204);
+    { const double2 a0 = A0_320, a1 = A1_320;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M5 * c2;
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 206), a1 = ld2(rk1 + 206);
+    { const double2 a0 = A0_321, a1 = A1_321;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M10 * c2 + 1.5 * scr[45 * SST];
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 208), a1 = ld2(rk1 + 208);
+    const double2 A0_322 = ld2(rk0 + 208), A1_322 = ld2(rk1 + 208);
+    const double2 A0_323 = ld2(rk0 + 230), A1_323 = ld2(rk1 + 230);
+    { const double2 a0 = A0_322, a1 = A1_322;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c24;
       jc9 += 1.0 * dq;
       jc15 += -1.0 * dq;
@@ -7215,7 +7538,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 230), a1 = ld2(rk1 + 230);
+    { const double2 a0 = A0_323, a1 = A1_323;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc9 += -1.0 * dq;
       jc15 += 1.0 * dq;
@@ -7228,13 +7551,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 266), a1 = ld2(rk1 + 266);
+    const double2 A0_324 = ld2(rk0 + 266), A1_324 = ld2(rk1 + 266);
+    const double2 A0_325 = ld2(rk0 + 364), A1_325 = ld2(rk1 + 364);
+    { const double2 a0 = A0_324, a1 = A1_324;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M11 * c18 + scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 364), a1 = ld2(rk1 + 364);
+    { const double2 a0 = A0_325, a1 = A1_325;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c13;
       jc0 += 1.0 * dq;
       jc9 += -1.0 * dq;
@@ -7328,6 +7653,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
+    const double2 A0_326 = ld2(rk0 + 36), A1_326 = ld2(rk1 + 36);
+    const double2 A0_327 = ld2(rk0 + 44), A1_327 = ld2(rk1 + 44);
     {
       const double dq = scr[75 * SST] * c7;
       jc7 += -1.0 * dq;
@@ -7335,14 +7662,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc17 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 36), a1 = ld2(rk1 + 36);
+    { const double2 a0 = A0_326, a1 = A1_326;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc7 += -1.0 * dq;
       jc10 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 44), a1 = ld2(rk1 + 44);
+    { const double2 a0 = A0_327, a1 = A1_327;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc10 += -1.0 * dq;
       jc15 += -1.0 * dq;
@@ -7378,13 +7705,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc3 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
+    const double2 A0_328 = ld2(rk0 + 80), A1_328 = ld2(rk1 + 80);
     {
       const double dq = -scr[19 * SST] * 1.0 + scr[17 * SST];
       jc2 += -1.0 * dq;
       jc10 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 80), a1 = ld2(rk1 + 80);
+    { const double2 a0 = A0_328, a1 = A1_328;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc10 += -1.0 * dq;
@@ -7426,49 +7754,56 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 104), a1 = ld2(rk1 + 104);
+    const double2 A0_329 = ld2(rk0 + 104), A1_329 = ld2(rk1 + 104);
+    const double2 A0_330 = ld2(rk0 + 108), A1_330 = ld2(rk1 + 108);
+    const double2 A0_331 = ld2(rk0 + 120), A1_331 = ld2(rk1 + 120);
+    const double2 A0_332 = ld2(rk0 + 136), A1_332 = ld2(rk1 + 136);
+    { const double2 a0 = A0_329, a1 = A1_329;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 108), a1 = ld2(rk1 + 108);
+    { const double2 a0 = A0_330, a1 = A1_330;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 120), a1 = ld2(rk1 + 120);
+    { const double2 a0 = A0_331, a1 = A1_331;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc10 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc17 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 136), a1 = ld2(rk1 + 136);
+    { const double2 a0 = A0_332, a1 = A1_332;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
       jc10 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 142), a1 = ld2(rk1 + 142);
+    const double2 A0_333 = ld2(rk0 + 142), A1_333 = ld2(rk1 + 142);
+    const double2 A0_334 = ld2(rk0 + 148), A1_334 = ld2(rk1 + 148);
+    const double2 A0_335 = ld2(rk0 + 158), A1_335 = ld2(rk1 + 158);
+    { const double2 a0 = A0_333, a1 = A1_333;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc8 += 1.0 * dq;
       jc10 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 148), a1 = ld2(rk1 + 148);
+    { const double2 a0 = A0_334, a1 = A1_334;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc14 += -1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 158), a1 = ld2(rk1 + 158);
+    { const double2 a0 = A0_335, a1 = A1_335;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
       jc2 += 1.0 * dq;
       jc10 += -1.0 * dq;
@@ -7481,14 +7816,17 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 186), a1 = ld2(rk1 + 186);
+    const double2 A0_336 = ld2(rk0 + 186), A1_336 = ld2(rk1 + 186);
+    const double2 A0_337 = ld2(rk0 + 190), A1_337 = ld2(rk1 + 190);
+    const double2 A0_338 = ld2(rk0 + 198), A1_338 = ld2(rk1 + 198);
+    { const double2 a0 = A0_336, a1 = A1_336;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc9 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc30 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 190), a1 = ld2(rk1 + 190);
+    { const double2 a0 = A0_337, a1 = A1_337;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
       jc10 += 1.0 * dq;
       jc15 += -1.0 * dq;
@@ -7500,20 +7838,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 198), a1 = ld2(rk1 + 198);
+    { const double2 a0 = A0_338, a1 = A1_338;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc10 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
+    const double2 A0_339 = ld2(rk0 + 210), A1_339 = ld2(rk1 + 210);
+    const double2 A0_340 = ld2(rk0 + 214), A1_340 = ld2(rk1 + 214);
     {
       const double dq = scr[45 * SST];
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 210), a1 = ld2(rk1 + 210);
+    { const double2 a0 = A0_339, a1 = A1_339;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
       jc6 += -1.0 * dq;
       jc10 += 1.0 * dq;
@@ -7526,14 +7866,16 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 214), a1 = ld2(rk1 + 214);
+    { const double2 a0 = A0_340, a1 = A1_340;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc10 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 242), a1 = ld2(rk1 + 242);
+    const double2 A0_341 = ld2(rk0 + 242), A1_341 = ld2(rk1 + 242);
+    const double2 A0_342 = ld2(rk0 + 302), A1_342 = ld2(rk1 + 302);
+    { const double2 a0 = A0_341, a1 = A1_341;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c17;
       jc10 += -1.0 * dq;
       jc12 += 1.0 * dq;
@@ -7552,14 +7894,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 302), a1 = ld2(rk1 + 302);
+    { const double2 a0 = A0_342, a1 = A1_342;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc10 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 326), a1 = ld2(rk1 + 326);
+    const double2 A0_343 = ld2(rk0 + 326), A1_343 = ld2(rk1 + 326);
+    { const double2 a0 = A0_343, a1 = A1_343;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc10 += -1.0 * dq;
       jc21 += 1.0 * dq;
@@ -7628,6 +7971,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc28 = 0.0;
     double jc29 = 0.0;
     double jc30 = 0.0;
+    const double2 A0_344 = ld2(rk0 + 38), A1_344 = ld2(rk1 + 38);
     {
       const double dq = 3.0 * scr[0 * SST];
       jc7 += -2.0 * dq;
@@ -7645,7 +7989,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 38), a1 = ld2(rk1 + 38);
+    { const double2 a0 = A0_344, a1 = A1_344;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc7 += -1.0 * dq;
       jc11 += -1.0 * dq;
@@ -7699,6 +8043,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc6 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
+    const double2 A0_345 = ld2(rk0 + 88), A1_345 = ld2(rk1 + 88);
     {
       const double dq = 3.0 * scr[26 * SST];
       jc2 += -1.0 * dq;
@@ -7711,7 +8056,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 88), a1 = ld2(rk1 + 88);
+    { const double2 a0 = A0_345, a1 = A1_345;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc11 += -1.0 * dq;
@@ -7724,12 +8069,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc10 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
+    const double2 A0_346 = ld2(rk0 + 130), A1_346 = ld2(rk1 + 130);
+    const double2 A0_347 = ld2(rk0 + 188), A1_347 = ld2(rk1 + 188);
     {
       const double dq = 3.0 * scr[35 * SST];
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 130), a1 = ld2(rk1 + 130);
+    { const double2 a0 = A0_346, a1 = A1_346;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc11 += -1.0 * dq;
       jc14 += 1.0 * dq;
@@ -7742,19 +8089,20 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 188), a1 = ld2(rk1 + 188);
+    { const double2 a0 = A0_347, a1 = A1_347;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
       jc11 += -1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc28 += 1.0 * dq;
     }
+    const double2 A0_348 = ld2(rk0 + 202), A1_348 = ld2(rk1 + 202);
     {
       const double dq = -scr[43 * SST] * 1.0 + 3.0 * scr[41 * SST];
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 202), a1 = ld2(rk1 + 202);
+    { const double2 a0 = A0_348, a1 = A1_348;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc11 += -1.0 * dq;
@@ -7773,21 +8121,24 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 244), a1 = ld2(rk1 + 244);
+    const double2 A0_349 = ld2(rk0 + 244), A1_349 = ld2(rk1 + 244);
+    const double2 A0_350 = ld2(rk0 + 248), A1_350 = ld2(rk1 + 248);
+    const double2 A0_351 = ld2(rk0 + 252), A1_351 = ld2(rk1 + 252);
+    { const double2 a0 = A0_349, a1 = A1_349;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc11 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 248), a1 = ld2(rk1 + 248);
+    { const double2 a0 = A0_350, a1 = A1_350;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc8 += 1.0 * dq;
       jc11 += -1.0 * dq;
       jc24 += -1.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 252), a1 = ld2(rk1 + 252);
+    { const double2 a0 = A0_351, a1 = A1_351;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc11 += -1.0 * dq;
       jc25 += -1.0 * dq;
@@ -7799,20 +8150,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_352 = ld2(rk0 + 292), A1_352 = ld2(rk1 + 292);
+    const double2 A0_353 = ld2(rk0 + 296), A1_353 = ld2(rk1 + 296);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 292), a1 = ld2(rk1 + 292);
+    { const double2 a0 = A0_352, a1 = A1_352;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
       jc11 += -1.0 * dq;
       jc26 += -1.0 * dq;
       jc28 += 1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 296), a1 = ld2(rk1 + 296);
+    { const double2 a0 = A0_353, a1 = A1_353;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c29;
       jc11 += -1.0 * dq;
       jc18 += 1.0 * dq;
@@ -7883,6 +8236,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc28 = 0.0;
     double jc29 = 0.0;
     double jc30 = 0.0;
+    const double2 A0_354 = ld2(rk0 + 32), A1_354 = ld2(rk1 + 32);
     {
       const double dq = scr[0 * SST];
       jc7 += -2.0 * dq;
@@ -7900,14 +8254,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 32), a1 = ld2(rk1 + 32);
+    { const double2 a0 = A0_354, a1 = A1_354;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc7 += -1.0 * dq;
       jc12 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 40), a1 = ld2(rk1 + 40);
+    const double2 A0_355 = ld2(rk0 + 40), A1_355 = ld2(rk1 + 40);
+    { const double2 a0 = A0_355, a1 = A1_355;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc7 += -1.0 * dq;
       jc12 += -1.0 * dq;
@@ -7955,6 +8310,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += -1.0 * dq;
       jc6 += 1.0 * dq;
     }
+    const double2 A0_356 = ld2(rk0 + 90), A1_356 = ld2(rk1 + 90);
     {
       const double dq = scr[23 * SST];
       jc2 += -1.0 * dq;
@@ -7973,13 +8329,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 90), a1 = ld2(rk1 + 90);
+    { const double2 a0 = A0_356, a1 = A1_356;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc12 += -1.0 * dq;
       jc18 += 1.0 * dq;
     }
+    const double2 A0_357 = ld2(rk0 + 122), A1_357 = ld2(rk1 + 122);
+    const double2 A0_358 = ld2(rk0 + 158), A1_358 = ld2(rk1 + 158);
     {
       const double dq = scr[32 * SST];
       jc9 += -1.0 * dq;
@@ -7991,14 +8349,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 122), a1 = ld2(rk1 + 122);
+    { const double2 a0 = A0_357, a1 = A1_357;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc4 += -1.0 * dq;
       jc12 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 158), a1 = ld2(rk1 + 158);
+    { const double2 a0 = A0_358, a1 = A1_358;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc10 += -1.0 * dq;
@@ -8028,13 +8386,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 236), a1 = ld2(rk1 + 236);
+    const double2 A0_359 = ld2(rk0 + 236), A1_359 = ld2(rk1 + 236);
+    const double2 A0_360 = ld2(rk0 + 240), A1_360 = ld2(rk1 + 240);
+    { const double2 a0 = A0_359, a1 = A1_359;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc7 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc17 += 2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 240), a1 = ld2(rk1 + 240);
+    { const double2 a0 = A0_360, a1 = A1_360;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc10 += 1.0 * dq;
       jc12 += -1.0 * dq;
@@ -8053,41 +8413,47 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 282), a1 = ld2(rk1 + 282);
+    const double2 A0_361 = ld2(rk0 + 282), A1_361 = ld2(rk1 + 282);
+    const double2 A0_362 = ld2(rk0 + 286), A1_362 = ld2(rk1 + 286);
+    const double2 A0_363 = ld2(rk0 + 340), A1_363 = ld2(rk1 + 340);
+    const double2 A0_364 = ld2(rk0 + 352), A1_364 = ld2(rk1 + 352);
+    { const double2 a0 = A0_361, a1 = A1_361;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc7 += 1.0 * dq;
       jc12 += -1.0 * dq;
       jc20 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 286), a1 = ld2(rk1 + 286);
+    { const double2 a0 = A0_362, a1 = A1_362;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 340), a1 = ld2(rk1 + 340);
+    { const double2 a0 = A0_363, a1 = A1_363;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c17;
       jc5 += 1.0 * dq;
       jc12 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 352), a1 = ld2(rk1 + 352);
+    { const double2 a0 = A0_364, a1 = A1_364;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc1 += 1.0 * dq;
       jc2 += -1.0 * dq;
       jc12 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 356), a1 = ld2(rk1 + 356);
+    const double2 A0_365 = ld2(rk0 + 356), A1_365 = ld2(rk1 + 356);
+    const double2 A0_366 = ld2(rk0 + 358), A1_366 = ld2(rk1 + 358);
+    { const double2 a0 = A0_365, a1 = A1_365;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc1 += 1.0 * dq;
       jc12 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 358), a1 = ld2(rk1 + 358);
+    { const double2 a0 = A0_366, a1 = A1_366;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc12 += -1.0 * dq;
       jc18 += 1.0 * dq;
@@ -8273,19 +8639,21 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_367 = ld2(rk0 + 276), A1_367 = ld2(rk1 + 276);
+    const double2 A0_368 = ld2(rk0 + 364), A1_368 = ld2(rk1 + 364);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 276), a1 = ld2(rk1 + 276);
+    { const double2 a0 = A0_367, a1 = A1_367;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 1.0;
       jc13 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 364), a1 = ld2(rk1 + 364);
+    { const double2 a0 = A0_368, a1 = A1_368;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
       jc0 += 1.0 * dq;
       jc9 += -1.0 * dq;
@@ -8379,25 +8747,28 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
+    const double2 A0_369 = ld2(rk0 + 62), A1_369 = ld2(rk1 + 62);
+    const double2 A0_370 = ld2(rk0 + 64), A1_370 = ld2(rk1 + 64);
+    const double2 A0_371 = ld2(rk0 + 72), A1_371 = ld2(rk1 + 72);
     {
       const double dq = scr[8 * SST];
       jc2 += -2.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 62), a1 = ld2(rk1 + 62);
+    { const double2 a0 = A0_369, a1 = A1_369;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M9 * 1.0 + 3.6500000953674316 * scr[10 * SST];
       jc2 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 64), a1 = ld2(rk1 + 64);
+    { const double2 a0 = A0_370, a1 = A1_370;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c7;
       jc2 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 72), a1 = ld2(rk1 + 72);
+    { const double2 a0 = A0_371, a1 = A1_371;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
       jc2 += -1.0 * dq;
       jc8 += -1.0 * dq;
@@ -8452,7 +8823,10 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc10 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 92), a1 = ld2(rk1 + 92);
+    const double2 A0_372 = ld2(rk0 + 92), A1_372 = ld2(rk1 + 92);
+    const double2 A0_373 = ld2(rk0 + 94), A1_373 = ld2(rk1 + 94);
+    const double2 A0_374 = ld2(rk0 + 96), A1_374 = ld2(rk1 + 96);
+    { const double2 a0 = A0_372, a1 = A1_372;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc14 += 1.0 * dq;
@@ -8464,117 +8838,131 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 94), a1 = ld2(rk1 + 94);
+    { const double2 a0 = A0_373, a1 = A1_373;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c7;
       jc7 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 96), a1 = ld2(rk1 + 96);
+    { const double2 a0 = A0_374, a1 = A1_374;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c15;
       jc14 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 98), a1 = ld2(rk1 + 98);
+    const double2 A0_375 = ld2(rk0 + 98), A1_375 = ld2(rk1 + 98);
+    const double2 A0_376 = ld2(rk0 + 100), A1_376 = ld2(rk1 + 100);
+    const double2 A0_377 = ld2(rk0 + 106), A1_377 = ld2(rk1 + 106);
+    const double2 A0_378 = ld2(rk0 + 110), A1_378 = ld2(rk1 + 110);
+    { const double2 a0 = A0_375, a1 = A1_375;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c24;
       jc8 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc24 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 100), a1 = ld2(rk1 + 100);
+    { const double2 a0 = A0_376, a1 = A1_376;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c24;
       jc8 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc24 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 106), a1 = ld2(rk1 + 106);
+    { const double2 a0 = A0_377, a1 = A1_377;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c26;
       jc14 += 1.0 * dq;
       jc26 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 110), a1 = ld2(rk1 + 110);
+    { const double2 a0 = A0_378, a1 = A1_378;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c29;
       jc14 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 112), a1 = ld2(rk1 + 112);
+    const double2 A0_379 = ld2(rk0 + 112), A1_379 = ld2(rk1 + 112);
+    const double2 A0_380 = ld2(rk0 + 114), A1_380 = ld2(rk1 + 114);
+    const double2 A0_381 = ld2(rk0 + 118), A1_381 = ld2(rk1 + 118);
+    const double2 A0_382 = ld2(rk0 + 120), A1_382 = ld2(rk1 + 120);
+    { const double2 a0 = A0_379, a1 = A1_379;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c16;
       jc14 += 1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 114), a1 = ld2(rk1 + 114);
+    { const double2 a0 = A0_380, a1 = A1_380;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc3 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 118), a1 = ld2(rk1 + 118);
+    { const double2 a0 = A0_381, a1 = A1_381;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc9 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 120), a1 = ld2(rk1 + 120);
+    { const double2 a0 = A0_382, a1 = A1_382;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
       jc10 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc17 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 126), a1 = ld2(rk1 + 126);
+    const double2 A0_383 = ld2(rk0 + 126), A1_383 = ld2(rk1 + 126);
+    const double2 A0_384 = ld2(rk0 + 128), A1_384 = ld2(rk1 + 128);
+    const double2 A0_385 = ld2(rk0 + 130), A1_385 = ld2(rk1 + 130);
+    const double2 A0_386 = ld2(rk0 + 148), A1_386 = ld2(rk1 + 148);
+    { const double2 a0 = A0_383, a1 = A1_383;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc4 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 128), a1 = ld2(rk1 + 128);
+    { const double2 a0 = A0_384, a1 = A1_384;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c6;
       jc6 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc25 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 130), a1 = ld2(rk1 + 130);
+    { const double2 a0 = A0_385, a1 = A1_385;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c28;
       jc11 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 148), a1 = ld2(rk1 + 148);
+    { const double2 a0 = A0_386, a1 = A1_386;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc14 += -1.0 * dq;
       jc26 += -1.0 * dq;
     }
+    const double2 A0_387 = ld2(rk0 + 172), A1_387 = ld2(rk1 + 172);
+    const double2 A0_388 = ld2(rk0 + 176), A1_388 = ld2(rk1 + 176);
     {
       const double dq = 6.0 * scr[38 * SST];
       jc9 += -1.0 * dq;
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 172), a1 = ld2(rk1 + 172);
+    { const double2 a0 = A0_387, a1 = A1_387;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc9 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc16 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 176), a1 = ld2(rk1 + 176);
+    { const double2 a0 = A0_388, a1 = A1_388;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16 - (a0.y + (a1.y - a0.y) * Td) * c29;
       jc14 += 0.0 * dq;
       jc16 += -1.0 * dq;
@@ -8609,7 +8997,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 310), a1 = ld2(rk1 + 310);
+    const double2 A0_389 = ld2(rk0 + 310), A1_389 = ld2(rk1 + 310);
+    { const double2 a0 = A0_389, a1 = A1_389;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c23;
       jc14 += -1.0 * dq;
       jc22 += 1.0 * dq;
@@ -8681,7 +9070,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc28 = 0.0;
     double jc29 = 0.0;
     double jc30 = 0.0;
-    { const double2 a0 = ld2(rk0 + 0), a1 = ld2(rk1 + 0);
+    const double2 A0_390 = ld2(rk0 + 0), A1_390 = ld2(rk1 + 0);
+    { const double2 a0 = A0_390, a1 = A1_390;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M0 * 1.0 + scr[0 * SST];
       jc7 += -2.0 * dq;
       jc15 += 1.0 * dq;
@@ -8705,39 +9095,45 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 42), a1 = ld2(rk1 + 42);
+    const double2 A0_391 = ld2(rk0 + 42), A1_391 = ld2(rk1 + 42);
+    const double2 A0_392 = ld2(rk0 + 44), A1_392 = ld2(rk1 + 44);
+    const double2 A0_393 = ld2(rk0 + 46), A1_393 = ld2(rk1 + 46);
+    const double2 A0_394 = ld2(rk0 + 48), A1_394 = ld2(rk1 + 48);
+    { const double2 a0 = A0_391, a1 = A1_391;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
       jc7 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 44), a1 = ld2(rk1 + 44);
+    { const double2 a0 = A0_392, a1 = A1_392;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c10;
       jc10 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 46), a1 = ld2(rk1 + 46);
+    { const double2 a0 = A0_393, a1 = A1_393;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M3 * c2;
       jc2 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 48), a1 = ld2(rk1 + 48);
+    { const double2 a0 = A0_394, a1 = A1_394;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M4 * c2 + scr[4 * SST];
       jc2 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 50), a1 = ld2(rk1 + 50);
+    const double2 A0_395 = ld2(rk0 + 50), A1_395 = ld2(rk1 + 50);
+    const double2 A0_396 = ld2(rk0 + 52), A1_396 = ld2(rk1 + 52);
+    { const double2 a0 = A0_395, a1 = A1_395;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M5 * c2;
       jc2 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 52), a1 = ld2(rk1 + 52);
+    { const double2 a0 = A0_396, a1 = A1_396;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc7 += 1.0 * dq;
@@ -8755,7 +9151,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 66), a1 = ld2(rk1 + 66);
+    const double2 A0_397 = ld2(rk0 + 66), A1_397 = ld2(rk1 + 66);
+    { const double2 a0 = A0_397, a1 = A1_397;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc2 += -1.0 * dq;
       jc15 += 1.0 * dq;
@@ -8804,6 +9201,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
+    const double2 A0_398 = ld2(rk0 + 96), A1_398 = ld2(rk1 + 96);
+    const double2 A0_399 = ld2(rk0 + 132), A1_399 = ld2(rk1 + 132);
     {
       const double dq = scr[32 * SST];
       jc9 += -1.0 * dq;
@@ -8815,46 +9214,52 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 96), a1 = ld2(rk1 + 96);
+    { const double2 a0 = A0_398, a1 = A1_398;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc14 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 132), a1 = ld2(rk1 + 132);
+    { const double2 a0 = A0_399, a1 = A1_399;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
       jc8 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 134), a1 = ld2(rk1 + 134);
+    const double2 A0_400 = ld2(rk0 + 134), A1_400 = ld2(rk1 + 134);
+    const double2 A0_401 = ld2(rk0 + 138), A1_401 = ld2(rk1 + 138);
+    const double2 A0_402 = ld2(rk0 + 144), A1_402 = ld2(rk1 + 144);
+    const double2 A0_403 = ld2(rk0 + 160), A1_403 = ld2(rk1 + 160);
+    { const double2 a0 = A0_400, a1 = A1_400;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
       jc8 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 138), a1 = ld2(rk1 + 138);
+    { const double2 a0 = A0_401, a1 = A1_401;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc3 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 144), a1 = ld2(rk1 + 144);
+    { const double2 a0 = A0_402, a1 = A1_402;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
       jc7 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 160), a1 = ld2(rk1 + 160);
+    { const double2 a0 = A0_403, a1 = A1_403;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c29;
       jc15 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc27 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
+    const double2 A0_404 = ld2(rk0 + 172), A1_404 = ld2(rk1 + 172);
+    const double2 A0_405 = ld2(rk0 + 190), A1_405 = ld2(rk1 + 190);
     {
       const double dq = scr[38 * SST];
       jc9 += -1.0 * dq;
@@ -8869,20 +9274,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc16 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 172), a1 = ld2(rk1 + 172);
+    { const double2 a0 = A0_404, a1 = A1_404;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
       jc9 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc16 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 190), a1 = ld2(rk1 + 190);
+    { const double2 a0 = A0_405, a1 = A1_405;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc10 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
+    const double2 A0_406 = ld2(rk0 + 208), A1_406 = ld2(rk1 + 208);
+    const double2 A0_407 = ld2(rk0 + 210), A1_407 = ld2(rk1 + 210);
     {
       const double dq = scr[41 * SST];
       jc11 += 1.0 * dq;
@@ -8894,54 +9301,59 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 208), a1 = ld2(rk1 + 208);
+    { const double2 a0 = A0_406, a1 = A1_406;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c17;
       jc9 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc17 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 210), a1 = ld2(rk1 + 210);
+    { const double2 a0 = A0_407, a1 = A1_407;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc6 += -1.0 * dq;
       jc10 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc17 += 1.0 * dq;
     }
+    const double2 A0_408 = ld2(rk0 + 212), A1_408 = ld2(rk1 + 212);
+    const double2 A0_409 = ld2(rk0 + 224), A1_409 = ld2(rk1 + 224);
+    const double2 A0_410 = ld2(rk0 + 228), A1_410 = ld2(rk1 + 228);
     {
       const double dq = scr[46 * SST];
       jc4 += 1.0 * dq;
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 212), a1 = ld2(rk1 + 212);
+    { const double2 a0 = A0_408, a1 = A1_408;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c28;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
       jc25 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 224), a1 = ld2(rk1 + 224);
+    { const double2 a0 = A0_409, a1 = A1_409;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c4;
       jc4 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc17 += 2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 228), a1 = ld2(rk1 + 228);
+    { const double2 a0 = A0_410, a1 = A1_410;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
       jc9 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc26 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 232), a1 = ld2(rk1 + 232);
+    const double2 A0_411 = ld2(rk0 + 232), A1_411 = ld2(rk1 + 232);
+    const double2 A0_412 = ld2(rk0 + 244), A1_412 = ld2(rk1 + 244);
+    { const double2 a0 = A0_411, a1 = A1_411;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc4 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 244), a1 = ld2(rk1 + 244);
+    { const double2 a0 = A0_412, a1 = A1_412;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc11 += -1.0 * dq;
       jc15 += -1.0 * dq;
@@ -8960,21 +9372,24 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 268), a1 = ld2(rk1 + 268);
+    const double2 A0_413 = ld2(rk0 + 268), A1_413 = ld2(rk1 + 268);
+    const double2 A0_414 = ld2(rk0 + 338), A1_414 = ld2(rk1 + 338);
+    const double2 A0_415 = ld2(rk0 + 354), A1_415 = ld2(rk1 + 354);
+    { const double2 a0 = A0_413, a1 = A1_413;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc6 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 338), a1 = ld2(rk1 + 338);
+    { const double2 a0 = A0_414, a1 = A1_414;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c5;
       jc5 += -1.0 * dq;
       jc12 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc17 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 354), a1 = ld2(rk1 + 354);
+    { const double2 a0 = A0_415, a1 = A1_415;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c1;
       jc1 += -1.0 * dq;
       jc12 += 1.0 * dq;
@@ -9092,13 +9507,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
+    const double2 A0_416 = ld2(rk0 + 74), A1_416 = ld2(rk1 + 74);
     {
       const double dq = scr[11 * SST];
       jc2 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 74), a1 = ld2(rk1 + 74);
+    { const double2 a0 = A0_416, a1 = A1_416;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc16 += -1.0 * dq;
@@ -9141,6 +9557,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
+    const double2 A0_417 = ld2(rk0 + 108), A1_417 = ld2(rk1 + 108);
+    const double2 A0_418 = ld2(rk0 + 112), A1_418 = ld2(rk1 + 112);
     {
       const double dq = scr[32 * SST];
       jc9 += -1.0 * dq;
@@ -9152,20 +9570,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 108), a1 = ld2(rk1 + 108);
+    { const double2 a0 = A0_417, a1 = A1_417;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 112), a1 = ld2(rk1 + 112);
+    { const double2 a0 = A0_418, a1 = A1_418;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc14 += 1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
+    const double2 A0_419 = ld2(rk0 + 172), A1_419 = ld2(rk1 + 172);
+    const double2 A0_420 = ld2(rk0 + 174), A1_420 = ld2(rk1 + 174);
     {
       const double dq = scr[38 * SST];
       jc9 += -1.0 * dq;
@@ -9180,59 +9600,66 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc16 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 172), a1 = ld2(rk1 + 172);
+    { const double2 a0 = A0_419, a1 = A1_419;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc9 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc16 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 174), a1 = ld2(rk1 + 174);
+    { const double2 a0 = A0_420, a1 = A1_420;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
       jc2 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 176), a1 = ld2(rk1 + 176);
+    const double2 A0_421 = ld2(rk0 + 176), A1_421 = ld2(rk1 + 176);
+    const double2 A0_422 = ld2(rk0 + 178), A1_422 = ld2(rk1 + 178);
+    const double2 A0_423 = ld2(rk0 + 180), A1_423 = ld2(rk1 + 180);
+    const double2 A0_424 = ld2(rk0 + 182), A1_424 = ld2(rk1 + 182);
+    { const double2 a0 = A0_421, a1 = A1_421;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c14;
       jc14 += 0.0 * dq;
       jc16 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 178), a1 = ld2(rk1 + 178);
+    { const double2 a0 = A0_422, a1 = A1_422;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc2 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 180), a1 = ld2(rk1 + 180);
+    { const double2 a0 = A0_423, a1 = A1_423;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += 2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 182), a1 = ld2(rk1 + 182);
+    { const double2 a0 = A0_424, a1 = A1_424;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
       jc9 += 0.0 * dq;
       jc16 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 184), a1 = ld2(rk1 + 184);
+    const double2 A0_425 = ld2(rk0 + 184), A1_425 = ld2(rk1 + 184);
+    const double2 A0_426 = ld2(rk0 + 186), A1_426 = ld2(rk1 + 186);
+    const double2 A0_427 = ld2(rk0 + 188), A1_427 = ld2(rk1 + 188);
+    { const double2 a0 = A0_425, a1 = A1_425;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c30;
       jc16 += -1.0 * dq;
       jc29 += 1.0 * dq;
       jc30 += 0.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 186), a1 = ld2(rk1 + 186);
+    { const double2 a0 = A0_426, a1 = A1_426;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c30;
       jc9 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc30 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 188), a1 = ld2(rk1 + 188);
+    { const double2 a0 = A0_427, a1 = A1_427;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc11 += -1.0 * dq;
       jc16 += -1.0 * dq;
@@ -9385,14 +9812,16 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc17 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 34), a1 = ld2(rk1 + 34);
+    const double2 A0_428 = ld2(rk0 + 34), A1_428 = ld2(rk1 + 34);
+    const double2 A0_429 = ld2(rk0 + 44), A1_429 = ld2(rk1 + 44);
+    { const double2 a0 = A0_428, a1 = A1_428;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc7 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 44), a1 = ld2(rk1 + 44);
+    { const double2 a0 = A0_429, a1 = A1_429;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c24;
       jc10 += -1.0 * dq;
       jc15 += -1.0 * dq;
@@ -9434,14 +9863,16 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc10 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 78), a1 = ld2(rk1 + 78);
+    const double2 A0_430 = ld2(rk0 + 78), A1_430 = ld2(rk1 + 78);
+    const double2 A0_431 = ld2(rk0 + 80), A1_431 = ld2(rk1 + 80);
+    { const double2 a0 = A0_430, a1 = A1_430;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 80), a1 = ld2(rk1 + 80);
+    { const double2 a0 = A0_431, a1 = A1_431;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc2 += -1.0 * dq;
       jc10 += -1.0 * dq;
@@ -9483,49 +9914,56 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 102), a1 = ld2(rk1 + 102);
+    const double2 A0_432 = ld2(rk0 + 102), A1_432 = ld2(rk1 + 102);
+    const double2 A0_433 = ld2(rk0 + 118), A1_433 = ld2(rk1 + 118);
+    const double2 A0_434 = ld2(rk0 + 120), A1_434 = ld2(rk1 + 120);
+    const double2 A0_435 = ld2(rk0 + 142), A1_435 = ld2(rk1 + 142);
+    { const double2 a0 = A0_432, a1 = A1_432;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc17 += 1.0 * dq;
       jc26 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 118), a1 = ld2(rk1 + 118);
+    { const double2 a0 = A0_433, a1 = A1_433;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc9 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 120), a1 = ld2(rk1 + 120);
+    { const double2 a0 = A0_434, a1 = A1_434;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc10 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc17 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 142), a1 = ld2(rk1 + 142);
+    { const double2 a0 = A0_435, a1 = A1_435;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c8;
       jc8 += 1.0 * dq;
       jc10 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 144), a1 = ld2(rk1 + 144);
+    const double2 A0_436 = ld2(rk0 + 144), A1_436 = ld2(rk1 + 144);
+    const double2 A0_437 = ld2(rk0 + 156), A1_437 = ld2(rk1 + 156);
+    const double2 A0_438 = ld2(rk0 + 160), A1_438 = ld2(rk1 + 160);
+    { const double2 a0 = A0_436, a1 = A1_436;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c7;
       jc7 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 156), a1 = ld2(rk1 + 156);
+    { const double2 a0 = A0_437, a1 = A1_437;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc9 += 1.0 * dq;
       jc17 += 1.0 * dq;
       jc26 += -1.0 * dq;
       jc30 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 160), a1 = ld2(rk1 + 160);
+    { const double2 a0 = A0_438, a1 = A1_438;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
       jc15 += -1.0 * dq;
       jc17 += 1.0 * dq;
@@ -9538,45 +9976,51 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
+    const double2 A0_439 = ld2(rk0 + 196), A1_439 = ld2(rk1 + 196);
+    const double2 A0_440 = ld2(rk0 + 198), A1_440 = ld2(rk1 + 198);
+    const double2 A0_441 = ld2(rk0 + 204), A1_441 = ld2(rk1 + 204);
     {
       const double dq = scr[41 * SST];
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 196), a1 = ld2(rk1 + 196);
+    { const double2 a0 = A0_439, a1 = A1_439;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 198), a1 = ld2(rk1 + 198);
+    { const double2 a0 = A0_440, a1 = A1_440;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc3 += 1.0 * dq;
       jc10 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 204), a1 = ld2(rk1 + 204);
+    { const double2 a0 = A0_441, a1 = A1_441;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M5 * 1.0;
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 206), a1 = ld2(rk1 + 206);
+    const double2 A0_442 = ld2(rk0 + 206), A1_442 = ld2(rk1 + 206);
+    const double2 A0_443 = ld2(rk0 + 208), A1_443 = ld2(rk1 + 208);
+    const double2 A0_444 = ld2(rk0 + 210), A1_444 = ld2(rk1 + 210);
+    { const double2 a0 = A0_442, a1 = A1_442;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M10 * 1.0 + scr[45 * SST];
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 208), a1 = ld2(rk1 + 208);
+    { const double2 a0 = A0_443, a1 = A1_443;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc9 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc17 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 210), a1 = ld2(rk1 + 210);
+    { const double2 a0 = A0_444, a1 = A1_444;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c10;
       jc6 += -1.0 * dq;
       jc10 += 1.0 * dq;
@@ -9589,19 +10033,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 226), a1 = ld2(rk1 + 226);
+    const double2 A0_445 = ld2(rk0 + 226), A1_445 = ld2(rk1 + 226);
+    const double2 A0_446 = ld2(rk0 + 238), A1_446 = ld2(rk1 + 238);
+    const double2 A0_447 = ld2(rk0 + 242), A1_447 = ld2(rk1 + 242);
+    { const double2 a0 = A0_445, a1 = A1_445;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c17;
       jc4 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc17 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 238), a1 = ld2(rk1 + 238);
+    { const double2 a0 = A0_446, a1 = A1_446;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c17;
       jc7 += 1.0 * dq;
       jc12 += 1.0 * dq;
       jc17 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 242), a1 = ld2(rk1 + 242);
+    { const double2 a0 = A0_447, a1 = A1_447;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c10;
       jc10 += -1.0 * dq;
       jc12 += 1.0 * dq;
@@ -9614,19 +10061,21 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_448 = ld2(rk0 + 306), A1_448 = ld2(rk1 + 306);
+    const double2 A0_449 = ld2(rk0 + 340), A1_449 = ld2(rk1 + 340);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 306), a1 = ld2(rk1 + 306);
+    { const double2 a0 = A0_448, a1 = A1_448;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc17 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc22 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 340), a1 = ld2(rk1 + 340);
+    { const double2 a0 = A0_449, a1 = A1_449;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc5 += 1.0 * dq;
       jc12 += -1.0 * dq;
@@ -9725,20 +10174,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
+    const double2 A0_450 = ld2(rk0 + 34), A1_450 = ld2(rk1 + 34);
+    const double2 A0_451 = ld2(rk0 + 36), A1_451 = ld2(rk1 + 36);
     {
       const double dq = scr[2 * SST];
       jc7 += -1.0 * dq;
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 34), a1 = ld2(rk1 + 34);
+    { const double2 a0 = A0_450, a1 = A1_450;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
       jc7 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 36), a1 = ld2(rk1 + 36);
+    { const double2 a0 = A0_451, a1 = A1_451;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c10;
       jc7 += -1.0 * dq;
       jc10 += 1.0 * dq;
@@ -9774,7 +10225,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc3 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 76), a1 = ld2(rk1 + 76);
+    const double2 A0_452 = ld2(rk0 + 76), A1_452 = ld2(rk1 + 76);
+    { const double2 a0 = A0_452, a1 = A1_452;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc2 += -1.0 * dq;
       jc3 += -1.0 * dq;
@@ -9799,6 +10251,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc6 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
+    const double2 A0_453 = ld2(rk0 + 90), A1_453 = ld2(rk1 + 90);
     {
       const double dq = scr[26 * SST];
       jc2 += -1.0 * dq;
@@ -9811,7 +10264,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 90), a1 = ld2(rk1 + 90);
+    { const double2 a0 = A0_453, a1 = A1_453;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
@@ -9824,68 +10277,78 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc10 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
+    const double2 A0_454 = ld2(rk0 + 110), A1_454 = ld2(rk1 + 110);
+    const double2 A0_455 = ld2(rk0 + 112), A1_455 = ld2(rk1 + 112);
+    const double2 A0_456 = ld2(rk0 + 114), A1_456 = ld2(rk1 + 114);
     {
       const double dq = scr[35 * SST];
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 110), a1 = ld2(rk1 + 110);
+    { const double2 a0 = A0_454, a1 = A1_454;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc14 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 112), a1 = ld2(rk1 + 112);
+    { const double2 a0 = A0_455, a1 = A1_455;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc14 += 1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 114), a1 = ld2(rk1 + 114);
+    { const double2 a0 = A0_456, a1 = A1_456;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc3 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 124), a1 = ld2(rk1 + 124);
+    const double2 A0_457 = ld2(rk0 + 124), A1_457 = ld2(rk1 + 124);
+    const double2 A0_458 = ld2(rk0 + 138), A1_458 = ld2(rk1 + 138);
+    const double2 A0_459 = ld2(rk0 + 152), A1_459 = ld2(rk1 + 152);
+    const double2 A0_460 = ld2(rk0 + 162), A1_460 = ld2(rk1 + 162);
+    { const double2 a0 = A0_457, a1 = A1_457;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc4 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 138), a1 = ld2(rk1 + 138);
+    { const double2 a0 = A0_458, a1 = A1_458;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc3 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 152), a1 = ld2(rk1 + 152);
+    { const double2 a0 = A0_459, a1 = A1_459;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
       jc2 += 1.0 * dq;
       jc6 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 162), a1 = ld2(rk1 + 162);
+    { const double2 a0 = A0_460, a1 = A1_460;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc19 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 166), a1 = ld2(rk1 + 166);
+    const double2 A0_461 = ld2(rk0 + 166), A1_461 = ld2(rk1 + 166);
+    const double2 A0_462 = ld2(rk0 + 168), A1_462 = ld2(rk1 + 168);
+    const double2 A0_463 = ld2(rk0 + 174), A1_463 = ld2(rk1 + 174);
+    { const double2 a0 = A0_461, a1 = A1_461;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c29;
       jc2 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 168), a1 = ld2(rk1 + 168);
+    { const double2 a0 = A0_462, a1 = A1_462;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * 2.0*c18;
       jc3 += -1.0 * dq;
       jc18 += 2.0 * dq;
@@ -9897,41 +10360,48 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 174), a1 = ld2(rk1 + 174);
+    { const double2 a0 = A0_463, a1 = A1_463;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 178), a1 = ld2(rk1 + 178);
+    const double2 A0_464 = ld2(rk0 + 178), A1_464 = ld2(rk1 + 178);
+    const double2 A0_465 = ld2(rk0 + 180), A1_465 = ld2(rk1 + 180);
+    const double2 A0_466 = ld2(rk0 + 188), A1_466 = ld2(rk1 + 188);
+    const double2 A0_467 = ld2(rk0 + 190), A1_467 = ld2(rk1 + 190);
+    { const double2 a0 = A0_464, a1 = A1_464;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
       jc2 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 180), a1 = ld2(rk1 + 180);
+    { const double2 a0 = A0_465, a1 = A1_465;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * 2.0*c18;
       jc3 += -1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += 2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 188), a1 = ld2(rk1 + 188);
+    { const double2 a0 = A0_466, a1 = A1_466;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c28;
       jc11 += -1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 190), a1 = ld2(rk1 + 190);
+    { const double2 a0 = A0_467, a1 = A1_467;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc10 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 192), a1 = ld2(rk1 + 192);
+    const double2 A0_468 = ld2(rk0 + 192), A1_468 = ld2(rk1 + 192);
+    const double2 A0_469 = ld2(rk0 + 194), A1_469 = ld2(rk1 + 194);
+    const double2 A0_470 = ld2(rk0 + 196), A1_470 = ld2(rk1 + 196);
+    { const double2 a0 = A0_468, a1 = A1_468;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc3 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -9943,34 +10413,37 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 194), a1 = ld2(rk1 + 194);
+    { const double2 a0 = A0_469, a1 = A1_469;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c18;
       jc2 += 1.0 * dq;
       jc18 += -2.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 196), a1 = ld2(rk1 + 196);
+    { const double2 a0 = A0_470, a1 = A1_470;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c17;
       jc3 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 198), a1 = ld2(rk1 + 198);
+    const double2 A0_471 = ld2(rk0 + 198), A1_471 = ld2(rk1 + 198);
+    const double2 A0_472 = ld2(rk0 + 200), A1_472 = ld2(rk1 + 200);
+    const double2 A0_473 = ld2(rk0 + 202), A1_473 = ld2(rk1 + 202);
+    { const double2 a0 = A0_471, a1 = A1_471;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c10;
       jc3 += 1.0 * dq;
       jc10 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 200), a1 = ld2(rk1 + 200);
+    { const double2 a0 = A0_472, a1 = A1_472;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc3 += 1.0 * dq;
       jc6 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 202), a1 = ld2(rk1 + 202);
+    { const double2 a0 = A0_473, a1 = A1_473;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc3 += 1.0 * dq;
       jc11 += -1.0 * dq;
@@ -9983,86 +10456,98 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
+    const double2 A0_474 = ld2(rk0 + 214), A1_474 = ld2(rk1 + 214);
+    const double2 A0_475 = ld2(rk0 + 256), A1_475 = ld2(rk1 + 256);
+    const double2 A0_476 = ld2(rk0 + 260), A1_476 = ld2(rk1 + 260);
     {
       const double dq = scr[46 * SST];
       jc4 += 1.0 * dq;
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 214), a1 = ld2(rk1 + 214);
+    { const double2 a0 = A0_474, a1 = A1_474;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc10 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 256), a1 = ld2(rk1 + 256);
+    { const double2 a0 = A0_475, a1 = A1_475;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc3 += 1.0 * dq;
       jc4 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 260), a1 = ld2(rk1 + 260);
+    { const double2 a0 = A0_476, a1 = A1_476;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c28;
       jc3 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
+    const double2 A0_477 = ld2(rk0 + 266), A1_477 = ld2(rk1 + 266);
+    const double2 A0_478 = ld2(rk0 + 278), A1_478 = ld2(rk1 + 278);
+    const double2 A0_479 = ld2(rk0 + 284), A1_479 = ld2(rk1 + 284);
     {
       const double dq = scr[49 * SST];
       jc9 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 266), a1 = ld2(rk1 + 266);
+    { const double2 a0 = A0_477, a1 = A1_477;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M11 * c9 + scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 278), a1 = ld2(rk1 + 278);
+    { const double2 a0 = A0_478, a1 = A1_478;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc13 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 284), a1 = ld2(rk1 + 284);
+    { const double2 a0 = A0_479, a1 = A1_479;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c20;
       jc3 += 1.0 * dq;
       jc12 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 298), a1 = ld2(rk1 + 298);
+    const double2 A0_480 = ld2(rk0 + 298), A1_480 = ld2(rk1 + 298);
+    const double2 A0_481 = ld2(rk0 + 302), A1_481 = ld2(rk1 + 302);
+    const double2 A0_482 = ld2(rk0 + 306), A1_482 = ld2(rk1 + 306);
+    const double2 A0_483 = ld2(rk0 + 320), A1_483 = ld2(rk1 + 320);
+    { const double2 a0 = A0_480, a1 = A1_480;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c28;
       jc11 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc28 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 302), a1 = ld2(rk1 + 302);
+    { const double2 a0 = A0_481, a1 = A1_481;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c10;
       jc10 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 306), a1 = ld2(rk1 + 306);
+    { const double2 a0 = A0_482, a1 = A1_482;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c17;
       jc17 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc22 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 320), a1 = ld2(rk1 + 320);
+    { const double2 a0 = A0_483, a1 = A1_483;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c22;
       jc3 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc22 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 336), a1 = ld2(rk1 + 336);
+    const double2 A0_484 = ld2(rk0 + 336), A1_484 = ld2(rk1 + 336);
+    const double2 A0_485 = ld2(rk0 + 360), A1_485 = ld2(rk1 + 360);
+    { const double2 a0 = A0_484, a1 = A1_484;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c21;
       jc3 += 1.0 * dq;
       jc6 += 1.0 * dq;
@@ -10070,7 +10555,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 360), a1 = ld2(rk1 + 360);
+    { const double2 a0 = A0_485, a1 = A1_485;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c30;
       jc12 += 1.0 * dq;
       jc18 += -1.0 * dq;
@@ -10167,6 +10652,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc16 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
+    const double2 A0_486 = ld2(rk0 + 54), A1_486 = ld2(rk1 + 54);
+    const double2 A0_487 = ld2(rk0 + 56), A1_487 = ld2(rk1 + 56);
     {
       const double dq = 2.0 * scr[2 * SST];
       jc7 += -1.0 * dq;
@@ -10179,22 +10666,25 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 54), a1 = ld2(rk1 + 54);
+    { const double2 a0 = A0_486, a1 = A1_486;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M6 * 1.0;
       jc2 += -2.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 56), a1 = ld2(rk1 + 56);
+    { const double2 a0 = A0_487, a1 = A1_487;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M7 * 1.0 + scr[7 * SST];
       jc2 += -2.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 58), a1 = ld2(rk1 + 58);
+    const double2 A0_488 = ld2(rk0 + 58), A1_488 = ld2(rk1 + 58);
+    const double2 A0_489 = ld2(rk0 + 60), A1_489 = ld2(rk1 + 60);
+    const double2 A0_490 = ld2(rk0 + 66), A1_490 = ld2(rk1 + 66);
+    { const double2 a0 = A0_488, a1 = A1_488;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M5 * 1.0;
       jc2 += -2.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 60), a1 = ld2(rk1 + 60);
+    { const double2 a0 = A0_489, a1 = A1_489;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M8 * 1.0;
       jc2 += -2.0 * dq;
       jc19 += 1.0 * dq;
@@ -10205,14 +10695,16 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 66), a1 = ld2(rk1 + 66);
+    { const double2 a0 = A0_490, a1 = A1_490;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c15;
       jc2 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc19 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 70), a1 = ld2(rk1 + 70);
+    const double2 A0_491 = ld2(rk0 + 70), A1_491 = ld2(rk1 + 70);
+    const double2 A0_492 = ld2(rk0 + 74), A1_492 = ld2(rk1 + 74);
+    { const double2 a0 = A0_491, a1 = A1_491;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c24;
       jc2 += -1.0 * dq;
       jc8 += -1.0 * dq;
@@ -10225,7 +10717,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 74), a1 = ld2(rk1 + 74);
+    { const double2 a0 = A0_492, a1 = A1_492;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c26;
       jc2 += -1.0 * dq;
       jc16 += -1.0 * dq;
@@ -10238,7 +10730,10 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc3 += 1.0 * dq;
       jc18 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 76), a1 = ld2(rk1 + 76);
+    const double2 A0_493 = ld2(rk0 + 76), A1_493 = ld2(rk1 + 76);
+    const double2 A0_494 = ld2(rk0 + 78), A1_494 = ld2(rk1 + 78);
+    const double2 A0_495 = ld2(rk0 + 80), A1_495 = ld2(rk1 + 80);
+    { const double2 a0 = A0_493, a1 = A1_493;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc2 += -1.0 * dq;
       jc3 += -1.0 * dq;
@@ -10251,20 +10746,21 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc10 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 78), a1 = ld2(rk1 + 78);
+    { const double2 a0 = A0_494, a1 = A1_494;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc2 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 80), a1 = ld2(rk1 + 80);
+    { const double2 a0 = A0_495, a1 = A1_495;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
       jc2 += -1.0 * dq;
       jc10 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc19 += 1.0 * dq;
     }
+    const double2 A0_496 = ld2(rk0 + 82), A1_496 = ld2(rk1 + 82);
     {
       const double dq = 2.0 * scr[20 * SST];
       jc2 += -1.0 * dq;
@@ -10277,7 +10773,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc6 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 82), a1 = ld2(rk1 + 82);
+    { const double2 a0 = A0_496, a1 = A1_496;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc2 += -1.0 * dq;
       jc4 += 1.0 * dq;
@@ -10290,7 +10786,10 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc25 += -1.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 84), a1 = ld2(rk1 + 84);
+    const double2 A0_497 = ld2(rk0 + 84), A1_497 = ld2(rk1 + 84);
+    const double2 A0_498 = ld2(rk0 + 86), A1_498 = ld2(rk1 + 86);
+    const double2 A0_499 = ld2(rk0 + 88), A1_499 = ld2(rk1 + 88);
+    { const double2 a0 = A0_497, a1 = A1_497;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c6;
       jc2 += -1.0 * dq;
       jc6 += 1.0 * dq;
@@ -10303,27 +10802,29 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 86), a1 = ld2(rk1 + 86);
+    { const double2 a0 = A0_498, a1 = A1_498;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c25;
       jc2 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc25 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 88), a1 = ld2(rk1 + 88);
+    { const double2 a0 = A0_499, a1 = A1_499;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c28;
       jc2 += -1.0 * dq;
       jc11 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc28 += 1.0 * dq;
     }
+    const double2 A0_500 = ld2(rk0 + 92), A1_500 = ld2(rk1 + 92);
+    const double2 A0_501 = ld2(rk0 + 146), A1_501 = ld2(rk1 + 146);
     {
       const double dq = scr[33 * SST] * c9 + 2.0 * scr[32 * SST];
       jc9 += -1.0 * dq;
       jc10 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 92), a1 = ld2(rk1 + 92);
+    { const double2 a0 = A0_500, a1 = A1_500;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc2 += 1.0 * dq;
       jc14 += 1.0 * dq;
@@ -10335,21 +10836,24 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 146), a1 = ld2(rk1 + 146);
+    { const double2 a0 = A0_501, a1 = A1_501;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
       jc2 += 1.0 * dq;
       jc19 += -1.0 * dq;
       jc26 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 162), a1 = ld2(rk1 + 162);
+    const double2 A0_502 = ld2(rk0 + 162), A1_502 = ld2(rk1 + 162);
+    const double2 A0_503 = ld2(rk0 + 164), A1_503 = ld2(rk1 + 164);
+    const double2 A0_504 = ld2(rk0 + 174), A1_504 = ld2(rk1 + 174);
+    { const double2 a0 = A0_502, a1 = A1_502;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c29;
       jc2 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc19 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 164), a1 = ld2(rk1 + 164);
+    { const double2 a0 = A0_503, a1 = A1_503;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c4;
       jc4 += 1.0 * dq;
       jc19 += 1.0 * dq;
@@ -10361,13 +10865,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 174), a1 = ld2(rk1 + 174);
+    { const double2 a0 = A0_504, a1 = A1_504;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
       jc2 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
+    const double2 A0_505 = ld2(rk0 + 214), A1_505 = ld2(rk1 + 214);
     {
       const double dq = 2.0 * scr[41 * SST];
       jc11 += 1.0 * dq;
@@ -10385,13 +10890,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 214), a1 = ld2(rk1 + 214);
+    { const double2 a0 = A0_505, a1 = A1_505;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c10;
       jc10 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
+    const double2 A0_506 = ld2(rk0 + 314), A1_506 = ld2(rk1 + 314);
     {
       const double dq = scr[49 * SST];
       jc9 += 1.0 * dq;
@@ -10404,7 +10910,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 314), a1 = ld2(rk1 + 314);
+    { const double2 a0 = A0_506, a1 = A1_506;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c23;
       jc2 += 1.0 * dq;
       jc19 += -1.0 * dq;
@@ -10566,6 +11072,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
+    const double2 A0_507 = ld2(rk0 + 264), A1_507 = ld2(rk1 + 264);
     {
       const double dq = scr[41 * SST];
       jc11 += 1.0 * dq;
@@ -10583,33 +11090,36 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 264), a1 = ld2(rk1 + 264);
+    { const double2 a0 = A0_507, a1 = A1_507;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M11 * 1.0 + scr[49 * SST];
       jc9 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_508 = ld2(rk0 + 280), A1_508 = ld2(rk1 + 280);
+    const double2 A0_509 = ld2(rk0 + 284), A1_509 = ld2(rk1 + 284);
+    const double2 A0_510 = ld2(rk0 + 344), A1_510 = ld2(rk1 + 344);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 280), a1 = ld2(rk1 + 280);
+    { const double2 a0 = A0_508, a1 = A1_508;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc7 += -1.0 * dq;
       jc12 += 1.0 * dq;
       jc20 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 284), a1 = ld2(rk1 + 284);
+    { const double2 a0 = A0_509, a1 = A1_509;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc12 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 344), a1 = ld2(rk1 + 344);
+    { const double2 a0 = A0_510, a1 = A1_510;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 1.0;
       jc20 += -1.0 * dq;
       jc23 += 1.0 * dq;
@@ -10792,19 +11302,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_511 = ld2(rk0 + 324), A1_511 = ld2(rk1 + 324);
+    const double2 A0_512 = ld2(rk0 + 328), A1_512 = ld2(rk1 + 328);
+    const double2 A0_513 = ld2(rk0 + 330), A1_513 = ld2(rk1 + 330);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 324), a1 = ld2(rk1 + 324);
+    { const double2 a0 = A0_511, a1 = A1_511;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 1.0;
       jc10 += 1.0 * dq;
       jc21 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 328), a1 = ld2(rk1 + 328);
+    { const double2 a0 = A0_512, a1 = A1_512;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc6 += 1.0 * dq;
       jc10 += 1.0 * dq;
@@ -10812,7 +11325,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc21 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 330), a1 = ld2(rk1 + 330);
+    { const double2 a0 = A0_513, a1 = A1_513;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc6 += 1.0 * dq;
       jc7 += -1.0 * dq;
@@ -10820,7 +11333,10 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc21 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 332), a1 = ld2(rk1 + 332);
+    const double2 A0_514 = ld2(rk0 + 332), A1_514 = ld2(rk1 + 332);
+    const double2 A0_515 = ld2(rk0 + 334), A1_515 = ld2(rk1 + 334);
+    const double2 A0_516 = ld2(rk0 + 336), A1_516 = ld2(rk1 + 336);
+    { const double2 a0 = A0_514, a1 = A1_514;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc6 += 1.0 * dq;
@@ -10828,7 +11344,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc21 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 334), a1 = ld2(rk1 + 334);
+    { const double2 a0 = A0_515, a1 = A1_515;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc6 += 1.0 * dq;
       jc8 += 1.0 * dq;
@@ -10836,7 +11352,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc21 += -1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 336), a1 = ld2(rk1 + 336);
+    { const double2 a0 = A0_516, a1 = A1_516;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc6 += 1.0 * dq;
@@ -11023,47 +11539,53 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_517 = ld2(rk0 + 304), A1_517 = ld2(rk1 + 304);
+    const double2 A0_518 = ld2(rk0 + 308), A1_518 = ld2(rk1 + 308);
+    const double2 A0_519 = ld2(rk0 + 312), A1_519 = ld2(rk1 + 312);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 304), a1 = ld2(rk1 + 304);
+    { const double2 a0 = A0_517, a1 = A1_517;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 1.0;
       jc17 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc22 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 308), a1 = ld2(rk1 + 308);
+    { const double2 a0 = A0_518, a1 = A1_518;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc14 += 1.0 * dq;
       jc22 += -1.0 * dq;
       jc23 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 312), a1 = ld2(rk1 + 312);
+    { const double2 a0 = A0_519, a1 = A1_519;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc22 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 316), a1 = ld2(rk1 + 316);
+    const double2 A0_520 = ld2(rk0 + 316), A1_520 = ld2(rk1 + 316);
+    const double2 A0_521 = ld2(rk0 + 320), A1_521 = ld2(rk1 + 320);
+    const double2 A0_522 = ld2(rk0 + 368), A1_522 = ld2(rk1 + 368);
+    { const double2 a0 = A0_520, a1 = A1_520;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc8 += 1.0 * dq;
       jc22 += -1.0 * dq;
       jc23 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 320), a1 = ld2(rk1 + 320);
+    { const double2 a0 = A0_521, a1 = A1_521;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc22 += -1.0 * dq;
       jc23 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 368), a1 = ld2(rk1 + 368);
+    { const double2 a0 = A0_522, a1 = A1_522;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc22 += -1.0 * dq;
       jc24 += 1.0 * dq;
@@ -11250,46 +11772,52 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_523 = ld2(rk0 + 310), A1_523 = ld2(rk1 + 310);
+    const double2 A0_524 = ld2(rk0 + 314), A1_524 = ld2(rk1 + 314);
+    const double2 A0_525 = ld2(rk0 + 318), A1_525 = ld2(rk1 + 318);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 310), a1 = ld2(rk1 + 310);
+    { const double2 a0 = A0_523, a1 = A1_523;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c14;
       jc14 += -1.0 * dq;
       jc22 += 1.0 * dq;
       jc23 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 314), a1 = ld2(rk1 + 314);
+    { const double2 a0 = A0_524, a1 = A1_524;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
       jc2 += 1.0 * dq;
       jc19 += -1.0 * dq;
       jc22 += 1.0 * dq;
       jc23 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 318), a1 = ld2(rk1 + 318);
+    { const double2 a0 = A0_525, a1 = A1_525;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc8 += -1.0 * dq;
       jc22 += 1.0 * dq;
       jc23 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 322), a1 = ld2(rk1 + 322);
+    const double2 A0_526 = ld2(rk0 + 322), A1_526 = ld2(rk1 + 322);
+    const double2 A0_527 = ld2(rk0 + 342), A1_527 = ld2(rk1 + 342);
+    const double2 A0_528 = ld2(rk0 + 346), A1_528 = ld2(rk1 + 346);
+    { const double2 a0 = A0_526, a1 = A1_526;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc22 += 1.0 * dq;
       jc23 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 342), a1 = ld2(rk1 + 342);
+    { const double2 a0 = A0_527, a1 = A1_527;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 1.0;
       jc20 += 1.0 * dq;
       jc23 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 346), a1 = ld2(rk1 + 346);
+    { const double2 a0 = A0_528, a1 = A1_528;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 1.0;
       jc1 += 1.0 * dq;
       jc23 += -1.0 * dq;
@@ -11384,32 +11912,37 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc24 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
+    const double2 A0_529 = ld2(rk0 + 44), A1_529 = ld2(rk1 + 44);
+    const double2 A0_530 = ld2(rk0 + 46), A1_530 = ld2(rk1 + 46);
+    const double2 A0_531 = ld2(rk0 + 48), A1_531 = ld2(rk1 + 48);
     {
       const double dq = scr[2 * SST];
       jc7 += -1.0 * dq;
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 44), a1 = ld2(rk1 + 44);
+    { const double2 a0 = A0_529, a1 = A1_529;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
       jc10 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 46), a1 = ld2(rk1 + 46);
+    { const double2 a0 = A0_530, a1 = A1_530;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M3 * 1.0 + scr[3 * SST];
       jc2 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 48), a1 = ld2(rk1 + 48);
+    { const double2 a0 = A0_531, a1 = A1_531;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M4 * 1.0;
       jc2 += -1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 50), a1 = ld2(rk1 + 50);
+    const double2 A0_532 = ld2(rk0 + 50), A1_532 = ld2(rk1 + 50);
+    const double2 A0_533 = ld2(rk0 + 64), A1_533 = ld2(rk1 + 64);
+    { const double2 a0 = A0_532, a1 = A1_532;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M5 * 1.0;
       jc2 += -1.0 * dq;
       jc15 += -1.0 * dq;
@@ -11426,27 +11959,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 64), a1 = ld2(rk1 + 64);
+    { const double2 a0 = A0_533, a1 = A1_533;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 66), a1 = ld2(rk1 + 66);
+    const double2 A0_534 = ld2(rk0 + 66), A1_534 = ld2(rk1 + 66);
+    const double2 A0_535 = ld2(rk0 + 68), A1_535 = ld2(rk1 + 68);
+    const double2 A0_536 = ld2(rk0 + 70), A1_536 = ld2(rk1 + 70);
+    { const double2 a0 = A0_534, a1 = A1_534;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc19 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 68), a1 = ld2(rk1 + 68);
+    { const double2 a0 = A0_535, a1 = A1_535;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 70), a1 = ld2(rk1 + 70);
+    { const double2 a0 = A0_536, a1 = A1_536;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc2 += -1.0 * dq;
       jc8 += -1.0 * dq;
@@ -11506,61 +12042,71 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 96), a1 = ld2(rk1 + 96);
+    const double2 A0_537 = ld2(rk0 + 96), A1_537 = ld2(rk1 + 96);
+    const double2 A0_538 = ld2(rk0 + 98), A1_538 = ld2(rk1 + 98);
+    const double2 A0_539 = ld2(rk0 + 100), A1_539 = ld2(rk1 + 100);
+    const double2 A0_540 = ld2(rk0 + 132), A1_540 = ld2(rk1 + 132);
+    { const double2 a0 = A0_537, a1 = A1_537;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc14 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 98), a1 = ld2(rk1 + 98);
+    { const double2 a0 = A0_538, a1 = A1_538;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc8 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc24 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 100), a1 = ld2(rk1 + 100);
+    { const double2 a0 = A0_539, a1 = A1_539;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc8 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc24 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 132), a1 = ld2(rk1 + 132);
+    { const double2 a0 = A0_540, a1 = A1_540;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c24;
       jc8 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 134), a1 = ld2(rk1 + 134);
+    const double2 A0_541 = ld2(rk0 + 134), A1_541 = ld2(rk1 + 134);
+    const double2 A0_542 = ld2(rk0 + 136), A1_542 = ld2(rk1 + 136);
+    const double2 A0_543 = ld2(rk0 + 138), A1_543 = ld2(rk1 + 138);
+    const double2 A0_544 = ld2(rk0 + 140), A1_544 = ld2(rk1 + 140);
+    { const double2 a0 = A0_541, a1 = A1_541;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c24;
       jc8 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 136), a1 = ld2(rk1 + 136);
+    { const double2 a0 = A0_542, a1 = A1_542;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c29;
       jc10 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 138), a1 = ld2(rk1 + 138);
+    { const double2 a0 = A0_543, a1 = A1_543;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 140), a1 = ld2(rk1 + 140);
+    { const double2 a0 = A0_544, a1 = A1_544;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
       jc9 += -1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 142), a1 = ld2(rk1 + 142);
+    const double2 A0_545 = ld2(rk0 + 142), A1_545 = ld2(rk1 + 142);
+    const double2 A0_546 = ld2(rk0 + 192), A1_546 = ld2(rk1 + 192);
+    { const double2 a0 = A0_545, a1 = A1_545;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c10;
       jc8 += 1.0 * dq;
       jc10 += -1.0 * dq;
@@ -11573,7 +12119,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 192), a1 = ld2(rk1 + 192);
+    { const double2 a0 = A0_546, a1 = A1_546;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc3 += 1.0 * dq;
       jc8 += -1.0 * dq;
@@ -11585,13 +12131,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
+    const double2 A0_547 = ld2(rk0 + 208), A1_547 = ld2(rk1 + 208);
+    const double2 A0_548 = ld2(rk0 + 212), A1_548 = ld2(rk1 + 212);
     {
       const double dq = scr[45 * SST];
       jc2 += 1.0 * dq;
       jc9 += 1.0 * dq;
       jc17 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 208), a1 = ld2(rk1 + 208);
+    { const double2 a0 = A0_547, a1 = A1_547;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc9 += 1.0 * dq;
       jc15 += -1.0 * dq;
@@ -11604,28 +12152,31 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 212), a1 = ld2(rk1 + 212);
+    { const double2 a0 = A0_548, a1 = A1_548;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c25;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
       jc25 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 234), a1 = ld2(rk1 + 234);
+    const double2 A0_549 = ld2(rk0 + 234), A1_549 = ld2(rk1 + 234);
+    const double2 A0_550 = ld2(rk0 + 246), A1_550 = ld2(rk1 + 246);
+    const double2 A0_551 = ld2(rk0 + 248), A1_551 = ld2(rk1 + 248);
+    { const double2 a0 = A0_549, a1 = A1_549;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c4;
       jc4 += -1.0 * dq;
       jc6 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 246), a1 = ld2(rk1 + 246);
+    { const double2 a0 = A0_550, a1 = A1_550;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c28;
       jc11 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 248), a1 = ld2(rk1 + 248);
+    { const double2 a0 = A0_551, a1 = A1_551;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc8 += 1.0 * dq;
       jc11 += -1.0 * dq;
@@ -11638,34 +12189,40 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_552 = ld2(rk0 + 270), A1_552 = ld2(rk1 + 270);
+    const double2 A0_553 = ld2(rk0 + 272), A1_553 = ld2(rk1 + 272);
+    const double2 A0_554 = ld2(rk0 + 316), A1_554 = ld2(rk1 + 316);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 270), a1 = ld2(rk1 + 270);
+    { const double2 a0 = A0_552, a1 = A1_552;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc6 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 272), a1 = ld2(rk1 + 272);
+    { const double2 a0 = A0_553, a1 = A1_553;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc6 += 1.0 * dq;
       jc8 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 316), a1 = ld2(rk1 + 316);
+    { const double2 a0 = A0_554, a1 = A1_554;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c22;
       jc8 += 1.0 * dq;
       jc22 += -1.0 * dq;
       jc23 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 334), a1 = ld2(rk1 + 334);
+    const double2 A0_555 = ld2(rk0 + 334), A1_555 = ld2(rk1 + 334);
+    const double2 A0_556 = ld2(rk0 + 356), A1_556 = ld2(rk1 + 356);
+    const double2 A0_557 = ld2(rk0 + 366), A1_557 = ld2(rk1 + 366);
+    { const double2 a0 = A0_555, a1 = A1_555;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c21;
       jc6 += 1.0 * dq;
       jc8 += 1.0 * dq;
@@ -11673,14 +12230,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc21 += -1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 356), a1 = ld2(rk1 + 356);
+    { const double2 a0 = A0_556, a1 = A1_556;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc1 += 1.0 * dq;
       jc12 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 366), a1 = ld2(rk1 + 366);
+    { const double2 a0 = A0_557, a1 = A1_557;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc22 += 1.0 * dq;
       jc24 += -1.0 * dq;
@@ -11754,6 +12311,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc28 = 0.0;
     double jc29 = 0.0;
     double jc30 = 0.0;
+    const double2 A0_558 = ld2(rk0 + 34), A1_558 = ld2(rk1 + 34);
     {
       const double dq = scr[0 * SST];
       jc7 += -2.0 * dq;
@@ -11771,7 +12329,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 34), a1 = ld2(rk1 + 34);
+    { const double2 a0 = A0_558, a1 = A1_558;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc7 += -1.0 * dq;
       jc17 += 1.0 * dq;
@@ -11825,13 +12383,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc6 += -1.0 * dq;
       jc25 += 1.0 * dq;
     }
+    const double2 A0_559 = ld2(rk0 + 84), A1_559 = ld2(rk1 + 84);
+    const double2 A0_560 = ld2(rk0 + 86), A1_560 = ld2(rk1 + 86);
     {
       const double dq = scr[27 * SST] * c2 + scr[26 * SST];
       jc2 += -1.0 * dq;
       jc25 += -1.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 84), a1 = ld2(rk1 + 84);
+    { const double2 a0 = A0_559, a1 = A1_559;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc6 += 1.0 * dq;
@@ -11844,13 +12404,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 86), a1 = ld2(rk1 + 86);
+    { const double2 a0 = A0_560, a1 = A1_560;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc2 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc25 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
+    const double2 A0_561 = ld2(rk0 + 128), A1_561 = ld2(rk1 + 128);
+    const double2 A0_562 = ld2(rk0 + 154), A1_562 = ld2(rk1 + 154);
     {
       const double dq = scr[32 * SST];
       jc9 += -1.0 * dq;
@@ -11862,21 +12424,23 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 128), a1 = ld2(rk1 + 128);
+    { const double2 a0 = A0_561, a1 = A1_561;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc6 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc25 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 154), a1 = ld2(rk1 + 154);
+    { const double2 a0 = A0_562, a1 = A1_562;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc25 += 1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 166), a1 = ld2(rk1 + 166);
+    const double2 A0_563 = ld2(rk0 + 166), A1_563 = ld2(rk1 + 166);
+    const double2 A0_564 = ld2(rk0 + 178), A1_564 = ld2(rk1 + 178);
+    { const double2 a0 = A0_563, a1 = A1_563;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc18 += -1.0 * dq;
@@ -11889,7 +12453,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 178), a1 = ld2(rk1 + 178);
+    { const double2 a0 = A0_564, a1 = A1_564;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc16 += -1.0 * dq;
@@ -11901,7 +12465,9 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 200), a1 = ld2(rk1 + 200);
+    const double2 A0_565 = ld2(rk0 + 200), A1_565 = ld2(rk1 + 200);
+    const double2 A0_566 = ld2(rk0 + 212), A1_566 = ld2(rk1 + 212);
+    { const double2 a0 = A0_565, a1 = A1_565;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc6 += 1.0 * dq;
@@ -11920,39 +12486,45 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 212), a1 = ld2(rk1 + 212);
+    { const double2 a0 = A0_566, a1 = A1_566;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c24;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
       jc25 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 218), a1 = ld2(rk1 + 218);
+    const double2 A0_567 = ld2(rk0 + 218), A1_567 = ld2(rk1 + 218);
+    const double2 A0_568 = ld2(rk0 + 220), A1_568 = ld2(rk1 + 220);
+    const double2 A0_569 = ld2(rk0 + 252), A1_569 = ld2(rk1 + 252);
+    const double2 A0_570 = ld2(rk0 + 262), A1_570 = ld2(rk1 + 262);
+    { const double2 a0 = A0_567, a1 = A1_567;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c25;
       jc6 += 1.0 * dq;
       jc25 += -2.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 220), a1 = ld2(rk1 + 220);
+    { const double2 a0 = A0_568, a1 = A1_568;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc6 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc25 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 252), a1 = ld2(rk1 + 252);
+    { const double2 a0 = A0_569, a1 = A1_569;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc11 += -1.0 * dq;
       jc25 += -1.0 * dq;
       jc28 += 2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 262), a1 = ld2(rk1 + 262);
+    { const double2 a0 = A0_570, a1 = A1_570;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc25 += -1.0 * dq;
       jc28 += 1.0 * dq;
     }
+    const double2 A0_571 = ld2(rk0 + 268), A1_571 = ld2(rk1 + 268);
+    const double2 A0_572 = ld2(rk0 + 272), A1_572 = ld2(rk1 + 272);
     {
       const double dq = scr[49 * SST];
       jc9 += 1.0 * dq;
@@ -11965,47 +12537,52 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 268), a1 = ld2(rk1 + 268);
+    { const double2 a0 = A0_571, a1 = A1_571;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc6 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 272), a1 = ld2(rk1 + 272);
+    { const double2 a0 = A0_572, a1 = A1_572;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc6 += 1.0 * dq;
       jc8 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 278), a1 = ld2(rk1 + 278);
+    const double2 A0_573 = ld2(rk0 + 278), A1_573 = ld2(rk1 + 278);
+    const double2 A0_574 = ld2(rk0 + 288), A1_574 = ld2(rk1 + 288);
+    const double2 A0_575 = ld2(rk0 + 300), A1_575 = ld2(rk1 + 300);
+    const double2 A0_576 = ld2(rk0 + 326), A1_576 = ld2(rk1 + 326);
+    { const double2 a0 = A0_573, a1 = A1_573;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc13 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 288), a1 = ld2(rk1 + 288);
+    { const double2 a0 = A0_574, a1 = A1_574;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc1 += 1.0 * dq;
       jc2 += 1.0 * dq;
       jc7 += -1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 300), a1 = ld2(rk1 + 300);
+    { const double2 a0 = A0_575, a1 = A1_575;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc10 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc25 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 326), a1 = ld2(rk1 + 326);
+    { const double2 a0 = A0_576, a1 = A1_576;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c10;
       jc10 += -1.0 * dq;
       jc21 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 366), a1 = ld2(rk1 + 366);
+    const double2 A0_577 = ld2(rk0 + 366), A1_577 = ld2(rk1 + 366);
+    { const double2 a0 = A0_577, a1 = A1_577;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc22 += 1.0 * dq;
       jc24 += -1.0 * dq;
@@ -12122,7 +12699,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 74), a1 = ld2(rk1 + 74);
+    const double2 A0_578 = ld2(rk0 + 74), A1_578 = ld2(rk1 + 74);
+    { const double2 a0 = A0_578, a1 = A1_578;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc2 += -1.0 * dq;
       jc16 += -1.0 * dq;
@@ -12171,75 +12749,85 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc10 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
+    const double2 A0_579 = ld2(rk0 + 102), A1_579 = ld2(rk1 + 102);
+    const double2 A0_580 = ld2(rk0 + 106), A1_580 = ld2(rk1 + 106);
+    const double2 A0_581 = ld2(rk0 + 144), A1_581 = ld2(rk1 + 144);
     {
       const double dq = scr[35 * SST];
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 102), a1 = ld2(rk1 + 102);
+    { const double2 a0 = A0_579, a1 = A1_579;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc2 += 1.0 * dq;
       jc17 += 1.0 * dq;
       jc26 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 106), a1 = ld2(rk1 + 106);
+    { const double2 a0 = A0_580, a1 = A1_580;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc14 += 1.0 * dq;
       jc26 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 144), a1 = ld2(rk1 + 144);
+    { const double2 a0 = A0_581, a1 = A1_581;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc7 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 146), a1 = ld2(rk1 + 146);
+    const double2 A0_582 = ld2(rk0 + 146), A1_582 = ld2(rk1 + 146);
+    const double2 A0_583 = ld2(rk0 + 148), A1_583 = ld2(rk1 + 148);
+    const double2 A0_584 = ld2(rk0 + 150), A1_584 = ld2(rk1 + 150);
+    const double2 A0_585 = ld2(rk0 + 152), A1_585 = ld2(rk1 + 152);
+    { const double2 a0 = A0_582, a1 = A1_582;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
       jc2 += 1.0 * dq;
       jc19 += -1.0 * dq;
       jc26 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 148), a1 = ld2(rk1 + 148);
+    { const double2 a0 = A0_583, a1 = A1_583;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c14;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc14 += -1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 150), a1 = ld2(rk1 + 150);
+    { const double2 a0 = A0_584, a1 = A1_584;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c29;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
       jc26 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 152), a1 = ld2(rk1 + 152);
+    { const double2 a0 = A0_585, a1 = A1_585;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc2 += 1.0 * dq;
       jc6 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 154), a1 = ld2(rk1 + 154);
+    const double2 A0_586 = ld2(rk0 + 154), A1_586 = ld2(rk1 + 154);
+    const double2 A0_587 = ld2(rk0 + 156), A1_587 = ld2(rk1 + 156);
+    const double2 A0_588 = ld2(rk0 + 158), A1_588 = ld2(rk1 + 158);
+    { const double2 a0 = A0_586, a1 = A1_586;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc2 += 1.0 * dq;
       jc3 += -1.0 * dq;
       jc25 += 1.0 * dq;
       jc26 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 156), a1 = ld2(rk1 + 156);
+    { const double2 a0 = A0_587, a1 = A1_587;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c30;
       jc9 += 1.0 * dq;
       jc17 += 1.0 * dq;
       jc26 += -1.0 * dq;
       jc30 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 158), a1 = ld2(rk1 + 158);
+    { const double2 a0 = A0_588, a1 = A1_588;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c10;
       jc2 += 1.0 * dq;
       jc10 += -1.0 * dq;
@@ -12252,6 +12840,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
+    const double2 A0_589 = ld2(rk0 + 228), A1_589 = ld2(rk1 + 228);
     {
       const double dq = scr[41 * SST];
       jc11 += 1.0 * dq;
@@ -12269,13 +12858,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 228), a1 = ld2(rk1 + 228);
+    { const double2 a0 = A0_589, a1 = A1_589;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc9 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc26 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
+    const double2 A0_590 = ld2(rk0 + 292), A1_590 = ld2(rk1 + 292);
     {
       const double dq = scr[49 * SST];
       jc9 += 1.0 * dq;
@@ -12288,7 +12878,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 292), a1 = ld2(rk1 + 292);
+    { const double2 a0 = A0_590, a1 = A1_590;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc11 += -1.0 * dq;
       jc26 += -1.0 * dq;
@@ -12361,12 +12951,13 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc28 = 0.0;
     double jc29 = 0.0;
     double jc30 = 0.0;
+    const double2 A0_591 = ld2(rk0 + 2), A1_591 = ld2(rk1 + 2);
     {
       const double dq = scr[0 * SST];
       jc7 += -2.0 * dq;
       jc15 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 2), a1 = ld2(rk1 + 2);
+    { const double2 a0 = A0_591, a1 = A1_591;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M1 * 1.0 + scr[1 * SST];
       jc2 += -1.0 * dq;
       jc7 += -1.0 * dq;
@@ -12413,6 +13004,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc17 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
+    const double2 A0_592 = ld2(rk0 + 38), A1_592 = ld2(rk1 + 38);
+    const double2 A0_593 = ld2(rk0 + 52), A1_593 = ld2(rk1 + 52);
     {
       const double dq = -scr[76 * SST] * c17;
       jc7 += -1.0 * dq;
@@ -12420,7 +13013,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc17 += 1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 38), a1 = ld2(rk1 + 38);
+    { const double2 a0 = A0_592, a1 = A1_592;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c28;
       jc7 += -1.0 * dq;
       jc11 += -1.0 * dq;
@@ -12433,31 +13026,34 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 52), a1 = ld2(rk1 + 52);
+    { const double2 a0 = A0_593, a1 = A1_593;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c7;
       jc2 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc15 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
+    const double2 A0_594 = ld2(rk0 + 62), A1_594 = ld2(rk1 + 62);
+    const double2 A0_595 = ld2(rk0 + 68), A1_595 = ld2(rk1 + 68);
+    const double2 A0_596 = ld2(rk0 + 72), A1_596 = ld2(rk1 + 72);
     {
       const double dq = scr[6 * SST];
       jc2 += -2.0 * dq;
       jc19 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 62), a1 = ld2(rk1 + 62);
+    { const double2 a0 = A0_594, a1 = A1_594;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * M9 * c2 + scr[10 * SST];
       jc2 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 68), a1 = ld2(rk1 + 68);
+    { const double2 a0 = A0_595, a1 = A1_595;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * 2.0*c27;
       jc2 += -1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 72), a1 = ld2(rk1 + 72);
+    { const double2 a0 = A0_596, a1 = A1_596;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc2 += -1.0 * dq;
       jc8 += -1.0 * dq;
@@ -12512,7 +13108,10 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc10 += 1.0 * dq;
       jc19 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 92), a1 = ld2(rk1 + 92);
+    const double2 A0_597 = ld2(rk0 + 92), A1_597 = ld2(rk1 + 92);
+    const double2 A0_598 = ld2(rk0 + 94), A1_598 = ld2(rk1 + 94);
+    const double2 A0_599 = ld2(rk0 + 96), A1_599 = ld2(rk1 + 96);
+    { const double2 a0 = A0_597, a1 = A1_597;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
       jc2 += 1.0 * dq;
       jc14 += 1.0 * dq;
@@ -12524,159 +13123,180 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 94), a1 = ld2(rk1 + 94);
+    { const double2 a0 = A0_598, a1 = A1_598;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c27;
       jc7 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 96), a1 = ld2(rk1 + 96);
+    { const double2 a0 = A0_599, a1 = A1_599;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc14 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 98), a1 = ld2(rk1 + 98);
+    const double2 A0_600 = ld2(rk0 + 98), A1_600 = ld2(rk1 + 98);
+    const double2 A0_601 = ld2(rk0 + 100), A1_601 = ld2(rk1 + 100);
+    const double2 A0_602 = ld2(rk0 + 102), A1_602 = ld2(rk1 + 102);
+    const double2 A0_603 = ld2(rk0 + 104), A1_603 = ld2(rk1 + 104);
+    { const double2 a0 = A0_600, a1 = A1_600;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc8 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc24 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 100), a1 = ld2(rk1 + 100);
+    { const double2 a0 = A0_601, a1 = A1_601;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc8 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc24 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 102), a1 = ld2(rk1 + 102);
+    { const double2 a0 = A0_602, a1 = A1_602;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
       jc2 += 1.0 * dq;
       jc17 += 1.0 * dq;
       jc26 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 104), a1 = ld2(rk1 + 104);
+    { const double2 a0 = A0_603, a1 = A1_603;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c29;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 106), a1 = ld2(rk1 + 106);
+    const double2 A0_604 = ld2(rk0 + 106), A1_604 = ld2(rk1 + 106);
+    const double2 A0_605 = ld2(rk0 + 108), A1_605 = ld2(rk1 + 108);
+    const double2 A0_606 = ld2(rk0 + 110), A1_606 = ld2(rk1 + 110);
+    const double2 A0_607 = ld2(rk0 + 112), A1_607 = ld2(rk1 + 112);
+    { const double2 a0 = A0_604, a1 = A1_604;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c29;
       jc14 += 1.0 * dq;
       jc26 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 108), a1 = ld2(rk1 + 108);
+    { const double2 a0 = A0_605, a1 = A1_605;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc16 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 110), a1 = ld2(rk1 + 110);
+    { const double2 a0 = A0_606, a1 = A1_606;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc14 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 112), a1 = ld2(rk1 + 112);
+    { const double2 a0 = A0_607, a1 = A1_607;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc14 += 1.0 * dq;
       jc16 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 114), a1 = ld2(rk1 + 114);
+    const double2 A0_608 = ld2(rk0 + 114), A1_608 = ld2(rk1 + 114);
+    const double2 A0_609 = ld2(rk0 + 116), A1_609 = ld2(rk1 + 116);
+    const double2 A0_610 = ld2(rk0 + 118), A1_610 = ld2(rk1 + 118);
+    const double2 A0_611 = ld2(rk0 + 120), A1_611 = ld2(rk1 + 120);
+    { const double2 a0 = A0_608, a1 = A1_608;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 116), a1 = ld2(rk1 + 116);
+    { const double2 a0 = A0_609, a1 = A1_609;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
       jc2 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc27 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 118), a1 = ld2(rk1 + 118);
+    { const double2 a0 = A0_610, a1 = A1_610;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c17;
       jc9 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc17 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 120), a1 = ld2(rk1 + 120);
+    { const double2 a0 = A0_611, a1 = A1_611;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c10;
       jc10 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc17 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 122), a1 = ld2(rk1 + 122);
+    const double2 A0_612 = ld2(rk0 + 122), A1_612 = ld2(rk1 + 122);
+    const double2 A0_613 = ld2(rk0 + 124), A1_613 = ld2(rk1 + 124);
+    const double2 A0_614 = ld2(rk0 + 126), A1_614 = ld2(rk1 + 126);
+    const double2 A0_615 = ld2(rk0 + 128), A1_615 = ld2(rk1 + 128);
+    { const double2 a0 = A0_612, a1 = A1_612;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c4;
       jc2 += 1.0 * dq;
       jc4 += -1.0 * dq;
       jc12 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 124), a1 = ld2(rk1 + 124);
+    { const double2 a0 = A0_613, a1 = A1_613;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c4;
       jc4 += -1.0 * dq;
       jc9 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 126), a1 = ld2(rk1 + 126);
+    { const double2 a0 = A0_614, a1 = A1_614;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc4 += 1.0 * dq;
       jc6 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 128), a1 = ld2(rk1 + 128);
+    { const double2 a0 = A0_615, a1 = A1_615;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc6 += 1.0 * dq;
       jc14 += 1.0 * dq;
       jc25 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 130), a1 = ld2(rk1 + 130);
+    const double2 A0_616 = ld2(rk0 + 130), A1_616 = ld2(rk1 + 130);
+    const double2 A0_617 = ld2(rk0 + 136), A1_617 = ld2(rk1 + 136);
+    const double2 A0_618 = ld2(rk0 + 140), A1_618 = ld2(rk1 + 140);
+    const double2 A0_619 = ld2(rk0 + 160), A1_619 = ld2(rk1 + 160);
+    { const double2 a0 = A0_616, a1 = A1_616;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc11 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 136), a1 = ld2(rk1 + 136);
+    { const double2 a0 = A0_617, a1 = A1_617;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c10;
       jc10 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 140), a1 = ld2(rk1 + 140);
+    { const double2 a0 = A0_618, a1 = A1_618;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c30;
       jc9 += -1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 160), a1 = ld2(rk1 + 160);
+    { const double2 a0 = A0_619, a1 = A1_619;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c17;
       jc15 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc27 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
+    const double2 A0_620 = ld2(rk0 + 190), A1_620 = ld2(rk1 + 190);
     {
       const double dq = scr[38 * SST];
       jc9 += -1.0 * dq;
@@ -12691,7 +13311,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc16 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 190), a1 = ld2(rk1 + 190);
+    { const double2 a0 = A0_620, a1 = A1_620;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c10;
       jc10 += 1.0 * dq;
       jc15 += -1.0 * dq;
@@ -12703,6 +13323,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
+    const double2 A0_621 = ld2(rk0 + 214), A1_621 = ld2(rk1 + 214);
+    const double2 A0_622 = ld2(rk0 + 222), A1_622 = ld2(rk1 + 222);
     {
       const double dq = scr[45 * SST];
       jc2 += 1.0 * dq;
@@ -12715,28 +13337,30 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 214), a1 = ld2(rk1 + 214);
+    { const double2 a0 = A0_621, a1 = A1_621;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc10 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 222), a1 = ld2(rk1 + 222);
+    { const double2 a0 = A0_622, a1 = A1_622;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc6 += -1.0 * dq;
       jc7 += 1.0 * dq;
       jc25 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 230), a1 = ld2(rk1 + 230);
+    const double2 A0_623 = ld2(rk0 + 230), A1_623 = ld2(rk1 + 230);
+    const double2 A0_624 = ld2(rk0 + 240), A1_624 = ld2(rk1 + 240);
+    { const double2 a0 = A0_623, a1 = A1_623;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c9;
       jc9 += -1.0 * dq;
       jc15 += 1.0 * dq;
       jc26 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 240), a1 = ld2(rk1 + 240);
+    { const double2 a0 = A0_624, a1 = A1_624;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc10 += 1.0 * dq;
       jc12 += -1.0 * dq;
@@ -12755,28 +13379,32 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 282), a1 = ld2(rk1 + 282);
+    const double2 A0_625 = ld2(rk0 + 282), A1_625 = ld2(rk1 + 282);
+    const double2 A0_626 = ld2(rk0 + 300), A1_626 = ld2(rk1 + 300);
+    const double2 A0_627 = ld2(rk0 + 308), A1_627 = ld2(rk1 + 308);
+    const double2 A0_628 = ld2(rk0 + 328), A1_628 = ld2(rk1 + 328);
+    { const double2 a0 = A0_625, a1 = A1_625;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc7 += 1.0 * dq;
       jc12 += -1.0 * dq;
       jc20 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 300), a1 = ld2(rk1 + 300);
+    { const double2 a0 = A0_626, a1 = A1_626;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c25;
       jc10 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc25 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 308), a1 = ld2(rk1 + 308);
+    { const double2 a0 = A0_627, a1 = A1_627;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c22;
       jc14 += 1.0 * dq;
       jc22 += -1.0 * dq;
       jc23 += 1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 328), a1 = ld2(rk1 + 328);
+    { const double2 a0 = A0_628, a1 = A1_628;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c21;
       jc6 += 1.0 * dq;
       jc10 += 1.0 * dq;
@@ -12784,14 +13412,16 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc21 += -1.0 * dq;
       jc27 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 358), a1 = ld2(rk1 + 358);
+    const double2 A0_629 = ld2(rk0 + 358), A1_629 = ld2(rk1 + 358);
+    const double2 A0_630 = ld2(rk0 + 368), A1_630 = ld2(rk1 + 368);
+    { const double2 a0 = A0_629, a1 = A1_629;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c12;
       jc12 += -1.0 * dq;
       jc18 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 368), a1 = ld2(rk1 + 368);
+    { const double2 a0 = A0_630, a1 = A1_630;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c22;
       jc22 += -1.0 * dq;
       jc24 += 1.0 * dq;
@@ -12861,6 +13491,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc28 = 0.0;
     double jc29 = 0.0;
     double jc30 = 0.0;
+    const double2 A0_631 = ld2(rk0 + 36), A1_631 = ld2(rk1 + 36);
     {
       const double dq = scr[0 * SST];
       jc7 += -2.0 * dq;
@@ -12878,14 +13509,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 36), a1 = ld2(rk1 + 36);
+    { const double2 a0 = A0_631, a1 = A1_631;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c7;
       jc7 += -1.0 * dq;
       jc10 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 38), a1 = ld2(rk1 + 38);
+    const double2 A0_632 = ld2(rk0 + 38), A1_632 = ld2(rk1 + 38);
+    { const double2 a0 = A0_632, a1 = A1_632;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
       jc7 += -1.0 * dq;
       jc11 += -1.0 * dq;
@@ -12933,6 +13565,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc4 += -1.0 * dq;
       jc6 += 1.0 * dq;
     }
+    const double2 A0_633 = ld2(rk0 + 86), A1_633 = ld2(rk1 + 86);
     {
       const double dq = scr[23 * SST];
       jc2 += -1.0 * dq;
@@ -12951,14 +13584,16 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 86), a1 = ld2(rk1 + 86);
+    { const double2 a0 = A0_633, a1 = A1_633;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c2;
       jc2 += -1.0 * dq;
       jc19 += 1.0 * dq;
       jc25 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 88), a1 = ld2(rk1 + 88);
+    const double2 A0_634 = ld2(rk0 + 88), A1_634 = ld2(rk1 + 88);
+    const double2 A0_635 = ld2(rk0 + 130), A1_635 = ld2(rk1 + 130);
+    { const double2 a0 = A0_634, a1 = A1_634;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c19;
       jc2 += -1.0 * dq;
       jc11 += -1.0 * dq;
@@ -12976,20 +13611,22 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 130), a1 = ld2(rk1 + 130);
+    { const double2 a0 = A0_635, a1 = A1_635;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc11 += -1.0 * dq;
       jc14 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc28 += 1.0 * dq;
     }
+    const double2 A0_636 = ld2(rk0 + 188), A1_636 = ld2(rk1 + 188);
+    const double2 A0_637 = ld2(rk0 + 194), A1_637 = ld2(rk1 + 194);
     {
       const double dq = scr[38 * SST];
       jc9 += -1.0 * dq;
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 188), a1 = ld2(rk1 + 188);
+    { const double2 a0 = A0_636, a1 = A1_636;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c18;
       jc11 += -1.0 * dq;
       jc16 += -1.0 * dq;
@@ -13001,13 +13638,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc18 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 194), a1 = ld2(rk1 + 194);
+    { const double2 a0 = A0_637, a1 = A1_637;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc18 += -2.0 * dq;
       jc28 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 202), a1 = ld2(rk1 + 202);
+    const double2 A0_638 = ld2(rk0 + 202), A1_638 = ld2(rk1 + 202);
+    const double2 A0_639 = ld2(rk0 + 212), A1_639 = ld2(rk1 + 212);
+    { const double2 a0 = A0_638, a1 = A1_638;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c3;
       jc3 += 1.0 * dq;
       jc11 += -1.0 * dq;
@@ -13026,40 +13665,46 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 212), a1 = ld2(rk1 + 212);
+    { const double2 a0 = A0_639, a1 = A1_639;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc15 += -1.0 * dq;
       jc24 += 1.0 * dq;
       jc25 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 216), a1 = ld2(rk1 + 216);
+    const double2 A0_640 = ld2(rk0 + 216), A1_640 = ld2(rk1 + 216);
+    const double2 A0_641 = ld2(rk0 + 246), A1_641 = ld2(rk1 + 246);
+    const double2 A0_642 = ld2(rk0 + 250), A1_642 = ld2(rk1 + 250);
+    const double2 A0_643 = ld2(rk0 + 254), A1_643 = ld2(rk1 + 254);
+    { const double2 a0 = A0_640, a1 = A1_640;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c6;
       jc6 += -1.0 * dq;
       jc25 += 2.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 246), a1 = ld2(rk1 + 246);
+    { const double2 a0 = A0_641, a1 = A1_641;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc11 += 1.0 * dq;
       jc15 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 250), a1 = ld2(rk1 + 250);
+    { const double2 a0 = A0_642, a1 = A1_642;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c8;
       jc8 += -1.0 * dq;
       jc11 += 1.0 * dq;
       jc24 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 254), a1 = ld2(rk1 + 254);
+    { const double2 a0 = A0_643, a1 = A1_643;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c28;
       jc11 += 1.0 * dq;
       jc25 += 1.0 * dq;
       jc28 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 260), a1 = ld2(rk1 + 260);
+    const double2 A0_644 = ld2(rk0 + 260), A1_644 = ld2(rk1 + 260);
+    const double2 A0_645 = ld2(rk0 + 294), A1_645 = ld2(rk1 + 294);
+    { const double2 a0 = A0_644, a1 = A1_644;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc3 += 1.0 * dq;
       jc18 += -1.0 * dq;
@@ -13078,14 +13723,15 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 294), a1 = ld2(rk1 + 294);
+    { const double2 a0 = A0_645, a1 = A1_645;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c29;
       jc11 += 1.0 * dq;
       jc26 += 1.0 * dq;
       jc28 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 298), a1 = ld2(rk1 + 298);
+    const double2 A0_646 = ld2(rk0 + 298), A1_646 = ld2(rk1 + 298);
+    { const double2 a0 = A0_646, a1 = A1_646;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc11 += 1.0 * dq;
       jc18 += -1.0 * dq;
@@ -13179,6 +13825,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
+    const double2 A0_647 = ld2(rk0 + 40), A1_647 = ld2(rk1 + 40);
     {
       const double dq = -scr[78 * SST] * c9;
       jc4 += -1.0 * dq;
@@ -13186,7 +13833,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc9 += 1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 40), a1 = ld2(rk1 + 40);
+    { const double2 a0 = A0_647, a1 = A1_647;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c30;
       jc7 += -1.0 * dq;
       jc12 += -1.0 * dq;
@@ -13252,6 +13899,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
+    const double2 A0_648 = ld2(rk0 + 104), A1_648 = ld2(rk1 + 104);
+    const double2 A0_649 = ld2(rk0 + 106), A1_649 = ld2(rk1 + 106);
     {
       const double dq = scr[32 * SST];
       jc9 += -1.0 * dq;
@@ -13263,76 +13912,87 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 104), a1 = ld2(rk1 + 104);
+    { const double2 a0 = A0_648, a1 = A1_648;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc2 += 1.0 * dq;
       jc10 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 106), a1 = ld2(rk1 + 106);
+    { const double2 a0 = A0_649, a1 = A1_649;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c27;
       jc14 += 1.0 * dq;
       jc26 += 1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 110), a1 = ld2(rk1 + 110);
+    const double2 A0_650 = ld2(rk0 + 110), A1_650 = ld2(rk1 + 110);
+    const double2 A0_651 = ld2(rk0 + 136), A1_651 = ld2(rk1 + 136);
+    const double2 A0_652 = ld2(rk0 + 146), A1_652 = ld2(rk1 + 146);
+    const double2 A0_653 = ld2(rk0 + 150), A1_653 = ld2(rk1 + 150);
+    { const double2 a0 = A0_650, a1 = A1_650;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc14 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc27 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 136), a1 = ld2(rk1 + 136);
+    { const double2 a0 = A0_651, a1 = A1_651;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c24;
       jc10 += 1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 146), a1 = ld2(rk1 + 146);
+    { const double2 a0 = A0_652, a1 = A1_652;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc19 += -1.0 * dq;
       jc26 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 150), a1 = ld2(rk1 + 150);
+    { const double2 a0 = A0_653, a1 = A1_653;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
       jc2 += 1.0 * dq;
       jc4 += 1.0 * dq;
       jc26 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 160), a1 = ld2(rk1 + 160);
+    const double2 A0_654 = ld2(rk0 + 160), A1_654 = ld2(rk1 + 160);
+    const double2 A0_655 = ld2(rk0 + 162), A1_655 = ld2(rk1 + 162);
+    const double2 A0_656 = ld2(rk0 + 164), A1_656 = ld2(rk1 + 164);
+    const double2 A0_657 = ld2(rk0 + 166), A1_657 = ld2(rk1 + 166);
+    { const double2 a0 = A0_654, a1 = A1_654;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c15;
       jc15 += -1.0 * dq;
       jc17 += 1.0 * dq;
       jc27 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 162), a1 = ld2(rk1 + 162);
+    { const double2 a0 = A0_655, a1 = A1_655;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c19;
       jc2 += 1.0 * dq;
       jc18 += 1.0 * dq;
       jc19 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 164), a1 = ld2(rk1 + 164);
+    { const double2 a0 = A0_656, a1 = A1_656;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * 2.0*c29;
       jc4 += 1.0 * dq;
       jc19 += 1.0 * dq;
       jc29 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 166), a1 = ld2(rk1 + 166);
+    { const double2 a0 = A0_657, a1 = A1_657;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc2 += 1.0 * dq;
       jc18 += -1.0 * dq;
       jc25 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 168), a1 = ld2(rk1 + 168);
+    const double2 A0_658 = ld2(rk0 + 168), A1_658 = ld2(rk1 + 168);
+    const double2 A0_659 = ld2(rk0 + 176), A1_659 = ld2(rk1 + 176);
+    const double2 A0_660 = ld2(rk0 + 182), A1_660 = ld2(rk1 + 182);
+    { const double2 a0 = A0_658, a1 = A1_658;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c3;
       jc3 += -1.0 * dq;
       jc18 += 2.0 * dq;
@@ -13344,19 +14004,20 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 176), a1 = ld2(rk1 + 176);
+    { const double2 a0 = A0_659, a1 = A1_659;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c14;
       jc14 += 0.0 * dq;
       jc16 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 182), a1 = ld2(rk1 + 182);
+    { const double2 a0 = A0_660, a1 = A1_660;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c9;
       jc9 += 0.0 * dq;
       jc16 += -1.0 * dq;
       jc29 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 184), a1 = ld2(rk1 + 184);
+    const double2 A0_661 = ld2(rk0 + 184), A1_661 = ld2(rk1 + 184);
+    { const double2 a0 = A0_661, a1 = A1_661;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c30;
       jc16 += -1.0 * dq;
       jc29 += 1.0 * dq;
@@ -13379,6 +14040,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc19 += 1.0 * dq;
       jc25 += -1.0 * dq;
     }
+    const double2 A0_662 = ld2(rk0 + 294), A1_662 = ld2(rk1 + 294);
+    const double2 A0_663 = ld2(rk0 + 296), A1_663 = ld2(rk1 + 296);
     {
       const double dq = scr[49 * SST];
       jc9 += 1.0 * dq;
@@ -13391,14 +14054,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 294), a1 = ld2(rk1 + 294);
+    { const double2 a0 = A0_662, a1 = A1_662;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c28;
       jc11 += 1.0 * dq;
       jc26 += 1.0 * dq;
       jc28 += -1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 296), a1 = ld2(rk1 + 296);
+    { const double2 a0 = A0_663, a1 = A1_663;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c11;
       jc11 += -1.0 * dq;
       jc18 += 1.0 * dq;
@@ -13468,6 +14131,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
     double jc28 = 0.0;
     double jc29 = 0.0;
     double jc30 = 0.0;
+    const double2 A0_664 = ld2(rk0 + 22), A1_664 = ld2(rk1 + 22);
     {
       const double dq = 3.5999999046325684 * scr[0 * SST];
       jc7 += -2.0 * dq;
@@ -13479,7 +14143,7 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc7 += -1.0 * dq;
       jc27 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 22), a1 = ld2(rk1 + 22);
+    { const double2 a0 = A0_664, a1 = A1_664;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * M2 * 1.0 + 3.5 * scr[2 * SST];
       jc7 += -1.0 * dq;
       jc9 += -1.0 * dq;
@@ -13492,14 +14156,16 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc17 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 40), a1 = ld2(rk1 + 40);
+    const double2 A0_665 = ld2(rk0 + 40), A1_665 = ld2(rk1 + 40);
+    const double2 A0_666 = ld2(rk0 + 42), A1_666 = ld2(rk1 + 42);
+    { const double2 a0 = A0_665, a1 = A1_665;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c29;
       jc7 += -1.0 * dq;
       jc12 += -1.0 * dq;
       jc29 += 1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 42), a1 = ld2(rk1 + 42);
+    { const double2 a0 = A0_666, a1 = A1_666;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c7;
       jc7 += 1.0 * dq;
       jc9 += -1.0 * dq;
@@ -13565,6 +14231,8 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc11 += 1.0 * dq;
       jc28 += -1.0 * dq;
     }
+    const double2 A0_667 = ld2(rk0 + 116), A1_667 = ld2(rk1 + 116);
+    const double2 A0_668 = ld2(rk0 + 140), A1_668 = ld2(rk1 + 140);
     {
       const double dq = 2.0 * scr[32 * SST];
       jc9 += -1.0 * dq;
@@ -13576,21 +14244,24 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc8 += 1.0 * dq;
       jc27 += -2.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 116), a1 = ld2(rk1 + 116);
+    { const double2 a0 = A0_667, a1 = A1_667;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c2;
       jc2 += 1.0 * dq;
       jc9 += -1.0 * dq;
       jc27 += -1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 140), a1 = ld2(rk1 + 140);
+    { const double2 a0 = A0_668, a1 = A1_668;
       const double dq = -(a0.y + (a1.y - a0.y) * Td) * c27;
       jc9 += -1.0 * dq;
       jc24 += -1.0 * dq;
       jc27 += 1.0 * dq;
       jc30 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 156), a1 = ld2(rk1 + 156);
+    const double2 A0_669 = ld2(rk0 + 156), A1_669 = ld2(rk1 + 156);
+    const double2 A0_670 = ld2(rk0 + 184), A1_670 = ld2(rk1 + 184);
+    const double2 A0_671 = ld2(rk0 + 186), A1_671 = ld2(rk1 + 186);
+    { const double2 a0 = A0_669, a1 = A1_669;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c26;
       jc9 += 1.0 * dq;
       jc17 += 1.0 * dq;
@@ -13603,13 +14274,13 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc12 += 1.0 * dq;
       jc29 += -1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 184), a1 = ld2(rk1 + 184);
+    { const double2 a0 = A0_670, a1 = A1_670;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16 - (a0.y + (a1.y - a0.y) * Td) * c29;
       jc16 += -1.0 * dq;
       jc29 += 1.0 * dq;
       jc30 += 0.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 186), a1 = ld2(rk1 + 186);
+    { const double2 a0 = A0_671, a1 = A1_671;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c16;
       jc9 += 1.0 * dq;
       jc10 += 1.0 * dq;
@@ -13639,13 +14310,14 @@ __device__ __forceinline__ int eval(const double* __restrict__ tabk, int Tk_min,
       jc18 += 1.0 * dq;
       jc20 += -1.0 * dq;
     }
+    const double2 A0_672 = ld2(rk0 + 360), A1_672 = ld2(rk1 + 360);
     {
       const double dq = scr[50 * SST];
       jc9 += -1.0 * dq;
       jc18 += -1.0 * dq;
       jc20 += 1.0 * dq;
     }
-    { const double2 a0 = ld2(rk0 + 360), a1 = ld2(rk1 + 360);
+    { const double2 a0 = A0_672, a1 = A1_672;
       const double dq = (a0.x + (a1.x - a0.x) * Td) * c18;
       jc12 += 1.0 * dq;
       jc18 += -1.0 * dq;
