@@ -42,12 +42,27 @@ __device__ __forceinline__ bool thermo(const CeaProg& P, const double* __restric
   const double* __restrict__ s0 = P.tabs + (long)(Ti - P.Th_min) * NS;
   const double* __restrict__ s1 = s0 + NS;
   Cpsum = 0.0;
+  // the table rows of eight species read before their arithmetic: read as it consumes them, each
+  // load is waited for before the next one goes out
+  constexpr int G = 8;
+  double2 ga[G], gb[G];
+  double gs0[G], gs1[G];
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
-    const double2 a = ld2(r0 + 2 * j), b = ld2(r1 + 2 * j);
+    if (j % G == 0) {
+#pragma unroll
+      for (int u = 0; u < G; ++u) {
+        const int jj = j + u < NS ? j + u : NS - 1;
+        ga[u] = ld2(r0 + 2 * jj);
+        gb[u] = ld2(r1 + 2 * jj);
+        gs0[u] = s0[jj];
+        gs1[u] = s1[jj];
+      }
+    }
+    const double2 a = ga[j % G], b = gb[j % G];
     const double cp = a.x + (b.x - a.x) * Td;
     const double h = a.y + (b.y - a.y) * Td;
-    const double s = s0[j] + (s1[j] - s0[j]) * Td;
+    const double s = gs0[j % G] + (gs1[j % G] - gs0[j % G]) * Td;
     const double Cp = cp * W[j] / RR;
     H0[j] = h * W[j] / (RR * T);
     G0[j] = H0[j] - s * W[j] / RR;
@@ -167,9 +182,10 @@ __global__ __launch_bounds__(64) void cea_kernel(CeaProg P, long n, long ld, con
 #pragma unroll
       for (int j = 0; j < NS; ++j) {
         const double pj = rho[(long)j * ld + c];
+        const double h0j = r0[2 * j + 1], h1j = r1[2 * j + 1];  // read unguarded (in bounds)
         if (pj <= 1e-15) continue;
         Wp = Wp + pj;
-        const double hi = r0[2 * j + 1] + (r1[2 * j + 1] - r0[2 * j + 1]) * Td;
+        const double hi = h0j + (h1j - h0j) * Td;
         double enth = hi * sW[j] / RR;
         if (VOL) enth = enth - T0;
         hpp = hpp + enth * pj / sRm[j];
