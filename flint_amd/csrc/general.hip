@@ -260,10 +260,19 @@ __global__ __launch_bounds__(BLOCK) void general_reg_kernel(GenProg P, const dou
   const int ns = P.ns;
   double c0[16], c1[16];
   double* om = smem + threadIdx.x;
+  // the densities read unguarded (rows clamped into the state) and the guards applied to the
+  // values: a guarded load is a branch with its own wait, and the species' loads would go out one
+  // after another
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    c0[k] = k < ns ? stream_load(rho + (long)k * ld + i) / W[k] : 0.0;
-    if (NCH > 1) c1[k] = k + 16 < ns ? stream_load(rho + (long)(k + 16) * ld + i) / W[k + 16] : 0.0;
+    const int k0 = min(k, ns - 1);
+    const double v0 = stream_load(rho + (long)k0 * ld + i) / W[k0];
+    c0[k] = k < ns ? v0 : 0.0;
+    if (NCH > 1) {
+      const int k1 = min(k + 16, ns - 1);
+      const double v1 = stream_load(rho + (long)k1 * ld + i) / W[k1];
+      c1[k] = k + 16 < ns ? v1 : 0.0;
+    }
   }
   for (int k = 0; k < ns; ++k) om[k * BLOCK] = 0.0;
   RegCell<NCH> S{c0, c1, om};
